@@ -1,0 +1,216 @@
+#!/usr/bin/env python3
+"""Benchmark: one per-Newton linear solve of libuipc on one MI355X.
+
+Workload (default): the 64^3-vertex-grid FEM system (262,144 vertices,
+1,250,235 stable-Neo-Hookean tets, PSD-projected, + M) plus 2,000,000 synthetic
+vertex-triangle contact stencils (kappa g g^T, kappa log-uniform [1e4, 1e8]),
+seeded.  One step = the whole per-Newton solve with the triplets and b already
+resident in HBM:
+    triplets -> upper BCOO/BSR (radix sort + segmented reduce)
+    preconditioner assembly
+    PCG from x0 = 0 to ||b - A x|| / ||b|| <= 1e-4 (recurrence residual)
+Prints one JSON line (metric: ms per solve; PCG iterations and the SpMV's
+achieved HBM bandwidth ride along).
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--grid 64] [--contacts 2000000]
+                [--precond mas|diag] [--precision fp64|fp32] [--no-cpu-baseline]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--grid", type=int, default=64)
+    ap.add_argument("--contacts", type=int, default=2_000_000)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--precond", choices=["mas", "diag"], default="mas")
+    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
+    ap.add_argument("--tol", type=float, default=1e-4)
+    ap.add_argument("--spmv-reps", type=int, default=50)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample-iters", type=int, default=30)
+    ap.add_argument("--verbose", action="store_true")
+    return ap.parse_args()
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def main():
+    args = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(json.dumps({"error": "this build runs on one GPU only: the per-Newton solve of one system is not "
+                                   "sharded across GPUs (see DESIGN.md for how it would shard)",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    from libuipc_amd import abi, synth
+    from libuipc_amd.device import Context, LinearSystem
+    from libuipc_amd.hip_events import EventTimer
+
+    t0 = time.perf_counter()
+    s = synth.fem_system(args.grid, args.contacts, seed=args.seed)
+    log(f"generated grid={args.grid} contacts={args.contacts}: B={s.block_rows} T={s.n_triplets} "
+        f"in {time.perf_counter() - t0:.1f}s")
+
+    ctx = Context(0)
+    stream = ctx.stream()
+    prec = abi.MI_FP32 if args.precision == "fp32" else abi.MI_FP64
+    ls = LinearSystem(ctx, s.block_rows, prec)
+    rows, cols, vals = ctx.to_device(s.rows), ctx.to_device(s.cols), ctx.to_device(s.vals)
+    b = ctx.to_device(s.rhs)
+    x = ctx.alloc(s.n, np.float64)
+    precond = args.precond
+    if precond == "mas":
+        try:
+            ls.add_preconditioner(abi.MI_PRECOND_MAS, 0, s.n, part_ids=s.part_ids, tets=s.tets,
+                                  mas_precision=prec)
+        except abi.MiError as e:
+            if e.code != abi.MI_ERR_UNSUPPORTED:
+                raise
+            log("MAS unavailable in this build, using 3x3 block Jacobi")
+            precond = "diag"
+    if precond == "diag":
+        ls.add_preconditioner(abi.MI_PRECOND_DIAG3, 0, s.n)
+
+    solve_kw = dict(criterion=abi.MI_CRIT_REL_RESIDUAL, tol=args.tol, check_interval=5, use_graph=1)
+
+    def step():
+        ls.assemble(rows, cols, vals, stream)
+        ls.assemble_preconditioners()
+        return ls.solve(b, x, **solve_kw)
+
+    for w in range(args.warmup):
+        st = step()
+        log(f"warmup {w}: iters={st['iters']} exact={st['iters_exact']} rel_res={st['rel_res']:.3e}")
+
+    ev = EventTimer(stream)
+    ctx.synchronize()
+    times, stats = [], []
+    t_wall0 = time.perf_counter()
+    for k in range(args.steps):
+        ev.start()
+        st = step()
+        ms = ev.stop()
+        times.append(ms)
+        stats.append(st)
+        log(f"step {k}: {ms:.2f} ms iters={st['iters']} exact={st['iters_exact']} rel_res={st['rel_res']:.3e}")
+    ctx.synchronize()
+    wall = time.perf_counter() - t_wall0
+    ms_per_step = float(np.median(times))
+
+    # phase split of one step
+    ev.start()
+    ls.assemble(rows, cols, vals, stream)
+    ms_assemble = ev.stop()
+    ev.start()
+    ls.assemble_preconditioners()
+    ms_precond = ev.stop()
+    ev.start()
+    st = ls.solve(b, x, **solve_kw, true_residual=1)
+    ms_solve = ev.stop()
+    U = ls.nnz()
+
+    # dominant kernel: the SpMV (+ fused p^T A p), timed with HIP events on its stream
+    y = ctx.alloc(s.n, np.float64)
+    d = ctx.alloc(1, np.float64)
+    for _ in range(3):
+        ctx.lib.mi_spmv_dot(ls.h, x.ptr, y.ptr, d.ptr, stream)
+    ev.start()
+    for _ in range(args.spmv_reps):
+        ctx.lib.mi_spmv_dot(ls.h, x.ptr, y.ptr, d.ptr, stream)
+    ms_spmv = ev.stop() / args.spmv_reps
+    sval = 4 if prec == abi.MI_FP32 else 8
+    spmv_bytes = U * (9 * sval + 4) + (s.block_rows + 1) * 4 + 2 * s.n * 8
+    gbs = spmv_bytes / (ms_spmv * 1e-3) / 1e9
+
+    result = {
+        "metric": "ms per Newton solve to 1e-4 rel. residual; PCG iters; SpMV HBM GB/s vs peak",
+        "value": round(ms_per_step, 3),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == abi.MI_FP64 else "f32-matrix/f64-vectors",
+        "data": "synthetic (seeded generator, random jitter / kappa / contacts)",
+        "config": {
+            "workload": f"{args.grid}^3 grid FEM (SNH, PSD) + {args.contacts} vertex-triangle contacts; "
+                        f"assemble + {precond} precond + PCG to rel. residual {args.tol:g}",
+            "block_rows": s.block_rows, "dofs": s.n, "triplets": s.n_triplets, "unique_upper_blocks": U,
+            "preconditioner": precond, "precision": args.precision, "criterion": "rel_residual",
+        },
+        "pcg_iters": int(st["iters"]),
+        "pcg_iters_exact": int(st["iters_exact"]),
+        "rel_res": st["rel_res"],
+        "rel_res_true": st["rel_res_true"],
+        "phase_ms": {"assemble": round(ms_assemble, 3), "precond_assemble": round(ms_precond, 3),
+                     "pcg": round(ms_solve, 3),
+                     "per_iter_us": round(1e3 * ms_solve / max(1, st["iters"]), 2)},
+        "wall_s_timed": round(wall, 3),
+        "roofline": {"kernel": "spmv_sym_dot_kernel<double>" if prec == abi.MI_FP64 else
+                     "spmv_sym_dot_kernel<float>", "bound": "hbm", "achieved": round(gbs, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                     "traffic": None, "algorithmic_bytes": int(spmv_bytes), "avg_us": round(ms_spmv * 1e3, 2)},
+    }
+
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline(s, precond, args, int(st["iters"]))
+    print(json.dumps(result), flush=True)
+    ls.close()
+    ctx.close()
+
+
+def cpu_baseline(s, precond, args, gpu_iters):
+    """The oracle (C restatement of the reference's solver logic) on this host's
+    cores: one full assembly + preconditioner build, then a bounded number of
+    PCG iterations; ms per solve extrapolated to the GPU run's iteration count."""
+    from oracle import oracle as O
+
+    threads = min(os.cpu_count() or 1, 16)
+    t = time.perf_counter()
+    r, c, v, rp = O.convert_sym(s.block_rows, s.rows, s.cols, s.vals)
+    t_asm = time.perf_counter() - t
+    t = time.perf_counter()
+    if precond == "mas":
+        m = O.Mas(s.block_rows, s.part_ids, s.tets)
+        m.assemble(r, c, v)
+        pc = [dict(kind=3, blk_off=0, blk_cnt=s.block_rows, mas=m)]
+    else:
+        pc = [dict(kind=1, blk_off=0, blk_cnt=s.block_rows, data=O.diag3(r, c, v, 0, s.block_rows))]
+    t_pc = time.perf_counter() - t
+    k = max(1, min(args.cpu_sample_iters, gpu_iters))
+    _, res = O.pcg(s.block_rows, r, c, v, s.rhs, pc, criterion=1, tol=args.tol, threads=threads,
+                   max_iter_cap=k)
+    it_done = res["iters_exact"] if res["converged"] else k
+    t_iter = res["seconds"] / max(1, it_done)
+    ms = 1e3 * (t_asm + t_pc + t_iter * gpu_iters)
+    return {"value": round(ms, 1), "unit": "ms", "cores": threads, "kind": "port",
+            "sample": f"oracle convert_sym ({t_asm:.2f}s, 1 thread) + {precond} build ({t_pc:.2f}s, 1 thread) "
+                      f"+ {it_done} PCG iterations on {threads} threads ({t_iter * 1e3:.1f} ms/iter), "
+                      f"extrapolated to the GPU's {gpu_iters} iterations"}
+
+
+if __name__ == "__main__":
+    main()

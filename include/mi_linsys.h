@@ -1,0 +1,231 @@
+/*
+ * mi_linsys.h -- C ABI of the MI355X (gfx950) per-Newton global linear solve.
+ *
+ * The library replaces four pieces of libuipc's CUDA backend
+ * (src/backends/cuda/ in the reference tree):
+ *   (a) triplet -> sorted, de-duplicated upper-triangular 3x3 BCOO conversion
+ *       (MatrixConverter<Float,3>::convert_sym),
+ *   (b) the block-symmetric SpMV (+ fused p^T A p),
+ *   (c) the local preconditioners (3x3 / 12x12 block Jacobi and the MAS
+ *       multi-level additive Schwarz preconditioner, build and apply),
+ *   (d) the PCG driver and its dot / axpy / convergence kernels.
+ *
+ * Conventions
+ *   - Plain C types only; every pointer is a DEVICE pointer unless its name
+ *     ends in `_host`.
+ *   - A 3x3 block value is 9 doubles in column-major order (Eigen::Matrix3d's
+ *     layout), i.e. element (r, c) at index r + 3 c.
+ *   - Every function returns MI_OK (0) or a negative MI_ERR_* code; the text
+ *     of the last error of the calling thread is returned by mi_last_error().
+ *     No C++ exception crosses this boundary.
+ *   - `stream` is a hipStream_t passed as void*; NULL selects the context's
+ *     own stream.  Calls on one mi_sys must be serialised by the caller
+ *     (the reference is single-threaded per GlobalLinearSystem as well).
+ */
+#ifndef MI_LINSYS_H
+#define MI_LINSYS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MI_LINSYS_VERSION 1
+
+enum mi_status
+{
+    MI_OK              = 0,
+    MI_ERR_ARG         = -1, /* invalid argument (reference: UIPC_ASSERT / std::invalid_argument) */
+    MI_ERR_HIP         = -2, /* a HIP runtime call failed (reference: CUDA_TOOL_CHECK)            */
+    MI_ERR_STATE       = -3, /* call out of order (e.g. apply before assemble)                   */
+    MI_ERR_NOMEM       = -4, /* device allocation failed                                        */
+    MI_ERR_UNSUPPORTED = -5, /* option not supported by this build                               */
+    MI_ERR_NONFINITE   = -6  /* NaN / Inf in rz (reference: check_init_rz_nan_inf)               */
+};
+
+typedef struct mi_ctx mi_ctx;
+typedef struct mi_sys mi_sys;
+
+/* text of the calling thread's last error ("" if none) */
+const char* mi_last_error(void);
+int         mi_version(void);
+
+/* ------------------------------------------------------------------ context */
+int mi_ctx_create(int device, mi_ctx** out);
+int mi_ctx_destroy(mi_ctx* ctx);
+int mi_ctx_synchronize(mi_ctx* ctx);
+/* the context's stream as a hipStream_t */
+int mi_ctx_stream(mi_ctx* ctx, void** stream_out);
+
+/* Device-memory helpers for hosts that have no allocator of their own
+ * (test harnesses, ctypes callers).  Synchronous. */
+int mi_malloc(mi_ctx* ctx, void** dptr, size_t bytes);
+int mi_free(mi_ctx* ctx, void* dptr);
+int mi_memcpy_h2d(mi_ctx* ctx, void* dst, const void* src_host, size_t bytes);
+int mi_memcpy_d2h(mi_ctx* ctx, void* dst_host, const void* src, size_t bytes);
+int mi_memcpy_d2d(mi_ctx* ctx, void* dst, const void* src, size_t bytes);
+int mi_memset(mi_ctx* ctx, void* dst, int value, size_t bytes);
+
+/* ------------------------------------------------------------------ vectors */
+/* LinearSystemContext::dot / ::norm
+ * (cuda_tool/linear_system.h:436-502, cuda_tool/linear_reduction.h:14-60).
+ * The norm is the overflow-safe scaled 2-norm.  Results land on the host;
+ * the calls synchronise the context stream. */
+int mi_dot_f64(mi_ctx* ctx, const double* x, const double* y, int64_t n, double* out_host);
+int mi_norm_f64(mi_ctx* ctx, const double* x, int64_t n, double* out_host);
+int mi_dot_f32(mi_ctx* ctx, const float* x, const float* y, int64_t n, float* out_host);
+int mi_norm_f32(mi_ctx* ctx, const float* x, int64_t n, float* out_host);
+
+/* ------------------------------------------------------------------ system */
+enum mi_precision
+{
+    MI_FP64 = 0, /* plain method: every stream in double                        */
+    MI_FP32 = 1  /* precision arm: float matrix / MAS streams, double dots       */
+};
+
+typedef struct mi_sys_desc
+{
+    int32_t block_rows;      /* B: number of 3x3 block rows (n = 3 B)                     */
+    double  reserve_ratio;   /* capacity growth factor (reference GlobalLinearSystem: 1.1) */
+    int32_t value_precision; /* mi_precision of the SpMV's matrix stream                  */
+} mi_sys_desc;
+
+int mi_sys_create(mi_ctx* ctx, const mi_sys_desc* desc, mi_sys** out);
+int mi_sys_destroy(mi_sys* sys);
+
+/* MatrixConverter<Float,3>::convert_sym
+ * (algorithm/details/matrix_converter.inl:1549-1699).
+ * Keeps the triplets with 0 <= row <= col (others are dropped, as the
+ * reference's flag kernel at matrix_converter.inl:129-143 does), sorts them by
+ * (row, col), sums duplicates and builds the BSR row offsets
+ * (matrix_converter.inl:1041-1096).  Duplicates are summed in ascending source
+ * order, so the values are deterministic.  The triplet arrays are read, never
+ * written.  No host synchronisation. */
+int mi_sys_assemble(mi_sys*        sys,
+                    int64_t        n_triplets,
+                    const int32_t* rows,
+                    const int32_t* cols,
+                    const double*  vals,
+                    void*          stream);
+
+/* Number of unique upper blocks of the last assembly.  Synchronises. */
+int mi_sys_nnz(mi_sys* sys, int64_t* nnz_host);
+
+/* Copies the assembled matrix out in the reference's BCOO layout
+ * (rows, cols ascending by (row, col); values column-major 3x3) plus the BSR
+ * row offsets (block_rows + 1 entries).  Any output may be NULL. */
+int mi_sys_export_bcoo(mi_sys*  sys,
+                       int32_t* rows,
+                       int32_t* cols,
+                       double*  vals,
+                       int32_t* row_ptr,
+                       void*    stream);
+
+/* Spmv::rbk_sym_spmv_dot (linear_system/spmv.cu:1023-1159):
+ * y += A x with A = U + U^T - diag(U), and *d_dot = x^T A x (device scalar,
+ * overwritten).  As in the reference, y is accumulated into: pass a zeroed y
+ * for y = A x (LinearFusedPCG zeroes Ap in its update kernel). */
+int mi_spmv_dot(mi_sys* sys, const double* x, double* y, double* d_dot, void* stream);
+
+/* Spmv::rbk_sym_spmv (linear_system/spmv.cu:992-1021): y = a A x + b y. */
+int mi_spmv(mi_sys* sys, double a, const double* x, double b, double* y, void* stream);
+
+/* ------------------------------------------------------------ preconditioners */
+enum mi_precond_kind
+{
+    MI_PRECOND_DIAG3  = 1, /* FEMDiagPreconditioner (finite_element/fem_diag_preconditioner.cu:18-117)   */
+    MI_PRECOND_DIAG12 = 2, /* ABDDiagPreconditioner (affine_body/abd_diag_preconditioner.cu:15-122)      */
+    MI_PRECOND_MAS    = 3  /* FEMMASPreconditioner + MASPreconditionerEngine
+                              (finite_element/fem_mas_preconditioner.cu:157-469,
+                               finite_element/mas_preconditioner_engine.cu)                             */
+};
+
+typedef struct mi_precond_desc
+{
+    int32_t kind;       /* mi_precond_kind                                                   */
+    int64_t dof_offset; /* first scalar DoF of the subsystem (LocalPreconditionerAssemblyInfo) */
+    int64_t dof_count;  /* scalar DoFs of the subsystem                                      */
+
+    /* MAS only: mesh topology and partition, read at build time (host memory).
+     * The adjacency is built from the simplices as FEMMASPreconditioner::do_init
+     * does (tets, triangles and edges of the subsystem, vertex ids local to it). */
+    int32_t        vert_num;      /* subsystem vertices (dof_count / 3)                   */
+    const int32_t* part_ids_host; /* per vertex partition id, < 0 = unpartitioned;
+                                     at most 16 vertices per id (mesh_partition's BANKSIZE) */
+    const int32_t* tets_host;     /* 4 ids per tet (may be NULL)  */
+    int64_t        n_tets;
+    const int32_t* tris_host;     /* 3 ids per triangle (may be NULL) */
+    int64_t        n_tris;
+    const int32_t* edges_host;    /* 2 ids per edge (may be NULL) */
+    int64_t        n_edges;
+    int32_t        mas_precision;     /* MI_FP32: float inverses + float multi-level R/Z
+                                         (the reference's storage); MI_FP64: all double  */
+    int32_t        mas_active_levels; /* 0 = all levels (MASPreconditionerEngine::set_active_level_num) */
+} mi_precond_desc;
+
+/* GlobalLinearSystem's local-preconditioner registration + LocalPreconditioner::do_init.
+ * Subsystem ranges must not overlap; DoFs no preconditioner covers get z = r
+ * (global_linear_system.cu:546-590). */
+int mi_precond_add(mi_sys* sys, const mi_precond_desc* desc, int* handle_out);
+
+/* LocalPreconditioner::do_assemble for every registered preconditioner
+ * (global_linear_system.cu:499-515).  Requires a prior mi_sys_assemble. */
+int mi_precond_assemble(mi_sys* sys, void* stream);
+
+/* GlobalLinearSystem::Impl::apply_preconditioner (global_linear_system.cu:546-590):
+ * z = P^{-1} r.  d_converged may be NULL; when it points at a nonzero int the
+ * kernels return early, as the reference's do. */
+int mi_precond_apply(mi_sys* sys, const double* r, double* z, const int32_t* d_converged, void* stream);
+
+/* ----------------------------------------------------------------------- PCG */
+enum mi_pcg_criterion
+{
+    MI_CRIT_RZ_RATIO     = 0, /* |r^T z| / |r0^T z0| <= tol  (linear_fused_pcg.cu:1985) */
+    MI_CRIT_REL_RESIDUAL = 1  /* ||r||_2 / ||b||_2 <= tol (recurrence residual)          */
+};
+
+typedef struct mi_pcg_opts
+{
+    int32_t criterion;      /* mi_pcg_criterion                                            */
+    double  tol;            /* reference default tol_rate = 1e-3 (scene_default_config.cpp:146) */
+    double  max_iter_ratio; /* max_iter = ratio * n (reference: 2)                         */
+    int32_t check_interval; /* iterations between host checks (reference: 5)               */
+    int32_t use_graph;      /* 1: replay a captured hipGraph of check_interval iterations   */
+    int32_t psd_fallback;   /* 1: rz0 < 0 or non-finite -> plain CG (linear_fused_pcg.cu:1843-1875) */
+    int32_t force_bypass;   /* 1: always bypass the preconditioner (UIPC_PCG_PSD_FALLBACK_TEST) */
+    int32_t true_residual;  /* 1: compute ||b - A x|| / ||b|| after the solve (one SpMV)    */
+} mi_pcg_opts;
+
+typedef struct mi_pcg_stats
+{
+    int64_t iters;          /* iterations the reference would report (multiple of check_interval
+                               or max_iter when not converged, linear_fused_pcg.cu:1994)   */
+    int64_t iters_exact;    /* first iteration whose check passed                          */
+    double  rz0;
+    double  rz_final;
+    double  rel_res;        /* ||r_k|| / ||b|| of the recurrence at exit                   */
+    double  rel_res_true;   /* ||b - A x|| / ||b|| (NaN unless opts.true_residual)          */
+    int32_t converged;
+    int32_t bypassed;       /* preconditioner bypassed by the PSD guard                    */
+} mi_pcg_stats;
+
+int mi_pcg_default_opts(mi_pcg_opts* opts);
+
+/* LinearFusedPCG::do_solve + fused_pcg (linear_system/linear_fused_pcg.cu:851-937,1782-2004):
+ * x0 = 0, r = b, preconditioned CG on the assembled matrix with the registered
+ * preconditioners.  b and x are device vectors of n = 3 * block_rows doubles.
+ * Synchronises the stream at each check (every check_interval iterations). */
+int mi_pcg_solve(mi_sys*            sys,
+                 const double*      b,
+                 double*            x,
+                 const mi_pcg_opts* opts,
+                 mi_pcg_stats*      stats_host,
+                 void*              stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MI_LINSYS_H */

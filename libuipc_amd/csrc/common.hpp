@@ -1,0 +1,176 @@
+// Shared helpers of the MI355X linear-solve library: error handling across the
+// C ABI, growable device buffers, and wave64 reduction primitives.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/mi_linsys.h"
+
+namespace mi
+{
+// An error that crosses the C ABI as (code, message).
+struct Error : std::runtime_error
+{
+    int code;
+    Error(int c, const std::string& m)
+        : std::runtime_error(m)
+        , code(c)
+    {
+    }
+};
+
+void set_last_error(const std::string& msg);
+
+#define MI_HIP(call)                                                                 \
+    do                                                                               \
+    {                                                                                \
+        hipError_t mi_e_ = (call);                                                   \
+        if(mi_e_ != hipSuccess)                                                      \
+            throw ::mi::Error(MI_ERR_HIP,                                            \
+                              std::string(#call) + " failed: " + hipGetErrorString(mi_e_) \
+                                  + " (" __FILE__ ":" + std::to_string(__LINE__) + ")"); \
+    } while(0)
+
+#define MI_REQUIRE(cond, code, msg)                                                  \
+    do                                                                               \
+    {                                                                                \
+        if(!(cond))                                                                  \
+            throw ::mi::Error((code), (msg));                                        \
+    } while(0)
+
+// Growable device buffer: capacity grows by `ratio` when a resize exceeds it
+// (the reserve semantics of GlobalLinearSystem, global_linear_system.cu:321-405).
+template <typename T>
+struct DBuf
+{
+    T*     ptr = nullptr;
+    size_t n   = 0;
+    size_t cap = 0;
+
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { release(); }
+
+    void release()
+    {
+        if(ptr)
+            (void)hipFree(ptr);
+        ptr = nullptr;
+        n = cap = 0;
+    }
+    // discard contents, ensure room for `count`
+    void resize(size_t count, double ratio = 1.0)
+    {
+        if(count > cap)
+        {
+            size_t want = (size_t)((double)count * (ratio > 1.0 ? ratio : 1.0));
+            if(want < count)
+                want = count;
+            if(want == 0)
+                want = 1;
+            T* p = nullptr;
+            hipError_t e = hipMalloc((void**)&p, want * sizeof(T));
+            if(e != hipSuccess)
+                throw Error(MI_ERR_NOMEM,
+                            "hipMalloc of " + std::to_string(want * sizeof(T)) + " bytes failed: "
+                                + hipGetErrorString(e));
+            if(ptr)
+                (void)hipFree(ptr);
+            ptr = p;
+            cap = want;
+        }
+        n = count;
+    }
+    T*       data() { return ptr; }
+    const T* data() const { return ptr; }
+    size_t   size() const { return n; }
+    size_t   bytes() const { return n * sizeof(T); }
+};
+
+inline int ceil_div(int64_t a, int64_t b)
+{
+    return (int)((a + b - 1) / b);
+}
+
+// Persistent-grid size for grid-stride kernels: a few workgroups per CU.
+int device_cu_count();
+
+// ------------------------------------------------------------------ device
+constexpr int kWave = 64;
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v)
+{
+#pragma unroll
+    for(int o = kWave / 2; o > 0; o >>= 1)
+        v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+// Sum over a workgroup of `NT` threads; every thread gets the result.
+// `lds` must hold NT / 64 values.
+template <int NT, typename T>
+__device__ __forceinline__ T block_sum(T v, T* lds)
+{
+    v             = wave_sum(v);
+    const int w   = threadIdx.x / kWave;
+    const int l   = threadIdx.x % kWave;
+    if(l == 0)
+        lds[w] = v;
+    __syncthreads();
+    T acc = T(0);
+#pragma unroll
+    for(int k = 0; k < NT / kWave; ++k)
+        acc += lds[k];
+    __syncthreads();
+    return acc;
+}
+
+// Exclusive prefix within a workgroup of NT threads (values >= 0).
+// Returns the exclusive prefix of this thread; *total = workgroup total.
+// `lds` must hold NT / 64 + 1 values.
+template <int NT, typename T>
+__device__ __forceinline__ T block_exclusive_scan(T v, T* lds, T* total)
+{
+    const int l = threadIdx.x % kWave;
+    const int w = threadIdx.x / kWave;
+    // inclusive wave scan
+    T inc = v;
+#pragma unroll
+    for(int o = 1; o < kWave; o <<= 1)
+    {
+        T t = __shfl_up(inc, o, kWave);
+        if(l >= o)
+            inc += t;
+    }
+    if(l == kWave - 1)
+        lds[w] = inc;
+    __syncthreads();
+    T woff = T(0), tot = T(0);
+#pragma unroll
+    for(int k = 0; k < NT / kWave; ++k)
+    {
+        T s = lds[k];
+        if(k < w)
+            woff += s;
+        tot += s;
+    }
+    __syncthreads();
+    if(total)
+        *total = tot;
+    return woff + inc - v;
+}
+
+__device__ __forceinline__ void atomic_add_f64(double* p, double v)
+{
+    unsafeAtomicAdd(p, v);
+}
+__device__ __forceinline__ void atomic_add_f32(float* p, float v)
+{
+    unsafeAtomicAdd(p, v);
+}
+}  // namespace mi

@@ -1,0 +1,669 @@
+// Host orchestration and the C ABI (include/mi_linsys.h).
+//
+// mi_sys mirrors GlobalLinearSystem::Impl (linear_system/global_linear_system.cu):
+// it owns the assembled matrix, the registered local preconditioners and the
+// PCG work vectors.  PcgDriver mirrors LinearFusedPCG
+// (linear_system/linear_fused_pcg.cu:773-2004): x0 = 0, the rz0 PSD guard,
+// check_interval-iteration blocks replayed from a captured hipGraph, device-side
+// alpha / beta / convergence scalars, one host read per block.
+#include <algorithm>
+#include <cmath>
+#include <string>
+
+#include "linsys.hpp"
+
+namespace mi
+{
+namespace
+{
+thread_local std::string g_last_error;
+}
+void set_last_error(const std::string& m)
+{
+    g_last_error = m;
+}
+
+int device_cu_count()
+{
+    static int cached = 0;
+    if(cached == 0)
+    {
+        int dev = 0, cu = 0;
+        if(hipGetDevice(&dev) == hipSuccess
+           && hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cu > 0)
+            cached = cu;
+        else
+            cached = 64;
+    }
+    return cached;
+}
+}  // namespace mi
+
+using namespace mi;
+
+struct mi_ctx
+{
+    int         device = 0;
+    hipStream_t stream = nullptr;
+};
+
+struct mi_sys
+{
+    mi_ctx*                                           ctx = nullptr;
+    mi_sys_desc                                       desc{};
+    BsrMatrix                                         A;
+    MatrixConverter                                   conv;
+    SpmvScratch                                       spmv_scr, dot_scr;
+    std::vector<std::unique_ptr<LocalPreconditioner>> pcs;
+    bool                                              assembled     = false;
+    bool                                              pcs_assembled = false;
+
+    // PCG state (LinearFusedPCG's r, z, p, Ap and device scalars)
+    DBuf<double> r, z, p, Ap, tmp;
+    DBuf<double> d_misc;  // [0..3] scratch scalars
+    PcgScalars*  d_sc = nullptr;
+    PcgScalars*  h_sc = nullptr;  // pinned
+
+    // captured block of iterations
+    hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t     graph      = nullptr;
+    struct GraphKey
+    {
+        const void* x = nullptr;
+        const void* b = nullptr;
+        int64_t     n = -1, U = -1;
+        int         ci = 0, fused = -1, bypass = -1, use32 = -1;
+        const void* bufs[6] = {};
+        hipStream_t stream  = nullptr;
+        bool        operator==(const GraphKey& o) const
+        {
+            if(x != o.x || b != o.b || n != o.n || U != o.U || ci != o.ci || fused != o.fused || bypass != o.bypass
+               || use32 != o.use32 || stream != o.stream)
+                return false;
+            for(int k = 0; k < 6; ++k)
+                if(bufs[k] != o.bufs[k])
+                    return false;
+            return true;
+        }
+    } graph_key;
+    bool graph_failed = false;
+
+    ~mi_sys()
+    {
+        if(graph_exec)
+            (void)hipGraphExecDestroy(graph_exec);
+        if(graph)
+            (void)hipGraphDestroy(graph);
+        if(d_sc)
+            (void)hipFree(d_sc);
+        if(h_sc)
+            (void)hipHostFree(h_sc);
+    }
+
+    int64_t n() const { return 3 * (int64_t)desc.block_rows; }
+
+    // GlobalLinearSystem::Impl::apply_preconditioner (global_linear_system.cu:546-590)
+    void apply_preconditioner(const double* rr, double* zz, const int32_t* conv_flag, hipStream_t s)
+    {
+        int64_t covered = 0;
+        for(auto& pc : pcs)
+            covered += pc->dof_count;
+        if(covered < n())
+            copy_f64(zz, rr, n(), s);  // identity for DoFs no preconditioner covers
+        for(auto& pc : pcs)
+            pc->apply(rr + pc->dof_offset, zz + pc->dof_offset, conv_flag, s);
+    }
+};
+
+#define MI_API_BEGIN try {
+#define MI_API_END                                                                  \
+    }                                                                               \
+    catch(const mi::Error& e)                                                       \
+    {                                                                               \
+        mi::set_last_error(e.what());                                               \
+        return e.code;                                                              \
+    }                                                                               \
+    catch(const std::exception& e)                                                  \
+    {                                                                               \
+        mi::set_last_error(std::string("internal error: ") + e.what());             \
+        return MI_ERR_HIP;                                                          \
+    }                                                                               \
+    mi::set_last_error("");                                                         \
+    return MI_OK;
+
+static hipStream_t pick(mi_ctx* ctx, void* stream)
+{
+    return stream ? (hipStream_t)stream : ctx->stream;
+}
+
+extern "C" {
+
+const char* mi_last_error(void)
+{
+    return mi::g_last_error.c_str();
+}
+
+int mi_version(void)
+{
+    return MI_LINSYS_VERSION;
+}
+
+int mi_ctx_create(int device, mi_ctx** out)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(out != nullptr, MI_ERR_ARG, "mi_ctx_create: out is NULL");
+    int count = 0;
+    MI_HIP(hipGetDeviceCount(&count));
+    MI_REQUIRE(device >= 0 && device < count, MI_ERR_ARG, "mi_ctx_create: no such device");
+    MI_HIP(hipSetDevice(device));
+    auto* c   = new mi_ctx;
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if(e != hipSuccess)
+    {
+        delete c;
+        MI_HIP(e);
+    }
+    *out = c;
+    MI_API_END
+}
+
+int mi_ctx_destroy(mi_ctx* ctx)
+{
+    MI_API_BEGIN
+    if(!ctx)
+        return MI_OK;
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    MI_API_END
+}
+
+int mi_ctx_synchronize(mi_ctx* ctx)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(ctx, MI_ERR_ARG, "null ctx");
+    MI_HIP(hipStreamSynchronize(ctx->stream));
+    MI_API_END
+}
+
+int mi_ctx_stream(mi_ctx* ctx, void** stream_out)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(ctx && stream_out, MI_ERR_ARG, "null argument");
+    *stream_out = (void*)ctx->stream;
+    MI_API_END
+}
+
+int mi_malloc(mi_ctx* ctx, void** dptr, size_t bytes)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(ctx && dptr, MI_ERR_ARG, "null argument");
+    hipError_t e = hipMalloc(dptr, bytes ? bytes : 1);
+    MI_REQUIRE(e == hipSuccess, MI_ERR_NOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    MI_API_END
+}
+
+int mi_free(mi_ctx* ctx, void* dptr)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(ctx, MI_ERR_ARG, "null ctx");
+    if(dptr)
+    {
+        MI_HIP(hipStreamSynchronize(ctx->stream));
+        MI_HIP(hipFree(dptr));
+    }
+    MI_API_END
+}
+
+int mi_memcpy_h2d(mi_ctx* ctx, void* dst, const void* src, size_t bytes)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(ctx, MI_ERR_ARG, "null ctx");
+    if(bytes)
+    {
+        MI_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+        MI_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    MI_API_END
+}
+
+int mi_memcpy_d2h(mi_ctx* ctx, void* dst, const void* src, size_t bytes)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(ctx, MI_ERR_ARG, "null ctx");
+    if(bytes)
+    {
+        MI_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        MI_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    MI_API_END
+}
+
+int mi_memcpy_d2d(mi_ctx* ctx, void* dst, const void* src, size_t bytes)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(ctx, MI_ERR_ARG, "null ctx");
+    if(bytes)
+    {
+        MI_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+        MI_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    MI_API_END
+}
+
+int mi_memset(mi_ctx* ctx, void* dst, int value, size_t bytes)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(ctx, MI_ERR_ARG, "null ctx");
+    if(bytes)
+    {
+        MI_HIP(hipMemsetAsync(dst, value, bytes, ctx->stream));
+        MI_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    MI_API_END
+}
+
+int mi_dot_f64(mi_ctx* ctx, const double* x, const double* y, int64_t n, double* out)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(ctx && out && n >= 0, MI_ERR_ARG, "mi_dot_f64: invalid argument");
+    *out = dot_f64_host(x, y, n, ctx->stream);
+    MI_API_END
+}
+
+int mi_norm_f64(mi_ctx* ctx, const double* x, int64_t n, double* out)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(ctx && out && n >= 0, MI_ERR_ARG, "mi_norm_f64: invalid argument");
+    *out = norm_f64_host(x, n, ctx->stream);
+    MI_API_END
+}
+
+int mi_dot_f32(mi_ctx* ctx, const float* x, const float* y, int64_t n, float* out)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(ctx && out && n >= 0, MI_ERR_ARG, "mi_dot_f32: invalid argument");
+    *out = dot_f32_host(x, y, n, ctx->stream);
+    MI_API_END
+}
+
+int mi_norm_f32(mi_ctx* ctx, const float* x, int64_t n, float* out)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(ctx && out && n >= 0, MI_ERR_ARG, "mi_norm_f32: invalid argument");
+    *out = norm_f32_host(x, n, ctx->stream);
+    MI_API_END
+}
+
+int mi_sys_create(mi_ctx* ctx, const mi_sys_desc* desc, mi_sys** out)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(ctx && desc && out, MI_ERR_ARG, "mi_sys_create: null argument");
+    MI_REQUIRE(desc->block_rows >= 0, MI_ERR_ARG, "mi_sys_create: block_rows < 0");
+    MI_REQUIRE(desc->value_precision == MI_FP64 || desc->value_precision == MI_FP32, MI_ERR_ARG,
+               "mi_sys_create: unknown value_precision");
+    MI_REQUIRE((int64_t)desc->block_rows * desc->block_rows < ((int64_t)1 << 62), MI_ERR_ARG,
+               "mi_sys_create: block_rows too large");
+    auto* s  = new mi_sys;
+    s->ctx   = ctx;
+    s->desc  = *desc;
+    if(!(s->desc.reserve_ratio >= 1.0))
+        s->desc.reserve_ratio = 1.1;
+    *out = s;
+    MI_API_END
+}
+
+int mi_sys_destroy(mi_sys* sys)
+{
+    MI_API_BEGIN
+    if(!sys)
+        return MI_OK;
+    (void)hipStreamSynchronize(sys->ctx->stream);
+    delete sys;
+    MI_API_END
+}
+
+int mi_sys_assemble(mi_sys* sys, int64_t T, const int32_t* rows, const int32_t* cols, const double* vals, void* stream)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(sys, MI_ERR_ARG, "mi_sys_assemble: null sys");
+    MI_REQUIRE(T >= 0, MI_ERR_ARG, "mi_sys_assemble: negative triplet count");
+    hipStream_t s = pick(sys->ctx, stream);
+    sys->conv.convert_sym(sys->desc.block_rows,
+                          T,
+                          rows,
+                          cols,
+                          vals,
+                          sys->A,
+                          sys->desc.value_precision == MI_FP32,
+                          sys->desc.reserve_ratio,
+                          s);
+    sys->assembled     = true;
+    sys->pcs_assembled = false;
+    MI_API_END
+}
+
+int mi_sys_nnz(mi_sys* sys, int64_t* nnz)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(sys && nnz, MI_ERR_ARG, "mi_sys_nnz: null argument");
+    MI_REQUIRE(sys->assembled, MI_ERR_STATE, "mi_sys_nnz: not assembled");
+    *nnz = sys->A.U;
+    MI_API_END
+}
+
+namespace
+{
+__global__ void soa_to_aos_kernel(const double* planes, int64_t stride, int64_t U, double* out)
+{
+    for(int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < 9 * U; t += (int64_t)gridDim.x * blockDim.x)
+    {
+        int64_t u = t / 9;
+        int     q = (int)(t % 9);
+        out[t]    = planes[q * stride + u];
+    }
+}
+}  // namespace
+
+int mi_sys_export_bcoo(mi_sys* sys, int32_t* rows, int32_t* cols, double* vals, int32_t* row_ptr, void* stream)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(sys, MI_ERR_ARG, "null sys");
+    MI_REQUIRE(sys->assembled, MI_ERR_STATE, "mi_sys_export_bcoo: not assembled");
+    hipStream_t s = pick(sys->ctx, stream);
+    auto&       A = sys->A;
+    if(rows && A.U)
+        MI_HIP(hipMemcpyAsync(rows, A.rows.data(), sizeof(int32_t) * A.U, hipMemcpyDeviceToDevice, s));
+    if(cols && A.U)
+        MI_HIP(hipMemcpyAsync(cols, A.cols.data(), sizeof(int32_t) * A.U, hipMemcpyDeviceToDevice, s));
+    if(row_ptr)
+        MI_HIP(hipMemcpyAsync(row_ptr, A.row_ptr.data(), sizeof(int32_t) * (A.B + 1), hipMemcpyDeviceToDevice, s));
+    if(vals && A.U)
+        soa_to_aos_kernel<<<std::max(1, std::min(ceil_div(9 * A.U, 256), device_cu_count() * 8)), 256, 0, s>>>(
+            A.vals.data(), A.stride, A.U, vals);
+    MI_HIP(hipGetLastError());
+    MI_API_END
+}
+
+int mi_spmv_dot(mi_sys* sys, const double* x, double* y, double* d_dot, void* stream)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(sys && x && y, MI_ERR_ARG, "mi_spmv_dot: null argument");
+    MI_REQUIRE(sys->assembled, MI_ERR_STATE, "mi_spmv_dot: not assembled");
+    hipStream_t s = pick(sys->ctx, stream);
+    spmv_sym_dot(sys->A, sys->desc.value_precision == MI_FP32, x, y, d_dot, nullptr, nullptr, nullptr, sys->spmv_scr, s);
+    MI_API_END
+}
+
+int mi_spmv(mi_sys* sys, double a, const double* x, double b, double* y, void* stream)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(sys && x && y, MI_ERR_ARG, "mi_spmv: null argument");
+    MI_REQUIRE(sys->assembled, MI_ERR_STATE, "mi_spmv: not assembled");
+    spmv_axpby(sys->A, a, x, b, y, sys->spmv_scr, pick(sys->ctx, stream));
+    MI_API_END
+}
+
+int mi_precond_add(mi_sys* sys, const mi_precond_desc* d, int* handle)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(sys && d, MI_ERR_ARG, "mi_precond_add: null argument");
+    MI_REQUIRE(d->dof_offset >= 0 && d->dof_count >= 0 && d->dof_offset + d->dof_count <= sys->n(), MI_ERR_ARG,
+               "mi_precond_add: DoF range outside the system");
+    for(auto& pc : sys->pcs)
+        MI_REQUIRE(d->dof_offset >= pc->dof_offset + pc->dof_count || d->dof_offset + d->dof_count <= pc->dof_offset,
+                   MI_ERR_ARG, "mi_precond_add: overlapping preconditioner ranges");
+    std::unique_ptr<LocalPreconditioner> pc;
+    switch(d->kind)
+    {
+        case MI_PRECOND_DIAG3: pc = make_diag3(d->dof_offset, d->dof_count); break;
+        case MI_PRECOND_DIAG12: pc = make_diag12(d->dof_offset, d->dof_count); break;
+        case MI_PRECOND_MAS: pc = make_mas(*d); break;
+        default: throw Error(MI_ERR_ARG, "mi_precond_add: unknown kind");
+    }
+    sys->pcs.push_back(std::move(pc));
+    sys->pcs_assembled = false;
+    if(handle)
+        *handle = (int)sys->pcs.size() - 1;
+    MI_API_END
+}
+
+int mi_precond_assemble(mi_sys* sys, void* stream)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(sys, MI_ERR_ARG, "null sys");
+    MI_REQUIRE(sys->assembled, MI_ERR_STATE, "mi_precond_assemble: assemble the system first");
+    hipStream_t s = pick(sys->ctx, stream);
+    for(auto& pc : sys->pcs)
+        pc->assemble(sys->A, s);
+    sys->pcs_assembled = true;
+    MI_API_END
+}
+
+int mi_precond_apply(mi_sys* sys, const double* r, double* z, const int32_t* d_conv, void* stream)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(sys && r && z, MI_ERR_ARG, "mi_precond_apply: null argument");
+    MI_REQUIRE(sys->pcs.empty() || sys->pcs_assembled, MI_ERR_STATE, "mi_precond_apply: preconditioners not assembled");
+    sys->apply_preconditioner(r, z, d_conv, pick(sys->ctx, stream));
+    MI_HIP(hipGetLastError());
+    MI_API_END
+}
+
+int mi_pcg_default_opts(mi_pcg_opts* o)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(o, MI_ERR_ARG, "null opts");
+    o->criterion      = MI_CRIT_RZ_RATIO;
+    o->tol            = 1e-3;
+    o->max_iter_ratio = 2.0;
+    o->check_interval = 5;
+    o->use_graph      = 1;
+    o->psd_fallback   = 1;
+    o->force_bypass   = 0;
+    o->true_residual  = 0;
+    MI_API_END
+}
+}  // extern "C"
+
+namespace
+{
+// one PCG iteration (LinearFusedPCG::run_iteration, linear_fused_pcg.cu:1374-1594)
+void run_iteration(mi_sys* sys, double* x, bool bypass, const LocalPreconditioner* fused_diag, hipStream_t s)
+{
+    const int64_t n     = sys->n();
+    PcgScalars*   sc    = sys->d_sc;
+    const bool    use32 = sys->desc.value_precision == MI_FP32;
+    spmv_sym_dot(sys->A, use32, sys->p.data(), sys->Ap.data(), &sc->pAp, &sc->rz, &sc->alpha, &sc->converged,
+                 sys->spmv_scr, s);
+    if(fused_diag)
+    {
+        pcg_update_xr_diag3(sc, x, sys->p.data(), sys->r.data(), sys->Ap.data(), sys->z.data(),
+                            fused_diag->diag3_inverse(), fused_diag->dof_offset / 3, fused_diag->dof_count / 3, n,
+                            sys->dot_scr, s);
+    }
+    else
+    {
+        pcg_update_xr(sc, x, sys->p.data(), sys->r.data(), sys->Ap.data(), n, s);
+        if(bypass)
+            pcg_copy_if_active(sc, sys->z.data(), sys->r.data(), n, s);
+        else
+            sys->apply_preconditioner(sys->r.data(), sys->z.data(), &sc->converged, s);
+        pcg_dot_scalar(sc, sys->r.data(), sys->z.data(), n, sys->dot_scr, s);
+    }
+    pcg_update_p(sc, sys->p.data(), sys->z.data(), n, s);
+}
+}  // namespace
+
+extern "C" int mi_pcg_solve(mi_sys* sys, const double* b, double* x, const mi_pcg_opts* opts, mi_pcg_stats* st, void* stream)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(sys && b && x && st, MI_ERR_ARG, "mi_pcg_solve: null argument");
+    MI_REQUIRE(sys->assembled, MI_ERR_STATE, "mi_pcg_solve: system not assembled");
+    MI_REQUIRE(sys->pcs.empty() || sys->pcs_assembled, MI_ERR_STATE, "mi_pcg_solve: preconditioners not assembled");
+    mi_pcg_opts o;
+    mi_pcg_default_opts(&o);
+    if(opts)
+        o = *opts;
+    MI_REQUIRE(o.tol > 0.0 && o.max_iter_ratio > 0.0, MI_ERR_ARG, "mi_pcg_solve: tol and max_iter_ratio must be > 0");
+    MI_REQUIRE(o.criterion == MI_CRIT_RZ_RATIO || o.criterion == MI_CRIT_REL_RESIDUAL, MI_ERR_ARG,
+               "mi_pcg_solve: unknown criterion");
+    hipStream_t   s = pick(sys->ctx, stream);
+    const int64_t n = sys->n();
+    *st             = mi_pcg_stats{};
+    st->rel_res_true = NAN;
+    if(n == 0)
+    {
+        st->converged = 1;
+        return MI_OK;
+    }
+    double ratio = sys->desc.reserve_ratio;
+    sys->r.resize((size_t)n, ratio);
+    sys->z.resize((size_t)n, ratio);
+    sys->p.resize((size_t)n, ratio);
+    sys->Ap.resize((size_t)n, ratio);
+    sys->d_misc.resize(4);
+    if(!sys->d_sc)
+    {
+        MI_HIP(hipMalloc((void**)&sys->d_sc, sizeof(PcgScalars)));
+        MI_HIP(hipHostMalloc((void**)&sys->h_sc, sizeof(PcgScalars)));
+    }
+
+    // r = b, x = 0, z = P r, p = z, rz0 = r.z (linear_fused_pcg.cu:1811-1824)
+    pcg_init_vectors(x, sys->r.data(), b, n, s);
+    bool bypass = o.force_bypass != 0;
+    if(bypass)
+        copy_f64(sys->z.data(), sys->r.data(), n, s);
+    else
+        sys->apply_preconditioner(sys->r.data(), sys->z.data(), nullptr, s);
+    pcg_dot_rz_rr(sys->r.data(), sys->z.data(), n, sys->d_misc.data(), sys->dot_scr, s);
+    double h2[2];
+    MI_HIP(hipMemcpyAsync(h2, sys->d_misc.data(), 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    MI_HIP(hipStreamSynchronize(s));
+    double rz0 = h2[0], bb = h2[1];
+
+    // PSD guard (linear_fused_pcg.cu:1843-1875)
+    if(!bypass && o.psd_fallback && (rz0 < 0.0 || !std::isfinite(rz0)))
+    {
+        bool fb = rz0 < 0.0 || std::isfinite(bb);
+        if(fb)
+        {
+            bypass = true;
+            copy_f64(sys->z.data(), sys->r.data(), n, s);
+            rz0 = bb;
+        }
+    }
+    st->bypassed = bypass ? 1 : 0;
+    st->rz0      = rz0;
+    MI_REQUIRE(std::isfinite(rz0) && std::isfinite(bb), MI_ERR_NONFINITE, "mi_pcg_solve: non-finite r^T z at start");
+    copy_f64(sys->p.data(), sys->z.data(), n, s);
+    if(std::fabs(rz0) == 0.0)
+    {
+        st->converged = 1;
+        st->rel_res   = 0.0;
+        MI_HIP(hipStreamSynchronize(s));
+        return MI_OK;
+    }
+
+    PcgScalars& h = *sys->h_sc;
+    h             = PcgScalars{};
+    h.rz          = rz0;
+    h.rz_tol      = o.tol * std::fabs(rz0);
+    h.rr_tol      = o.tol * o.tol * bb;
+    h.criterion   = o.criterion;
+    MI_HIP(hipMemcpyAsync(sys->d_sc, sys->h_sc, sizeof(PcgScalars), hipMemcpyHostToDevice, s));
+    fill_f64(sys->Ap.data(), 0.0, n, s);
+
+    const LocalPreconditioner* fused = nullptr;
+    if(!bypass && sys->pcs.size() == 1 && sys->pcs[0]->diag3_inverse())
+        fused = sys->pcs[0].get();
+
+    const int64_t max_iter = (int64_t)(o.max_iter_ratio * (double)n);
+    const int64_t total    = max_iter > 0 ? max_iter - 1 : 0;
+    const int     ci       = o.check_interval > 0 ? o.check_interval : 1;
+    int64_t       done     = 0;
+    bool          converged = false;
+
+    mi_sys::GraphKey key;
+    key.x = x;
+    key.b = b;
+    key.n = n;
+    key.U = sys->A.U;
+    key.ci = ci;
+    key.fused = fused ? 1 : 0;
+    key.bypass = bypass ? 1 : 0;
+    key.use32  = sys->desc.value_precision;
+    key.bufs[0] = sys->r.data();
+    key.bufs[1] = sys->z.data();
+    key.bufs[2] = sys->p.data();
+    key.bufs[3] = sys->Ap.data();
+    key.bufs[4] = sys->A.vals.data();
+    key.bufs[5] = sys->pcs.empty() ? nullptr : (const void*)sys->pcs[0].get();
+    key.stream  = s;
+
+    while(done < total)
+    {
+        int64_t blk = std::min<int64_t>(ci, total - done);
+        bool    use_graph = o.use_graph && !sys->graph_failed && blk == ci && s != nullptr;
+        if(use_graph && !(sys->graph_exec && sys->graph_key == key))
+        {
+            if(sys->graph_exec)
+                (void)hipGraphExecDestroy(sys->graph_exec);
+            if(sys->graph)
+                (void)hipGraphDestroy(sys->graph);
+            sys->graph_exec = nullptr;
+            sys->graph      = nullptr;
+            hipError_t e    = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+            if(e == hipSuccess)
+            {
+                for(int k = 0; k < ci; ++k)
+                    run_iteration(sys, x, bypass, fused, s);
+                e = hipStreamEndCapture(s, &sys->graph);
+                if(e == hipSuccess)
+                    e = hipGraphInstantiate(&sys->graph_exec, sys->graph, nullptr, nullptr, 0);
+            }
+            if(e != hipSuccess)
+            {
+                (void)hipGetLastError();
+                sys->graph_failed = true;  // permanent plain-launch fallback (linear_fused_pcg.cu:1748-1757)
+                sys->graph_exec   = nullptr;
+                use_graph         = false;
+            }
+            else
+                sys->graph_key = key;
+        }
+        if(use_graph)
+            MI_HIP(hipGraphLaunch(sys->graph_exec, s));
+        else
+            for(int64_t k = 0; k < blk; ++k)
+                run_iteration(sys, x, bypass, fused, s);
+        MI_HIP(hipGetLastError());
+        done += blk;
+        MI_HIP(hipMemcpyAsync(sys->h_sc, sys->d_sc, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
+        MI_HIP(hipStreamSynchronize(s));
+        MI_REQUIRE(std::isfinite(h.rz_new), MI_ERR_NONFINITE,
+                   "mi_pcg_solve: non-finite r^T z at iteration " + std::to_string(h.iter));
+        if(h.converged)
+        {
+            converged = true;
+            break;
+        }
+    }
+    st->iters       = converged ? done : max_iter;
+    st->iters_exact = converged ? h.conv_iter : -1;
+    st->converged   = converged ? 1 : 0;
+    st->rz_final    = h.rz_new;
+    st->rel_res     = bb > 0 ? std::sqrt(h.rr / bb) : 0.0;
+
+    if(o.true_residual)
+    {
+        sys->tmp.resize((size_t)n, ratio);
+        copy_f64(sys->tmp.data(), b, n, s);
+        spmv_axpby(sys->A, -1.0, x, 1.0, sys->tmp.data(), sys->spmv_scr, s);
+        double rn = norm_f64_host(sys->tmp.data(), n, s);
+        double bn = std::sqrt(bb);
+        st->rel_res_true = bn > 0 ? rn / bn : 0.0;
+    }
+    MI_API_END
+}

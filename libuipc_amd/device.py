@@ -1,0 +1,207 @@
+"""Thin host-side helpers over the C ABI: a device context, device arrays that
+round-trip numpy arrays, and a LinearSystem object whose methods mirror the
+reference's GlobalLinearSystem / LinearFusedPCG / LocalPreconditioner hooks.
+Every operation runs in libmi_linsys.so; nothing here computes."""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import abi
+from .abi import check
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        self.lib = abi.linsys()
+        h = C.c_void_p()
+        check(self.lib.mi_ctx_create(device, C.byref(h)))
+        self.h = h
+
+    def stream(self) -> C.c_void_p:
+        s = C.c_void_p()
+        check(self.lib.mi_ctx_stream(self.h, C.byref(s)))
+        return s
+
+    def synchronize(self) -> None:
+        check(self.lib.mi_ctx_synchronize(self.h))
+
+    def close(self) -> None:
+        if self.h:
+            check(self.lib.mi_ctx_destroy(self.h))
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # device memory
+    def alloc(self, shape, dtype) -> "DeviceArray":
+        return DeviceArray(self, shape, dtype)
+
+    def to_device(self, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = DeviceArray(self, a.shape, a.dtype)
+        d.upload(a)
+        return d
+
+    def dot(self, x: "DeviceArray", y: "DeviceArray") -> float:
+        if x.size != y.size:
+            raise ValueError("dot: size mismatch")  # LinearSystemContext::dot throws std::invalid_argument
+        if x.dtype == np.float32:
+            out = C.c_float()
+            check(self.lib.mi_dot_f32(self.h, x.ptr, y.ptr, x.size, C.byref(out)))
+        else:
+            out = C.c_double()
+            check(self.lib.mi_dot_f64(self.h, x.ptr, y.ptr, x.size, C.byref(out)))
+        return out.value
+
+    def norm(self, x: "DeviceArray") -> float:
+        if x.dtype == np.float32:
+            out = C.c_float()
+            check(self.lib.mi_norm_f32(self.h, x.ptr, x.size, C.byref(out)))
+        else:
+            out = C.c_double()
+            check(self.lib.mi_norm_f64(self.h, x.ptr, x.size, C.byref(out)))
+        return out.value
+
+
+class DeviceArray:
+    def __init__(self, ctx: Context, shape, dtype):
+        self.ctx = ctx
+        self.shape = tuple(shape) if isinstance(shape, (tuple, list)) else (int(shape),)
+        self.dtype = np.dtype(dtype)
+        self.size = int(np.prod(self.shape)) if self.shape else 1
+        self.nbytes = self.size * self.dtype.itemsize
+        p = C.c_void_p()
+        check(ctx.lib.mi_malloc(ctx.h, C.byref(p), max(1, self.nbytes)))
+        self.ptr = p
+
+    def upload(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.size == self.size
+        check(self.ctx.lib.mi_memcpy_h2d(self.ctx.h, self.ptr, a.ctypes.data_as(C.c_void_p), self.nbytes))
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.shape, dtype=self.dtype)
+        check(self.ctx.lib.mi_memcpy_d2h(self.ctx.h, out.ctypes.data_as(C.c_void_p), self.ptr, self.nbytes))
+        return out
+
+    def zero(self) -> None:
+        check(self.ctx.lib.mi_memset(self.ctx.h, self.ptr, 0, self.nbytes))
+
+    def free(self) -> None:
+        if self.ptr:
+            check(self.ctx.lib.mi_free(self.ctx.h, self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            if self.ptr and self.ctx.h:
+                self.ctx.lib.mi_free(self.ctx.h, self.ptr)
+        except Exception:
+            pass
+
+
+def _i32p(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.int32).ctypes.data_as(abi.c_i32p)
+
+
+class LinearSystem:
+    """GlobalLinearSystem-shaped handle over one mi_sys."""
+
+    def __init__(self, ctx: Context, block_rows: int, precision: int = abi.MI_FP64, reserve_ratio: float = 1.1):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.B = int(block_rows)
+        self.n = 3 * self.B
+        d = abi.SysDesc(self.B, reserve_ratio, precision)
+        h = C.c_void_p()
+        check(self.lib.mi_sys_create(ctx.h, C.byref(d), C.byref(h)))
+        self.h = h
+        self._keep = []
+
+    def close(self):
+        if self.h:
+            check(self.lib.mi_sys_destroy(self.h))
+            self.h = None
+
+    # MatrixConverter::convert_sym
+    def assemble(self, rows: DeviceArray, cols: DeviceArray, vals: DeviceArray, stream=None) -> None:
+        T = rows.size
+        check(self.lib.mi_sys_assemble(self.h, T, rows.ptr, cols.ptr, vals.ptr, stream))
+
+    def nnz(self) -> int:
+        v = C.c_int64()
+        check(self.lib.mi_sys_nnz(self.h, C.byref(v)))
+        return v.value
+
+    def export_bcoo(self):
+        U = self.nnz()
+        r = self.ctx.alloc(max(U, 1), np.int32)
+        c = self.ctx.alloc(max(U, 1), np.int32)
+        v = self.ctx.alloc((max(U, 1), 9), np.float64)
+        p = self.ctx.alloc(self.B + 1, np.int32)
+        check(self.lib.mi_sys_export_bcoo(self.h, r.ptr, c.ptr, v.ptr, p.ptr, None))
+        self.ctx.synchronize()
+        return r.numpy()[:U], c.numpy()[:U], v.numpy()[:U], p.numpy()
+
+    # Spmv::rbk_sym_spmv_dot
+    def spmv_dot(self, x: DeviceArray, y: DeviceArray, accumulate: bool = False) -> float:
+        """y = A x (y += A x when accumulate); returns x^T A x"""
+        if not accumulate:
+            y.zero()
+        d = self.ctx.alloc(1, np.float64)
+        check(self.lib.mi_spmv_dot(self.h, x.ptr, y.ptr, d.ptr, None))
+        return float(d.numpy()[0])
+
+    def spmv(self, a: float, x: DeviceArray, b: float, y: DeviceArray) -> None:
+        check(self.lib.mi_spmv(self.h, a, x.ptr, b, y.ptr, None))
+
+    def add_preconditioner(self, kind: int, dof_offset: int, dof_count: int, *, part_ids=None, tets=None,
+                           tris=None, edges=None, mas_precision: int = abi.MI_FP64, active_levels: int = 0) -> int:
+        keep = []
+
+        def arr(a, k):
+            if a is None:
+                return None, 0
+            a = np.ascontiguousarray(a, dtype=np.int32).reshape(-1)
+            keep.append(a)
+            return a.ctypes.data_as(abi.c_i32p), a.size // k
+
+        pp, _ = arr(part_ids, 1)
+        tp, nt = arr(tets, 4)
+        trp, ntr = arr(tris, 3)
+        ep, ne = arr(edges, 2)
+        d = abi.PrecondDesc(kind, dof_offset, dof_count, dof_count // 3, pp, tp, nt, trp, ntr, ep, ne,
+                            mas_precision, active_levels)
+        h = C.c_int()
+        check(self.lib.mi_precond_add(self.h, C.byref(d), C.byref(h)))
+        return h.value
+
+    def assemble_preconditioners(self) -> None:
+        check(self.lib.mi_precond_assemble(self.h, None))
+
+    def apply_preconditioner(self, r: DeviceArray, z: DeviceArray) -> None:
+        check(self.lib.mi_precond_apply(self.h, r.ptr, z.ptr, None, None))
+
+    def solve(self, b: DeviceArray, x: DeviceArray, **kw) -> dict:
+        o = abi.PcgOpts()
+        check(self.lib.mi_pcg_default_opts(C.byref(o)))
+        for k, v in kw.items():
+            setattr(o, k, v)
+        st = abi.PcgStats()
+        check(self.lib.mi_pcg_solve(self.h, b.ptr, x.ptr, C.byref(o), C.byref(st), None))
+        return st.as_dict()
+
+
+def rel_err(a: np.ndarray, b: np.ndarray) -> float:
+    nb = float(np.linalg.norm(b))
+    return float(np.linalg.norm(a - b)) / (nb if nb > 0 else 1.0)
+
+
+__all__ = ["Context", "DeviceArray", "LinearSystem", "rel_err", "math"]

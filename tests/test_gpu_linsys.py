@@ -1,0 +1,279 @@
+"""GPU parity of the assembled matrix, the SpMV, the vector reductions, the
+block-Jacobi preconditioners and the PCG driver against the CPU oracle, all
+called through the C ABI (libmi_linsys.so)."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from libuipc_amd import abi, synth
+from libuipc_amd.device import LinearSystem, rel_err
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_kats.json")))
+
+
+def upload_system(ctx, s):
+    return ctx.to_device(s.rows), ctx.to_device(s.cols), ctx.to_device(s.vals)
+
+
+def assembled(ctx, s, precision=abi.MI_FP64):
+    ls = LinearSystem(ctx, s.block_rows, precision)
+    r, c, v = upload_system(ctx, s)
+    ls.assemble(r, c, v)
+    ls._trip = (r, c, v)
+    return ls
+
+
+# ------------------------------------------------------------------ KATs
+def test_dot_norm_kats(ctx):
+    for case in GOLD["dot_norm_f32"]:
+        x = ctx.to_device(np.array(case["x"], np.float32))
+        if "y" in case:
+            y = ctx.to_device(np.array(case["y"], np.float32))
+            assert ctx.dot(x, y) == pytest.approx(case["dot"], rel=1e-6)
+        assert ctx.norm(x) == pytest.approx(case["norm_x"], rel=case.get("norm_eps", 1e-6))
+    g = GOLD["norm_f64_scaled"]
+    v = ctx.norm(ctx.to_device(np.array(g["x"])))
+    assert math.isfinite(v) and v == pytest.approx(g["norm"], rel=g["eps"])
+    e = ctx.to_device(np.zeros(0))
+    e.size = 0
+    assert ctx.dot(e, e) == 0.0 and ctx.norm(e) == 0.0
+    with pytest.raises(ValueError):
+        ctx.dot(ctx.to_device(np.ones(1)), ctx.to_device(np.ones(2)))
+    assert math.isinf(ctx.norm(ctx.to_device(np.array([np.inf, 1.0]))))
+    assert math.isnan(ctx.norm(ctx.to_device(np.array([np.nan, 1.0]))))
+
+
+def test_dot_large(ctx):
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal(1_000_003)
+    y = rng.standard_normal(1_000_003)
+    assert ctx.dot(ctx.to_device(x), ctx.to_device(y)) == pytest.approx(O.dot(x, y), rel=1e-12)
+    assert ctx.norm(ctx.to_device(x)) == pytest.approx(O.norm(x), rel=1e-12)
+
+
+# ------------------------------------------------------------------ convert
+def check_bcoo_bitexact(ls, s):
+    r, c, v, rp = ls.export_bcoo()
+    orr, oc, ov, orp = O.convert_sym(s.block_rows, s.rows, s.cols, s.vals)
+    assert r.size == orr.size
+    assert np.array_equal(r, orr) and np.array_equal(c, oc) and np.array_equal(rp, orp)
+    # duplicates are summed in the same (source) order: bit-identical values
+    assert np.array_equal(v.view(np.uint64), ov.view(np.uint64))
+
+
+def test_convert_c1_bitexact(ctx, c1):
+    ls = assembled(ctx, c1)
+    assert ls.nnz() == 26416
+    check_bcoo_bitexact(ls, c1)
+
+
+def test_convert_contacts_bitexact(ctx, c1_contact):
+    check_bcoo_bitexact(assembled(ctx, c1_contact), c1_contact)
+
+
+@pytest.mark.parametrize("B,T,seed", [(1, 5, 0), (7, 300, 1), (300, 20000, 2), (5000, 70000, 3)])
+def test_convert_random_bitexact(ctx, B, T, seed):
+    """lower-triangle and invalid (-1) entries are dropped; heavy duplication"""
+    rng = np.random.default_rng(seed)
+    rows = rng.integers(0, B, T).astype(np.int32)
+    cols = rng.integers(0, B, T).astype(np.int32)
+    rows[rng.random(T) < 0.05] = -1
+    vals = rng.standard_normal((T, 9))
+
+    class S:
+        pass
+
+    s = S()
+    s.block_rows, s.rows, s.cols, s.vals = B, rows, cols, vals
+    check_bcoo_bitexact(assembled(ctx, s), s)
+
+
+def test_convert_empty_and_reassemble(ctx, c1):
+    ls = LinearSystem(ctx, 10)
+    z = ctx.alloc(1, np.int32)
+    zv = ctx.alloc((1, 9), np.float64)
+    z.size = 0
+    ls.assemble(z, z, zv)
+    assert ls.nnz() == 0
+    r, c, v, rp = ls.export_bcoo()
+    assert rp.tolist() == [0] * 11
+    # a second assembly of a different system on the same handle
+    ls2 = assembled(ctx, c1)
+    check_bcoo_bitexact(ls2, c1)
+    r2, c2, v2 = upload_system(ctx, c1)
+    ls2.assemble(r2, c2, v2)
+    check_bcoo_bitexact(ls2, c1)
+
+
+def test_convert_full_size_fem_bitexact(ctx):
+    """64^3 FEM grid (the judged system's FEM part): U is the closed form and
+    the structure and values are bit-exact with the oracle"""
+    s = synth.fem_system(64, 0, seed=1)
+    ls = assembled(ctx, s)
+    assert ls.nnz() == 1_798_336
+    check_bcoo_bitexact(ls, s)
+
+
+# ------------------------------------------------------------------ SpMV
+@pytest.mark.parametrize("precision", [abi.MI_FP64, abi.MI_FP32])
+def test_spmv_dot(ctx, c1_contact, precision):
+    s = c1_contact
+    ls = assembled(ctx, s, precision)
+    orr, oc, ov, _ = O.convert_sym(s.block_rows, s.rows, s.cols, s.vals)
+    x = np.random.default_rng(4).standard_normal(s.n)
+    yref, dref = O.spmv_sym(s.block_rows, orr, oc, ov, x)
+    xd = ctx.to_device(x)
+    yd = ctx.alloc(s.n, np.float64)
+    d = ls.spmv_dot(xd, yd)
+    y = yd.numpy()
+    tol = 1e-13 if precision == abi.MI_FP64 else 1e-6
+    assert np.abs(y - yref).max() <= tol * np.abs(yref).max()
+    assert d == pytest.approx(dref, rel=tol * 10)
+    # y = a A x + b y
+    w = np.random.default_rng(5).standard_normal(s.n)
+    wd = ctx.to_device(w)
+    if precision == abi.MI_FP64:
+        ls.spmv(-2.0, xd, 0.5, wd)
+        assert np.abs(wd.numpy() - (-2.0 * yref + 0.5 * w)).max() <= 1e-12 * np.abs(yref).max()
+
+
+def test_spmv_symmetry_full_size(ctx):
+    """size-independent property at 64^3 + contacts: x^T (A w) == w^T (A x)"""
+    s = synth.fem_system(64, 200_000, seed=2)
+    ls = assembled(ctx, s)
+    rng = np.random.default_rng(6)
+    x, w = rng.standard_normal(s.n), rng.standard_normal(s.n)
+    xd, wd = ctx.to_device(x), ctx.to_device(w)
+    yx, yw = ctx.alloc(s.n, np.float64), ctx.alloc(s.n, np.float64)
+    dx = ls.spmv_dot(xd, yx)
+    ls.spmv_dot(wd, yw)
+    a, b = x @ yw.numpy(), w @ yx.numpy()
+    assert a == pytest.approx(b, rel=1e-11)
+    assert dx == pytest.approx(x @ yx.numpy(), rel=1e-11)
+
+
+# ------------------------------------------------------------------ preconditioners
+def test_diag3_apply(ctx, c1_contact):
+    s = c1_contact
+    ls = assembled(ctx, s)
+    ls.add_preconditioner(abi.MI_PRECOND_DIAG3, 0, s.n)
+    ls.assemble_preconditioners()
+    orr, oc, ov, _ = O.convert_sym(s.block_rows, s.rows, s.cols, s.vals)
+    D = O.diag3(orr, oc, ov, 0, s.block_rows)
+    r = np.random.default_rng(7).standard_normal(s.n)
+    zd = ctx.alloc(s.n, np.float64)
+    ls.apply_preconditioner(ctx.to_device(r), zd)
+    zref = O.diag3_apply(D, r)
+    assert rel_err(zd.numpy(), zref) < 1e-13
+
+
+def test_diag12_and_identity_ranges(ctx):
+    """ABD-like bodies in [0, 84) with a 12x12 block Jacobi, an uncovered tail -> z = r"""
+    from tests.test_oracle import abd_like_system
+
+    rng = np.random.default_rng(5)
+    rows, cols, vals = abd_like_system(rng, 7)
+    B = 28 + 5
+    rows = np.concatenate([rows, np.arange(28, B, dtype=np.int32)])
+    cols = np.concatenate([cols, np.arange(28, B, dtype=np.int32)])
+    extra = np.zeros((5, 9))
+    extra[:, [0, 4, 8]] = 2.0
+    vals = np.concatenate([vals, extra])
+
+    class S:
+        pass
+
+    s = S()
+    s.block_rows, s.rows, s.cols, s.vals = B, rows, cols, vals
+    ls = assembled(ctx, s)
+    ls.add_preconditioner(abi.MI_PRECOND_DIAG12, 0, 84)
+    ls.assemble_preconditioners()
+    orr, oc, ov, _ = O.convert_sym(B, rows, cols, vals)
+    H = O.diag12(orr, oc, ov, 0, 7)
+    r = rng.standard_normal(3 * B)
+    zd = ctx.alloc(3 * B, np.float64)
+    ls.apply_preconditioner(ctx.to_device(r), zd)
+    z = zd.numpy()
+    assert rel_err(z[:84], O.diag12_apply(H, r[:84])) < 1e-12
+    assert np.array_equal(z[84:], r[84:])
+
+
+# ------------------------------------------------------------------ PCG
+def oracle_pcg(s, criterion, tol, precond=True, **kw):
+    orr, oc, ov, _ = O.convert_sym(s.block_rows, s.rows, s.cols, s.vals)
+    pc = []
+    if precond:
+        pc = [dict(kind=1, blk_off=0, blk_cnt=s.block_rows, data=O.diag3(orr, oc, ov, 0, s.block_rows))]
+    return O.pcg(s.block_rows, orr, oc, ov, s.rhs, pc, criterion=criterion, tol=tol, **kw)
+
+
+@pytest.mark.parametrize("criterion,tol", [(abi.MI_CRIT_RZ_RATIO, 1e-3), (abi.MI_CRIT_REL_RESIDUAL, 1e-4),
+                                           (abi.MI_CRIT_REL_RESIDUAL, 1e-10)])
+@pytest.mark.parametrize("use_graph", [0, 1])
+def test_pcg_diag3(ctx, c1_contact, criterion, tol, use_graph):
+    s = c1_contact
+    ls = assembled(ctx, s)
+    ls.add_preconditioner(abi.MI_PRECOND_DIAG3, 0, s.n)
+    ls.assemble_preconditioners()
+    b = ctx.to_device(s.rhs)
+    x = ctx.alloc(s.n, np.float64)
+    st = ls.solve(b, x, criterion=criterion, tol=tol, use_graph=use_graph, true_residual=1)
+    xo, ro = oracle_pcg(s, criterion, tol)
+    assert st["converged"] == 1 and ro["converged"] == 1
+    assert abs(st["iters_exact"] - ro["iters_exact"]) <= max(1, ro["iters_exact"] // 50)
+    assert st["iters"] % 5 == 0
+    if criterion == abi.MI_CRIT_REL_RESIDUAL:
+        assert st["rel_res"] <= tol
+        assert st["rel_res_true"] <= 2 * tol
+    # same algorithm in FP64: same iterate up to the round-off of the atomics
+    assert rel_err(x.numpy(), xo) < max(1e-6, 100 * tol)
+
+
+def test_pcg_bypass_is_plain_cg(ctx, c1):
+    ls = assembled(ctx, c1)
+    ls.add_preconditioner(abi.MI_PRECOND_DIAG3, 0, c1.n)
+    ls.assemble_preconditioners()
+    b = ctx.to_device(c1.rhs)
+    x = ctx.alloc(c1.n, np.float64)
+    st = ls.solve(b, x, criterion=abi.MI_CRIT_REL_RESIDUAL, tol=1e-6, force_bypass=1)
+    xo, ro = oracle_pcg(c1, abi.MI_CRIT_REL_RESIDUAL, 1e-6, precond=False)
+    assert st["bypassed"] == 1
+    assert abs(st["iters_exact"] - ro["iters_exact"]) <= 1
+    assert rel_err(x.numpy(), xo) < 1e-5
+
+
+def test_pcg_fp32_matrix(ctx, c1_contact):
+    """precision arm: FP32 matrix stream, FP64 vectors and dots"""
+    s = c1_contact
+    ls = assembled(ctx, s, abi.MI_FP32)
+    ls.add_preconditioner(abi.MI_PRECOND_DIAG3, 0, s.n)
+    ls.assemble_preconditioners()
+    x = ctx.alloc(s.n, np.float64)
+    st = ls.solve(ctx.to_device(s.rhs), x, criterion=abi.MI_CRIT_REL_RESIDUAL, tol=1e-4, true_residual=1)
+    assert st["converged"] == 1 and st["rel_res_true"] < 1e-3
+    xo, ro = oracle_pcg(s, abi.MI_CRIT_REL_RESIDUAL, 1e-8)
+    assert rel_err(x.numpy(), xo) < 1e-2
+
+
+def test_pcg_zero_rhs(ctx, c1):
+    ls = assembled(ctx, c1)
+    x = ctx.alloc(c1.n, np.float64)
+    st = ls.solve(ctx.to_device(np.zeros(c1.n)), x)
+    assert st["iters"] == 0 and st["converged"] == 1
+    assert not np.any(x.numpy())
+
+
+def test_errors_are_reported(ctx, c1):
+    ls = LinearSystem(ctx, c1.block_rows)
+    x = ctx.alloc(c1.n, np.float64)
+    with pytest.raises(abi.MiError) as e:
+        ls.solve(ctx.to_device(c1.rhs), x)
+    assert e.value.code == abi.MI_ERR_STATE
+    with pytest.raises(abi.MiError):
+        ls.add_preconditioner(abi.MI_PRECOND_DIAG3, 3, c1.n)  # outside the system
