@@ -179,6 +179,26 @@ int mi_precond_assemble(mi_sys* sys, void* stream);
  * kernels return early, as the reference's do. */
 int mi_precond_apply(mi_sys* sys, const double* r, double* z, const int32_t* d_converged, void* stream);
 
+/* State of a MAS preconditioner for parity checks and debugging, the analogue
+ * of MASPreconditionerEngine::dump_cluster_matrices_debug
+ * (finite_element/mas_preconditioner_engine.cu:3154-3301).  Host outputs; any
+ * may be NULL.  info_host[6] = {vert_num, map_nodes, levels, total_slots,
+ * level1_begin, clusters}.  Sizes: part_to_real map_nodes, real_to_part
+ * vert_num, going_next bank_align(max(map_nodes, vert_num)) * levels,
+ * level_sizes 2 (levels + 1) as (count, begin) pairs, coarse_tables 6 vert_num,
+ * cluster_hess / cluster_inv 136 * 9 per cluster (upper 3x3 blocks,
+ * column-major; the inverse widened to double).  Synchronises. */
+int mi_precond_mas_export(mi_sys*  sys,
+                          int      handle,
+                          int32_t* info_host,
+                          int32_t* part_to_real_host,
+                          int32_t* real_to_part_host,
+                          int32_t* going_next_host,
+                          int32_t* level_sizes_host,
+                          int32_t* coarse_tables_host,
+                          double*  cluster_hess_host,
+                          double*  cluster_inv_host);
+
 /* ----------------------------------------------------------------------- PCG */
 enum mi_pcg_criterion
 {

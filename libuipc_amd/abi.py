@@ -135,6 +135,8 @@ LINSYS_PROTOS = {
     "mi_precond_add": (C.c_int, [C.c_void_p, C.POINTER(PrecondDesc), C.POINTER(C.c_int)]),
     "mi_precond_assemble": (C.c_int, [C.c_void_p, C.c_void_p]),
     "mi_precond_apply": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "mi_precond_mas_export": (C.c_int, [C.c_void_p, C.c_int, c_i32p, c_i32p, c_i32p, c_i32p, c_i32p, c_i32p, c_f64p,
+                                        c_f64p]),
     "mi_pcg_default_opts": (C.c_int, [C.POINTER(PcgOpts)]),
     "mi_pcg_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(PcgOpts), C.POINTER(PcgStats),
                                C.c_void_p]),

@@ -451,6 +451,24 @@ int mi_precond_apply(mi_sys* sys, const double* r, double* z, const int32_t* d_c
     MI_API_END
 }
 
+int mi_precond_mas_export(mi_sys*  sys,
+                          int      handle,
+                          int32_t* info,
+                          int32_t* p2r,
+                          int32_t* r2p,
+                          int32_t* gn,
+                          int32_t* ls,
+                          int32_t* ct,
+                          double*  hess,
+                          double*  inv)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(sys && handle >= 0 && handle < (int)sys->pcs.size(), MI_ERR_ARG, "mi_precond_mas_export: bad handle");
+    MI_REQUIRE(sys->pcs[handle]->mas_export(info, p2r, r2p, gn, ls, ct, hess, inv), MI_ERR_ARG,
+               "mi_precond_mas_export: not a MAS preconditioner");
+    MI_API_END
+}
+
 int mi_pcg_default_opts(mi_pcg_opts* o)
 {
     MI_API_BEGIN
@@ -574,6 +592,7 @@ extern "C" int mi_pcg_solve(mi_sys* sys, const double* b, double* x, const mi_pc
     h.criterion   = o.criterion;
     MI_HIP(hipMemcpyAsync(sys->d_sc, sys->h_sc, sizeof(PcgScalars), hipMemcpyHostToDevice, s));
     fill_f64(sys->Ap.data(), 0.0, n, s);
+    spmv_prepare(sys->spmv_scr);  // no allocation may happen inside a captured block
 
     const LocalPreconditioner* fused = nullptr;
     if(!bypass && sys->pcs.size() == 1 && sys->pcs[0]->diag3_inverse())

@@ -73,6 +73,9 @@ void spmv_sym_dot(const BsrMatrix& A,
                   SpmvScratch&     scr,
                   hipStream_t      s);
 
+// allocates the SpMV's reduction scratch (must run outside graph capture)
+void spmv_prepare(SpmvScratch& scr);
+
 // y = a A x + b y (Spmv::rbk_sym_spmv)
 void spmv_axpby(const BsrMatrix& A, double a, const double* x, double b, double* y, SpmvScratch& scr, hipStream_t s);
 
@@ -98,6 +101,11 @@ class LocalPreconditioner
     virtual void apply(const double* r, double* z, const int32_t* d_converged, hipStream_t s) = 0;
     // 3x3 diagonal-block preconditioners can be fused into the PCG update
     virtual const double* diag3_inverse() const { return nullptr; }
+    // MAS only: host copies of the hierarchy and cluster matrices
+    virtual bool mas_export(int32_t*, int32_t*, int32_t*, int32_t*, int32_t*, int32_t*, double*, double*)
+    {
+        return false;
+    }
 };
 
 std::unique_ptr<LocalPreconditioner> make_diag3(int64_t dof_offset, int64_t dof_count);

@@ -168,7 +168,7 @@ __global__ void scale_kernel(double* y, double b, int64_t n)
 }
 }  // namespace
 
-static void ensure_scratch(SpmvScratch& scr)
+void spmv_prepare(SpmvScratch& scr)
 {
     if(scr.grid == 0)
     {
@@ -202,7 +202,7 @@ void spmv_sym_dot(const BsrMatrix& A,
                   SpmvScratch&     scr,
                   hipStream_t      s)
 {
-    ensure_scratch(scr);
+    spmv_prepare(scr);
     if(A.U == 0)
     {
         if(d_dot)
@@ -250,7 +250,7 @@ void spmv_sym_dot(const BsrMatrix& A,
 
 void spmv_axpby(const BsrMatrix& A, double a, const double* x, double b, double* y, SpmvScratch& scr, hipStream_t s)
 {
-    ensure_scratch(scr);
+    spmv_prepare(scr);
     int64_t n = 3 * (int64_t)A.B;
     if(n > 0)
         scale_kernel<<<std::max(1, std::min(ceil_div(n, 256), device_cu_count() * 8)), 256, 0, s>>>(y, b, n);

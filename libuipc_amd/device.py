@@ -183,6 +183,32 @@ class LinearSystem:
         check(self.lib.mi_precond_add(self.h, C.byref(d), C.byref(h)))
         return h.value
 
+    def mas_export(self, handle: int, matrices: bool = True) -> dict:
+        info = np.zeros(6, np.int32)
+        check(self.lib.mi_precond_mas_export(self.h, handle, info.ctypes.data_as(abi.c_i32p), None, None, None, None,
+                                             None, None, None))
+        V, N, L, total, l1b, Cn = (int(v) for v in info)
+        padded = (max(N, V) + 15) // 16 * 16
+        out = {"V": V, "N": N, "L": L, "total": total, "level1_begin": l1b, "clusters": Cn,
+               "part_to_real": np.zeros(max(N, 1), np.int32), "real_to_part": np.zeros(max(V, 1), np.int32),
+               "going_next": np.zeros(max(padded * L, 1), np.int32), "level_sizes": np.zeros(2 * (L + 1), np.int32),
+               "coarse_tables": np.zeros(max(6 * V, 1), np.int32)}
+        hess = inv = None
+        if matrices:
+            hess = np.zeros((max(Cn, 1), 136, 9))
+            inv = np.zeros((max(Cn, 1), 136, 9))
+        p = lambda a: None if a is None else a.ctypes.data_as(abi.c_f64p)  # noqa: E731
+        q = lambda a: a.ctypes.data_as(abi.c_i32p)  # noqa: E731
+        check(self.lib.mi_precond_mas_export(self.h, handle, None, q(out["part_to_real"]), q(out["real_to_part"]),
+                                             q(out["going_next"]), q(out["level_sizes"]), q(out["coarse_tables"]),
+                                             p(hess), p(inv)))
+        out["part_to_real"] = out["part_to_real"][:N]
+        out["level_sizes"] = out["level_sizes"].reshape(-1, 2)
+        out["coarse_tables"] = out["coarse_tables"].reshape(V, 6)
+        if matrices:
+            out["hess"], out["inv"] = hess[:Cn], inv[:Cn]
+        return out
+
     def assemble_preconditioners(self) -> None:
         check(self.lib.mi_precond_assemble(self.h, None))
 

@@ -170,7 +170,8 @@ def test_diag3_apply(ctx, c1_contact):
     zd = ctx.alloc(s.n, np.float64)
     ls.apply_preconditioner(ctx.to_device(r), zd)
     zref = O.diag3_apply(D, r)
-    assert rel_err(zd.numpy(), zref) < 1e-13
+    # 3x3 inverses of contact-stiffened blocks: rounding x block condition number
+    assert rel_err(zd.numpy(), zref) < 1e-10
 
 
 def test_diag12_and_identity_ranges(ctx):

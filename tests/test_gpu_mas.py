@@ -1,0 +1,107 @@
+"""GPU parity of the MAS preconditioner (hierarchy, cluster assembly, inverse,
+apply, MAS-PCG) against the oracle, through the C ABI."""
+import numpy as np
+import pytest
+
+from libuipc_amd import abi, synth
+from libuipc_amd.device import LinearSystem, rel_err
+from oracle import oracle as O
+from tests.test_oracle_mas import system
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_mas(ctx, s, precision=abi.MI_FP64, active_levels=0):
+    ls = LinearSystem(ctx, s.block_rows)
+    rows, cols, vals = ctx.to_device(s.rows), ctx.to_device(s.cols), ctx.to_device(s.vals)
+    ls.assemble(rows, cols, vals)
+    ls._keep = (rows, cols, vals)
+    h = ls.add_preconditioner(abi.MI_PRECOND_MAS, 0, s.n, part_ids=s.part_ids, tets=s.tets,
+                              mas_precision=precision, active_levels=active_levels)
+    ls.assemble_preconditioners()
+    return ls, h
+
+
+@pytest.mark.parametrize("grid,contacts,partition", [(6, 50, "geometric"), (7, 80, "random"), (6, 40, "unpart"),
+                                                     (16, 2000, "geometric")])
+def test_mas_tables_and_clusters(ctx, grid, contacts, partition):
+    s, (r, c, v) = system(grid, contacts, 11, partition)
+    ls, h = gpu_mas(ctx, s)
+    g = ls.mas_export(h)
+    m = O.Mas(s.block_rows, s.part_ids, s.tets)
+    m.assemble(r, c, v)
+    t = m.tables()
+    # integer hierarchy: bit-exact
+    assert (g["V"], g["N"], g["L"], g["total"], g["level1_begin"]) == (m.V, m.N, m.L, m.total, m.level1_begin)
+    for k in ("part_to_real", "real_to_part", "going_next", "coarse_tables"):
+        assert np.array_equal(g[k], t[k]), k
+    assert np.array_equal(g["level_sizes"], t["level_sizes"])
+    # cluster Hessians: same sums, different order of the coarse additions
+    H = m.hessians()
+    assert np.allclose(g["hess"], H, rtol=1e-11, atol=1e-12 * np.abs(H).max())
+    # FP64 inverses (same sweep, FMA rounding)
+    I = m.inverses()
+    assert np.abs(g["inv"] - I).max() <= 1e-8 * np.abs(I).max()
+
+
+@pytest.mark.parametrize("precision,tol", [(abi.MI_FP64, 1e-9), (abi.MI_FP32, 2e-4)])
+@pytest.mark.parametrize("partition", ["geometric", "random", "unpart"])
+def test_mas_apply(ctx, precision, tol, partition):
+    s, (r, c, v) = system(8, 300, 12, partition)
+    ls, h = gpu_mas(ctx, s, precision)
+    m = O.Mas(s.block_rows, s.part_ids, s.tets)
+    m.assemble(r, c, v)
+    rv = np.random.default_rng(3).standard_normal(s.n)
+    zd = ctx.alloc(s.n, np.float64)
+    ls.apply_preconditioner(ctx.to_device(rv), zd)
+    z = zd.numpy()
+    want = m.apply(rv)
+    assert rel_err(z, want) < tol
+    assert rv @ z > 0
+
+
+def test_mas_one_level_block_jacobi(ctx):
+    s, (r, c, v) = system(8, 300, 13, "random")
+    ls, h = gpu_mas(ctx, s, active_levels=1)
+    m = O.Mas(s.block_rows, s.part_ids, s.tets)
+    m.assemble(r, c, v)
+    rv = np.random.default_rng(4).standard_normal(s.n)
+    zd = ctx.alloc(s.n, np.float64)
+    ls.apply_preconditioner(ctx.to_device(rv), zd)
+    assert rel_err(zd.numpy(), m.apply(rv, active_levels=1)) < 1e-9
+
+
+@pytest.mark.parametrize("precision", [abi.MI_FP64, abi.MI_FP32])
+def test_mas_pcg(ctx, precision):
+    s, (r, c, v) = system(16, 20000, 14)
+    ls, h = gpu_mas(ctx, s, precision)
+    x = ctx.alloc(s.n, np.float64)
+    st = ls.solve(ctx.to_device(s.rhs), x, criterion=abi.MI_CRIT_REL_RESIDUAL, tol=1e-4, true_residual=1)
+    m = O.Mas(s.block_rows, s.part_ids, s.tets)
+    m.assemble(r, c, v)
+    xo, ro = O.pcg(s.block_rows, r, c, v, s.rhs, [dict(kind=3, blk_cnt=s.block_rows, mas=m,
+                                                        mas_fp32=precision == abi.MI_FP32)], criterion=1, tol=1e-4)
+    assert st["converged"] == 1 and ro["converged"] == 1
+    assert st["rel_res"] <= 1e-4 and st["rel_res_true"] < 2e-4
+    assert abs(st["iters_exact"] - ro["iters_exact"]) <= max(2, ro["iters_exact"] // 10)
+    # vs a tight FP64 solve
+    xt, rt = O.pcg(s.block_rows, r, c, v, s.rhs, [dict(kind=3, blk_cnt=s.block_rows, mas=m)], criterion=1, tol=1e-11)
+    assert rel_err(x.numpy(), xt) < 1e-2
+
+
+def test_mas_full_size_apply(ctx):
+    """64^3 grid + 200k contacts: apply vs the oracle, and r^T z > 0"""
+    s = synth.fem_system(64, 200_000, seed=3)
+    ls, h = gpu_mas(ctx, s)
+    r, c, v, _ = O.convert_sym(s.block_rows, s.rows, s.cols, s.vals)
+    m = O.Mas(s.block_rows, s.part_ids, s.tets)
+    m.assemble(r, c, v)
+    g = ls.mas_export(h, matrices=False)
+    t = m.tables()
+    assert g["L"] == 6 and np.array_equal(g["going_next"], t["going_next"])
+    rv = np.random.default_rng(5).standard_normal(s.n)
+    zd = ctx.alloc(s.n, np.float64)
+    ls.apply_preconditioner(ctx.to_device(rv), zd)
+    z = zd.numpy()
+    assert rel_err(z, m.apply(rv)) < 1e-8
+    assert rv @ z > 0
