@@ -1,4 +1,5 @@
-// Triplet -> sorted upper-triangular 3x3 BCOO + BSR row offsets on gfx950.
+// Triplet -> sorted upper-triangular 3x3 BCOO + BSR row offsets on gfx950,
+// plus the full (both triangles) BSR the SpMV streams.
 //
 // Replaces MatrixConverter<Float,3>::convert_sym
 // (algorithm/details/matrix_converter.inl:1549-1699) and the segmented reduce
@@ -13,7 +14,10 @@
 //     thread per output block, so the values are deterministic (the reference
 //     sums in atomic arrival order, fast_segmental_reduce.h:23-26);
 //   * values land in 9 SoA planes (see BsrMatrix);
-//   * one host read-back (the unique count, to size the outputs) instead of two.
+//   * one host read-back (the unique count, to size the outputs) instead of two;
+//   * the upper BCOO is then transposed (the same radix sort on col * B + row
+//     of the off-diagonal blocks) and merged row by row into a full BSR, so the
+//     SpMV is a pure row gather without atomics.
 #include "linsys.hpp"
 
 namespace mi
@@ -25,14 +29,48 @@ constexpr int kItems   = 16;
 constexpr int kTile    = kThreads * kItems;  // 4096 keys per tile
 constexpr int kWaves   = kThreads / kWave;
 
-__device__ __forceinline__ bool keep_triplet(int32_t r, int32_t c, int32_t B)
+// kept entries and their sort keys.  Mode 0: the triplet filter of
+// matrix_converter.inl:142 (plus a bound on col, which the reference leaves
+// to its producers), key row * B + col.  Mode 1: the off-diagonal blocks of
+// the upper BCOO, key col * B + row (the transpose).
+template <int MODE>
+__device__ __forceinline__ bool keep_entry(int32_t r, int32_t c, int32_t B)
 {
-    // the reference's flag (matrix_converter.inl:142) plus a bound on col,
-    // which the reference leaves to its producers
-    return r >= 0 && r <= c && c < B;
+    if constexpr(MODE == 0)
+        return r >= 0 && r <= c && c < B;
+    else
+        return r != c;
+}
+template <int MODE>
+__device__ __forceinline__ uint64_t entry_key(int32_t r, int32_t c, int32_t B)
+{
+    if constexpr(MODE == 0)
+        return (uint64_t)r * (uint64_t)B + (uint64_t)c;
+    else
+        return (uint64_t)c * (uint64_t)B + (uint64_t)r;
 }
 
-// ---- 1. count kept triplets per tile
+// stage one tile of (row, col) into LDS with coalesced loads
+__device__ __forceinline__ void stage_rc(const int32_t* rows,
+                                         const int32_t* cols,
+                                         int64_t        n,
+                                         int64_t        base,
+                                         int32_t*       sr,
+                                         int32_t*       sc)
+{
+#pragma unroll
+    for(int k = 0; k < kItems; ++k)
+    {
+        int     t = k * kThreads + threadIdx.x;
+        int64_t i = base + t;
+        sr[t]     = i < n ? rows[i] : -1;
+        sc[t]     = i < n ? cols[i] : -2;
+    }
+    __syncthreads();
+}
+
+// ---- 1. count kept entries per tile
+template <int MODE>
 __global__ __launch_bounds__(kThreads) void cv_count_kernel(int64_t        T,
                                                            const int32_t* rows,
                                                            const int32_t* cols,
@@ -46,7 +84,7 @@ __global__ __launch_bounds__(kThreads) void cv_count_kernel(int64_t        T,
     for(int k = 0; k < kItems; ++k)
     {
         int64_t i = base + (int64_t)k * kThreads + threadIdx.x;
-        if(i < T && keep_triplet(rows[i], cols[i], B))
+        if(i < T && keep_entry<MODE>(rows[i], cols[i], B))
             ++c;
     }
     uint32_t s = block_sum<kThreads>(c, lds);
@@ -54,9 +92,10 @@ __global__ __launch_bounds__(kThreads) void cv_count_kernel(int64_t        T,
         tile_cnt[blockIdx.x] = s;
 }
 
-// ---- 2. stable compaction of (key, source index); thread t owns the
-// contiguous items [t * kItems, (t + 1) * kItems) of the tile so that the
-// block prefix follows index order
+// ---- 2. stable compaction of (key, source index).  The tile is staged in
+// LDS with coalesced loads; thread t then owns the contiguous items
+// [t * kItems, (t + 1) * kItems) so that the block prefix follows index order.
+template <int MODE>
 __global__ __launch_bounds__(kThreads) void cv_compact_kernel(int64_t         T,
                                                              const int32_t*  rows,
                                                              const int32_t*  cols,
@@ -65,29 +104,26 @@ __global__ __launch_bounds__(kThreads) void cv_compact_kernel(int64_t         T,
                                                              uint64_t*       keys,
                                                              int32_t*        srcs)
 {
+    __shared__ int32_t  sr[kTile], sc[kTile];
     __shared__ uint32_t lds[kWaves + 1];
-    const int64_t       first = (int64_t)blockIdx.x * kTile + (int64_t)threadIdx.x * kItems;
-    uint32_t            flags = 0;
-    uint32_t            c     = 0;
+    const int64_t       base = (int64_t)blockIdx.x * kTile;
+    stage_rc(rows, cols, T, base, sr, sc);
+    const int first = threadIdx.x * kItems;
+    uint32_t  flags = 0, c = 0;
 #pragma unroll
     for(int k = 0; k < kItems; ++k)
-    {
-        int64_t i = first + k;
-        if(i < T && keep_triplet(rows[i], cols[i], B))
+        if(base + first + k < T && keep_entry<MODE>(sr[first + k], sc[first + k], B))
         {
             flags |= 1u << k;
             ++c;
         }
-    }
-    uint32_t total;
-    uint32_t o = block_exclusive_scan<kThreads>(c, lds, &total) + tile_off[blockIdx.x];
+    uint32_t o = block_exclusive_scan<kThreads>(c, lds, (uint32_t*)nullptr) + tile_off[blockIdx.x];
 #pragma unroll
     for(int k = 0; k < kItems; ++k)
         if(flags & (1u << k))
         {
-            int64_t i = first + k;
-            keys[o]   = (uint64_t)rows[i] * (uint64_t)B + (uint64_t)cols[i];
-            srcs[o]   = (int32_t)i;
+            keys[o] = entry_key<MODE>(sr[first + k], sc[first + k], B);
+            srcs[o] = (int32_t)(base + first + k);
             ++o;
         }
 }
@@ -96,7 +132,6 @@ __global__ __launch_bounds__(kThreads) void cv_compact_kernel(int64_t         T,
 __global__ __launch_bounds__(1024) void scan_small_kernel(uint32_t* data, int64_t n, int64_t* total)
 {
     __shared__ uint32_t lds[1024 / kWave + 1];
-    __shared__ uint32_t carry_s;
     uint32_t            carry = 0;
     for(int64_t base = 0; base < n; base += 1024)
     {
@@ -108,12 +143,8 @@ __global__ __launch_bounds__(1024) void scan_small_kernel(uint32_t* data, int64_
             data[i] = carry + e;
         carry += tot;
     }
-    if(threadIdx.x == 0)
-    {
-        carry_s = carry;
-        if(total)
-            *total = (int64_t)carry;
-    }
+    if(threadIdx.x == 0 && total)
+        *total = (int64_t)carry;
 }
 
 // ---- large exclusive scan: per-chunk sums, scan of sums, down-sweep
@@ -136,25 +167,38 @@ __global__ __launch_bounds__(kThreads) void scan_reduce_kernel(const uint32_t* d
 }
 __global__ __launch_bounds__(kThreads) void scan_down_kernel(uint32_t* data, int64_t n, const uint32_t* sums)
 {
+    __shared__ uint32_t sv[kScanChunk];
     __shared__ uint32_t lds[kWaves + 1];
-    const int64_t       first = (int64_t)blockIdx.x * kScanChunk + (int64_t)threadIdx.x * kItems;
-    uint32_t            v[kItems];
-    uint32_t            c = 0;
+    const int64_t       base = (int64_t)blockIdx.x * kScanChunk;
 #pragma unroll
     for(int k = 0; k < kItems; ++k)
     {
-        int64_t i = first + k;
-        v[k]      = i < n ? data[i] : 0u;
-        c += v[k];
+        int     t = k * kThreads + threadIdx.x;
+        int64_t i = base + t;
+        sv[t]     = i < n ? data[i] : 0u;
     }
+    __syncthreads();
+    const int first = threadIdx.x * kItems;
+    uint32_t  c     = 0;
+#pragma unroll
+    for(int k = 0; k < kItems; ++k)
+        c += sv[first + k];
     uint32_t o = block_exclusive_scan<kThreads>(c, lds, (uint32_t*)nullptr) + sums[blockIdx.x];
 #pragma unroll
     for(int k = 0; k < kItems; ++k)
     {
-        int64_t i = first + k;
+        uint32_t v      = sv[first + k];
+        sv[first + k] = o;
+        o += v;
+    }
+    __syncthreads();
+#pragma unroll
+    for(int k = 0; k < kItems; ++k)
+    {
+        int     t = k * kThreads + threadIdx.x;
+        int64_t i = base + t;
         if(i < n)
-            data[i] = o;
-        o += v[k];
+            data[i] = sv[t];
     }
 }
 
@@ -219,10 +263,10 @@ __global__ __launch_bounds__(kThreads) void rs_scatter_kernel(const int64_t*  d_
         wcnt[q][threadIdx.x] = 0;
     __syncthreads();
 
-    uint64_t key[kItems];
-    int32_t  val[kItems];
-    uint32_t rank[kItems];
-    uint32_t dig[kItems];
+    uint64_t       key[kItems];
+    int32_t        val[kItems];
+    uint32_t       rank[kItems];
+    uint32_t       dig[kItems];
     const uint64_t lt = lanemask_lt64();
 #pragma unroll
     for(int k = 0; k < kItems; ++k)
@@ -294,6 +338,21 @@ __global__ __launch_bounds__(kThreads) void rs_scatter_kernel(const int64_t*  d_
     }
 }
 
+// stage one tile of sorted keys into LDS (plus the key before the tile)
+__device__ __forceinline__ void stage_keys(const uint64_t* keys, int64_t m, int64_t base, uint64_t* sk)
+{
+#pragma unroll
+    for(int k = 0; k < kItems; ++k)
+    {
+        int     t = k * kThreads + threadIdx.x;
+        int64_t i = base + t;
+        sk[t + 1] = i < m ? keys[i] : ~0ull;
+    }
+    if(threadIdx.x == 0)
+        sk[0] = base > 0 ? keys[base - 1] : ~0ull;
+    __syncthreads();
+}
+
 // ---- run-length encode: heads per tile
 __global__ __launch_bounds__(kThreads) void rle_count_kernel(const int64_t* d_m, const uint64_t* keys, uint32_t* tile_cnt)
 {
@@ -321,15 +380,18 @@ __global__ __launch_bounds__(kThreads) void rle_write_kernel(const int64_t*  d_m
                                                             int32_t*        out_cols,
                                                             int32_t*        seg_start)
 {
+    __shared__ uint64_t sk[kTile + 1];
     __shared__ uint32_t lds[kWaves + 1];
-    const int64_t       m     = *d_m;
-    const int64_t       first = (int64_t)blockIdx.x * kTile + (int64_t)threadIdx.x * kItems;
-    uint32_t            flags = 0, c = 0;
+    const int64_t       m    = *d_m;
+    const int64_t       base = (int64_t)blockIdx.x * kTile;
+    stage_keys(keys, m, base, sk);
+    const int first = threadIdx.x * kItems;
+    uint32_t  flags = 0, c = 0;
 #pragma unroll
     for(int k = 0; k < kItems; ++k)
     {
-        int64_t i = first + k;
-        if(i < m && (i == 0 || keys[i] != keys[i - 1]))
+        int64_t i = base + first + k;
+        if(i < m && (i == 0 || sk[first + k + 1] != sk[first + k]))
         {
             flags |= 1u << k;
             ++c;
@@ -340,25 +402,24 @@ __global__ __launch_bounds__(kThreads) void rle_write_kernel(const int64_t*  d_m
     for(int k = 0; k < kItems; ++k)
         if(flags & (1u << k))
         {
-            int64_t  i   = first + k;
-            uint64_t key = keys[i];
+            uint64_t key = sk[first + k + 1];
             int32_t  r   = (int32_t)(key / (uint64_t)B);
             out_rows[o]  = r;
             out_cols[o]  = (int32_t)(key - (uint64_t)r * (uint64_t)B);
-            seg_start[o] = (int32_t)i;
+            seg_start[o] = (int32_t)(base + first + k);
             ++o;
         }
 }
 
 // ---- segmented reduce: one thread per unique block, duplicates summed in
-// ascending source order (bit-identical to a sequential left fold)
+// ascending source order (bit-identical to a sequential left fold).  The
+// gathers of a group of 4 are issued before their (ordered) additions.
 __global__ __launch_bounds__(kThreads) void seg_reduce_kernel(int64_t        U,
                                                              const int64_t* d_m,
                                                              const int32_t* seg_start,
                                                              const int32_t* srcs,
                                                              const double*  vals,
                                                              double*        out,
-                                                             float*         out32,
                                                              int64_t        stride)
 {
     const int64_t m = *d_m;
@@ -371,7 +432,25 @@ __global__ __launch_bounds__(kThreads) void seg_reduce_kernel(int64_t        U,
 #pragma unroll
         for(int q = 0; q < 9; ++q)
             acc[q] = v0[q];
-        for(int64_t k = b + 1; k < e; ++k)
+        int64_t k = b + 1;
+        for(; k + 4 <= e; k += 4)
+        {
+            double t[4][9];
+#pragma unroll
+            for(int g = 0; g < 4; ++g)
+            {
+                const double* v = vals + 9 * (int64_t)srcs[k + g];
+#pragma unroll
+                for(int q = 0; q < 9; ++q)
+                    t[g][q] = v[q];
+            }
+#pragma unroll
+            for(int g = 0; g < 4; ++g)
+#pragma unroll
+                for(int q = 0; q < 9; ++q)
+                    acc[q] = acc[q] + t[g][q];
+        }
+        for(; k < e; ++k)
         {
             const double* v = vals + 9 * (int64_t)srcs[k];
 #pragma unroll
@@ -381,24 +460,88 @@ __global__ __launch_bounds__(kThreads) void seg_reduce_kernel(int64_t        U,
 #pragma unroll
         for(int q = 0; q < 9; ++q)
             out[q * stride + u] = acc[q];
-        if(out32)
-        {
-#pragma unroll
-            for(int q = 0; q < 9; ++q)
-                out32[q * stride + u] = (float)acc[q];
-        }
     }
 }
 
-// ---- BSR row offsets: row_ptr[r] = first u with rows[u] >= r
-__global__ __launch_bounds__(kThreads) void row_ptr_kernel(int64_t U, const int32_t* rows, int32_t* row_ptr, int32_t B)
+// ---- BSR row offsets from sorted rows: row_ptr[r] = first u with rows[u] >= r
+__global__ __launch_bounds__(kThreads) void row_ptr_kernel(const int64_t* d_n, int64_t nmax, const int32_t* rows, int32_t* row_ptr, int32_t B)
 {
+    const int64_t U = d_n ? *d_n : nmax;
     for(int64_t u = (int64_t)blockIdx.x * kThreads + threadIdx.x; u <= U; u += (int64_t)gridDim.x * kThreads)
     {
         int32_t prev = (u == 0) ? -1 : rows[u - 1];
         int32_t cur  = (u == U) ? B : rows[u];
         for(int32_t r = prev + 1; r <= cur; ++r)
             row_ptr[r] = (int32_t)u;
+    }
+}
+
+// ---- full BSR: lower part (transposed upper blocks, sorted by (col, row)) then
+// upper part of each row
+__global__ void unpack_lower_kernel(const int64_t* d_n, const uint64_t* keys, int32_t B, int32_t* lrow)
+{
+    const int64_t n = *d_n;
+    for(int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
+        lrow[k] = (int32_t)(keys[k] / (uint64_t)B);
+}
+
+__global__ void full_ptr_kernel(const int32_t* lower_ptr, const int32_t* upper_ptr, int32_t* full_ptr, int32_t B)
+{
+    for(int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= B; i += (int64_t)gridDim.x * blockDim.x)
+        full_ptr[i] = lower_ptr[i] + upper_ptr[i];
+}
+
+template <typename VT>
+__global__ void full_lower_kernel(const int64_t*  d_n,
+                                  const uint64_t* keys,
+                                  const int32_t*  src,
+                                  const int32_t*  lower_ptr,
+                                  const int32_t*  full_ptr,
+                                  const double*   uvals,
+                                  int64_t         ustride,
+                                  int32_t         B,
+                                  int32_t*        fcols,
+                                  VT*             fvals,
+                                  int64_t         fstride)
+{
+    const int64_t n = *d_n;
+    for(int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
+    {
+        const uint64_t key = keys[k];
+        const int32_t  i   = (int32_t)(key / (uint64_t)B);        // lower row = upper col
+        const int32_t  j   = (int32_t)(key - (uint64_t)i * B);    // lower col = upper row
+        const int64_t  u   = src[k];
+        const int64_t  d   = (int64_t)full_ptr[i] + (k - lower_ptr[i]);
+        fcols[d]           = j;
+#pragma unroll
+        for(int r = 0; r < 3; ++r)
+#pragma unroll
+            for(int c = 0; c < 3; ++c)
+                fvals[(r + 3 * c) * fstride + d] = (VT)uvals[(c + 3 * r) * ustride + u];
+    }
+}
+
+template <typename VT>
+__global__ void full_upper_kernel(int64_t        U,
+                                  const int32_t* rows,
+                                  const int32_t* cols,
+                                  const int32_t* upper_ptr,
+                                  const int32_t* lower_ptr,
+                                  const int32_t* full_ptr,
+                                  const double*  uvals,
+                                  int64_t        ustride,
+                                  int32_t*       fcols,
+                                  VT*            fvals,
+                                  int64_t        fstride)
+{
+    for(int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < U; u += (int64_t)gridDim.x * blockDim.x)
+    {
+        const int32_t i = rows[u];
+        const int64_t d = (int64_t)full_ptr[i] + (lower_ptr[i + 1] - lower_ptr[i]) + (u - upper_ptr[i]);
+        fcols[d]        = cols[u];
+#pragma unroll
+        for(int q = 0; q < 9; ++q)
+            fvals[q * fstride + d] = (VT)uvals[q * ustride + u];
     }
 }
 
@@ -426,12 +569,62 @@ void scan_exclusive(uint32_t* data, int64_t n, DBuf<uint32_t>& chunk, int64_t* t
     scan_small_kernel<<<1, 1024, 0, s>>>(chunk.data(), nch, total);
     scan_down_kernel<<<nch, kThreads, 0, s>>>(data, n, chunk.data());
 }
+
+int grid_of(int64_t n)
+{
+    return std::max(1, std::min(ceil_div(n, kThreads), device_cu_count() * 16));
+}
 }  // namespace
 
 MatrixConverter::~MatrixConverter()
 {
     if(pinned_)
         (void)hipHostFree(pinned_);
+}
+
+// compaction + stable radix sort of (key, source) pairs; returns the buffer
+// index holding the result, *d_count = number of kept entries (device)
+template <int MODE>
+int MatrixConverter::compact_sort(int64_t        n,
+                                  const int32_t* rows,
+                                  const int32_t* cols,
+                                  int32_t        B,
+                                  int64_t*       d_count,
+                                  double         ratio,
+                                  hipStream_t    s)
+{
+    const int ntiles = ceil_div(n, kTile);
+    tile_cnt_.resize((size_t)ntiles + 1, ratio);
+    keys_[0].resize((size_t)n, ratio);
+    keys_[1].resize((size_t)n, ratio);
+    srcs_[0].resize((size_t)n, ratio);
+    srcs_[1].resize((size_t)n, ratio);
+    hist_.resize((size_t)256 * ntiles, ratio);
+    cv_count_kernel<MODE><<<ntiles, kThreads, 0, s>>>(n, rows, cols, B, tile_cnt_.data());
+    scan_exclusive(tile_cnt_.data(), ntiles, chunk_, d_count, s);
+    cv_compact_kernel<MODE><<<ntiles, kThreads, 0, s>>>(n, rows, cols, B, tile_cnt_.data(), keys_[0].data(),
+                                                        srcs_[0].data());
+    // stable LSD radix sort (matrix_converter.inl:1610-1616)
+    const int bits = key_bits_for((uint64_t)B * (uint64_t)B - 1ull);
+    int       cur  = 0;
+    for(int shift = 0; shift < bits; shift += 8)
+    {
+        int      w     = (bits - shift) < 8 ? (bits - shift) : 8;
+        uint32_t dmask = (1u << w) - 1u;
+        rs_hist_kernel<<<ntiles, kThreads, 0, s>>>(d_count, keys_[cur].data(), shift, dmask, hist_.data(), ntiles);
+        scan_exclusive(hist_.data(), (int64_t)256 * ntiles, chunk_, nullptr, s);
+        rs_scatter_kernel<<<ntiles, kThreads, 0, s>>>(d_count,
+                                                      keys_[cur].data(),
+                                                      srcs_[cur].data(),
+                                                      keys_[cur ^ 1].data(),
+                                                      srcs_[cur ^ 1].data(),
+                                                      shift,
+                                                      dmask,
+                                                      hist_.data(),
+                                                      ntiles);
+        cur ^= 1;
+    }
+    return cur;
 }
 
 void MatrixConverter::convert_sym(int32_t        B,
@@ -449,54 +642,26 @@ void MatrixConverter::convert_sym(int32_t        B,
     MI_REQUIRE(T == 0 || (rows && cols && vals), MI_ERR_ARG, "convert_sym: null triplet arrays");
     out.B = B;
     out.row_ptr.resize((size_t)B + 1, ratio);
+    out.full_ptr.resize((size_t)B + 1, ratio);
+    out.has32 = want32;
     if(!pinned_)
-        MI_HIP(hipHostMalloc((void**)&pinned_, 2 * sizeof(int64_t)));
-    counts_.resize(2);
-    MI_HIP(hipMemsetAsync(counts_.data(), 0, 2 * sizeof(int64_t), s));
+        MI_HIP(hipHostMalloc((void**)&pinned_, 4 * sizeof(int64_t)));
+    counts_.resize(4);
+    MI_HIP(hipMemsetAsync(counts_.data(), 0, 4 * sizeof(int64_t), s));
     if(T == 0 || B == 0)
     {
         out.U = 0;
-        out.stride = 0;
         MI_HIP(hipMemsetAsync(out.row_ptr.data(), 0, sizeof(int32_t) * ((size_t)B + 1), s));
+        MI_HIP(hipMemsetAsync(out.full_ptr.data(), 0, sizeof(int32_t) * ((size_t)B + 1), s));
         return;
     }
+    int64_t* d_m   = counts_.data();      // kept triplets
+    int64_t* d_u   = counts_.data() + 1;  // unique upper blocks
+    int64_t* d_off = counts_.data() + 2;  // off-diagonal upper blocks
+
+    // 1-4. flag, scan, compact, stable radix sort (matrix_converter.inl:1579-1616)
+    int cur          = compact_sort<0>(T, rows, cols, B, d_m, ratio, s);
     const int ntiles = ceil_div(T, kTile);
-    tile_cnt_.resize((size_t)ntiles + 1, ratio);
-    keys_[0].resize((size_t)T, ratio);
-    keys_[1].resize((size_t)T, ratio);
-    srcs_[0].resize((size_t)T, ratio);
-    srcs_[1].resize((size_t)T, ratio);
-    hist_.resize((size_t)256 * ntiles, ratio);
-
-    int64_t* d_m = counts_.data();
-    int64_t* d_u = counts_.data() + 1;
-
-    // 1-3. flag + scan + compact (matrix_converter.inl:1579-1595)
-    cv_count_kernel<<<ntiles, kThreads, 0, s>>>(T, rows, cols, B, tile_cnt_.data());
-    scan_exclusive(tile_cnt_.data(), ntiles, chunk_, d_m, s);
-    cv_compact_kernel<<<ntiles, kThreads, 0, s>>>(
-        T, rows, cols, B, tile_cnt_.data(), keys_[0].data(), srcs_[0].data());
-
-    // 4. stable LSD radix sort (matrix_converter.inl:1610-1616)
-    const int bits = key_bits_for((uint64_t)B * (uint64_t)B - 1ull);
-    int       cur  = 0;
-    for(int shift = 0; shift < bits; shift += 8)
-    {
-        int      w     = (bits - shift) < 8 ? (bits - shift) : 8;
-        uint32_t dmask = (1u << w) - 1u;
-        rs_hist_kernel<<<ntiles, kThreads, 0, s>>>(d_m, keys_[cur].data(), shift, dmask, hist_.data(), ntiles);
-        scan_exclusive(hist_.data(), (int64_t)256 * ntiles, chunk_, nullptr, s);
-        rs_scatter_kernel<<<ntiles, kThreads, 0, s>>>(d_m,
-                                                      keys_[cur].data(),
-                                                      srcs_[cur].data(),
-                                                      keys_[cur ^ 1].data(),
-                                                      srcs_[cur ^ 1].data(),
-                                                      shift,
-                                                      dmask,
-                                                      hist_.data(),
-                                                      ntiles);
-        cur ^= 1;
-    }
 
     // 5-6. run-length encode (1626-1650); the one host read-back sizes the outputs
     rle_count_kernel<<<ntiles, kThreads, 0, s>>>(d_m, keys_[cur].data(), tile_cnt_.data());
@@ -507,33 +672,52 @@ void MatrixConverter::convert_sym(int32_t        B,
     out.U           = U;
     out.rows.resize((size_t)(U > 0 ? U : 1), ratio);
     out.cols.resize((size_t)(U > 0 ? U : 1), ratio);
-    if((int64_t)out.rows.cap > out.stride || out.stride < U)
-    {
-        out.stride = (int64_t)out.rows.cap;
-    }
+    out.stride = (int64_t)out.rows.cap;
     out.vals.resize((size_t)9 * out.stride);
-    if(want32)
-    {
-        out.vals32.resize((size_t)9 * out.stride);
-        out.has32 = true;
-    }
     seg_start_.resize((size_t)(U > 0 ? U : 1), ratio);
     rle_write_kernel<<<ntiles, kThreads, 0, s>>>(
         d_m, keys_[cur].data(), tile_cnt_.data(), B, out.rows.data(), out.cols.data(), seg_start_.data());
 
     // 7-8. segmented reduce over the sorted permutation (1652-1698)
-    const int grid = std::max(1, std::min(ceil_div(U, kThreads), device_cu_count() * 16));
     if(U > 0)
-        seg_reduce_kernel<<<grid, kThreads, 0, s>>>(U,
-                                                    d_m,
-                                                    seg_start_.data(),
-                                                    srcs_[cur].data(),
-                                                    vals,
-                                                    out.vals.data(),
-                                                    want32 ? out.vals32.data() : nullptr,
-                                                    out.stride);
-    row_ptr_kernel<<<std::max(1, std::min(ceil_div(U + 1, kThreads), device_cu_count() * 16)), kThreads, 0, s>>>(
-        U, out.rows.data(), out.row_ptr.data(), B);
+        seg_reduce_kernel<<<grid_of(U), kThreads, 0, s>>>(
+            U, d_m, seg_start_.data(), srcs_[cur].data(), vals, out.vals.data(), out.stride);
+    row_ptr_kernel<<<grid_of(U + 1), kThreads, 0, s>>>(nullptr, U, out.rows.data(), out.row_ptr.data(), B);
+
+    // full BSR: transpose of the off-diagonal upper blocks + the upper blocks
+    int tcur = compact_sort<1>(U, out.rows.data(), out.cols.data(), B, d_off, ratio, s);
+    lower_rows_.resize((size_t)(U > 0 ? U : 1), ratio);
+    lower_ptr_.resize((size_t)B + 1, ratio);
+    unpack_lower_kernel<<<grid_of(U), kThreads, 0, s>>>(d_off, keys_[tcur].data(), B, lower_rows_.data());
+    row_ptr_kernel<<<grid_of(U + 1), kThreads, 0, s>>>(d_off, U, lower_rows_.data(), lower_ptr_.data(), B);
+    full_ptr_kernel<<<grid_of(B + 1), kThreads, 0, s>>>(lower_ptr_.data(), out.row_ptr.data(), out.full_ptr.data(), B);
+    const int64_t fcap = 2 * U;
+    out.fcols.resize((size_t)std::max<int64_t>(1, fcap), ratio);
+    out.fstride = (int64_t)out.fcols.cap;
+    if(want32)
+    {
+        out.fvals32.resize((size_t)9 * out.fstride);
+        full_lower_kernel<float><<<grid_of(U), kThreads, 0, s>>>(d_off, keys_[tcur].data(), srcs_[tcur].data(),
+                                                                 lower_ptr_.data(), out.full_ptr.data(),
+                                                                 out.vals.data(), out.stride, B, out.fcols.data(),
+                                                                 out.fvals32.data(), out.fstride);
+        full_upper_kernel<float><<<grid_of(U), kThreads, 0, s>>>(U, out.rows.data(), out.cols.data(),
+                                                                 out.row_ptr.data(), lower_ptr_.data(),
+                                                                 out.full_ptr.data(), out.vals.data(), out.stride,
+                                                                 out.fcols.data(), out.fvals32.data(), out.fstride);
+    }
+    else
+    {
+        out.fvals.resize((size_t)9 * out.fstride);
+        full_lower_kernel<double><<<grid_of(U), kThreads, 0, s>>>(d_off, keys_[tcur].data(), srcs_[tcur].data(),
+                                                                  lower_ptr_.data(), out.full_ptr.data(),
+                                                                  out.vals.data(), out.stride, B, out.fcols.data(),
+                                                                  out.fvals.data(), out.fstride);
+        full_upper_kernel<double><<<grid_of(U), kThreads, 0, s>>>(U, out.rows.data(), out.cols.data(),
+                                                                  out.row_ptr.data(), lower_ptr_.data(),
+                                                                  out.full_ptr.data(), out.vals.data(), out.stride,
+                                                                  out.fcols.data(), out.fvals.data(), out.fstride);
+    }
     MI_HIP(hipGetLastError());
 }
 }  // namespace mi

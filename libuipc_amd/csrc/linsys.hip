@@ -60,7 +60,6 @@ struct mi_sys
 
     // PCG state (LinearFusedPCG's r, z, p, Ap and device scalars)
     DBuf<double> r, z, p, Ap, tmp;
-    DBuf<double> d_misc;  // [0..3] scratch scalars
     PcgScalars*  d_sc = nullptr;
     PcgScalars*  h_sc = nullptr;  // pinned
 
@@ -73,14 +72,14 @@ struct mi_sys
         const void* b = nullptr;
         int64_t     n = -1, U = -1;
         int         ci = 0, fused = -1, bypass = -1, use32 = -1;
-        const void* bufs[6] = {};
+        const void* bufs[10] = {};
         hipStream_t stream  = nullptr;
         bool        operator==(const GraphKey& o) const
         {
             if(x != o.x || b != o.b || n != o.n || U != o.U || ci != o.ci || fused != o.fused || bypass != o.bypass
                || use32 != o.use32 || stream != o.stream)
                 return false;
-            for(int k = 0; k < 6; ++k)
+            for(int k = 0; k < 10; ++k)
                 if(bufs[k] != o.bufs[k])
                     return false;
             return true;
@@ -392,7 +391,7 @@ int mi_spmv_dot(mi_sys* sys, const double* x, double* y, double* d_dot, void* st
     MI_REQUIRE(sys && x && y, MI_ERR_ARG, "mi_spmv_dot: null argument");
     MI_REQUIRE(sys->assembled, MI_ERR_STATE, "mi_spmv_dot: not assembled");
     hipStream_t s = pick(sys->ctx, stream);
-    spmv_sym_dot(sys->A, sys->desc.value_precision == MI_FP32, x, y, d_dot, nullptr, nullptr, nullptr, sys->spmv_scr, s);
+    spmv_accumulate_dot(sys->A, sys->desc.value_precision == MI_FP32, 1.0, x, y, d_dot, sys->spmv_scr, s);
     MI_API_END
 }
 
@@ -401,7 +400,7 @@ int mi_spmv(mi_sys* sys, double a, const double* x, double b, double* y, void* s
     MI_API_BEGIN
     MI_REQUIRE(sys && x && y, MI_ERR_ARG, "mi_spmv: null argument");
     MI_REQUIRE(sys->assembled, MI_ERR_STATE, "mi_spmv: not assembled");
-    spmv_axpby(sys->A, a, x, b, y, sys->spmv_scr, pick(sys->ctx, stream));
+    spmv_axpby(sys->A, sys->desc.value_precision == MI_FP32, a, x, b, y, sys->spmv_scr, pick(sys->ctx, stream));
     MI_API_END
 }
 
@@ -487,30 +486,41 @@ int mi_pcg_default_opts(mi_pcg_opts* o)
 
 namespace
 {
-// one PCG iteration (LinearFusedPCG::run_iteration, linear_fused_pcg.cu:1374-1594)
+// one PCG iteration (LinearFusedPCG::run_iteration, linear_fused_pcg.cu:1374-1594):
+// SpMV + pAp partials -> x, r update (alpha) [+ block Jacobi + r.z partials]
+// -> preconditioner -> r.z partials -> convergence, beta, p update
 void run_iteration(mi_sys* sys, double* x, bool bypass, const LocalPreconditioner* fused_diag, hipStream_t s)
 {
     const int64_t n     = sys->n();
     PcgScalars*   sc    = sys->d_sc;
     const bool    use32 = sys->desc.value_precision == MI_FP32;
-    spmv_sym_dot(sys->A, use32, sys->p.data(), sys->Ap.data(), &sc->pAp, &sc->rz, &sc->alpha, &sc->converged,
-                 sys->spmv_scr, s);
+    double*       partA = sys->spmv_scr.partials.data();
+    double*       partB = sys->dot_scr.partials.data();
+    const int     gA    = spmv_full(sys->A, use32, sys->p.data(), sys->Ap.data(), partA, &sc->converged,
+                                    sys->spmv_scr, s);
+    int gB;
     if(fused_diag)
     {
-        pcg_update_xr_diag3(sc, x, sys->p.data(), sys->r.data(), sys->Ap.data(), sys->z.data(),
-                            fused_diag->diag3_inverse(), fused_diag->dof_offset / 3, fused_diag->dof_count / 3, n,
-                            sys->dot_scr, s);
+        gB = pcg_update_xr_diag3(sc, partA, gA, x, sys->p.data(), sys->r.data(), sys->Ap.data(), sys->z.data(),
+                                 fused_diag->diag3_inverse(), fused_diag->dof_offset / 3,
+                                 fused_diag->dof_count / 3, n, partB, sys->dot_scr, s);
     }
     else
     {
-        pcg_update_xr(sc, x, sys->p.data(), sys->r.data(), sys->Ap.data(), n, s);
+        pcg_update_xr(sc, partA, gA, x, sys->p.data(), sys->r.data(), sys->Ap.data(), n, s);
         if(bypass)
             pcg_copy_if_active(sc, sys->z.data(), sys->r.data(), n, s);
         else
             sys->apply_preconditioner(sys->r.data(), sys->z.data(), &sc->converged, s);
-        pcg_dot_scalar(sc, sys->r.data(), sys->z.data(), n, sys->dot_scr, s);
+        gB = pcg_dot_partials(sys->r.data(), sys->z.data(), n, partB, &sc->converged, sys->dot_scr, s);
     }
-    pcg_update_p(sc, sys->p.data(), sys->z.data(), n, s);
+    pcg_update_p(sc, partB, gB, sys->p.data(), sys->z.data(), n, s);
+}
+
+void read_scalars(mi_sys* sys, hipStream_t s)
+{
+    MI_HIP(hipMemcpyAsync(sys->h_sc, sys->d_sc, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
+    MI_HIP(hipStreamSynchronize(s));
 }
 }  // namespace
 
@@ -541,12 +551,13 @@ extern "C" int mi_pcg_solve(mi_sys* sys, const double* b, double* x, const mi_pc
     sys->z.resize((size_t)n, ratio);
     sys->p.resize((size_t)n, ratio);
     sys->Ap.resize((size_t)n, ratio);
-    sys->d_misc.resize(4);
     if(!sys->d_sc)
     {
         MI_HIP(hipMalloc((void**)&sys->d_sc, sizeof(PcgScalars)));
         MI_HIP(hipHostMalloc((void**)&sys->h_sc, sizeof(PcgScalars)));
     }
+    spmv_prepare(sys->spmv_scr);  // no allocation may happen inside a captured block
+    PcgScalars& h = *sys->h_sc;
 
     // r = b, x = 0, z = P r, p = z, rz0 = r.z (linear_fused_pcg.cu:1811-1824)
     pcg_init_vectors(x, sys->r.data(), b, n, s);
@@ -555,25 +566,27 @@ extern "C" int mi_pcg_solve(mi_sys* sys, const double* b, double* x, const mi_pc
         copy_f64(sys->z.data(), sys->r.data(), n, s);
     else
         sys->apply_preconditioner(sys->r.data(), sys->z.data(), nullptr, s);
-    pcg_dot_rz_rr(sys->r.data(), sys->z.data(), n, sys->d_misc.data(), sys->dot_scr, s);
-    double h2[2];
-    MI_HIP(hipMemcpyAsync(h2, sys->d_misc.data(), 2 * sizeof(double), hipMemcpyDeviceToHost, s));
-    MI_HIP(hipStreamSynchronize(s));
-    double rz0 = h2[0], bb = h2[1];
+    int g0 = pcg_dot_partials(sys->r.data(), sys->z.data(), n, sys->dot_scr.partials.data(), nullptr, sys->dot_scr, s);
+    pcg_init_scalars(sys->d_sc, sys->dot_scr.partials.data(), g0, o.criterion, o.tol, s);
+    read_scalars(sys, s);
 
-    // PSD guard (linear_fused_pcg.cu:1843-1875)
-    if(!bypass && o.psd_fallback && (rz0 < 0.0 || !std::isfinite(rz0)))
+    // PSD guard (linear_fused_pcg.cu:1843-1875): rz0 < 0, or non-finite z with a
+    // finite r, means the preconditioner is not usable -> plain CG for this solve
+    if(!bypass && o.psd_fallback && (h.rz < 0.0 || !std::isfinite(h.rz)))
     {
-        bool fb = rz0 < 0.0 || std::isfinite(bb);
-        if(fb)
+        if(h.rz < 0.0 || std::isfinite(h.bb))
         {
             bypass = true;
             copy_f64(sys->z.data(), sys->r.data(), n, s);
-            rz0 = bb;
+            g0 = pcg_dot_partials(sys->r.data(), sys->z.data(), n, sys->dot_scr.partials.data(), nullptr,
+                                  sys->dot_scr, s);
+            pcg_init_scalars(sys->d_sc, sys->dot_scr.partials.data(), g0, o.criterion, o.tol, s);
+            read_scalars(sys, s);
         }
     }
-    st->bypassed = bypass ? 1 : 0;
-    st->rz0      = rz0;
+    const double rz0 = h.rz, bb = h.bb;
+    st->bypassed     = bypass ? 1 : 0;
+    st->rz0          = rz0;
     MI_REQUIRE(std::isfinite(rz0) && std::isfinite(bb), MI_ERR_NONFINITE, "mi_pcg_solve: non-finite r^T z at start");
     copy_f64(sys->p.data(), sys->z.data(), n, s);
     if(std::fabs(rz0) == 0.0)
@@ -583,16 +596,6 @@ extern "C" int mi_pcg_solve(mi_sys* sys, const double* b, double* x, const mi_pc
         MI_HIP(hipStreamSynchronize(s));
         return MI_OK;
     }
-
-    PcgScalars& h = *sys->h_sc;
-    h             = PcgScalars{};
-    h.rz          = rz0;
-    h.rz_tol      = o.tol * std::fabs(rz0);
-    h.rr_tol      = o.tol * o.tol * bb;
-    h.criterion   = o.criterion;
-    MI_HIP(hipMemcpyAsync(sys->d_sc, sys->h_sc, sizeof(PcgScalars), hipMemcpyHostToDevice, s));
-    fill_f64(sys->Ap.data(), 0.0, n, s);
-    spmv_prepare(sys->spmv_scr);  // no allocation may happen inside a captured block
 
     const LocalPreconditioner* fused = nullptr;
     if(!bypass && sys->pcs.size() == 1 && sys->pcs[0]->diag3_inverse())
@@ -605,25 +608,28 @@ extern "C" int mi_pcg_solve(mi_sys* sys, const double* b, double* x, const mi_pc
     bool          converged = false;
 
     mi_sys::GraphKey key;
-    key.x = x;
-    key.b = b;
-    key.n = n;
-    key.U = sys->A.U;
-    key.ci = ci;
-    key.fused = fused ? 1 : 0;
-    key.bypass = bypass ? 1 : 0;
-    key.use32  = sys->desc.value_precision;
+    key.x       = x;
+    key.b       = b;
+    key.n       = n;
+    key.U       = sys->A.U;
+    key.ci      = ci;
+    key.fused   = fused ? 1 : 0;
+    key.bypass  = bypass ? 1 : 0;
+    key.use32   = sys->desc.value_precision;
     key.bufs[0] = sys->r.data();
     key.bufs[1] = sys->z.data();
     key.bufs[2] = sys->p.data();
     key.bufs[3] = sys->Ap.data();
-    key.bufs[4] = sys->A.vals.data();
-    key.bufs[5] = sys->pcs.empty() ? nullptr : (const void*)sys->pcs[0].get();
+    key.bufs[4] = sys->A.full_ptr.data();
+    key.bufs[5] = sys->A.fcols.data();
+    key.bufs[6] = sys->A.has32 ? (const void*)sys->A.fvals32.data() : (const void*)sys->A.fvals.data();
+    key.bufs[7] = sys->pcs.empty() ? nullptr : (const void*)sys->pcs[0].get();
+    key.bufs[8] = fused ? (const void*)fused->diag3_inverse() : nullptr;
     key.stream  = s;
 
     while(done < total)
     {
-        int64_t blk = std::min<int64_t>(ci, total - done);
+        int64_t blk       = std::min<int64_t>(ci, total - done);
         bool    use_graph = o.use_graph && !sys->graph_failed && blk == ci && s != nullptr;
         if(use_graph && !(sys->graph_exec && sys->graph_key == key))
         {
@@ -659,8 +665,7 @@ extern "C" int mi_pcg_solve(mi_sys* sys, const double* b, double* x, const mi_pc
                 run_iteration(sys, x, bypass, fused, s);
         MI_HIP(hipGetLastError());
         done += blk;
-        MI_HIP(hipMemcpyAsync(sys->h_sc, sys->d_sc, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
-        MI_HIP(hipStreamSynchronize(s));
+        read_scalars(sys, s);
         MI_REQUIRE(std::isfinite(h.rz_new), MI_ERR_NONFINITE,
                    "mi_pcg_solve: non-finite r^T z at iteration " + std::to_string(h.iter));
         if(h.converged)
@@ -679,7 +684,7 @@ extern "C" int mi_pcg_solve(mi_sys* sys, const double* b, double* x, const mi_pc
     {
         sys->tmp.resize((size_t)n, ratio);
         copy_f64(sys->tmp.data(), b, n, s);
-        spmv_axpby(sys->A, -1.0, x, 1.0, sys->tmp.data(), sys->spmv_scr, s);
+        spmv_axpby(sys->A, sys->desc.value_precision == MI_FP32, -1.0, x, 1.0, sys->tmp.data(), sys->spmv_scr, s);
         double rn = norm_f64_host(sys->tmp.data(), n, s);
         double bn = std::sqrt(bb);
         st->rel_res_true = bn > 0 ? rn / bn : 0.0;

@@ -10,20 +10,29 @@
 namespace mi
 {
 // ----------------------------------------------------------------- matrix
-// The assembled system: sorted, de-duplicated upper-triangular 3x3 BCOO
-// (the reference's bcoo_A, linear_system/global_linear_system.h:313) plus the
-// BSR row offsets.  HBM layout: rows/cols int32[U]; values as 9 planes
-// (plane q holds entry q of every block, column-major entry order), so that a
-// wave's 64 lanes read 512 contiguous bytes per plane.
+// The assembled system.
+//   upper  sorted, de-duplicated upper-triangular 3x3 BCOO (the reference's
+//          bcoo_A, linear_system/global_linear_system.h:313) with its BSR row
+//          offsets; FP64 values.  Exported for parity and read by the
+//          preconditioner assemblies.
+//   full   both triangles as BSR (full_ptr, fcols, values), FP64 or FP32:
+//          the stream the SpMV reads.
+// HBM layout: indices int32; values as 9 planes (plane q holds entry q of
+// every block, column-major entry order), so that the lanes of a wave read
+// contiguous bytes per plane.
 struct BsrMatrix
 {
-    int32_t          B      = 0;  // block rows
-    int64_t          U      = 0;  // unique upper blocks (host copy, valid after assemble)
-    int64_t          stride = 0;  // plane stride (>= U)
-    DBuf<int32_t>    rows, cols, row_ptr;
-    DBuf<double>     vals;    // 9 * stride
-    DBuf<float>      vals32;  // 9 * stride, FP32 copy for the precision arm
-    bool             has32 = false;
+    int32_t       B      = 0;  // block rows
+    int64_t       U      = 0;  // unique upper blocks (host copy, valid after assemble)
+    int64_t       stride = 0;  // upper plane stride (>= U)
+    DBuf<int32_t> rows, cols, row_ptr;
+    DBuf<double>  vals;  // 9 * stride
+    // full BSR
+    int64_t       fstride = 0;  // full plane stride (>= 2U - diagonal blocks)
+    DBuf<int32_t> full_ptr, fcols;
+    DBuf<double>  fvals;    // 9 * fstride (FP64 systems)
+    DBuf<float>   fvals32;  // 9 * fstride (FP32 systems)
+    bool          has32 = false;
 };
 
 // ----------------------------------------------------------------- converter
@@ -40,48 +49,49 @@ class MatrixConverter
                      bool           want32,
                      double         reserve_ratio,
                      hipStream_t    s);
+    ~MatrixConverter();
 
   private:
+    template <int MODE>
+    int compact_sort(int64_t        n,
+                     const int32_t* rows,
+                     const int32_t* cols,
+                     int32_t        B,
+                     int64_t*       d_count,
+                     double         ratio,
+                     hipStream_t    s);
+
     DBuf<uint64_t> keys_[2];
     DBuf<int32_t>  srcs_[2];
     DBuf<uint32_t> hist_, chunk_;
     DBuf<uint32_t> tile_cnt_;
-    DBuf<int32_t>  seg_start_;
-    DBuf<int64_t>  counts_;  // [0] = m (upper triplets), [1] = U
+    DBuf<int32_t>  seg_start_, lower_rows_, lower_ptr_;
+    DBuf<int64_t>  counts_;  // [0] kept triplets, [1] U, [2] off-diagonal upper blocks
     int64_t*       pinned_ = nullptr;
-
-  public:
-    ~MatrixConverter();
 };
 
 // ----------------------------------------------------------------- SpMV
 // Spmv::rbk_sym_spmv_dot (linear_system/spmv.cu:1023-1159)
 struct SpmvScratch
 {
-    DBuf<double>   partials;
-    DBuf<uint32_t> ticket;
-    int            grid = 0;
+    DBuf<double> partials;
+    int          grid = 0;
 };
-void spmv_sym_dot(const BsrMatrix& A,
-                  bool             use32,
-                  const double*    x,
-                  double*          y,  // accumulated into: must be zero on entry
-                  double*          d_dot,
-                  double*          d_alpha_num,  // when non-null: *d_alpha_out = *d_alpha_num / dot
-                  double*          d_alpha_out,
-                  const int32_t*   d_converged,
-                  SpmvScratch&     scr,
-                  hipStream_t      s);
-
 // allocates the SpMV's reduction scratch (must run outside graph capture)
 void spmv_prepare(SpmvScratch& scr);
-
+// y = A x (overwritten); partials[g] = per-workgroup x^T A x; returns the grid g
+int spmv_full(const BsrMatrix& A, bool use32, const double* x, double* y, double* partials, const int32_t* d_conv,
+              SpmvScratch& scr, hipStream_t s);
+// y += a A x; *d_dot = x^T A x when d_dot != null
+void spmv_accumulate_dot(const BsrMatrix& A, bool use32, double a, const double* x, double* y, double* d_dot,
+                         SpmvScratch& scr, hipStream_t s);
 // y = a A x + b y (Spmv::rbk_sym_spmv)
-void spmv_axpby(const BsrMatrix& A, double a, const double* x, double b, double* y, SpmvScratch& scr, hipStream_t s);
+void spmv_axpby(const BsrMatrix& A, bool use32, double a, const double* x, double b, double* y, SpmvScratch& scr,
+                hipStream_t s);
 
 // ----------------------------------------------------------------- vectors
-void fill_f64(double* p, double v, int64_t n, hipStream_t s);
-void copy_f64(double* dst, const double* src, int64_t n, hipStream_t s);
+void   fill_f64(double* p, double v, int64_t n, hipStream_t s);
+void   copy_f64(double* dst, const double* src, int64_t n, hipStream_t s);
 double dot_f64_host(const double* x, const double* y, int64_t n, hipStream_t s);
 double norm_f64_host(const double* x, int64_t n, hipStream_t s);
 float  dot_f32_host(const float* x, const float* y, int64_t n, hipStream_t s);
@@ -113,40 +123,43 @@ std::unique_ptr<LocalPreconditioner> make_diag12(int64_t dof_offset, int64_t dof
 std::unique_ptr<LocalPreconditioner> make_mas(const mi_precond_desc& desc);
 
 // ----------------------------------------------------------------- PCG
+// Device scalars of one solve (LinearFusedPCG's d_rz, d_pAp, d_rz_new, d_beta,
+// d_converged, d_rz_tol).  Reductions reach them without fences: a kernel
+// writes per-workgroup partials, and every workgroup of the next kernel sums
+// the partials in the same order; workgroup 0 records the result here.
 struct PcgScalars
 {
-    double   rz;      // r^T z of the previous iteration
-    double   rz_new;  // r^T z of this iteration
-    double   pAp;
-    double   alpha;
-    double   beta;
-    double   rr;      // r^T r of this iteration
-    double   rz_tol;  // tol * |rz0|          (criterion 0)
-    double   rr_tol;  // tol^2 * ||b||^2      (criterion 1)
-    int32_t  converged;
-    int32_t  criterion;
-    int32_t  iter;
-    int32_t  conv_iter;
-    uint32_t ticket[4];
+    double  rz;          // r^T z belonging to the current p (beta's denominator)
+    double  rz_pending;  // r^T z of the last iteration (alpha's numerator)
+    double  rz_new;      // same value, as reported
+    double  pAp;
+    double  alpha;
+    double  beta;
+    double  rr;      // r^T r of the last iteration
+    double  bb;      // b^T b
+    double  rz_tol;  // tol * |rz0|          (criterion 0)
+    double  rr_tol;  // tol^2 * ||b||^2      (criterion 1)
+    int32_t converged;
+    int32_t criterion;
+    int32_t iter;
+    int32_t conv_iter;
 };
 
 void pcg_init_vectors(double* x, double* r, const double* b, int64_t n, hipStream_t s);
-// p = z and PcgScalars init given rz0 (device reduction of r.z, r.r, b.b)
-void pcg_dot_rz_rr(const double* r, const double* z, int64_t n, double* out3, SpmvScratch& scr, hipStream_t s);
-void pcg_update_xr(PcgScalars* sc, double* x, const double* p, double* r, double* Ap, int64_t n, hipStream_t s);
-void pcg_update_xr_diag3(PcgScalars*   sc,
-                         double*       x,
-                         const double* p,
-                         double*       r,
-                         double*       Ap,
-                         double*       z,
-                         const double* diag_inv,
-                         int64_t       blk_off,
-                         int64_t       blk_cnt,
-                         int64_t       n,
-                         SpmvScratch&  scr,
-                         hipStream_t   s);
-void pcg_dot_scalar(PcgScalars* sc, const double* r, const double* z, int64_t n, SpmvScratch& scr, hipStream_t s);
-void pcg_update_p(PcgScalars* sc, double* p, const double* z, int64_t n, hipStream_t s);
+// partials of (r.z, r.r) -> part[2 g]; returns g
+int pcg_dot_partials(const double* r, const double* z, int64_t n, double* part, const int32_t* conv,
+                     SpmvScratch& scr, hipStream_t s);
+// PcgScalars from the initial partials (rz0 = r.z, bb = r.r with r = b)
+void pcg_init_scalars(PcgScalars* sc, const double* part, int g, int criterion, double tol, hipStream_t s);
+// x += alpha p, r -= alpha Ap with alpha = rz / sum(pAp partials)
+void pcg_update_xr(PcgScalars* sc, const double* pap_part, int gp, double* x, const double* p, double* r,
+                   const double* Ap, int64_t n, hipStream_t s);
+// the same, fused with z = D^-1 r on [blk_off, blk_off + blk_cnt) (z = r elsewhere)
+// and the (r.z, r.r) partials -> part; returns the grid
+int pcg_update_xr_diag3(PcgScalars* sc, const double* pap_part, int gp, double* x, const double* p, double* r,
+                        const double* Ap, double* z, const double* diag_inv, int64_t blk_off, int64_t blk_cnt,
+                        int64_t n, double* part, SpmvScratch& scr, hipStream_t s);
+// convergence / beta from the (r.z, r.r) partials, then p = z + beta p
+void pcg_update_p(PcgScalars* sc, const double* part, int gd, double* p, const double* z, int64_t n, hipStream_t s);
 void pcg_copy_if_active(PcgScalars* sc, double* dst, const double* src, int64_t n, hipStream_t s);
 }  // namespace mi

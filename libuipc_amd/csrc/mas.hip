@@ -39,6 +39,7 @@
 #include <algorithm>
 #include <cstring>
 #include <numeric>
+#include <type_traits>
 
 #include "linsys.hpp"
 
@@ -596,51 +597,58 @@ __global__ __launch_bounds__(256) void mas_push_kernel(int64_t        c_first,
     }
 }
 
-// one wave per cluster: symmetric Gauss-Jordan sweep in FP64 (lane = column),
-// -A^{-1} in registers, upper blocks stored (diagonal block mirrored)
+// one wave per cluster: symmetric Gauss-Jordan sweep in FP64 (lane = column;
+// the sweep of mas_preconditioner_engine.cu:1077-1175, in double).  The column
+// is kept rotated so that the current pivot row is always register 0: after
+// each pivot the registers shift down by one and the pivot's new value goes to
+// the end, so every register index is static without unrolling the 48 pivots
+// (a fully unrolled sweep spills).  After 48 pivots the rotation is the
+// identity and the registers hold -A^{-1}.
 template <typename TO>
-__global__ __launch_bounds__(256) void mas_invert_kernel(const double* __restrict__ hess, TO* inv, int64_t C)
+__global__ __launch_bounds__(64) void mas_invert_kernel(const double* __restrict__ hess, TO* inv, int64_t C)
 {
     constexpr int D = 3 * kBank;
-    __shared__ double prow[4][2][D];
-    const int         w    = threadIdx.x / kWave;
-    const int         lane = threadIdx.x % kWave;
-    const int64_t     k    = (int64_t)blockIdx.x * 4 + w;
-    const bool        live = k < C && lane < D;
-    const int         col  = lane < D ? lane : 0;
+    __shared__ double prow[2][2 * D];  // pivot row, stored twice: entry (p + k) needs no modulo
+    const int         lane = threadIdx.x;
+    const int64_t     k    = blockIdx.x;
+    const bool        live = lane < D;
+    const int         col  = live ? lane : 0;
     const int         nc = col / 3, a = col % 3;
+    const double*     M  = hess + k * kCm;
     double            c[D];
 #pragma unroll
     for(int row = 0; row < D; ++row)
     {
         const int nr = row / 3;
-        double    v  = 0.0;
-        if(live)
-        {
-            const double* M = hess + k * kCm;
-            v = (nc >= nr) ? M[9 * sym_index(nr, nc) + (row % 3) + 3 * a] : M[9 * sym_index(nc, nr) + a + 3 * (row % 3)];
-        }
+        double    v  = (nc >= nr) ? M[9 * sym_index(nr, nc) + (row % 3) + 3 * a]
+                                  : M[9 * sym_index(nc, nr) + a + 3 * (row % 3)];
         if(row == col && v == 0.0)
             v = 1.0;
         c[row] = v;
     }
-#pragma unroll
+#pragma unroll 1
     for(int p = 0; p < D; ++p)
     {
-        const double f = c[p];
-        if(lane < D)
-            prow[w][p & 1][lane] = f;
+        const double f = c[0];  // A(p, col)
+        if(live)
+        {
+            prow[p & 1][lane]     = f;
+            prow[p & 1][lane + D] = f;
+        }
         __syncthreads();
-        const double* sr    = prow[w][p & 1];
-        const double  akk   = sr[p];
+        const double* sr    = &prow[p & 1][p];  // sr[q] = A(p, (p + q) mod 48)
+        const double  inv_k = 1.0 / sr[0];
         const bool    piv   = (col == p);
-        const double  inv_k = 1.0 / akk;
         const double  s     = piv ? 0.0 : f * inv_k;
         const double  scale = piv ? inv_k : 1.0;
 #pragma unroll
-        for(int i = 0; i < D; ++i)
-            c[i] = fma(-sr[i], s, c[i]) * scale;
-        c[p] = piv ? -inv_k : s;
+        for(int q = 1; q < D; ++q)
+            c[q] = fma(-sr[q], s, c[q]) * scale;
+        const double nv = piv ? -inv_k : s;
+#pragma unroll
+        for(int q = 0; q < D - 1; ++q)
+            c[q] = c[q + 1];
+        c[D - 1] = nv;
     }
     if(!live)
         return;
@@ -726,53 +734,75 @@ __device__ __forceinline__ T cluster_rowdot(const T* M, const T (*R)[4], int row
     return acc;
 }
 
+// the 1224 scalars of one cluster inverse -> LDS with 16-byte loads, all issued
+// before the first LDS store (the cluster base is 16-byte aligned)
 template <typename T>
-__global__ __launch_bounds__(256) void mas_apply_fine_kernel(const double* __restrict__ r,
-                                                            const int32_t* __restrict__ part_to_real,
-                                                            const int32_t* __restrict__ parent,
-                                                            const T* __restrict__ inv,
-                                                            T*             Rb,
-                                                            T*             Zb,
-                                                            const int32_t* conv,
-                                                            int64_t        C0)
+__device__ __forceinline__ void stage_cluster(const T* __restrict__ inv, int64_t k, T* sM)
+{
+    using V                = typename std::conditional<sizeof(T) == 8, double2, float4>::type;
+    constexpr int kVec     = (int)(kCm * sizeof(T) / sizeof(V));
+    constexpr int kPerLane = (kVec + kWave - 1) / kWave;
+    const V*      src      = reinterpret_cast<const V*>(inv + k * kCm);
+    V*            dst      = reinterpret_cast<V*>(sM);
+    V             t[kPerLane];
+#pragma unroll
+    for(int q = 0; q < kPerLane; ++q)
+    {
+        const int i = q * kWave + (int)threadIdx.x;
+        if(i < kVec)
+            t[q] = src[i];
+    }
+#pragma unroll
+    for(int q = 0; q < kPerLane; ++q)
+    {
+        const int i = q * kWave + (int)threadIdx.x;
+        if(i < kVec)
+            dst[i] = t[q];
+    }
+}
+
+// one 64-lane workgroup per fine cluster: gather r, restrict each in-bank
+// component to its level-1 node, local solve
+template <typename T>
+__global__ __launch_bounds__(64) void mas_apply_fine_kernel(const double* __restrict__ r,
+                                                           const int32_t* __restrict__ part_to_real,
+                                                           const int32_t* __restrict__ parent,
+                                                           const T* __restrict__ inv,
+                                                           T*             Rb,
+                                                           T*             Zb,
+                                                           const int32_t* conv)
 {
     if(conv && *conv != 0)
         return;
-    __shared__ T       sM[4][kCm];
-    __shared__ T       sR[4][kBank][4];
-    __shared__ int32_t sP[4][kBank];
-    const int     w    = threadIdx.x / kWave;
-    const int     lane = threadIdx.x % kWave;
-    const int64_t k    = (int64_t)blockIdx.x * 4 + w;
-    const bool    live = k < C0;
-    for(int q = lane; q < kCm; q += kWave)
-        sM[w][q] = live ? inv[k * kCm + q] : T(0);
+    __shared__ __align__(16) T sM[kCm];
+    __shared__ T               sR[kBank][4];
+    __shared__ int32_t         sP[kBank];
+    const int                  lane = threadIdx.x;
+    const int64_t              k    = blockIdx.x;
+    stage_cluster<T>(inv, k, sM);
     if(lane < kBank)
     {
-        int32_t real = live ? part_to_real[k * kBank + lane] : -1;
+        const int32_t real = part_to_real[k * kBank + lane];
 #pragma unroll
         for(int c = 0; c < 3; ++c)
-            sR[w][lane][c] = real >= 0 ? (T)r[3 * (int64_t)real + c] : T(0);
-        sP[w][lane] = real >= 0 ? parent[k * kBank + lane] : -1;
+            sR[lane][c] = real >= 0 ? (T)r[3 * (int64_t)real + c] : T(0);
+        sP[lane] = real >= 0 ? parent[k * kBank + lane] : -1;
     }
     __syncthreads();
-    if(!live)
-        return;
     if(lane < kBank)
     {
-        // each in-bank component's sum -> its level-1 node (restriction)
-        int32_t p = sP[w][lane];
+        const int32_t p = sP[lane];
         if(p >= 0)
         {
             bool first = true;
             T    s0 = 0, s1 = 0, s2 = 0;
             for(int q = 0; q < kBank; ++q)
-                if(sP[w][q] == p)
+                if(sP[q] == p)
                 {
                     first = first && q >= lane;
-                    s0 += sR[w][q][0];
-                    s1 += sR[w][q][1];
-                    s2 += sR[w][q][2];
+                    s0 += sR[q][0];
+                    s1 += sR[q][1];
+                    s2 += sR[q][2];
                 }
             if(first)
             {
@@ -785,63 +815,73 @@ __global__ __launch_bounds__(256) void mas_apply_fine_kernel(const double* __res
     if(lane < 3 * kBank)
     {
         const int row = lane / 3, comp = lane % 3;
-        Zb[3 * (k * kBank + row) + comp] = cluster_rowdot<T>(sM[w], sR[w], row, comp);
+        Zb[3 * (k * kBank + row) + comp] = cluster_rowdot<T>(sM, sR, row, comp);
     }
 }
 
+struct LevelSpan
+{
+    int32_t begin[kMaxLvl + 1];
+    int32_t count[kMaxLvl + 1];
+};
+
+// restriction above level 1, level by level in one workgroup: every group of
+// in-bank siblings (sib = bank mask | 0x10000 on the lowest sibling) adds up
+// into its parent, in a fixed order (deterministic, no atomics)
 template <typename T>
-__global__ __launch_bounds__(256) void mas_apply_coarse_kernel(const T* __restrict__ inv,
-                                                              const T* __restrict__ Rb,
-                                                              T*             Zb,
-                                                              const int32_t* __restrict__ cluster_ids,
-                                                              const int32_t* __restrict__ desc_ptr,
-                                                              const int32_t* __restrict__ desc,
-                                                              const int32_t* __restrict__ desc_anc,
-                                                              const int32_t* conv)
+__global__ __launch_bounds__(1024) void mas_restrict_kernel(T*             Rb,
+                                                           const int32_t* __restrict__ parent,
+                                                           const int32_t* __restrict__ sib,
+                                                           LevelSpan      lv,
+                                                           int            L,
+                                                           const int32_t* conv)
 {
     if(conv && *conv != 0)
         return;
-    __shared__ T sM[kCm];
-    __shared__ T sR[kBank][4];
-    const int64_t k = cluster_ids[blockIdx.x];
-    for(int q = threadIdx.x; q < kCm; q += blockDim.x)
-        sM[q] = inv[k * kCm + q];
-    if(threadIdx.x < kBank * 4)
-        (&sR[0][0])[threadIdx.x] = T(0);
-    __syncthreads();
-    // this cluster's R from the level-1 sums of its descendants: every thread
-    // sums a contiguous run and flushes once per ancestor change
-    const int32_t d0 = desc_ptr[blockIdx.x], d1 = desc_ptr[blockIdx.x + 1];
-    const int32_t n  = d1 - d0;
-    const int32_t per = (n + (int)blockDim.x - 1) / (int)blockDim.x;
-    const int32_t b0 = d0 + per * (int)threadIdx.x, b1 = min(d1, b0 + per);
-    int32_t       cur = -1;
-    T             s0 = 0, s1 = 0, s2 = 0;
-    for(int32_t q = b0; q < b1; ++q)
+    for(int l = 1; l + 1 < L; ++l)
     {
-        int32_t a = desc_anc[q] - (int32_t)(k * kBank);
-        if(a != cur)
+        const int32_t begin = lv.begin[l], cnt = lv.count[l];
+        for(int32_t x = threadIdx.x; x < cnt; x += blockDim.x)
         {
-            if(cur >= 0)
-            {
-                atomicAdd(&sR[cur][0], s0);
-                atomicAdd(&sR[cur][1], s1);
-                atomicAdd(&sR[cur][2], s2);
-            }
-            cur = a;
-            s0 = s1 = s2 = T(0);
+            const int32_t slot = begin + x;
+            const int32_t m    = sib[slot];
+            if(!(m & 0x10000))
+                continue;
+            const int32_t b0 = slot & ~(kBank - 1);
+            T             s0 = 0, s1 = 0, s2 = 0;
+            for(int q = 0; q < kBank; ++q)
+                if(m & (1 << q))
+                {
+                    const int64_t y = 3 * (int64_t)(b0 + q);
+                    s0 += Rb[y];
+                    s1 += Rb[y + 1];
+                    s2 += Rb[y + 2];
+                }
+            const int64_t pp = 3 * (int64_t)parent[slot];
+            Rb[pp]           = s0;
+            Rb[pp + 1]       = s1;
+            Rb[pp + 2]       = s2;
         }
-        const int64_t m = desc[q];
-        s0 += Rb[3 * m + 0];
-        s1 += Rb[3 * m + 1];
-        s2 += Rb[3 * m + 2];
+        __syncthreads();
     }
-    if(cur >= 0)
-    {
-        atomicAdd(&sR[cur][0], s0);
-        atomicAdd(&sR[cur][1], s1);
-        atomicAdd(&sR[cur][2], s2);
-    }
+}
+
+// one 64-lane workgroup per coarse cluster: local solve on the restricted R
+template <typename T>
+__global__ __launch_bounds__(64) void mas_apply_coarse_kernel(const T* __restrict__ inv,
+                                                             const T* __restrict__ Rb,
+                                                             T*             Zb,
+                                                             int64_t        c_first,
+                                                             const int32_t* conv)
+{
+    if(conv && *conv != 0)
+        return;
+    __shared__ __align__(16) T sM[kCm];
+    __shared__ T               sR[kBank][4];
+    const int64_t              k = c_first + blockIdx.x;
+    stage_cluster<T>(inv, k, sM);
+    if(threadIdx.x < 3 * kBank)
+        sR[threadIdx.x / 3][threadIdx.x % 3] = Rb[3 * k * kBank + threadIdx.x];
     __syncthreads();
     if(threadIdx.x < 3 * kBank)
     {
@@ -909,8 +949,9 @@ class MasPreconditioner final : public LocalPreconditioner
     int       fp32 = 0, active_levels = 0;
     int64_t   C = 0, C0 = 0;  // clusters, fine clusters
     // device tables
-    DBuf<int32_t> d_part_to_real, d_real_to_part, d_anc, d_parent, d_desc_ptr, d_desc, d_desc_anc, d_coarse_ids;
-    int64_t       n_coarse = 0;
+    DBuf<int32_t> d_part_to_real, d_real_to_part, d_anc, d_parent, d_sib;
+    LevelSpan     span{};
+    int64_t       c_first = 0;  // first coarse cluster
     DBuf<double>  d_hess, d_inv64, d_Dun;
     DBuf<float>   d_inv32;
     DBuf<double>  d_R64, d_Z64;
@@ -948,55 +989,37 @@ class MasPreconditioner final : public LocalPreconditioner
         level_clusters.push_back({0, C0});
         for(int l = 1; l < L; ++l)
             level_clusters.push_back({h.level_begin[l] / kBank, bank_align(h.level_count[l]) / kBank});
-        // coarse clusters: level-1 descendants (each level-1 node once per level),
-        // grouped by cluster, in level-1 order
-        std::vector<int32_t> l1_anc((size_t)std::max(1, L - 1) * std::max(1, h.level_count[L > 1 ? 1 : 0]), -1);
-        const int32_t        N1 = L > 1 ? h.level_count[1] : 0;
-        for(int32_t m = 0; m < N1; ++m)
-        {
-            int32_t cur                    = h.level_begin[1] + m;
-            l1_anc[m]                      = cur;  // level 1
-            for(int l = 2; l < L; ++l)
+        // sibling groups above level 1: bank mask of the nodes sharing a parent,
+        // bit 16 on the lowest of them
+        std::vector<int32_t> sib((size_t)std::max(1, h.total), 0);
+        for(int l = 1; l + 1 < L; ++l)
+            for(int32_t i = 0; i < h.level_count[l]; ++i)
             {
-                cur                                    = h.going_next[cur];
-                l1_anc[(size_t)(l - 1) * N1 + m]       = cur;
-            }
-        }
-        std::vector<int32_t> dptr, dl, danc, cids;
-        dptr.push_back(0);
-        for(int l = 1; l < L; ++l)
-        {
-            int64_t first = level_clusters[l].first, cnt = level_clusters[l].second;
-            std::vector<std::vector<int32_t>> bucket((size_t)cnt);
-            for(int32_t m = 0; m < N1; ++m)
-            {
-                int32_t a = l1_anc[(size_t)(l - 1) * N1 + m];
-                if(a < 0)
-                    continue;
-                int64_t kc = a / kBank - first;
-                MI_REQUIRE(kc >= 0 && kc < cnt, MI_ERR_STATE, "MAS: inconsistent hierarchy");
-                bucket[kc].push_back(m);
-            }
-            for(int64_t kc = 0; kc < cnt; ++kc)
-            {
-                for(int32_t m : bucket[kc])
+                const int32_t slot = h.level_begin[l] + i, b0 = slot & ~(kBank - 1);
+                int32_t       mask = 0, low = -1;
+                for(int q = 0; q < kBank; ++q)
                 {
-                    dl.push_back(h.level_begin[1] + m);
-                    danc.push_back(l1_anc[(size_t)(l - 1) * N1 + m]);
+                    const int32_t y = b0 + q;
+                    if(y - h.level_begin[l] < h.level_count[l] && parent[y] == parent[slot])
+                    {
+                        mask |= 1 << q;
+                        if(low < 0)
+                            low = q;
+                    }
                 }
-                dptr.push_back((int32_t)dl.size());
-                cids.push_back((int32_t)(first + kc));
+                sib[slot] = mask | ((slot - b0) == low ? 0x10000 : 0);
             }
+        for(int l = 0; l <= L && l <= kMaxLvl; ++l)
+        {
+            span.begin[l] = h.level_begin[l];
+            span.count[l] = h.level_count[l];
         }
+        c_first = L > 1 ? h.level_begin[1] / kBank : C;
+        upload(d_sib, sib);
         upload(d_part_to_real, h.part_to_real);
         upload(d_real_to_part, h.real_to_part);
         upload(d_anc, anc);
         upload(d_parent, parent);
-        upload(d_desc_ptr, dptr);
-        upload(d_desc, dl);
-        upload(d_desc_anc, danc);
-        upload(d_coarse_ids, cids);
-        n_coarse = (int64_t)cids.size();
         d_hess.resize((size_t)std::max<int64_t>(1, C * kCm));
         if(fp32)
         {
@@ -1042,9 +1065,9 @@ class MasPreconditioner final : public LocalPreconditioner
                 mas_push_kernel<<<ceil_div(cnt, 4), 256, 0, s>>>(first, cnt, d_parent.data(), d_hess.data());
         }
         if(fp32)
-            mas_invert_kernel<float><<<ceil_div(C, 4), 256, 0, s>>>(d_hess.data(), d_inv32.data(), C);
+            mas_invert_kernel<float><<<(unsigned)C, 64, 0, s>>>(d_hess.data(), d_inv32.data(), C);
         else
-            mas_invert_kernel<double><<<ceil_div(C, 4), 256, 0, s>>>(d_hess.data(), d_inv64.data(), C);
+            mas_invert_kernel<double><<<(unsigned)C, 64, 0, s>>>(d_hess.data(), d_inv64.data(), C);
         if(h.has_unpart)
             mas_unpart_kernel<<<grid_for(h.V), 256, 0, s>>>(A.row_ptr.data(), A.cols.data(), A.vals.data(), A.stride,
                                                             blk_off, d_real_to_part.data(), h.V, d_Dun.data());
@@ -1055,11 +1078,12 @@ class MasPreconditioner final : public LocalPreconditioner
     void apply_t(const double* r, double* z, const int32_t* conv, hipStream_t s, const T* inv, T* R, T* Z)
     {
         if(C0 > 0)
-            mas_apply_fine_kernel<T><<<ceil_div(C0, 4), 256, 0, s>>>(
-                r, d_part_to_real.data(), d_parent.data(), inv, R, Z, conv, C0);
-        if(n_coarse > 0)
-            mas_apply_coarse_kernel<T><<<(int)n_coarse, 256, 0, s>>>(
-                inv, R, Z, d_coarse_ids.data(), d_desc_ptr.data(), d_desc.data(), d_desc_anc.data(), conv);
+            mas_apply_fine_kernel<T><<<(unsigned)C0, 64, 0, s>>>(r, d_part_to_real.data(), d_parent.data(), inv, R, Z,
+                                                                 conv);
+        if(h.L > 2)
+            mas_restrict_kernel<T><<<1, 1024, 0, s>>>(R, d_parent.data(), d_sib.data(), span, h.L, conv);
+        if(C > c_first)
+            mas_apply_coarse_kernel<T><<<(unsigned)(C - c_first), 64, 0, s>>>(inv, R, Z, c_first, conv);
         int Lact = (active_levels > 0 && active_levels <= h.L) ? active_levels : h.L;
         mas_collect_kernel<T><<<grid_for(h.V), 256, 0, s>>>(
             r, z, d_real_to_part.data(), d_anc.data(), Lact, Z, d_Dun.data(), conv, h.V);

@@ -2,12 +2,15 @@
 //
 // Replaces LinearSystemContext::dot / norm (cuda_tool/linear_system.h:436-502,
 // cuda_tool/linear_reduction.h:14-60) and LinearFusedPCG's per-iteration
-// vector kernels (linear_system/linear_fused_pcg.cu): fused_dot (245-279) with
-// the scalar node folded into its last workgroup (pcg_scalar_body, 309-327),
-// fused_update_xr (504-549) and fused_update_p_beta (593-611).
+// vector kernels (linear_system/linear_fused_pcg.cu): fused_dot (245-279),
+// fused_update_xr (504-549), the scalar node (pcg_scalar_body, 309-327) and
+// fused_update_p_beta (593-611).
 //
-// Reductions are deterministic: every workgroup writes a partial, the last
-// workgroup to arrive (atomic ticket) sums the partials in index order.
+// Reductions are deterministic and need neither fences nor atomics: every
+// workgroup writes a partial, and every workgroup of the consuming kernel sums
+// all partials in index order (an 8 KB read that stays in L2).  An agent-scope
+// fence per workgroup, which the last-workgroup-finishes pattern needs, costs
+// an L2 write-back on each of the 8 XCDs.
 #include <cmath>
 
 #include "linsys.hpp"
@@ -111,89 +114,23 @@ __global__ void pcg_init_kernel(double* x, double* r, const double* b, int64_t n
     }
 }
 
-// last-workgroup finish of a pair of sums (s0, s1) -> device
-__device__ __forceinline__ bool finish_pair(double s0, double s1, double* partials, uint32_t* ticket, double* lds, double out[2])
+// the fixed-order sum of a reduction's partials, redone by every workgroup of
+// the consuming kernel (no fence, no atomics, deterministic)
+__device__ __forceinline__ double sum_partials(const double* part, int g, int stride, double* lds)
 {
-    __shared__ int s_last;
-    double         b0 = block_sum<kThreads>(s0, lds);
-    double         b1 = block_sum<kThreads>(s1, lds);
-    if(threadIdx.x == 0)
-    {
-        partials[2 * blockIdx.x]     = b0;
-        partials[2 * blockIdx.x + 1] = b1;
-        __threadfence();
-        uint32_t t = atomicAdd(ticket, 1u);
-        s_last     = (t == gridDim.x - 1u);
-    }
-    __syncthreads();
-    if(!s_last)
-        return false;
-    __threadfence();
-    double a0 = 0.0, a1 = 0.0;
-    for(int k = threadIdx.x; k < (int)gridDim.x; k += kThreads)
-    {
-        a0 += __hip_atomic_load(&partials[2 * k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        a1 += __hip_atomic_load(&partials[2 * k + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    out[0] = block_sum<kThreads>(a0, lds);
-    out[1] = block_sum<kThreads>(a1, lds);
-    if(threadIdx.x == 0)
-        *ticket = 0u;
-    return true;
+    double acc = 0.0;
+    for(int k = threadIdx.x; k < g; k += kThreads)
+        acc += part[stride * k];
+    return block_sum<kThreads>(acc, lds);
 }
 
-// the scalar node of one iteration (pcg_scalar_body, linear_fused_pcg.cu:309-327):
-// converged = |rz_new| <= rz_tol (or ||r||^2 <= rr_tol), beta = rz_new / rz,
-// rz <- rz_new unless converged
-__device__ __forceinline__ void pcg_scalar_update(PcgScalars* sc, double rz_new, double rr)
+__global__ __launch_bounds__(kThreads) void pcg_dot_partials_kernel(const double* r,
+                                                                   const double* z,
+                                                                   int64_t       n,
+                                                                   double*       part,
+                                                                   const int32_t* conv)
 {
-    sc->rz_new = rz_new;
-    sc->rr     = rr;
-    sc->iter += 1;
-    int conv = sc->criterion == 0 ? (fabs(rz_new) <= sc->rz_tol) : (rr <= sc->rr_tol);
-    if(!isfinite(rz_new) || !isfinite(rr))
-        conv = 0;
-    sc->beta = rz_new / sc->rz;
-    if(conv)
-    {
-        sc->converged = 1;
-        sc->conv_iter = sc->iter;
-    }
-    else
-        sc->rz = rz_new;
-}
-
-__global__ __launch_bounds__(kThreads) void pcg_dot_rz_rr_kernel(const double* r,
-                                                                const double* z,
-                                                                int64_t       n,
-                                                                double*       partials,
-                                                                uint32_t*     ticket,
-                                                                double*       out3)
-{
-    __shared__ double lds[kThreads / kWave];
-    double            s0 = 0.0, s1 = 0.0;
-    for(int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
-    {
-        double ri = r[i];
-        s0 += ri * z[i];
-        s1 += ri * ri;
-    }
-    double out[2];
-    if(finish_pair(s0, s1, partials, ticket, lds, out) && threadIdx.x == 0)
-    {
-        out3[0] = out[0];
-        out3[1] = out[1];
-    }
-}
-
-__global__ __launch_bounds__(kThreads) void pcg_dot_scalar_kernel(PcgScalars*   sc,
-                                                                 const double* r,
-                                                                 const double* z,
-                                                                 int64_t       n,
-                                                                 double*       partials,
-                                                                 uint32_t*     ticket)
-{
-    if(sc->converged)
+    if(conv && *conv != 0)
         return;
     __shared__ double lds[kThreads / kWave];
     double            s0 = 0.0, s1 = 0.0;
@@ -203,45 +140,95 @@ __global__ __launch_bounds__(kThreads) void pcg_dot_scalar_kernel(PcgScalars*   
         s0 += ri * z[i];
         s1 += ri * ri;
     }
-    double out[2];
-    if(finish_pair(s0, s1, partials, ticket, lds, out) && threadIdx.x == 0)
-        pcg_scalar_update(sc, out[0], out[1]);
+    s0 = block_sum<kThreads>(s0, lds);
+    s1 = block_sum<kThreads>(s1, lds);
+    if(threadIdx.x == 0)
+    {
+        part[2 * blockIdx.x]     = s0;
+        part[2 * blockIdx.x + 1] = s1;
+    }
 }
 
-// fused_update_xr (linear_fused_pcg.cu:504-549): x += alpha p, r -= alpha Ap,
-// and Ap is zeroed for the next SpMV's atomics
-__global__ void pcg_update_xr_kernel(PcgScalars* sc, double* x, const double* p, double* r, double* Ap, int64_t n)
+__global__ __launch_bounds__(kThreads) void pcg_init_scalars_kernel(PcgScalars* sc, const double* part, int g,
+                                                                   int criterion, double tol)
+{
+    __shared__ double lds[kThreads / kWave];
+    double            rz = sum_partials(part, g, 2, lds);
+    double            bb = sum_partials(part + 1, g, 2, lds);
+    if(threadIdx.x == 0)
+    {
+        sc->rz         = rz;
+        sc->rz_pending = rz;
+        sc->rz_new     = rz;
+        sc->bb         = bb;
+        sc->rr         = bb;
+        sc->pAp = sc->alpha = sc->beta = 0.0;
+        sc->rz_tol     = tol * fabs(rz);
+        sc->rr_tol     = tol * tol * bb;
+        sc->converged  = 0;
+        sc->criterion  = criterion;
+        sc->iter       = 0;
+        sc->conv_iter  = 0;
+    }
+}
+
+// fused_update_xr (linear_fused_pcg.cu:504-549): alpha = rz / pAp, x += alpha p,
+// r -= alpha Ap.  Workgroup 0 records pAp / alpha and moves rz_pending to rz
+// (nothing else reads rz in this kernel).
+__device__ __forceinline__ double pcg_alpha(PcgScalars* sc, const double* pap_part, int gp, double* lds)
+{
+    const double pAp   = sum_partials(pap_part, gp, 1, lds);
+    const double rz    = sc->rz_pending;
+    const double alpha = rz / pAp;
+    if(blockIdx.x == 0 && threadIdx.x == 0)
+    {
+        sc->pAp   = pAp;
+        sc->alpha = alpha;
+        sc->rz    = rz;
+    }
+    return alpha;
+}
+
+__global__ __launch_bounds__(kThreads) void pcg_update_xr_kernel(PcgScalars*   sc,
+                                                                const double* pap_part,
+                                                                int           gp,
+                                                                double*       x,
+                                                                const double* p,
+                                                                double*       r,
+                                                                const double* Ap,
+                                                                int64_t       n)
 {
     if(sc->converged)
         return;
-    const double alpha = sc->alpha;
-    for(int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    __shared__ double lds[kThreads / kWave];
+    const double      alpha = pcg_alpha(sc, pap_part, gp, lds);
+    for(int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
     {
         x[i] += alpha * p[i];
         r[i] -= alpha * Ap[i];
-        Ap[i] = 0.0;
     }
 }
 
-// fused_update_xr + FEMDiagPreconditioner apply + fused_dot + scalar node, one
+// fused_update_xr + FEMDiagPreconditioner apply + fused_dot partials, one
 // thread per 3x3 block row; DoFs outside [blk_off, blk_off + blk_cnt) get z = r
 __global__ __launch_bounds__(kThreads) void pcg_update_xr_diag3_kernel(PcgScalars*   sc,
+                                                                      const double* pap_part,
+                                                                      int           gp,
                                                                       double*       x,
                                                                       const double* p,
                                                                       double*       r,
-                                                                      double*       Ap,
+                                                                      const double* Ap,
                                                                       double*       z,
                                                                       const double* __restrict__ D,
                                                                       int64_t       blk_off,
                                                                       int64_t       blk_cnt,
                                                                       int64_t       nb,
-                                                                      double*       partials,
-                                                                      uint32_t*     ticket)
+                                                                      double*       part)
 {
     if(sc->converged)
         return;
     __shared__ double lds[kThreads / kWave];
-    const double      alpha = sc->alpha;
+    const double      alpha = pcg_alpha(sc, pap_part, gp, lds);
     double            s0 = 0.0, s1 = 0.0;
     for(int64_t k = (int64_t)blockIdx.x * kThreads + threadIdx.x; k < nb; k += (int64_t)gridDim.x * kThreads)
     {
@@ -253,7 +240,6 @@ __global__ __launch_bounds__(kThreads) void pcg_update_xr_diag3_kernel(PcgScalar
             x[i] += alpha * p[i];
             rv[c] = r[i] - alpha * Ap[i];
             r[i]  = rv[c];
-            Ap[i] = 0.0;
         }
         double  zv[3];
         int64_t lb = k - blk_off;
@@ -278,17 +264,53 @@ __global__ __launch_bounds__(kThreads) void pcg_update_xr_diag3_kernel(PcgScalar
             s1 += rv[c] * rv[c];
         }
     }
-    double out[2];
-    if(finish_pair(s0, s1, partials, ticket, lds, out) && threadIdx.x == 0)
-        pcg_scalar_update(sc, out[0], out[1]);
+    s0 = block_sum<kThreads>(s0, lds);
+    s1 = block_sum<kThreads>(s1, lds);
+    if(threadIdx.x == 0)
+    {
+        part[2 * blockIdx.x]     = s0;
+        part[2 * blockIdx.x + 1] = s1;
+    }
 }
 
-__global__ void pcg_update_p_kernel(PcgScalars* sc, double* p, const double* z, int64_t n)
+// the scalar node (pcg_scalar_body, linear_fused_pcg.cu:309-327) folded into
+// fused_update_p_beta (593-611): converged = |rz_new| <= rz_tol (or
+// ||r||^2 <= rr_tol), beta = rz_new / rz, p = z + beta p unless converged.
+// Every workgroup derives the same values from the partials; workgroup 0
+// records them.
+__global__ __launch_bounds__(kThreads) void pcg_update_p_kernel(PcgScalars*   sc,
+                                                               const double* part,
+                                                               int           gd,
+                                                               double*       p,
+                                                               const double* z,
+                                                               int64_t       n)
 {
     if(sc->converged)
         return;
-    const double beta = sc->beta;
-    for(int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    __shared__ double lds[kThreads / kWave];
+    const double      rz_new = sum_partials(part, gd, 2, lds);
+    const double      rr     = sum_partials(part + 1, gd, 2, lds);
+    bool conv = sc->criterion == 0 ? (fabs(rz_new) <= sc->rz_tol) : (rr <= sc->rr_tol);
+    if(!isfinite(rz_new) || !isfinite(rr))
+        conv = false;
+    const double beta = rz_new / sc->rz;
+    if(blockIdx.x == 0 && threadIdx.x == 0)
+    {
+        const int it   = sc->iter + 1;
+        sc->iter       = it;
+        sc->rz_new     = rz_new;
+        sc->rr         = rr;
+        sc->beta       = beta;
+        sc->rz_pending = rz_new;
+        if(conv)
+        {
+            sc->converged = 1;
+            sc->conv_iter = it;
+        }
+    }
+    if(conv)
+        return;
+    for(int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
         p[i] = z[i] + beta * p[i];
 }
 
@@ -395,8 +417,6 @@ static void ensure(SpmvScratch& scr)
     {
         scr.grid = device_cu_count() * 4;
         scr.partials.resize((size_t)2 * scr.grid);
-        scr.ticket.resize(4);
-        MI_HIP(hipMemset(scr.ticket.data(), 0, 4 * sizeof(uint32_t)));
     }
 }
 
@@ -406,48 +426,41 @@ void pcg_init_vectors(double* x, double* r, const double* b, int64_t n, hipStrea
         pcg_init_kernel<<<vec_grid(n), kThreads, 0, s>>>(x, r, b, n);
 }
 
-void pcg_dot_rz_rr(const double* r, const double* z, int64_t n, double* out3, SpmvScratch& scr, hipStream_t s)
+int pcg_dot_partials(const double* r, const double* z, int64_t n, double* part, const int32_t* conv,
+                     SpmvScratch& scr, hipStream_t s)
 {
     ensure(scr);
     int g = std::min(scr.grid, std::max(1, ceil_div(n, kThreads)));
-    pcg_dot_rz_rr_kernel<<<g, kThreads, 0, s>>>(r, z, n, scr.partials.data(), scr.ticket.data(), out3);
+    pcg_dot_partials_kernel<<<g, kThreads, 0, s>>>(r, z, n, part, conv);
+    return g;
 }
 
-void pcg_update_xr(PcgScalars* sc, double* x, const double* p, double* r, double* Ap, int64_t n, hipStream_t s)
+void pcg_init_scalars(PcgScalars* sc, const double* part, int g, int criterion, double tol, hipStream_t s)
 {
-    pcg_update_xr_kernel<<<vec_grid(n), kThreads, 0, s>>>(sc, x, p, r, Ap, n);
+    pcg_init_scalars_kernel<<<1, kThreads, 0, s>>>(sc, part, g, criterion, tol);
 }
 
-void pcg_update_xr_diag3(PcgScalars*   sc,
-                         double*       x,
-                         const double* p,
-                         double*       r,
-                         double*       Ap,
-                         double*       z,
-                         const double* D,
-                         int64_t       blk_off,
-                         int64_t       blk_cnt,
-                         int64_t       n,
-                         SpmvScratch&  scr,
-                         hipStream_t   s)
+void pcg_update_xr(PcgScalars* sc, const double* pap_part, int gp, double* x, const double* p, double* r,
+                   const double* Ap, int64_t n, hipStream_t s)
+{
+    pcg_update_xr_kernel<<<vec_grid(n), kThreads, 0, s>>>(sc, pap_part, gp, x, p, r, Ap, n);
+}
+
+int pcg_update_xr_diag3(PcgScalars* sc, const double* pap_part, int gp, double* x, const double* p, double* r,
+                        const double* Ap, double* z, const double* D, int64_t blk_off, int64_t blk_cnt, int64_t n,
+                        double* part, SpmvScratch& scr, hipStream_t s)
 {
     ensure(scr);
     int64_t nb = n / 3;
     int     g  = std::min(scr.grid, std::max(1, ceil_div(nb, kThreads)));
-    pcg_update_xr_diag3_kernel<<<g, kThreads, 0, s>>>(
-        sc, x, p, r, Ap, z, D, blk_off, blk_cnt, nb, scr.partials.data(), scr.ticket.data());
+    pcg_update_xr_diag3_kernel<<<g, kThreads, 0, s>>>(sc, pap_part, gp, x, p, r, Ap, z, D, blk_off, blk_cnt, nb,
+                                                      part);
+    return g;
 }
 
-void pcg_dot_scalar(PcgScalars* sc, const double* r, const double* z, int64_t n, SpmvScratch& scr, hipStream_t s)
+void pcg_update_p(PcgScalars* sc, const double* part, int gd, double* p, const double* z, int64_t n, hipStream_t s)
 {
-    ensure(scr);
-    int g = std::min(scr.grid, std::max(1, ceil_div(n, kThreads)));
-    pcg_dot_scalar_kernel<<<g, kThreads, 0, s>>>(sc, r, z, n, scr.partials.data(), scr.ticket.data());
-}
-
-void pcg_update_p(PcgScalars* sc, double* p, const double* z, int64_t n, hipStream_t s)
-{
-    pcg_update_p_kernel<<<vec_grid(n), kThreads, 0, s>>>(sc, p, z, n);
+    pcg_update_p_kernel<<<vec_grid(n), kThreads, 0, s>>>(sc, part, gd, p, z, n);
 }
 
 void pcg_copy_if_active(PcgScalars* sc, double* dst, const double* src, int64_t n, hipStream_t s)
