@@ -556,7 +556,10 @@ extern "C" int mi_pcg_solve(mi_sys* sys, const double* b, double* x, const mi_pc
         MI_HIP(hipMalloc((void**)&sys->d_sc, sizeof(PcgScalars)));
         MI_HIP(hipHostMalloc((void**)&sys->h_sc, sizeof(PcgScalars)));
     }
-    spmv_prepare(sys->spmv_scr);  // no allocation may happen inside a captured block
+    // scratch first: the partial buffers' addresses are taken below and baked
+    // into the captured graph; no allocation may happen inside a captured block
+    spmv_prepare(sys->spmv_scr);
+    pcg_prepare(sys->dot_scr);
     PcgScalars& h = *sys->h_sc;
 
     // r = b, x = 0, z = P r, p = z, rz0 = r.z (linear_fused_pcg.cu:1811-1824)

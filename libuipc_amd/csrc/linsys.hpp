@@ -145,6 +145,8 @@ struct PcgScalars
     int32_t conv_iter;
 };
 
+// allocates the PCG reductions' partials (must run before any pointer into them is taken)
+void pcg_prepare(SpmvScratch& scr);
 void pcg_init_vectors(double* x, double* r, const double* b, int64_t n, hipStream_t s);
 // partials of (r.z, r.r) -> part[2 g]; returns g
 int pcg_dot_partials(const double* r, const double* z, int64_t n, double* part, const int32_t* conv,

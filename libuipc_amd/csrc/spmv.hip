@@ -151,6 +151,9 @@ int spmv_full(const BsrMatrix& A, bool use32, const double* x, double* y, double
     const int g = spmv_grid(scr, A.B);
     if(A.B == 0)
         return 0;
+    MI_REQUIRE(x && y && partials && A.full_ptr.data() && A.fcols.data(), MI_ERR_STATE, "spmv_full: null buffer");
+    MI_REQUIRE(partials >= scr.partials.data() && partials + g <= scr.partials.data() + scr.partials.size(),
+               MI_ERR_STATE, "spmv_full: partials outside the scratch");
     if(use32)
     {
         MI_REQUIRE(A.has32, MI_ERR_STATE, "spmv: FP32 values were not assembled");

@@ -411,13 +411,17 @@ float norm_f32_host(const float* x, int64_t n, hipStream_t s)
     return (float)host_norm<float>(x, n, s);
 }
 
-static void ensure(SpmvScratch& scr)
+void pcg_prepare(SpmvScratch& scr)
 {
     if(scr.grid == 0)
     {
         scr.grid = device_cu_count() * 4;
         scr.partials.resize((size_t)2 * scr.grid);
     }
+}
+static void ensure(SpmvScratch& scr)
+{
+    pcg_prepare(scr);
 }
 
 void pcg_init_vectors(double* x, double* r, const double* b, int64_t n, hipStream_t s)
@@ -430,19 +434,24 @@ int pcg_dot_partials(const double* r, const double* z, int64_t n, double* part, 
                      SpmvScratch& scr, hipStream_t s)
 {
     ensure(scr);
+    MI_REQUIRE(r && z && part, MI_ERR_STATE, "pcg_dot_partials: null buffer");
     int g = std::min(scr.grid, std::max(1, ceil_div(n, kThreads)));
+    MI_REQUIRE(part != scr.partials.data() || (size_t)(2 * g) <= scr.partials.size(), MI_ERR_STATE,
+               "pcg_dot_partials: partials too small");
     pcg_dot_partials_kernel<<<g, kThreads, 0, s>>>(r, z, n, part, conv);
     return g;
 }
 
 void pcg_init_scalars(PcgScalars* sc, const double* part, int g, int criterion, double tol, hipStream_t s)
 {
+    MI_REQUIRE(sc && part, MI_ERR_STATE, "pcg_init_scalars: null buffer");
     pcg_init_scalars_kernel<<<1, kThreads, 0, s>>>(sc, part, g, criterion, tol);
 }
 
 void pcg_update_xr(PcgScalars* sc, const double* pap_part, int gp, double* x, const double* p, double* r,
                    const double* Ap, int64_t n, hipStream_t s)
 {
+    MI_REQUIRE(sc && pap_part && x && p && r && Ap, MI_ERR_STATE, "pcg_update_xr: null buffer");
     pcg_update_xr_kernel<<<vec_grid(n), kThreads, 0, s>>>(sc, pap_part, gp, x, p, r, Ap, n);
 }
 
@@ -451,6 +460,7 @@ int pcg_update_xr_diag3(PcgScalars* sc, const double* pap_part, int gp, double* 
                         double* part, SpmvScratch& scr, hipStream_t s)
 {
     ensure(scr);
+    MI_REQUIRE(sc && pap_part && x && p && r && Ap && z && D && part, MI_ERR_STATE, "pcg_update_xr_diag3: null buffer");
     int64_t nb = n / 3;
     int     g  = std::min(scr.grid, std::max(1, ceil_div(nb, kThreads)));
     pcg_update_xr_diag3_kernel<<<g, kThreads, 0, s>>>(sc, pap_part, gp, x, p, r, Ap, z, D, blk_off, blk_cnt, nb,
@@ -460,6 +470,7 @@ int pcg_update_xr_diag3(PcgScalars* sc, const double* pap_part, int gp, double* 
 
 void pcg_update_p(PcgScalars* sc, const double* part, int gd, double* p, const double* z, int64_t n, hipStream_t s)
 {
+    MI_REQUIRE(sc && part && p && z, MI_ERR_STATE, "pcg_update_p: null buffer");
     pcg_update_p_kernel<<<vec_grid(n), kThreads, 0, s>>>(sc, part, gd, p, z, n);
 }
 
