@@ -46,7 +46,32 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-iters", type=int, default=30)
     ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch the PCG iterations directly instead of through captured HIP graphs")
+    ap.add_argument("--calibrate-copy-mb", type=int, default=0,
+                    help="only run N device-to-device copies of this many MiB (the known-size kernel that "
+                         "scales rocprofv3 FETCH_SIZE / WRITE_SIZE) and print the copy bandwidth")
     return ap.parse_args()
+
+
+def calibrate_copy(mb: int) -> None:
+    from libuipc_amd.device import Context
+    from libuipc_amd.hip_events import EventTimer
+
+    ctx = Context(0)
+    nbytes = mb << 20
+    a = ctx.alloc(nbytes // 8, np.float64)
+    b = ctx.alloc(nbytes // 8, np.float64)
+    a.zero()
+    ev = EventTimer(ctx.stream())
+    ctx.lib.mi_memcpy_d2d(ctx.h, b.ptr, a.ptr, nbytes)
+    reps = 5
+    ev.start()
+    for _ in range(reps):
+        ctx.lib.mi_memcpy_d2d(ctx.h, b.ptr, a.ptr, nbytes)
+    ms = ev.stop() / reps
+    print(json.dumps({"copy_bytes": nbytes, "copy_ms": ms, "copy_GBps_read_plus_write": 2 * nbytes / ms / 1e6}))
+    ctx.close()
 
 
 def log(*a):
@@ -55,6 +80,9 @@ def log(*a):
 
 def main():
     args = parse()
+    if args.calibrate_copy_mb > 0:
+        calibrate_copy(args.calibrate_copy_mb)
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(json.dumps({"error": "this build runs on one GPU only: the per-Newton solve of one system is not "
@@ -91,7 +119,8 @@ def main():
     if precond == "diag":
         ls.add_preconditioner(abi.MI_PRECOND_DIAG3, 0, s.n)
 
-    solve_kw = dict(criterion=abi.MI_CRIT_REL_RESIDUAL, tol=args.tol, check_interval=5, use_graph=1)
+    solve_kw = dict(criterion=abi.MI_CRIT_REL_RESIDUAL, tol=args.tol, check_interval=5,
+                    use_graph=0 if args.no_graph else 1)
 
     def step():
         ls.assemble(rows, cols, vals, stream)
@@ -129,7 +158,9 @@ def main():
     ms_solve = ev.stop()
     U = ls.nnz()
 
-    # dominant kernel: the SpMV (+ fused p^T A p), timed with HIP events on its stream
+    # dominant kernel: the SpMV (+ fused x^T A x), timed with HIP events on its
+    # stream.  mi_spmv_dot launches spmv_full_kernel (y += A x over the full
+    # BSR, per-workgroup x^T A x partials) and a one-workgroup partials sum.
     y = ctx.alloc(s.n, np.float64)
     d = ctx.alloc(1, np.float64)
     for _ in range(3):
@@ -169,8 +200,8 @@ def main():
                      "pcg": round(ms_solve, 3),
                      "per_iter_us": round(1e3 * ms_solve / max(1, st["iters"]), 2)},
         "wall_s_timed": round(wall, 3),
-        "roofline": {"kernel": "spmv_sym_dot_kernel<double>" if prec == abi.MI_FP64 else
-                     "spmv_sym_dot_kernel<float>", "bound": "hbm", "achieved": round(gbs, 1),
+        "roofline": {"kernel": "spmv_full_kernel<double>" if prec == abi.MI_FP64 else
+                     "spmv_full_kernel<float>", "bound": "hbm", "achieved": round(gbs, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
                      "traffic": None, "algorithmic_bytes": int(spmv_bytes), "avg_us": round(ms_spmv * 1e3, 2)},
     }

@@ -199,6 +199,30 @@ int mi_precond_mas_export(mi_sys*  sys,
                           double*  cluster_hess_host,
                           double*  cluster_inv_host);
 
+/* ------------------------------------------------------------ Matrix Market */
+/* The reference's debug dumps (extras/debug/dump_linear_system,
+ * linear_system/global_linear_system.cu:42-70; format utils/matrix_market.h:26-158).
+ * mi_sys_write_mtx: the assembled upper BCOO as "coordinate real general", the
+ * 9 scalar entries of every block including zeros, %.17g, 1-based indices
+ * unless one_based == 0.  mi_vec_write_mtx: a device vector as
+ * "array real general".  Both synchronise. */
+int mi_sys_write_mtx(mi_sys* sys, const char* path, int one_based);
+int mi_vec_write_mtx(mi_ctx* ctx, const char* path, const double* x, int64_t n);
+
+/* Host-side readers.  mi_mtx_read_block_triplets groups the scalar entries of
+ * a coordinate file into 3x3 blocks (duplicates summed), ascending by
+ * (block row, block col); call it with NULL arrays to get the sizes, then
+ * again with arrays of n_triplets entries (vals: 9 per block, column-major).
+ * mi_vec_read_mtx reads an "array" file; x_host may be NULL to get n. */
+int mi_mtx_read_block_triplets(const char* path,
+                               int         one_based,
+                               int32_t*    block_rows_out,
+                               int64_t*    n_triplets_out,
+                               int32_t*    rows_host,
+                               int32_t*    cols_host,
+                               double*     vals_host);
+int mi_vec_read_mtx(const char* path, int64_t* n_out, double* x_host);
+
 /* ----------------------------------------------------------------------- PCG */
 enum mi_pcg_criterion
 {

@@ -19,6 +19,7 @@ OBJ = os.path.join(PKG, "_build")
 ROOT = os.path.dirname(PKG)
 
 HIP_SOURCES = ["convert.hip", "spmv.hip", "vec.hip", "precond.hip", "mas.hip", "linsys.hip"]
+HOST_SOURCES = ["mtx.cpp"]  # linked into libmi_linsys.so, no device code
 ARCH = "gfx950"
 
 HIPCC_FLAGS = [
@@ -57,6 +58,11 @@ def build(verbose: bool = False) -> dict[str, str]:
         o = os.path.join(OBJ, src.replace(".hip", ".o"))
         objs.append(o)
         cmds.append([hipcc, *HIPCC_FLAGS, "-c", os.path.join(CSRC, src), "-o", o])
+    for src in HOST_SOURCES:
+        o = os.path.join(OBJ, src.replace(".cpp", ".o"))
+        objs.append(o)
+        cmds.append(["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", f"-I{os.path.join(ROOT, 'include')}",
+                     "-c", os.path.join(CSRC, src), "-o", o])
     with ThreadPoolExecutor(max_workers=min(len(cmds), os.cpu_count() or 4)) as ex:
         list(ex.map(_run, cmds))
     lib = os.path.join(PKG, "libmi_linsys.so")

@@ -385,6 +385,75 @@ int mi_sys_export_bcoo(mi_sys* sys, int32_t* rows, int32_t* cols, double* vals, 
     MI_API_END
 }
 
+// Matrix Market dumps in the reference's debug format (utils/matrix_market.h:34-110):
+// banner, four comment lines, "rows cols entries", then every block's 9 scalars
+// row by row within the block, zeros included, %.17g.
+int mi_sys_write_mtx(mi_sys* sys, const char* path, int one_based)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(sys && path, MI_ERR_ARG, "mi_sys_write_mtx: null argument");
+    MI_REQUIRE(sys->assembled, MI_ERR_STATE, "mi_sys_write_mtx: not assembled");
+    hipStream_t          s = sys->ctx->stream;
+    auto&                A = sys->A;
+    std::vector<int32_t> rows(A.U), cols(A.U);
+    std::vector<double>  vals(9 * A.U);
+    if(A.U)
+    {
+        DBuf<double> aos;
+        aos.resize(9 * A.U);
+        soa_to_aos_kernel<<<std::max(1, std::min(ceil_div(9 * A.U, 256), device_cu_count() * 8)), 256, 0, s>>>(
+            A.vals.data(), A.stride, A.U, aos.data());
+        MI_HIP(hipGetLastError());
+        MI_HIP(hipMemcpyAsync(rows.data(), A.rows.data(), sizeof(int32_t) * A.U, hipMemcpyDeviceToHost, s));
+        MI_HIP(hipMemcpyAsync(cols.data(), A.cols.data(), sizeof(int32_t) * A.U, hipMemcpyDeviceToHost, s));
+        MI_HIP(hipMemcpyAsync(vals.data(), aos.data(), sizeof(double) * 9 * A.U, hipMemcpyDeviceToHost, s));
+        MI_HIP(hipStreamSynchronize(s));
+    }
+    FILE* fp = std::fopen(path, "w");
+    MI_REQUIRE(fp, MI_ERR_ARG, std::string("mi_sys_write_mtx: cannot open ") + path);
+    const long long n = 3LL * A.B, entries = 9LL * A.U;
+    const int       o = one_based ? 1 : 0;
+    std::fprintf(fp, "%%%%MatrixMarket matrix coordinate real general\n");
+    std::fprintf(fp, "%% Exported from block-sparse triplet matrix\n");
+    std::fprintf(fp, "%% Block dimension: 3x3\n");
+    std::fprintf(fp, "%% Total rows: %lld, Total cols: %lld\n", n, n);
+    std::fprintf(fp, "%% Total entries (including zeros): %lld\n", entries);
+    std::fprintf(fp, "%lld %lld %lld\n", n, n, entries);
+    for(int64_t t = 0; t < A.U; ++t)
+        for(int bi = 0; bi < 3; ++bi)
+            for(int bj = 0; bj < 3; ++bj)
+                std::fprintf(fp, "%lld %lld %.17g\n", 3LL * rows[t] + bi + o, 3LL * cols[t] + bj + o,
+                             vals[9 * t + bi + 3 * bj]);
+    const bool ok = std::ferror(fp) == 0;
+    std::fclose(fp);
+    MI_REQUIRE(ok, MI_ERR_ARG, std::string("mi_sys_write_mtx: write failed: ") + path);
+    MI_API_END
+}
+
+int mi_vec_write_mtx(mi_ctx* ctx, const char* path, const double* x, int64_t n)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(ctx && path && (x || n == 0) && n >= 0, MI_ERR_ARG, "mi_vec_write_mtx: bad argument");
+    std::vector<double> h(n);
+    if(n)
+    {
+        MI_HIP(hipMemcpyAsync(h.data(), x, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+        MI_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    FILE* fp = std::fopen(path, "w");
+    MI_REQUIRE(fp, MI_ERR_ARG, std::string("mi_vec_write_mtx: cannot open ") + path);
+    std::fprintf(fp, "%%%%MatrixMarket matrix array real general\n");
+    std::fprintf(fp, "%% Exported from dense vector\n");
+    std::fprintf(fp, "%% Vector size: %lld\n", (long long)n);
+    std::fprintf(fp, "%lld 1\n", (long long)n);
+    for(int64_t i = 0; i < n; ++i)
+        std::fprintf(fp, "%.17g\n", h[i]);
+    const bool ok = std::ferror(fp) == 0;
+    std::fclose(fp);
+    MI_REQUIRE(ok, MI_ERR_ARG, std::string("mi_vec_write_mtx: write failed: ") + path);
+    MI_API_END
+}
+
 int mi_spmv_dot(mi_sys* sys, const double* x, double* y, double* d_dot, void* stream)
 {
     MI_API_BEGIN

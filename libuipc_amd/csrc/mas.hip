@@ -29,12 +29,15 @@
 //          FP64 or FP32 upper blocks.
 //
 // Apply (per PCG iteration, GPU; 2887-2960):
-//   fine    one wave per fine cluster: gather r through part_to_real, write each
-//           in-bank component's sum to its level-1 node (plain store), local
-//           solve Z = C^-1 R by row-dots from the LDS-staged inverse;
-//   coarse  one workgroup per coarse cluster: its R from the level-1 sums of its
-//           descendants (contiguous runs in LDS), then the same local solve;
-//   collect z_i = Z[slot(i)] + sum_L Z[ancestor_L(i)] (injection prolongation).
+//   restrict level by level, the lowest node of every in-bank sibling group
+//            stores the group's sum into the parent (level 0 gathers r through
+//            part_to_real); the small top levels run in one workgroup;
+//   coarse   one wave per coarse cluster, local solve Z = C^-1 R;
+//   fine     one wave per fine cluster: gather r, local solve, and the
+//            prolongation z_i = Z0_i + sum_L Z_L[ancestor_L(i)] (injection),
+//            written straight to z.
+//   The local solve reads the inverse as 9 coalesced planes of 136 blocks and
+//   sums each row in a fixed order.
 //   No global atomics in the apply.
 #include <algorithm>
 #include <cstring>
@@ -56,6 +59,14 @@ __host__ __device__ __forceinline__ int sym_index(int row, int col)
 {
     return kBank * row - row * (row + 1) / 2 + col;
 }
+// Cluster inverses are stored as 9 planes of the 136 upper blocks:
+// inv[k * kCm + q * kSym + blk] = entry q (column-major, r + 3 c) of block blk,
+// so a wave reading one plane for 64 consecutive blocks is one coalesced load.
+__host__ __device__ __forceinline__ int inv_at(int blk, int q)
+{
+    return q * kSym + blk;
+}
+
 inline int bank_align(int n)
 {
     return (n + kBank - 1) / kBank * kBank;
@@ -659,14 +670,14 @@ __global__ __launch_bounds__(64) void mas_invert_kernel(const double* __restrict
         const int    nr = row / 3, b = row % 3;
         const double v  = -c[row];
         if(nr < nc)
-            O[9 * sym_index(nr, nc) + b + 3 * a] = (TO)v;
+            O[inv_at(sym_index(nr, nc), b + 3 * a)] = (TO)v;
         else if(nr == nc)
         {
             const int si = sym_index(nc, nc);
             if(b <= a)
-                O[9 * si + b + 3 * a] = (TO)v;
+                O[inv_at(si, b + 3 * a)] = (TO)v;
             if(b < a)
-                O[9 * si + a + 3 * b] = (TO)v;
+                O[inv_at(si, a + 3 * b)] = (TO)v;
         }
     }
 }
@@ -717,106 +728,83 @@ __global__ void mas_unpart_kernel(const int32_t* row_ptr,
 }
 
 // ------------------------------------------------------------------ apply kernels
-// row-dot of one cluster from its LDS-staged inverse (schwarz_local_solve, 1603-1700)
-template <typename T>
-__device__ __forceinline__ T cluster_rowdot(const T* M, const T (*R)[4], int row, int comp)
+// upper block index -> (row node a, column node b), a <= b (inverse of sym_index)
+__device__ __forceinline__ void sym_rc(int blk, int& a, int& b)
 {
-    T acc = T(0);
+    int r = 0;
 #pragma unroll
-    for(int cn = 0; cn < kBank; ++cn)
-    {
-        const bool up   = cn >= row;
-        const int  blk  = up ? sym_index(row, cn) : sym_index(cn, row);
-        const int  base = 9 * blk + (up ? comp : 3 * comp);
-        const int  st   = up ? 3 : 1;
-        acc += M[base] * R[cn][0] + M[base + st] * R[cn][1] + M[base + 2 * st] * R[cn][2];
-    }
-    return acc;
+    for(int t = 1; t < kBank; ++t)
+        r += blk >= kBank * t - t * (t - 1) / 2 ? 1 : 0;
+    a = r;
+    b = blk - (kBank * r - r * (r - 1) / 2) + r;
 }
 
-// the 1224 scalars of one cluster inverse -> LDS with 16-byte loads, all issued
-// before the first LDS store (the cluster base is 16-byte aligned)
+// Z = C^-1 R of one cluster on one wave (schwarz_local_solve, 1603-1700).
+// Each lane owns the upper blocks lane, lane + 64, lane + 128; block (a, b)
+// gives M_ab R_b to row node a and, off the diagonal, M_ab^T R_a to row node
+// b.  Every (row node, column node) pair gets exactly one contribution in
+// sC[column node][3 row node + component]; a row's sum runs over the column
+// nodes in ascending order, so the result does not depend on scheduling.
 template <typename T>
-__device__ __forceinline__ void stage_cluster(const T* __restrict__ inv, int64_t k, T* sM)
+struct ClusterBlocks
 {
-    using V                = typename std::conditional<sizeof(T) == 8, double2, float4>::type;
-    constexpr int kVec     = (int)(kCm * sizeof(T) / sizeof(V));
-    constexpr int kPerLane = (kVec + kWave - 1) / kWave;
-    const V*      src      = reinterpret_cast<const V*>(inv + k * kCm);
-    V*            dst      = reinterpret_cast<V*>(sM);
-    V             t[kPerLane];
-#pragma unroll
-    for(int q = 0; q < kPerLane; ++q)
-    {
-        const int i = q * kWave + (int)threadIdx.x;
-        if(i < kVec)
-            t[q] = src[i];
-    }
-#pragma unroll
-    for(int q = 0; q < kPerLane; ++q)
-    {
-        const int i = q * kWave + (int)threadIdx.x;
-        if(i < kVec)
-            dst[i] = t[q];
-    }
-}
+    T m[3][9];
+};
 
-// one 64-lane workgroup per fine cluster: gather r, restrict each in-bank
-// component to its level-1 node, local solve
 template <typename T>
-__global__ __launch_bounds__(64) void mas_apply_fine_kernel(const double* __restrict__ r,
-                                                           const int32_t* __restrict__ part_to_real,
-                                                           const int32_t* __restrict__ parent,
-                                                           const T* __restrict__ inv,
-                                                           T*             Rb,
-                                                           T*             Zb,
-                                                           const int32_t* conv)
+__device__ __forceinline__ void cluster_load(const T* __restrict__ M, int lane, ClusterBlocks<T>& cb)
 {
-    if(conv && *conv != 0)
-        return;
-    __shared__ __align__(16) T sM[kCm];
-    __shared__ T               sR[kBank][4];
-    __shared__ int32_t         sP[kBank];
-    const int                  lane = threadIdx.x;
-    const int64_t              k    = blockIdx.x;
-    stage_cluster<T>(inv, k, sM);
-    if(lane < kBank)
-    {
-        const int32_t real = part_to_real[k * kBank + lane];
 #pragma unroll
-        for(int c = 0; c < 3; ++c)
-            sR[lane][c] = real >= 0 ? (T)r[3 * (int64_t)real + c] : T(0);
-        sP[lane] = real >= 0 ? parent[k * kBank + lane] : -1;
-    }
-    __syncthreads();
-    if(lane < kBank)
+    for(int t = 0; t < 3; ++t)
     {
-        const int32_t p = sP[lane];
-        if(p >= 0)
+        const int blk = lane + kWave * t;
+        if(blk < kSym)
         {
-            bool first = true;
-            T    s0 = 0, s1 = 0, s2 = 0;
-            for(int q = 0; q < kBank; ++q)
-                if(sP[q] == p)
-                {
-                    first = first && q >= lane;
-                    s0 += sR[q][0];
-                    s1 += sR[q][1];
-                    s2 += sR[q][2];
-                }
-            if(first)
+#pragma unroll
+            for(int q = 0; q < 9; ++q)
+                cb.m[t][q] = M[inv_at(blk, q)];
+        }
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void cluster_contrib(const ClusterBlocks<T>& cb,
+                                                int                     lane,
+                                                const T (*sR)[4],
+                                                T (*sC)[3 * kBank])
+{
+#pragma unroll
+    for(int t = 0; t < 3; ++t)
+    {
+        const int blk = lane + kWave * t;
+        if(blk < kSym)
+        {
+            int a, b;
+            sym_rc(blk, a, b);
+            const T* m  = cb.m[t];
+            const T  b0 = sR[b][0], b1 = sR[b][1], b2 = sR[b][2];
+#pragma unroll
+            for(int i = 0; i < 3; ++i)
+                sC[b][3 * a + i] = m[i] * b0 + m[i + 3] * b1 + m[i + 6] * b2;
+            if(a != b)
             {
-                Rb[3 * (int64_t)p + 0] = s0;
-                Rb[3 * (int64_t)p + 1] = s1;
-                Rb[3 * (int64_t)p + 2] = s2;
+                const T a0 = sR[a][0], a1 = sR[a][1], a2 = sR[a][2];
+#pragma unroll
+                for(int j = 0; j < 3; ++j)
+                    sC[a][3 * b + j] = m[3 * j] * a0 + m[3 * j + 1] * a1 + m[3 * j + 2] * a2;
             }
         }
     }
-    if(lane < 3 * kBank)
-    {
-        const int row = lane / 3, comp = lane % 3;
-        Zb[3 * (k * kBank + row) + comp] = cluster_rowdot<T>(sM, sR, row, comp);
-    }
+}
+
+template <typename T>
+__device__ __forceinline__ T cluster_rowsum(const T (*sC)[3 * kBank], int row3)
+{
+    T acc = sC[0][row3];
+#pragma unroll
+    for(int j = 1; j < kBank; ++j)
+        acc += sC[j][row3];
+    return acc;
 }
 
 struct LevelSpan
@@ -825,20 +813,69 @@ struct LevelSpan
     int32_t count[kMaxLvl + 1];
 };
 
-// restriction above level 1, level by level in one workgroup: every group of
-// in-bank siblings (sib = bank mask | 0x10000 on the lowest sibling) adds up
-// into its parent, in a fixed order (deterministic, no atomics)
-template <typename T>
-__global__ __launch_bounds__(1024) void mas_restrict_kernel(T*             Rb,
-                                                           const int32_t* __restrict__ parent,
-                                                           const int32_t* __restrict__ sib,
-                                                           LevelSpan      lv,
-                                                           int            L,
-                                                           const int32_t* conv)
+// Restriction R_{l+1} = sum of children, for one level: the lowest node of
+// every group of in-bank siblings (sib = bank mask | 0x10000 on the lowest)
+// adds the group in ascending slot order and stores the parent's sum (a
+// parent's children all live in one bank: no atomics, deterministic).  At
+// level 0 the values are r gathered through part_to_real.
+template <typename T, bool kLevel0>
+__global__ __launch_bounds__(256) void mas_restrict_level_kernel(const double* __restrict__ r,
+                                                                const int32_t* __restrict__ part_to_real,
+                                                                T*             Rb,
+                                                                const int32_t* __restrict__ parent,
+                                                                const int32_t* __restrict__ sib,
+                                                                int32_t        begin,
+                                                                int32_t        cnt,
+                                                                const int32_t* conv)
 {
     if(conv && *conv != 0)
         return;
-    for(int l = 1; l + 1 < L; ++l)
+    const int32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    if(x >= cnt)
+        return;
+    const int32_t slot = begin + x;
+    const int32_t m    = sib[slot];
+    if(!(m & 0x10000))
+        return;
+    const int32_t b0 = slot & ~(kBank - 1);
+    T             s0 = 0, s1 = 0, s2 = 0;
+    for(int q = 0; q < kBank; ++q)
+        if(m & (1 << q))
+        {
+            if constexpr(kLevel0)
+            {
+                const int64_t y = 3 * (int64_t)part_to_real[b0 + q];
+                s0 += (T)r[y];
+                s1 += (T)r[y + 1];
+                s2 += (T)r[y + 2];
+            }
+            else
+            {
+                const int64_t y = 3 * (int64_t)(b0 + q);
+                s0 += Rb[y];
+                s1 += Rb[y + 1];
+                s2 += Rb[y + 2];
+            }
+        }
+    const int64_t pp = 3 * (int64_t)parent[slot];
+    Rb[pp]           = s0;
+    Rb[pp + 1]       = s1;
+    Rb[pp + 2]       = s2;
+}
+
+// the small top levels (from level lo up) in one workgroup, level by level
+template <typename T>
+__global__ __launch_bounds__(1024) void mas_restrict_tail_kernel(T*             Rb,
+                                                                const int32_t* __restrict__ parent,
+                                                                const int32_t* __restrict__ sib,
+                                                                LevelSpan      lv,
+                                                                int            lo,
+                                                                int            L,
+                                                                const int32_t* conv)
+{
+    if(conv && *conv != 0)
+        return;
+    for(int l = lo; l + 1 < L; ++l)
     {
         const int32_t begin = lv.begin[l], cnt = lv.count[l];
         for(int32_t x = threadIdx.x; x < cnt; x += blockDim.x)
@@ -866,7 +903,7 @@ __global__ __launch_bounds__(1024) void mas_restrict_kernel(T*             Rb,
     }
 }
 
-// one 64-lane workgroup per coarse cluster: local solve on the restricted R
+// one wave per coarse cluster: Z = C^-1 R on the restricted residual
 template <typename T>
 __global__ __launch_bounds__(64) void mas_apply_coarse_kernel(const T* __restrict__ inv,
                                                              const T* __restrict__ Rb,
@@ -876,55 +913,85 @@ __global__ __launch_bounds__(64) void mas_apply_coarse_kernel(const T* __restric
 {
     if(conv && *conv != 0)
         return;
-    __shared__ __align__(16) T sM[kCm];
-    __shared__ T               sR[kBank][4];
-    const int64_t              k = c_first + blockIdx.x;
-    stage_cluster<T>(inv, k, sM);
-    if(threadIdx.x < 3 * kBank)
-        sR[threadIdx.x / 3][threadIdx.x % 3] = Rb[3 * k * kBank + threadIdx.x];
+    __shared__ T     sR[kBank][4];
+    __shared__ T     sC[kBank][3 * kBank];
+    const int        lane = threadIdx.x;
+    const int64_t    k    = c_first + blockIdx.x;
+    ClusterBlocks<T> cb;
+    cluster_load<T>(inv + k * kCm, lane, cb);
+    if(lane < 3 * kBank)
+        sR[lane / 3][lane % 3] = Rb[3 * k * kBank + lane];
     __syncthreads();
-    if(threadIdx.x < 3 * kBank)
-    {
-        const int row = threadIdx.x / 3, comp = threadIdx.x % 3;
-        Zb[3 * (k * kBank + row) + comp] = cluster_rowdot<T>(sM, sR, row, comp);
-    }
+    cluster_contrib<T>(cb, lane, sR, sC);
+    __syncthreads();
+    if(lane < 3 * kBank)
+        Zb[3 * k * kBank + lane] = cluster_rowsum<T>(sC, lane);
 }
 
+// one wave per fine cluster: gather r, local solve, and the prolongation of
+// every level (injection): z_i = Z0_i + sum_{l >= 1} Z_l[ancestor_l(i)].
+// Blocks past the fine clusters take the un-partitioned vertices (3x3 inverse
+// diagonal block, fem_mas_preconditioner.cu:24-45).
 template <typename T>
-__global__ void mas_collect_kernel(const double* __restrict__ r,
-                                   double*        z,
-                                   const int32_t* __restrict__ real_to_part,
-                                   const int32_t* __restrict__ anc,
-                                   int            Lact,
-                                   const T* __restrict__ Zb,
-                                   const double* __restrict__ Dun,
-                                   const int32_t* conv,
-                                   int32_t        V)
+__global__ __launch_bounds__(64) void mas_apply_fine_kernel(const double* __restrict__ r,
+                                                           double*        z,
+                                                           const int32_t* __restrict__ part_to_real,
+                                                           const T* __restrict__ inv,
+                                                           const T* __restrict__ Zb,
+                                                           const int32_t* __restrict__ anc_slot,
+                                                           int32_t        N,
+                                                           int            Lact,
+                                                           int64_t        C0,
+                                                           const int32_t* __restrict__ unpart,
+                                                           int32_t        n_unpart,
+                                                           const double* __restrict__ Dun,
+                                                           const int32_t* conv)
 {
     if(conv && *conv != 0)
         return;
-    for(int32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x)
+    const int lane = threadIdx.x;
+    if((int64_t)blockIdx.x >= C0)
     {
-        const int32_t s = real_to_part[v];
-        if(s < 0)
+        const int64_t u = ((int64_t)blockIdx.x - C0) * kWave + lane;
+        if(u < n_unpart)
         {
+            const int32_t v  = unpart[u];
             const double* D  = Dun + 9 * (int64_t)v;
-            double        r0 = r[3 * (int64_t)v], r1 = r[3 * (int64_t)v + 1], r2 = r[3 * (int64_t)v + 2];
+            const double  r0 = r[3 * (int64_t)v], r1 = r[3 * (int64_t)v + 1], r2 = r[3 * (int64_t)v + 2];
             for(int a = 0; a < 3; ++a)
                 z[3 * (int64_t)v + a] = D[a] * r0 + D[a + 3] * r1 + D[a + 6] * r2;
-            continue;
         }
-        T c0 = Zb[3 * (int64_t)s], c1 = Zb[3 * (int64_t)s + 1], c2 = Zb[3 * (int64_t)s + 2];
-        for(int l = 1; l < Lact; ++l)
+        return;
+    }
+    __shared__ T       sR[kBank][4];
+    __shared__ T       sC[kBank][3 * kBank];
+    __shared__ int32_t sV[kBank];
+    const int64_t      k = blockIdx.x;
+    ClusterBlocks<T>   cb;
+    cluster_load<T>(inv + k * kCm, lane, cb);
+    if(lane < kBank)
+    {
+        const int32_t v = part_to_real[k * kBank + lane];
+#pragma unroll
+        for(int c = 0; c < 3; ++c)
+            sR[lane][c] = v >= 0 ? (T)r[3 * (int64_t)v + c] : T(0);
+        sV[lane] = v;
+    }
+    __syncthreads();
+    cluster_contrib<T>(cb, lane, sR, sC);
+    __syncthreads();
+    if(lane < 3 * kBank)
+    {
+        const int     i = lane / 3, c = lane % 3;
+        const int32_t v = sV[i];
+        if(v >= 0)
         {
-            const int64_t x = anc[(int64_t)(l - 1) * V + v];
-            c0 += Zb[3 * x];
-            c1 += Zb[3 * x + 1];
-            c2 += Zb[3 * x + 2];
+            T             acc  = cluster_rowsum<T>(sC, lane);
+            const int64_t slot = k * kBank + i;
+            for(int l = 1; l < Lact; ++l)
+                acc += Zb[3 * (int64_t)anc_slot[(int64_t)(l - 1) * N + slot] + c];
+            z[3 * (int64_t)v + c] = (double)acc;
         }
-        z[3 * (int64_t)v + 0] = (double)c0;
-        z[3 * (int64_t)v + 1] = (double)c1;
-        z[3 * (int64_t)v + 2] = (double)c2;
     }
 }
 
@@ -949,7 +1016,8 @@ class MasPreconditioner final : public LocalPreconditioner
     int       fp32 = 0, active_levels = 0;
     int64_t   C = 0, C0 = 0;  // clusters, fine clusters
     // device tables
-    DBuf<int32_t> d_part_to_real, d_real_to_part, d_anc, d_parent, d_sib;
+    DBuf<int32_t> d_part_to_real, d_real_to_part, d_anc, d_anc_slot, d_parent, d_sib, d_unpart;
+    int32_t       n_unpart = 0;
     LevelSpan     span{};
     int64_t       c_first = 0;  // first coarse cluster
     DBuf<double>  d_hess, d_inv64, d_Dun;
@@ -989,18 +1057,22 @@ class MasPreconditioner final : public LocalPreconditioner
         level_clusters.push_back({0, C0});
         for(int l = 1; l < L; ++l)
             level_clusters.push_back({h.level_begin[l] / kBank, bank_align(h.level_count[l]) / kBank});
-        // sibling groups above level 1: bank mask of the nodes sharing a parent,
-        // bit 16 on the lowest of them
+        // sibling groups: bank mask of the nodes sharing a parent, bit 16 on
+        // the lowest of them (level 0: the partition slots that hold a vertex)
         std::vector<int32_t> sib((size_t)std::max(1, h.total), 0);
-        for(int l = 1; l + 1 < L; ++l)
-            for(int32_t i = 0; i < h.level_count[l]; ++i)
+        for(int l = 0; l + 1 < L; ++l)
+        {
+            const int32_t lb = l == 0 ? 0 : h.level_begin[l], lc = l == 0 ? h.N : h.level_count[l];
+            for(int32_t i = 0; i < lc; ++i)
             {
-                const int32_t slot = h.level_begin[l] + i, b0 = slot & ~(kBank - 1);
-                int32_t       mask = 0, low = -1;
+                const int32_t slot = lb + i, b0 = slot & ~(kBank - 1);
+                if(parent[slot] < 0)
+                    continue;
+                int32_t mask = 0, low = -1;
                 for(int q = 0; q < kBank; ++q)
                 {
                     const int32_t y = b0 + q;
-                    if(y - h.level_begin[l] < h.level_count[l] && parent[y] == parent[slot])
+                    if(y - lb < lc && parent[y] == parent[slot])
                     {
                         mask |= 1 << q;
                         if(low < 0)
@@ -1009,11 +1081,25 @@ class MasPreconditioner final : public LocalPreconditioner
                 }
                 sib[slot] = mask | ((slot - b0) == low ? 0x10000 : 0);
             }
+        }
         for(int l = 0; l <= L && l <= kMaxLvl; ++l)
         {
-            span.begin[l] = h.level_begin[l];
-            span.count[l] = h.level_count[l];
+            span.begin[l] = l == 0 ? 0 : h.level_begin[l];
+            span.count[l] = l == 0 ? h.N : h.level_count[l];
         }
+        // ancestors by partition slot (the apply reads them in slot order)
+        std::vector<int32_t> anc_slot((size_t)std::max(1, (L - 1) * h.N), 0);
+        for(int32_t slot = 0; slot < h.N; ++slot)
+            if(h.part_to_real[slot] >= 0)
+                for(int l = 1; l < L; ++l)
+                    anc_slot[(size_t)(l - 1) * h.N + slot] = anc[(size_t)(l - 1) * h.V + h.part_to_real[slot]];
+        std::vector<int32_t> unpart;
+        for(int32_t v = 0; v < h.V; ++v)
+            if(h.real_to_part[v] < 0)
+                unpart.push_back(v);
+        n_unpart = (int32_t)unpart.size();
+        upload(d_anc_slot, anc_slot);
+        upload(d_unpart, unpart);
         c_first = L > 1 ? h.level_begin[1] / kBank : C;
         upload(d_sib, sib);
         upload(d_part_to_real, h.part_to_real);
@@ -1074,19 +1160,38 @@ class MasPreconditioner final : public LocalPreconditioner
         MI_HIP(hipGetLastError());
     }
 
+    // levels at most this size are restricted together by one workgroup
+    static constexpr int32_t kTailLevel = 4096;
+
     template <typename T>
     void apply_t(const double* r, double* z, const int32_t* conv, hipStream_t s, const T* inv, T* R, T* Z)
     {
-        if(C0 > 0)
-            mas_apply_fine_kernel<T><<<(unsigned)C0, 64, 0, s>>>(r, d_part_to_real.data(), d_parent.data(), inv, R, Z,
-                                                                 conv);
-        if(h.L > 2)
-            mas_restrict_kernel<T><<<1, 1024, 0, s>>>(R, d_parent.data(), d_sib.data(), span, h.L, conv);
+        const int L = h.L;
+        for(int l = 0; l + 1 < L; ++l)
+        {
+            const int32_t cnt = span.count[l];
+            if(l > 0 && cnt <= kTailLevel)
+            {
+                mas_restrict_tail_kernel<T><<<1, 1024, 0, s>>>(R, d_parent.data(), d_sib.data(), span, l, L, conv);
+                break;
+            }
+            if(cnt == 0)
+                continue;
+            if(l == 0)
+                mas_restrict_level_kernel<T, true><<<ceil_div(cnt, 256), 256, 0, s>>>(
+                    r, d_part_to_real.data(), R, d_parent.data(), d_sib.data(), span.begin[l], cnt, conv);
+            else
+                mas_restrict_level_kernel<T, false><<<ceil_div(cnt, 256), 256, 0, s>>>(
+                    r, d_part_to_real.data(), R, d_parent.data(), d_sib.data(), span.begin[l], cnt, conv);
+        }
         if(C > c_first)
             mas_apply_coarse_kernel<T><<<(unsigned)(C - c_first), 64, 0, s>>>(inv, R, Z, c_first, conv);
-        int Lact = (active_levels > 0 && active_levels <= h.L) ? active_levels : h.L;
-        mas_collect_kernel<T><<<grid_for(h.V), 256, 0, s>>>(
-            r, z, d_real_to_part.data(), d_anc.data(), Lact, Z, d_Dun.data(), conv, h.V);
+        const int     Lact   = (active_levels > 0 && active_levels <= L) ? active_levels : L;
+        const int64_t blocks = C0 + ceil_div(n_unpart, kWave);
+        if(blocks > 0)
+            mas_apply_fine_kernel<T><<<(unsigned)blocks, 64, 0, s>>>(r, z, d_part_to_real.data(), inv, Z,
+                                                                     d_anc_slot.data(), h.N, Lact, C0,
+                                                                     d_unpart.data(), n_unpart, d_Dun.data(), conv);
     }
 
     bool mas_export(int32_t* info,
@@ -1122,15 +1227,20 @@ class MasPreconditioner final : public LocalPreconditioner
             MI_HIP(hipMemcpy(hess, d_hess.data(), sizeof(double) * C * kCm, hipMemcpyDeviceToHost));
         if(inv && C > 0)
         {
+            // device planes -> 9 scalars per upper block, block after block
+            std::vector<double> t((size_t)C * kCm);
             if(fp32)
             {
-                std::vector<float> t((size_t)C * kCm);
-                MI_HIP(hipMemcpy(t.data(), d_inv32.data(), sizeof(float) * t.size(), hipMemcpyDeviceToHost));
-                for(size_t q = 0; q < t.size(); ++q)
-                    inv[q] = t[q];
+                std::vector<float> f((size_t)C * kCm);
+                MI_HIP(hipMemcpy(f.data(), d_inv32.data(), sizeof(float) * f.size(), hipMemcpyDeviceToHost));
+                std::copy(f.begin(), f.end(), t.begin());
             }
             else
-                MI_HIP(hipMemcpy(inv, d_inv64.data(), sizeof(double) * C * kCm, hipMemcpyDeviceToHost));
+                MI_HIP(hipMemcpy(t.data(), d_inv64.data(), sizeof(double) * t.size(), hipMemcpyDeviceToHost));
+            for(int64_t k = 0; k < C; ++k)
+                for(int blk = 0; blk < kSym; ++blk)
+                    for(int q = 0; q < 9; ++q)
+                        inv[k * kCm + 9 * blk + q] = t[k * kCm + inv_at(blk, q)];
         }
         return true;
     }

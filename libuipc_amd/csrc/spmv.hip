@@ -18,6 +18,8 @@
 //   * a persistent grid; workgroups are renumbered so that each XCD (blocks are
 //     dispatched round-robin over the 8 XCDs) walks one contiguous eighth of
 //     the rows and keeps its part of x in its own L2.
+#include <cstdlib>
+
 #include "linsys.hpp"
 
 namespace mi
@@ -29,7 +31,7 @@ constexpr int kLanes   = 16;  // lanes per block row
 constexpr int kGroups  = kThreads / kLanes;
 constexpr int kXcds    = 8;
 
-template <typename VT, bool ACCUM>
+template <typename VT, bool ACCUM, int UNROLL>
 __global__ __launch_bounds__(kThreads) void spmv_full_kernel(int32_t        B,
                                                             const int32_t* __restrict__ fptr,
                                                             const int32_t* __restrict__ fcols,
@@ -59,17 +61,39 @@ __global__ __launch_bounds__(kThreads) void spmv_full_kernel(int32_t        B,
     {
         const int32_t s = fptr[row], e = fptr[row + 1];
         double        acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
-        for(int32_t k = s + lane; k < e; k += kLanes)
+        // UNROLL blocks per lane per trip: all column indices, value sets and
+        // x gathers are in flight before the first FMA.  Plane bases are
+        // uniform and offsets 32-bit (scalar base + vector offset addressing).
+        for(int32_t k = s + lane; k < e; k += UNROLL * kLanes)
         {
-            const int64_t j = fcols[k];
-            double        A[9];
+            uint32_t jj[UNROLL];
+            bool     ok[UNROLL];
 #pragma unroll
-            for(int q = 0; q < 9; ++q)
-                A[q] = (double)fvals[q * fstride + k];
-            const double x0 = x[3 * j], x1 = x[3 * j + 1], x2 = x[3 * j + 2];
-            acc0 += A[0] * x0 + A[3] * x1 + A[6] * x2;
-            acc1 += A[1] * x0 + A[4] * x1 + A[7] * x2;
-            acc2 += A[2] * x0 + A[5] * x1 + A[8] * x2;
+            for(int u = 0; u < UNROLL; ++u)
+            {
+                const int32_t ku = k + u * kLanes;
+                ok[u]            = ku < e;
+                jj[u]            = ok[u] ? (uint32_t)fcols[ku] : 0u;
+            }
+            VT A[UNROLL][9];
+#pragma unroll
+            for(int u = 0; u < UNROLL; ++u)
+#pragma unroll
+                for(int q = 0; q < 9; ++q)
+                    A[u][q] = ok[u] ? fvals[q * fstride + k + u * kLanes] : VT(0);
+            double xv[UNROLL][3];
+#pragma unroll
+            for(int u = 0; u < UNROLL; ++u)
+#pragma unroll
+                for(int c = 0; c < 3; ++c)
+                    xv[u][c] = x[3u * jj[u] + c];
+#pragma unroll
+            for(int u = 0; u < UNROLL; ++u)
+            {
+                acc0 += (double)A[u][0] * xv[u][0] + (double)A[u][3] * xv[u][1] + (double)A[u][6] * xv[u][2];
+                acc1 += (double)A[u][1] * xv[u][0] + (double)A[u][4] * xv[u][1] + (double)A[u][7] * xv[u][2];
+                acc2 += (double)A[u][2] * xv[u][0] + (double)A[u][5] * xv[u][1] + (double)A[u][8] * xv[u][2];
+            }
         }
 #pragma unroll
         for(int o = kLanes / 2; o > 0; o >>= 1)
@@ -121,13 +145,59 @@ __global__ __launch_bounds__(kThreads) void sum_partials_kernel(const double* pa
     if(threadIdx.x == 0)
         *out = acc;
 }
+
+// unroll factor of the row loop (MI_SPMV_UNROLL = 1, 2 or 4 for launch sweeps)
+int spmv_unroll()
+{
+    static int u = [] {
+        int v = 2;
+        if(const char* e = std::getenv("MI_SPMV_UNROLL"))
+            v = std::atoi(e);
+        return (v == 1 || v == 2 || v == 4) ? v : 2;
+    }();
+    return u;
+}
+
+template <typename VT, bool ACCUM>
+void launch_spmv(int g, hipStream_t s, const BsrMatrix& A, const VT* vals, const double* x, double* y, double a,
+                 double* partials, const int32_t* d_conv)
+{
+    switch(spmv_unroll())
+    {
+        case 1:
+            spmv_full_kernel<VT, ACCUM, 1><<<g, kThreads, 0, s>>>(A.B, A.full_ptr.data(), A.fcols.data(), vals,
+                                                                  A.fstride, x, y, a, partials, d_conv);
+            break;
+        case 4:
+            spmv_full_kernel<VT, ACCUM, 4><<<g, kThreads, 0, s>>>(A.B, A.full_ptr.data(), A.fcols.data(), vals,
+                                                                  A.fstride, x, y, a, partials, d_conv);
+            break;
+        default:
+            spmv_full_kernel<VT, ACCUM, 2><<<g, kThreads, 0, s>>>(A.B, A.full_ptr.data(), A.fcols.data(), vals,
+                                                                  A.fstride, x, y, a, partials, d_conv);
+    }
+}
 }  // namespace
 
 void spmv_prepare(SpmvScratch& scr)
 {
     if(scr.grid == 0)
     {
-        scr.grid = device_cu_count() * 4;
+        // a persistent grid of as many 4-wave workgroups per CU as fit at once
+        // (the kernel's occupancy); MI_SPMV_BLOCKS_PER_CU overrides for sweeps
+        int per_cu = 0;
+        if(const char* e = std::getenv("MI_SPMV_BLOCKS_PER_CU"))
+            per_cu = std::max(1, std::min(32, std::atoi(e)));
+        if(per_cu == 0)
+        {
+            const int u  = spmv_unroll();
+            auto      fn = u == 1   ? (const void*)spmv_full_kernel<double, false, 1>
+                           : u == 4 ? (const void*)spmv_full_kernel<double, false, 4>
+                                    : (const void*)spmv_full_kernel<double, false, 2>;
+            MI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kThreads, 0));
+            per_cu = std::max(1, per_cu);
+        }
+        scr.grid = device_cu_count() * per_cu;
         scr.grid = (scr.grid / kXcds) * kXcds;
         if(scr.grid < kXcds)
             scr.grid = kXcds;
@@ -157,12 +227,10 @@ int spmv_full(const BsrMatrix& A, bool use32, const double* x, double* y, double
     if(use32)
     {
         MI_REQUIRE(A.has32, MI_ERR_STATE, "spmv: FP32 values were not assembled");
-        spmv_full_kernel<float, false><<<g, kThreads, 0, s>>>(A.B, A.full_ptr.data(), A.fcols.data(),
-                                                              A.fvals32.data(), A.fstride, x, y, 1.0, partials, d_conv);
+        launch_spmv<float, false>(g, s, A, A.fvals32.data(), x, y, 1.0, partials, d_conv);
     }
     else
-        spmv_full_kernel<double, false><<<g, kThreads, 0, s>>>(A.B, A.full_ptr.data(), A.fcols.data(),
-                                                               A.fvals.data(), A.fstride, x, y, 1.0, partials, d_conv);
+        launch_spmv<double, false>(g, s, A, A.fvals.data(), x, y, 1.0, partials, d_conv);
     MI_HIP(hipGetLastError());
     return g;
 }
@@ -182,12 +250,10 @@ void spmv_accumulate_dot(const BsrMatrix& A, bool use32, double a, const double*
     if(use32)
     {
         MI_REQUIRE(A.has32, MI_ERR_STATE, "spmv: FP32 values were not assembled");
-        spmv_full_kernel<float, true><<<g, kThreads, 0, s>>>(A.B, A.full_ptr.data(), A.fcols.data(),
-                                                             A.fvals32.data(), A.fstride, x, y, a, p, nullptr);
+        launch_spmv<float, true>(g, s, A, A.fvals32.data(), x, y, a, p, nullptr);
     }
     else
-        spmv_full_kernel<double, true><<<g, kThreads, 0, s>>>(A.B, A.full_ptr.data(), A.fcols.data(),
-                                                              A.fvals.data(), A.fstride, x, y, a, p, nullptr);
+        launch_spmv<double, true>(g, s, A, A.fvals.data(), x, y, a, p, nullptr);
     if(d_dot)
         sum_partials_kernel<<<1, kThreads, 0, s>>>(scr.partials.data(), g, d_dot);
     MI_HIP(hipGetLastError());

@@ -6,6 +6,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import os
 
 import numpy as np
 
@@ -68,6 +69,10 @@ class Context:
             out = C.c_double()
             check(self.lib.mi_norm_f64(self.h, x.ptr, x.size, C.byref(out)))
         return out.value
+
+    # utils/matrix_market.h:120-158 (export_vector_market)
+    def write_vector_mtx(self, path: str, x: "DeviceArray") -> None:
+        check(self.lib.mi_vec_write_mtx(self.h, os.fsencode(path), x.ptr, x.size))
 
 
 class DeviceArray:
@@ -150,6 +155,10 @@ class LinearSystem:
         self.ctx.synchronize()
         return r.numpy()[:U], c.numpy()[:U], v.numpy()[:U], p.numpy()
 
+    # extras/debug/dump_linear_system (global_linear_system.cu:42-70)
+    def write_mtx(self, path: str, one_based: bool = True) -> None:
+        check(self.lib.mi_sys_write_mtx(self.h, os.fsencode(path), int(one_based)))
+
     # Spmv::rbk_sym_spmv_dot
     def spmv_dot(self, x: DeviceArray, y: DeviceArray, accumulate: bool = False) -> float:
         """y = A x (y += A x when accumulate); returns x^T A x"""
@@ -225,9 +234,35 @@ class LinearSystem:
         return st.as_dict()
 
 
+def read_mtx_block_triplets(path: str, one_based: bool = True):
+    """A coordinate Matrix Market file as 3x3 block triplets (duplicates summed,
+    ascending by (block row, block col)): (block_rows, rows, cols, vals[T, 9]
+    column-major)."""
+    lib = abi.linsys()
+    B, T = C.c_int32(), C.c_int64()
+    fp = os.fsencode(path)
+    check(lib.mi_mtx_read_block_triplets(fp, int(one_based), C.byref(B), C.byref(T), None, None, None))
+    rows = np.empty(T.value, np.int32)
+    cols = np.empty(T.value, np.int32)
+    vals = np.empty((T.value, 9), np.float64)
+    check(lib.mi_mtx_read_block_triplets(fp, int(one_based), C.byref(B), C.byref(T), _i32p(rows), _i32p(cols),
+                                         vals.ctypes.data_as(abi.c_f64p)))
+    return B.value, rows, cols, vals
+
+
+def read_vector_mtx(path: str) -> np.ndarray:
+    lib = abi.linsys()
+    n = C.c_int64()
+    fp = os.fsencode(path)
+    check(lib.mi_vec_read_mtx(fp, C.byref(n), None))
+    x = np.empty(n.value, np.float64)
+    check(lib.mi_vec_read_mtx(fp, C.byref(n), x.ctypes.data_as(abi.c_f64p)))
+    return x
+
+
 def rel_err(a: np.ndarray, b: np.ndarray) -> float:
     nb = float(np.linalg.norm(b))
     return float(np.linalg.norm(a - b)) / (nb if nb > 0 else 1.0)
 
 
-__all__ = ["Context", "DeviceArray", "LinearSystem", "rel_err", "math"]
+__all__ = ["Context", "DeviceArray", "LinearSystem", "read_mtx_block_triplets", "read_vector_mtx", "rel_err"]
