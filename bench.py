@@ -59,7 +59,7 @@ def calibrate_copy(mb: int) -> None:
     from libuipc_amd.hip_events import EventTimer
 
     ctx = Context(0)
-    nbytes = mb << 20
+    nbytes = mb << 20  # one hipMemcpyAsync + stream sync per repetition
     a = ctx.alloc(nbytes // 8, np.float64)
     b = ctx.alloc(nbytes // 8, np.float64)
     a.zero()

@@ -199,6 +199,16 @@ int mi_precond_mas_export(mi_sys*  sys,
                           double*  cluster_hess_host,
                           double*  cluster_inv_host);
 
+/* The debug dump of MASPreconditionerEngine::dump_cluster_matrices_debug
+ * (finite_element/mas_preconditioner_engine.cu:3154-3301) into output_dir:
+ * mas_cluster_hess.f<frame>.n<newton_iter>.mtx and mas_cluster_inv.(...).mtx
+ * (coordinate real general, block diagonal of all clusters, 136 upper 3x3
+ * blocks each, %.17g) and mas_cluster_meta.(...).json (sizes, levels,
+ * part_to_real, going_next, real_to_part).  The reference writes its
+ * allocation's 5% slack clusters too; this writes the clusters in use.
+ * Synchronises. */
+int mi_precond_mas_dump(mi_sys* sys, int handle, const char* output_dir, int64_t frame, int64_t newton_iter);
+
 /* ------------------------------------------------------------ Matrix Market */
 /* The reference's debug dumps (extras/debug/dump_linear_system,
  * linear_system/global_linear_system.cu:42-70; format utils/matrix_market.h:26-158).

@@ -691,9 +691,11 @@ void MatrixConverter::convert_sym(int32_t        B,
     unpack_lower_kernel<<<grid_of(U), kThreads, 0, s>>>(d_off, keys_[tcur].data(), B, lower_rows_.data());
     row_ptr_kernel<<<grid_of(U + 1), kThreads, 0, s>>>(d_off, U, lower_rows_.data(), lower_ptr_.data(), B);
     full_ptr_kernel<<<grid_of(B + 1), kThreads, 0, s>>>(lower_ptr_.data(), out.row_ptr.data(), out.full_ptr.data(), B);
-    const int64_t fcap = 2 * U;
-    out.fcols.resize((size_t)std::max<int64_t>(1, fcap), ratio);
-    out.fstride = (int64_t)out.fcols.cap;
+    // room for one block past the last row: the FP32 SpMV reads blocks in
+    // aligned pairs; an even plane stride keeps every pair 8-byte aligned
+    const int64_t fcap = 2 * U + 2;
+    out.fcols.resize((size_t)fcap, ratio);
+    out.fstride = ((int64_t)out.fcols.cap + 1) & ~(int64_t)1;
     if(want32)
     {
         out.fvals32.resize((size_t)9 * out.fstride);

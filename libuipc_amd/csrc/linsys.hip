@@ -537,6 +537,92 @@ int mi_precond_mas_export(mi_sys*  sys,
     MI_API_END
 }
 
+namespace
+{
+void write_int_array(FILE* fp, const char* key, const int32_t* a, int64_t n, bool last)
+{
+    std::fprintf(fp, "    \"%s\": [", key);
+    for(int64_t i = 0; i < n; ++i)
+        std::fprintf(fp, i ? ", %d" : "%d", a[i]);
+    std::fprintf(fp, "]%s\n", last ? "" : ",");
+}
+}  // namespace
+
+// The files of MASPreconditionerEngine::dump_cluster_matrices_debug
+// (finite_element/mas_preconditioner_engine.cu:3154-3301):
+//   mas_cluster_hess.f<frame>.n<iter>.mtx, mas_cluster_inv.f<frame>.n<iter>.mtx
+//     coordinate real general, (clusters * 48)^2 block diagonal, per cluster the
+//     136 upper 3x3 blocks row by row within the block, %.17g, 1-based;
+//   mas_cluster_meta.f<frame>.n<iter>.json  sizes, levels, part_to_real,
+//     going_next, real_to_part.
+int mi_precond_mas_dump(mi_sys* sys, int handle, const char* output_dir, int64_t frame, int64_t newton_iter)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(sys && output_dir && handle >= 0 && handle < (int)sys->pcs.size(), MI_ERR_ARG,
+               "mi_precond_mas_dump: bad argument");
+    auto&   pc = *sys->pcs[handle];
+    int32_t info[6];
+    MI_REQUIRE(pc.mas_export(info, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr), MI_ERR_ARG,
+               "mi_precond_mas_dump: not a MAS preconditioner");
+    const int32_t V = info[0], N = info[1], L = info[2], total = info[3];
+    const int64_t C = info[5];
+    constexpr int kSymBlocks = 136, kDim = 48;
+    std::vector<int32_t> p2r(std::max(N, 1)), r2p(std::max(V, 1)), ls(2 * (L + 1));
+    std::vector<int32_t> gn((size_t)std::max(1, ((std::max(N, V) + 15) / 16 * 16) * L));
+    std::vector<double>  hess((size_t)std::max<int64_t>(1, C * kSymBlocks * 9)), inv(hess.size());
+    pc.mas_export(info, p2r.data(), r2p.data(), gn.data(), ls.data(), nullptr, hess.data(), inv.data());
+    const std::string dir = output_dir;
+    const std::string tag = ".f" + std::to_string(frame) + ".n" + std::to_string(newton_iter);
+    for(int which = 0; which < 2; ++which)
+    {
+        const char*       kind = which == 0 ? "hess" : "inv";
+        const double*     M    = which == 0 ? hess.data() : inv.data();
+        const std::string path = dir + "/mas_cluster_" + kind + tag + ".mtx";
+        FILE*             fp   = std::fopen(path.c_str(), "w");
+        MI_REQUIRE(fp, MI_ERR_ARG, "mi_precond_mas_dump: cannot open " + path);
+        std::fprintf(fp, "%%%%MatrixMarket matrix coordinate real general\n");
+        std::fprintf(fp, "%% MAS cluster %s block-upper-triangle (%lld clusters, banksize=16)\n", kind, (long long)C);
+        std::fprintf(fp, "%lld %lld %lld\n", (long long)C * kDim, (long long)C * kDim, (long long)C * kSymBlocks * 9);
+        for(int64_t c = 0; c < C; ++c)
+        {
+            const long long base = c * kDim;
+            for(int br = 0; br < 16; ++br)
+                for(int bc = br; bc < 16; ++bc)
+                {
+                    const int     k   = 16 * br - br * (br + 1) / 2 + bc;
+                    const double* blk = M + (c * kSymBlocks + k) * 9;
+                    for(int i = 0; i < 3; ++i)
+                        for(int j = 0; j < 3; ++j)
+                            std::fprintf(fp, "%lld %lld %.17g\n", base + br * 3 + i + 1, base + bc * 3 + j + 1,
+                                         blk[i + 3 * j]);
+                }
+        }
+        const bool ok = std::ferror(fp) == 0;
+        std::fclose(fp);
+        MI_REQUIRE(ok, MI_ERR_ARG, "mi_precond_mas_dump: write failed: " + path);
+    }
+    const std::string path = dir + "/mas_cluster_meta" + tag + ".json";
+    FILE*             fp   = std::fopen(path.c_str(), "w");
+    MI_REQUIRE(fp, MI_ERR_ARG, "mi_precond_mas_dump: cannot open " + path);
+    std::fprintf(fp, "{\n    \"banksize\": 16,\n    \"frame\": %lld,\n", (long long)frame);
+    write_int_array(fp, "going_next", gn.data(), total, false);
+    std::fprintf(fp, "    \"level_num\": %d,\n    \"levels\": [", L);
+    for(int l = 0; l <= L; ++l)
+        std::fprintf(fp, "%s{\"count\": %d, \"level\": %d, \"offset\": %d}", l ? ", " : "", ls[2 * l], l,
+                     ls[2 * l + 1]);
+    std::fprintf(fp, "],\n    \"newton_iter\": %lld,\n    \"num_cluster_blocks\": %lld,\n", (long long)newton_iter,
+                 (long long)C);
+    write_int_array(fp, "part_to_real", p2r.data(), N, false);
+    write_int_array(fp, "real_to_part", r2p.data(), V, false);
+    std::fprintf(fp, "    \"sym_block_count\": 136,\n    \"total_clusters\": %d,\n    \"total_map_nodes\": %d,\n"
+                     "    \"total_nodes\": %d\n}\n",
+                 total, N, V);
+    const bool ok = std::ferror(fp) == 0;
+    std::fclose(fp);
+    MI_REQUIRE(ok, MI_ERR_ARG, "mi_precond_mas_dump: write failed: " + path);
+    MI_API_END
+}
+
 int mi_pcg_default_opts(mi_pcg_opts* o)
 {
     MI_API_BEGIN
@@ -627,7 +713,7 @@ extern "C" int mi_pcg_solve(mi_sys* sys, const double* b, double* x, const mi_pc
     }
     // scratch first: the partial buffers' addresses are taken below and baked
     // into the captured graph; no allocation may happen inside a captured block
-    spmv_prepare(sys->spmv_scr);
+    spmv_prepare(sys->spmv_scr, sys->desc.value_precision == MI_FP32);
     pcg_prepare(sys->dot_scr);
     PcgScalars& h = *sys->h_sc;
 

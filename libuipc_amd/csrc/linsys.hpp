@@ -75,10 +75,13 @@ class MatrixConverter
 struct SpmvScratch
 {
     DBuf<double> partials;
-    int          grid = 0;
+    int          grid   = 0;
+    int          unroll = 1;
+    bool         use32  = false;
 };
-// allocates the SpMV's reduction scratch (must run outside graph capture)
-void spmv_prepare(SpmvScratch& scr);
+// allocates the SpMV's reduction scratch and fixes its launch shape for one
+// value precision (must run outside graph capture)
+void spmv_prepare(SpmvScratch& scr, bool use32);
 // y = A x (overwritten); partials[g] = per-workgroup x^T A x; returns the grid g
 int spmv_full(const BsrMatrix& A, bool use32, const double* x, double* y, double* partials, const int32_t* d_conv,
               SpmvScratch& scr, hipStream_t s);

@@ -762,7 +762,7 @@ __device__ __forceinline__ void cluster_load(const T* __restrict__ M, int lane, 
         {
 #pragma unroll
             for(int q = 0; q < 9; ++q)
-                cb.m[t][q] = M[inv_at(blk, q)];
+                cb.m[t][q] = __builtin_nontemporal_load(M + inv_at(blk, q));
         }
     }
 }
@@ -813,11 +813,67 @@ struct LevelSpan
     int32_t count[kMaxLvl + 1];
 };
 
-// Restriction R_{l+1} = sum of children, for one level: the lowest node of
-// every group of in-bank siblings (sib = bank mask | 0x10000 on the lowest)
-// adds the group in ascending slot order and stores the parent's sum (a
-// parent's children all live in one bank: no atomics, deterministic).  At
-// level 0 the values are r gathered through part_to_real.
+// Restriction R_{l+1} = sum of children.  Every lane loads its own node's
+// value (coalesced); the lowest node of each in-bank sibling group (sib = bank
+// mask | 0x10000 on the lowest) adds the group's values, taken from the 16
+// lanes of its bank by shuffles in ascending slot order, and stores the
+// parent's sum: a parent's children all live in one bank, so there are no
+// atomics and the sum is deterministic.  At level 0 the values are r gathered
+// through part_to_real.  Level starts are bank-aligned, so a bank never
+// straddles two waves.
+template <typename T, bool kLevel0>
+__device__ __forceinline__ void restrict_bank(const double* __restrict__ r,
+                                              const int32_t* __restrict__ part_to_real,
+                                              T*             Rb,
+                                              const int32_t* __restrict__ parent,
+                                              const int32_t* __restrict__ sib,
+                                              int32_t        slot,
+                                              bool           live)
+{
+    T       v0 = 0, v1 = 0, v2 = 0;
+    int32_t m  = 0;
+    if(live)
+    {
+        m = sib[slot];
+        if constexpr(kLevel0)
+        {
+            const int32_t v = part_to_real[slot];
+            if(v >= 0)
+            {
+                v0 = (T)r[3 * (int64_t)v];
+                v1 = (T)r[3 * (int64_t)v + 1];
+                v2 = (T)r[3 * (int64_t)v + 2];
+            }
+        }
+        else
+        {
+            v0 = Rb[3 * (int64_t)slot];
+            v1 = Rb[3 * (int64_t)slot + 1];
+            v2 = Rb[3 * (int64_t)slot + 2];
+        }
+    }
+    const int b0 = (int)(threadIdx.x % kWave) & ~(kBank - 1);
+    T         s0 = 0, s1 = 0, s2 = 0;
+#pragma unroll
+    for(int q = 0; q < kBank; ++q)
+    {
+        const T w0 = __shfl(v0, b0 + q, kWave), w1 = __shfl(v1, b0 + q, kWave), w2 = __shfl(v2, b0 + q, kWave);
+        if(m & (1 << q))
+        {
+            s0 += w0;
+            s1 += w1;
+            s2 += w2;
+        }
+    }
+    if(m & 0x10000)
+    {
+        const int64_t pp = 3 * (int64_t)parent[slot];
+        Rb[pp]           = s0;
+        Rb[pp + 1]       = s1;
+        Rb[pp + 2]       = s2;
+    }
+}
+
 template <typename T, bool kLevel0>
 __global__ __launch_bounds__(256) void mas_restrict_level_kernel(const double* __restrict__ r,
                                                                 const int32_t* __restrict__ part_to_real,
@@ -831,36 +887,7 @@ __global__ __launch_bounds__(256) void mas_restrict_level_kernel(const double* _
     if(conv && *conv != 0)
         return;
     const int32_t x = blockIdx.x * blockDim.x + threadIdx.x;
-    if(x >= cnt)
-        return;
-    const int32_t slot = begin + x;
-    const int32_t m    = sib[slot];
-    if(!(m & 0x10000))
-        return;
-    const int32_t b0 = slot & ~(kBank - 1);
-    T             s0 = 0, s1 = 0, s2 = 0;
-    for(int q = 0; q < kBank; ++q)
-        if(m & (1 << q))
-        {
-            if constexpr(kLevel0)
-            {
-                const int64_t y = 3 * (int64_t)part_to_real[b0 + q];
-                s0 += (T)r[y];
-                s1 += (T)r[y + 1];
-                s2 += (T)r[y + 2];
-            }
-            else
-            {
-                const int64_t y = 3 * (int64_t)(b0 + q);
-                s0 += Rb[y];
-                s1 += Rb[y + 1];
-                s2 += Rb[y + 2];
-            }
-        }
-    const int64_t pp = 3 * (int64_t)parent[slot];
-    Rb[pp]           = s0;
-    Rb[pp + 1]       = s1;
-    Rb[pp + 2]       = s2;
+    restrict_bank<T, kLevel0>(r, part_to_real, Rb, parent, sib, begin + x, x < cnt);
 }
 
 // the small top levels (from level lo up) in one workgroup, level by level
@@ -878,26 +905,10 @@ __global__ __launch_bounds__(1024) void mas_restrict_tail_kernel(T*             
     for(int l = lo; l + 1 < L; ++l)
     {
         const int32_t begin = lv.begin[l], cnt = lv.count[l];
-        for(int32_t x = threadIdx.x; x < cnt; x += blockDim.x)
+        for(int32_t x0 = 0; x0 < cnt; x0 += blockDim.x)
         {
-            const int32_t slot = begin + x;
-            const int32_t m    = sib[slot];
-            if(!(m & 0x10000))
-                continue;
-            const int32_t b0 = slot & ~(kBank - 1);
-            T             s0 = 0, s1 = 0, s2 = 0;
-            for(int q = 0; q < kBank; ++q)
-                if(m & (1 << q))
-                {
-                    const int64_t y = 3 * (int64_t)(b0 + q);
-                    s0 += Rb[y];
-                    s1 += Rb[y + 1];
-                    s2 += Rb[y + 2];
-                }
-            const int64_t pp = 3 * (int64_t)parent[slot];
-            Rb[pp]           = s0;
-            Rb[pp + 1]       = s1;
-            Rb[pp + 2]       = s2;
+            const int32_t x = x0 + threadIdx.x;
+            restrict_bank<T, false>(nullptr, nullptr, Rb, parent, sib, begin + x, x < cnt);
         }
         __syncthreads();
     }

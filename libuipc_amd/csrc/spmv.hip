@@ -31,6 +31,9 @@ constexpr int kLanes   = 16;  // lanes per block row
 constexpr int kGroups  = kThreads / kLanes;
 constexpr int kXcds    = 8;
 
+typedef int   v2i __attribute__((ext_vector_type(2)));
+typedef float v2f __attribute__((ext_vector_type(2)));
+
 template <typename VT, bool ACCUM, int UNROLL>
 __global__ __launch_bounds__(kThreads) void spmv_full_kernel(int32_t        B,
                                                             const int32_t* __restrict__ fptr,
@@ -64,35 +67,91 @@ __global__ __launch_bounds__(kThreads) void spmv_full_kernel(int32_t        B,
         // UNROLL blocks per lane per trip: all column indices, value sets and
         // x gathers are in flight before the first FMA.  Plane bases are
         // uniform and offsets 32-bit (scalar base + vector offset addressing).
-        for(int32_t k = s + lane; k < e; k += UNROLL * kLanes)
+        if constexpr(sizeof(VT) == 8)
         {
-            uint32_t jj[UNROLL];
-            bool     ok[UNROLL];
-#pragma unroll
-            for(int u = 0; u < UNROLL; ++u)
+            for(int32_t k = s + lane; k < e; k += UNROLL * kLanes)
             {
-                const int32_t ku = k + u * kLanes;
-                ok[u]            = ku < e;
-                jj[u]            = ok[u] ? (uint32_t)fcols[ku] : 0u;
+                uint32_t jj[UNROLL];
+                bool     ok[UNROLL];
+#pragma unroll
+                for(int u = 0; u < UNROLL; ++u)
+                {
+                    const int32_t ku = k + u * kLanes;
+                    ok[u]            = ku < e;
+                    jj[u]            = ok[u] ? (uint32_t)__builtin_nontemporal_load(fcols + ku) : 0u;
+                }
+                VT A[UNROLL][9];
+#pragma unroll
+                for(int u = 0; u < UNROLL; ++u)
+#pragma unroll
+                    for(int q = 0; q < 9; ++q)
+                        A[u][q] = ok[u] ? __builtin_nontemporal_load(fvals + q * fstride + k + u * kLanes) : VT(0);
+                double xv[UNROLL][3];
+#pragma unroll
+                for(int u = 0; u < UNROLL; ++u)
+#pragma unroll
+                    for(int c = 0; c < 3; ++c)
+                        xv[u][c] = x[3u * jj[u] + c];
+#pragma unroll
+                for(int u = 0; u < UNROLL; ++u)
+                {
+                    acc0 += (double)A[u][0] * xv[u][0] + (double)A[u][3] * xv[u][1] + (double)A[u][6] * xv[u][2];
+                    acc1 += (double)A[u][1] * xv[u][0] + (double)A[u][4] * xv[u][1] + (double)A[u][7] * xv[u][2];
+                    acc2 += (double)A[u][2] * xv[u][0] + (double)A[u][5] * xv[u][1] + (double)A[u][8] * xv[u][2];
+                }
             }
-            VT A[UNROLL][9];
-#pragma unroll
-            for(int u = 0; u < UNROLL; ++u)
-#pragma unroll
-                for(int q = 0; q < 9; ++q)
-                    A[u][q] = ok[u] ? fvals[q * fstride + k + u * kLanes] : VT(0);
-            double xv[UNROLL][3];
-#pragma unroll
-            for(int u = 0; u < UNROLL; ++u)
-#pragma unroll
-                for(int c = 0; c < 3; ++c)
-                    xv[u][c] = x[3u * jj[u] + c];
-#pragma unroll
-            for(int u = 0; u < UNROLL; ++u)
+        }
+        else
+        {
+            // FP32 planes: each lane takes two neighbouring blocks with 8-byte
+            // loads, so 16 lanes read 128 contiguous bytes per plane.  The pair
+            // grid starts at the even block at or before the row start; the
+            // halves outside [s, e) are masked.  The planes are 8-byte aligned
+            // (the plane stride is even).
+            const int32_t s2 = s & ~1;
+            for(int32_t k = s2 + 2 * lane; k < e; k += UNROLL * 2 * kLanes)
             {
-                acc0 += (double)A[u][0] * xv[u][0] + (double)A[u][3] * xv[u][1] + (double)A[u][6] * xv[u][2];
-                acc1 += (double)A[u][1] * xv[u][0] + (double)A[u][4] * xv[u][1] + (double)A[u][7] * xv[u][2];
-                acc2 += (double)A[u][2] * xv[u][0] + (double)A[u][5] * xv[u][1] + (double)A[u][8] * xv[u][2];
+                uint32_t jj[UNROLL][2];
+                bool     ok[UNROLL][2];
+#pragma unroll
+                for(int u = 0; u < UNROLL; ++u)
+                {
+                    const int32_t ku = k + u * 2 * kLanes;
+                    ok[u][0]         = ku >= s && ku < e;
+                    ok[u][1]         = ku + 1 < e;
+                    const bool any   = ku < e;
+                    const v2i  cc    = any ? __builtin_nontemporal_load(reinterpret_cast<const v2i*>(fcols + ku))
+                                           : v2i{0, 0};
+                    jj[u][0]         = ok[u][0] ? (uint32_t)cc.x : 0u;
+                    jj[u][1]         = ok[u][1] ? (uint32_t)cc.y : 0u;
+                }
+                v2f A[UNROLL][9];
+#pragma unroll
+                for(int u = 0; u < UNROLL; ++u)
+                {
+                    const int32_t ku  = k + u * 2 * kLanes;
+                    const bool    any = ku < e;
+#pragma unroll
+                    for(int q = 0; q < 9; ++q)
+                        A[u][q] = any ? __builtin_nontemporal_load(reinterpret_cast<const v2f*>(fvals + q * fstride + ku))
+                                      : v2f{0.f, 0.f};
+                }
+#pragma unroll
+                for(int u = 0; u < UNROLL; ++u)
+#pragma unroll
+                    for(int h = 0; h < 2; ++h)
+                    {
+                        if(!ok[u][h])
+                            continue;
+                        const double x0 = x[3u * jj[u][h]], x1 = x[3u * jj[u][h] + 1], x2 = x[3u * jj[u][h] + 2];
+                        float        a[9];
+#pragma unroll
+                        for(int q = 0; q < 9; ++q)
+                            a[q] = h ? A[u][q].y : A[u][q].x;
+                        acc0 += (double)a[0] * x0 + (double)a[3] * x1 + (double)a[6] * x2;
+                        acc1 += (double)a[1] * x0 + (double)a[4] * x1 + (double)a[7] * x2;
+                        acc2 += (double)a[2] * x0 + (double)a[5] * x1 + (double)a[8] * x2;
+                    }
             }
         }
 #pragma unroll
@@ -146,23 +205,33 @@ __global__ __launch_bounds__(kThreads) void sum_partials_kernel(const double* pa
         *out = acc;
 }
 
-// unroll factor of the row loop (MI_SPMV_UNROLL = 1, 2 or 4 for launch sweeps)
-int spmv_unroll()
+// blocks per lane per trip of the row loop: 4 for FP64 planes, 1 (pairs) for
+// FP32; MI_SPMV_UNROLL = 1, 2 or 4 overrides for launch sweeps
+int spmv_unroll(bool use32)
 {
-    static int u = [] {
-        int v = 2;
-        if(const char* e = std::getenv("MI_SPMV_UNROLL"))
-            v = std::atoi(e);
-        return (v == 1 || v == 2 || v == 4) ? v : 2;
-    }();
-    return u;
+    int v = use32 ? 1 : 4;
+    if(const char* e = std::getenv("MI_SPMV_UNROLL"))
+    {
+        const int w = std::atoi(e);
+        if(w == 1 || w == 2 || w == 4)
+            v = w;
+    }
+    return v;
 }
 
 template <typename VT, bool ACCUM>
-void launch_spmv(int g, hipStream_t s, const BsrMatrix& A, const VT* vals, const double* x, double* y, double a,
-                 double* partials, const int32_t* d_conv)
+const void* spmv_kernel_of(int unroll)
 {
-    switch(spmv_unroll())
+    return unroll == 1   ? (const void*)spmv_full_kernel<VT, ACCUM, 1>
+           : unroll == 4 ? (const void*)spmv_full_kernel<VT, ACCUM, 4>
+                         : (const void*)spmv_full_kernel<VT, ACCUM, 2>;
+}
+
+template <typename VT, bool ACCUM>
+void launch_spmv(const SpmvScratch& scr, int g, hipStream_t s, const BsrMatrix& A, const VT* vals, const double* x,
+                 double* y, double a, double* partials, const int32_t* d_conv)
+{
+    switch(scr.unroll)
     {
         case 1:
             spmv_full_kernel<VT, ACCUM, 1><<<g, kThreads, 0, s>>>(A.B, A.full_ptr.data(), A.fcols.data(), vals,
@@ -179,10 +248,12 @@ void launch_spmv(int g, hipStream_t s, const BsrMatrix& A, const VT* vals, const
 }
 }  // namespace
 
-void spmv_prepare(SpmvScratch& scr)
+void spmv_prepare(SpmvScratch& scr, bool use32)
 {
-    if(scr.grid == 0)
+    if(scr.grid == 0 || scr.use32 != use32)
     {
+        scr.use32  = use32;
+        scr.unroll = spmv_unroll(use32);
         // a persistent grid of as many 4-wave workgroups per CU as fit at once
         // (the kernel's occupancy); MI_SPMV_BLOCKS_PER_CU overrides for sweeps
         int per_cu = 0;
@@ -190,18 +261,14 @@ void spmv_prepare(SpmvScratch& scr)
             per_cu = std::max(1, std::min(32, std::atoi(e)));
         if(per_cu == 0)
         {
-            const int u  = spmv_unroll();
-            auto      fn = u == 1   ? (const void*)spmv_full_kernel<double, false, 1>
-                           : u == 4 ? (const void*)spmv_full_kernel<double, false, 4>
-                                    : (const void*)spmv_full_kernel<double, false, 2>;
+            const void* fn = use32 ? spmv_kernel_of<float, false>(scr.unroll) : spmv_kernel_of<double, false>(scr.unroll);
             MI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kThreads, 0));
             per_cu = std::max(1, per_cu);
         }
-        scr.grid = device_cu_count() * per_cu;
-        scr.grid = (scr.grid / kXcds) * kXcds;
-        if(scr.grid < kXcds)
-            scr.grid = kXcds;
-        scr.partials.resize((size_t)2 * scr.grid);
+        int grid = device_cu_count() * per_cu;
+        grid     = std::max(kXcds, (grid / kXcds) * kXcds);
+        scr.partials.resize((size_t)2 * grid);
+        scr.grid = grid;
     }
 }
 
@@ -217,7 +284,7 @@ int spmv_grid(const SpmvScratch& scr, int32_t B)
 int spmv_full(const BsrMatrix& A, bool use32, const double* x, double* y, double* partials, const int32_t* d_conv,
               SpmvScratch& scr, hipStream_t s)
 {
-    spmv_prepare(scr);
+    spmv_prepare(scr, use32);
     const int g = spmv_grid(scr, A.B);
     if(A.B == 0)
         return 0;
@@ -227,10 +294,10 @@ int spmv_full(const BsrMatrix& A, bool use32, const double* x, double* y, double
     if(use32)
     {
         MI_REQUIRE(A.has32, MI_ERR_STATE, "spmv: FP32 values were not assembled");
-        launch_spmv<float, false>(g, s, A, A.fvals32.data(), x, y, 1.0, partials, d_conv);
+        launch_spmv<float, false>(scr, g, s, A, A.fvals32.data(), x, y, 1.0, partials, d_conv);
     }
     else
-        launch_spmv<double, false>(g, s, A, A.fvals.data(), x, y, 1.0, partials, d_conv);
+        launch_spmv<double, false>(scr, g, s, A, A.fvals.data(), x, y, 1.0, partials, d_conv);
     MI_HIP(hipGetLastError());
     return g;
 }
@@ -238,7 +305,7 @@ int spmv_full(const BsrMatrix& A, bool use32, const double* x, double* y, double
 void spmv_accumulate_dot(const BsrMatrix& A, bool use32, double a, const double* x, double* y, double* d_dot,
                          SpmvScratch& scr, hipStream_t s)
 {
-    spmv_prepare(scr);
+    spmv_prepare(scr, use32);
     if(A.B == 0)
     {
         if(d_dot)
@@ -250,10 +317,10 @@ void spmv_accumulate_dot(const BsrMatrix& A, bool use32, double a, const double*
     if(use32)
     {
         MI_REQUIRE(A.has32, MI_ERR_STATE, "spmv: FP32 values were not assembled");
-        launch_spmv<float, true>(g, s, A, A.fvals32.data(), x, y, a, p, nullptr);
+        launch_spmv<float, true>(scr, g, s, A, A.fvals32.data(), x, y, a, p, nullptr);
     }
     else
-        launch_spmv<double, true>(g, s, A, A.fvals.data(), x, y, a, p, nullptr);
+        launch_spmv<double, true>(scr, g, s, A, A.fvals.data(), x, y, a, p, nullptr);
     if(d_dot)
         sum_partials_kernel<<<1, kThreads, 0, s>>>(scr.partials.data(), g, d_dot);
     MI_HIP(hipGetLastError());

@@ -218,6 +218,10 @@ class LinearSystem:
             out["hess"], out["inv"] = hess[:Cn], inv[:Cn]
         return out
 
+    # MASPreconditionerEngine::dump_cluster_matrices_debug
+    def mas_dump(self, handle: int, output_dir: str, frame: int = 0, newton_iter: int = 0) -> None:
+        check(self.lib.mi_precond_mas_dump(self.h, handle, os.fsencode(output_dir), frame, newton_iter))
+
     def assemble_preconditioners(self) -> None:
         check(self.lib.mi_precond_assemble(self.h, None))
 

@@ -105,3 +105,39 @@ def test_mas_full_size_apply(ctx):
     z = zd.numpy()
     assert rel_err(z, m.apply(rv)) < 1e-8
     assert rv @ z > 0
+
+
+def test_mas_dump_files(ctx, tmp_path):
+    """The debug dump (reference file names, Matrix Market block layout, JSON
+    metadata) carries exactly the exported state."""
+    import json
+
+    s, _ = system(8, 300, 12, "geometric")
+    ls, h = gpu_mas(ctx, s)
+    g = ls.mas_export(h)
+    ls.mas_dump(h, str(tmp_path), frame=3, newton_iter=7)
+    meta = json.loads((tmp_path / "mas_cluster_meta.f3.n7.json").read_text())
+    C = g["clusters"]
+    assert meta["total_nodes"] == g["V"] and meta["total_map_nodes"] == g["N"]
+    assert meta["total_clusters"] == g["total"] and meta["num_cluster_blocks"] == C
+    assert meta["banksize"] == 16 and meta["sym_block_count"] == 136 and meta["level_num"] == g["L"]
+    assert meta["part_to_real"] == g["part_to_real"].tolist()
+    assert meta["real_to_part"] == g["real_to_part"].tolist()
+    assert meta["going_next"] == g["going_next"][:g["total"]].tolist()
+    assert [(e["count"], e["offset"]) for e in meta["levels"]] == [tuple(x) for x in g["level_sizes"].tolist()]
+    for kind in ("hess", "inv"):
+        lines = (tmp_path / f"mas_cluster_{kind}.f3.n7.mtx").read_text().splitlines()
+        assert lines[0] == "%%MatrixMarket matrix coordinate real general"
+        n, m, e = (int(t) for t in lines[2].split())
+        assert n == m == 48 * C and e == C * 136 * 9
+        ent = np.array([ln.split() for ln in lines[3:]], dtype=np.float64)
+        assert len(ent) == e
+        i = ent[:, 0].astype(np.int64) - 1
+        j = ent[:, 1].astype(np.int64) - 1
+        want = g[kind].reshape(C, 136, 9)
+        # entry order: cluster, upper block (row-major over the bank), row, column
+        cl = i // 48
+        br, bc = (i % 48) // 3, (j % 48) // 3
+        k = 16 * br - br * (br + 1) // 2 + bc
+        assert np.all(bc >= br) and np.array_equal(cl, j // 48)
+        assert np.array_equal(ent[:, 2], want[cl, k, (i % 3) + 3 * (j % 3)])
