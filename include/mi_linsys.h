@@ -102,7 +102,8 @@ int mi_sys_destroy(mi_sys* sys);
  * (row, col), sums duplicates and builds the BSR row offsets
  * (matrix_converter.inl:1041-1096).  Duplicates are summed in ascending source
  * order, so the values are deterministic.  The triplet arrays are read, never
- * written.  No host synchronisation. */
+ * written.  One host synchronisation: the unique-block count is read back
+ * to size the outputs (the reference reads two counts, matrix_converter.inl). */
 int mi_sys_assemble(mi_sys*        sys,
                     int64_t        n_triplets,
                     const int32_t* rows,
@@ -208,6 +209,32 @@ int mi_precond_mas_export(mi_sys*  sys,
  * allocation's 5% slack clusters too; this writes the clusters in use.
  * Synchronises. */
 int mi_precond_mas_dump(mi_sys* sys, int handle, const char* output_dir, int64_t frame, int64_t newton_iter);
+
+/* ------------------------------------------------ FEM subsystem, around the solve */
+/* FEMLinearSubsystem_assemble_k1/k2_kernel (finite_element/fem_linear_subsystem.cu:23-53),
+ * on one subsystem's slice of the triplet buffer before mi_sys_assemble: every
+ * triplet whose (row - block_offset) or (col - block_offset) is a fixed vertex
+ * (is_fixed[v] != 0, v < vert_num) becomes 0 off the diagonal and the identity
+ * on it (per triplet, as in the reference, so k triplets on a fixed diagonal
+ * sum to k I); the fixed vertices' gradient entries become 0.  gradient (3
+ * vert_num doubles, the subsystem's slice) may be NULL.  Triplets with a
+ * negative row are left alone. */
+int mi_subsystem_fix_vertices(mi_ctx*        ctx,
+                              int64_t        n_triplets,
+                              const int32_t* rows,
+                              const int32_t* cols,
+                              double*        vals,
+                              int32_t        block_offset,
+                              const int32_t* is_fixed,
+                              int32_t        vert_num,
+                              double*        gradient,
+                              void*          stream);
+
+/* GlobalLinearSystem::Impl::distribute_solution (global_linear_system.cu:531-544)
+ * -> FEMLinearSubsystem_retrieve_solution_kernel (fem_linear_subsystem.cu:119-126):
+ * dxs[i] = -x[dof_offset + i] for the subsystem's 3 vert_num DoFs. */
+int mi_subsystem_distribute(mi_ctx* ctx, const double* x, int64_t dof_offset, int32_t vert_num, double* dxs,
+                            void* stream);
 
 /* ------------------------------------------------------------ Matrix Market */
 /* The reference's debug dumps (extras/debug/dump_linear_system,

@@ -138,6 +138,9 @@ LINSYS_PROTOS = {
     "mi_precond_mas_export": (C.c_int, [C.c_void_p, C.c_int, c_i32p, c_i32p, c_i32p, c_i32p, c_i32p, c_i32p, c_f64p,
                                         c_f64p]),
     "mi_sys_write_mtx": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int]),
+    "mi_subsystem_fix_vertices": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32,
+                                            C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
+    "mi_subsystem_distribute": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_void_p]),
     "mi_precond_mas_dump": (C.c_int, [C.c_void_p, C.c_int, C.c_char_p, C.c_int64, C.c_int64]),
     "mi_vec_write_mtx": (C.c_int, [C.c_void_p, C.c_char_p, C.c_void_p, C.c_int64]),
     "mi_mtx_read_block_triplets": (C.c_int, [C.c_char_p, C.c_int, c_i32p, C.POINTER(C.c_int64), c_i32p, c_i32p,

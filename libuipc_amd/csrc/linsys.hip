@@ -537,6 +537,37 @@ int mi_precond_mas_export(mi_sys*  sys,
     MI_API_END
 }
 
+int mi_subsystem_fix_vertices(mi_ctx*        ctx,
+                              int64_t        n_triplets,
+                              const int32_t* rows,
+                              const int32_t* cols,
+                              double*        vals,
+                              int32_t        block_offset,
+                              const int32_t* is_fixed,
+                              int32_t        vert_num,
+                              double*        gradient,
+                              void*          stream)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(ctx && n_triplets >= 0 && vert_num >= 0 && block_offset >= 0, MI_ERR_ARG,
+               "mi_subsystem_fix_vertices: bad argument");
+    MI_REQUIRE(n_triplets == 0 || (rows && cols && vals), MI_ERR_ARG, "mi_subsystem_fix_vertices: null triplets");
+    MI_REQUIRE(vert_num == 0 || is_fixed, MI_ERR_ARG, "mi_subsystem_fix_vertices: null is_fixed");
+    subsystem_fix_vertices(rows, cols, vals, n_triplets, block_offset, is_fixed, vert_num, gradient,
+                           pick(ctx, stream));
+    MI_API_END
+}
+
+int mi_subsystem_distribute(mi_ctx* ctx, const double* x, int64_t dof_offset, int32_t vert_num, double* dxs,
+                            void* stream)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(ctx && dof_offset >= 0 && vert_num >= 0, MI_ERR_ARG, "mi_subsystem_distribute: bad argument");
+    MI_REQUIRE(vert_num == 0 || (x && dxs), MI_ERR_ARG, "mi_subsystem_distribute: null vector");
+    subsystem_distribute(x + dof_offset, dxs, 3 * (int64_t)vert_num, pick(ctx, stream));
+    MI_API_END
+}
+
 namespace
 {
 void write_int_array(FILE* fp, const char* key, const int32_t* a, int64_t n, bool last)

@@ -94,6 +94,17 @@ void spmv_axpby(const BsrMatrix& A, bool use32, double a, const double* x, doubl
 
 // ----------------------------------------------------------------- vectors
 void   fill_f64(double* p, double v, int64_t n, hipStream_t s);
+// FEM subsystem steps around the solve (subsystem.hip)
+void subsystem_fix_vertices(const int32_t* rows,
+                            const int32_t* cols,
+                            double*        vals,
+                            int64_t        T,
+                            int32_t        block_offset,
+                            const int32_t* is_fixed,
+                            int32_t        V,
+                            double*        gradient,
+                            hipStream_t    s);
+void subsystem_distribute(const double* x, double* dxs, int64_t n, hipStream_t s);
 void   copy_f64(double* dst, const double* src, int64_t n, hipStream_t s);
 double dot_f64_host(const double* x, const double* y, int64_t n, hipStream_t s);
 double norm_f64_host(const double* x, int64_t n, hipStream_t s);

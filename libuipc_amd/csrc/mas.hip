@@ -762,7 +762,7 @@ __device__ __forceinline__ void cluster_load(const T* __restrict__ M, int lane, 
         {
 #pragma unroll
             for(int q = 0; q < 9; ++q)
-                cb.m[t][q] = __builtin_nontemporal_load(M + inv_at(blk, q));
+                cb.m[t][q] = M[inv_at(blk, q)];
         }
     }
 }

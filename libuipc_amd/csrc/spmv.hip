@@ -78,14 +78,14 @@ __global__ __launch_bounds__(kThreads) void spmv_full_kernel(int32_t        B,
                 {
                     const int32_t ku = k + u * kLanes;
                     ok[u]            = ku < e;
-                    jj[u]            = ok[u] ? (uint32_t)__builtin_nontemporal_load(fcols + ku) : 0u;
+                    jj[u]            = ok[u] ? (uint32_t)fcols[ku] : 0u;
                 }
                 VT A[UNROLL][9];
 #pragma unroll
                 for(int u = 0; u < UNROLL; ++u)
 #pragma unroll
                     for(int q = 0; q < 9; ++q)
-                        A[u][q] = ok[u] ? __builtin_nontemporal_load(fvals + q * fstride + k + u * kLanes) : VT(0);
+                        A[u][q] = ok[u] ? fvals[q * fstride + k + u * kLanes] : VT(0);
                 double xv[UNROLL][3];
 #pragma unroll
                 for(int u = 0; u < UNROLL; ++u)
@@ -120,8 +120,7 @@ __global__ __launch_bounds__(kThreads) void spmv_full_kernel(int32_t        B,
                     ok[u][0]         = ku >= s && ku < e;
                     ok[u][1]         = ku + 1 < e;
                     const bool any   = ku < e;
-                    const v2i  cc    = any ? __builtin_nontemporal_load(reinterpret_cast<const v2i*>(fcols + ku))
-                                           : v2i{0, 0};
+                    const v2i  cc    = any ? *reinterpret_cast<const v2i*>(fcols + ku) : v2i{0, 0};
                     jj[u][0]         = ok[u][0] ? (uint32_t)cc.x : 0u;
                     jj[u][1]         = ok[u][1] ? (uint32_t)cc.y : 0u;
                 }
@@ -133,8 +132,7 @@ __global__ __launch_bounds__(kThreads) void spmv_full_kernel(int32_t        B,
                     const bool    any = ku < e;
 #pragma unroll
                     for(int q = 0; q < 9; ++q)
-                        A[u][q] = any ? __builtin_nontemporal_load(reinterpret_cast<const v2f*>(fvals + q * fstride + ku))
-                                      : v2f{0.f, 0.f};
+                        A[u][q] = any ? *reinterpret_cast<const v2f*>(fvals + q * fstride + ku) : v2f{0.f, 0.f};
                 }
 #pragma unroll
                 for(int u = 0; u < UNROLL; ++u)
