@@ -485,6 +485,31 @@ __global__ void unpack_lower_kernel(const int64_t* d_n, const uint64_t* keys, in
         lrow[k] = (int32_t)(keys[k] / (uint64_t)B);
 }
 
+// chunk_row[c] = the first row whose first full-BSR block is at or after
+// c kChunkStep (binary search in full_ptr); chunk_row[nchunks] = B
+__global__ void chunk_row_kernel(const int32_t* __restrict__ fptr, int32_t B, int32_t nchunks, int32_t* chunk_row)
+{
+    for(int32_t c = blockIdx.x * blockDim.x + threadIdx.x; c <= nchunks; c += gridDim.x * blockDim.x)
+    {
+        if(c == nchunks)
+        {
+            chunk_row[c] = B;
+            continue;
+        }
+        const int64_t target = (int64_t)c * kChunkStep;
+        int32_t       lo = 0, hi = B;  // answer in [0, B]
+        while(lo < hi)
+        {
+            const int32_t mid = lo + (hi - lo) / 2;
+            if((int64_t)fptr[mid] >= target)
+                hi = mid;
+            else
+                lo = mid + 1;
+        }
+        chunk_row[c] = lo;
+    }
+}
+
 __global__ void full_ptr_kernel(const int32_t* lower_ptr, const int32_t* upper_ptr, int32_t* full_ptr, int32_t B)
 {
     for(int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= B; i += (int64_t)gridDim.x * blockDim.x)
@@ -691,6 +716,11 @@ void MatrixConverter::convert_sym(int32_t        B,
     unpack_lower_kernel<<<grid_of(U), kThreads, 0, s>>>(d_off, keys_[tcur].data(), B, lower_rows_.data());
     row_ptr_kernel<<<grid_of(U + 1), kThreads, 0, s>>>(d_off, U, lower_rows_.data(), lower_ptr_.data(), B);
     full_ptr_kernel<<<grid_of(B + 1), kThreads, 0, s>>>(lower_ptr_.data(), out.row_ptr.data(), out.full_ptr.data(), B);
+    // SpMV chunks over the full rows (the full BSR has at most 2U blocks)
+    out.nchunks = (int32_t)std::max<int64_t>(1, (2 * U + kChunkStep - 1) / kChunkStep);
+    out.chunk_row.resize((size_t)out.nchunks + 1, ratio);
+    chunk_row_kernel<<<grid_of(out.nchunks + 1), kThreads, 0, s>>>(out.full_ptr.data(), B, out.nchunks,
+                                                                   out.chunk_row.data());
     // room for one block past the last row: the FP32 SpMV reads blocks in
     // aligned pairs; an even plane stride keeps every pair 8-byte aligned
     const int64_t fcap = 2 * U + 2;

@@ -33,7 +33,13 @@ struct BsrMatrix
     DBuf<double>  fvals;    // 9 * fstride (FP64 systems)
     DBuf<float>   fvals32;  // 9 * fstride (FP32 systems)
     bool          has32 = false;
+    // SpMV chunks: chunk c = rows [chunk_row[c], chunk_row[c + 1]), the rows
+    // whose first full-BSR block lies in [c kChunkStep, (c + 1) kChunkStep)
+    DBuf<int32_t> chunk_row;  // nchunks + 1
+    int32_t       nchunks = 0;
 };
+constexpr int kChunkStep = 512;   // nominal blocks per SpMV chunk
+constexpr int kChunkCap  = 1024;  // blocks a chunk may hold to be staged in LDS
 
 // ----------------------------------------------------------------- converter
 // MatrixConverter<Float,3>::convert_sym (algorithm/details/matrix_converter.inl:1549-1699)
@@ -75,9 +81,8 @@ class MatrixConverter
 struct SpmvScratch
 {
     DBuf<double> partials;
-    int          grid   = 0;
-    int          unroll = 1;
-    bool         use32  = false;
+    int          grid  = 0;
+    bool         use32 = false;
 };
 // allocates the SpMV's reduction scratch and fixes its launch shape for one
 // value precision (must run outside graph capture)

@@ -54,6 +54,9 @@ constexpr int kBank   = 16;
 constexpr int kSym    = kBank * (kBank + 1) / 2;  // 136
 constexpr int kCm     = kSym * 9;                 // 1224 scalars per cluster
 constexpr int kMaxLvl = 6;
+// the MAS restriction kernels run up to the first level with at most this many
+// nodes; coarse clusters above it sum their nodes' descendants at that level
+constexpr int32_t kTailLevel = 4096;
 
 __host__ __device__ __forceinline__ int sym_index(int row, int col)
 {
@@ -890,36 +893,22 @@ __global__ __launch_bounds__(256) void mas_restrict_level_kernel(const double* _
     restrict_bank<T, kLevel0>(r, part_to_real, Rb, parent, sib, begin + x, x < cnt);
 }
 
-// the small top levels (from level lo up) in one workgroup, level by level
-template <typename T>
-__global__ __launch_bounds__(1024) void mas_restrict_tail_kernel(T*             Rb,
-                                                                const int32_t* __restrict__ parent,
-                                                                const int32_t* __restrict__ sib,
-                                                                LevelSpan      lv,
-                                                                int            lo,
-                                                                int            L,
-                                                                const int32_t* conv)
-{
-    if(conv && *conv != 0)
-        return;
-    for(int l = lo; l + 1 < L; ++l)
-    {
-        const int32_t begin = lv.begin[l], cnt = lv.count[l];
-        for(int32_t x0 = 0; x0 < cnt; x0 += blockDim.x)
-        {
-            const int32_t x = x0 + threadIdx.x;
-            restrict_bank<T, false>(nullptr, nullptr, Rb, parent, sib, begin + x, x < cnt);
-        }
-        __syncthreads();
-    }
-}
-
-// one wave per coarse cluster: Z = C^-1 R on the restricted residual
+// one wave per coarse cluster: Z = C^-1 R on the restricted residual.  The
+// restriction kernels stop at the base level b (the first level small enough);
+// clusters above it take each node's R as the sum of its level-b descendants
+// (desc_ptr / desc_idx, ascending).  A cluster with m live nodes gives each
+// node lpn = 64 / m' lanes (m' = m rounded up to a power of two, from the
+// host, desc_lpn); each lane sums a strided share, and the lpn shares are
+// combined by a fixed xor tree.
 template <typename T>
 __global__ __launch_bounds__(64) void mas_apply_coarse_kernel(const T* __restrict__ inv,
-                                                             const T* __restrict__ Rb,
+                                                             T*             Rb,
                                                              T*             Zb,
                                                              int64_t        c_first,
+                                                             int64_t        c_desc,
+                                                             const int32_t* __restrict__ desc_ptr,
+                                                             const int32_t* __restrict__ desc_idx,
+                                                             const int32_t* __restrict__ desc_lpn,
                                                              const int32_t* conv)
 {
     if(conv && *conv != 0)
@@ -930,8 +919,49 @@ __global__ __launch_bounds__(64) void mas_apply_coarse_kernel(const T* __restric
     const int64_t    k    = c_first + blockIdx.x;
     ClusterBlocks<T> cb;
     cluster_load<T>(inv + k * kCm, lane, cb);
-    if(lane < 3 * kBank)
-        sR[lane / 3][lane % 3] = Rb[3 * k * kBank + lane];
+    if(k < c_desc)
+    {
+        if(lane < 3 * kBank)
+            sR[lane / 3][lane % 3] = Rb[3 * k * kBank + lane];
+    }
+    else
+    {
+        const int lpn  = desc_lpn[k - c_desc];  // 4, 8, 16, 32 or 64 (uniform)
+        const int node = lane / lpn, part = lane % lpn;
+        T         s0 = 0, s1 = 0, s2 = 0;
+        if(node < kBank)
+        {
+            const int64_t q  = (k - c_desc) * kBank + node;
+            const int32_t d0 = desc_ptr[q], d1 = desc_ptr[q + 1];
+#pragma unroll 4
+            for(int32_t d = d0 + part; d < d1; d += lpn)
+            {
+                const int64_t y = 3 * (int64_t)desc_idx[d];
+                s0 += Rb[y];
+                s1 += Rb[y + 1];
+                s2 += Rb[y + 2];
+            }
+        }
+        for(int o = 1; o < lpn; o <<= 1)
+        {
+            s0 += __shfl_xor(s0, o, kWave);
+            s1 += __shfl_xor(s1, o, kWave);
+            s2 += __shfl_xor(s2, o, kWave);
+        }
+        if(lane < kBank)
+        {
+            sR[lane][0] = 0;
+            sR[lane][1] = 0;
+            sR[lane][2] = 0;
+        }
+        __syncthreads();
+        if(part == 0 && node < kBank)
+        {
+            sR[node][0] = s0;
+            sR[node][1] = s1;
+            sR[node][2] = s2;
+        }
+    }
     __syncthreads();
     cluster_contrib<T>(cb, lane, sR, sC);
     __syncthreads();
@@ -1028,7 +1058,10 @@ class MasPreconditioner final : public LocalPreconditioner
     int64_t   C = 0, C0 = 0;  // clusters, fine clusters
     // device tables
     DBuf<int32_t> d_part_to_real, d_real_to_part, d_anc, d_anc_slot, d_parent, d_sib, d_unpart;
-    int32_t       n_unpart = 0;
+    DBuf<int32_t> d_desc_ptr, d_desc_idx, d_desc_lpn;
+    int32_t       n_unpart  = 0;
+    int           base_lvl  = 1;  // restriction kernels produce R up to this level
+    int64_t       c_desc    = 0;  // first cluster above the base level
     LevelSpan     span{};
     int64_t       c_first = 0;  // first coarse cluster
     DBuf<double>  d_hess, d_inv64, d_Dun;
@@ -1111,6 +1144,52 @@ class MasPreconditioner final : public LocalPreconditioner
         n_unpart = (int32_t)unpart.size();
         upload(d_anc_slot, anc_slot);
         upload(d_unpart, unpart);
+        // base level: the first level at most kTailLevel nodes (restriction
+        // kernels stop there); level-b descendants of every node above it
+        base_lvl = L - 1;
+        for(int l = 1; l < L; ++l)
+            if(h.level_count[l] <= kTailLevel)
+            {
+                base_lvl = l;
+                break;
+            }
+        const int32_t hi_begin = base_lvl + 1 < L ? h.level_begin[base_lvl + 1] : h.total;
+        c_desc                 = hi_begin / kBank;
+        const int32_t        hi_slots = h.total - hi_begin;
+        std::vector<int32_t> cnt((size_t)hi_slots + 1, 0), desc_ptr((size_t)hi_slots + 1, 0), desc_idx;
+        auto                 walk = [&](auto&& visit) {
+            for(int32_t i = 0; i < h.level_count[base_lvl]; ++i)
+            {
+                const int32_t sb = h.level_begin[base_lvl] + i;
+                for(int32_t a = parent[sb]; a >= 0; a = parent[a])
+                    visit(a - hi_begin, sb);
+            }
+        };
+        walk([&](int32_t q, int32_t) { ++cnt[q]; });
+        for(int32_t q = 0; q < hi_slots; ++q)
+            desc_ptr[q + 1] = desc_ptr[q] + cnt[q];
+        desc_idx.resize((size_t)desc_ptr[hi_slots]);
+        std::vector<int32_t> fillp(desc_ptr.begin(), desc_ptr.end() - 1);
+        walk([&](int32_t q, int32_t sb) { desc_idx[fillp[q]++] = sb; });
+        // lanes per node for every cluster above the base level
+        std::vector<int32_t> lpn((size_t)std::max<int64_t>(1, C - c_desc), 4);
+        for(int64_t k = c_desc; k < C; ++k)
+        {
+            int live = 0;
+            for(int i = 0; i < kBank; ++i)
+            {
+                const int64_t q = (k - c_desc) * kBank + i;
+                if(desc_ptr[q + 1] > desc_ptr[q])
+                    live = i + 1;
+            }
+            int m = 1;
+            while(m < live)
+                m <<= 1;
+            lpn[k - c_desc] = std::max(4, kWave / m);
+        }
+        upload(d_desc_ptr, desc_ptr);
+        upload(d_desc_idx, desc_idx);
+        upload(d_desc_lpn, lpn);
         c_first = L > 1 ? h.level_begin[1] / kBank : C;
         upload(d_sib, sib);
         upload(d_part_to_real, h.part_to_real);
@@ -1171,21 +1250,13 @@ class MasPreconditioner final : public LocalPreconditioner
         MI_HIP(hipGetLastError());
     }
 
-    // levels at most this size are restricted together by one workgroup
-    static constexpr int32_t kTailLevel = 4096;
-
     template <typename T>
     void apply_t(const double* r, double* z, const int32_t* conv, hipStream_t s, const T* inv, T* R, T* Z)
     {
         const int L = h.L;
-        for(int l = 0; l + 1 < L; ++l)
+        for(int l = 0; l < base_lvl; ++l)
         {
             const int32_t cnt = span.count[l];
-            if(l > 0 && cnt <= kTailLevel)
-            {
-                mas_restrict_tail_kernel<T><<<1, 1024, 0, s>>>(R, d_parent.data(), d_sib.data(), span, l, L, conv);
-                break;
-            }
             if(cnt == 0)
                 continue;
             if(l == 0)
@@ -1196,7 +1267,8 @@ class MasPreconditioner final : public LocalPreconditioner
                     r, d_part_to_real.data(), R, d_parent.data(), d_sib.data(), span.begin[l], cnt, conv);
         }
         if(C > c_first)
-            mas_apply_coarse_kernel<T><<<(unsigned)(C - c_first), 64, 0, s>>>(inv, R, Z, c_first, conv);
+            mas_apply_coarse_kernel<T><<<(unsigned)(C - c_first), 64, 0, s>>>(
+                inv, R, Z, c_first, c_desc, d_desc_ptr.data(), d_desc_idx.data(), d_desc_lpn.data(), conv);
         const int     Lact   = (active_levels > 0 && active_levels <= L) ? active_levels : L;
         const int64_t blocks = C0 + ceil_div(n_unpart, kWave);
         if(blocks > 0)

@@ -8,16 +8,24 @@
 // y_j with FP64 atomics (spmv.cu:683-723).  On MI355X the device-scope atomics
 // of eight non-coherent XCD L2s serialise badly, so the assembly also builds
 // the full BSR (both triangles, BsrMatrix::full_*) and the SpMV is a pure row
-// gather:
-//   * 16 lanes per block row stride over the row's blocks; each of the 9 SoA
-//     value planes is read as 128 contiguous bytes per row group;
-//   * the row sum is reduced across the 16 lanes with xor shuffles and stored
-//     once (no atomics, deterministic);
+// gather over it, streamed in row-aligned chunks:
+//   * the assembly cuts the full BSR into chunks of whole rows: chunk c holds
+//     the rows whose first block lies in [c S, (c+1) S) (kChunkStep S), so a
+//     chunk has at most S + (longest row) blocks;
+//   * a workgroup loads a chunk's blocks with consecutive lanes on consecutive
+//     blocks: every value plane is read as 64 contiguous entries per wave
+//     instruction, which streams at close to the copy rate (rows of ~40 blocks
+//     read 16 lanes per row reached ~60% of it);
+//   * each lane forms w_k = A_k x_col(k) for its blocks into LDS; then 16
+//     lanes per row add the row's w in a fixed order and one lane stores y
+//     (no atomics, deterministic);
+//   * a chunk longer than kChunkCap blocks (a row of more than S blocks) is
+//     done row by row straight from memory instead;
 //   * x^T A x = sum_i x_i . (A x)_i per workgroup into partials[blockIdx.x];
 //     the consumer kernel (PCG update) sums the partials in a fixed order;
-//   * a persistent grid; workgroups are renumbered so that each XCD (blocks are
-//     dispatched round-robin over the 8 XCDs) walks one contiguous eighth of
-//     the rows and keeps its part of x in its own L2.
+//   * a persistent grid; workgroups are renumbered so that each XCD (blocks
+//     are dispatched round-robin over the 8 XCDs) walks one contiguous eighth
+//     of the chunks and keeps its part of x in its own L2.
 #include <cstdlib>
 
 #include "linsys.hpp"
@@ -27,154 +35,155 @@ namespace mi
 namespace
 {
 constexpr int kThreads = 256;
-constexpr int kLanes   = 16;  // lanes per block row
+constexpr int kLanes   = 16;  // lanes per block row in the row sums
 constexpr int kGroups  = kThreads / kLanes;
 constexpr int kXcds    = 8;
+constexpr int kPer     = kChunkCap / kThreads;  // blocks per lane per chunk
+static_assert(kChunkCap % (2 * kThreads) == 0, "chunk capacity must be a multiple of two workgroups");
 
-typedef int   v2i __attribute__((ext_vector_type(2)));
-typedef float v2f __attribute__((ext_vector_type(2)));
+template <bool ACCUM>
+__device__ __forceinline__ void store_row(double* y, int64_t row, double a, double acc0, double acc1, double acc2)
+{
+    double* yr = y + 3 * row;
+    if constexpr(ACCUM)
+    {
+        yr[0] += a * acc0;
+        yr[1] += a * acc1;
+        yr[2] += a * acc2;
+    }
+    else
+    {
+        yr[0] = acc0;
+        yr[1] = acc1;
+        yr[2] = acc2;
+    }
+}
 
-template <typename VT, bool ACCUM, int UNROLL>
-__global__ __launch_bounds__(kThreads) void spmv_full_kernel(int32_t        B,
-                                                            const int32_t* __restrict__ fptr,
-                                                            const int32_t* __restrict__ fcols,
-                                                            const VT* __restrict__ fvals,
-                                                            int64_t       fstride,
-                                                            const double* __restrict__ x,
-                                                            double*       y,
-                                                            double        a,
-                                                            double*       partials,
-                                                            const int32_t* d_conv)
+template <typename VT, bool ACCUM>
+__global__ __launch_bounds__(kThreads) void spmv_chunk_kernel(int32_t        nchunks,
+                                                             const int32_t* __restrict__ chunk_row,
+                                                             const int32_t* __restrict__ fptr,
+                                                             const int32_t* __restrict__ fcols,
+                                                             const VT* __restrict__ fvals,
+                                                             int64_t       fstride,
+                                                             const double* __restrict__ x,
+                                                             double*       y,
+                                                             double        a,
+                                                             double*       partials,
+                                                             const int32_t* d_conv)
 {
     if(d_conv && *d_conv != 0)
         return;
+    __shared__ double w[3][kChunkCap];
     __shared__ double lds[kThreads / kWave];
 
     const int G   = gridDim.x;
     const int b   = blockIdx.x;
     const int per = G / kXcds;
     const int lb  = (per > 0 && b < per * kXcds) ? (b % kXcds) * per + b / kXcds : b;
-    const int64_t r0 = (int64_t)B * lb / G;
-    const int64_t r1 = (int64_t)B * (lb + 1) / G;
+    const int c0  = (int)((int64_t)nchunks * lb / G);
+    const int c1  = (int)((int64_t)nchunks * (lb + 1) / G);
 
-    const int g    = threadIdx.x / kLanes;
-    const int lane = threadIdx.x % kLanes;
+    const int tid  = threadIdx.x;
+    const int g    = tid / kLanes;
+    const int lane = tid % kLanes;
     double    dot  = 0.0;
-    for(int64_t row = r0 + g; row < r1; row += kGroups)
+    for(int c = c0; c < c1; ++c)
     {
-        const int32_t s = fptr[row], e = fptr[row + 1];
-        double        acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
-        // UNROLL blocks per lane per trip: all column indices, value sets and
-        // x gathers are in flight before the first FMA.  Plane bases are
-        // uniform and offsets 32-bit (scalar base + vector offset addressing).
-        if constexpr(sizeof(VT) == 8)
+        const int32_t row0 = chunk_row[c], row1 = chunk_row[c + 1];
+        const int32_t k0 = fptr[row0], k1 = fptr[row1];
+        if(k1 - k0 <= kChunkCap)
         {
-            for(int32_t k = s + lane; k < e; k += UNROLL * kLanes)
+            // products of the chunk's blocks, consecutive lanes on consecutive blocks
+#pragma unroll
+            for(int h = 0; h < kPer; h += 2)
             {
-                uint32_t jj[UNROLL];
-                bool     ok[UNROLL];
+                uint32_t jj[2];
+                bool     ok[2];
+                VT       A[2][9];
 #pragma unroll
-                for(int u = 0; u < UNROLL; ++u)
+                for(int u = 0; u < 2; ++u)
                 {
-                    const int32_t ku = k + u * kLanes;
-                    ok[u]            = ku < e;
-                    jj[u]            = ok[u] ? (uint32_t)fcols[ku] : 0u;
-                }
-                VT A[UNROLL][9];
-#pragma unroll
-                for(int u = 0; u < UNROLL; ++u)
+                    const int32_t k = k0 + (h + u) * kThreads + tid;
+                    ok[u]           = k < k1;
+                    jj[u]           = ok[u] ? (uint32_t)fcols[k] : 0u;
 #pragma unroll
                     for(int q = 0; q < 9; ++q)
-                        A[u][q] = ok[u] ? fvals[q * fstride + k + u * kLanes] : VT(0);
-                double xv[UNROLL][3];
+                        A[u][q] = ok[u] ? fvals[q * fstride + k] : VT(0);
+                }
 #pragma unroll
-                for(int u = 0; u < UNROLL; ++u)
-#pragma unroll
-                    for(int c = 0; c < 3; ++c)
-                        xv[u][c] = x[3u * jj[u] + c];
-#pragma unroll
-                for(int u = 0; u < UNROLL; ++u)
+                for(int u = 0; u < 2; ++u)
                 {
-                    acc0 += (double)A[u][0] * xv[u][0] + (double)A[u][3] * xv[u][1] + (double)A[u][6] * xv[u][2];
-                    acc1 += (double)A[u][1] * xv[u][0] + (double)A[u][4] * xv[u][1] + (double)A[u][7] * xv[u][2];
-                    acc2 += (double)A[u][2] * xv[u][0] + (double)A[u][5] * xv[u][1] + (double)A[u][8] * xv[u][2];
+                    const double x0 = x[3u * jj[u]], x1 = x[3u * jj[u] + 1], x2 = x[3u * jj[u] + 2];
+                    if(ok[u])
+                    {
+                        const int i = (h + u) * kThreads + tid;
+                        w[0][i]     = (double)A[u][0] * x0 + (double)A[u][3] * x1 + (double)A[u][6] * x2;
+                        w[1][i]     = (double)A[u][1] * x0 + (double)A[u][4] * x1 + (double)A[u][7] * x2;
+                        w[2][i]     = (double)A[u][2] * x0 + (double)A[u][5] * x1 + (double)A[u][8] * x2;
+                    }
                 }
             }
+            __syncthreads();
+            for(int32_t row = row0 + g; row < row1; row += kGroups)
+            {
+                const int32_t s = fptr[row] - k0, e = fptr[row + 1] - k0;
+                double        acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
+                for(int32_t i = s + lane; i < e; i += kLanes)
+                {
+                    acc0 += w[0][i];
+                    acc1 += w[1][i];
+                    acc2 += w[2][i];
+                }
+#pragma unroll
+                for(int o = kLanes / 2; o > 0; o >>= 1)
+                {
+                    acc0 += __shfl_xor(acc0, o, kWave);
+                    acc1 += __shfl_xor(acc1, o, kWave);
+                    acc2 += __shfl_xor(acc2, o, kWave);
+                }
+                if(lane == 0)
+                {
+                    store_row<ACCUM>(y, row, a, acc0, acc1, acc2);
+                    dot += x[3 * (int64_t)row] * acc0 + x[3 * (int64_t)row + 1] * acc1
+                           + x[3 * (int64_t)row + 2] * acc2;
+                }
+            }
+            __syncthreads();  // w is rewritten by the next chunk
         }
         else
         {
-            // FP32 planes: each lane takes two neighbouring blocks with 8-byte
-            // loads, so 16 lanes read 128 contiguous bytes per plane.  The pair
-            // grid starts at the even block at or before the row start; the
-            // halves outside [s, e) are masked.  The planes are 8-byte aligned
-            // (the plane stride is even).
-            const int32_t s2 = s & ~1;
-            for(int32_t k = s2 + 2 * lane; k < e; k += UNROLL * 2 * kLanes)
+            // a chunk with a very long row: 16 lanes per row straight from memory
+            for(int32_t row = row0 + g; row < row1; row += kGroups)
             {
-                uint32_t jj[UNROLL][2];
-                bool     ok[UNROLL][2];
-#pragma unroll
-                for(int u = 0; u < UNROLL; ++u)
+                const int32_t s = fptr[row], e = fptr[row + 1];
+                double        acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
+                for(int32_t k = s + lane; k < e; k += kLanes)
                 {
-                    const int32_t ku = k + u * 2 * kLanes;
-                    ok[u][0]         = ku >= s && ku < e;
-                    ok[u][1]         = ku + 1 < e;
-                    const bool any   = ku < e;
-                    const v2i  cc    = any ? *reinterpret_cast<const v2i*>(fcols + ku) : v2i{0, 0};
-                    jj[u][0]         = ok[u][0] ? (uint32_t)cc.x : 0u;
-                    jj[u][1]         = ok[u][1] ? (uint32_t)cc.y : 0u;
-                }
-                v2f A[UNROLL][9];
-#pragma unroll
-                for(int u = 0; u < UNROLL; ++u)
-                {
-                    const int32_t ku  = k + u * 2 * kLanes;
-                    const bool    any = ku < e;
+                    const uint32_t j = (uint32_t)fcols[k];
+                    double         A[9];
 #pragma unroll
                     for(int q = 0; q < 9; ++q)
-                        A[u][q] = any ? *reinterpret_cast<const v2f*>(fvals + q * fstride + ku) : v2f{0.f, 0.f};
+                        A[q] = (double)fvals[q * fstride + k];
+                    const double x0 = x[3u * j], x1 = x[3u * j + 1], x2 = x[3u * j + 2];
+                    acc0 += A[0] * x0 + A[3] * x1 + A[6] * x2;
+                    acc1 += A[1] * x0 + A[4] * x1 + A[7] * x2;
+                    acc2 += A[2] * x0 + A[5] * x1 + A[8] * x2;
                 }
 #pragma unroll
-                for(int u = 0; u < UNROLL; ++u)
-#pragma unroll
-                    for(int h = 0; h < 2; ++h)
-                    {
-                        if(!ok[u][h])
-                            continue;
-                        const double x0 = x[3u * jj[u][h]], x1 = x[3u * jj[u][h] + 1], x2 = x[3u * jj[u][h] + 2];
-                        float        a[9];
-#pragma unroll
-                        for(int q = 0; q < 9; ++q)
-                            a[q] = h ? A[u][q].y : A[u][q].x;
-                        acc0 += (double)a[0] * x0 + (double)a[3] * x1 + (double)a[6] * x2;
-                        acc1 += (double)a[1] * x0 + (double)a[4] * x1 + (double)a[7] * x2;
-                        acc2 += (double)a[2] * x0 + (double)a[5] * x1 + (double)a[8] * x2;
-                    }
+                for(int o = kLanes / 2; o > 0; o >>= 1)
+                {
+                    acc0 += __shfl_xor(acc0, o, kWave);
+                    acc1 += __shfl_xor(acc1, o, kWave);
+                    acc2 += __shfl_xor(acc2, o, kWave);
+                }
+                if(lane == 0)
+                {
+                    store_row<ACCUM>(y, row, a, acc0, acc1, acc2);
+                    dot += x[3 * (int64_t)row] * acc0 + x[3 * (int64_t)row + 1] * acc1
+                           + x[3 * (int64_t)row + 2] * acc2;
+                }
             }
-        }
-#pragma unroll
-        for(int o = kLanes / 2; o > 0; o >>= 1)
-        {
-            acc0 += __shfl_xor(acc0, o, kWave);
-            acc1 += __shfl_xor(acc1, o, kWave);
-            acc2 += __shfl_xor(acc2, o, kWave);
-        }
-        if(lane == 0)
-        {
-            double* yr = y + 3 * row;
-            if constexpr(ACCUM)
-            {
-                yr[0] += a * acc0;
-                yr[1] += a * acc1;
-                yr[2] += a * acc2;
-            }
-            else
-            {
-                yr[0] = acc0;
-                yr[1] = acc1;
-                yr[2] = acc2;
-            }
-            dot += x[3 * row] * acc0 + x[3 * row + 1] * acc1 + x[3 * row + 2] * acc2;
         }
     }
     if(partials)
@@ -203,46 +212,20 @@ __global__ __launch_bounds__(kThreads) void sum_partials_kernel(const double* pa
         *out = acc;
 }
 
-// blocks per lane per trip of the row loop: 4 for FP64 planes, 1 (pairs) for
-// FP32; MI_SPMV_UNROLL = 1, 2 or 4 overrides for launch sweeps
-int spmv_unroll(bool use32)
+template <typename VT, bool ACCUM>
+void launch_spmv(int g, hipStream_t s, const BsrMatrix& A, const VT* vals, const double* x, double* y, double a,
+                 double* partials, const int32_t* d_conv)
 {
-    int v = use32 ? 1 : 4;
-    if(const char* e = std::getenv("MI_SPMV_UNROLL"))
-    {
-        const int w = std::atoi(e);
-        if(w == 1 || w == 2 || w == 4)
-            v = w;
-    }
-    return v;
+    spmv_chunk_kernel<VT, ACCUM><<<g, kThreads, 0, s>>>(A.nchunks, A.chunk_row.data(), A.full_ptr.data(),
+                                                        A.fcols.data(), vals, A.fstride, x, y, a, partials, d_conv);
 }
 
-template <typename VT, bool ACCUM>
-const void* spmv_kernel_of(int unroll)
+void check_matrix(const BsrMatrix& A, bool use32)
 {
-    return unroll == 1   ? (const void*)spmv_full_kernel<VT, ACCUM, 1>
-           : unroll == 4 ? (const void*)spmv_full_kernel<VT, ACCUM, 4>
-                         : (const void*)spmv_full_kernel<VT, ACCUM, 2>;
-}
-
-template <typename VT, bool ACCUM>
-void launch_spmv(const SpmvScratch& scr, int g, hipStream_t s, const BsrMatrix& A, const VT* vals, const double* x,
-                 double* y, double a, double* partials, const int32_t* d_conv)
-{
-    switch(scr.unroll)
-    {
-        case 1:
-            spmv_full_kernel<VT, ACCUM, 1><<<g, kThreads, 0, s>>>(A.B, A.full_ptr.data(), A.fcols.data(), vals,
-                                                                  A.fstride, x, y, a, partials, d_conv);
-            break;
-        case 4:
-            spmv_full_kernel<VT, ACCUM, 4><<<g, kThreads, 0, s>>>(A.B, A.full_ptr.data(), A.fcols.data(), vals,
-                                                                  A.fstride, x, y, a, partials, d_conv);
-            break;
-        default:
-            spmv_full_kernel<VT, ACCUM, 2><<<g, kThreads, 0, s>>>(A.B, A.full_ptr.data(), A.fcols.data(), vals,
-                                                                  A.fstride, x, y, a, partials, d_conv);
-    }
+    MI_REQUIRE(A.full_ptr.data() && A.fcols.data() && A.chunk_row.data() && A.nchunks > 0, MI_ERR_STATE,
+               "spmv: matrix not assembled");
+    MI_REQUIRE(use32 ? A.has32 : (A.fvals.data() != nullptr), MI_ERR_STATE,
+               "spmv: values of this precision were not assembled");
 }
 }  // namespace
 
@@ -250,16 +233,16 @@ void spmv_prepare(SpmvScratch& scr, bool use32)
 {
     if(scr.grid == 0 || scr.use32 != use32)
     {
-        scr.use32  = use32;
-        scr.unroll = spmv_unroll(use32);
-        // a persistent grid of as many 4-wave workgroups per CU as fit at once
-        // (the kernel's occupancy); MI_SPMV_BLOCKS_PER_CU overrides for sweeps
+        scr.use32 = use32;
+        // a persistent grid of as many workgroups per CU as fit at once (the
+        // kernel's occupancy); MI_SPMV_BLOCKS_PER_CU overrides for sweeps
         int per_cu = 0;
         if(const char* e = std::getenv("MI_SPMV_BLOCKS_PER_CU"))
             per_cu = std::max(1, std::min(32, std::atoi(e)));
         if(per_cu == 0)
         {
-            const void* fn = use32 ? spmv_kernel_of<float, false>(scr.unroll) : spmv_kernel_of<double, false>(scr.unroll);
+            const void* fn = use32 ? (const void*)spmv_chunk_kernel<float, false>
+                                   : (const void*)spmv_chunk_kernel<double, false>;
             MI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kThreads, 0));
             per_cu = std::max(1, per_cu);
         }
@@ -270,32 +253,26 @@ void spmv_prepare(SpmvScratch& scr, bool use32)
     }
 }
 
-int spmv_grid(const SpmvScratch& scr, int32_t B)
+int spmv_grid(const SpmvScratch& scr, const BsrMatrix& A)
 {
-    int64_t need = ((int64_t)B + kGroups - 1) / kGroups;
-    int     g    = scr.grid;
-    if(need < g)
-        g = (int)std::max<int64_t>(1, need);
-    return g;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(scr.grid, A.nchunks));
 }
 
 int spmv_full(const BsrMatrix& A, bool use32, const double* x, double* y, double* partials, const int32_t* d_conv,
               SpmvScratch& scr, hipStream_t s)
 {
     spmv_prepare(scr, use32);
-    const int g = spmv_grid(scr, A.B);
     if(A.B == 0)
         return 0;
-    MI_REQUIRE(x && y && partials && A.full_ptr.data() && A.fcols.data(), MI_ERR_STATE, "spmv_full: null buffer");
+    check_matrix(A, use32);
+    const int g = spmv_grid(scr, A);
+    MI_REQUIRE(x && y && partials, MI_ERR_STATE, "spmv_full: null buffer");
     MI_REQUIRE(partials >= scr.partials.data() && partials + g <= scr.partials.data() + scr.partials.size(),
                MI_ERR_STATE, "spmv_full: partials outside the scratch");
     if(use32)
-    {
-        MI_REQUIRE(A.has32, MI_ERR_STATE, "spmv: FP32 values were not assembled");
-        launch_spmv<float, false>(scr, g, s, A, A.fvals32.data(), x, y, 1.0, partials, d_conv);
-    }
+        launch_spmv<float, false>(g, s, A, A.fvals32.data(), x, y, 1.0, partials, d_conv);
     else
-        launch_spmv<double, false>(scr, g, s, A, A.fvals.data(), x, y, 1.0, partials, d_conv);
+        launch_spmv<double, false>(g, s, A, A.fvals.data(), x, y, 1.0, partials, d_conv);
     MI_HIP(hipGetLastError());
     return g;
 }
@@ -310,15 +287,13 @@ void spmv_accumulate_dot(const BsrMatrix& A, bool use32, double a, const double*
             fill_f64(d_dot, 0.0, 1, s);
         return;
     }
-    const int g = spmv_grid(scr, A.B);
+    check_matrix(A, use32);
+    const int g = spmv_grid(scr, A);
     double*   p = d_dot ? scr.partials.data() : nullptr;
     if(use32)
-    {
-        MI_REQUIRE(A.has32, MI_ERR_STATE, "spmv: FP32 values were not assembled");
-        launch_spmv<float, true>(scr, g, s, A, A.fvals32.data(), x, y, a, p, nullptr);
-    }
+        launch_spmv<float, true>(g, s, A, A.fvals32.data(), x, y, a, p, nullptr);
     else
-        launch_spmv<double, true>(scr, g, s, A, A.fvals.data(), x, y, a, p, nullptr);
+        launch_spmv<double, true>(g, s, A, A.fvals.data(), x, y, a, p, nullptr);
     if(d_dot)
         sum_partials_kernel<<<1, kThreads, 0, s>>>(scr.partials.data(), g, d_dot);
     MI_HIP(hipGetLastError());
