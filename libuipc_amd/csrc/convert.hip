@@ -510,10 +510,40 @@ __global__ void chunk_row_kernel(const int32_t* __restrict__ fptr, int32_t B, in
     }
 }
 
-__global__ void full_ptr_kernel(const int32_t* lower_ptr, const int32_t* upper_ptr, int32_t* full_ptr, int32_t B)
+// blocks per full-BSR row (lower + upper; an empty row gets one zero diagonal
+// block, so every row holds at least one), scanned into full_ptr afterwards
+__global__ void full_count_kernel(const int32_t* lower_ptr, const int32_t* upper_ptr, uint32_t* cnt, int32_t B)
 {
     for(int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= B; i += (int64_t)gridDim.x * blockDim.x)
-        full_ptr[i] = lower_ptr[i] + upper_ptr[i];
+    {
+        if(i == B)
+        {
+            cnt[i] = 0;
+            continue;
+        }
+        const int32_t c = (lower_ptr[i + 1] - lower_ptr[i]) + (upper_ptr[i + 1] - upper_ptr[i]);
+        cnt[i]          = c > 0 ? (uint32_t)c : 1u;
+    }
+}
+
+template <typename VT>
+__global__ void full_empty_kernel(const int32_t* lower_ptr,
+                                  const int32_t* upper_ptr,
+                                  const int32_t* full_ptr,
+                                  int32_t        B,
+                                  int32_t*       fcols,
+                                  VT*            fvals,
+                                  int64_t        fstride)
+{
+    for(int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < B; i += gridDim.x * blockDim.x)
+        if(lower_ptr[i + 1] == lower_ptr[i] && upper_ptr[i + 1] == upper_ptr[i])
+        {
+            const int64_t d = full_ptr[i];
+            fcols[d]        = i;
+#pragma unroll
+            for(int q = 0; q < 9; ++q)
+                fvals[q * fstride + d] = VT(0);
+        }
 }
 
 template <typename VT>
@@ -715,17 +745,19 @@ void MatrixConverter::convert_sym(int32_t        B,
     lower_ptr_.resize((size_t)B + 1, ratio);
     unpack_lower_kernel<<<grid_of(U), kThreads, 0, s>>>(d_off, keys_[tcur].data(), B, lower_rows_.data());
     row_ptr_kernel<<<grid_of(U + 1), kThreads, 0, s>>>(d_off, U, lower_rows_.data(), lower_ptr_.data(), B);
-    full_ptr_kernel<<<grid_of(B + 1), kThreads, 0, s>>>(lower_ptr_.data(), out.row_ptr.data(), out.full_ptr.data(), B);
-    // SpMV chunks over the full rows (the full BSR has at most 2U blocks)
-    out.nchunks = (int32_t)std::max<int64_t>(1, (2 * U + kChunkStep - 1) / kChunkStep);
+    {
+        uint32_t* cnt = reinterpret_cast<uint32_t*>(out.full_ptr.data());  // scanned in place
+        full_count_kernel<<<grid_of(B + 1), kThreads, 0, s>>>(lower_ptr_.data(), out.row_ptr.data(), cnt, B);
+        scan_exclusive(cnt, (int64_t)B + 1, chunk_, counts_.data() + 3, s);
+    }
+    // SpMV chunks over the full rows (the full BSR has at most 2U + B blocks)
+    const int64_t fmax = 2 * U + B;
+    out.nchunks = (int32_t)std::max<int64_t>(1, (fmax + kChunkStep - 1) / kChunkStep);
     out.chunk_row.resize((size_t)out.nchunks + 1, ratio);
     chunk_row_kernel<<<grid_of(out.nchunks + 1), kThreads, 0, s>>>(out.full_ptr.data(), B, out.nchunks,
                                                                    out.chunk_row.data());
-    // room for one block past the last row: the FP32 SpMV reads blocks in
-    // aligned pairs; an even plane stride keeps every pair 8-byte aligned
-    const int64_t fcap = 2 * U + 2;
-    out.fcols.resize((size_t)fcap, ratio);
-    out.fstride = ((int64_t)out.fcols.cap + 1) & ~(int64_t)1;
+    out.fcols.resize((size_t)std::max<int64_t>(1, fmax), ratio);
+    out.fstride = (int64_t)out.fcols.cap;
     if(want32)
     {
         out.fvals32.resize((size_t)9 * out.fstride);
@@ -737,6 +769,9 @@ void MatrixConverter::convert_sym(int32_t        B,
                                                                  out.row_ptr.data(), lower_ptr_.data(),
                                                                  out.full_ptr.data(), out.vals.data(), out.stride,
                                                                  out.fcols.data(), out.fvals32.data(), out.fstride);
+        full_empty_kernel<float><<<grid_of(B), kThreads, 0, s>>>(lower_ptr_.data(), out.row_ptr.data(),
+                                                                 out.full_ptr.data(), B, out.fcols.data(),
+                                                                 out.fvals32.data(), out.fstride);
     }
     else
     {
@@ -749,6 +784,9 @@ void MatrixConverter::convert_sym(int32_t        B,
                                                                   out.row_ptr.data(), lower_ptr_.data(),
                                                                   out.full_ptr.data(), out.vals.data(), out.stride,
                                                                   out.fcols.data(), out.fvals.data(), out.fstride);
+        full_empty_kernel<double><<<grid_of(B), kThreads, 0, s>>>(lower_ptr_.data(), out.row_ptr.data(),
+                                                                  out.full_ptr.data(), B, out.fcols.data(),
+                                                                  out.fvals.data(), out.fstride);
     }
     MI_HIP(hipGetLastError());
 }

@@ -38,8 +38,7 @@ struct BsrMatrix
     DBuf<int32_t> chunk_row;  // nchunks + 1
     int32_t       nchunks = 0;
 };
-constexpr int kChunkStep = 512;   // nominal blocks per SpMV chunk
-constexpr int kChunkCap  = 1024;  // blocks a chunk may hold to be staged in LDS
+constexpr int kChunkStep = 512;  // nominal blocks per SpMV chunk
 
 // ----------------------------------------------------------------- converter
 // MatrixConverter<Float,3>::convert_sym (algorithm/details/matrix_converter.inl:1549-1699)

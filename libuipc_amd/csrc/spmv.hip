@@ -7,25 +7,27 @@
 // gfx950 design: the reference accumulates A^T x_i of every upper block into
 // y_j with FP64 atomics (spmv.cu:683-723).  On MI355X the device-scope atomics
 // of eight non-coherent XCD L2s serialise badly, so the assembly also builds
-// the full BSR (both triangles, BsrMatrix::full_*) and the SpMV is a pure row
-// gather over it, streamed in row-aligned chunks:
-//   * the assembly cuts the full BSR into chunks of whole rows: chunk c holds
-//     the rows whose first block lies in [c S, (c+1) S) (kChunkStep S), so a
-//     chunk has at most S + (longest row) blocks;
-//   * a workgroup loads a chunk's blocks with consecutive lanes on consecutive
-//     blocks: every value plane is read as 64 contiguous entries per wave
-//     instruction, which streams at close to the copy rate (rows of ~40 blocks
-//     read 16 lanes per row reached ~60% of it);
-//   * each lane forms w_k = A_k x_col(k) for its blocks into LDS; then 16
-//     lanes per row add the row's w in a fixed order and one lane stores y
-//     (no atomics, deterministic);
-//   * a chunk longer than kChunkCap blocks (a row of more than S blocks) is
-//     done row by row straight from memory instead;
-//   * x^T A x = sum_i x_i . (A x)_i per workgroup into partials[blockIdx.x];
-//     the consumer kernel (PCG update) sums the partials in a fixed order;
+// the full BSR (both triangles, BsrMatrix::full_*; every row holds at least
+// one block, an empty row gets a zero diagonal block) and the SpMV is a row
+// gather over it, streamed as one flat sequence of blocks:
+//   * each wave owns a run of whole rows (the chunks of the assembly: rows
+//     whose first block lies in [c S, (c+1) S), kChunkStep S) and walks their
+//     blocks in windows of 64, lane l on block k + l, so every value plane is
+//     read as 64 contiguous entries per instruction (the rate of a plain
+//     stream; 16 lanes per row of ~40 blocks reached ~60% of it);
+//   * a lane's row is found by a binary search over the 64 row starts that
+//     can meet the window (fptr[row .. row + 63], held one per lane, read
+//     with shuffles); w = A_k x_col(k) is summed per row by a segmented
+//     inclusive scan over the wave (xor-free shuffle-up steps, fixed order:
+//     deterministic, no LDS, no barrier); a row that runs past the window
+//     carries its partial sum into the next one;
+//   * the lane holding a row's last block stores y (no atomics) and adds
+//     x_row . (A x)_row to the wave's x^T A x; workgroup partials go to
+//     partials[blockIdx.x], which the consumer kernel (PCG update) sums in a
+//     fixed order;
 //   * a persistent grid; workgroups are renumbered so that each XCD (blocks
 //     are dispatched round-robin over the 8 XCDs) walks one contiguous eighth
-//     of the chunks and keeps its part of x in its own L2.
+//     of the rows and keeps its part of x in its own L2.
 #include <cstdlib>
 
 #include "linsys.hpp"
@@ -35,155 +37,133 @@ namespace mi
 namespace
 {
 constexpr int kThreads = 256;
-constexpr int kLanes   = 16;  // lanes per block row in the row sums
-constexpr int kGroups  = kThreads / kLanes;
+constexpr int kWaves   = kThreads / kWave;
 constexpr int kXcds    = 8;
-constexpr int kPer     = kChunkCap / kThreads;  // blocks per lane per chunk
-static_assert(kChunkCap % (2 * kThreads) == 0, "chunk capacity must be a multiple of two workgroups");
-
-template <bool ACCUM>
-__device__ __forceinline__ void store_row(double* y, int64_t row, double a, double acc0, double acc1, double acc2)
-{
-    double* yr = y + 3 * row;
-    if constexpr(ACCUM)
-    {
-        yr[0] += a * acc0;
-        yr[1] += a * acc1;
-        yr[2] += a * acc2;
-    }
-    else
-    {
-        yr[0] = acc0;
-        yr[1] = acc1;
-        yr[2] = acc2;
-    }
-}
 
 template <typename VT, bool ACCUM>
-__global__ __launch_bounds__(kThreads) void spmv_chunk_kernel(int32_t        nchunks,
-                                                             const int32_t* __restrict__ chunk_row,
-                                                             const int32_t* __restrict__ fptr,
-                                                             const int32_t* __restrict__ fcols,
-                                                             const VT* __restrict__ fvals,
-                                                             int64_t       fstride,
-                                                             const double* __restrict__ x,
-                                                             double*       y,
-                                                             double        a,
-                                                             double*       partials,
-                                                             const int32_t* d_conv)
+__global__ __launch_bounds__(kThreads) void spmv_seg_kernel(int32_t        nchunks,
+                                                           const int32_t* __restrict__ chunk_row,
+                                                           const int32_t* __restrict__ fptr,
+                                                           const int32_t* __restrict__ fcols,
+                                                           const VT* __restrict__ fvals,
+                                                           int64_t       fstride,
+                                                           const double* __restrict__ x,
+                                                           double*       y,
+                                                           double        a,
+                                                           double*       partials,
+                                                           const int32_t* d_conv)
 {
     if(d_conv && *d_conv != 0)
         return;
-    __shared__ double w[3][kChunkCap];
     __shared__ double lds[kThreads / kWave];
 
     const int G   = gridDim.x;
     const int b   = blockIdx.x;
     const int per = G / kXcds;
     const int lb  = (per > 0 && b < per * kXcds) ? (b % kXcds) * per + b / kXcds : b;
-    const int c0  = (int)((int64_t)nchunks * lb / G);
-    const int c1  = (int)((int64_t)nchunks * (lb + 1) / G);
+    const int W   = G * kWaves;
+    const int wv  = lb * kWaves + (int)(threadIdx.x / kWave);
+    const int c0  = (int)((int64_t)nchunks * wv / W);
+    const int c1  = (int)((int64_t)nchunks * (wv + 1) / W);
 
-    const int tid  = threadIdx.x;
-    const int g    = tid / kLanes;
-    const int lane = tid % kLanes;
+    const int lane = (int)(threadIdx.x % kWave);
     double    dot  = 0.0;
-    for(int c = c0; c < c1; ++c)
+    if(c0 < c1)
     {
-        const int32_t row0 = chunk_row[c], row1 = chunk_row[c + 1];
-        const int32_t k0 = fptr[row0], k1 = fptr[row1];
-        if(k1 - k0 <= kChunkCap)
+        int32_t       row    = chunk_row[c0];
+        const int32_t rowEnd = chunk_row[c1];
+        const int32_t kEnd   = fptr[rowEnd];
+        double        carry0 = 0.0, carry1 = 0.0, carry2 = 0.0;
+        for(int32_t k = fptr[row]; k < kEnd; k += kWave)
         {
-            // products of the chunk's blocks, consecutive lanes on consecutive blocks
+            const int32_t kb = k + lane;
+            const bool    ok = kb < kEnd;
+            // this window's blocks: consecutive lanes on consecutive blocks
+            const uint32_t j = ok ? (uint32_t)fcols[kb] : 0u;
+            VT             A[9];
 #pragma unroll
-            for(int h = 0; h < kPer; h += 2)
+            for(int q = 0; q < 9; ++q)
+                A[q] = ok ? fvals[q * fstride + kb] : VT(0);
+            // starts of the rows that can meet the window (every row holds a
+            // block, so at most 64 of them); past rowEnd the start is kEnd
+            const int32_t f = fptr[min(row + lane, rowEnd)];
+            const double  x0 = x[3u * j], x1 = x[3u * j + 1], x2 = x[3u * j + 2];
+            // row of block kb: the last i with f_i <= kb
+            int idx = 0;
+#pragma unroll
+            for(int step = kWave / 2; step > 0; step >>= 1)
             {
-                uint32_t jj[2];
-                bool     ok[2];
-                VT       A[2][9];
+                const int32_t fc = __shfl(f, idx + step, kWave);
+                if(fc <= kb)
+                    idx += step;
+            }
+            const int32_t fcur  = __shfl(f, idx, kWave);
+            int32_t       fnext = __shfl(f, min(idx + 1, kWave - 1), kWave);
+            const int32_t myrow = row + idx;
+            if(idx == kWave - 1)
+                fnext = fptr[min(myrow + 1, rowEnd)];
+            const bool head = ok && fcur == kb;
+            double     v0 = 0.0, v1 = 0.0, v2 = 0.0;
+            if(ok)
+            {
+                v0 = (double)A[0] * x0 + (double)A[3] * x1 + (double)A[6] * x2;
+                v1 = (double)A[1] * x0 + (double)A[4] * x1 + (double)A[7] * x2;
+                v2 = (double)A[2] * x0 + (double)A[5] * x1 + (double)A[8] * x2;
+            }
+            // segmented inclusive scan: each lane ends with the sum of its row's
+            // blocks from the row's head (or lane 0) up to itself
+            const uint64_t hmask = __ballot(head || !ok);
+            int            fl    = (head || !ok) ? 1 : 0;
 #pragma unroll
-                for(int u = 0; u < 2; ++u)
+            for(int d = 1; d < kWave; d <<= 1)
+            {
+                const double t0 = __shfl_up(v0, d, kWave), t1 = __shfl_up(v1, d, kWave), t2 = __shfl_up(v2, d, kWave);
+                const int    tf = __shfl_up(fl, d, kWave);
+                if(lane >= d)
                 {
-                    const int32_t k = k0 + (h + u) * kThreads + tid;
-                    ok[u]           = k < k1;
-                    jj[u]           = ok[u] ? (uint32_t)fcols[k] : 0u;
-#pragma unroll
-                    for(int q = 0; q < 9; ++q)
-                        A[u][q] = ok[u] ? fvals[q * fstride + k] : VT(0);
-                }
-#pragma unroll
-                for(int u = 0; u < 2; ++u)
-                {
-                    const double x0 = x[3u * jj[u]], x1 = x[3u * jj[u] + 1], x2 = x[3u * jj[u] + 2];
-                    if(ok[u])
+                    if(!fl)
                     {
-                        const int i = (h + u) * kThreads + tid;
-                        w[0][i]     = (double)A[u][0] * x0 + (double)A[u][3] * x1 + (double)A[u][6] * x2;
-                        w[1][i]     = (double)A[u][1] * x0 + (double)A[u][4] * x1 + (double)A[u][7] * x2;
-                        w[2][i]     = (double)A[u][2] * x0 + (double)A[u][5] * x1 + (double)A[u][8] * x2;
+                        v0 += t0;
+                        v1 += t1;
+                        v2 += t2;
                     }
+                    fl |= tf;
                 }
             }
-            __syncthreads();
-            for(int32_t row = row0 + g; row < row1; row += kGroups)
+            // the row continuing from the previous window adds the carry
+            if((hmask & ((2ull << lane) - 1ull)) == 0ull)
             {
-                const int32_t s = fptr[row] - k0, e = fptr[row + 1] - k0;
-                double        acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
-                for(int32_t i = s + lane; i < e; i += kLanes)
-                {
-                    acc0 += w[0][i];
-                    acc1 += w[1][i];
-                    acc2 += w[2][i];
-                }
-#pragma unroll
-                for(int o = kLanes / 2; o > 0; o >>= 1)
-                {
-                    acc0 += __shfl_xor(acc0, o, kWave);
-                    acc1 += __shfl_xor(acc1, o, kWave);
-                    acc2 += __shfl_xor(acc2, o, kWave);
-                }
-                if(lane == 0)
-                {
-                    store_row<ACCUM>(y, row, a, acc0, acc1, acc2);
-                    dot += x[3 * (int64_t)row] * acc0 + x[3 * (int64_t)row + 1] * acc1
-                           + x[3 * (int64_t)row + 2] * acc2;
-                }
+                v0 += carry0;
+                v1 += carry1;
+                v2 += carry2;
             }
-            __syncthreads();  // w is rewritten by the next chunk
-        }
-        else
-        {
-            // a chunk with a very long row: 16 lanes per row straight from memory
-            for(int32_t row = row0 + g; row < row1; row += kGroups)
+            const bool row_end = ok && kb + 1 == fnext;
+            if(row_end)
             {
-                const int32_t s = fptr[row], e = fptr[row + 1];
-                double        acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
-                for(int32_t k = s + lane; k < e; k += kLanes)
+                double* yr = y + 3 * (int64_t)myrow;
+                if constexpr(ACCUM)
                 {
-                    const uint32_t j = (uint32_t)fcols[k];
-                    double         A[9];
-#pragma unroll
-                    for(int q = 0; q < 9; ++q)
-                        A[q] = (double)fvals[q * fstride + k];
-                    const double x0 = x[3u * j], x1 = x[3u * j + 1], x2 = x[3u * j + 2];
-                    acc0 += A[0] * x0 + A[3] * x1 + A[6] * x2;
-                    acc1 += A[1] * x0 + A[4] * x1 + A[7] * x2;
-                    acc2 += A[2] * x0 + A[5] * x1 + A[8] * x2;
+                    yr[0] += a * v0;
+                    yr[1] += a * v1;
+                    yr[2] += a * v2;
                 }
-#pragma unroll
-                for(int o = kLanes / 2; o > 0; o >>= 1)
+                else
                 {
-                    acc0 += __shfl_xor(acc0, o, kWave);
-                    acc1 += __shfl_xor(acc1, o, kWave);
-                    acc2 += __shfl_xor(acc2, o, kWave);
+                    yr[0] = v0;
+                    yr[1] = v1;
+                    yr[2] = v2;
                 }
-                if(lane == 0)
-                {
-                    store_row<ACCUM>(y, row, a, acc0, acc1, acc2);
-                    dot += x[3 * (int64_t)row] * acc0 + x[3 * (int64_t)row + 1] * acc1
-                           + x[3 * (int64_t)row + 2] * acc2;
-                }
+                dot += x[3 * (int64_t)myrow] * v0 + x[3 * (int64_t)myrow + 1] * v1 + x[3 * (int64_t)myrow + 2] * v2;
             }
+            // lane 63 hands its row on: either the row continues (carry) or the
+            // next window starts a new row
+            const int  end63 = __shfl(row_end ? 1 : 0, kWave - 1, kWave);
+            const int  row63 = __shfl(myrow, kWave - 1, kWave);
+            const bool cont  = end63 == 0;
+            carry0           = cont ? __shfl(v0, kWave - 1, kWave) : 0.0;
+            carry1           = cont ? __shfl(v1, kWave - 1, kWave) : 0.0;
+            carry2           = cont ? __shfl(v2, kWave - 1, kWave) : 0.0;
+            row              = cont ? row63 : row63 + 1;
         }
     }
     if(partials)
@@ -216,8 +196,8 @@ template <typename VT, bool ACCUM>
 void launch_spmv(int g, hipStream_t s, const BsrMatrix& A, const VT* vals, const double* x, double* y, double a,
                  double* partials, const int32_t* d_conv)
 {
-    spmv_chunk_kernel<VT, ACCUM><<<g, kThreads, 0, s>>>(A.nchunks, A.chunk_row.data(), A.full_ptr.data(),
-                                                        A.fcols.data(), vals, A.fstride, x, y, a, partials, d_conv);
+    spmv_seg_kernel<VT, ACCUM><<<g, kThreads, 0, s>>>(A.nchunks, A.chunk_row.data(), A.full_ptr.data(),
+                                                      A.fcols.data(), vals, A.fstride, x, y, a, partials, d_conv);
 }
 
 void check_matrix(const BsrMatrix& A, bool use32)
@@ -241,8 +221,8 @@ void spmv_prepare(SpmvScratch& scr, bool use32)
             per_cu = std::max(1, std::min(32, std::atoi(e)));
         if(per_cu == 0)
         {
-            const void* fn = use32 ? (const void*)spmv_chunk_kernel<float, false>
-                                   : (const void*)spmv_chunk_kernel<double, false>;
+            const void* fn = use32 ? (const void*)spmv_seg_kernel<float, false>
+                                   : (const void*)spmv_seg_kernel<double, false>;
             MI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kThreads, 0));
             per_cu = std::max(1, per_cu);
         }
@@ -255,7 +235,7 @@ void spmv_prepare(SpmvScratch& scr, bool use32)
 
 int spmv_grid(const SpmvScratch& scr, const BsrMatrix& A)
 {
-    return (int)std::max<int64_t>(1, std::min<int64_t>(scr.grid, A.nchunks));
+    return (int)std::max<int64_t>(1, std::min<int64_t>(scr.grid, (A.nchunks + kWaves - 1) / kWaves));
 }
 
 int spmv_full(const BsrMatrix& A, bool use32, const double* x, double* y, double* partials, const int32_t* d_conv,

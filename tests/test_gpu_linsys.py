@@ -143,6 +143,44 @@ def test_spmv_dot(ctx, c1_contact, precision):
         assert np.abs(wd.numpy() - (-2.0 * yref + 0.5 * w)).max() <= 1e-12 * np.abs(yref).max()
 
 
+@pytest.mark.parametrize("precision", [abi.MI_FP64, abi.MI_FP32])
+def test_spmv_irregular_rows(ctx, precision):
+    """Rows the SpMV's 64-block windows must get right: runs of one-block rows
+    (64 row starts in one window), empty rows (a zero block is stored for
+    them), a row of 1,900 blocks that carries across many windows, and short
+    random rows; checked against the oracle's expansion of U + U^T - D."""
+    rng = np.random.default_rng(11)
+    B = 3000
+    rows, cols = [np.arange(1000)], [np.arange(1000)]          # one-block rows 0..999
+    rows.append(np.full(1900, 1100)); cols.append(np.arange(1100, 3000))  # long row 1100
+    r = rng.integers(1101, 3000, 9000)
+    c = np.minimum(B - 1, r + rng.integers(0, 40, 9000))
+    rows.append(r); cols.append(c)                               # rows 1000..1099 stay empty
+    rows = np.concatenate(rows).astype(np.int32)
+    cols = np.concatenate(cols).astype(np.int32)
+    vals = rng.standard_normal((rows.size, 9))
+    ls = LinearSystem(ctx, B, precision)
+    rd, cd, vd = ctx.to_device(rows), ctx.to_device(cols), ctx.to_device(vals)
+    ls.assemble(rd, cd, vd)
+    orr, oc, ov, _ = O.convert_sym(B, rows, cols, vals)
+    x = rng.standard_normal(3 * B)
+    yref, dref = O.spmv_sym(B, orr, oc, ov, x)
+    xd = ctx.to_device(x)
+    yd = ctx.alloc(3 * B, np.float64)
+    d = ls.spmv_dot(xd, yd)
+    y = yd.numpy()
+    tol = 1e-13 if precision == abi.MI_FP64 else 1e-6
+    assert np.abs(y - yref).max() <= tol * np.abs(yref).max()
+    assert not y[3000:3300].any()                   # the empty rows
+    assert d == pytest.approx(dref, rel=tol * 10)
+    if precision == abi.MI_FP64:
+        w = rng.standard_normal(3 * B)
+        wd = ctx.to_device(w)
+        ls.spmv(1.5, xd, -1.0, wd)
+        assert np.abs(wd.numpy() - (1.5 * yref - w)).max() <= 1e-12 * np.abs(yref).max()
+    ls.close()
+
+
 def test_spmv_symmetry_full_size(ctx):
     """size-independent property at 64^3 + contacts: x^T (A w) == w^T (A x)"""
     s = synth.fem_system(64, 200_000, seed=2)
