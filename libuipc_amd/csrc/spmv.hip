@@ -10,9 +10,9 @@
 // the full BSR (both triangles, BsrMatrix::full_*; every row holds at least
 // one block, an empty row gets a zero diagonal block) and the SpMV is a row
 // gather over it, streamed as one flat sequence of blocks:
-//   * each wave owns a run of whole rows (the chunks of the assembly: rows
-//     whose first block lies in [c S, (c+1) S), kChunkStep S) and walks their
-//     blocks in windows of 64, lane l on block k + l, so every value plane is
+//   * work comes in chunks of whole rows (the rows whose first block lies in
+//     [c S, (c+1) S), kChunkStep S); a wave walks a chunk's blocks in windows
+//     of 64, lane l on block k + l, so every value plane is
 //     read as 64 contiguous entries per instruction (the rate of a plain
 //     stream; 16 lanes per row of ~40 blocks reached ~60% of it);
 //   * a lane's row is found by a binary search over the 64 row starts that
@@ -25,9 +25,12 @@
 //     x_row . (A x)_row to the wave's x^T A x; workgroup partials go to
 //     partials[blockIdx.x], which the consumer kernel (PCG update) sums in a
 //     fixed order;
-//   * a persistent grid; workgroups are renumbered so that each XCD (blocks
-//     are dispatched round-robin over the 8 XCDs) walks one contiguous eighth
-//     of the rows and keeps its part of x in its own L2.
+//   * a persistent grid; each XCD (workgroups are dispatched round-robin over
+//     the 8 XCDs) takes one contiguous eighth of the chunks and deals them to
+//     its waves round-robin, so the rows in flight form a narrow band that
+//     sweeps the eighth and x is re-read from the XCD's L2, not from HBM;
+//   * each wave loads its next window while the current one's x gathers are
+//     outstanding (double buffer).
 #include <cstdlib>
 
 #include "linsys.hpp"
@@ -39,6 +42,30 @@ namespace
 constexpr int kThreads = 256;
 constexpr int kWaves   = kThreads / kWave;
 constexpr int kXcds    = 8;
+
+template <typename VT>
+struct Window
+{
+    uint32_t j;
+    VT       A[9];
+};
+
+template <typename VT>
+__device__ __forceinline__ void load_window(Window<VT>& w,
+                                            int32_t     k,
+                                            int32_t     kEnd,
+                                            int         lane,
+                                            const int32_t* __restrict__ fcols,
+                                            const VT* __restrict__ fvals,
+                                            int64_t fstride)
+{
+    const int32_t kb = k + lane;
+    const bool    ok = kb < kEnd;
+    w.j              = ok ? (uint32_t)fcols[kb] : 0u;
+#pragma unroll
+    for(int q = 0; q < 9; ++q)
+        w.A[q] = ok ? fvals[q * fstride + kb] : VT(0);
+}
 
 template <typename VT, bool ACCUM>
 __global__ __launch_bounds__(kThreads) void spmv_seg_kernel(int32_t        nchunks,
@@ -57,37 +84,61 @@ __global__ __launch_bounds__(kThreads) void spmv_seg_kernel(int32_t        nchun
         return;
     __shared__ double lds[kThreads / kWave];
 
-    const int G   = gridDim.x;
-    const int b   = blockIdx.x;
-    const int per = G / kXcds;
-    const int lb  = (per > 0 && b < per * kXcds) ? (b % kXcds) * per + b / kXcds : b;
-    const int W   = G * kWaves;
-    const int wv  = lb * kWaves + (int)(threadIdx.x / kWave);
-    const int c0  = (int)((int64_t)nchunks * wv / W);
-    const int c1  = (int)((int64_t)nchunks * (wv + 1) / W);
+    // chunks of one XCD are dealt to its waves round-robin, so the rows in
+    // flight on an XCD form a narrow band whose slice of x stays in its L2
+    const int G    = gridDim.x;
+    const int b    = blockIdx.x;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
+    const int nx   = (G % kXcds == 0) ? kXcds : 1;
+    const int xcd  = b % nx;
+    const int Wx   = (G / nx) * kWaves;
+    const int t    = (b / nx) * kWaves + wave;
+    const int cx1  = (int)((int64_t)nchunks * (xcd + 1) / nx);
+    int       c    = (int)((int64_t)nchunks * xcd / nx) + t;
 
     const int lane = (int)(threadIdx.x % kWave);
     double    dot  = 0.0;
-    if(c0 < c1)
+    if(c < cx1)
     {
-        int32_t       row    = chunk_row[c0];
-        const int32_t rowEnd = chunk_row[c1];
-        const int32_t kEnd   = fptr[rowEnd];
-        double        carry0 = 0.0, carry1 = 0.0, carry2 = 0.0;
-        for(int32_t k = fptr[row]; k < kEnd; k += kWave)
+        int32_t row    = chunk_row[c];
+        int32_t rowEnd = chunk_row[c + 1];
+        int32_t k      = fptr[row];
+        int32_t kEnd   = fptr[rowEnd];
+        double  carry0 = 0.0, carry1 = 0.0, carry2 = 0.0;
+        // double buffer: the next window's columns and values are loaded while
+        // this window's x gathers are outstanding, so a window costs one memory
+        // round trip instead of two (columns, then x)
+        Window<VT> nw;
+        load_window<VT>(nw, k, kEnd, lane, fcols, fvals, fstride);
+        for(;;)
         {
-            const int32_t kb = k + lane;
-            const bool    ok = kb < kEnd;
-            // this window's blocks: consecutive lanes on consecutive blocks
-            const uint32_t j = ok ? (uint32_t)fcols[kb] : 0u;
-            VT             A[9];
-#pragma unroll
-            for(int q = 0; q < 9; ++q)
-                A[q] = ok ? fvals[q * fstride + kb] : VT(0);
-            // starts of the rows that can meet the window (every row holds a
-            // block, so at most 64 of them); past rowEnd the start is kEnd
-            const int32_t f = fptr[min(row + lane, rowEnd)];
-            const double  x0 = x[3u * j], x1 = x[3u * j + 1], x2 = x[3u * j + 2];
+            const Window<VT> cw = nw;
+            const int32_t    kb = k + lane;
+            const bool       ok = kb < kEnd;
+            // x gathers and the starts of the rows that can meet the window
+            // (every row holds a block, so at most 64 of them; past rowEnd the
+            // start is kEnd)
+            const double  x0 = x[3u * cw.j], x1 = x[3u * cw.j + 1], x2 = x[3u * cw.j + 2];
+            const int32_t f  = fptr[min(row + lane, rowEnd)];
+            // where the next window is: further on in this chunk, or the start
+            // of the wave's next chunk
+            const bool same = k + kWave < kEnd;
+            int32_t    nrow = 0, nrowEnd = 0, nk = 0, nkEnd = 0;
+            const int  cn   = c + Wx;
+            if(same)
+            {
+                nk    = k + kWave;
+                nkEnd = kEnd;
+            }
+            else if(cn < cx1)
+            {
+                nrow    = chunk_row[cn];
+                nrowEnd = chunk_row[cn + 1];
+                nk      = fptr[nrow];
+                nkEnd   = fptr[nrowEnd];
+            }
+            if(same || cn < cx1)
+                load_window<VT>(nw, nk, nkEnd, lane, fcols, fvals, fstride);
             // row of block kb: the last i with f_i <= kb
             int idx = 0;
 #pragma unroll
@@ -106,9 +157,9 @@ __global__ __launch_bounds__(kThreads) void spmv_seg_kernel(int32_t        nchun
             double     v0 = 0.0, v1 = 0.0, v2 = 0.0;
             if(ok)
             {
-                v0 = (double)A[0] * x0 + (double)A[3] * x1 + (double)A[6] * x2;
-                v1 = (double)A[1] * x0 + (double)A[4] * x1 + (double)A[7] * x2;
-                v2 = (double)A[2] * x0 + (double)A[5] * x1 + (double)A[8] * x2;
+                v0 = (double)cw.A[0] * x0 + (double)cw.A[3] * x1 + (double)cw.A[6] * x2;
+                v1 = (double)cw.A[1] * x0 + (double)cw.A[4] * x1 + (double)cw.A[7] * x2;
+                v2 = (double)cw.A[2] * x0 + (double)cw.A[5] * x1 + (double)cw.A[8] * x2;
             }
             // segmented inclusive scan: each lane ends with the sum of its row's
             // blocks from the row's head (or lane 0) up to itself
@@ -155,15 +206,31 @@ __global__ __launch_bounds__(kThreads) void spmv_seg_kernel(int32_t        nchun
                 }
                 dot += x[3 * (int64_t)myrow] * v0 + x[3 * (int64_t)myrow + 1] * v1 + x[3 * (int64_t)myrow + 2] * v2;
             }
-            // lane 63 hands its row on: either the row continues (carry) or the
-            // next window starts a new row
-            const int  end63 = __shfl(row_end ? 1 : 0, kWave - 1, kWave);
-            const int  row63 = __shfl(myrow, kWave - 1, kWave);
-            const bool cont  = end63 == 0;
-            carry0           = cont ? __shfl(v0, kWave - 1, kWave) : 0.0;
-            carry1           = cont ? __shfl(v1, kWave - 1, kWave) : 0.0;
-            carry2           = cont ? __shfl(v2, kWave - 1, kWave) : 0.0;
-            row              = cont ? row63 : row63 + 1;
+            if(same)
+            {
+                // lane 63 hands its row on: either the row continues (carry)
+                // or the next window starts the next row
+                const int  end63 = __shfl(row_end ? 1 : 0, kWave - 1, kWave);
+                const int  row63 = __shfl(myrow, kWave - 1, kWave);
+                const bool cont  = end63 == 0;
+                carry0           = cont ? __shfl(v0, kWave - 1, kWave) : 0.0;
+                carry1           = cont ? __shfl(v1, kWave - 1, kWave) : 0.0;
+                carry2           = cont ? __shfl(v2, kWave - 1, kWave) : 0.0;
+                row              = cont ? row63 : row63 + 1;
+                k                = nk;
+            }
+            else
+            {
+                // a chunk ends with a whole row: nothing carries over
+                if(cn >= cx1)
+                    break;
+                c      = cn;
+                row    = nrow;
+                rowEnd = nrowEnd;
+                k      = nk;
+                kEnd   = nkEnd;
+                carry0 = carry1 = carry2 = 0.0;
+            }
         }
     }
     if(partials)
