@@ -272,11 +272,15 @@ typedef struct mi_pcg_opts
     int32_t criterion;      /* mi_pcg_criterion                                            */
     double  tol;            /* reference default tol_rate = 1e-3 (scene_default_config.cpp:146) */
     double  max_iter_ratio; /* max_iter = ratio * n (reference: 2)                         */
-    int32_t check_interval; /* iterations between host checks (reference: 5)               */
-    int32_t use_graph;      /* 1: replay a captured hipGraph of check_interval iterations   */
+    int32_t check_interval; /* iterations between convergence checks (reference: 5)        */
+    int32_t use_graph;      /* 1: replay a captured hipGraph of graph_checks check intervals */
     int32_t psd_fallback;   /* 1: rz0 < 0 or non-finite -> plain CG (linear_fused_pcg.cu:1843-1875) */
     int32_t force_bypass;   /* 1: always bypass the preconditioner (UIPC_PCG_PSD_FALLBACK_TEST) */
     int32_t true_residual;  /* 1: compute ||b - A x|| / ||b|| after the solve (one SpMV)    */
+    int32_t graph_checks;   /* check intervals per host round trip (0: 4).  The device tests
+                               convergence every iteration and idles once converged, so the
+                               reported counts do not depend on it; larger values cut host
+                               round trips and cost at most that many idle intervals.      */
 } mi_pcg_opts;
 
 typedef struct mi_pcg_stats

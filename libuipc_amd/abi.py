@@ -67,6 +67,7 @@ class PcgOpts(C.Structure):
         ("psd_fallback", C.c_int32),
         ("force_bypass", C.c_int32),
         ("true_residual", C.c_int32),
+        ("graph_checks", C.c_int32),
     ]
 
 

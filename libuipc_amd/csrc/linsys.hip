@@ -666,6 +666,7 @@ int mi_pcg_default_opts(mi_pcg_opts* o)
     o->psd_fallback   = 1;
     o->force_bypass   = 0;
     o->true_residual  = 0;
+    o->graph_checks   = 4;
     MI_API_END
 }
 }  // extern "C"
@@ -793,6 +794,7 @@ extern "C" int mi_pcg_solve(mi_sys* sys, const double* b, double* x, const mi_pc
     const int64_t max_iter = (int64_t)(o.max_iter_ratio * (double)n);
     const int64_t total    = max_iter > 0 ? max_iter - 1 : 0;
     const int     ci       = o.check_interval > 0 ? o.check_interval : 1;
+    const int     gb       = ci * (o.graph_checks > 0 ? o.graph_checks : 4);  // iterations per host round trip
     int64_t       done     = 0;
     bool          converged = false;
 
@@ -801,7 +803,7 @@ extern "C" int mi_pcg_solve(mi_sys* sys, const double* b, double* x, const mi_pc
     key.b       = b;
     key.n       = n;
     key.U       = sys->A.U;
-    key.ci      = ci;
+    key.ci      = gb;
     key.fused   = fused ? 1 : 0;
     key.bypass  = bypass ? 1 : 0;
     key.use32   = sys->desc.value_precision;
@@ -818,8 +820,8 @@ extern "C" int mi_pcg_solve(mi_sys* sys, const double* b, double* x, const mi_pc
 
     while(done < total)
     {
-        int64_t blk       = std::min<int64_t>(ci, total - done);
-        bool    use_graph = o.use_graph && !sys->graph_failed && blk == ci && s != nullptr;
+        int64_t blk       = std::min<int64_t>(gb, total - done);
+        bool    use_graph = o.use_graph && !sys->graph_failed && blk == gb && s != nullptr;
         if(use_graph && !(sys->graph_exec && sys->graph_key == key))
         {
             if(sys->graph_exec)
@@ -831,7 +833,7 @@ extern "C" int mi_pcg_solve(mi_sys* sys, const double* b, double* x, const mi_pc
             hipError_t e    = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
             if(e == hipSuccess)
             {
-                for(int k = 0; k < ci; ++k)
+                for(int k = 0; k < gb; ++k)
                     run_iteration(sys, x, bypass, fused, s);
                 e = hipStreamEndCapture(s, &sys->graph);
                 if(e == hipSuccess)
@@ -863,7 +865,8 @@ extern "C" int mi_pcg_solve(mi_sys* sys, const double* b, double* x, const mi_pc
             break;
         }
     }
-    st->iters       = converged ? done : max_iter;
+    // the reference stops at the first check (every ci iterations) that passes
+    st->iters       = converged ? std::min<int64_t>(((int64_t)h.conv_iter + ci - 1) / ci * ci, done) : max_iter;
     st->iters_exact = converged ? h.conv_iter : -1;
     st->converged   = converged ? 1 : 0;
     st->rz_final    = h.rz_new;
