@@ -32,6 +32,7 @@
 //   * each wave loads its next window while the current one's x gathers are
 //     outstanding (double buffer).
 #include <cstdlib>
+#include <string>
 
 #include "linsys.hpp"
 
@@ -241,6 +242,210 @@ __global__ __launch_bounds__(kThreads) void spmv_seg_kernel(int32_t        nchun
     }
 }
 
+// ---- front variant: the 64-block windows of the whole matrix are dealt
+// round-robin to all waves, so the blocks in flight form one contiguous front
+// (the access pattern of a plain stream).  A window's rows that start and end
+// in it are stored at once; a row crossing windows leaves its head part (S,
+// the segment from its first block to the window end) and every later window
+// its part (P, flagged when the row ends there), and spmv_fixup_kernel adds
+// them up in window order.
+enum : int32_t
+{
+    kWinHasP  = 1,  // the window's first segment belongs to a row begun earlier
+    kWinPEnds = 2,  // ... and that row ends in this window
+    kWinHasS  = 4   // a row begun in this window runs past its end
+};
+
+template <bool ACCUM>
+__device__ __forceinline__ void put_row(double* y, int64_t row, double a, double v0, double v1, double v2)
+{
+    double* yr = y + 3 * row;
+    if constexpr(ACCUM)
+    {
+        yr[0] += a * v0;
+        yr[1] += a * v1;
+        yr[2] += a * v2;
+    }
+    else
+    {
+        yr[0] = v0;
+        yr[1] = v1;
+        yr[2] = v2;
+    }
+}
+
+template <typename VT, bool ACCUM>
+__global__ __launch_bounds__(kThreads) void spmv_front_kernel(int32_t        nwin,
+                                                             int32_t        B,
+                                                             const int32_t* __restrict__ win_row,
+                                                             const int32_t* __restrict__ fptr,
+                                                             const int32_t* __restrict__ fcols,
+                                                             const VT* __restrict__ fvals,
+                                                             int64_t       fstride,
+                                                             const double* __restrict__ x,
+                                                             double*       y,
+                                                             double        a,
+                                                             double*       P,
+                                                             double*       S,
+                                                             int32_t*      srow,
+                                                             int32_t*      flags,
+                                                             double*       partials,
+                                                             const int32_t* d_conv)
+{
+    if(d_conv && *d_conv != 0)
+        return;
+    __shared__ double lds[kThreads / kWave];
+    const int     lane = (int)(threadIdx.x % kWave);
+    const int     wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
+    const int64_t W    = (int64_t)gridDim.x * kWaves;
+    const int32_t nnz  = fptr[B];
+    double        dot  = 0.0;
+    for(int64_t w = (int64_t)blockIdx.x * kWaves + wave; w < nwin; w += W)
+    {
+        const int32_t k = (int32_t)(w * kWave);
+        if(k >= nnz)
+            break;
+        const int32_t  kb = k + lane;
+        const bool     ok = kb < nnz;
+        const uint32_t j  = ok ? (uint32_t)fcols[kb] : 0u;
+        VT             A[9];
+#pragma unroll
+        for(int q = 0; q < 9; ++q)
+            A[q] = ok ? fvals[q * fstride + kb] : VT(0);
+        const int32_t row = win_row[w];
+        const int32_t f   = fptr[min(row + lane, B)];
+        const double  x0 = x[3u * j], x1 = x[3u * j + 1], x2 = x[3u * j + 2];
+        int           idx = 0;
+#pragma unroll
+        for(int step = kWave / 2; step > 0; step >>= 1)
+        {
+            const int32_t fc = __shfl(f, idx + step, kWave);
+            if(fc <= kb)
+                idx += step;
+        }
+        const int32_t fcur  = __shfl(f, idx, kWave);
+        int32_t       fnext = __shfl(f, min(idx + 1, kWave - 1), kWave);
+        const int32_t myrow = row + idx;
+        if(idx == kWave - 1)
+            fnext = fptr[min(myrow + 1, B)];
+        const bool head = ok && fcur == kb;
+        double     v0 = 0.0, v1 = 0.0, v2 = 0.0;
+        if(ok)
+        {
+            v0 = (double)A[0] * x0 + (double)A[3] * x1 + (double)A[6] * x2;
+            v1 = (double)A[1] * x0 + (double)A[4] * x1 + (double)A[7] * x2;
+            v2 = (double)A[2] * x0 + (double)A[5] * x1 + (double)A[8] * x2;
+        }
+        const uint64_t hm = __ballot(head);
+        const uint64_t bm = __ballot(head || !ok);
+        int            fl = (head || !ok) ? 1 : 0;
+#pragma unroll
+        for(int d = 1; d < kWave; d <<= 1)
+        {
+            const double t0 = __shfl_up(v0, d, kWave), t1 = __shfl_up(v1, d, kWave), t2 = __shfl_up(v2, d, kWave);
+            const int    tf = __shfl_up(fl, d, kWave);
+            if(lane >= d)
+            {
+                if(!fl)
+                {
+                    v0 += t0;
+                    v1 += t1;
+                    v2 += t2;
+                }
+                fl |= tf;
+            }
+        }
+        const bool first_seg = (bm & ((2ull << lane) - 1ull)) == 0ull;  // row begun before the window
+        const bool row_end   = ok && kb + 1 == fnext;
+        if(row_end && !first_seg)
+        {
+            put_row<ACCUM>(y, myrow, a, v0, v1, v2);
+            dot += x[3 * (int64_t)myrow] * v0 + x[3 * (int64_t)myrow + 1] * v1 + x[3 * (int64_t)myrow + 2] * v2;
+        }
+        const int      last_valid = min(kWave - 1, nnz - 1 - k);
+        const uint64_t rem        = bm & ~1ull;
+        const int      fend       = rem ? (__builtin_ctzll(rem) - 1) : last_valid;  // end of the first segment
+        const int      fend_row   = __shfl(row_end ? 1 : 0, fend, kWave);
+        const int      last_end   = __shfl(row_end ? 1 : 0, last_valid, kWave);
+        int            flg        = 0;
+        if(!(hm & 1ull))
+        {
+            flg |= kWinHasP | (fend_row ? kWinPEnds : 0);
+            if(lane == fend)
+            {
+                P[3 * w]     = v0;
+                P[3 * w + 1] = v1;
+                P[3 * w + 2] = v2;
+            }
+        }
+        if(!last_end && hm != 0ull && last_valid == kWave - 1)
+        {
+            flg |= kWinHasS;
+            if(lane == kWave - 1)
+            {
+                S[3 * w]     = v0;
+                S[3 * w + 1] = v1;
+                S[3 * w + 2] = v2;
+                srow[w]      = myrow;
+            }
+        }
+        if(lane == 0)
+            flags[w] = flg;
+    }
+    if(partials)
+    {
+        double bs = block_sum<kThreads>(dot, lds);
+        if(threadIdx.x == 0)
+            partials[blockIdx.x] = bs;
+    }
+}
+
+// rows that cross windows: S of the window where the row begins plus P of every
+// later window up to the one where it ends, in window order
+template <bool ACCUM>
+__global__ __launch_bounds__(kThreads) void spmv_fixup_kernel(int32_t        nwin,
+                                                             int32_t        B,
+                                                             const int32_t* __restrict__ fptr,
+                                                             const double* __restrict__ P,
+                                                             const double* __restrict__ S,
+                                                             const int32_t* __restrict__ srow,
+                                                             const int32_t* __restrict__ flags,
+                                                             const double* __restrict__ x,
+                                                             double*       y,
+                                                             double        a,
+                                                             double*       partials,
+                                                             const int32_t* d_conv)
+{
+    if(d_conv && *d_conv != 0)
+        return;
+    __shared__ double lds[kThreads / kWave];
+    const int32_t nw  = (int32_t)std::min<int64_t>(nwin, ((int64_t)fptr[B] + kWave - 1) / kWave);
+    double        dot = 0.0;
+    for(int32_t w = blockIdx.x * kThreads + threadIdx.x; w < nw; w += gridDim.x * kThreads)
+    {
+        if(!(flags[w] & kWinHasS))
+            continue;
+        double s0 = S[3 * (int64_t)w], s1 = S[3 * (int64_t)w + 1], s2 = S[3 * (int64_t)w + 2];
+        for(int32_t u = w + 1; u < nw; ++u)
+        {
+            s0 += P[3 * (int64_t)u];
+            s1 += P[3 * (int64_t)u + 1];
+            s2 += P[3 * (int64_t)u + 2];
+            if(flags[u] & kWinPEnds)
+                break;
+        }
+        const int64_t r = srow[w];
+        put_row<ACCUM>(y, r, a, s0, s1, s2);
+        dot += x[3 * r] * s0 + x[3 * r + 1] * s1 + x[3 * r + 2] * s2;
+    }
+    if(partials)
+    {
+        double bs = block_sum<kThreads>(dot, lds);
+        if(threadIdx.x == 0)
+            partials[blockIdx.x] = bs;
+    }
+}
+
 __global__ void scale_kernel(double* y, double b, int64_t n)
 {
     for(int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -259,12 +464,31 @@ __global__ __launch_bounds__(kThreads) void sum_partials_kernel(const double* pa
         *out = acc;
 }
 
-template <typename VT, bool ACCUM>
-void launch_spmv(int g, hipStream_t s, const BsrMatrix& A, const VT* vals, const double* x, double* y, double a,
-                 double* partials, const int32_t* d_conv)
+int fix_grid(const BsrMatrix& A)
 {
+    return std::max(1, std::min(ceil_div(A.nwin, kThreads), device_cu_count() * 4));
+}
+
+// launches the SpMV; returns how many partials it wrote from partials[0]
+template <typename VT, bool ACCUM>
+int launch_spmv(const SpmvScratch& scr, int g, hipStream_t s, const BsrMatrix& A, const VT* vals, const double* x,
+                double* y, double a, double* partials, const int32_t* d_conv)
+{
+    if(scr.front)
+    {
+        const int gf = fix_grid(A);
+        spmv_front_kernel<VT, ACCUM><<<g, kThreads, 0, s>>>(A.nwin, A.B, A.win_row.data(), A.full_ptr.data(),
+                                                            A.fcols.data(), vals, A.fstride, x, y, a, A.win_P.data(),
+                                                            A.win_S.data(), A.win_srow.data(), A.win_flags.data(),
+                                                            partials, d_conv);
+        spmv_fixup_kernel<ACCUM><<<gf, kThreads, 0, s>>>(A.nwin, A.B, A.full_ptr.data(), A.win_P.data(),
+                                                         A.win_S.data(), A.win_srow.data(), A.win_flags.data(), x, y,
+                                                         a, partials ? partials + g : nullptr, d_conv);
+        return g + gf;
+    }
     spmv_seg_kernel<VT, ACCUM><<<g, kThreads, 0, s>>>(A.nchunks, A.chunk_row.data(), A.full_ptr.data(),
                                                       A.fcols.data(), vals, A.fstride, x, y, a, partials, d_conv);
+    return g;
 }
 
 void check_matrix(const BsrMatrix& A, bool use32)
@@ -281,6 +505,9 @@ void spmv_prepare(SpmvScratch& scr, bool use32)
     if(scr.grid == 0 || scr.use32 != use32)
     {
         scr.use32 = use32;
+        // MI_SPMV_KERNEL=front selects the whole-matrix front variant
+        const char* kv = std::getenv("MI_SPMV_KERNEL");
+        scr.front      = kv && std::string(kv) == "front";
         // a persistent grid of as many workgroups per CU as fit at once (the
         // kernel's occupancy); MI_SPMV_BLOCKS_PER_CU overrides for sweeps
         int per_cu = 0;
@@ -295,7 +522,8 @@ void spmv_prepare(SpmvScratch& scr, bool use32)
         }
         int grid = device_cu_count() * per_cu;
         grid     = std::max(kXcds, (grid / kXcds) * kXcds);
-        scr.partials.resize((size_t)2 * grid);
+        // the front variant's fix-up adds up to 4 partials per CU after the grid's
+        scr.partials.resize((size_t)2 * grid + 4 * device_cu_count());
         scr.grid = grid;
     }
 }
@@ -316,12 +544,10 @@ int spmv_full(const BsrMatrix& A, bool use32, const double* x, double* y, double
     MI_REQUIRE(x && y && partials, MI_ERR_STATE, "spmv_full: null buffer");
     MI_REQUIRE(partials >= scr.partials.data() && partials + g <= scr.partials.data() + scr.partials.size(),
                MI_ERR_STATE, "spmv_full: partials outside the scratch");
-    if(use32)
-        launch_spmv<float, false>(g, s, A, A.fvals32.data(), x, y, 1.0, partials, d_conv);
-    else
-        launch_spmv<double, false>(g, s, A, A.fvals.data(), x, y, 1.0, partials, d_conv);
+    const int np = use32 ? launch_spmv<float, false>(scr, g, s, A, A.fvals32.data(), x, y, 1.0, partials, d_conv)
+                         : launch_spmv<double, false>(scr, g, s, A, A.fvals.data(), x, y, 1.0, partials, d_conv);
     MI_HIP(hipGetLastError());
-    return g;
+    return np;
 }
 
 void spmv_accumulate_dot(const BsrMatrix& A, bool use32, double a, const double* x, double* y, double* d_dot,
@@ -337,12 +563,10 @@ void spmv_accumulate_dot(const BsrMatrix& A, bool use32, double a, const double*
     check_matrix(A, use32);
     const int g = spmv_grid(scr, A);
     double*   p = d_dot ? scr.partials.data() : nullptr;
-    if(use32)
-        launch_spmv<float, true>(g, s, A, A.fvals32.data(), x, y, a, p, nullptr);
-    else
-        launch_spmv<double, true>(g, s, A, A.fvals.data(), x, y, a, p, nullptr);
+    const int np = use32 ? launch_spmv<float, true>(scr, g, s, A, A.fvals32.data(), x, y, a, p, nullptr)
+                         : launch_spmv<double, true>(scr, g, s, A, A.fvals.data(), x, y, a, p, nullptr);
     if(d_dot)
-        sum_partials_kernel<<<1, kThreads, 0, s>>>(scr.partials.data(), g, d_dot);
+        sum_partials_kernel<<<1, kThreads, 0, s>>>(scr.partials.data(), np, d_dot);
     MI_HIP(hipGetLastError());
 }
 
