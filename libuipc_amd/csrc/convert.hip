@@ -512,32 +512,6 @@ __global__ void chunk_row_kernel(const int32_t* __restrict__ fptr, int32_t B, in
 
 // blocks per full-BSR row (lower + upper; an empty row gets one zero diagonal
 // block, so every row holds at least one), scanned into full_ptr afterwards
-// win_row[w] = the row holding block 64 w (the last r with full_ptr[r] <= 64 w);
-// windows at or past the last block get -1
-__global__ void win_row_kernel(const int32_t* __restrict__ fptr, int32_t B, int32_t nwin, int32_t* win_row)
-{
-    const int64_t nnz = fptr[B];
-    for(int32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < nwin; w += gridDim.x * blockDim.x)
-    {
-        const int64_t k = (int64_t)w * 64;
-        if(k >= nnz)
-        {
-            win_row[w] = -1;
-            continue;
-        }
-        int32_t lo = 0, hi = B - 1;  // last r in [0, B) with fptr[r] <= k
-        while(lo < hi)
-        {
-            const int32_t mid = lo + (hi - lo + 1) / 2;
-            if((int64_t)fptr[mid] <= k)
-                lo = mid;
-            else
-                hi = mid - 1;
-        }
-        win_row[w] = lo;
-    }
-}
-
 __global__ void full_count_kernel(const int32_t* lower_ptr, const int32_t* upper_ptr, uint32_t* cnt, int32_t B)
 {
     for(int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= B; i += (int64_t)gridDim.x * blockDim.x)
@@ -782,13 +756,6 @@ void MatrixConverter::convert_sym(int32_t        B,
     out.chunk_row.resize((size_t)out.nchunks + 1, ratio);
     chunk_row_kernel<<<grid_of(out.nchunks + 1), kThreads, 0, s>>>(out.full_ptr.data(), B, out.nchunks,
                                                                    out.chunk_row.data());
-    out.nwin = (int32_t)std::max<int64_t>(1, (fmax + 63) / 64);
-    out.win_row.resize((size_t)out.nwin, ratio);
-    out.win_flags.resize((size_t)out.nwin, ratio);
-    out.win_srow.resize((size_t)out.nwin, ratio);
-    out.win_P.resize((size_t)3 * out.nwin, ratio);
-    out.win_S.resize((size_t)3 * out.nwin, ratio);
-    win_row_kernel<<<grid_of(out.nwin), kThreads, 0, s>>>(out.full_ptr.data(), B, out.nwin, out.win_row.data());
     out.fcols.resize((size_t)std::max<int64_t>(1, fmax), ratio);
     out.fstride = (int64_t)out.fcols.cap;
     if(want32)

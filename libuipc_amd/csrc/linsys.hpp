@@ -37,13 +37,6 @@ struct BsrMatrix
     // whose first full-BSR block lies in [c kChunkStep, (c + 1) kChunkStep)
     DBuf<int32_t> chunk_row;  // nchunks + 1
     int32_t       nchunks = 0;
-    // 64-block windows of the full BSR: row holding each window's first block
-    // (windows past the last block are skipped), and the per-window partial
-    // sums of rows that cross windows (filled by the SpMV, used by its fix-up)
-    DBuf<int32_t>         win_row;              // nwin
-    mutable DBuf<int32_t> win_flags, win_srow;  // nwin each (SpMV scratch)
-    mutable DBuf<double>  win_P, win_S;         // 3 nwin each (SpMV scratch)
-    int32_t       nwin = 0;                      // capacity (ceil of the block bound / 64)
 };
 constexpr int kChunkStep = 512;  // nominal blocks per SpMV chunk
 
@@ -89,7 +82,6 @@ struct SpmvScratch
     DBuf<double> partials;
     int          grid  = 0;
     bool         use32 = false;
-    bool         front = false;  // SpMV variant: whole-matrix front + fix-up
 };
 // allocates the SpMV's reduction scratch and fixes its launch shape for one
 // value precision (must run outside graph capture)

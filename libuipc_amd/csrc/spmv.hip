@@ -9,30 +9,25 @@
 // of eight non-coherent XCD L2s serialise badly, so the assembly also builds
 // the full BSR (both triangles, BsrMatrix::full_*; every row holds at least
 // one block, an empty row gets a zero diagonal block) and the SpMV is a row
-// gather over it, streamed as one flat sequence of blocks:
+// gather over it, deterministic and free of atomics:
 //   * work comes in chunks of whole rows (the rows whose first block lies in
-//     [c S, (c+1) S), kChunkStep S); a wave walks a chunk's blocks in windows
-//     of 64, lane l on block k + l, so every value plane is
-//     read as 64 contiguous entries per instruction (the rate of a plain
-//     stream; 16 lanes per row of ~40 blocks reached ~60% of it);
-//   * a lane's row is found by a binary search over the 64 row starts that
-//     can meet the window (fptr[row .. row + 63], held one per lane, read
-//     with shuffles); w = A_k x_col(k) is summed per row by a segmented
-//     inclusive scan over the wave (xor-free shuffle-up steps, fixed order:
-//     deterministic, no LDS, no barrier); a row that runs past the window
-//     carries its partial sum into the next one;
-//   * the lane holding a row's last block stores y (no atomics) and adds
-//     x_row . (A x)_row to the wave's x^T A x; workgroup partials go to
-//     partials[blockIdx.x], which the consumer kernel (PCG update) sums in a
-//     fixed order;
+//     [c S, (c+1) S), kChunkStep S), read with consecutive lanes on
+//     consecutive blocks, so every value plane streams as 64 contiguous
+//     entries per wave instruction (16 lanes per row of ~40 blocks reached
+//     ~60% of that rate);
+//   * FP64 planes (spmv_chunk_kernel): a workgroup stages a chunk's products
+//     w_k = A_k x_col(k) in LDS and sums each row from there, 16 lanes per row;
+//   * FP32 planes (spmv_seg_kernel): a wave walks a chunk in windows of 64
+//     blocks, finds each lane's row by a shuffle binary search over the row
+//     starts, sums rows with a segmented shuffle scan (carrying a row across
+//     windows) and loads the next window while this one's x gathers are
+//     outstanding;
+//   * x^T A x = sum_i x_i . (A x)_i per workgroup into partials[blockIdx.x];
+//     the consumer kernel (PCG update) sums the partials in a fixed order;
 //   * a persistent grid; each XCD (workgroups are dispatched round-robin over
-//     the 8 XCDs) takes one contiguous eighth of the chunks and deals them to
-//     its waves round-robin, so the rows in flight form a narrow band that
-//     sweeps the eighth and x is re-read from the XCD's L2, not from HBM;
-//   * each wave loads its next window while the current one's x gathers are
-//     outstanding (double buffer).
+//     the 8 XCDs) takes a contiguous range of chunks, so x is re-read from its
+//     own L2.
 #include <cstdlib>
-#include <string>
 
 #include "linsys.hpp"
 
@@ -40,9 +35,14 @@ namespace mi
 {
 namespace
 {
-constexpr int kThreads = 256;
-constexpr int kWaves   = kThreads / kWave;
-constexpr int kXcds    = 8;
+constexpr int kThreads  = 256;
+constexpr int kWaves    = kThreads / kWave;
+constexpr int kXcds     = 8;
+constexpr int kLanes    = 16;  // lanes per block row in the chunk kernel's row sums
+constexpr int kGroups   = kThreads / kLanes;
+constexpr int kChunkCap = 2 * kChunkStep;  // blocks a chunk may hold to be staged in LDS
+constexpr int kPer      = kChunkCap / kThreads;
+static_assert(kChunkCap % (2 * kThreads) == 0, "chunk capacity must be a multiple of two workgroups");
 
 template <typename VT>
 struct Window
@@ -242,20 +242,6 @@ __global__ __launch_bounds__(kThreads) void spmv_seg_kernel(int32_t        nchun
     }
 }
 
-// ---- front variant: the 64-block windows of the whole matrix are dealt
-// round-robin to all waves, so the blocks in flight form one contiguous front
-// (the access pattern of a plain stream).  A window's rows that start and end
-// in it are stored at once; a row crossing windows leaves its head part (S,
-// the segment from its first block to the window end) and every later window
-// its part (P, flagged when the row ends there), and spmv_fixup_kernel adds
-// them up in window order.
-enum : int32_t
-{
-    kWinHasP  = 1,  // the window's first segment belongs to a row begun earlier
-    kWinPEnds = 2,  // ... and that row ends in this window
-    kWinHasS  = 4   // a row begun in this window runs past its end
-};
-
 template <bool ACCUM>
 __device__ __forceinline__ void put_row(double* y, int64_t row, double a, double v0, double v1, double v2)
 {
@@ -274,10 +260,14 @@ __device__ __forceinline__ void put_row(double* y, int64_t row, double a, double
     }
 }
 
+// FP64 planes: a workgroup takes whole chunks (rows whose first block lies in
+// [c S, (c+1) S)), loads the chunk's blocks with consecutive lanes on
+// consecutive blocks, stages w_k = A_k x_col(k) in LDS and sums each row from
+// LDS with 16 lanes in a fixed order.  A chunk longer than kChunkCap blocks
+// (a row of more than S blocks) is done row by row straight from memory.
 template <typename VT, bool ACCUM>
-__global__ __launch_bounds__(kThreads) void spmv_front_kernel(int32_t        nwin,
-                                                             int32_t        B,
-                                                             const int32_t* __restrict__ win_row,
+__global__ __launch_bounds__(kThreads) void spmv_chunk_kernel(int32_t        nchunks,
+                                                             const int32_t* __restrict__ chunk_row,
                                                              const int32_t* __restrict__ fptr,
                                                              const int32_t* __restrict__ fcols,
                                                              const VT* __restrict__ fvals,
@@ -285,158 +275,122 @@ __global__ __launch_bounds__(kThreads) void spmv_front_kernel(int32_t        nwi
                                                              const double* __restrict__ x,
                                                              double*       y,
                                                              double        a,
-                                                             double*       P,
-                                                             double*       S,
-                                                             int32_t*      srow,
-                                                             int32_t*      flags,
                                                              double*       partials,
                                                              const int32_t* d_conv)
 {
     if(d_conv && *d_conv != 0)
         return;
+    __shared__ double w[3][kChunkCap];
     __shared__ double lds[kThreads / kWave];
-    const int     lane = (int)(threadIdx.x % kWave);
-    const int     wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
-    const int64_t W    = (int64_t)gridDim.x * kWaves;
-    const int32_t nnz  = fptr[B];
-    double        dot  = 0.0;
-    for(int64_t w = (int64_t)blockIdx.x * kWaves + wave; w < nwin; w += W)
-    {
-        const int32_t k = (int32_t)(w * kWave);
-        if(k >= nnz)
-            break;
-        const int32_t  kb = k + lane;
-        const bool     ok = kb < nnz;
-        const uint32_t j  = ok ? (uint32_t)fcols[kb] : 0u;
-        VT             A[9];
-#pragma unroll
-        for(int q = 0; q < 9; ++q)
-            A[q] = ok ? fvals[q * fstride + kb] : VT(0);
-        const int32_t row = win_row[w];
-        const int32_t f   = fptr[min(row + lane, B)];
-        const double  x0 = x[3u * j], x1 = x[3u * j + 1], x2 = x[3u * j + 2];
-        int           idx = 0;
-#pragma unroll
-        for(int step = kWave / 2; step > 0; step >>= 1)
-        {
-            const int32_t fc = __shfl(f, idx + step, kWave);
-            if(fc <= kb)
-                idx += step;
-        }
-        const int32_t fcur  = __shfl(f, idx, kWave);
-        int32_t       fnext = __shfl(f, min(idx + 1, kWave - 1), kWave);
-        const int32_t myrow = row + idx;
-        if(idx == kWave - 1)
-            fnext = fptr[min(myrow + 1, B)];
-        const bool head = ok && fcur == kb;
-        double     v0 = 0.0, v1 = 0.0, v2 = 0.0;
-        if(ok)
-        {
-            v0 = (double)A[0] * x0 + (double)A[3] * x1 + (double)A[6] * x2;
-            v1 = (double)A[1] * x0 + (double)A[4] * x1 + (double)A[7] * x2;
-            v2 = (double)A[2] * x0 + (double)A[5] * x1 + (double)A[8] * x2;
-        }
-        const uint64_t hm = __ballot(head);
-        const uint64_t bm = __ballot(head || !ok);
-        int            fl = (head || !ok) ? 1 : 0;
-#pragma unroll
-        for(int d = 1; d < kWave; d <<= 1)
-        {
-            const double t0 = __shfl_up(v0, d, kWave), t1 = __shfl_up(v1, d, kWave), t2 = __shfl_up(v2, d, kWave);
-            const int    tf = __shfl_up(fl, d, kWave);
-            if(lane >= d)
-            {
-                if(!fl)
-                {
-                    v0 += t0;
-                    v1 += t1;
-                    v2 += t2;
-                }
-                fl |= tf;
-            }
-        }
-        const bool first_seg = (bm & ((2ull << lane) - 1ull)) == 0ull;  // row begun before the window
-        const bool row_end   = ok && kb + 1 == fnext;
-        if(row_end && !first_seg)
-        {
-            put_row<ACCUM>(y, myrow, a, v0, v1, v2);
-            dot += x[3 * (int64_t)myrow] * v0 + x[3 * (int64_t)myrow + 1] * v1 + x[3 * (int64_t)myrow + 2] * v2;
-        }
-        const int      last_valid = min(kWave - 1, nnz - 1 - k);
-        const uint64_t rem        = bm & ~1ull;
-        const int      fend       = rem ? (__builtin_ctzll(rem) - 1) : last_valid;  // end of the first segment
-        const int      fend_row   = __shfl(row_end ? 1 : 0, fend, kWave);
-        const int      last_end   = __shfl(row_end ? 1 : 0, last_valid, kWave);
-        int            flg        = 0;
-        if(!(hm & 1ull))
-        {
-            flg |= kWinHasP | (fend_row ? kWinPEnds : 0);
-            if(lane == fend)
-            {
-                P[3 * w]     = v0;
-                P[3 * w + 1] = v1;
-                P[3 * w + 2] = v2;
-            }
-        }
-        if(!last_end && hm != 0ull && last_valid == kWave - 1)
-        {
-            flg |= kWinHasS;
-            if(lane == kWave - 1)
-            {
-                S[3 * w]     = v0;
-                S[3 * w + 1] = v1;
-                S[3 * w + 2] = v2;
-                srow[w]      = myrow;
-            }
-        }
-        if(lane == 0)
-            flags[w] = flg;
-    }
-    if(partials)
-    {
-        double bs = block_sum<kThreads>(dot, lds);
-        if(threadIdx.x == 0)
-            partials[blockIdx.x] = bs;
-    }
-}
 
-// rows that cross windows: S of the window where the row begins plus P of every
-// later window up to the one where it ends, in window order
-template <bool ACCUM>
-__global__ __launch_bounds__(kThreads) void spmv_fixup_kernel(int32_t        nwin,
-                                                             int32_t        B,
-                                                             const int32_t* __restrict__ fptr,
-                                                             const double* __restrict__ P,
-                                                             const double* __restrict__ S,
-                                                             const int32_t* __restrict__ srow,
-                                                             const int32_t* __restrict__ flags,
-                                                             const double* __restrict__ x,
-                                                             double*       y,
-                                                             double        a,
-                                                             double*       partials,
-                                                             const int32_t* d_conv)
-{
-    if(d_conv && *d_conv != 0)
-        return;
-    __shared__ double lds[kThreads / kWave];
-    const int32_t nw  = (int32_t)std::min<int64_t>(nwin, ((int64_t)fptr[B] + kWave - 1) / kWave);
-    double        dot = 0.0;
-    for(int32_t w = blockIdx.x * kThreads + threadIdx.x; w < nw; w += gridDim.x * kThreads)
+    const int G   = gridDim.x;
+    const int b   = blockIdx.x;
+    const int per = G / kXcds;
+    const int lb  = (per > 0 && b < per * kXcds) ? (b % kXcds) * per + b / kXcds : b;
+    const int c0  = (int)((int64_t)nchunks * lb / G);
+    const int c1  = (int)((int64_t)nchunks * (lb + 1) / G);
+
+    const int tid  = threadIdx.x;
+    const int g    = tid / kLanes;
+    const int lane = tid % kLanes;
+    double    dot  = 0.0;
+    for(int c = c0; c < c1; ++c)
     {
-        if(!(flags[w] & kWinHasS))
-            continue;
-        double s0 = S[3 * (int64_t)w], s1 = S[3 * (int64_t)w + 1], s2 = S[3 * (int64_t)w + 2];
-        for(int32_t u = w + 1; u < nw; ++u)
+        const int32_t row0 = chunk_row[c], row1 = chunk_row[c + 1];
+        const int32_t k0 = fptr[row0], k1 = fptr[row1];
+        if(k1 - k0 <= kChunkCap)
         {
-            s0 += P[3 * (int64_t)u];
-            s1 += P[3 * (int64_t)u + 1];
-            s2 += P[3 * (int64_t)u + 2];
-            if(flags[u] & kWinPEnds)
-                break;
+            // products of the chunk's blocks, consecutive lanes on consecutive blocks
+#pragma unroll
+            for(int h = 0; h < kPer; h += 2)
+            {
+                uint32_t jj[2];
+                bool     ok[2];
+                VT       A[2][9];
+#pragma unroll
+                for(int u = 0; u < 2; ++u)
+                {
+                    const int32_t k = k0 + (h + u) * kThreads + tid;
+                    ok[u]           = k < k1;
+                    jj[u]           = ok[u] ? (uint32_t)fcols[k] : 0u;
+#pragma unroll
+                    for(int q = 0; q < 9; ++q)
+                        A[u][q] = ok[u] ? fvals[q * fstride + k] : VT(0);
+                }
+#pragma unroll
+                for(int u = 0; u < 2; ++u)
+                {
+                    const double x0 = x[3u * jj[u]], x1 = x[3u * jj[u] + 1], x2 = x[3u * jj[u] + 2];
+                    if(ok[u])
+                    {
+                        const int i = (h + u) * kThreads + tid;
+                        w[0][i]     = (double)A[u][0] * x0 + (double)A[u][3] * x1 + (double)A[u][6] * x2;
+                        w[1][i]     = (double)A[u][1] * x0 + (double)A[u][4] * x1 + (double)A[u][7] * x2;
+                        w[2][i]     = (double)A[u][2] * x0 + (double)A[u][5] * x1 + (double)A[u][8] * x2;
+                    }
+                }
+            }
+            __syncthreads();
+            for(int32_t row = row0 + g; row < row1; row += kGroups)
+            {
+                const int32_t s = fptr[row] - k0, e = fptr[row + 1] - k0;
+                double        acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
+                for(int32_t i = s + lane; i < e; i += kLanes)
+                {
+                    acc0 += w[0][i];
+                    acc1 += w[1][i];
+                    acc2 += w[2][i];
+                }
+#pragma unroll
+                for(int o = kLanes / 2; o > 0; o >>= 1)
+                {
+                    acc0 += __shfl_xor(acc0, o, kWave);
+                    acc1 += __shfl_xor(acc1, o, kWave);
+                    acc2 += __shfl_xor(acc2, o, kWave);
+                }
+                if(lane == 0)
+                {
+                    put_row<ACCUM>(y, row, a, acc0, acc1, acc2);
+                    dot += x[3 * (int64_t)row] * acc0 + x[3 * (int64_t)row + 1] * acc1
+                           + x[3 * (int64_t)row + 2] * acc2;
+                }
+            }
+            __syncthreads();  // w is rewritten by the next chunk
         }
-        const int64_t r = srow[w];
-        put_row<ACCUM>(y, r, a, s0, s1, s2);
-        dot += x[3 * r] * s0 + x[3 * r + 1] * s1 + x[3 * r + 2] * s2;
+        else
+        {
+            // a chunk with a very long row: 16 lanes per row straight from memory
+            for(int32_t row = row0 + g; row < row1; row += kGroups)
+            {
+                const int32_t s = fptr[row], e = fptr[row + 1];
+                double        acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
+                for(int32_t k = s + lane; k < e; k += kLanes)
+                {
+                    const uint32_t j = (uint32_t)fcols[k];
+                    double         A[9];
+#pragma unroll
+                    for(int q = 0; q < 9; ++q)
+                        A[q] = (double)fvals[q * fstride + k];
+                    const double x0 = x[3u * j], x1 = x[3u * j + 1], x2 = x[3u * j + 2];
+                    acc0 += A[0] * x0 + A[3] * x1 + A[6] * x2;
+                    acc1 += A[1] * x0 + A[4] * x1 + A[7] * x2;
+                    acc2 += A[2] * x0 + A[5] * x1 + A[8] * x2;
+                }
+#pragma unroll
+                for(int o = kLanes / 2; o > 0; o >>= 1)
+                {
+                    acc0 += __shfl_xor(acc0, o, kWave);
+                    acc1 += __shfl_xor(acc1, o, kWave);
+                    acc2 += __shfl_xor(acc2, o, kWave);
+                }
+                if(lane == 0)
+                {
+                    put_row<ACCUM>(y, row, a, acc0, acc1, acc2);
+                    dot += x[3 * (int64_t)row] * acc0 + x[3 * (int64_t)row + 1] * acc1
+                           + x[3 * (int64_t)row + 2] * acc2;
+                }
+            }
+        }
     }
     if(partials)
     {
@@ -464,30 +418,21 @@ __global__ __launch_bounds__(kThreads) void sum_partials_kernel(const double* pa
         *out = acc;
 }
 
-int fix_grid(const BsrMatrix& A)
-{
-    return std::max(1, std::min(ceil_div(A.nwin, kThreads), device_cu_count() * 4));
-}
-
-// launches the SpMV; returns how many partials it wrote from partials[0]
+// launches the SpMV; returns how many partials it wrote from partials[0].
+// FP64 planes go through the LDS chunk kernel, FP32 planes through the
+// segmented-scan kernel (each measured fastest for its precision on the 64^3
+// FEM + 2M contact system).
 template <typename VT, bool ACCUM>
-int launch_spmv(const SpmvScratch& scr, int g, hipStream_t s, const BsrMatrix& A, const VT* vals, const double* x,
+int launch_spmv(const SpmvScratch&, int g, hipStream_t s, const BsrMatrix& A, const VT* vals, const double* x,
                 double* y, double a, double* partials, const int32_t* d_conv)
 {
-    if(scr.front)
-    {
-        const int gf = fix_grid(A);
-        spmv_front_kernel<VT, ACCUM><<<g, kThreads, 0, s>>>(A.nwin, A.B, A.win_row.data(), A.full_ptr.data(),
-                                                            A.fcols.data(), vals, A.fstride, x, y, a, A.win_P.data(),
-                                                            A.win_S.data(), A.win_srow.data(), A.win_flags.data(),
-                                                            partials, d_conv);
-        spmv_fixup_kernel<ACCUM><<<gf, kThreads, 0, s>>>(A.nwin, A.B, A.full_ptr.data(), A.win_P.data(),
-                                                         A.win_S.data(), A.win_srow.data(), A.win_flags.data(), x, y,
-                                                         a, partials ? partials + g : nullptr, d_conv);
-        return g + gf;
-    }
-    spmv_seg_kernel<VT, ACCUM><<<g, kThreads, 0, s>>>(A.nchunks, A.chunk_row.data(), A.full_ptr.data(),
-                                                      A.fcols.data(), vals, A.fstride, x, y, a, partials, d_conv);
+    if constexpr(sizeof(VT) == 8)
+        spmv_chunk_kernel<VT, ACCUM><<<g, kThreads, 0, s>>>(A.nchunks, A.chunk_row.data(), A.full_ptr.data(),
+                                                            A.fcols.data(), vals, A.fstride, x, y, a, partials,
+                                                            d_conv);
+    else
+        spmv_seg_kernel<VT, ACCUM><<<g, kThreads, 0, s>>>(A.nchunks, A.chunk_row.data(), A.full_ptr.data(),
+                                                          A.fcols.data(), vals, A.fstride, x, y, a, partials, d_conv);
     return g;
 }
 
@@ -505,9 +450,6 @@ void spmv_prepare(SpmvScratch& scr, bool use32)
     if(scr.grid == 0 || scr.use32 != use32)
     {
         scr.use32 = use32;
-        // MI_SPMV_KERNEL=front selects the whole-matrix front variant
-        const char* kv = std::getenv("MI_SPMV_KERNEL");
-        scr.front      = kv && std::string(kv) == "front";
         // a persistent grid of as many workgroups per CU as fit at once (the
         // kernel's occupancy); MI_SPMV_BLOCKS_PER_CU overrides for sweeps
         int per_cu = 0;
@@ -516,21 +458,21 @@ void spmv_prepare(SpmvScratch& scr, bool use32)
         if(per_cu == 0)
         {
             const void* fn = use32 ? (const void*)spmv_seg_kernel<float, false>
-                                   : (const void*)spmv_seg_kernel<double, false>;
+                                   : (const void*)spmv_chunk_kernel<double, false>;
             MI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kThreads, 0));
             per_cu = std::max(1, per_cu);
         }
         int grid = device_cu_count() * per_cu;
         grid     = std::max(kXcds, (grid / kXcds) * kXcds);
-        // the front variant's fix-up adds up to 4 partials per CU after the grid's
-        scr.partials.resize((size_t)2 * grid + 4 * device_cu_count());
+        scr.partials.resize((size_t)2 * grid);
         scr.grid = grid;
     }
 }
 
 int spmv_grid(const SpmvScratch& scr, const BsrMatrix& A)
 {
-    return (int)std::max<int64_t>(1, std::min<int64_t>(scr.grid, (A.nchunks + kWaves - 1) / kWaves));
+    const int64_t per_wg = scr.use32 ? kWaves : 1;  // chunks a workgroup takes at once
+    return (int)std::max<int64_t>(1, std::min<int64_t>(scr.grid, (A.nchunks + per_wg - 1) / per_wg));
 }
 
 int spmv_full(const BsrMatrix& A, bool use32, const double* x, double* y, double* partials, const int32_t* d_conv,
