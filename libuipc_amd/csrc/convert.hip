@@ -756,8 +756,10 @@ void MatrixConverter::convert_sym(int32_t        B,
     out.chunk_row.resize((size_t)out.nchunks + 1, ratio);
     chunk_row_kernel<<<grid_of(out.nchunks + 1), kThreads, 0, s>>>(out.full_ptr.data(), B, out.nchunks,
                                                                    out.chunk_row.data());
-    out.fcols.resize((size_t)std::max<int64_t>(1, fmax), ratio);
-    out.fstride = (int64_t)out.fcols.cap;
+    // two blocks past the end: the FP32 SpMV reads blocks in aligned pairs;
+    // an even plane stride keeps every pair 8-byte aligned
+    out.fcols.resize((size_t)fmax + 2, ratio);
+    out.fstride = ((int64_t)out.fcols.cap + 1) & ~(int64_t)1;
     if(want32)
     {
         out.fvals32.resize((size_t)9 * out.fstride);

@@ -17,11 +17,11 @@
 //     ~60% of that rate);
 //   * FP64 planes (spmv_chunk_kernel): a workgroup stages a chunk's products
 //     w_k = A_k x_col(k) in LDS and sums each row from there, 16 lanes per row;
-//   * FP32 planes (spmv_seg_kernel): a wave walks a chunk in windows of 64
-//     blocks, finds each lane's row by a shuffle binary search over the row
-//     starts, sums rows with a segmented shuffle scan (carrying a row across
-//     windows) and loads the next window while this one's x gathers are
-//     outstanding;
+//   * FP32 planes (spmv_seg2_kernel): a wave walks a chunk in windows of 128
+//     blocks (two per lane), finds each lane's rows by a shuffle binary search
+//     over the row starts, sums rows with a segmented shuffle scan (carrying a
+//     row across windows) and loads the next window while this one's x
+//     gathers are outstanding;
 //   * x^T A x = sum_i x_i . (A x)_i per workgroup into partials[blockIdx.x];
 //     the consumer kernel (PCG update) sums the partials in a fixed order;
 //   * a persistent grid; each XCD (workgroups are dispatched round-robin over
@@ -44,49 +44,83 @@ constexpr int kChunkCap = 2 * kChunkStep;  // blocks a chunk may hold to be stag
 constexpr int kPer      = kChunkCap / kThreads;
 static_assert(kChunkCap % (2 * kThreads) == 0, "chunk capacity must be a multiple of two workgroups");
 
-template <typename VT>
-struct Window
+template <bool ACCUM>
+__device__ __forceinline__ void put_row(double* y, int64_t row, double a, double v0, double v1, double v2)
 {
-    uint32_t j;
-    VT       A[9];
-};
-
-template <typename VT>
-__device__ __forceinline__ void load_window(Window<VT>& w,
-                                            int32_t     k,
-                                            int32_t     kEnd,
-                                            int         lane,
-                                            const int32_t* __restrict__ fcols,
-                                            const VT* __restrict__ fvals,
-                                            int64_t fstride)
-{
-    const int32_t kb = k + lane;
-    const bool    ok = kb < kEnd;
-    w.j              = ok ? (uint32_t)fcols[kb] : 0u;
-#pragma unroll
-    for(int q = 0; q < 9; ++q)
-        w.A[q] = ok ? fvals[q * fstride + kb] : VT(0);
+    double* yr = y + 3 * row;
+    if constexpr(ACCUM)
+    {
+        yr[0] += a * v0;
+        yr[1] += a * v1;
+        yr[2] += a * v2;
+    }
+    else
+    {
+        yr[0] = v0;
+        yr[1] = v1;
+        yr[2] = v2;
+    }
 }
 
-template <typename VT, bool ACCUM>
-__global__ __launch_bounds__(kThreads) void spmv_seg_kernel(int32_t        nchunks,
-                                                           const int32_t* __restrict__ chunk_row,
-                                                           const int32_t* __restrict__ fptr,
-                                                           const int32_t* __restrict__ fcols,
-                                                           const VT* __restrict__ fvals,
-                                                           int64_t       fstride,
-                                                           const double* __restrict__ x,
-                                                           double*       y,
-                                                           double        a,
-                                                           double*       partials,
-                                                           const int32_t* d_conv)
+// FP32 planes, two blocks per lane: a wave walks a chunk in windows of 128
+// blocks, lane l on blocks k + 2l and k + 2l + 1 (8-byte loads: 512 bytes
+// per plane per wave instruction).  Windows start at an even block; a chunk
+// that starts at an odd block masks the block before it.  A lane's blocks are
+// first combined in the lane (the sum of its last segment), then the lanes
+// run the segmented scan; the segment that ends at a lane's first block (the
+// second one starts a row) is the previous lane's inclusive value plus that
+// block.
+typedef int   v2i __attribute__((ext_vector_type(2)));
+typedef float v2f __attribute__((ext_vector_type(2)));
+
+struct Window2
+{
+    v2i j;
+    v2f A[9];
+};
+
+__device__ __forceinline__ void load_window2(Window2& w,
+                                             int32_t  kw,
+                                             int32_t  kEnd,
+                                             int      lane,
+                                             const int32_t* __restrict__ fcols,
+                                             const float* __restrict__ fvals,
+                                             int64_t fstride)
+{
+    const int32_t kb = kw + 2 * lane;
+    const bool    ok = kb < kEnd;
+    w.j              = ok ? *reinterpret_cast<const v2i*>(fcols + kb) : v2i{0, 0};
+#pragma unroll
+    for(int q = 0; q < 9; ++q)
+        w.A[q] = ok ? *reinterpret_cast<const v2f*>(fvals + q * fstride + kb) : v2f{0.f, 0.f};
+}
+
+// row start i of the 128 that can meet a window: i < 64 in fa, else in fb
+__device__ __forceinline__ int32_t row_start(int32_t fa, int32_t fb, int i)
+{
+    const int32_t va = __shfl(fa, i & (kWave - 1), kWave);
+    const int32_t vb = __shfl(fb, i & (kWave - 1), kWave);
+    return i < kWave ? va : vb;
+}
+
+template <bool ACCUM>
+__global__ __launch_bounds__(kThreads) void spmv_seg2_kernel(int32_t        nchunks,
+                                                            const int32_t* __restrict__ chunk_row,
+                                                            const int32_t* __restrict__ fptr,
+                                                            const int32_t* __restrict__ fcols,
+                                                            const float* __restrict__ fvals,
+                                                            int64_t       fstride,
+                                                            const double* __restrict__ x,
+                                                            double*       y,
+                                                            double        a,
+                                                            double*       partials,
+                                                            const int32_t* d_conv)
 {
     if(d_conv && *d_conv != 0)
         return;
     __shared__ double lds[kThreads / kWave];
+    constexpr int     kWin = 2 * kWave;
 
-    // chunks of one XCD are dealt to its waves round-robin, so the rows in
-    // flight on an XCD form a narrow band whose slice of x stays in its L2
     const int G    = gridDim.x;
     const int b    = blockIdx.x;
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
@@ -103,69 +137,104 @@ __global__ __launch_bounds__(kThreads) void spmv_seg_kernel(int32_t        nchun
     {
         int32_t row    = chunk_row[c];
         int32_t rowEnd = chunk_row[c + 1];
-        int32_t k      = fptr[row];
+        int32_t kStart = fptr[row];
         int32_t kEnd   = fptr[rowEnd];
+        int32_t k      = kStart & ~1;
         double  carry0 = 0.0, carry1 = 0.0, carry2 = 0.0;
-        // double buffer: the next window's columns and values are loaded while
-        // this window's x gathers are outstanding, so a window costs one memory
-        // round trip instead of two (columns, then x)
-        Window<VT> nw;
-        load_window<VT>(nw, k, kEnd, lane, fcols, fvals, fstride);
+        Window2 nw;
+        load_window2(nw, k, kEnd, lane, fcols, fvals, fstride);
         for(;;)
         {
-            const Window<VT> cw = nw;
-            const int32_t    kb = k + lane;
-            const bool       ok = kb < kEnd;
-            // x gathers and the starts of the rows that can meet the window
-            // (every row holds a block, so at most 64 of them; past rowEnd the
-            // start is kEnd)
-            const double  x0 = x[3u * cw.j], x1 = x[3u * cw.j + 1], x2 = x[3u * cw.j + 2];
-            const int32_t f  = fptr[min(row + lane, rowEnd)];
-            // where the next window is: further on in this chunk, or the start
-            // of the wave's next chunk
-            const bool same = k + kWave < kEnd;
-            int32_t    nrow = 0, nrowEnd = 0, nk = 0, nkEnd = 0;
-            const int  cn   = c + Wx;
+            const Window2 cw  = nw;
+            const int32_t kb0 = k + 2 * lane, kb1 = kb0 + 1;
+            const bool    lead = kb0 < kStart;  // the block before an odd chunk start
+            const bool    ok0 = !lead && kb0 < kEnd, ok1 = kb1 < kEnd;
+            // columns of blocks outside [kStart, kEnd) are never used (the pair's
+            // second half past a chunk's end may be unwritten slack)
+            const uint32_t j0 = ok0 ? (uint32_t)cw.j.x : 0u, j1 = ok1 ? (uint32_t)cw.j.y : 0u;
+            const double   x00 = x[3u * j0], x01 = x[3u * j0 + 1], x02 = x[3u * j0 + 2];
+            const double   x10 = x[3u * j1], x11 = x[3u * j1 + 1], x12 = x[3u * j1 + 2];
+            const int32_t fa = fptr[min(row + lane, rowEnd)];
+            const int32_t fb = fptr[min(row + kWave + lane, rowEnd)];
+            const bool    same = k + kWin < kEnd;
+            int32_t       nrow = 0, nrowEnd = 0, nk = 0, nkStart = 0, nkEnd = 0;
+            const int     cn   = c + Wx;
             if(same)
             {
-                nk    = k + kWave;
-                nkEnd = kEnd;
+                nk      = k + kWin;
+                nkStart = kStart;
+                nkEnd   = kEnd;
             }
             else if(cn < cx1)
             {
                 nrow    = chunk_row[cn];
                 nrowEnd = chunk_row[cn + 1];
-                nk      = fptr[nrow];
+                nkStart = fptr[nrow];
                 nkEnd   = fptr[nrowEnd];
+                nk      = nkStart & ~1;
             }
             if(same || cn < cx1)
-                load_window<VT>(nw, nk, nkEnd, lane, fcols, fvals, fstride);
-            // row of block kb: the last i with f_i <= kb
-            int idx = 0;
+                load_window2(nw, nk, nkEnd, lane, fcols, fvals, fstride);
+            // rows of the lane's blocks: search the first, the second is the
+            // same row or the next one (every row holds a block)
+            const int32_t kq  = lead ? kStart : kb0;
+            int           idx = 0;
 #pragma unroll
-            for(int step = kWave / 2; step > 0; step >>= 1)
-            {
-                const int32_t fc = __shfl(f, idx + step, kWave);
-                if(fc <= kb)
+            for(int step = kWave; step > 0; step >>= 1)
+                if(row_start(fa, fb, idx + step < kWin ? idx + step : kWin - 1) <= kq && idx + step < kWin)
                     idx += step;
-            }
-            const int32_t fcur  = __shfl(f, idx, kWave);
-            int32_t       fnext = __shfl(f, min(idx + 1, kWave - 1), kWave);
-            const int32_t myrow = row + idx;
-            if(idx == kWave - 1)
-                fnext = fptr[min(myrow + 1, rowEnd)];
-            const bool head = ok && fcur == kb;
-            double     v0 = 0.0, v1 = 0.0, v2 = 0.0;
-            if(ok)
+            int32_t       f0cur  = row_start(fa, fb, idx);
+            int32_t       f0next = row_start(fa, fb, min(idx + 1, kWin - 1));
+            const int32_t row0   = row + idx;
+            if(idx + 1 >= kWin)
+                f0next = fptr[min(row0 + 1, rowEnd)];
+            const bool    h1   = ok1 && (lead || kb1 == f0next);
+            const int32_t row1 = (h1 && !lead) ? row0 + 1 : row0;
+            const int     idx1 = row1 - row;
+            int32_t       f1next = row_start(fa, fb, min(idx1 + 1, kWin - 1));
+            if(idx1 + 1 >= kWin)
+                f1next = fptr[min(row1 + 1, rowEnd)];
+            const bool h0 = ok0 && f0cur == kb0;
+            // block products
+            double w00 = 0.0, w01 = 0.0, w02 = 0.0, w10 = 0.0, w11 = 0.0, w12 = 0.0;
+            if(ok0)
             {
-                v0 = (double)cw.A[0] * x0 + (double)cw.A[3] * x1 + (double)cw.A[6] * x2;
-                v1 = (double)cw.A[1] * x0 + (double)cw.A[4] * x1 + (double)cw.A[7] * x2;
-                v2 = (double)cw.A[2] * x0 + (double)cw.A[5] * x1 + (double)cw.A[8] * x2;
+                w00 = (double)cw.A[0].x * x00 + (double)cw.A[3].x * x01 + (double)cw.A[6].x * x02;
+                w01 = (double)cw.A[1].x * x00 + (double)cw.A[4].x * x01 + (double)cw.A[7].x * x02;
+                w02 = (double)cw.A[2].x * x00 + (double)cw.A[5].x * x01 + (double)cw.A[8].x * x02;
             }
-            // segmented inclusive scan: each lane ends with the sum of its row's
-            // blocks from the row's head (or lane 0) up to itself
-            const uint64_t hmask = __ballot(head || !ok);
-            int            fl    = (head || !ok) ? 1 : 0;
+            if(ok1)
+            {
+                w10 = (double)cw.A[0].y * x10 + (double)cw.A[3].y * x11 + (double)cw.A[6].y * x12;
+                w11 = (double)cw.A[1].y * x10 + (double)cw.A[4].y * x11 + (double)cw.A[7].y * x12;
+                w12 = (double)cw.A[2].y * x10 + (double)cw.A[5].y * x11 + (double)cw.A[8].y * x12;
+            }
+            // the lane's last segment inside the lane, and whether it starts here
+            double v0, v1, v2;
+            bool   brk;
+            if(ok1 && !h1)
+            {
+                v0  = w00 + w10;
+                v1  = w01 + w11;
+                v2  = w02 + w12;
+                brk = h0;
+            }
+            else if(ok1)
+            {
+                v0  = w10;
+                v1  = w11;
+                v2  = w12;
+                brk = true;
+            }
+            else
+            {
+                v0  = w00;
+                v1  = w01;
+                v2  = w02;
+                brk = h0 || !ok0;
+            }
+            const uint64_t bm = __ballot(brk);
+            int            fl = brk ? 1 : 0;
 #pragma unroll
             for(int d = 1; d < kWave; d <<= 1)
             {
@@ -182,52 +251,58 @@ __global__ __launch_bounds__(kThreads) void spmv_seg_kernel(int32_t        nchun
                     fl |= tf;
                 }
             }
-            // the row continuing from the previous window adds the carry
-            if((hmask & ((2ull << lane) - 1ull)) == 0ull)
+            // the previous lane's inclusive value (the segment the first block continues)
+            double p0 = __shfl_up(v0, 1, kWave), p1 = __shfl_up(v1, 1, kWave), p2 = __shfl_up(v2, 1, kWave);
+            if(lane == 0)
+                p0 = p1 = p2 = 0.0;
+            const bool none_below = (bm & ((1ull << lane) - 1ull)) == 0ull;   // no break in lanes [0, l)
+            const bool none_upto  = (bm & ((2ull << lane) - 1ull)) == 0ull;   // ... in lanes [0, l]
+            if(none_below && !h0)
+            {
+                p0 += carry0;
+                p1 += carry1;
+                p2 += carry2;
+            }
+            if(none_upto)
             {
                 v0 += carry0;
                 v1 += carry1;
                 v2 += carry2;
             }
-            const bool row_end = ok && kb + 1 == fnext;
-            if(row_end)
+            const bool end0 = ok0 && (h1 || !ok1) && kb0 + 1 == f0next;
+            const bool end1 = ok1 && kb1 + 1 == f1next;
+            if(end0)
             {
-                double* yr = y + 3 * (int64_t)myrow;
-                if constexpr(ACCUM)
-                {
-                    yr[0] += a * v0;
-                    yr[1] += a * v1;
-                    yr[2] += a * v2;
-                }
-                else
-                {
-                    yr[0] = v0;
-                    yr[1] = v1;
-                    yr[2] = v2;
-                }
-                dot += x[3 * (int64_t)myrow] * v0 + x[3 * (int64_t)myrow + 1] * v1 + x[3 * (int64_t)myrow + 2] * v2;
+                const double e0 = (h0 ? 0.0 : p0) + w00, e1 = (h0 ? 0.0 : p1) + w01, e2 = (h0 ? 0.0 : p2) + w02;
+                put_row<ACCUM>(y, row0, a, e0, e1, e2);
+                dot += x[3 * (int64_t)row0] * e0 + x[3 * (int64_t)row0 + 1] * e1 + x[3 * (int64_t)row0 + 2] * e2;
+            }
+            if(end1)
+            {
+                put_row<ACCUM>(y, row1, a, v0, v1, v2);
+                dot += x[3 * (int64_t)row1] * v0 + x[3 * (int64_t)row1 + 1] * v1 + x[3 * (int64_t)row1 + 2] * v2;
             }
             if(same)
             {
-                // lane 63 hands its row on: either the row continues (carry)
-                // or the next window starts the next row
-                const int  end63 = __shfl(row_end ? 1 : 0, kWave - 1, kWave);
-                const int  row63 = __shfl(myrow, kWave - 1, kWave);
-                const bool cont  = end63 == 0;
+                // lane 63 hands its row on: its second block is valid (the
+                // window is not the chunk's last); the row continues or ends
+                const int  e63   = __shfl(end1 ? 1 : 0, kWave - 1, kWave);
+                const int  r63   = __shfl(row1, kWave - 1, kWave);
+                const bool cont  = e63 == 0;
                 carry0           = cont ? __shfl(v0, kWave - 1, kWave) : 0.0;
                 carry1           = cont ? __shfl(v1, kWave - 1, kWave) : 0.0;
                 carry2           = cont ? __shfl(v2, kWave - 1, kWave) : 0.0;
-                row              = cont ? row63 : row63 + 1;
+                row              = cont ? r63 : r63 + 1;
                 k                = nk;
             }
             else
             {
-                // a chunk ends with a whole row: nothing carries over
                 if(cn >= cx1)
                     break;
                 c      = cn;
                 row    = nrow;
                 rowEnd = nrowEnd;
+                kStart = nkStart;
                 k      = nk;
                 kEnd   = nkEnd;
                 carry0 = carry1 = carry2 = 0.0;
@@ -239,24 +314,6 @@ __global__ __launch_bounds__(kThreads) void spmv_seg_kernel(int32_t        nchun
         double bs = block_sum<kThreads>(dot, lds);
         if(threadIdx.x == 0)
             partials[blockIdx.x] = bs;
-    }
-}
-
-template <bool ACCUM>
-__device__ __forceinline__ void put_row(double* y, int64_t row, double a, double v0, double v1, double v2)
-{
-    double* yr = y + 3 * row;
-    if constexpr(ACCUM)
-    {
-        yr[0] += a * v0;
-        yr[1] += a * v1;
-        yr[2] += a * v2;
-    }
-    else
-    {
-        yr[0] = v0;
-        yr[1] = v1;
-        yr[2] = v2;
     }
 }
 
@@ -420,8 +477,7 @@ __global__ __launch_bounds__(kThreads) void sum_partials_kernel(const double* pa
 
 // launches the SpMV; returns how many partials it wrote from partials[0].
 // FP64 planes go through the LDS chunk kernel, FP32 planes through the
-// segmented-scan kernel (each measured fastest for its precision on the 64^3
-// FEM + 2M contact system).
+// two-blocks-per-lane segmented-scan kernel.
 template <typename VT, bool ACCUM>
 int launch_spmv(const SpmvScratch&, int g, hipStream_t s, const BsrMatrix& A, const VT* vals, const double* x,
                 double* y, double a, double* partials, const int32_t* d_conv)
@@ -431,8 +487,8 @@ int launch_spmv(const SpmvScratch&, int g, hipStream_t s, const BsrMatrix& A, co
                                                             A.fcols.data(), vals, A.fstride, x, y, a, partials,
                                                             d_conv);
     else
-        spmv_seg_kernel<VT, ACCUM><<<g, kThreads, 0, s>>>(A.nchunks, A.chunk_row.data(), A.full_ptr.data(),
-                                                          A.fcols.data(), vals, A.fstride, x, y, a, partials, d_conv);
+        spmv_seg2_kernel<ACCUM><<<g, kThreads, 0, s>>>(A.nchunks, A.chunk_row.data(), A.full_ptr.data(),
+                                                       A.fcols.data(), vals, A.fstride, x, y, a, partials, d_conv);
     return g;
 }
 
@@ -457,7 +513,7 @@ void spmv_prepare(SpmvScratch& scr, bool use32)
             per_cu = std::max(1, std::min(32, std::atoi(e)));
         if(per_cu == 0)
         {
-            const void* fn = use32 ? (const void*)spmv_seg_kernel<float, false>
+            const void* fn = use32 ? (const void*)spmv_seg2_kernel<false>
                                    : (const void*)spmv_chunk_kernel<double, false>;
             MI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kThreads, 0));
             per_cu = std::max(1, per_cu);
