@@ -60,6 +60,7 @@ struct mi_sys
 
     // PCG state (LinearFusedPCG's r, z, p, Ap and device scalars)
     DBuf<double> r, z, p, Ap, tmp;
+    DBuf<float4> p32;  // FP32 systems: the SpMV's packed copy of p
     PcgScalars*  d_sc = nullptr;
     PcgScalars*  h_sc = nullptr;  // pinned
 
@@ -683,7 +684,8 @@ void run_iteration(mi_sys* sys, double* x, bool bypass, const LocalPreconditione
     const bool    use32 = sys->desc.value_precision == MI_FP32;
     double*       partA = sys->spmv_scr.partials.data();
     double*       partB = sys->dot_scr.partials.data();
-    const int     gA    = spmv_full(sys->A, use32, sys->p.data(), sys->Ap.data(), partA, &sc->converged,
+    float4*       p32   = use32 ? sys->p32.data() : nullptr;
+    const int     gA    = spmv_full(sys->A, use32, sys->p.data(), p32, sys->Ap.data(), partA, &sc->converged,
                                     sys->spmv_scr, s);
     int gB;
     if(fused_diag)
@@ -701,7 +703,7 @@ void run_iteration(mi_sys* sys, double* x, bool bypass, const LocalPreconditione
             sys->apply_preconditioner(sys->r.data(), sys->z.data(), &sc->converged, s);
         gB = pcg_dot_partials(sys->r.data(), sys->z.data(), n, partB, &sc->converged, sys->dot_scr, s);
     }
-    pcg_update_p(sc, partB, gB, sys->p.data(), sys->z.data(), n, s);
+    pcg_update_p(sc, partB, gB, sys->p.data(), sys->z.data(), n, p32, s);
 }
 
 void read_scalars(mi_sys* sys, hipStream_t s)
@@ -779,6 +781,11 @@ extern "C" int mi_pcg_solve(mi_sys* sys, const double* b, double* x, const mi_pc
     st->rz0          = rz0;
     MI_REQUIRE(std::isfinite(rz0) && std::isfinite(bb), MI_ERR_NONFINITE, "mi_pcg_solve: non-finite r^T z at start");
     copy_f64(sys->p.data(), sys->z.data(), n, s);
+    if(sys->desc.value_precision == MI_FP32)
+    {
+        sys->p32.resize((size_t)sys->desc.block_rows, ratio);
+        pack_f32x4(sys->p.data(), sys->p32.data(), sys->desc.block_rows, s);
+    }
     if(std::fabs(rz0) == 0.0)
     {
         st->converged = 1;
@@ -816,6 +823,7 @@ extern "C" int mi_pcg_solve(mi_sys* sys, const double* b, double* x, const mi_pc
     key.bufs[6] = sys->A.has32 ? (const void*)sys->A.fvals32.data() : (const void*)sys->A.fvals.data();
     key.bufs[7] = sys->pcs.empty() ? nullptr : (const void*)sys->pcs[0].get();
     key.bufs[8] = fused ? (const void*)fused->diag3_inverse() : nullptr;
+    key.bufs[9] = sys->p32.data();
     key.stream  = s;
 
     while(done < total)

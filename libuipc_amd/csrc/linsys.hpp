@@ -80,6 +80,7 @@ class MatrixConverter
 struct SpmvScratch
 {
     DBuf<double> partials;
+    DBuf<float4> x32;  // FP32 systems: the packed copy of x when the caller has none
     int          grid  = 0;
     bool         use32 = false;
 };
@@ -87,7 +88,10 @@ struct SpmvScratch
 // value precision (must run outside graph capture)
 void spmv_prepare(SpmvScratch& scr, bool use32);
 // y = A x (overwritten); partials[g] = per-workgroup x^T A x; returns the grid g
-int spmv_full(const BsrMatrix& A, bool use32, const double* x, double* y, double* partials, const int32_t* d_conv,
+// x32: the FP32 SpMV's padded float4 copy of x (FP32 systems; packed into the
+// scratch when null)
+int spmv_full(const BsrMatrix& A, bool use32, const double* x, const float4* x32, double* y, double* partials,
+              const int32_t* d_conv,
               SpmvScratch& scr, hipStream_t s);
 // y += a A x; *d_dot = x^T A x when d_dot != null
 void spmv_accumulate_dot(const BsrMatrix& A, bool use32, double a, const double* x, double* y, double* d_dot,
@@ -180,6 +184,10 @@ int pcg_update_xr_diag3(PcgScalars* sc, const double* pap_part, int gp, double* 
                         const double* Ap, double* z, const double* diag_inv, int64_t blk_off, int64_t blk_cnt,
                         int64_t n, double* part, SpmvScratch& scr, hipStream_t s);
 // convergence / beta from the (r.z, r.r) partials, then p = z + beta p
-void pcg_update_p(PcgScalars* sc, const double* part, int gd, double* p, const double* z, int64_t n, hipStream_t s);
+// p = z + beta p; with p32 also the FP32 SpMV's padded float4 copy of p
+void pcg_update_p(PcgScalars* sc, const double* part, int gd, double* p, const double* z, int64_t n, float4* p32,
+                  hipStream_t s);
+// x -> one padded float4 per block row (the FP32 SpMV's gather vector)
+void pack_f32x4(const double* x, float4* x32, int64_t nb, hipStream_t s);
 void pcg_copy_if_active(PcgScalars* sc, double* dst, const double* src, int64_t n, hipStream_t s);
 }  // namespace mi

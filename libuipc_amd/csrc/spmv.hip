@@ -111,6 +111,7 @@ __global__ __launch_bounds__(kThreads) void spmv_seg2_kernel(int32_t        nchu
                                                             const float* __restrict__ fvals,
                                                             int64_t       fstride,
                                                             const double* __restrict__ x,
+                                                            const float4* __restrict__ x32,
                                                             double*       y,
                                                             double        a,
                                                             double*       partials,
@@ -151,9 +152,11 @@ __global__ __launch_bounds__(kThreads) void spmv_seg2_kernel(int32_t        nchu
             const bool    ok0 = !lead && kb0 < kEnd, ok1 = kb1 < kEnd;
             // columns of blocks outside [kStart, kEnd) are never used (the pair's
             // second half past a chunk's end may be unwritten slack)
+            // x is gathered from its FP32 copy, one 16-byte load per block
             const uint32_t j0 = ok0 ? (uint32_t)cw.j.x : 0u, j1 = ok1 ? (uint32_t)cw.j.y : 0u;
-            const double   x00 = x[3u * j0], x01 = x[3u * j0 + 1], x02 = x[3u * j0 + 2];
-            const double   x10 = x[3u * j1], x11 = x[3u * j1 + 1], x12 = x[3u * j1 + 2];
+            const float4   g0 = x32[j0], g1 = x32[j1];
+            const double   x00 = g0.x, x01 = g0.y, x02 = g0.z;
+            const double   x10 = g1.x, x11 = g1.y, x12 = g1.z;
             const int32_t fa = fptr[min(row + lane, rowEnd)];
             const int32_t fb = fptr[min(row + kWave + lane, rowEnd)];
             const bool    same = k + kWin < kEnd;
@@ -480,7 +483,7 @@ __global__ __launch_bounds__(kThreads) void sum_partials_kernel(const double* pa
 // two-blocks-per-lane segmented-scan kernel.
 template <typename VT, bool ACCUM>
 int launch_spmv(const SpmvScratch&, int g, hipStream_t s, const BsrMatrix& A, const VT* vals, const double* x,
-                double* y, double a, double* partials, const int32_t* d_conv)
+                const float4* x32, double* y, double a, double* partials, const int32_t* d_conv)
 {
     if constexpr(sizeof(VT) == 8)
         spmv_chunk_kernel<VT, ACCUM><<<g, kThreads, 0, s>>>(A.nchunks, A.chunk_row.data(), A.full_ptr.data(),
@@ -488,7 +491,8 @@ int launch_spmv(const SpmvScratch&, int g, hipStream_t s, const BsrMatrix& A, co
                                                             d_conv);
     else
         spmv_seg2_kernel<ACCUM><<<g, kThreads, 0, s>>>(A.nchunks, A.chunk_row.data(), A.full_ptr.data(),
-                                                       A.fcols.data(), vals, A.fstride, x, y, a, partials, d_conv);
+                                                       A.fcols.data(), vals, A.fstride, x, x32, y, a, partials,
+                                                       d_conv);
     return g;
 }
 
@@ -531,8 +535,19 @@ int spmv_grid(const SpmvScratch& scr, const BsrMatrix& A)
     return (int)std::max<int64_t>(1, std::min<int64_t>(scr.grid, (A.nchunks + per_wg - 1) / per_wg));
 }
 
-int spmv_full(const BsrMatrix& A, bool use32, const double* x, double* y, double* partials, const int32_t* d_conv,
-              SpmvScratch& scr, hipStream_t s)
+// the FP32 kernel gathers x from a float4 copy: the caller's (the PCG keeps
+// one next to p), else packed here into the scratch (not under capture)
+const float4* x32_of(const BsrMatrix& A, const double* x, const float4* x32, SpmvScratch& scr, hipStream_t s)
+{
+    if(x32)
+        return x32;
+    scr.x32.resize((size_t)std::max(1, A.B));
+    pack_f32x4(x, scr.x32.data(), A.B, s);
+    return scr.x32.data();
+}
+
+int spmv_full(const BsrMatrix& A, bool use32, const double* x, const float4* x32, double* y, double* partials,
+              const int32_t* d_conv, SpmvScratch& scr, hipStream_t s)
 {
     spmv_prepare(scr, use32);
     if(A.B == 0)
@@ -542,8 +557,10 @@ int spmv_full(const BsrMatrix& A, bool use32, const double* x, double* y, double
     MI_REQUIRE(x && y && partials, MI_ERR_STATE, "spmv_full: null buffer");
     MI_REQUIRE(partials >= scr.partials.data() && partials + g <= scr.partials.data() + scr.partials.size(),
                MI_ERR_STATE, "spmv_full: partials outside the scratch");
-    const int np = use32 ? launch_spmv<float, false>(scr, g, s, A, A.fvals32.data(), x, y, 1.0, partials, d_conv)
-                         : launch_spmv<double, false>(scr, g, s, A, A.fvals.data(), x, y, 1.0, partials, d_conv);
+    const int np = use32 ? launch_spmv<float, false>(scr, g, s, A, A.fvals32.data(), x, x32_of(A, x, x32, scr, s), y,
+                                                     1.0, partials, d_conv)
+                         : launch_spmv<double, false>(scr, g, s, A, A.fvals.data(), x, nullptr, y, 1.0, partials,
+                                                      d_conv);
     MI_HIP(hipGetLastError());
     return np;
 }
@@ -561,8 +578,9 @@ void spmv_accumulate_dot(const BsrMatrix& A, bool use32, double a, const double*
     check_matrix(A, use32);
     const int g = spmv_grid(scr, A);
     double*   p = d_dot ? scr.partials.data() : nullptr;
-    const int np = use32 ? launch_spmv<float, true>(scr, g, s, A, A.fvals32.data(), x, y, a, p, nullptr)
-                         : launch_spmv<double, true>(scr, g, s, A, A.fvals.data(), x, y, a, p, nullptr);
+    const int np = use32 ? launch_spmv<float, true>(scr, g, s, A, A.fvals32.data(), x, x32_of(A, x, nullptr, scr, s),
+                                                    y, a, p, nullptr)
+                         : launch_spmv<double, true>(scr, g, s, A, A.fvals.data(), x, nullptr, y, a, p, nullptr);
     if(d_dot)
         sum_partials_kernel<<<1, kThreads, 0, s>>>(scr.partials.data(), np, d_dot);
     MI_HIP(hipGetLastError());

@@ -283,7 +283,8 @@ __global__ __launch_bounds__(kThreads) void pcg_update_p_kernel(PcgScalars*   sc
                                                                int           gd,
                                                                double*       p,
                                                                const double* z,
-                                                               int64_t       n)
+                                                               int64_t       n,
+                                                               float4*       p32)
 {
     if(sc->converged)
         return;
@@ -310,8 +311,28 @@ __global__ __launch_bounds__(kThreads) void pcg_update_p_kernel(PcgScalars*   sc
     }
     if(conv)
         return;
+    if(p32)
+    {
+        // the FP32 SpMV's copy of p: one padded float4 per block row
+        for(int64_t k = (int64_t)blockIdx.x * kThreads + threadIdx.x; 3 * k < n; k += (int64_t)gridDim.x * kThreads)
+        {
+            const double q0 = z[3 * k] + beta * p[3 * k], q1 = z[3 * k + 1] + beta * p[3 * k + 1],
+                         q2 = z[3 * k + 2] + beta * p[3 * k + 2];
+            p[3 * k]     = q0;
+            p[3 * k + 1] = q1;
+            p[3 * k + 2] = q2;
+            p32[k]       = make_float4((float)q0, (float)q1, (float)q2, 0.f);
+        }
+        return;
+    }
     for(int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
         p[i] = z[i] + beta * p[i];
+}
+
+__global__ void pack_f32x4_kernel(const double* __restrict__ x, float4* x32, int64_t nb)
+{
+    for(int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nb; k += (int64_t)gridDim.x * blockDim.x)
+        x32[k] = make_float4((float)x[3 * k], (float)x[3 * k + 1], (float)x[3 * k + 2], 0.f);
 }
 
 __global__ void pcg_copy_if_active_kernel(PcgScalars* sc, double* dst, const double* src, int64_t n)
@@ -468,10 +489,17 @@ int pcg_update_xr_diag3(PcgScalars* sc, const double* pap_part, int gp, double* 
     return g;
 }
 
-void pcg_update_p(PcgScalars* sc, const double* part, int gd, double* p, const double* z, int64_t n, hipStream_t s)
+void pcg_update_p(PcgScalars* sc, const double* part, int gd, double* p, const double* z, int64_t n, float4* p32,
+                  hipStream_t s)
 {
     MI_REQUIRE(sc && part && p && z, MI_ERR_STATE, "pcg_update_p: null buffer");
-    pcg_update_p_kernel<<<vec_grid(n), kThreads, 0, s>>>(sc, part, gd, p, z, n);
+    pcg_update_p_kernel<<<vec_grid(n), kThreads, 0, s>>>(sc, part, gd, p, z, n, p32);
+}
+
+void pack_f32x4(const double* x, float4* x32, int64_t nb, hipStream_t s)
+{
+    if(nb > 0)
+        pack_f32x4_kernel<<<vec_grid(3 * nb), kThreads, 0, s>>>(x, x32, nb);
 }
 
 void pcg_copy_if_active(PcgScalars* sc, double* dst, const double* src, int64_t n, hipStream_t s)
