@@ -60,6 +60,19 @@ int mi_synth_fem_fill(const mi_synth_fem_desc* desc,
                       int32_t*                 part_ids,
                       double*                  positions);
 
+/* The inputs of the device producers (include/mi_producers.h) for the same
+ * system (any pointer may be NULL):
+ *   rest_positions [3 block_rows]   the undeformed grid
+ *   mass [block_rows]               lumped vertex masses (the kinetic blocks m_i I)
+ *   contact_idx [4 n_contacts], contact_g [12 n_contacts], contact_kappa [n_contacts]
+ *                                   the contact stencils (vertex, face) and their kappa g g^T data */
+int mi_synth_fem_aux(const mi_synth_fem_desc* desc,
+                     double*                  rest_positions,
+                     double*                  mass,
+                     int32_t*                 contact_idx,
+                     double*                  contact_g,
+                     double*                  contact_kappa);
+
 const char* mi_synth_last_error(void);
 
 #ifdef __cplusplus

@@ -147,6 +147,12 @@ LINSYS_PROTOS = {
     "mi_mtx_read_block_triplets": (C.c_int, [C.c_char_p, C.c_int, c_i32p, C.POINTER(C.c_int64), c_i32p, c_i32p,
                                              c_f64p]),
     "mi_vec_read_mtx": (C.c_int, [C.c_char_p, C.POINTER(C.c_int64), c_f64p]),
+    "mi_produce_kinetic": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
+                                     C.c_void_p, C.c_void_p]),
+    "mi_produce_snh": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double,
+                                 C.c_double, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "mi_produce_contact": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
+                                     C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "mi_pcg_default_opts": (C.c_int, [C.POINTER(PcgOpts)]),
     "mi_pcg_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(PcgOpts), C.POINTER(PcgStats),
                                C.c_void_p]),
@@ -156,6 +162,7 @@ SYNTH_PROTOS = {
     "mi_synth_fem_sizes_of": (C.c_int, [C.POINTER(SynthFemDesc), C.POINTER(SynthFemSizes)]),
     "mi_synth_fem_fill": (C.c_int, [C.POINTER(SynthFemDesc), c_i32p, c_i32p, c_f64p, c_f64p, c_i32p, c_i32p,
                                     c_f64p]),
+    "mi_synth_fem_aux": (C.c_int, [C.POINTER(SynthFemDesc), c_f64p, c_f64p, c_i32p, c_f64p, c_f64p]),
     "mi_synth_last_error": (C.c_char_p, []),
 }
 

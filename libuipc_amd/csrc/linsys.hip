@@ -569,6 +569,67 @@ int mi_subsystem_distribute(mi_ctx* ctx, const double* x, int64_t dof_offset, in
     MI_API_END
 }
 
+int mi_produce_kinetic(mi_ctx*       ctx,
+                       int32_t       vert_num,
+                       const double* mass,
+                       int32_t       vert_offset,
+                       int32_t*      rows,
+                       int32_t*      cols,
+                       double*       vals,
+                       void*         stream)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(ctx && vert_num >= 0 && vert_offset >= 0, MI_ERR_ARG, "mi_produce_kinetic: bad argument");
+    MI_REQUIRE(vert_num == 0 || (mass && rows && cols && vals), MI_ERR_ARG, "mi_produce_kinetic: null buffer");
+    produce_kinetic(vert_num, mass, vert_offset, rows, cols, vals, pick(ctx, stream));
+    MI_API_END
+}
+
+int mi_produce_snh(mi_ctx*        ctx,
+                   int64_t        n_tets,
+                   const int32_t* tets,
+                   const double*  rest_positions,
+                   const double*  positions,
+                   double         mu,
+                   double         lambda,
+                   double         dt,
+                   int32_t        vert_offset,
+                   int32_t*       rows,
+                   int32_t*       cols,
+                   double*        vals,
+                   void*          stream)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(ctx && n_tets >= 0 && vert_offset >= 0, MI_ERR_ARG, "mi_produce_snh: bad argument");
+    MI_REQUIRE(mu > 0.0 && lambda > 0.0 && dt > 0.0, MI_ERR_ARG, "mi_produce_snh: mu, lambda and dt must be > 0");
+    MI_REQUIRE(n_tets == 0 || (tets && rest_positions && positions && rows && cols && vals), MI_ERR_ARG,
+               "mi_produce_snh: null buffer");
+    produce_snh(n_tets, tets, rest_positions, positions, mu, lambda, dt, vert_offset, rows, cols, vals,
+                pick(ctx, stream));
+    MI_API_END
+}
+
+int mi_produce_contact(mi_ctx*        ctx,
+                       int64_t        n_stencils,
+                       const int32_t* stencils,
+                       const double*  g,
+                       const double*  kappa,
+                       double         dt,
+                       int32_t        vert_offset,
+                       int32_t*       rows,
+                       int32_t*       cols,
+                       double*        vals,
+                       void*          stream)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(ctx && n_stencils >= 0 && vert_offset >= 0 && dt > 0.0, MI_ERR_ARG,
+               "mi_produce_contact: bad argument");
+    MI_REQUIRE(n_stencils == 0 || (stencils && g && kappa && rows && cols && vals), MI_ERR_ARG,
+               "mi_produce_contact: null buffer");
+    produce_contact(n_stencils, stencils, g, kappa, dt, vert_offset, rows, cols, vals, pick(ctx, stream));
+    MI_API_END
+}
+
 namespace
 {
 void write_int_array(FILE* fp, const char* key, const int32_t* a, int64_t n, bool last)

@@ -113,6 +113,31 @@ void subsystem_fix_vertices(const int32_t* rows,
                             double*        gradient,
                             hipStream_t    s);
 void subsystem_distribute(const double* x, double* dxs, int64_t n, hipStream_t s);
+// Hessian producers (producers.hip)
+void produce_kinetic(int32_t V, const double* mass, int32_t vert_offset, int32_t* rows, int32_t* cols, double* vals,
+                     hipStream_t s);
+void produce_snh(int64_t        n_tets,
+                 const int32_t* tets,
+                 const double*  X,
+                 const double*  x,
+                 double         mu,
+                 double         lambda,
+                 double         dt,
+                 int32_t        vert_offset,
+                 int32_t*       rows,
+                 int32_t*       cols,
+                 double*        vals,
+                 hipStream_t    s);
+void produce_contact(int64_t        n,
+                     const int32_t* stencils,
+                     const double*  g,
+                     const double*  kappa,
+                     double         dt,
+                     int32_t        vert_offset,
+                     int32_t*       rows,
+                     int32_t*       cols,
+                     double*        vals,
+                     hipStream_t    s);
 void   copy_f64(double* dst, const double* src, int64_t n, hipStream_t s);
 double dot_f64_host(const double* x, const double* y, int64_t n, hipStream_t s);
 double norm_f64_host(const double* x, int64_t n, hipStream_t s);

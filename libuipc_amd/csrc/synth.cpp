@@ -279,6 +279,49 @@ inline void emit_half4(const int32_t idx[4], const double K[12][12], int32_t* ro
         }
 }
 
+// contact stencil s: a random vertex p and a face (3 vertices, p excluded when
+// possible) of a tet in a cell within contact_cells of p; g ~ N(0,1)^12 and
+// kappa log-uniform in [kappa_lo, kappa_hi]
+void contact_stencil(const mi_synth_fem_desc* d, const Grid& g, const int32_t* tv, int64_t s, int32_t idx[4],
+                     double gv[12], double& kappa)
+{
+    const int32_t nc = g.N - 1;
+    Rng     rng(d->seed, TAG_CONTACT, (uint64_t)s);
+    int64_t p  = rng.below(g.V);
+    int     pi = (int)(p % g.N), pj = (int)((p / g.N) % g.N), pk = (int)(p / ((int64_t)g.N * g.N));
+    int     R  = std::max(1, d->contact_cells);
+    auto    pick = [&](int pc)
+    {
+        int lo = std::max(0, pc - R), hi = std::min(nc - 1, pc + R - 1);
+        if(hi < lo)
+            hi = lo;
+        return lo + (int)rng.below(hi - lo + 1);
+    };
+    int            ci = pick(pi), cj = pick(pj), ck = pick(pk);
+    int64_t        cell = ci + (int64_t)nc * (cj + (int64_t)nc * ck);
+    int            sub  = (int)rng.below(5);
+    int            f0   = (int)rng.below(4);
+    const int32_t* tq   = &tv[4 * (cell * 5 + sub)];
+    idx[0]              = (int32_t)p;
+    for(int f = 0; f < 4; ++f)
+    {
+        int  omit  = (f0 + f) % 4;  // face = the 3 tet vertices except `omit`
+        int  o     = 1;
+        bool has_p = false;
+        for(int c = 0; c < 4; ++c)
+            if(c != omit)
+            {
+                idx[o++] = tq[c];
+                has_p |= (tq[c] == (int32_t)p);
+            }
+        if(!has_p)
+            break;
+    }
+    for(int c = 0; c < 12; ++c)
+        gv[c] = rng.normal();
+    kappa = d->kappa_lo * std::pow(d->kappa_hi / d->kappa_lo, rng.uniform01());
+}
+
 void parallel_for(int64_t n, int threads, const std::function<void(int64_t, int64_t)>& fn);
 
 int hw_threads(int requested)
@@ -326,6 +369,75 @@ uint32_t part1by2(uint32_t x)  // spread 10 bits to every third bit
     x = (x | (x << 4)) & 0x030C30C3;
     x = (x | (x << 2)) & 0x09249249;
     return x;
+}
+
+// rest positions X (the regular grid), deformed positions x (jittered), tets
+// (oriented by rest volume) and the lumped vertex masses
+void build_mesh(const mi_synth_fem_desc* d,
+                const Grid&              g,
+                int                      T,
+                std::vector<double>&     X,
+                std::vector<double>&     x,
+                std::vector<int32_t>&    tv,
+                std::vector<double>&     mass)
+{
+    const double h = 1.0 / (double)(g.N - 1);
+        // rest and deformed positions
+        X.assign(3 * g.V, 0.0);
+        x.assign(3 * g.V, 0.0);
+        parallel_for(g.V,
+                     T,
+                     [&](int64_t b, int64_t e)
+                     {
+                         for(int64_t v = b; v < e; ++v)
+                         {
+                             int i = (int)(v % g.N), j = (int)((v / g.N) % g.N),
+                                 k = (int)(v / ((int64_t)g.N * g.N));
+                             Rng rng(d->seed, TAG_JITTER, (uint64_t)v);
+                             X[3 * v + 0] = i * h;
+                             X[3 * v + 1] = j * h;
+                             X[3 * v + 2] = k * h;
+                             for(int c = 0; c < 3; ++c)
+                                 x[3 * v + c] = X[3 * v + c] + d->jitter * h * rng.uniform(-1.0, 1.0);
+                         }
+                     });
+
+        // tets (oriented by rest volume)
+        tv.assign(4 * g.tets, 0);
+        parallel_for(g.tets,
+                     T,
+                     [&](int64_t b, int64_t e)
+                     {
+                         for(int64_t t = b; t < e; ++t)
+                         {
+                             int32_t q[4];
+                             cell_tet(g, t, q);
+                             double Dm[9];
+                             for(int c = 0; c < 3; ++c)
+                                 for(int r = 0; r < 3; ++r)
+                                     Dm[r + 3 * c] = X[3 * q[c + 1] + r] - X[3 * q[0] + r];
+                             if(det3(Dm) < 0)
+                                 std::swap(q[2], q[3]);
+                             for(int c = 0; c < 4; ++c)
+                                 tv[4 * t + c] = q[c];
+                         }
+                     });
+
+        // lumped mass: every tet of the regular grid has rest volume h^3 / 6
+        // or h^3 / 3 (the central tet); accumulate per vertex exactly
+        mass.assign(g.V, 0.0);
+        for(int64_t t = 0; t < g.tets; ++t)
+        {
+            const int32_t* q = &tv[4 * t];
+            double         Dm[9];
+            for(int c = 0; c < 3; ++c)
+                for(int r = 0; r < 3; ++r)
+                    Dm[r + 3 * c] = X[3 * q[c + 1] + r] - X[3 * q[0] + r];
+            double m = d->density * std::fabs(det3(Dm)) / 6.0 / 4.0;
+            for(int c = 0; c < 4; ++c)
+                mass[q[c]] += m;
+        }
+
 }
 
 int check_desc(const mi_synth_fem_desc* d)
@@ -389,69 +501,17 @@ int mi_synth_fem_fill(const mi_synth_fem_desc* d,
         g.cells = (int64_t)(g.N - 1) * (g.N - 1) * (g.N - 1);
         g.tets  = sz.n_tets;
         const int    T   = hw_threads(d->threads);
-        const double h   = 1.0 / (double)(g.N - 1);
         const double mu  = d->youngs / (2.0 * (1.0 + d->poisson));
         const double lam = d->youngs * d->poisson / ((1.0 + d->poisson) * (1.0 - 2.0 * d->poisson));
         const double dt2 = d->dt * d->dt;
 
-        // rest and deformed positions
-        std::vector<double> X(3 * g.V), x(3 * g.V);
-        parallel_for(g.V,
-                     T,
-                     [&](int64_t b, int64_t e)
-                     {
-                         for(int64_t v = b; v < e; ++v)
-                         {
-                             int i = (int)(v % g.N), j = (int)((v / g.N) % g.N),
-                                 k = (int)(v / ((int64_t)g.N * g.N));
-                             Rng rng(d->seed, TAG_JITTER, (uint64_t)v);
-                             X[3 * v + 0] = i * h;
-                             X[3 * v + 1] = j * h;
-                             X[3 * v + 2] = k * h;
-                             for(int c = 0; c < 3; ++c)
-                                 x[3 * v + c] = X[3 * v + c] + d->jitter * h * rng.uniform(-1.0, 1.0);
-                         }
-                     });
+        std::vector<double>  X, x, mass;
+        std::vector<int32_t> tv;
+        build_mesh(d, g, T, X, x, tv, mass);
         if(positions)
             std::memcpy(positions, x.data(), sizeof(double) * 3 * g.V);
-
-        // tets (oriented by rest volume)
-        std::vector<int32_t> tv(4 * g.tets);
-        parallel_for(g.tets,
-                     T,
-                     [&](int64_t b, int64_t e)
-                     {
-                         for(int64_t t = b; t < e; ++t)
-                         {
-                             int32_t q[4];
-                             cell_tet(g, t, q);
-                             double Dm[9];
-                             for(int c = 0; c < 3; ++c)
-                                 for(int r = 0; r < 3; ++r)
-                                     Dm[r + 3 * c] = X[3 * q[c + 1] + r] - X[3 * q[0] + r];
-                             if(det3(Dm) < 0)
-                                 std::swap(q[2], q[3]);
-                             for(int c = 0; c < 4; ++c)
-                                 tv[4 * t + c] = q[c];
-                         }
-                     });
         if(tets)
             std::memcpy(tets, tv.data(), sizeof(int32_t) * 4 * g.tets);
-
-        // lumped mass: every tet of the regular grid has rest volume h^3 / 6
-        // or h^3 / 3 (the central tet); accumulate per vertex exactly
-        std::vector<double> mass(g.V, 0.0);
-        for(int64_t t = 0; t < g.tets; ++t)
-        {
-            const int32_t* q = &tv[4 * t];
-            double         Dm[9];
-            for(int c = 0; c < 3; ++c)
-                for(int r = 0; r < 3; ++r)
-                    Dm[r + 3 * c] = X[3 * q[c + 1] + r] - X[3 * q[0] + r];
-            double m = d->density * std::fabs(det3(Dm)) / 6.0 / 4.0;
-            for(int c = 0; c < 4; ++c)
-                mass[q[c]] += m;
-        }
 
         const int64_t off_fem     = g.V;
         const int64_t off_contact = g.V + 10 * g.tets;
@@ -529,7 +589,6 @@ int mi_synth_fem_fill(const mi_synth_fem_desc* d,
                 });
 
             // contacts
-            const int32_t nc = g.N - 1;
             parallel_for(
                 d->n_contacts,
                 T,
@@ -537,44 +596,9 @@ int mi_synth_fem_fill(const mi_synth_fem_desc* d,
                 {
                     for(int64_t s = b; s < e; ++s)
                     {
-                        Rng     rng(d->seed, TAG_CONTACT, (uint64_t)s);
-                        int64_t p  = rng.below(g.V);
-                        int     pi = (int)(p % g.N), pj = (int)((p / g.N) % g.N),
-                            pk = (int)(p / ((int64_t)g.N * g.N));
-                        int R      = std::max(1, d->contact_cells);
-                        auto pick  = [&](int pc)
-                        {
-                            int lo = std::max(0, pc - R), hi = std::min(nc - 1, pc + R - 1);
-                            if(hi < lo)
-                                hi = lo;
-                            return lo + (int)rng.below(hi - lo + 1);
-                        };
-                        int     ci = pick(pi), cj = pick(pj), ck = pick(pk);
-                        int64_t cell = ci + (int64_t)nc * (cj + (int64_t)nc * ck);
-                        int     sub  = (int)rng.below(5);
-                        int     f0   = (int)rng.below(4);
-                        const int32_t* tq = &tv[4 * (cell * 5 + sub)];
-                        int32_t        idx[4];
-                        idx[0] = (int32_t)p;
-                        for(int f = 0; f < 4; ++f)
-                        {
-                            int omit = (f0 + f) % 4;  // face = the 3 tet vertices except `omit`
-                            int o    = 1;
-                            bool has_p = false;
-                            for(int c = 0; c < 4; ++c)
-                                if(c != omit)
-                                {
-                                    idx[o++] = tq[c];
-                                    has_p |= (tq[c] == (int32_t)p);
-                                }
-                            if(!has_p)
-                                break;
-                        }
-                        double gv[12];
-                        for(int c = 0; c < 12; ++c)
-                            gv[c] = rng.normal();
-                        double kappa = d->kappa_lo
-                                       * std::pow(d->kappa_hi / d->kappa_lo, rng.uniform01());
+                        int32_t idx[4];
+                        double  gv[12], kappa;
+                        contact_stencil(d, g, tv.data(), s, idx, gv, kappa);
                         double K[12][12];
                         for(int r = 0; r < 12; ++r)
                             for(int c = 0; c < 12; ++c)
@@ -623,6 +647,57 @@ int mi_synth_fem_fill(const mi_synth_fem_desc* d,
                 part_ids[v] = rank[(i / 2) + nbx * ((j / 2) + nby * (k / 4))];
             }
         }
+        return 0;
+    }
+    catch(const std::exception& e)
+    {
+        g_err = std::string("mi_synth: ") + e.what();
+        return -1;
+    }
+}
+int mi_synth_fem_aux(const mi_synth_fem_desc* d,
+                     double*                  rest_positions,
+                     double*                  mass,
+                     int32_t*                 contact_idx,
+                     double*                  contact_g,
+                     double*                  contact_kappa)
+{
+    try
+    {
+        mi_synth_fem_sizes sz;
+        if(mi_synth_fem_sizes_of(d, &sz) != 0)
+            return -1;
+        Grid g;
+        g.N     = d->grid;
+        g.V     = sz.block_rows;
+        g.cells = (int64_t)(g.N - 1) * (g.N - 1) * (g.N - 1);
+        g.tets  = sz.n_tets;
+        const int            T = hw_threads(d->threads);
+        std::vector<double>  X, x, m;
+        std::vector<int32_t> tv;
+        build_mesh(d, g, T, X, x, tv, m);
+        if(rest_positions)
+            std::memcpy(rest_positions, X.data(), sizeof(double) * 3 * g.V);
+        if(mass)
+            std::memcpy(mass, m.data(), sizeof(double) * g.V);
+        if(contact_idx || contact_g || contact_kappa)
+            parallel_for(d->n_contacts,
+                         T,
+                         [&](int64_t b, int64_t e)
+                         {
+                             for(int64_t s = b; s < e; ++s)
+                             {
+                                 int32_t idx[4];
+                                 double  gv[12], kappa;
+                                 contact_stencil(d, g, tv.data(), s, idx, gv, kappa);
+                                 if(contact_idx)
+                                     std::memcpy(contact_idx + 4 * s, idx, sizeof(idx));
+                                 if(contact_g)
+                                     std::memcpy(contact_g + 12 * s, gv, sizeof(gv));
+                                 if(contact_kappa)
+                                     contact_kappa[s] = kappa;
+                             }
+                         });
         return 0;
     }
     catch(const std::exception& e)

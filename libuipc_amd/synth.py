@@ -71,3 +71,46 @@ def sizes(grid: int, n_contacts: int = 0) -> dict:
     if lib.mi_synth_fem_sizes_of(C.byref(d), C.byref(sz)) != 0:
         raise RuntimeError(lib.mi_synth_last_error().decode())
     return {"block_rows": sz.block_rows, "n_tets": sz.n_tets, "n_triplets": sz.n_triplets, "n_parts": sz.n_parts}
+
+
+@dataclass
+class ProducerInputs:
+    """What the device producers (include/mi_producers.h) need for the system
+    fem_system(grid, n_contacts, seed) generates on the host."""
+    rest_positions: np.ndarray  # float64[3B]
+    positions: np.ndarray  # float64[3B], deformed
+    mass: np.ndarray  # float64[B]
+    tets: np.ndarray  # int32[n_tets, 4]
+    contact_idx: np.ndarray  # int32[n_contacts, 4]
+    contact_g: np.ndarray  # float64[n_contacts, 12]
+    contact_kappa: np.ndarray  # float64[n_contacts]
+    mu: float
+    lam: float
+    dt: float
+
+
+def producer_inputs(grid: int, n_contacts: int = 0, seed: int = 1, **kw) -> ProducerInputs:
+    lib = abi.synth()
+    d = fem_desc(grid, n_contacts, seed, **kw)
+    sz = abi.SynthFemSizes()
+    if lib.mi_synth_fem_sizes_of(C.byref(d), C.byref(sz)) != 0:
+        raise RuntimeError(lib.mi_synth_last_error().decode())
+    B, nt = sz.block_rows, sz.n_tets
+    X = np.empty(3 * B)
+    x = np.empty(3 * B)
+    m = np.empty(B)
+    tets = np.empty((nt, 4), np.int32)
+    ci = np.empty((max(n_contacts, 1), 4), np.int32)
+    cg = np.empty((max(n_contacts, 1), 12))
+    ck = np.empty(max(n_contacts, 1))
+    if lib.mi_synth_fem_fill(C.byref(d), None, None, None, None, tets.ctypes.data_as(abi.c_i32p), None,
+                             x.ctypes.data_as(abi.c_f64p)) != 0:
+        raise RuntimeError(lib.mi_synth_last_error().decode())
+    if lib.mi_synth_fem_aux(C.byref(d), X.ctypes.data_as(abi.c_f64p), m.ctypes.data_as(abi.c_f64p),
+                            ci.ctypes.data_as(abi.c_i32p), cg.ctypes.data_as(abi.c_f64p),
+                            ck.ctypes.data_as(abi.c_f64p)) != 0:
+        raise RuntimeError(lib.mi_synth_last_error().decode())
+    E, nu = d.youngs, d.poisson
+    mu = E / (2.0 * (1.0 + nu))
+    lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu))
+    return ProducerInputs(X, x, m, tets, ci[:n_contacts], cg[:n_contacts], ck[:n_contacts], mu, lam, d.dt)

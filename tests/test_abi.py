@@ -18,10 +18,11 @@ def declared(header):
     return sorted(set(re.findall(r"\b(mi_[a-z0-9_]+)\s*\(", txt)))
 
 
-def test_linsys_exports_every_declared_symbol():
+@pytest.mark.parametrize("header", ["mi_linsys.h", "mi_producers.h"])
+def test_linsys_exports_every_declared_symbol(header):
     lib = abi.linsys()
-    names = declared("mi_linsys.h")
-    assert len(names) >= 25
+    names = declared(header)
+    assert len(names) >= (25 if header == "mi_linsys.h" else 3)
     for n in names:
         assert hasattr(lib, n), n
         assert n in abi.LINSYS_PROTOS, f"{n} has no ctypes prototype"
