@@ -161,7 +161,7 @@ __device__ __forceinline__ void emit_block(const int32_t idx[4],
                                            int32_t*      cols,
                                            double*       vals)
 {
-    const bool swap = !(idx[a] < idx[b]);
+    const bool swap = idx[a] > idx[b];
     rows[t]         = swap ? idx[b] : idx[a];
     cols[t]         = swap ? idx[a] : idx[b];
     double* v       = vals + 9 * t;
@@ -348,13 +348,19 @@ __global__ __launch_bounds__(kThreads) void contact_kernel(int64_t n,
 #pragma unroll
             for(int b = a; b < 4; ++b)
             {
-                double K[3][3];
+                // the block in its written orientation (L, R) = (a, b) or (b, a),
+                // each entry k g_(L, r) g_(R, c) as one product chain
+                const bool swap = idx[a] > idx[b];
+                const int  L = swap ? b : a, R = swap ? a : b;
+                rows[slot]   = idx[L];
+                cols[slot]   = idx[R];
+                double* v    = vals + 9 * slot;
 #pragma unroll
-                for(int r = 0; r < 3; ++r)
+                for(int c = 0; c < 3; ++c)
 #pragma unroll
-                    for(int c = 0; c < 3; ++c)
-                        K[r][c] = k * gv[3 * a + r] * gv[3 * b + c];
-                emit_block(idx, a, b, K, slot++, rows, cols, vals);
+                    for(int r = 0; r < 3; ++r)
+                        v[r + 3 * c] = k * gv[3 * L + r] * gv[3 * R + c];
+                ++slot;
             }
     }
 }
