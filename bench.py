@@ -46,6 +46,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-iters", type=int, default=30)
     ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--producers", choices=["host", "device"], default="host",
+                    help="device: write the triplets with the device Hessian producers (timed apart, outside "
+                         "the solve step) instead of uploading the host generator's")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch the PCG iterations directly instead of through captured HIP graphs")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
@@ -121,6 +124,9 @@ def main():
     ls = LinearSystem(ctx, s.block_rows, prec)
     rows, cols, vals = ctx.to_device(s.rows), ctx.to_device(s.cols), ctx.to_device(s.vals)
     b = ctx.to_device(s.rhs)
+    producers_ms = None
+    if args.producers == "device":
+        producers_ms = run_producers(ctx, s, args, rows, cols, vals)
     x = ctx.alloc(s.n, np.float64)
     precond = args.precond
     if precond == "mas":
@@ -218,6 +224,7 @@ def main():
                      "pcg": round(ms_solve, 3),
                      "per_iter_us": round(1e3 * ms_solve / max(1, st["iters"]), 2)},
         "wall_s_timed": round(wall, 3),
+        "producers_ms": producers_ms,
         "roofline": {"kernel": kname, "bound": "hbm", "achieved": round(gbs, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic(kname, args), "algorithmic_bytes": int(spmv_bytes),
@@ -229,6 +236,48 @@ def main():
     print(json.dumps(result), flush=True)
     ls.close()
     ctx.close()
+
+
+def run_producers(ctx, s, args, rows, cols, vals) -> dict:
+    """Overwrites the triplet buffers with the device producers' output for the
+    same system, times each producer with HIP events (median of 5) and checks
+    the result against the host generator's triplets."""
+    from libuipc_amd import synth
+    from libuipc_amd.abi import check
+    from libuipc_amd.hip_events import EventTimer
+
+    p = synth.producer_inputs(args.grid, args.contacts, seed=args.seed)
+    B, nt, nc = s.block_rows, len(p.tets), len(p.contact_kappa)
+    dm, dtets, dX, dx = (ctx.to_device(a) for a in (p.mass, p.tets, p.rest_positions, p.positions))
+    ci, cg, ck = (ctx.to_device(a) for a in (p.contact_idx, p.contact_g, p.contact_kappa)) if nc else (None,) * 3
+    lib, st = ctx.lib, ctx.stream()
+    o_fem, o_con = B, B + 10 * nt
+    r0, c0, v0 = rows.ptr.value, cols.ptr.value, vals.ptr.value
+    calls = {
+        "kinetic": lambda: lib.mi_produce_kinetic(ctx.h, B, dm.ptr, 0, r0, c0, v0, st),
+        "snh": lambda: lib.mi_produce_snh(ctx.h, nt, dtets.ptr, dX.ptr, dx.ptr, p.mu, p.lam, p.dt, 0,
+                                          r0 + 4 * o_fem, c0 + 4 * o_fem, v0 + 72 * o_fem, st),
+    }
+    if nc:
+        calls["contact"] = lambda: lib.mi_produce_contact(ctx.h, nc, ci.ptr, cg.ptr, ck.ptr, p.dt, 0, r0 + 4 * o_con,
+                                                          c0 + 4 * o_con, v0 + 72 * o_con, st)
+    ev = EventTimer(st)
+    out = {}
+    for name, fn in calls.items():
+        times = []
+        for _ in range(5):
+            ev.start()
+            check(fn())
+            times.append(ev.stop())
+        out[name] = round(float(np.median(times)), 3)
+    ctx.synchronize()
+    same = np.array_equal(rows.numpy(), s.rows) and np.array_equal(cols.numpy(), s.cols)
+    dv = float(np.abs(vals.numpy() - s.vals).max() / np.abs(s.vals).max())
+    log(f"device producers: {out} ms; structure equal to the host generator's: {same}; max rel value diff {dv:.1e}")
+    if not same or dv > 1e-9:
+        raise RuntimeError("device producers disagree with the host generator")
+    out["triplets_per_s"] = round((B + 10 * nt + 10 * nc) / (sum(out.values()) * 1e-3), 1)
+    return out
 
 
 def cpu_baseline(s, precond, args, gpu_iters):
