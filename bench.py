@@ -195,7 +195,7 @@ def main():
     sval = 4 if prec == abi.MI_FP32 else 8
     spmv_bytes = U * (9 * sval + 4) + (s.block_rows + 1) * 4 + 2 * s.n * 8
     gbs = spmv_bytes / (ms_spmv * 1e-3) / 1e9
-    kname = "spmv_chunk_kernel<double, true>" if prec == abi.MI_FP64 else "spmv_seg_kernel<float, true>"
+    kname = "spmv_chunk_kernel<double, true>" if prec == abi.MI_FP64 else "spmv_seg2_kernel<true>"
 
     result = {
         "metric": "ms per Newton solve to 1e-4 rel. residual; PCG iters; SpMV HBM GB/s vs peak",

@@ -164,7 +164,25 @@ typedef struct mi_precond_desc
     int32_t        mas_precision;     /* MI_FP32: float inverses + float multi-level R/Z
                                          (the reference's storage); MI_FP64: all double  */
     int32_t        mas_active_levels; /* 0 = all levels (MASPreconditionerEngine::set_active_level_num) */
+
+    /* Where the MAS adjacency, partition slots and level hierarchy are built
+     * (the init kernels of mas_preconditioner_engine.cu:57-466 and the host
+     * adjacency of fem_mas_preconditioner.cu:163-356).  Both give the same
+     * tables bit for bit.  Zero-initialise the descriptor: 0 is the host build. */
+    int32_t        mas_build;     /* mi_mas_build                                          */
+    /* MI_MAS_BUILD_DEVICE only: device copies of the arrays above; a non-NULL one
+     * is read instead of its host array (same layout and counts). */
+    const int32_t* part_ids_dev;
+    const int32_t* tets_dev;
+    const int32_t* tris_dev;
+    const int32_t* edges_dev;
 } mi_precond_desc;
+
+enum mi_mas_build
+{
+    MI_MAS_BUILD_HOST   = 0,
+    MI_MAS_BUILD_DEVICE = 1
+};
 
 /* GlobalLinearSystem's local-preconditioner registration + LocalPreconditioner::do_init.
  * Subsystem ranges must not overlap; DoFs no preconditioner covers get z = r
@@ -209,6 +227,30 @@ int mi_precond_mas_export(mi_sys*  sys,
  * allocation's 5% slack clusters too; this writes the clusters in use.
  * Synchronises. */
 int mi_precond_mas_dump(mi_sys* sys, int handle, const char* output_dir, int64_t frame, int64_t newton_iter);
+
+/* geometry::mesh_partition (src/geometry/mesh_partition.cpp:77-230), the MAS
+ * partition FEMMASPreconditioner::do_init computes with BANKSIZE
+ * (fem_mas_preconditioner.cu:205-249), on the device.  Same contract: the
+ * adjacency of the highest-dimensional simplices given (tets, else triangles,
+ * else edges); no adjacency -> part i / part_max_size; vert_num <=
+ * part_max_size -> connected-component labels (numbered by each component's
+ * lowest vertex); otherwise parts of at most part_max_size vertices, ids
+ * 0..P-1, none empty.  The k-way step is GPU handshake matching plus packing,
+ * not METIS, so the ids differ from the reference's; it is deterministic.
+ * Device arrays in and out (part_ids: vert_num int32); *n_parts_out = P
+ * (host, may be NULL).  Synchronises. */
+int mi_mesh_partition(mi_ctx*        ctx,
+                      int32_t        vert_num,
+                      const int32_t* tets,
+                      int64_t        n_tets,
+                      const int32_t* tris,
+                      int64_t        n_tris,
+                      const int32_t* edges,
+                      int64_t        n_edges,
+                      int32_t        part_max_size,
+                      int32_t*       part_ids,
+                      int32_t*       n_parts_out,
+                      void*          stream);
 
 /* ------------------------------------------------ FEM subsystem, around the solve */
 /* FEMLinearSubsystem_assemble_k1/k2_kernel (finite_element/fem_linear_subsystem.cu:23-53),

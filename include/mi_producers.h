@@ -72,6 +72,44 @@ int mi_produce_contact(mi_ctx*        ctx,
                        double*        vals,
                        void*          stream);
 
+/* ------------------------------------------------ line search, after the solve */
+/* LineSearchReporter::step_forward of a vertex set: x = x0 + alpha dx over n
+ * doubles (the FEM reporter's trial positions; engine/advance_ipc.cu:490-560
+ * calls it for every alpha it tries). */
+int mi_step_forward(mi_ctx* ctx, const double* x0, const double* dx, double alpha, double* x, int64_t n,
+                    void* stream);
+
+/* Line-search energies, LineSearcher::compute_energy
+ * (line_search/line_searcher.cu:60-124): each writes its total to *energy
+ * (one device double).  The sum runs over fixed per-workgroup partials and
+ * then one workgroup, so a given input always gives the same bits.  Calls on
+ * one context share its partials buffer: serialise them.
+ *   kinetic  sum_v 1/2 mass[v] |x_v - x_tilde_v|^2
+ *            (FiniteElementBDF1Kinetic_do_compute_energy_kernel,
+ *             finite_element/bdf/finite_element_bdf1_kinetic.cu:8-33)
+ *   SNH      sum_t dt^2 V_t Psi(F_t), V_t and F_t as mi_produce_snh,
+ *            Psi = 1/2 (mu (|F|^2 - 3) + lambda (J - 1 - mu/lambda)^2)
+ *            (StableNeoHookean3D_do_compute_energy_kernel,
+ *             finite_element/constitutions/stable_neo_hookean_3d.cu:22-52;
+ *             constitutions/detail/stable_neo_hookean_3d_snk1.inl:14-20) */
+int mi_energy_kinetic(mi_ctx*       ctx,
+                      int32_t       vert_num,
+                      const double* x,
+                      const double* x_tilde,
+                      const double* mass,
+                      double*       energy,
+                      void*         stream);
+int mi_energy_snh(mi_ctx*        ctx,
+                  int64_t        n_tets,
+                  const int32_t* tets,
+                  const double*  rest_positions,
+                  const double*  positions,
+                  double         mu,
+                  double         lambda,
+                  double         dt,
+                  double*        energy,
+                  void*          stream);
+
 #ifdef __cplusplus
 }
 #endif

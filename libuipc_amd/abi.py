@@ -27,6 +27,9 @@ MI_PRECOND_DIAG3 = 1
 MI_PRECOND_DIAG12 = 2
 MI_PRECOND_MAS = 3
 
+MI_MAS_BUILD_HOST = 0
+MI_MAS_BUILD_DEVICE = 1
+
 MI_CRIT_RZ_RATIO = 0
 MI_CRIT_REL_RESIDUAL = 1
 
@@ -54,6 +57,11 @@ class PrecondDesc(C.Structure):
         ("n_edges", C.c_int64),
         ("mas_precision", C.c_int32),
         ("mas_active_levels", C.c_int32),
+        ("mas_build", C.c_int32),
+        ("part_ids_dev", C.c_void_p),
+        ("tets_dev", C.c_void_p),
+        ("tris_dev", C.c_void_p),
+        ("edges_dev", C.c_void_p),
     ]
 
 
@@ -141,12 +149,20 @@ LINSYS_PROTOS = {
     "mi_sys_write_mtx": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int]),
     "mi_subsystem_fix_vertices": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32,
                                             C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
+    "mi_mesh_partition": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p,
+                                    C.c_int64, C.c_int32, C.c_void_p, C.POINTER(C.c_int32), C.c_void_p]),
     "mi_subsystem_distribute": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_void_p]),
     "mi_precond_mas_dump": (C.c_int, [C.c_void_p, C.c_int, C.c_char_p, C.c_int64, C.c_int64]),
     "mi_vec_write_mtx": (C.c_int, [C.c_void_p, C.c_char_p, C.c_void_p, C.c_int64]),
     "mi_mtx_read_block_triplets": (C.c_int, [C.c_char_p, C.c_int, c_i32p, C.POINTER(C.c_int64), c_i32p, c_i32p,
                                              c_f64p]),
     "mi_vec_read_mtx": (C.c_int, [C.c_char_p, C.POINTER(C.c_int64), c_f64p]),
+    "mi_step_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p, C.c_int64,
+                                  C.c_void_p]),
+    "mi_energy_kinetic": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                    C.c_void_p]),
+    "mi_energy_snh": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double,
+                                C.c_double, C.c_void_p, C.c_void_p]),
     "mi_produce_kinetic": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
                                      C.c_void_p, C.c_void_p]),
     "mi_produce_snh": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double,

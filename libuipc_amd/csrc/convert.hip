@@ -661,25 +661,49 @@ int MatrixConverter::compact_sort(int64_t        n,
                                                         srcs_[0].data());
     // stable LSD radix sort (matrix_converter.inl:1610-1616)
     const int bits = key_bits_for((uint64_t)B * (uint64_t)B - 1ull);
-    int       cur  = 0;
+    return radix_sort_pairs(keys_[0].data(), srcs_[0].data(), keys_[1].data(), srcs_[1].data(), d_count, n, bits,
+                            hist_, chunk_, s);
+}
+
+int radix_sort_pairs(uint64_t*       k0,
+                     int32_t*        v0,
+                     uint64_t*       k1,
+                     int32_t*        v1,
+                     const int64_t*  d_n,
+                     int64_t         nmax,
+                     int             bits,
+                     DBuf<uint32_t>& hist,
+                     DBuf<uint32_t>& chunk,
+                     hipStream_t     s)
+{
+    const int ntiles = ceil_div(nmax, kTile);
+    if(ntiles == 0)
+        return 0;
+    hist.resize((size_t)256 * ntiles);
+    uint64_t* kb[2] = {k0, k1};
+    int32_t*  vb[2] = {v0, v1};
+    int       cur   = 0;
     for(int shift = 0; shift < bits; shift += 8)
     {
         int      w     = (bits - shift) < 8 ? (bits - shift) : 8;
         uint32_t dmask = (1u << w) - 1u;
-        rs_hist_kernel<<<ntiles, kThreads, 0, s>>>(d_count, keys_[cur].data(), shift, dmask, hist_.data(), ntiles);
-        scan_exclusive(hist_.data(), (int64_t)256 * ntiles, chunk_, nullptr, s);
-        rs_scatter_kernel<<<ntiles, kThreads, 0, s>>>(d_count,
-                                                      keys_[cur].data(),
-                                                      srcs_[cur].data(),
-                                                      keys_[cur ^ 1].data(),
-                                                      srcs_[cur ^ 1].data(),
-                                                      shift,
-                                                      dmask,
-                                                      hist_.data(),
-                                                      ntiles);
+        rs_hist_kernel<<<ntiles, kThreads, 0, s>>>(d_n, kb[cur], shift, dmask, hist.data(), ntiles);
+        scan_exclusive(hist.data(), (int64_t)256 * ntiles, chunk, nullptr, s);
+        rs_scatter_kernel<<<ntiles, kThreads, 0, s>>>(d_n, kb[cur], vb[cur], kb[cur ^ 1], vb[cur ^ 1], shift, dmask,
+                                                      hist.data(), ntiles);
         cur ^= 1;
     }
     return cur;
+}
+
+void exclusive_scan_u32(uint32_t* data, int64_t n, DBuf<uint32_t>& chunk, int64_t* d_total, hipStream_t s)
+{
+    scan_exclusive(data, n, chunk, d_total, s);
+}
+
+int radix_key_bits(uint64_t max_key)
+{
+    return key_bits_for(max_key);
 }
 
 void MatrixConverter::convert_sym(int32_t        B,

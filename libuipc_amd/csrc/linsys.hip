@@ -11,6 +11,7 @@
 #include <string>
 
 #include "linsys.hpp"
+#include "mas.hpp"
 
 namespace mi
 {
@@ -43,8 +44,9 @@ using namespace mi;
 
 struct mi_ctx
 {
-    int         device = 0;
-    hipStream_t stream = nullptr;
+    int          device = 0;
+    hipStream_t  stream = nullptr;
+    DBuf<double> energy_part;  // per-workgroup partial energies
 };
 
 struct mi_sys
@@ -559,6 +561,29 @@ int mi_subsystem_fix_vertices(mi_ctx*        ctx,
     MI_API_END
 }
 
+int mi_mesh_partition(mi_ctx*        ctx,
+                      int32_t        vert_num,
+                      const int32_t* tets,
+                      int64_t        n_tets,
+                      const int32_t* tris,
+                      int64_t        n_tris,
+                      const int32_t* edges,
+                      int64_t        n_edges,
+                      int32_t        part_max_size,
+                      int32_t*       part_ids,
+                      int32_t*       n_parts_out,
+                      void*          stream)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(ctx, MI_ERR_ARG, "mi_mesh_partition: null context");
+    MI_REQUIRE(n_tets >= 0 && n_tris >= 0 && n_edges >= 0, MI_ERR_ARG, "mi_mesh_partition: negative count");
+    const int32_t p = mesh_partition_device(vert_num, tets, n_tets, tris, n_tris, edges, n_edges, part_max_size,
+                                            part_ids, pick(ctx, stream));
+    if(n_parts_out)
+        *n_parts_out = p;
+    MI_API_END
+}
+
 int mi_subsystem_distribute(mi_ctx* ctx, const double* x, int64_t dof_offset, int32_t vert_num, double* dxs,
                             void* stream)
 {
@@ -606,6 +631,49 @@ int mi_produce_snh(mi_ctx*        ctx,
                "mi_produce_snh: null buffer");
     produce_snh(n_tets, tets, rest_positions, positions, mu, lambda, dt, vert_offset, rows, cols, vals,
                 pick(ctx, stream));
+    MI_API_END
+}
+
+int mi_energy_kinetic(mi_ctx*       ctx,
+                      int32_t       vert_num,
+                      const double* x,
+                      const double* x_tilde,
+                      const double* mass,
+                      double*       energy,
+                      void*         stream)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(ctx && vert_num >= 0 && energy, MI_ERR_ARG, "mi_energy_kinetic: bad argument");
+    MI_REQUIRE(vert_num == 0 || (x && x_tilde && mass), MI_ERR_ARG, "mi_energy_kinetic: null buffer");
+    energy_kinetic(vert_num, x, x_tilde, mass, energy, ctx->energy_part, pick(ctx, stream));
+    MI_API_END
+}
+
+int mi_energy_snh(mi_ctx*        ctx,
+                  int64_t        n_tets,
+                  const int32_t* tets,
+                  const double*  rest_positions,
+                  const double*  positions,
+                  double         mu,
+                  double         lambda,
+                  double         dt,
+                  double*        energy,
+                  void*          stream)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(ctx && n_tets >= 0 && energy, MI_ERR_ARG, "mi_energy_snh: bad argument");
+    MI_REQUIRE(mu > 0.0 && lambda > 0.0 && dt > 0.0, MI_ERR_ARG, "mi_energy_snh: mu, lambda and dt must be > 0");
+    MI_REQUIRE(n_tets == 0 || (tets && rest_positions && positions), MI_ERR_ARG, "mi_energy_snh: null buffer");
+    energy_snh(n_tets, tets, rest_positions, positions, mu, lambda, dt, energy, ctx->energy_part, pick(ctx, stream));
+    MI_API_END
+}
+
+int mi_step_forward(mi_ctx* ctx, const double* x0, const double* dx, double alpha, double* x, int64_t n, void* stream)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(ctx && n >= 0, MI_ERR_ARG, "mi_step_forward: bad argument");
+    MI_REQUIRE(n == 0 || (x0 && dx && x), MI_ERR_ARG, "mi_step_forward: null buffer");
+    step_forward(x0, dx, alpha, x, n, pick(ctx, stream));
     MI_API_END
 }
 

@@ -75,6 +75,24 @@ class MatrixConverter
     int64_t*       pinned_ = nullptr;
 };
 
+// Stable LSD radix sort (8-bit digits) of *d_n (device count, at most nmax)
+// (key, value) pairs on the low `bits` key bits; input in (k0, v0), (k1, v1)
+// is the other buffer.  Returns 0 or 1: the buffer that holds the result.
+int radix_sort_pairs(uint64_t*       k0,
+                     int32_t*        v0,
+                     uint64_t*       k1,
+                     int32_t*        v1,
+                     const int64_t*  d_n,
+                     int64_t         nmax,
+                     int             bits,
+                     DBuf<uint32_t>& hist,
+                     DBuf<uint32_t>& chunk,
+                     hipStream_t     s);
+// bits needed for keys up to max_key
+int radix_key_bits(uint64_t max_key);
+// in-place exclusive scan of n uint32 values; *d_total (device, may be NULL) = sum
+void exclusive_scan_u32(uint32_t* data, int64_t n, DBuf<uint32_t>& chunk, int64_t* d_total, hipStream_t s);
+
 // ----------------------------------------------------------------- SpMV
 // Spmv::rbk_sym_spmv_dot (linear_system/spmv.cu:1023-1159)
 struct SpmvScratch
@@ -128,6 +146,19 @@ void produce_snh(int64_t        n_tets,
                  int32_t*       cols,
                  double*        vals,
                  hipStream_t    s);
+void energy_kinetic(int32_t V, const double* x, const double* xt, const double* m, double* energy,
+                    DBuf<double>& part, hipStream_t s);
+void energy_snh(int64_t        n_tets,
+                const int32_t* tets,
+                const double*  X,
+                const double*  x,
+                double         mu,
+                double         lambda,
+                double         dt,
+                double*        energy,
+                DBuf<double>&  part,
+                hipStream_t    s);
+void step_forward(const double* x0, const double* dx, double alpha, double* x, int64_t n, hipStream_t s);
 void produce_contact(int64_t        n,
                      const int32_t* stencils,
                      const double*  g,

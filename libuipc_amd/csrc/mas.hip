@@ -5,11 +5,13 @@
 // BANKSIZE = 16 nodes (48 DoFs) per cluster, at most 6 levels, upper-triangle
 // cluster storage of 136 3x3 blocks (mas_preconditioner_engine.h:27-45).
 //
-// Init (once per topology, host C++): adjacency from the simplices, partition
-// slots (part_to_real / real_to_part) and the level hierarchy (connect masks,
+// Init (once per topology): adjacency from the simplices, partition slots
+// (part_to_real / real_to_part) and the level hierarchy (connect masks,
 // closure inside each 16-bank, elected-lane prefix -> going_next, level sizes,
 // per-vertex level tables), restating the reference's integer kernels
 // (mas_preconditioner_engine.cu:57-466, 1958-2078) -- bit-identical tables.
+// Built here in host C++, or by the device kernels of mas_init.hip
+// (MI_MAS_BUILD_DEVICE), which give the same tables.
 //
 // Assembly (per Newton iteration, GPU):
 //   cluster matrices at level L are the in-bank blocks of the Galerkin operator
@@ -44,16 +46,16 @@
 #include <numeric>
 #include <type_traits>
 
-#include "linsys.hpp"
+#include "mas.hpp"
 
 namespace mi
 {
 namespace
 {
-constexpr int kBank   = 16;
+constexpr int kBank   = kMasBank;
 constexpr int kSym    = kBank * (kBank + 1) / 2;  // 136
 constexpr int kCm     = kSym * 9;                 // 1224 scalars per cluster
-constexpr int kMaxLvl = 6;
+constexpr int kMaxLvl = kMasMaxLvl;
 // the MAS restriction kernels run up to the first level with at most this many
 // nodes; coarse clusters above it sum their nodes' descendants at that level
 constexpr int32_t kTailLevel = 4096;
@@ -72,7 +74,7 @@ __host__ __device__ __forceinline__ int inv_at(int blk, int q)
 
 inline int bank_align(int n)
 {
-    return (n + kBank - 1) / kBank * kBank;
+    return mas_bank_align(n);
 }
 inline int popc(uint32_t x)
 {
@@ -88,28 +90,7 @@ inline uint32_t lt_mask(int lane)
 }
 
 // ------------------------------------------------------------------ host init
-struct Hierarchy
-{
-    int32_t               V = 0, N = 0, L = 0, total = 0, level1_begin = 0, padded = 0;
-    std::vector<int32_t>  part_to_real, real_to_part, going_next, prefix_orig;
-    std::vector<uint32_t> fine_connect;
-    std::vector<int32_t>  level_count, level_begin;  // [0..L]
-    std::vector<int32_t>  table;                     // V * kMaxLvl
-    bool                  has_unpart = false;
-};
-
-int compute_num_levels(int vert_num)  // mas_preconditioner_engine.cu:1958-1972
-{
-    int n_level = 1, sz = bank_align(vert_num);
-    while(sz > kBank)
-    {
-        sz /= kBank;
-        n_level++;
-        sz = bank_align(sz);
-    }
-    n_level++;
-    return std::min(n_level, kMaxLvl);
-}
+using Hierarchy = MasHierarchy;
 
 uint32_t bank_closure(const uint32_t cache[kBank], int lane, uint32_t start)
 {
@@ -195,7 +176,7 @@ Hierarchy build_hierarchy(const mi_precond_desc& d)
     // init_neighbor (1974-2024)
     const int max_nodes = std::max(N, V);
     h.padded            = bank_align(max_nodes);
-    h.L                 = compute_num_levels(max_nodes);
+    h.L                 = mas_num_levels(max_nodes);
     const int L         = h.L;
     h.going_next.assign((size_t)h.padded * L, -1);
     std::vector<int32_t> coarse((size_t)V * L, -1);
@@ -1072,14 +1053,18 @@ class MasPreconditioner final : public LocalPreconditioner
 
     explicit MasPreconditioner(const mi_precond_desc& d)
     {
-        MI_REQUIRE(d.part_ids_host != nullptr, MI_ERR_ARG, "MAS: part_ids_host is required");
+        MI_REQUIRE(d.mas_build == MI_MAS_BUILD_HOST || d.mas_build == MI_MAS_BUILD_DEVICE, MI_ERR_ARG,
+                   "MAS: unknown build location");
+        const bool on_device = d.mas_build == MI_MAS_BUILD_DEVICE;
+        MI_REQUIRE(d.part_ids_host != nullptr || (on_device && d.part_ids_dev != nullptr), MI_ERR_ARG,
+                   "MAS: partition ids are required");
         MI_REQUIRE(d.vert_num == d.dof_count / 3, MI_ERR_ARG, "MAS: vert_num must equal dof_count / 3");
         MI_REQUIRE(d.mas_precision == MI_FP64 || d.mas_precision == MI_FP32, MI_ERR_ARG, "MAS: unknown precision");
         dof_offset    = d.dof_offset;
         dof_count     = d.dof_count;
         fp32          = d.mas_precision == MI_FP32;
         active_levels = d.mas_active_levels;
-        h             = build_hierarchy(d);
+        h             = on_device ? mas_build_hierarchy_device(d) : build_hierarchy(d);
         const int L   = h.L;
         C             = h.total / kBank;
         C0            = h.N / kBank;

@@ -14,6 +14,9 @@
 //            are clamped to 0
 //   contact  dt^2 kappa g g^T per vertex-triangle stencil (the synthetic
 //            stand-in for contact_system/simplex_normal_contact.cu:74-161)
+// and, for the line search after the solve, the kinetic and SNH energies
+// (each reduced in a fixed order: per-workgroup partials, then one workgroup)
+// and the trial step x = x0 + alpha dx.
 //
 // Each 4-node stencil writes its 10 upper blocks as the reference's half-block
 // writer does (utils/matrix_assembler.h:312-437): slot (a, b), a <= b, in the
@@ -364,7 +367,137 @@ __global__ __launch_bounds__(kThreads) void contact_kernel(int64_t n,
             }
     }
 }
+// ---- line-search side: energies and the trial step
+constexpr int kEnergyBlocks = 1024;
+
+int energy_grid(int64_t n)
+{
+    return (int)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, kThreads), kEnergyBlocks));
+}
+
+// FiniteElementBDF1Kinetic_do_compute_energy_kernel: 1/2 m |x - x_tilde|^2
+__global__ __launch_bounds__(kThreads) void kinetic_energy_kernel(int32_t V,
+                                                                 const double* __restrict__ x,
+                                                                 const double* __restrict__ xt,
+                                                                 const double* __restrict__ m,
+                                                                 double* part)
+{
+    __shared__ double lds[kThreads / kWave];
+    double            acc = 0.0;
+    for(int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (int64_t)gridDim.x * blockDim.x)
+    {
+        const double d0 = x[3 * v] - xt[3 * v], d1 = x[3 * v + 1] - xt[3 * v + 1], d2 = x[3 * v + 2] - xt[3 * v + 2];
+        acc += 0.5 * m[v] * (d0 * d0 + d1 * d1 + d2 * d2);
+    }
+    acc = block_sum<kThreads>(acc, lds);
+    if(threadIdx.x == 0)
+        part[blockIdx.x] = acc;
+}
+
+// StableNeoHookean3D_do_compute_energy_kernel: dt^2 V_rest Psi(F),
+// Psi = 1/2 (mu (|F|^2 - 3) + lambda (J - 1 - mu / lambda)^2), J = det F
+__global__ __launch_bounds__(kThreads) void snh_energy_kernel(int64_t n_tets,
+                                                             const int32_t* __restrict__ tets,
+                                                             const double* __restrict__ X,
+                                                             const double* __restrict__ x,
+                                                             double  mu,
+                                                             double  lambda,
+                                                             double  dt2,
+                                                             double* part)
+{
+    __shared__ double lds[kThreads / kWave];
+    double            acc = 0.0;
+    for(int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n_tets; t += (int64_t)gridDim.x * blockDim.x)
+    {
+        int32_t q[4];
+#pragma unroll
+        for(int c = 0; c < 4; ++c)
+            q[c] = tets[4 * t + c];
+        double Dm[9], Ds[9], Dmi[9], F[9];
+#pragma unroll
+        for(int c = 0; c < 3; ++c)
+#pragma unroll
+            for(int r = 0; r < 3; ++r)
+            {
+                Dm[r + 3 * c] = X[3 * (int64_t)q[c + 1] + r] - X[3 * (int64_t)q[0] + r];
+                Ds[r + 3 * c] = x[3 * (int64_t)q[c + 1] + r] - x[3 * (int64_t)q[0] + r];
+            }
+        const double vol = fabs(det3(Dm)) / 6.0;
+        inv3(Dm, Dmi);
+        double Ic = 0.0;
+#pragma unroll
+        for(int c = 0; c < 3; ++c)
+#pragma unroll
+            for(int r = 0; r < 3; ++r)
+            {
+                F[r + 3 * c] = Ds[r] * Dmi[3 * c] + Ds[r + 3] * Dmi[1 + 3 * c] + Ds[r + 6] * Dmi[2 + 3 * c];
+                Ic += F[r + 3 * c] * F[r + 3 * c];
+            }
+        const double Jm = det3(F) - 1.0 - mu / lambda;
+        acc += 0.5 * (mu * (Ic - 3.0) + lambda * Jm * Jm) * dt2 * vol;
+    }
+    acc = block_sum<kThreads>(acc, lds);
+    if(threadIdx.x == 0)
+        part[blockIdx.x] = acc;
+}
+
+// the partials in a fixed order (one workgroup), so the total is reproducible
+__global__ __launch_bounds__(kThreads) void energy_total_kernel(const double* __restrict__ part, int n, double* out)
+{
+    __shared__ double lds[kThreads / kWave];
+    double            acc = 0.0;
+    for(int i = threadIdx.x; i < n; i += kThreads)
+        acc += part[i];
+    acc = block_sum<kThreads>(acc, lds);
+    if(threadIdx.x == 0)
+        *out = acc;
+}
+
+__global__ __launch_bounds__(kThreads) void step_forward_kernel(const double* __restrict__ x0,
+                                                               const double* __restrict__ dx,
+                                                               double  alpha,
+                                                               double* x,
+                                                               int64_t n)
+{
+    for(int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        x[i] = x0[i] + alpha * dx[i];
+}
 }  // namespace
+
+void energy_kinetic(int32_t V, const double* x, const double* xt, const double* m, double* energy,
+                    DBuf<double>& part, hipStream_t s)
+{
+    const int g = energy_grid(V);
+    part.resize(kEnergyBlocks);
+    kinetic_energy_kernel<<<g, kThreads, 0, s>>>(V, x, xt, m, part.data());
+    energy_total_kernel<<<1, kThreads, 0, s>>>(part.data(), g, energy);
+    MI_HIP(hipGetLastError());
+}
+
+void energy_snh(int64_t        n_tets,
+                const int32_t* tets,
+                const double*  X,
+                const double*  x,
+                double         mu,
+                double         lambda,
+                double         dt,
+                double*        energy,
+                DBuf<double>&  part,
+                hipStream_t    s)
+{
+    const int g = energy_grid(n_tets);
+    part.resize(kEnergyBlocks);
+    snh_energy_kernel<<<g, kThreads, 0, s>>>(n_tets, tets, X, x, mu, lambda, dt * dt, part.data());
+    energy_total_kernel<<<1, kThreads, 0, s>>>(part.data(), g, energy);
+    MI_HIP(hipGetLastError());
+}
+
+void step_forward(const double* x0, const double* dx, double alpha, double* x, int64_t n, hipStream_t s)
+{
+    if(n > 0)
+        step_forward_kernel<<<grid_of(n), kThreads, 0, s>>>(x0, dx, alpha, x, n);
+    MI_HIP(hipGetLastError());
+}
 
 void produce_kinetic(int32_t V, const double* mass, int32_t vert_offset, int32_t* rows, int32_t* cols, double* vals,
                      hipStream_t s)

@@ -74,6 +74,34 @@ class Context:
     def write_vector_mtx(self, path: str, x: "DeviceArray") -> None:
         check(self.lib.mi_vec_write_mtx(self.h, os.fsencode(path), x.ptr, x.size))
 
+    # geometry::mesh_partition (src/geometry/mesh_partition.cpp:77-230), on the device
+    def mesh_partition(self, vert_num: int, part_max_size: int = 16, *, tets=None, tris=None,
+                       edges=None) -> tuple[np.ndarray, int]:
+        keep = []
+
+        def dev(a, k):
+            if a is None:
+                return None, 0
+            a = np.ascontiguousarray(a, dtype=np.int32).reshape(-1)
+            d = self.to_device(a) if a.size else None
+            keep.append(d)
+            return (d.ptr if d is not None else None), a.size // k
+
+        tp, nt = dev(tets, 4)
+        rp, nr = dev(tris, 3)
+        ep, ne = dev(edges, 2)
+        out = self.alloc(max(vert_num, 1), np.int32)
+        n = C.c_int32(0)
+        try:
+            check(self.lib.mi_mesh_partition(self.h, vert_num, tp, nt, rp, nr, ep, ne, part_max_size, out.ptr,
+                                             C.byref(n), None))
+            return out.numpy()[:vert_num], int(n.value)
+        finally:
+            out.free()
+            for d in keep:
+                if d is not None:
+                    d.free()
+
 
 class DeviceArray:
     def __init__(self, ctx: Context, shape, dtype):
@@ -172,7 +200,11 @@ class LinearSystem:
         check(self.lib.mi_spmv(self.h, a, x.ptr, b, y.ptr, None))
 
     def add_preconditioner(self, kind: int, dof_offset: int, dof_count: int, *, part_ids=None, tets=None,
-                           tris=None, edges=None, mas_precision: int = abi.MI_FP64, active_levels: int = 0) -> int:
+                           tris=None, edges=None, mas_precision: int = abi.MI_FP64, active_levels: int = 0,
+                           build: int = abi.MI_MAS_BUILD_HOST, device_arrays: dict | None = None) -> int:
+        """`device_arrays` (MI_MAS_BUILD_DEVICE only) maps "part_ids", "tets",
+        "tris", "edges" to device pointers read instead of the host arrays;
+        their counts still come from the host arrays' shapes."""
         keep = []
 
         def arr(a, k):
@@ -186,8 +218,10 @@ class LinearSystem:
         tp, nt = arr(tets, 4)
         trp, ntr = arr(tris, 3)
         ep, ne = arr(edges, 2)
+        dev = {k: (v.value if isinstance(v, C.c_void_p) else v) for k, v in (device_arrays or {}).items()}
         d = abi.PrecondDesc(kind, dof_offset, dof_count, dof_count // 3, pp, tp, nt, trp, ntr, ep, ne,
-                            mas_precision, active_levels)
+                            mas_precision, active_levels, build, dev.get("part_ids"), dev.get("tets"),
+                            dev.get("tris"), dev.get("edges"))
         h = C.c_int()
         check(self.lib.mi_precond_add(self.h, C.byref(d), C.byref(h)))
         return h.value

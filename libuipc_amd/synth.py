@@ -64,6 +64,28 @@ def fem_system(grid: int, n_contacts: int = 0, seed: int = 1, **kw) -> FemSystem
     return FemSystem(grid, B, rows, cols, vals, rhs, tets, parts, n_contacts)
 
 
+@dataclass
+class Topology:
+    """Mesh and MAS partition of fem_system(grid, ..., seed), without the matrix."""
+    block_rows: int
+    tets: np.ndarray  # int32[n_tets, 4]
+    part_ids: np.ndarray  # int32[B]
+
+
+def generate_topology(grid: int, seed: int = 1, **kw) -> Topology:
+    lib = abi.synth()
+    d = fem_desc(grid, 0, seed, **kw)
+    sz = abi.SynthFemSizes()
+    if lib.mi_synth_fem_sizes_of(C.byref(d), C.byref(sz)) != 0:
+        raise RuntimeError(lib.mi_synth_last_error().decode())
+    tets = np.empty((sz.n_tets, 4), np.int32)
+    parts = np.empty(sz.block_rows, np.int32)
+    if lib.mi_synth_fem_fill(C.byref(d), None, None, None, None, tets.ctypes.data_as(abi.c_i32p),
+                             parts.ctypes.data_as(abi.c_i32p), None) != 0:
+        raise RuntimeError(lib.mi_synth_last_error().decode())
+    return Topology(sz.block_rows, tets, parts)
+
+
 def sizes(grid: int, n_contacts: int = 0) -> dict:
     lib = abi.synth()
     d = fem_desc(grid, n_contacts)

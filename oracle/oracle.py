@@ -35,7 +35,8 @@ _lib = None
 
 
 def build() -> str:
-    r = subprocess.run(["make", "-s", "-C", HERE], capture_output=True, text=True)
+    # `ref` builds oracle/_ref/ only where the reference sources are present
+    r = subprocess.run(["make", "-s", "-j8", "-C", HERE, "all", "ref"], capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"oracle build failed:\n{r.stdout}\n{r.stderr}")
     return SO
@@ -269,3 +270,42 @@ def pcg(B, rows, cols, vals, b, precond=None, *, criterion=0, tol=1e-3, max_iter
                  arr, len(precond), b.ctypes.data_as(f64p), x.ctypes.data_as(f64p), criterion, tol, max_iter_ratio,
                  check_interval, psd_fallback, threads, max_iter_cap, C.byref(res))
     return x, res.as_dict()
+
+
+# ---- the reference's own partition (oracle/_ref/libref_partition.so: the
+# reference's METIS sources compiled in place, mesh_partition's control flow)
+REF_PART_SO = os.path.join(HERE, "_ref", "libref_partition.so")
+_ref_part = None
+
+
+def ref_partition_available() -> bool:
+    return os.path.exists(REF_PART_SO)
+
+
+def ref_mesh_partition(V: int, part_max_size: int = 16, *, tets=None, tris=None, edges=None) -> np.ndarray:
+    """geometry::mesh_partition (src/geometry/mesh_partition.cpp:77-230) as
+    the reference computes it, METIS included."""
+    global _ref_part
+    if _ref_part is None:
+        L = C.CDLL(REF_PART_SO)
+        L.ref_mesh_partition.restype = C.c_int
+        L.ref_mesh_partition.argtypes = [C.c_int32, i32p, C.c_int64, i32p, C.c_int64, i32p, C.c_int64, C.c_int32,
+                                         i32p]
+        _ref_part = L
+    keep = []
+
+    def arr(a, k):
+        if a is None:
+            return None, 0
+        a = np.ascontiguousarray(a, dtype=np.int32).reshape(-1)
+        keep.append(a)
+        return a.ctypes.data_as(i32p), a.size // k
+
+    tp, nt = arr(tets, 4)
+    rp, nr = arr(tris, 3)
+    ep, ne = arr(edges, 2)
+    out = np.empty(max(V, 1), np.int32)
+    rc = _ref_part.ref_mesh_partition(V, tp, nt, rp, nr, ep, ne, part_max_size, out.ctypes.data_as(i32p))
+    if rc != 0:
+        raise ValueError("mesh_partition: part_max_size must be greater than zero")
+    return out[:V]

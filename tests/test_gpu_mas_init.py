@@ -1,0 +1,114 @@
+"""The MAS hierarchy built by device kernels (MI_MAS_BUILD_DEVICE) against the
+host build and the oracle: every table bit-exact, on tet meshes with
+geometric, random (disconnected parts) and partly unpartitioned partitions,
+on triangle / edge topologies, with device-resident inputs, and at the
+64^3-grid size."""
+import time
+
+import numpy as np
+import pytest
+
+from libuipc_amd import abi, synth
+from libuipc_amd.abi import MiError
+from libuipc_amd.device import LinearSystem
+from oracle import oracle as O
+from tests.test_oracle_mas import random_partition, system
+
+pytestmark = pytest.mark.gpu
+
+TABLES = ("part_to_real", "real_to_part", "going_next", "level_sizes", "coarse_tables")
+
+
+def built(ctx, n_vert, part_ids, build, **topo):
+    ls = LinearSystem(ctx, n_vert)
+    t0 = time.perf_counter()
+    h = ls.add_preconditioner(abi.MI_PRECOND_MAS, 0, 3 * n_vert, part_ids=part_ids, build=build, **topo)
+    ms = (time.perf_counter() - t0) * 1e3
+    return ls.mas_export(h, matrices=False), ms
+
+
+def assert_same(a, b):
+    for k in ("V", "N", "L", "total", "level1_begin", "clusters"):
+        assert a[k] == b[k], k
+    for k in TABLES:
+        assert np.array_equal(a[k], b[k]), k
+
+
+@pytest.mark.parametrize("grid,partition", [(6, "geometric"), (7, "random"), (6, "unpart"), (16, "geometric"),
+                                            (12, "random")])
+def test_device_build_matches_host_and_oracle(ctx, grid, partition):
+    s, _ = system(grid, 0, 21, partition)
+    host, _ = built(ctx, s.block_rows, s.part_ids, abi.MI_MAS_BUILD_HOST, tets=s.tets)
+    dev, _ = built(ctx, s.block_rows, s.part_ids, abi.MI_MAS_BUILD_DEVICE, tets=s.tets)
+    assert_same(host, dev)
+    t = O.Mas(s.block_rows, s.part_ids, s.tets).tables()
+    for k in TABLES:
+        assert np.array_equal(dev[k], t[k]), k
+
+
+def random_topology(rng, V, n_tris, n_edges):
+    tris = rng.integers(0, V, size=(n_tris, 3)).astype(np.int32)
+    edges = rng.integers(0, V, size=(n_edges, 2)).astype(np.int32)
+    # degenerate and out-of-range entries are skipped by both builds
+    tris[::17, 1] = tris[::17, 0]
+    edges[::13, 1] = V + 5
+    edges[::29, 0] = -1
+    return tris, edges
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_device_build_triangles_and_edges(ctx, seed):
+    rng = np.random.default_rng(seed)
+    V = 3000
+    tris, edges = random_topology(rng, V, 4000, 2500)
+    parts = random_partition(rng, V, None)
+    parts[rng.random(V) < 0.05] = -1
+    host, _ = built(ctx, V, parts, abi.MI_MAS_BUILD_HOST, tris=tris, edges=edges)
+    dev, _ = built(ctx, V, parts, abi.MI_MAS_BUILD_DEVICE, tris=tris, edges=edges)
+    assert_same(host, dev)
+
+
+def test_device_build_from_device_arrays(ctx):
+    s, _ = system(10, 0, 22, "random")
+    host, _ = built(ctx, s.block_rows, s.part_ids, abi.MI_MAS_BUILD_HOST, tets=s.tets)
+    d_tets, d_part = ctx.to_device(s.tets.reshape(-1)), ctx.to_device(s.part_ids)
+    ls = LinearSystem(ctx, s.block_rows)
+    # the host arrays give the counts; their contents are replaced by zeros so
+    # that only the device copies can produce the host build's tables
+    h = ls.add_preconditioner(abi.MI_PRECOND_MAS, 0, s.n, part_ids=np.zeros_like(s.part_ids),
+                              tets=np.zeros_like(s.tets), build=abi.MI_MAS_BUILD_DEVICE,
+                              device_arrays={"part_ids": d_part.ptr, "tets": d_tets.ptr})
+    assert_same(host, ls.mas_export(h, matrices=False))
+
+
+def test_device_build_no_topology_and_all_unpartitioned(ctx):
+    V = 100
+    parts = np.arange(V, dtype=np.int32) // 16
+    host, _ = built(ctx, V, parts, abi.MI_MAS_BUILD_HOST)
+    dev, _ = built(ctx, V, parts, abi.MI_MAS_BUILD_DEVICE)
+    assert_same(host, dev)
+    parts = np.full(V, -1, np.int32)
+    host, _ = built(ctx, V, parts, abi.MI_MAS_BUILD_HOST)
+    dev, _ = built(ctx, V, parts, abi.MI_MAS_BUILD_DEVICE)
+    assert_same(host, dev)
+    assert dev["N"] == 0
+
+
+def test_device_build_rejects_oversized_partition(ctx):
+    V = 64
+    parts = np.zeros(V, np.int32)
+    parts[:17] = 0
+    parts[17:] = 1 + np.arange(V - 17, dtype=np.int32) // 16
+    ls = LinearSystem(ctx, V)
+    with pytest.raises(MiError, match="more than 16"):
+        ls.add_preconditioner(abi.MI_PRECOND_MAS, 0, 3 * V, part_ids=parts, build=abi.MI_MAS_BUILD_DEVICE)
+
+
+def test_device_build_full_size(ctx):
+    """64^3 vertex grid, 1.25M tets: tables bit-exact, both build times printed"""
+    g = synth.generate_topology(64, seed=5)
+    host, t_host = built(ctx, g.block_rows, g.part_ids, abi.MI_MAS_BUILD_HOST, tets=g.tets)
+    dev, t_dev = built(ctx, g.block_rows, g.part_ids, abi.MI_MAS_BUILD_DEVICE, tets=g.tets)
+    assert_same(host, dev)
+    print(f"\nMAS init at 64^3 ({g.block_rows} vertices, {len(g.tets)} tets): host {t_host:.1f} ms, "
+          f"device {t_dev:.1f} ms (levels {dev['L']}, clusters {dev['clusters']})", flush=True)
