@@ -148,8 +148,9 @@ def test_device_partition_surfaces_and_curves(ctx, kind):
     check_valid(p, n * n, 16)
     assert np_ == p.max() + 1
     if kind == "edges":
-        # a chain packs into consecutive runs
-        assert np_ <= (n * n) // 12
+        # a chain is cut into consecutive runs of at most 16 (about 11 on average)
+        assert np.all(np.diff(p) >= 0)
+        assert np_ <= (n * n) // 10
 
 
 @pytest.mark.gpu
