@@ -183,7 +183,7 @@ def main():
     # dominant kernel: the SpMV (+ fused x^T A x), timed with HIP events on its
     # stream.  mi_spmv_dot launches the accumulating SpMV kernel (y += A x over
     # the full BSR, per-workgroup x^T A x partials; FP64 planes: the LDS chunk
-    # kernel, FP32: the segmented-scan kernel) and a one-workgroup partials sum.
+    # kernel with FP32 planes) and a one-workgroup partials sum.
     y = ctx.alloc(s.n, np.float64)
     d = ctx.alloc(1, np.float64)
     for _ in range(3):
@@ -195,7 +195,7 @@ def main():
     sval = 4 if prec == abi.MI_FP32 else 8
     spmv_bytes = U * (9 * sval + 4) + (s.block_rows + 1) * 4 + 2 * s.n * 8
     gbs = spmv_bytes / (ms_spmv * 1e-3) / 1e9
-    kname = "spmv_chunk_kernel<double, true>" if prec == abi.MI_FP64 else "spmv_seg2_kernel<true>"
+    kname = "spmv_chunk_kernel<double, true>" if prec == abi.MI_FP64 else "spmv_chunk_kernel<float, true>"
 
     result = {
         "metric": "ms per Newton solve to 1e-4 rel. residual; PCG iters; SpMV HBM GB/s vs peak",

@@ -985,35 +985,45 @@ __global__ __launch_bounds__(64) void mas_apply_fine_kernel(const double* __rest
         }
         return;
     }
-    __shared__ T       sR[kBank][4];
-    __shared__ T       sC[kBank][3 * kBank];
-    __shared__ int32_t sV[kBank];
-    const int64_t      k = blockIdx.x;
-    ClusterBlocks<T>   cb;
-    cluster_load<T>(inv + k * kCm, lane, cb);
-    if(lane < kBank)
+    __shared__ T  sR[kBank][4];
+    __shared__ T  sC[kBank][3 * kBank];
+    const int64_t k = blockIdx.x;
+    // lane 3 i + c (< 48) owns component c of node i: its vertex and its
+    // ancestors are loaded first, then the inverse stream is issued, then the
+    // gathers of r and of the coarse Z, so the gathers overlap the stream
+    const bool    own  = lane < 3 * kBank;
+    const int     i    = lane / 3, c = lane % 3;
+    const int64_t slot = k * kBank + i;
+    int32_t       v    = -1;
+    int32_t       anc[kMaxLvl - 1];
+    if(own)
     {
-        const int32_t v = part_to_real[k * kBank + lane];
+        v = part_to_real[slot];
 #pragma unroll
-        for(int c = 0; c < 3; ++c)
-            sR[lane][c] = v >= 0 ? (T)r[3 * (int64_t)v + c] : T(0);
-        sV[lane] = v;
+        for(int l = 1; l < kMaxLvl; ++l)
+            anc[l - 1] = l < Lact ? anc_slot[(int64_t)(l - 1) * N + slot] : 0;
+    }
+    ClusterBlocks<T> cb;
+    cluster_load<T>(inv + k * kCm, lane, cb);
+    T zl[kMaxLvl - 1];
+    if(own)
+    {
+        sR[i][c] = v >= 0 ? (T)r[3 * (int64_t)v + c] : T(0);
+#pragma unroll
+        for(int l = 1; l < kMaxLvl; ++l)
+            zl[l - 1] = (v >= 0 && l < Lact) ? Zb[3 * (int64_t)anc[l - 1] + c] : T(0);
     }
     __syncthreads();
     cluster_contrib<T>(cb, lane, sR, sC);
     __syncthreads();
-    if(lane < 3 * kBank)
+    if(own && v >= 0)
     {
-        const int     i = lane / 3, c = lane % 3;
-        const int32_t v = sV[i];
-        if(v >= 0)
-        {
-            T             acc  = cluster_rowsum<T>(sC, lane);
-            const int64_t slot = k * kBank + i;
-            for(int l = 1; l < Lact; ++l)
-                acc += Zb[3 * (int64_t)anc_slot[(int64_t)(l - 1) * N + slot] + c];
-            z[3 * (int64_t)v + c] = (double)acc;
-        }
+        T acc = cluster_rowsum<T>(sC, lane);
+#pragma unroll
+        for(int l = 1; l < kMaxLvl; ++l)
+            if(l < Lact)
+                acc += zl[l - 1];
+        z[3 * (int64_t)v + c] = (double)acc;
     }
 }
 

@@ -15,13 +15,12 @@
 //     consecutive blocks, so every value plane streams as 64 contiguous
 //     entries per wave instruction (16 lanes per row of ~40 blocks reached
 //     ~60% of that rate);
-//   * FP64 planes (spmv_chunk_kernel): a workgroup stages a chunk's products
-//     w_k = A_k x_col(k) in LDS and sums each row from there, 16 lanes per row;
-//   * FP32 planes (spmv_seg2_kernel): a wave walks a chunk in windows of 128
-//     blocks (two per lane), finds each lane's rows by a shuffle binary search
-//     over the row starts, sums rows with a segmented shuffle scan (carrying a
-//     row across windows) and loads the next window while this one's x
-//     gathers are outstanding;
+//   * a workgroup stages a chunk's products w_k = A_k x_col(k) in LDS (one
+//     block per lane and round: load, gather, product) and sums each row from
+//     there, 16 lanes per row in a fixed order; 18 KB of LDS and at most 64
+//     VGPRs keep 8 waves per SIMD resident to hide the load latency;
+//   * FP64 planes gather x as doubles; FP32 planes gather the float4 copy of
+//     x (the PCG keeps one next to p) and form the products in double;
 //   * x^T A x = sum_i x_i . (A x)_i per workgroup into partials[blockIdx.x];
 //     the consumer kernel (PCG update) sums the partials in a fixed order;
 //   * a persistent grid; each XCD (workgroups are dispatched round-robin over
@@ -40,9 +39,12 @@ constexpr int kWaves    = kThreads / kWave;
 constexpr int kXcds     = 8;
 constexpr int kLanes    = 16;  // lanes per block row in the chunk kernel's row sums
 constexpr int kGroups   = kThreads / kLanes;
-constexpr int kChunkCap = 2 * kChunkStep;  // blocks a chunk may hold to be staged in LDS
+// blocks a chunk may hold to be staged in LDS: the chunk step plus the longest
+// row that may run past it; 18 KB of LDS per workgroup, so that LDS does not
+// limit the occupancy below 8 waves per SIMD
+constexpr int kChunkCap = kChunkStep + kChunkStep / 2;
 constexpr int kPer      = kChunkCap / kThreads;
-static_assert(kChunkCap % (2 * kThreads) == 0, "chunk capacity must be a multiple of two workgroups");
+static_assert(kChunkCap % kThreads == 0, "chunk capacity must be a multiple of the workgroup");
 
 template <bool ACCUM>
 __device__ __forceinline__ void put_row(double* y, int64_t row, double a, double v0, double v1, double v2)
@@ -62,277 +64,44 @@ __device__ __forceinline__ void put_row(double* y, int64_t row, double a, double
     }
 }
 
-// FP32 planes, two blocks per lane: a wave walks a chunk in windows of 128
-// blocks, lane l on blocks k + 2l and k + 2l + 1 (8-byte loads: 512 bytes
-// per plane per wave instruction).  Windows start at an even block; a chunk
-// that starts at an odd block masks the block before it.  A lane's blocks are
-// first combined in the lane (the sum of its last segment), then the lanes
-// run the segmented scan; the segment that ends at a lane's first block (the
-// second one starts a row) is the previous lane's inclusive value plus that
-// block.
-typedef int   v2i __attribute__((ext_vector_type(2)));
-typedef float v2f __attribute__((ext_vector_type(2)));
-
-struct Window2
-{
-    v2i j;
-    v2f A[9];
-};
-
-__device__ __forceinline__ void load_window2(Window2& w,
-                                             int32_t  kw,
-                                             int32_t  kEnd,
-                                             int      lane,
-                                             const int32_t* __restrict__ fcols,
-                                             const float* __restrict__ fvals,
-                                             int64_t fstride)
-{
-    const int32_t kb = kw + 2 * lane;
-    const bool    ok = kb < kEnd;
-    w.j              = ok ? *reinterpret_cast<const v2i*>(fcols + kb) : v2i{0, 0};
-#pragma unroll
-    for(int q = 0; q < 9; ++q)
-        w.A[q] = ok ? *reinterpret_cast<const v2f*>(fvals + q * fstride + kb) : v2f{0.f, 0.f};
-}
-
-// row start i of the 128 that can meet a window: i < 64 in fa, else in fb
-__device__ __forceinline__ int32_t row_start(int32_t fa, int32_t fb, int i)
-{
-    const int32_t va = __shfl(fa, i & (kWave - 1), kWave);
-    const int32_t vb = __shfl(fb, i & (kWave - 1), kWave);
-    return i < kWave ? va : vb;
-}
-
-template <bool ACCUM>
-__global__ __launch_bounds__(kThreads) void spmv_seg2_kernel(int32_t        nchunks,
-                                                            const int32_t* __restrict__ chunk_row,
-                                                            const int32_t* __restrict__ fptr,
-                                                            const int32_t* __restrict__ fcols,
-                                                            const float* __restrict__ fvals,
-                                                            int64_t       fstride,
-                                                            const double* __restrict__ x,
-                                                            const float4* __restrict__ x32,
-                                                            double*       y,
-                                                            double        a,
-                                                            double*       partials,
-                                                            const int32_t* d_conv)
-{
-    if(d_conv && *d_conv != 0)
-        return;
-    __shared__ double lds[kThreads / kWave];
-    constexpr int     kWin = 2 * kWave;
-
-    const int G    = gridDim.x;
-    const int b    = blockIdx.x;
-    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
-    const int nx   = (G % kXcds == 0) ? kXcds : 1;
-    const int xcd  = b % nx;
-    const int Wx   = (G / nx) * kWaves;
-    const int t    = (b / nx) * kWaves + wave;
-    const int cx1  = (int)((int64_t)nchunks * (xcd + 1) / nx);
-    int       c    = (int)((int64_t)nchunks * xcd / nx) + t;
-
-    const int lane = (int)(threadIdx.x % kWave);
-    double    dot  = 0.0;
-    if(c < cx1)
-    {
-        int32_t row    = chunk_row[c];
-        int32_t rowEnd = chunk_row[c + 1];
-        int32_t kStart = fptr[row];
-        int32_t kEnd   = fptr[rowEnd];
-        int32_t k      = kStart & ~1;
-        double  carry0 = 0.0, carry1 = 0.0, carry2 = 0.0;
-        Window2 nw;
-        load_window2(nw, k, kEnd, lane, fcols, fvals, fstride);
-        for(;;)
-        {
-            const Window2 cw  = nw;
-            const int32_t kb0 = k + 2 * lane, kb1 = kb0 + 1;
-            const bool    lead = kb0 < kStart;  // the block before an odd chunk start
-            const bool    ok0 = !lead && kb0 < kEnd, ok1 = kb1 < kEnd;
-            // columns of blocks outside [kStart, kEnd) are never used (the pair's
-            // second half past a chunk's end may be unwritten slack)
-            // x is gathered from its FP32 copy, one 16-byte load per block
-            const uint32_t j0 = ok0 ? (uint32_t)cw.j.x : 0u, j1 = ok1 ? (uint32_t)cw.j.y : 0u;
-            const float4   g0 = x32[j0], g1 = x32[j1];
-            const double   x00 = g0.x, x01 = g0.y, x02 = g0.z;
-            const double   x10 = g1.x, x11 = g1.y, x12 = g1.z;
-            const int32_t fa = fptr[min(row + lane, rowEnd)];
-            const int32_t fb = fptr[min(row + kWave + lane, rowEnd)];
-            const bool    same = k + kWin < kEnd;
-            int32_t       nrow = 0, nrowEnd = 0, nk = 0, nkStart = 0, nkEnd = 0;
-            const int     cn   = c + Wx;
-            if(same)
-            {
-                nk      = k + kWin;
-                nkStart = kStart;
-                nkEnd   = kEnd;
-            }
-            else if(cn < cx1)
-            {
-                nrow    = chunk_row[cn];
-                nrowEnd = chunk_row[cn + 1];
-                nkStart = fptr[nrow];
-                nkEnd   = fptr[nrowEnd];
-                nk      = nkStart & ~1;
-            }
-            if(same || cn < cx1)
-                load_window2(nw, nk, nkEnd, lane, fcols, fvals, fstride);
-            // rows of the lane's blocks: search the first, the second is the
-            // same row or the next one (every row holds a block)
-            const int32_t kq  = lead ? kStart : kb0;
-            int           idx = 0;
-#pragma unroll
-            for(int step = kWave; step > 0; step >>= 1)
-                if(row_start(fa, fb, idx + step < kWin ? idx + step : kWin - 1) <= kq && idx + step < kWin)
-                    idx += step;
-            int32_t       f0cur  = row_start(fa, fb, idx);
-            int32_t       f0next = row_start(fa, fb, min(idx + 1, kWin - 1));
-            const int32_t row0   = row + idx;
-            if(idx + 1 >= kWin)
-                f0next = fptr[min(row0 + 1, rowEnd)];
-            const bool    h1   = ok1 && (lead || kb1 == f0next);
-            const int32_t row1 = (h1 && !lead) ? row0 + 1 : row0;
-            const int     idx1 = row1 - row;
-            int32_t       f1next = row_start(fa, fb, min(idx1 + 1, kWin - 1));
-            if(idx1 + 1 >= kWin)
-                f1next = fptr[min(row1 + 1, rowEnd)];
-            const bool h0 = ok0 && f0cur == kb0;
-            // block products
-            double w00 = 0.0, w01 = 0.0, w02 = 0.0, w10 = 0.0, w11 = 0.0, w12 = 0.0;
-            if(ok0)
-            {
-                w00 = (double)cw.A[0].x * x00 + (double)cw.A[3].x * x01 + (double)cw.A[6].x * x02;
-                w01 = (double)cw.A[1].x * x00 + (double)cw.A[4].x * x01 + (double)cw.A[7].x * x02;
-                w02 = (double)cw.A[2].x * x00 + (double)cw.A[5].x * x01 + (double)cw.A[8].x * x02;
-            }
-            if(ok1)
-            {
-                w10 = (double)cw.A[0].y * x10 + (double)cw.A[3].y * x11 + (double)cw.A[6].y * x12;
-                w11 = (double)cw.A[1].y * x10 + (double)cw.A[4].y * x11 + (double)cw.A[7].y * x12;
-                w12 = (double)cw.A[2].y * x10 + (double)cw.A[5].y * x11 + (double)cw.A[8].y * x12;
-            }
-            // the lane's last segment inside the lane, and whether it starts here
-            double v0, v1, v2;
-            bool   brk;
-            if(ok1 && !h1)
-            {
-                v0  = w00 + w10;
-                v1  = w01 + w11;
-                v2  = w02 + w12;
-                brk = h0;
-            }
-            else if(ok1)
-            {
-                v0  = w10;
-                v1  = w11;
-                v2  = w12;
-                brk = true;
-            }
-            else
-            {
-                v0  = w00;
-                v1  = w01;
-                v2  = w02;
-                brk = h0 || !ok0;
-            }
-            const uint64_t bm = __ballot(brk);
-            int            fl = brk ? 1 : 0;
-#pragma unroll
-            for(int d = 1; d < kWave; d <<= 1)
-            {
-                const double t0 = __shfl_up(v0, d, kWave), t1 = __shfl_up(v1, d, kWave), t2 = __shfl_up(v2, d, kWave);
-                const int    tf = __shfl_up(fl, d, kWave);
-                if(lane >= d)
-                {
-                    if(!fl)
-                    {
-                        v0 += t0;
-                        v1 += t1;
-                        v2 += t2;
-                    }
-                    fl |= tf;
-                }
-            }
-            // the previous lane's inclusive value (the segment the first block continues)
-            double p0 = __shfl_up(v0, 1, kWave), p1 = __shfl_up(v1, 1, kWave), p2 = __shfl_up(v2, 1, kWave);
-            if(lane == 0)
-                p0 = p1 = p2 = 0.0;
-            const bool none_below = (bm & ((1ull << lane) - 1ull)) == 0ull;   // no break in lanes [0, l)
-            const bool none_upto  = (bm & ((2ull << lane) - 1ull)) == 0ull;   // ... in lanes [0, l]
-            if(none_below && !h0)
-            {
-                p0 += carry0;
-                p1 += carry1;
-                p2 += carry2;
-            }
-            if(none_upto)
-            {
-                v0 += carry0;
-                v1 += carry1;
-                v2 += carry2;
-            }
-            const bool end0 = ok0 && (h1 || !ok1) && kb0 + 1 == f0next;
-            const bool end1 = ok1 && kb1 + 1 == f1next;
-            if(end0)
-            {
-                const double e0 = (h0 ? 0.0 : p0) + w00, e1 = (h0 ? 0.0 : p1) + w01, e2 = (h0 ? 0.0 : p2) + w02;
-                put_row<ACCUM>(y, row0, a, e0, e1, e2);
-                dot += x[3 * (int64_t)row0] * e0 + x[3 * (int64_t)row0 + 1] * e1 + x[3 * (int64_t)row0 + 2] * e2;
-            }
-            if(end1)
-            {
-                put_row<ACCUM>(y, row1, a, v0, v1, v2);
-                dot += x[3 * (int64_t)row1] * v0 + x[3 * (int64_t)row1 + 1] * v1 + x[3 * (int64_t)row1 + 2] * v2;
-            }
-            if(same)
-            {
-                // lane 63 hands its row on: its second block is valid (the
-                // window is not the chunk's last); the row continues or ends
-                const int  e63   = __shfl(end1 ? 1 : 0, kWave - 1, kWave);
-                const int  r63   = __shfl(row1, kWave - 1, kWave);
-                const bool cont  = e63 == 0;
-                carry0           = cont ? __shfl(v0, kWave - 1, kWave) : 0.0;
-                carry1           = cont ? __shfl(v1, kWave - 1, kWave) : 0.0;
-                carry2           = cont ? __shfl(v2, kWave - 1, kWave) : 0.0;
-                row              = cont ? r63 : r63 + 1;
-                k                = nk;
-            }
-            else
-            {
-                if(cn >= cx1)
-                    break;
-                c      = cn;
-                row    = nrow;
-                rowEnd = nrowEnd;
-                kStart = nkStart;
-                k      = nk;
-                kEnd   = nkEnd;
-                carry0 = carry1 = carry2 = 0.0;
-            }
-        }
-    }
-    if(partials)
-    {
-        double bs = block_sum<kThreads>(dot, lds);
-        if(threadIdx.x == 0)
-            partials[blockIdx.x] = bs;
-    }
-}
-
 // FP64 planes: a workgroup takes whole chunks (rows whose first block lies in
 // [c S, (c+1) S)), loads the chunk's blocks with consecutive lanes on
 // consecutive blocks, stages w_k = A_k x_col(k) in LDS and sums each row from
 // LDS with 16 lanes in a fixed order.  A chunk longer than kChunkCap blocks
 // (a row of more than S blocks) is done row by row straight from memory.
+// x_j as three doubles: FP64 planes read x, FP32 planes the float4 copy
+template <typename VT>
+__device__ __forceinline__ void gather_x(const double* __restrict__ x,
+                                         const float4* __restrict__ x32,
+                                         uint32_t j,
+                                         double&  x0,
+                                         double&  x1,
+                                         double&  x2)
+{
+    if constexpr(sizeof(VT) == 4)
+    {
+        const float4 t = x32[j];
+        x0             = t.x;
+        x1             = t.y;
+        x2             = t.z;
+    }
+    else
+    {
+        x0 = x[3u * j];
+        x1 = x[3u * j + 1];
+        x2 = x[3u * j + 2];
+    }
+}
+
 template <typename VT, bool ACCUM>
-__global__ __launch_bounds__(kThreads) void spmv_chunk_kernel(int32_t        nchunks,
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) void spmv_chunk_kernel(int32_t        nchunks,
                                                              const int32_t* __restrict__ chunk_row,
                                                              const int32_t* __restrict__ fptr,
                                                              const int32_t* __restrict__ fcols,
                                                              const VT* __restrict__ fvals,
                                                              int64_t       fstride,
                                                              const double* __restrict__ x,
+                                                             const float4* __restrict__ x32,
                                                              double*       y,
                                                              double        a,
                                                              double*       partials,
@@ -360,34 +129,26 @@ __global__ __launch_bounds__(kThreads) void spmv_chunk_kernel(int32_t        nch
         const int32_t k0 = fptr[row0], k1 = fptr[row1];
         if(k1 - k0 <= kChunkCap)
         {
-            // products of the chunk's blocks, consecutive lanes on consecutive blocks
+            // products of the chunk's blocks, consecutive lanes on consecutive
+            // blocks, one block per lane and round (load, gather, product)
 #pragma unroll
-            for(int h = 0; h < kPer; h += 2)
+            for(int h = 0; h < kPer; ++h)
             {
-                uint32_t jj[2];
-                bool     ok[2];
-                VT       A[2][9];
+                const int32_t  k  = k0 + h * kThreads + tid;
+                const bool     ok = k < k1;
+                const uint32_t j  = ok ? (uint32_t)fcols[k] : 0u;
+                VT             A[9];
 #pragma unroll
-                for(int u = 0; u < 2; ++u)
+                for(int q = 0; q < 9; ++q)
+                    A[q] = ok ? fvals[q * fstride + k] : VT(0);
+                double x0, x1, x2;
+                gather_x<VT>(x, x32, j, x0, x1, x2);
+                if(ok)
                 {
-                    const int32_t k = k0 + (h + u) * kThreads + tid;
-                    ok[u]           = k < k1;
-                    jj[u]           = ok[u] ? (uint32_t)fcols[k] : 0u;
-#pragma unroll
-                    for(int q = 0; q < 9; ++q)
-                        A[u][q] = ok[u] ? fvals[q * fstride + k] : VT(0);
-                }
-#pragma unroll
-                for(int u = 0; u < 2; ++u)
-                {
-                    const double x0 = x[3u * jj[u]], x1 = x[3u * jj[u] + 1], x2 = x[3u * jj[u] + 2];
-                    if(ok[u])
-                    {
-                        const int i = (h + u) * kThreads + tid;
-                        w[0][i]     = (double)A[u][0] * x0 + (double)A[u][3] * x1 + (double)A[u][6] * x2;
-                        w[1][i]     = (double)A[u][1] * x0 + (double)A[u][4] * x1 + (double)A[u][7] * x2;
-                        w[2][i]     = (double)A[u][2] * x0 + (double)A[u][5] * x1 + (double)A[u][8] * x2;
-                    }
+                    const int i = h * kThreads + tid;
+                    w[0][i]     = (double)A[0] * x0 + (double)A[3] * x1 + (double)A[6] * x2;
+                    w[1][i]     = (double)A[1] * x0 + (double)A[4] * x1 + (double)A[7] * x2;
+                    w[2][i]     = (double)A[2] * x0 + (double)A[5] * x1 + (double)A[8] * x2;
                 }
             }
             __syncthreads();
@@ -431,7 +192,8 @@ __global__ __launch_bounds__(kThreads) void spmv_chunk_kernel(int32_t        nch
 #pragma unroll
                     for(int q = 0; q < 9; ++q)
                         A[q] = (double)fvals[q * fstride + k];
-                    const double x0 = x[3u * j], x1 = x[3u * j + 1], x2 = x[3u * j + 2];
+                    double x0, x1, x2;
+                    gather_x<VT>(x, x32, j, x0, x1, x2);
                     acc0 += A[0] * x0 + A[3] * x1 + A[6] * x2;
                     acc1 += A[1] * x0 + A[4] * x1 + A[7] * x2;
                     acc2 += A[2] * x0 + A[5] * x1 + A[8] * x2;
@@ -479,20 +241,13 @@ __global__ __launch_bounds__(kThreads) void sum_partials_kernel(const double* pa
 }
 
 // launches the SpMV; returns how many partials it wrote from partials[0].
-// FP64 planes go through the LDS chunk kernel, FP32 planes through the
-// two-blocks-per-lane segmented-scan kernel.
 template <typename VT, bool ACCUM>
 int launch_spmv(const SpmvScratch&, int g, hipStream_t s, const BsrMatrix& A, const VT* vals, const double* x,
                 const float4* x32, double* y, double a, double* partials, const int32_t* d_conv)
 {
-    if constexpr(sizeof(VT) == 8)
-        spmv_chunk_kernel<VT, ACCUM><<<g, kThreads, 0, s>>>(A.nchunks, A.chunk_row.data(), A.full_ptr.data(),
-                                                            A.fcols.data(), vals, A.fstride, x, y, a, partials,
-                                                            d_conv);
-    else
-        spmv_seg2_kernel<ACCUM><<<g, kThreads, 0, s>>>(A.nchunks, A.chunk_row.data(), A.full_ptr.data(),
-                                                       A.fcols.data(), vals, A.fstride, x, x32, y, a, partials,
-                                                       d_conv);
+    spmv_chunk_kernel<VT, ACCUM><<<g, kThreads, 0, s>>>(A.nchunks, A.chunk_row.data(), A.full_ptr.data(),
+                                                        A.fcols.data(), vals, A.fstride, x, x32, y, a, partials,
+                                                        d_conv);
     return g;
 }
 
@@ -517,7 +272,7 @@ void spmv_prepare(SpmvScratch& scr, bool use32)
             per_cu = std::max(1, std::min(32, std::atoi(e)));
         if(per_cu == 0)
         {
-            const void* fn = use32 ? (const void*)spmv_seg2_kernel<false>
+            const void* fn = use32 ? (const void*)spmv_chunk_kernel<float, false>
                                    : (const void*)spmv_chunk_kernel<double, false>;
             MI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kThreads, 0));
             per_cu = std::max(1, per_cu);
@@ -531,8 +286,7 @@ void spmv_prepare(SpmvScratch& scr, bool use32)
 
 int spmv_grid(const SpmvScratch& scr, const BsrMatrix& A)
 {
-    const int64_t per_wg = scr.use32 ? kWaves : 1;  // chunks a workgroup takes at once
-    return (int)std::max<int64_t>(1, std::min<int64_t>(scr.grid, (A.nchunks + per_wg - 1) / per_wg));
+    return (int)std::max<int64_t>(1, std::min<int64_t>(scr.grid, A.nchunks));
 }
 
 // the FP32 kernel gathers x from a float4 copy: the caller's (the PCG keeps
