@@ -5,8 +5,8 @@
  * Declared with the rest of the C ABI in libmi_linsys.so; mi_ctx is the
  * context of mi_linsys.h.
  *
- * Every 4-node stencil writes 10 triplets, its upper blocks (a, b), a <= b,
- * in the order a = 0..3, b = a..3, at (min, max) of the two global vertex ids
+ * Every N-node stencil writes N (N + 1) / 2 triplets, its upper blocks (a, b),
+ * a <= b, in the order a = 0..N-1, b = a..N-1, at (min, max) of the two global vertex ids
  * and transposed when the stencil order is reversed: the half-block writer
  * of utils/matrix_assembler.h:312-437 (TripletMatrixAssembler::ProxyRangeHalf).
  * Vertex ids in the inputs are local to the subsystem; vert_offset (its first
@@ -71,6 +71,57 @@ int mi_produce_contact(mi_ctx*        ctx,
                        int32_t*       cols,
                        double*        vals,
                        void*          stream);
+
+/* Affine bodies (12 DoFs q = (p, a_1, a_2, a_3), a_i the rows of A, x = p + A xbar):
+ * per body I, the 12x12 block H = M + volume dt^2 PSD(d2E/dA2) written as 16
+ * triplets 16 I .. 16 I + 15, a row-major 4x4 grid of 3x3 blocks at block rows
+ * body_offset + 4 I + (0..3), the blocks below the diagonal set to zero
+ * (abd_linear_subsystem.cu:21-31, 80-113).  M is the affine mass matrix of
+ * AffineBodyBDF1Kinetic (affine_body/bdf/affine_body_bdf1_kinetic.cu:38-71,
+ * ABDJacobiDyadicMass::add_to, abd_jacobi_matrix.cu:115-135) from
+ * dyadic_mass[10 I ..] = (m, m xbar (3), m <xbar xbar^T> as xx, xy, xz, yy,
+ * yz, zz); E = kappa sum_ij (a_i . a_j - delta_ij)^2 is the orthogonality
+ * potential (affine_body/constitutions/ortho_potential.cu:36-70), its 9x9
+ * Hessian projected to PSD by eigenvalue clamping (utils/make_spd.h).
+ * is_fixed (may be NULL): a nonzero entry writes the identity instead. */
+int mi_produce_abd(mi_ctx*        ctx,
+                   int64_t        n_bodies,
+                   const double*  q,
+                   const double*  dyadic_mass,
+                   const double*  volume,
+                   const double*  kappa,
+                   const int32_t* is_fixed,
+                   double         dt,
+                   int32_t        body_offset,
+                   int32_t*       rows,
+                   int32_t*       cols,
+                   double*        vals,
+                   void*          stream);
+
+/* Cloth triangles, the strain-limiting Baraff-Witkin shell
+ * (finite_element/constitutions/strain_limiting_baraff_witkin_shell_2d.cu:78-148):
+ * per triangle t (tris[3 t .. 3 t + 2]), F = [x1 - x0, x2 - x0] Dm^-1 with Dm the
+ * rest triangle's edges in its own plane, and the 9x9 block
+ * A dt^2 dF/dx^T (stretch_stiffness H_stretch + shear_stiffness H_shear) dF/dx
+ * (A the rest area; H_stretch, H_shear the closed-form PSD Hessians of
+ * (s - 1)^2 + strain_rate (s - 1)^3 [s > 1] per direction and of I6^2,
+ * strain_limiting_baraff_witkin_shell_2d.h:165-248).  Writes the 6 upper
+ * blocks of the 3-node stencil, triplets 6 t .. 6 t + 5, in the half-block
+ * order described above. */
+int mi_produce_bw_shell(mi_ctx*        ctx,
+                        int64_t        n_tris,
+                        const int32_t* tris,
+                        const double*  rest_positions,
+                        const double*  positions,
+                        double         stretch_stiffness,
+                        double         shear_stiffness,
+                        double         strain_rate,
+                        double         dt,
+                        int32_t        vert_offset,
+                        int32_t*       rows,
+                        int32_t*       cols,
+                        double*        vals,
+                        void*          stream);
 
 /* ------------------------------------------------ line search, after the solve */
 /* LineSearchReporter::step_forward of a vertex set: x = x0 + alpha dx over n

@@ -157,6 +157,11 @@ LINSYS_PROTOS = {
     "mi_mtx_read_block_triplets": (C.c_int, [C.c_char_p, C.c_int, c_i32p, C.POINTER(C.c_int64), c_i32p, c_i32p,
                                              c_f64p]),
     "mi_vec_read_mtx": (C.c_int, [C.c_char_p, C.POINTER(C.c_int64), c_f64p]),
+    "mi_produce_abd": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                 C.c_double, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "mi_produce_bw_shell": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
+                                      C.c_double, C.c_double, C.c_double, C.c_int32, C.c_void_p, C.c_void_p,
+                                      C.c_void_p, C.c_void_p]),
     "mi_step_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p, C.c_int64,
                                   C.c_void_p]),
     "mi_energy_kinetic": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,

@@ -634,6 +634,55 @@ int mi_produce_snh(mi_ctx*        ctx,
     MI_API_END
 }
 
+int mi_produce_abd(mi_ctx*        ctx,
+                   int64_t        n_bodies,
+                   const double*  q,
+                   const double*  dyadic_mass,
+                   const double*  volume,
+                   const double*  kappa,
+                   const int32_t* is_fixed,
+                   double         dt,
+                   int32_t        body_offset,
+                   int32_t*       rows,
+                   int32_t*       cols,
+                   double*        vals,
+                   void*          stream)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(ctx && n_bodies >= 0 && body_offset >= 0, MI_ERR_ARG, "mi_produce_abd: bad argument");
+    MI_REQUIRE(dt > 0.0, MI_ERR_ARG, "mi_produce_abd: dt must be > 0");
+    MI_REQUIRE(n_bodies == 0 || (q && dyadic_mass && volume && kappa && rows && cols && vals), MI_ERR_ARG,
+               "mi_produce_abd: null buffer");
+    produce_abd(n_bodies, q, dyadic_mass, volume, kappa, is_fixed, dt, body_offset, rows, cols, vals,
+                pick(ctx, stream));
+    MI_API_END
+}
+
+int mi_produce_bw_shell(mi_ctx*        ctx,
+                        int64_t        n_tris,
+                        const int32_t* tris,
+                        const double*  rest_positions,
+                        const double*  positions,
+                        double         stretch_stiffness,
+                        double         shear_stiffness,
+                        double         strain_rate,
+                        double         dt,
+                        int32_t        vert_offset,
+                        int32_t*       rows,
+                        int32_t*       cols,
+                        double*        vals,
+                        void*          stream)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(ctx && n_tris >= 0 && vert_offset >= 0, MI_ERR_ARG, "mi_produce_bw_shell: bad argument");
+    MI_REQUIRE(dt > 0.0, MI_ERR_ARG, "mi_produce_bw_shell: dt must be > 0");
+    MI_REQUIRE(n_tris == 0 || (tris && rest_positions && positions && rows && cols && vals), MI_ERR_ARG,
+               "mi_produce_bw_shell: null buffer");
+    produce_bw_shell(n_tris, tris, rest_positions, positions, stretch_stiffness, shear_stiffness, strain_rate, dt,
+                     vert_offset, rows, cols, vals, pick(ctx, stream));
+    MI_API_END
+}
+
 int mi_energy_kinetic(mi_ctx*       ctx,
                       int32_t       vert_num,
                       const double* x,

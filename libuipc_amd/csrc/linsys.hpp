@@ -146,6 +146,31 @@ void produce_snh(int64_t        n_tets,
                  int32_t*       cols,
                  double*        vals,
                  hipStream_t    s);
+void produce_abd(int64_t        n,
+                 const double*  q,
+                 const double*  dyadic,
+                 const double*  volume,
+                 const double*  kappa,
+                 const int32_t* is_fixed,
+                 double         dt,
+                 int32_t        body_offset,
+                 int32_t*       rows,
+                 int32_t*       cols,
+                 double*        vals,
+                 hipStream_t    s);
+void produce_bw_shell(int64_t        n,
+                      const int32_t* tris,
+                      const double*  X,
+                      const double*  x,
+                      double         stretch,
+                      double         shear,
+                      double         rate,
+                      double         dt,
+                      int32_t        vert_offset,
+                      int32_t*       rows,
+                      int32_t*       cols,
+                      double*        vals,
+                      hipStream_t    s);
 void energy_kinetic(int32_t V, const double* x, const double* xt, const double* m, double* energy,
                     DBuf<double>& part, hipStream_t s);
 void energy_snh(int64_t        n_tets,

@@ -136,3 +136,64 @@ def producer_inputs(grid: int, n_contacts: int = 0, seed: int = 1, **kw) -> Prod
     mu = E / (2.0 * (1.0 + nu))
     lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu))
     return ProducerInputs(X, x, m, tets, ci[:n_contacts], cg[:n_contacts], ck[:n_contacts], mu, lam, d.dt)
+
+
+@dataclass
+class AbdInputs:
+    """Affine bodies for mi_produce_abd: cubes of random size and position,
+    q near a rotation (A = R + noise), kappa = 1e7, density 1e3."""
+    q: np.ndarray  # float64[n, 12]
+    dyadic: np.ndarray  # float64[n, 10]: m, m xbar, m <xbar xbar^T> (xx xy xz yy yz zz)
+    volume: np.ndarray  # float64[n]
+    kappa: np.ndarray  # float64[n]
+    dt: float
+
+
+def abd_inputs(n: int, seed: int = 1, *, kappa: float = 1e7, density: float = 1e3, dt: float = 0.01,
+               noise: float = 1e-2) -> AbdInputs:
+    rng = np.random.default_rng(seed)
+    side = rng.uniform(0.05, 0.2, n)
+    c = rng.uniform(-1.0, 1.0, (n, 3))  # centre of each cube in its rest frame
+    m = density * side ** 3
+    S = m[:, None, None] * (c[:, :, None] * c[:, None, :] + (side ** 2 / 12.0)[:, None, None] * np.eye(3))
+    dyadic = np.column_stack([m, m[:, None] * c, S[:, 0, 0], S[:, 0, 1], S[:, 0, 2], S[:, 1, 1], S[:, 1, 2],
+                              S[:, 2, 2]])
+    R, _ = np.linalg.qr(rng.standard_normal((n, 3, 3)))
+    A = R + noise * rng.standard_normal((n, 3, 3))
+    q = np.concatenate([rng.uniform(-5.0, 5.0, (n, 3)), A.reshape(n, 9)], axis=1)
+    return AbdInputs(q, dyadic, side ** 3, np.full(n, kappa), dt)
+
+
+@dataclass
+class ClothInputs:
+    """A square cloth grid for mi_produce_bw_shell: the rest grid in the z = 0
+    plane, deformed positions (a stretch or compression per direction, a shear,
+    a rigid rotation and per-vertex noise, so that every branch of the
+    constitution is taken)."""
+    tris: np.ndarray  # int32[T, 3]
+    rest_positions: np.ndarray  # float64[V, 3]
+    positions: np.ndarray  # float64[V, 3]
+    stretch: float
+    shear: float
+    strain_rate: float
+    dt: float
+
+
+def cloth_inputs(n: int, seed: int = 1, *, youngs: float = 1e5, poisson: float = 0.3, thickness: float = 1e-3,
+                 strain_rate: float = 100.0, dt: float = 0.01, noise: float = 0.05) -> ClothInputs:
+    """n x n vertices, 2 (n - 1)^2 triangles; stiffness as
+    StrainLimitingBaraffWitkin::apply_to: stretch = (lambda + 2 mu) 2 thickness,
+    shear = mu (src/constitution/strain_limiting_baraff_witkin.cpp:35-44)"""
+    rng = np.random.default_rng(seed)
+    h = 1.0 / (n - 1)
+    gx, gy = np.meshgrid(np.arange(n) * h, np.arange(n) * h, indexing="xy")
+    X = np.column_stack([gx.ravel(), gy.ravel(), np.zeros(n * n)])
+    idx = np.arange(n * n, dtype=np.int32).reshape(n, n)
+    a, b, c, d = idx[:-1, :-1].ravel(), idx[:-1, 1:].ravel(), idx[1:, :-1].ravel(), idx[1:, 1:].ravel()
+    tris = np.concatenate([np.stack([a, b, d], 1), np.stack([a, d, c], 1)]).astype(np.int32)
+    Fd = np.array([[1.05, 0.08, 0.0], [-0.03, 0.97, 0.0], [0.0, 0.0, 1.0]])
+    R, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+    x = X @ Fd.T @ R.T + noise * h * rng.standard_normal(X.shape)
+    mu = youngs / (2 * (1 + poisson))
+    lam = youngs * poisson / (1 - poisson * poisson)  # plane-stress lambda
+    return ClothInputs(tris, X, x, (lam + 2 * mu) * 2 * thickness, mu, strain_rate, dt)

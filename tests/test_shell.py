@@ -1,0 +1,74 @@
+"""Pins the oracle's cloth restatement (oracle/shell.py): where the reference's
+closed-form projection is inactive (orthogonal, stretched F columns) the 6x6
+Hessian equals central differences of the energy, and the 9x9 triangle
+Hessian equals central differences of the energy in the positions; the
+projection is PSD everywhere; the rest frame preserves the edge lengths and
+area.  CPU only."""
+import numpy as np
+
+from libuipc_amd import synth
+from oracle import shell as OS
+
+
+def fd_hessian(f, x0, h):
+    n = x0.size
+    H = np.zeros((n, n))
+    for i in range(n):
+        for j in range(n):
+            e = np.zeros(n)
+            g = np.zeros(n)
+            e[i] = h
+            g[j] = h
+            H[i, j] = (f(x0 + e + g) - f(x0 + e - g) - f(x0 - e + g) + f(x0 - e - g)) / (4 * h * h)
+    return H
+
+
+def test_hessian_F_is_exact_where_nothing_is_clamped():
+    rng = np.random.default_rng(1)
+    for trial in range(3):
+        Q, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+        F = Q[:, :2] * np.array([1.2 + 0.1 * trial, 1.07])  # orthogonal columns, both stretched
+        f = lambda v: OS.energy_F(v.reshape(2, 3).T, 3.0, 2.0, 100.0)  # noqa: E731
+        H = OS.hessian_F(F, 3.0, 2.0, 100.0)
+        fd = fd_hessian(f, F.T.reshape(-1), 1e-5)
+        assert np.abs(fd - H).max() <= 2e-5 * np.abs(H).max()
+
+
+def test_triangle_hessian_is_the_second_derivative_in_x():
+    rng = np.random.default_rng(2)
+    Xr = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.2, 0.9, 0.0]]) + 0.01 * rng.standard_normal((3, 3))
+    B = np.linalg.inv(OS.rest_dm(*Xr))
+    Q, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+    # x such that F = Q[:, :2] diag(1.15, 1.1): orthogonal, stretched columns
+    F = Q[:, :2] * np.array([1.15, 1.1])
+    Ds = F @ np.linalg.inv(B)
+    xc = np.stack([np.zeros(3), Ds[:, 0], Ds[:, 1]]) + 0.3
+    H, _ = OS.triangle(Xr, xc, 3.0, 2.0, 100.0, 0.1)
+    f = lambda v: OS.triangle(Xr, v.reshape(3, 3), 3.0, 2.0, 100.0, 0.1)[1]  # noqa: E731
+    fd = fd_hessian(f, xc.reshape(-1), 1e-5)
+    assert np.abs(fd - H).max() <= 2e-5 * np.abs(H).max()
+
+
+def test_projection_is_psd():
+    rng = np.random.default_rng(3)
+    for _ in range(50):
+        F = rng.standard_normal((3, 2)) * rng.uniform(0.2, 2.0)
+        w = np.linalg.eigvalsh(OS.hessian_F(F, 3.0, 2.0, 100.0))
+        assert w.min() >= -1e-9 * max(1.0, w.max())
+
+
+def test_rest_frame():
+    rng = np.random.default_rng(4)
+    for _ in range(10):
+        Xr = rng.standard_normal((3, 3))
+        Dm = OS.rest_dm(*Xr)
+        E = np.column_stack([Xr[1] - Xr[0], Xr[2] - Xr[0]])
+        assert np.allclose(Dm.T @ Dm, E.T @ E)
+        assert np.isclose(abs(np.linalg.det(Dm)), np.linalg.norm(np.cross(E[:, 0], E[:, 1])))
+
+
+def test_generated_cloth():
+    c = synth.cloth_inputs(16, seed=1)
+    assert c.tris.shape == (2 * 15 * 15, 3)
+    r, cc, v = OS.triplets(c.tris[:50], c.rest_positions, c.positions, c.stretch, c.shear, c.strain_rate, c.dt)
+    assert np.all(r <= cc)
