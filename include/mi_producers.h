@@ -123,6 +123,28 @@ int mi_produce_bw_shell(mi_ctx*        ctx,
                         double*        vals,
                         void*          stream);
 
+/* Cloth bending, the discrete shell hinge (finite_element/constitutions/
+ * discrete_shell_bending.cu:84-156): per hinge h (hinges[4 h .. 4 h + 3] =
+ * V0, V1, V2, V3, the edge V1-V2 shared by triangles (V0, V1, V2) and
+ * (V3, V2, V1)), E = L0 kappa (theta - theta_bar)^2 / h_bar with the rest
+ * constants of discrete_shell_bending_reference.h:8-27 (L0 the rest edge
+ * length, h_bar = A / (3 L0), A the two rest triangles' area) and the 12x12
+ * Gauss-Newton Hessian dt^2 (2 L0 kappa / h_bar) grad(theta) grad(theta)^T
+ * (discrete_shell_bending_function.h:74-130; grad(theta) as
+ * utils/dihedral_angle.h:36-60).  Writes triplets 10 h .. 10 h + 9. */
+int mi_produce_bending(mi_ctx*        ctx,
+                       int64_t        n_hinges,
+                       const int32_t* hinges,
+                       const double*  rest_positions,
+                       const double*  positions,
+                       double         bending_stiffness,
+                       double         dt,
+                       int32_t        vert_offset,
+                       int32_t*       rows,
+                       int32_t*       cols,
+                       double*        vals,
+                       void*          stream);
+
 /* ------------------------------------------------ line search, after the solve */
 /* LineSearchReporter::step_forward of a vertex set: x = x0 + alpha dx over n
  * doubles (the FEM reporter's trial positions; engine/advance_ipc.cu:490-560

@@ -683,6 +683,29 @@ int mi_produce_bw_shell(mi_ctx*        ctx,
     MI_API_END
 }
 
+int mi_produce_bending(mi_ctx*        ctx,
+                       int64_t        n_hinges,
+                       const int32_t* hinges,
+                       const double*  rest_positions,
+                       const double*  positions,
+                       double         bending_stiffness,
+                       double         dt,
+                       int32_t        vert_offset,
+                       int32_t*       rows,
+                       int32_t*       cols,
+                       double*        vals,
+                       void*          stream)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(ctx && n_hinges >= 0 && vert_offset >= 0, MI_ERR_ARG, "mi_produce_bending: bad argument");
+    MI_REQUIRE(dt > 0.0 && bending_stiffness >= 0.0, MI_ERR_ARG, "mi_produce_bending: dt must be > 0, kappa >= 0");
+    MI_REQUIRE(n_hinges == 0 || (hinges && rest_positions && positions && rows && cols && vals), MI_ERR_ARG,
+               "mi_produce_bending: null buffer");
+    produce_bending(n_hinges, hinges, rest_positions, positions, bending_stiffness, dt, vert_offset, rows, cols, vals,
+                    pick(ctx, stream));
+    MI_API_END
+}
+
 int mi_energy_kinetic(mi_ctx*       ctx,
                       int32_t       vert_num,
                       const double* x,

@@ -158,6 +158,17 @@ void produce_abd(int64_t        n,
                  int32_t*       cols,
                  double*        vals,
                  hipStream_t    s);
+void produce_bending(int64_t        n,
+                     const int32_t* hinges,
+                     const double*  X,
+                     const double*  x,
+                     double         kappa,
+                     double         dt,
+                     int32_t        vert_offset,
+                     int32_t*       rows,
+                     int32_t*       cols,
+                     double*        vals,
+                     hipStream_t    s);
 void produce_bw_shell(int64_t        n,
                       const int32_t* tris,
                       const double*  X,

@@ -14,7 +14,13 @@
 //   H        = A dt^2 dF/dx^T (stretch_k H_stretch + shear_k H_shear) dF/dx,
 //            written as the 6 upper blocks of the 3-node stencil in the
 //            half-block order (a = 0..2, b = a..2; (min, max) of the global
-//            ids, transposed when reversed; utils/matrix_assembler.h:312-437).
+//            ids, transposed when reversed; utils/matrix_assembler.h:312-437);
+// and the discrete shell bending of interior edges
+// (finite_element/constitutions/discrete_shell_bending.cu:84-156): per hinge
+// (V0, V1, V2, V3) with edge V1-V2, E = L0 kappa (theta - theta_bar)^2 / h_bar,
+// its Gauss-Newton Hessian dt^2 E'' grad(theta) grad(theta)^T (PSD by
+// construction, discrete_shell_bending_function.h:74-130) as the 10 upper
+// blocks of the 4-node stencil.
 #include "linsys.hpp"
 
 namespace mi
@@ -218,7 +224,128 @@ __global__ __launch_bounds__(kThreads) void bw_shell_kernel(int64_t n,
             }
     }
 }
+// ---- discrete shell bending (hinge V0, V1 = V2 edge, V3; Gauss-Newton)
+__device__ __forceinline__ void sub3(const double a[3], const double b[3], double o[3])
+{
+    o[0] = a[0] - b[0];
+    o[1] = a[1] - b[1];
+    o[2] = a[2] - b[2];
+}
+__device__ __forceinline__ void cross3(const double a[3], const double b[3], double o[3])
+{
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+__device__ __forceinline__ double dot3(const double a[3], const double b[3])
+{
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+// d theta / d(V0, V1, V2, V3) (the closed form of utils/dihedral_angle.h:36-60)
+__device__ void dihedral_gradient(const double V0[3], const double V1[3], const double V2[3], const double V3[3],
+                                  double g[12])
+{
+    double e0[3], e1[3], e2[3], e3[3], e4[3], n1[3], n2[3];
+    sub3(V2, V1, e0);
+    sub3(V0, V1, e1);
+    sub3(V3, V1, e2);
+    sub3(V0, V2, e3);
+    sub3(V3, V2, e4);
+    cross3(e0, e1, n1);
+    cross3(e2, e0, n2);
+    const double q1 = dot3(n1, n1), q2 = dot3(n2, n2), l0 = sqrt(dot3(e0, e0));
+    for(int i = 0; i < 3; ++i)
+    {
+        g[i]     = -l0 / q1 * n1[i];
+        g[3 + i] = -dot3(e0, e3) / (l0 * q1) * n1[i] - dot3(e0, e4) / (l0 * q2) * n2[i];
+        g[6 + i] = dot3(e0, e1) / (l0 * q1) * n1[i] + dot3(e0, e2) / (l0 * q2) * n2[i];
+        g[9 + i] = -l0 / q2 * n2[i];
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void bending_kernel(int64_t n,
+                                                          const int32_t* __restrict__ hinges,
+                                                          const double* __restrict__ X,
+                                                          const double* __restrict__ x,
+                                                          double   kappa,
+                                                          double   dt2,
+                                                          int32_t  vert_offset,
+                                                          int32_t* rows,
+                                                          int32_t* cols,
+                                                          double*  vals)
+{
+    for(int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; h < n; h += (int64_t)gridDim.x * blockDim.x)
+    {
+        int32_t idx[4];
+        double  Xr[4][3], xc[4][3];
+        for(int k = 0; k < 4; ++k)
+        {
+            const int32_t q = hinges[4 * h + k];
+            idx[k]          = q + vert_offset;
+            for(int r = 0; r < 3; ++r)
+            {
+                Xr[k][r] = X[3 * (int64_t)q + r];
+                xc[k][r] = x[3 * (int64_t)q + r];
+            }
+        }
+        // rest constants (discrete_shell_bending_reference.h:8-27): hinge
+        // length, h = A / (3 L0) with A the two triangles' area
+        double a[3], b[3], n1[3], n2[3], e[3];
+        sub3(Xr[2], Xr[1], e);
+        const double L0 = sqrt(dot3(e, e));
+        sub3(Xr[1], Xr[0], a);
+        sub3(Xr[2], Xr[0], b);
+        cross3(a, b, n1);
+        sub3(Xr[2], Xr[3], a);
+        sub3(Xr[1], Xr[3], b);
+        cross3(a, b, n2);
+        const double A    = 0.5 * (sqrt(dot3(n1, n1)) + sqrt(dot3(n2, n2)));
+        const double hbar = A / 3.0 / L0;
+        // E = L0 kappa (theta - theta_bar)^2 / h; Gauss-Newton Hessian
+        // E'' grad grad^T, E'' = 2 L0 kappa / h, times dt^2
+        double g[12];
+        dihedral_gradient(xc[0], xc[1], xc[2], xc[3], g);
+        const double c    = 2.0 * L0 * kappa / hbar * dt2;
+        int64_t      slot = 10 * h;
+        for(int p = 0; p < 4; ++p)
+            for(int q = p; q < 4; ++q)
+            {
+                const bool swap = idx[p] > idx[q];
+                rows[slot]      = swap ? idx[q] : idx[p];
+                cols[slot]      = swap ? idx[p] : idx[q];
+                double* v       = vals + 9 * slot;
+                for(int cc = 0; cc < 3; ++cc)
+                    for(int r = 0; r < 3; ++r)
+                    {
+                        // H_ij = (c g_i) g_j with i <= j, mirrored (exactly
+                        // symmetric); the block is transposed when swapped
+                        const int i = 3 * p + (swap ? cc : r), j = 3 * q + (swap ? r : cc);
+                        v[r + 3 * cc] = (c * g[min(i, j)]) * g[max(i, j)];
+                    }
+                ++slot;
+            }
+    }
+}
 }  // namespace
+
+void produce_bending(int64_t        n,
+                     const int32_t* hinges,
+                     const double*  X,
+                     const double*  x,
+                     double         kappa,
+                     double         dt,
+                     int32_t        vert_offset,
+                     int32_t*       rows,
+                     int32_t*       cols,
+                     double*        vals,
+                     hipStream_t    s)
+{
+    if(n > 0)
+        bending_kernel<<<grid_shell(n), kThreads, 0, s>>>(n, hinges, X, x, kappa, dt * dt, vert_offset, rows, cols,
+                                                         vals);
+    MI_HIP(hipGetLastError());
+}
 
 void produce_bw_shell(int64_t        n,
                       const int32_t* tris,

@@ -177,6 +177,7 @@ class ClothInputs:
     shear: float
     strain_rate: float
     dt: float
+    bending: float  # discrete shell bending stiffness E t^3 / (12 (1 - nu^2))
 
 
 def cloth_inputs(n: int, seed: int = 1, *, youngs: float = 1e5, poisson: float = 0.3, thickness: float = 1e-3,
@@ -196,4 +197,52 @@ def cloth_inputs(n: int, seed: int = 1, *, youngs: float = 1e5, poisson: float =
     x = X @ Fd.T @ R.T + noise * h * rng.standard_normal(X.shape)
     mu = youngs / (2 * (1 + poisson))
     lam = youngs * poisson / (1 - poisson * poisson)  # plane-stress lambda
-    return ClothInputs(tris, X, x, (lam + 2 * mu) * 2 * thickness, mu, strain_rate, dt)
+    bending = youngs * thickness ** 3 / (12 * (1 - poisson * poisson))
+    return ClothInputs(tris, X, x, (lam + 2 * mu) * 2 * thickness, mu, strain_rate, dt, bending)
+
+
+def hinges_of(tris: np.ndarray) -> np.ndarray:
+    """Bending stencils (X0, X1, X2, X3) of the interior edges, as
+    DiscreteShellBending::do_init builds them (discrete_shell_bending.cu:258-290):
+    per edge (X1 < X2) shared by exactly two triangles, X0 and X3 the lower and
+    the higher opposite vertex, edges in ascending (X1, X2) order."""
+    T = np.sort(np.asarray(tris, np.int64), axis=1)
+    e = np.concatenate([T[:, [0, 1]], T[:, [0, 2]], T[:, [1, 2]]])
+    o = np.concatenate([T[:, 2], T[:, 1], T[:, 0]])
+    order = np.lexsort((o, e[:, 1], e[:, 0]))
+    e, o = e[order], o[order]
+    key = e[:, 0] * (int(T.max()) + 1) + e[:, 1]
+    start = np.flatnonzero(np.r_[True, key[1:] != key[:-1]])
+    count = np.diff(np.r_[start, len(key)])
+    two = start[count == 2]
+    return np.column_stack([o[two], e[two, 0], e[two, 1], o[two + 1]]).astype(np.int32)
+
+
+def cloth_contacts(n: int, n_contacts: int, seed: int = 1, *, cells: int = 2, kappa_lo: float = 1e4,
+                   kappa_hi: float = 1e8) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """Vertex-triangle stencils on the n x n cloth grid of cloth_inputs (the
+    recipe of the FEM generator's contacts): a random vertex p and a random
+    triangle of a cell within `cells` cells of it that does not contain p;
+    g ~ N(0, 1)^12 and kappa log-uniform in [kappa_lo, kappa_hi].
+    Returns (stencils int32[m, 4], g float64[m, 12], kappa float64[m])."""
+    rng = np.random.default_rng(seed)
+    got = []
+    have = 0
+    while have < n_contacts:
+        k = 2 * (n_contacts - have) + 16
+        pi = rng.integers(0, n, k)
+        pj = rng.integers(0, n, k)
+        ci = np.clip(pi + rng.integers(-cells, cells, k), 0, n - 2)
+        cj = np.clip(pj + rng.integers(-cells, cells, k), 0, n - 2)
+        upper = rng.integers(0, 2, k).astype(bool)
+        a = ci * n + cj
+        tri = np.where(upper[:, None], np.stack([a, a + 1, a + n + 1], 1), np.stack([a, a + n + 1, a + n], 1))
+        p = pi * n + pj
+        keep = ~(tri == p[:, None]).any(1)  # the vertex is not a corner of its triangle
+        got.append(np.column_stack([p, tri])[keep])
+        have += int(keep.sum())
+    st = np.concatenate(got)[:n_contacts].astype(np.int32)
+    m = len(st)
+    g = rng.standard_normal((m, 12))
+    kappa = np.exp(rng.uniform(np.log(kappa_lo), np.log(kappa_hi), m))
+    return st, g, kappa

@@ -103,3 +103,64 @@ def triplets(tris, X, x, stretch, shear, rate, dt, vert_offset=0):
                 vals[s] = K.reshape(-1, order="F")
                 s += 1
     return rows, cols, vals
+
+
+# ---- discrete shell bending (finite_element/constitutions/discrete_shell_bending.cu:84-156)
+def dihedral_angle(v0, v1, v2, v3) -> float:
+    """utils/dihedral_angle.h:14-30: hinge V1-V2, wings V0 and V3"""
+    n1 = np.cross(v1 - v0, v2 - v0)
+    n2 = np.cross(v2 - v3, v1 - v3)
+    th = np.arccos(np.clip(n1 @ n2 / np.sqrt((n1 @ n1) * (n2 @ n2)), -1.0, 1.0))
+    return -th if np.cross(n2, n1) @ (v1 - v2) < 0 else th
+
+
+def dihedral_gradient(V0, V1, V2, V3) -> np.ndarray:
+    e0, e1, e2, e3, e4 = V2 - V1, V0 - V1, V3 - V1, V0 - V2, V3 - V2
+    n1, n2 = np.cross(e0, e1), np.cross(e2, e0)
+    q1, q2, l0 = n1 @ n1, n2 @ n2, np.linalg.norm(e0)
+    return np.concatenate([-l0 / q1 * n1,
+                           -(e0 @ e3) / (l0 * q1) * n1 - (e0 @ e4) / (l0 * q2) * n2,
+                           (e0 @ e1) / (l0 * q1) * n1 + (e0 @ e2) / (l0 * q2) * n2,
+                           -l0 / q2 * n2])
+
+
+def bending_rest(X0, X1, X2, X3):
+    """(L0, h_bar, theta_bar) of discrete_shell_bending_reference.h:8-27"""
+    L0 = np.linalg.norm(X2 - X1)
+    A = 0.5 * (np.linalg.norm(np.cross(X1 - X0, X2 - X0)) + np.linalg.norm(np.cross(X2 - X3, X1 - X3)))
+    return L0, A / 3.0 / L0, dihedral_angle(X0, X1, X2, X3)
+
+
+def bending_energy(Xh, xh, kappa, dt) -> float:
+    L0, hb, tb = bending_rest(*Xh)
+    return dt * dt * L0 * kappa * (dihedral_angle(*xh) - tb) ** 2 / hb
+
+
+def bending_hessian(Xh, xh, kappa, dt) -> np.ndarray:
+    """Gauss-Newton: dt^2 (2 L0 kappa / h_bar) grad grad^T"""
+    L0, hb, _ = bending_rest(*Xh)
+    g = dihedral_gradient(*xh)
+    return dt * dt * 2 * L0 * kappa / hb * np.outer(g, g)
+
+
+def bending_triplets(hinges, X, x, kappa, dt, vert_offset=0):
+    X = np.asarray(X).reshape(-1, 3)
+    x = np.asarray(x).reshape(-1, 3)
+    n = len(hinges)
+    rows = np.empty(10 * n, np.int32)
+    cols = np.empty(10 * n, np.int32)
+    vals = np.empty((10 * n, 9))
+    for h, st in enumerate(hinges):
+        H = bending_hessian(X[st], x[st], kappa, dt)
+        idx = st + vert_offset
+        s = 10 * h
+        for a in range(4):
+            for b in range(a, 4):
+                K = H[3 * a:3 * a + 3, 3 * b:3 * b + 3]
+                if idx[a] > idx[b]:
+                    rows[s], cols[s], K = idx[b], idx[a], K.T
+                else:
+                    rows[s], cols[s] = idx[a], idx[b]
+                vals[s] = K.reshape(-1, order="F")
+                s += 1
+    return rows, cols, vals

@@ -1,8 +1,9 @@
-"""mi_produce_bw_shell against the oracle's restatement of the strain-limiting
-Baraff-Witkin shell: triplet rows and columns bit for bit, values to 1e-10 of
-the largest entry (same closed forms, different summation order), and the
-assembled 256 x 256 cloth (config 3's grid) solved with the 3x3 block
-Jacobi preconditioner."""
+"""mi_produce_bw_shell and mi_produce_bending against the oracle's
+restatements of the strain-limiting Baraff-Witkin shell and the discrete shell
+hinge: triplet rows and columns bit for bit, values to 1e-10 of the largest
+entry (same closed forms, different summation order), and the assembled
+256 x 256 cloth (config 3's grid: stretch/shear + bending + kinetic) solved
+with the 3x3 block Jacobi preconditioner."""
 import numpy as np
 import pytest
 
@@ -37,6 +38,27 @@ def test_bw_shell_matches_oracle(ctx, seed, noise):
     assert np.abs(vals.numpy() - v).max() <= 1e-10 * np.abs(v).max()
 
 
+def produce_bending(ctx, c, hinges, vert_offset=0):
+    H = len(hinges)
+    rows, cols, vals = ctx.alloc(10 * H, np.int32), ctx.alloc(10 * H, np.int32), ctx.alloc((10 * H, 9), np.float64)
+    keep = [ctx.to_device(a) for a in (hinges, c.rest_positions, c.positions)]
+    check(ctx.lib.mi_produce_bending(ctx.h, H, keep[0].ptr, keep[1].ptr, keep[2].ptr, c.bending, c.dt, vert_offset,
+                                     rows.ptr, cols.ptr, vals.ptr, None))
+    ctx.synchronize()
+    return rows, cols, vals
+
+
+@pytest.mark.parametrize("seed,noise", [(1, 0.05), (2, 0.4)])
+def test_bending_matches_oracle(ctx, seed, noise):
+    c = synth.cloth_inputs(20, seed=seed, noise=noise)
+    hg = synth.hinges_of(c.tris)
+    hg[::4] = hg[::4, ::-1]  # reversed stencils take the transposed path
+    rows, cols, vals = produce_bending(ctx, c, hg, vert_offset=3)
+    r, cc, v = OS.bending_triplets(hg, c.rest_positions, c.positions, c.bending, c.dt, vert_offset=3)
+    assert np.array_equal(rows.numpy(), r) and np.array_equal(cols.numpy(), cc)
+    assert np.abs(vals.numpy() - v).max() <= 1e-10 * np.abs(v).max()
+
+
 def test_bw_cloth_256_solve(ctx):
     c = synth.cloth_inputs(256, seed=3)
     V, T = len(c.rest_positions), len(c.tris)
@@ -46,9 +68,11 @@ def test_bw_cloth_256_solve(ctx):
     mr, mc, mv = ctx.alloc(V, np.int32), ctx.alloc(V, np.int32), ctx.alloc((V, 9), np.float64)
     dm = ctx.to_device(mass)
     check(ctx.lib.mi_produce_kinetic(ctx.h, V, dm.ptr, 0, mr.ptr, mc.ptr, mv.ptr, None))
-    r = np.concatenate([mr.numpy(), rows.numpy()])
-    cc = np.concatenate([mc.numpy(), cols.numpy()])
-    v = np.concatenate([mv.numpy(), vals.numpy()])
+    hg = synth.hinges_of(c.tris)
+    br, bc, bv = produce_bending(ctx, c, hg)
+    r = np.concatenate([mr.numpy(), rows.numpy(), br.numpy()])
+    cc = np.concatenate([mc.numpy(), cols.numpy(), bc.numpy()])
+    v = np.concatenate([mv.numpy(), vals.numpy(), bv.numpy()])
     ls = LinearSystem(ctx, V)
     rd, cd, vd = ctx.to_device(r), ctx.to_device(cc), ctx.to_device(v)
     ls.assemble(rd, cd, vd)
@@ -62,5 +86,6 @@ def test_bw_cloth_256_solve(ctx):
     orr, oc, ov, _ = O.convert_sym(V, r, cc, v)
     Ax, _ = O.spmv_sym(V, orr, oc, ov, x.numpy())
     assert np.linalg.norm(rhs - Ax) <= 2e-6 * np.linalg.norm(rhs)
-    print(f"\n[cloth] 256x256: {T} triangles, {st['iters_exact']} PCG iterations to 1e-6", flush=True)
+    print(f"\n[cloth] 256x256: {T} triangles, {len(hg)} hinges, {st['iters_exact']} PCG iterations to 1e-6",
+          flush=True)
     ls.close()

@@ -72,3 +72,39 @@ def test_generated_cloth():
     assert c.tris.shape == (2 * 15 * 15, 3)
     r, cc, v = OS.triplets(c.tris[:50], c.rest_positions, c.positions, c.stretch, c.shear, c.strain_rate, c.dt)
     assert np.all(r <= cc)
+
+
+def test_dihedral_gradient_is_the_derivative_of_the_angle():
+    rng = np.random.default_rng(5)
+    for _ in range(5):
+        V = rng.standard_normal((4, 3))
+        g = OS.dihedral_gradient(*V)
+        h = 1e-6
+        fd = np.zeros(12)
+        for i in range(12):
+            e = np.zeros(12)
+            e[i] = h
+            fd[i] = (OS.dihedral_angle(*(V.reshape(-1) + e).reshape(4, 3))
+                     - OS.dihedral_angle(*(V.reshape(-1) - e).reshape(4, 3))) / (2 * h)
+        assert np.abs(fd - g).max() <= 1e-6 * np.abs(g).max()
+
+
+def test_bending_hessian_at_rest_is_the_second_derivative():
+    """at the rest shape E' = 0, so the Gauss-Newton Hessian is exact"""
+    rng = np.random.default_rng(6)
+    X = np.array([[0.0, 1.0, 0.1], [0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [1.0, 1.0, -0.2]]) + 0.05 * rng.standard_normal((4, 3))
+    H = OS.bending_hessian(X, X, 2.0, 0.1)
+    f = lambda v: OS.bending_energy(X, v.reshape(4, 3), 2.0, 0.1)  # noqa: E731
+    fd = fd_hessian(f, X.reshape(-1), 1e-5)
+    assert np.abs(fd - H).max() <= 1e-5 * np.abs(H).max()
+
+
+def test_hinges_of_a_grid():
+    c = synth.cloth_inputs(5, seed=1)
+    hg = synth.hinges_of(c.tris)
+    # interior edges of a 4x4-cell grid split into 2 triangles per cell
+    n = 5
+    assert len(hg) == 2 * (n - 1) * (n - 2) + (n - 1) ** 2
+    assert np.all(hg[:, 1] < hg[:, 2]) and np.all(hg[:, 0] < hg[:, 3])
+    k = hg[:, 1].astype(np.int64) * 100 + hg[:, 2]
+    assert np.all(np.diff(k) > 0)
