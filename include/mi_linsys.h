@@ -189,6 +189,14 @@ enum mi_mas_build
  * (global_linear_system.cu:546-590). */
 int mi_precond_add(mi_sys* sys, const mi_precond_desc* desc, int* handle_out);
 
+/* Rebuilds preconditioner `handle` in place from a new descriptor (any kind;
+ * the same overlap rules as mi_precond_add), for a topology change: the MAS
+ * adjacency, partition and hierarchy are rebuilt (on the device with
+ * MI_MAS_BUILD_DEVICE).  The reference re-runs LocalPreconditioner::do_init
+ * for it (fem_mas_preconditioner.cu:157-404).  Synchronises the device; call
+ * mi_precond_assemble before the next apply or solve. */
+int mi_precond_reinit(mi_sys* sys, int handle, const mi_precond_desc* desc);
+
 /* LocalPreconditioner::do_assemble for every registered preconditioner
  * (global_linear_system.cu:499-515).  Requires a prior mi_sys_assemble. */
 int mi_precond_assemble(mi_sys* sys, void* stream);

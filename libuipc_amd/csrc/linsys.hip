@@ -57,6 +57,7 @@ struct mi_sys
     MatrixConverter                                   conv;
     SpmvScratch                                       spmv_scr, dot_scr;
     std::vector<std::unique_ptr<LocalPreconditioner>> pcs;
+    uint64_t                                          pcs_gen       = 0;  // bumped when pcs change
     bool                                              assembled     = false;
     bool                                              pcs_assembled = false;
 
@@ -74,13 +75,14 @@ struct mi_sys
         const void* x = nullptr;
         const void* b = nullptr;
         int64_t     n = -1, U = -1;
+        uint64_t    pcs_gen = ~0ull;
         int         ci = 0, fused = -1, bypass = -1, use32 = -1;
         const void* bufs[10] = {};
         hipStream_t stream  = nullptr;
         bool        operator==(const GraphKey& o) const
         {
-            if(x != o.x || b != o.b || n != o.n || U != o.U || ci != o.ci || fused != o.fused || bypass != o.bypass
-               || use32 != o.use32 || stream != o.stream)
+            if(x != o.x || b != o.b || n != o.n || U != o.U || pcs_gen != o.pcs_gen || ci != o.ci || fused != o.fused
+               || bypass != o.bypass || use32 != o.use32 || stream != o.stream)
                 return false;
             for(int k = 0; k < 10; ++k)
                 if(bufs[k] != o.bufs[k])
@@ -476,27 +478,53 @@ int mi_spmv(mi_sys* sys, double a, const double* x, double b, double* y, void* s
     MI_API_END
 }
 
+namespace
+{
+// a preconditioner for `d`, checked against the system and the other ranges
+// (the one at `skip` excepted)
+std::unique_ptr<LocalPreconditioner> build_precond(mi_sys* sys, const mi_precond_desc* d, int skip, const char* who)
+{
+    MI_REQUIRE(sys && d, MI_ERR_ARG, std::string(who) + ": null argument");
+    MI_REQUIRE(d->dof_offset >= 0 && d->dof_count >= 0 && d->dof_offset + d->dof_count <= sys->n(), MI_ERR_ARG,
+               std::string(who) + ": DoF range outside the system");
+    for(int k = 0; k < (int)sys->pcs.size(); ++k)
+    {
+        const auto& pc = sys->pcs[k];
+        MI_REQUIRE(k == skip || d->dof_offset >= pc->dof_offset + pc->dof_count
+                       || d->dof_offset + d->dof_count <= pc->dof_offset,
+                   MI_ERR_ARG, std::string(who) + ": overlapping preconditioner ranges");
+    }
+    switch(d->kind)
+    {
+        case MI_PRECOND_DIAG3: return make_diag3(d->dof_offset, d->dof_count);
+        case MI_PRECOND_DIAG12: return make_diag12(d->dof_offset, d->dof_count);
+        case MI_PRECOND_MAS: return make_mas(*d);
+        default: throw Error(MI_ERR_ARG, std::string(who) + ": unknown kind");
+    }
+}
+}  // namespace
+
 int mi_precond_add(mi_sys* sys, const mi_precond_desc* d, int* handle)
 {
     MI_API_BEGIN
-    MI_REQUIRE(sys && d, MI_ERR_ARG, "mi_precond_add: null argument");
-    MI_REQUIRE(d->dof_offset >= 0 && d->dof_count >= 0 && d->dof_offset + d->dof_count <= sys->n(), MI_ERR_ARG,
-               "mi_precond_add: DoF range outside the system");
-    for(auto& pc : sys->pcs)
-        MI_REQUIRE(d->dof_offset >= pc->dof_offset + pc->dof_count || d->dof_offset + d->dof_count <= pc->dof_offset,
-                   MI_ERR_ARG, "mi_precond_add: overlapping preconditioner ranges");
-    std::unique_ptr<LocalPreconditioner> pc;
-    switch(d->kind)
-    {
-        case MI_PRECOND_DIAG3: pc = make_diag3(d->dof_offset, d->dof_count); break;
-        case MI_PRECOND_DIAG12: pc = make_diag12(d->dof_offset, d->dof_count); break;
-        case MI_PRECOND_MAS: pc = make_mas(*d); break;
-        default: throw Error(MI_ERR_ARG, "mi_precond_add: unknown kind");
-    }
+    auto pc = build_precond(sys, d, -1, "mi_precond_add");
     sys->pcs.push_back(std::move(pc));
     sys->pcs_assembled = false;
+    ++sys->pcs_gen;
     if(handle)
         *handle = (int)sys->pcs.size() - 1;
+    MI_API_END
+}
+
+int mi_precond_reinit(mi_sys* sys, int handle, const mi_precond_desc* d)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(sys && handle >= 0 && handle < (int)sys->pcs.size(), MI_ERR_ARG, "mi_precond_reinit: bad handle");
+    auto pc = build_precond(sys, d, handle, "mi_precond_reinit");
+    MI_HIP(hipDeviceSynchronize());  // no launch may still read the preconditioner being replaced
+    sys->pcs[handle]   = std::move(pc);
+    sys->pcs_assembled = false;
+    ++sys->pcs_gen;
     MI_API_END
 }
 
@@ -1011,6 +1039,7 @@ extern "C" int mi_pcg_solve(mi_sys* sys, const double* b, double* x, const mi_pc
     key.b       = b;
     key.n       = n;
     key.U       = sys->A.U;
+    key.pcs_gen = sys->pcs_gen;
     key.ci      = gb;
     key.fused   = fused ? 1 : 0;
     key.bypass  = bypass ? 1 : 0;

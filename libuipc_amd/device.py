@@ -201,10 +201,12 @@ class LinearSystem:
 
     def add_preconditioner(self, kind: int, dof_offset: int, dof_count: int, *, part_ids=None, tets=None,
                            tris=None, edges=None, mas_precision: int = abi.MI_FP64, active_levels: int = 0,
-                           build: int = abi.MI_MAS_BUILD_HOST, device_arrays: dict | None = None) -> int:
+                           build: int = abi.MI_MAS_BUILD_HOST, device_arrays: dict | None = None,
+                           reinit: int | None = None) -> int:
         """`device_arrays` (MI_MAS_BUILD_DEVICE only) maps "part_ids", "tets",
         "tris", "edges" to device pointers read instead of the host arrays;
-        their counts still come from the host arrays' shapes."""
+        their counts still come from the host arrays' shapes.  `reinit` =
+        a handle rebuilds that preconditioner in place (mi_precond_reinit)."""
         keep = []
 
         def arr(a, k):
@@ -222,6 +224,9 @@ class LinearSystem:
         d = abi.PrecondDesc(kind, dof_offset, dof_count, dof_count // 3, pp, tp, nt, trp, ntr, ep, ne,
                             mas_precision, active_levels, build, dev.get("part_ids"), dev.get("tets"),
                             dev.get("tris"), dev.get("edges"))
+        if reinit is not None:
+            check(self.lib.mi_precond_reinit(self.h, reinit, C.byref(d)))
+            return reinit
         h = C.c_int()
         check(self.lib.mi_precond_add(self.h, C.byref(d), C.byref(h)))
         return h.value

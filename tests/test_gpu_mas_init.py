@@ -2,7 +2,7 @@
 host build and the oracle: every table bit-exact, on tet meshes with
 geometric, random (disconnected parts) and partly unpartitioned partitions,
 on triangle / edge topologies, with device-resident inputs, and at the
-64^3-grid size."""
+64^3-grid size; and the rebuild after a topology change (mi_precond_reinit)."""
 import time
 
 import numpy as np
@@ -112,3 +112,64 @@ def test_device_build_full_size(ctx):
     assert_same(host, dev)
     print(f"\nMAS init at 64^3 ({g.block_rows} vertices, {len(g.tets)} tets): host {t_host:.1f} ms, "
           f"device {t_dev:.1f} ms (levels {dev['L']}, clusters {dev['clusters']})", flush=True)
+
+
+def test_reinit_after_a_topology_change(ctx):
+    """mi_precond_reinit rebuilds a MAS preconditioner in place: the tables
+    and the next solve equal those of a system built fresh with the new
+    partition, and a solve captured before the change is not replayed"""
+    s, _ = system(10, 300, 23, "geometric")
+    s2parts = random_partition(np.random.default_rng(7), s.block_rows, None)
+    rows, cols, vals = ctx.to_device(s.rows), ctx.to_device(s.cols), ctx.to_device(s.vals)
+    b = ctx.to_device(s.rhs)
+
+    def solved(ls):
+        ls.assemble(rows, cols, vals)
+        ls.assemble_preconditioners()
+        x = ctx.alloc(s.n, np.float64)
+        st = ls.solve(b, x, criterion=abi.MI_CRIT_REL_RESIDUAL, tol=1e-8)
+        return x.numpy(), st
+
+    ls = LinearSystem(ctx, s.block_rows)
+    h = ls.add_preconditioner(abi.MI_PRECOND_MAS, 0, s.n, part_ids=s.part_ids, tets=s.tets)
+    x1, _ = solved(ls)
+    ls.add_preconditioner(abi.MI_PRECOND_MAS, 0, s.n, part_ids=s2parts, tets=s.tets, build=abi.MI_MAS_BUILD_DEVICE,
+                          reinit=h)
+    x2, st2 = solved(ls)
+    fresh = LinearSystem(ctx, s.block_rows)
+    hf = fresh.add_preconditioner(abi.MI_PRECOND_MAS, 0, s.n, part_ids=s2parts, tets=s.tets)
+    xf, stf = solved(fresh)
+    assert_same(ls.mas_export(h, matrices=False), fresh.mas_export(hf, matrices=False))
+    # the coarse cluster sums use FP64 atomics, so equal to rounding only
+    assert abs(st2["iters_exact"] - stf["iters_exact"]) <= 1
+    assert np.linalg.norm(x2 - xf) <= 1e-9 * np.linalg.norm(xf)
+    assert not np.array_equal(x1, x2)
+    ls.close()
+    fresh.close()
+
+
+def test_adding_a_preconditioner_after_a_solve(ctx):
+    """a preconditioner added after a captured solve takes part in the next one"""
+    s, _ = system(8, 200, 24, "geometric")
+    rows, cols, vals = ctx.to_device(s.rows), ctx.to_device(s.cols), ctx.to_device(s.vals)
+    b = ctx.to_device(s.rhs)
+    half = 3 * (s.block_rows // 2)
+
+    def run(ls):
+        ls.assemble(rows, cols, vals)
+        ls.assemble_preconditioners()
+        x = ctx.alloc(s.n, np.float64)
+        ls.solve(b, x, criterion=abi.MI_CRIT_REL_RESIDUAL, tol=1e-8)
+        return x.numpy()
+
+    ls = LinearSystem(ctx, s.block_rows)
+    ls.add_preconditioner(abi.MI_PRECOND_DIAG3, 0, half)
+    run(ls)
+    ls.add_preconditioner(abi.MI_PRECOND_DIAG3, half, s.n - half)
+    x = run(ls)
+    fresh = LinearSystem(ctx, s.block_rows)
+    fresh.add_preconditioner(abi.MI_PRECOND_DIAG3, 0, half)
+    fresh.add_preconditioner(abi.MI_PRECOND_DIAG3, half, s.n - half)
+    assert np.array_equal(x, run(fresh))
+    ls.close()
+    fresh.close()
