@@ -359,6 +359,26 @@ int mi_pcg_solve(mi_sys*            sys,
                  mi_pcg_stats*      stats_host,
                  void*              stream);
 
+/* Batched PCG: the system holds nseg independent systems on the block-row
+ * ranges [seg_rows_host[k], seg_rows_host[k + 1]) (seg_rows_host[0] = 0,
+ * seg_rows_host[nseg] = block_rows; no block may couple two ranges, and the
+ * registered preconditioners must not either -- a MAS partition must not
+ * cross them).  Every system runs FusedPCG's semantics with its own alpha,
+ * beta, convergence test and cap (max_iter_ratio times its own DoFs); all of
+ * them advance in the same captured iterations (one graph replay serves the
+ * batch), and a system that has stopped keeps its x.  The PSD fallback is not
+ * run: a system whose rz0 is not positive stops at once, converged = 0.
+ * stats_host: nseg entries.  Synchronises at each block of graph_checks
+ * check intervals. */
+int mi_pcg_solve_batched(mi_sys*            sys,
+                         int32_t            nseg,
+                         const int32_t*     seg_rows_host,
+                         const double*      b,
+                         double*            x,
+                         const mi_pcg_opts* opts,
+                         mi_pcg_stats*      stats_host,
+                         void*              stream);
+
 #ifdef __cplusplus
 }
 #endif

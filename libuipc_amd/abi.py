@@ -180,6 +180,8 @@ LINSYS_PROTOS = {
     "mi_pcg_default_opts": (C.c_int, [C.POINTER(PcgOpts)]),
     "mi_pcg_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(PcgOpts), C.POINTER(PcgStats),
                                C.c_void_p]),
+    "mi_pcg_solve_batched": (C.c_int, [C.c_void_p, C.c_int32, c_i32p, C.c_void_p, C.c_void_p, C.POINTER(PcgOpts),
+                                       C.POINTER(PcgStats), C.c_void_p]),
 }
 
 SYNTH_PROTOS = {

@@ -92,8 +92,33 @@ struct mi_sys
     } graph_key;
     bool graph_failed = false;
 
+    // batched solves (mi_pcg_solve_batched): the tile table of the last
+    // segmentation, per-system scalars, tile partials and their own graph
+    struct Batch
+    {
+        std::vector<int32_t> seg_rows;
+        DBuf<int64_t>        tile_lo, sys_lo;
+        DBuf<int32_t>        tile_sys, sys_tile;
+        BatchTiles           tiles;
+        DBuf<BatchScalars>   sc;
+        BatchScalars*        h_sc = nullptr;  // pinned, nseg
+        size_t               h_cap = 0;
+        DBuf<double>         part_a, part_b;
+        std::vector<double>  h_part;
+        hipGraphExec_t       graph_exec = nullptr;
+        hipGraph_t           graph      = nullptr;
+        GraphKey             key;
+        uint64_t             seg_gen = 0, key_seg_gen = ~0ull;
+    } batch;
+
     ~mi_sys()
     {
+        if(batch.graph_exec)
+            (void)hipGraphExecDestroy(batch.graph_exec);
+        if(batch.graph)
+            (void)hipGraphDestroy(batch.graph);
+        if(batch.h_sc)
+            (void)hipHostFree(batch.h_sc);
         if(graph_exec)
             (void)hipGraphExecDestroy(graph_exec);
         if(graph)
@@ -1118,6 +1143,263 @@ extern "C" int mi_pcg_solve(mi_sys* sys, const double* b, double* x, const mi_pc
         double rn = norm_f64_host(sys->tmp.data(), n, s);
         double bn = std::sqrt(bb);
         st->rel_res_true = bn > 0 ? rn / bn : 0.0;
+    }
+    MI_API_END
+}
+
+namespace
+{
+// one batched iteration: the union SpMV, then per-system alpha, x / r,
+// preconditioner, per-system convergence, beta and p
+void run_batch_iteration(mi_sys* sys, double* x, hipStream_t s)
+{
+    auto&         bt    = sys->batch;
+    const int32_t nseg  = (int32_t)bt.seg_rows.size() - 1;
+    const bool    use32 = sys->desc.value_precision == MI_FP32;
+    float4*       p32   = use32 ? sys->p32.data() : nullptr;
+    spmv_full(sys->A, use32, sys->p.data(), p32, sys->Ap.data(), sys->spmv_scr.partials.data(), nullptr,
+              sys->spmv_scr, s);
+    batch_tile_dot(bt.tiles, sys->p.data(), sys->Ap.data(), bt.part_a.data(), nullptr, s);
+    batch_alpha_xr(bt.tiles, bt.sc.data(), nseg, bt.part_a.data(), x, sys->p.data(), sys->r.data(), sys->Ap.data(),
+                   s);
+    sys->apply_preconditioner(sys->r.data(), sys->z.data(), nullptr, s);
+    batch_tile_dot(bt.tiles, sys->r.data(), sys->z.data(), bt.part_a.data(), bt.part_b.data(), s);
+    batch_beta_p(bt.tiles, bt.sc.data(), nseg, bt.part_a.data(), bt.part_b.data(), sys->p.data(), sys->z.data(), p32,
+                 s);
+}
+
+void read_batch(mi_sys* sys, int32_t nseg, hipStream_t s)
+{
+    MI_HIP(hipMemcpyAsync(sys->batch.h_sc, sys->batch.sc.data(), sizeof(BatchScalars) * nseg, hipMemcpyDeviceToHost,
+                          s));
+    MI_HIP(hipStreamSynchronize(s));
+}
+
+// per-system sums of tile partials on the host, in tile order
+std::vector<double> host_seg_sums(mi_sys* sys, const double* part, int32_t nseg, hipStream_t s)
+{
+    auto& bt = sys->batch;
+    bt.h_part.resize((size_t)bt.tiles.n);
+    MI_HIP(hipMemcpyAsync(bt.h_part.data(), part, sizeof(double) * bt.tiles.n, hipMemcpyDeviceToHost, s));
+    MI_HIP(hipStreamSynchronize(s));
+    std::vector<double> out((size_t)nseg, 0.0);
+    int32_t             t = 0;
+    for(int32_t g = 0; g < nseg; ++g)
+    {
+        const int64_t hi = 3 * (int64_t)bt.seg_rows[g + 1];
+        double        acc = 0.0;
+        for(int64_t lo = 3 * (int64_t)bt.seg_rows[g]; lo < hi; lo += kBatchTile)
+            acc += bt.h_part[t++];
+        out[g] = acc;
+    }
+    return out;
+}
+}  // namespace
+
+extern "C" int mi_pcg_solve_batched(mi_sys*            sys,
+                                    int32_t            nseg,
+                                    const int32_t*     seg_rows,
+                                    const double*      b,
+                                    double*            x,
+                                    const mi_pcg_opts* opts,
+                                    mi_pcg_stats*      st,
+                                    void*              stream)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(sys && seg_rows && b && x && st && nseg > 0, MI_ERR_ARG, "mi_pcg_solve_batched: null argument");
+    MI_REQUIRE(sys->assembled, MI_ERR_STATE, "mi_pcg_solve_batched: system not assembled");
+    MI_REQUIRE(sys->pcs.empty() || sys->pcs_assembled, MI_ERR_STATE,
+               "mi_pcg_solve_batched: preconditioners not assembled");
+    MI_REQUIRE(seg_rows[0] == 0 && seg_rows[nseg] == sys->desc.block_rows, MI_ERR_ARG,
+               "mi_pcg_solve_batched: the segments must cover the block rows from 0");
+    for(int32_t g = 0; g < nseg; ++g)
+        MI_REQUIRE(seg_rows[g + 1] > seg_rows[g], MI_ERR_ARG, "mi_pcg_solve_batched: empty or decreasing segment");
+    mi_pcg_opts o;
+    mi_pcg_default_opts(&o);
+    if(opts)
+        o = *opts;
+    MI_REQUIRE(o.tol > 0.0 && o.max_iter_ratio > 0.0, MI_ERR_ARG,
+               "mi_pcg_solve_batched: tol and max_iter_ratio must be > 0");
+    MI_REQUIRE(o.criterion == MI_CRIT_RZ_RATIO || o.criterion == MI_CRIT_REL_RESIDUAL, MI_ERR_ARG,
+               "mi_pcg_solve_batched: unknown criterion");
+    hipStream_t   s     = pick(sys->ctx, stream);
+    const int64_t n     = sys->n();
+    const double  ratio = sys->desc.reserve_ratio;
+    auto&         bt    = sys->batch;
+
+    // the tile table (rebuilt when the segmentation changes)
+    if(bt.seg_rows.size() != (size_t)nseg + 1 || !std::equal(bt.seg_rows.begin(), bt.seg_rows.end(), seg_rows))
+    {
+        bt.seg_rows.assign(seg_rows, seg_rows + nseg + 1);
+        std::vector<int64_t> lo, slo((size_t)nseg + 1);
+        std::vector<int32_t> tsys, stile((size_t)nseg + 1);
+        for(int32_t g = 0; g < nseg; ++g)
+        {
+            stile[g]        = (int32_t)lo.size();
+            slo[g]          = 3 * (int64_t)seg_rows[g];
+            const int64_t e = 3 * (int64_t)seg_rows[g + 1];
+            for(int64_t t = slo[g]; t < e; t += kBatchTile)
+            {
+                lo.push_back(t);
+                tsys.push_back(g);
+            }
+        }
+        stile[nseg] = (int32_t)lo.size();
+        slo[nseg]   = n;
+        lo.push_back(n);
+        MI_REQUIRE(lo.size() < ((size_t)1 << 31), MI_ERR_ARG, "mi_pcg_solve_batched: too many tiles");
+        auto up = [&](auto& d, const auto& h)
+        {
+            d.resize(h.size());
+            MI_HIP(hipMemcpyAsync(d.data(), h.data(), sizeof(h[0]) * h.size(), hipMemcpyHostToDevice, s));
+        };
+        up(bt.tile_lo, lo);
+        up(bt.tile_sys, tsys);
+        up(bt.sys_tile, stile);
+        up(bt.sys_lo, slo);
+        MI_HIP(hipStreamSynchronize(s));
+        bt.tiles          = BatchTiles{(int32_t)tsys.size(), bt.tile_lo.data(), bt.tile_sys.data(), bt.sys_tile.data(),
+                                       bt.sys_lo.data()};
+        bt.part_a.resize(tsys.size());
+        bt.part_b.resize(tsys.size());
+        bt.sc.resize((size_t)nseg);
+        if(bt.h_cap < (size_t)nseg)
+        {
+            if(bt.h_sc)
+                (void)hipHostFree(bt.h_sc);
+            bt.h_sc = nullptr;
+            MI_HIP(hipHostMalloc((void**)&bt.h_sc, sizeof(BatchScalars) * nseg));
+            bt.h_cap = (size_t)nseg;
+        }
+        ++bt.seg_gen;
+    }
+    for(int32_t g = 0; g < nseg; ++g)
+    {
+        st[g]              = mi_pcg_stats{};
+        st[g].rel_res_true = NAN;
+    }
+    sys->r.resize((size_t)n, ratio);
+    sys->z.resize((size_t)n, ratio);
+    sys->p.resize((size_t)n, ratio);
+    sys->Ap.resize((size_t)n, ratio);
+    const bool use32 = sys->desc.value_precision == MI_FP32;
+    if(use32)
+        sys->p32.resize((size_t)sys->desc.block_rows, ratio);
+    spmv_prepare(sys->spmv_scr, use32);
+
+    // r = b, x = 0, z = P r, p = z; per-system rz0, b^T b
+    pcg_init_vectors(x, sys->r.data(), b, n, s);
+    sys->apply_preconditioner(sys->r.data(), sys->z.data(), nullptr, s);
+    batch_tile_dot(bt.tiles, sys->r.data(), sys->z.data(), bt.part_a.data(), bt.part_b.data(), s);
+    batch_init(bt.tiles, bt.sc.data(), nseg, bt.part_a.data(), bt.part_b.data(), o.criterion, o.tol, o.max_iter_ratio,
+               s);
+    copy_f64(sys->p.data(), sys->z.data(), n, s);
+    if(use32)
+        pack_f32x4(sys->p.data(), sys->p32.data(), sys->desc.block_rows, s);
+    read_batch(sys, nseg, s);
+    std::vector<double> rz0((size_t)nseg), bb((size_t)nseg);
+    int64_t             longest = 0;
+    bool                all     = true;
+    for(int32_t g = 0; g < nseg; ++g)
+    {
+        rz0[g] = bt.h_sc[g].rz;
+        bb[g]  = bt.h_sc[g].bb;
+        longest = std::max<int64_t>(longest, bt.h_sc[g].max_iter);
+        all     = all && bt.h_sc[g].stopped;
+        MI_REQUIRE(std::isfinite(rz0[g]) && std::isfinite(bb[g]), MI_ERR_NONFINITE,
+                   "mi_pcg_solve_batched: non-finite r^T z at start of system " + std::to_string(g));
+    }
+
+    const int     ci    = o.check_interval > 0 ? o.check_interval : 1;
+    const int     gb    = ci * (o.graph_checks > 0 ? o.graph_checks : 4);
+    const int64_t total = longest > 0 ? longest - 1 : 0;
+    mi_sys::GraphKey key;
+    key.x       = x;
+    key.b       = b;
+    key.n       = n;
+    key.U       = sys->A.U;
+    key.pcs_gen = sys->pcs_gen;
+    key.ci      = gb;
+    key.use32   = sys->desc.value_precision;
+    key.bufs[0] = sys->r.data();
+    key.bufs[1] = sys->z.data();
+    key.bufs[2] = sys->p.data();
+    key.bufs[3] = sys->Ap.data();
+    key.bufs[4] = sys->A.full_ptr.data();
+    key.bufs[5] = sys->A.fcols.data();
+    key.bufs[6] = sys->A.has32 ? (const void*)sys->A.fvals32.data() : (const void*)sys->A.fvals.data();
+    key.bufs[7] = bt.sc.data();
+    key.bufs[8] = bt.part_a.data();
+    key.bufs[9] = sys->p32.data();
+    key.stream  = s;
+    int64_t done = 0;
+    while(!all && done < total)
+    {
+        const int64_t blk       = std::min<int64_t>(gb, total - done);
+        bool          use_graph = o.use_graph && !sys->graph_failed && blk == gb && s != nullptr;
+        if(use_graph && !(bt.graph_exec && bt.key == key && bt.key_seg_gen == bt.seg_gen))
+        {
+            if(bt.graph_exec)
+                (void)hipGraphExecDestroy(bt.graph_exec);
+            if(bt.graph)
+                (void)hipGraphDestroy(bt.graph);
+            bt.graph_exec = nullptr;
+            bt.graph      = nullptr;
+            hipError_t e  = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+            if(e == hipSuccess)
+            {
+                for(int k = 0; k < gb; ++k)
+                    run_batch_iteration(sys, x, s);
+                e = hipStreamEndCapture(s, &bt.graph);
+                if(e == hipSuccess)
+                    e = hipGraphInstantiate(&bt.graph_exec, bt.graph, nullptr, nullptr, 0);
+            }
+            if(e != hipSuccess)
+            {
+                (void)hipGetLastError();
+                sys->graph_failed = true;
+                bt.graph_exec     = nullptr;
+                use_graph         = false;
+            }
+            else
+            {
+                bt.key         = key;
+                bt.key_seg_gen = bt.seg_gen;
+            }
+        }
+        if(use_graph)
+            MI_HIP(hipGraphLaunch(bt.graph_exec, s));
+        else
+            for(int64_t k = 0; k < blk; ++k)
+                run_batch_iteration(sys, x, s);
+        MI_HIP(hipGetLastError());
+        done += blk;
+        read_batch(sys, nseg, s);
+        all = true;
+        for(int32_t g = 0; g < nseg; ++g)
+            all = all && bt.h_sc[g].stopped;
+    }
+    read_batch(sys, nseg, s);
+    for(int32_t g = 0; g < nseg; ++g)
+    {
+        const BatchScalars& q = bt.h_sc[g];
+        const bool          c = q.converged != 0;
+        st[g].iters       = c ? std::min<int64_t>((q.conv_iter + ci - 1) / ci * ci, done) : q.max_iter;
+        st[g].iters_exact = c ? q.conv_iter : -1;
+        st[g].converged   = c ? 1 : 0;
+        st[g].rz0         = rz0[g];
+        st[g].rz_final    = q.rz_new;
+        st[g].rel_res     = bb[g] > 0 ? std::sqrt(q.rr / bb[g]) : 0.0;
+    }
+    if(o.true_residual)
+    {
+        sys->tmp.resize((size_t)n, ratio);
+        copy_f64(sys->tmp.data(), b, n, s);
+        spmv_axpby(sys->A, use32, -1.0, x, 1.0, sys->tmp.data(), sys->spmv_scr, s);
+        batch_tile_dot(bt.tiles, sys->tmp.data(), sys->tmp.data(), bt.part_a.data(), nullptr, s);
+        const auto rr = host_seg_sums(sys, bt.part_a.data(), nseg, s);
+        for(int32_t g = 0; g < nseg; ++g)
+            st[g].rel_res_true = bb[g] > 0 ? std::sqrt(rr[g] / bb[g]) : 0.0;
     }
     MI_API_END
 }

@@ -259,6 +259,32 @@ struct PcgScalars
     int32_t conv_iter;
 };
 
+// ---- batched PCG: independent systems on consecutive block-row ranges
+// Tiles of at most kBatchTile DoFs (whole block rows) that never cross a
+// system boundary; every per-system sum runs over its tiles in order.
+constexpr int64_t kBatchTile = 3 * 1365;
+struct BatchTiles
+{
+    int32_t        n        = 0;        // tiles
+    const int64_t* lo       = nullptr;  // n + 1 DoF boundaries
+    const int32_t* sys      = nullptr;  // system of every tile
+    const int32_t* sys_tile = nullptr;  // nseg + 1: first tile of every system
+    const int64_t* sys_lo   = nullptr;  // nseg + 1: first DoF of every system
+};
+struct BatchScalars
+{
+    double  rz, rz_pending, rz_new, pAp, alpha, beta, rr, bb, rz_tol, rr_tol;
+    int64_t iter, conv_iter, max_iter;
+    int32_t criterion, converged, stopped, pad;
+};
+void batch_tile_dot(const BatchTiles& t, const double* a, const double* b, double* ab, double* aa, hipStream_t s);
+void batch_init(const BatchTiles& t, BatchScalars* sc, int32_t nseg, const double* rz, const double* rr,
+                int criterion, double tol, double max_iter_ratio, hipStream_t s);
+void batch_alpha_xr(const BatchTiles& t, BatchScalars* sc, int32_t nseg, const double* pap, double* x,
+                    const double* p, double* r, const double* Ap, hipStream_t s);
+void batch_beta_p(const BatchTiles& t, BatchScalars* sc, int32_t nseg, const double* rz, const double* rr,
+                  double* p, const double* z, float4* p32, hipStream_t s);
+
 // allocates the PCG reductions' partials (must run before any pointer into them is taken)
 void pcg_prepare(SpmvScratch& scr);
 void pcg_init_vectors(double* x, double* r, const double* b, int64_t n, hipStream_t s);

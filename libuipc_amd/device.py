@@ -276,6 +276,20 @@ class LinearSystem:
         check(self.lib.mi_pcg_solve(self.h, b.ptr, x.ptr, C.byref(o), C.byref(st), None))
         return st.as_dict()
 
+    def solve_batched(self, seg_rows, b: DeviceArray, x: DeviceArray, **kw) -> list[dict]:
+        """mi_pcg_solve_batched: independent systems on the block-row ranges
+        seg_rows[k] .. seg_rows[k + 1]; one stats dict per system"""
+        o = abi.PcgOpts()
+        check(self.lib.mi_pcg_default_opts(C.byref(o)))
+        for k, v in kw.items():
+            setattr(o, k, v)
+        seg = np.ascontiguousarray(seg_rows, dtype=np.int32)
+        nseg = seg.size - 1
+        st = (abi.PcgStats * max(nseg, 1))()
+        check(self.lib.mi_pcg_solve_batched(self.h, nseg, seg.ctypes.data_as(abi.c_i32p), b.ptr, x.ptr, C.byref(o),
+                                            st, None))
+        return [st[k].as_dict() for k in range(nseg)]
+
 
 def read_mtx_block_triplets(path: str, one_based: bool = True):
     """A coordinate Matrix Market file as 3x3 block triplets (duplicates summed,

@@ -246,3 +246,46 @@ def cloth_contacts(n: int, n_contacts: int, seed: int = 1, *, cells: int = 2, ka
     g = rng.standard_normal((m, 12))
     kappa = np.exp(rng.uniform(np.log(kappa_lo), np.log(kappa_hi), m))
     return st, g, kappa
+
+
+@dataclass
+class Batch:
+    """Independent FEM + contact systems on consecutive block rows (config 4's
+    recipe): per system a grid of N = round((DoF / 3)^(1/3)) vertices per side
+    for DoF log-uniform in [dof_lo, dof_hi], contacts at the 64^3 system's
+    density (2M per 262,144 vertices)."""
+    systems: list  # FemSystem per system
+    seg_rows: np.ndarray  # int32[nseg + 1]
+    rows: np.ndarray
+    cols: np.ndarray
+    vals: np.ndarray
+    rhs: np.ndarray
+    tets: np.ndarray  # global vertex ids
+    part_ids: np.ndarray  # global, never crossing a system
+
+    @property
+    def block_rows(self) -> int:
+        return int(self.seg_rows[-1])
+
+
+def batch_systems(n_systems: int, seed: int = 1, *, dof_lo: float = 3e3, dof_hi: float = 6e5) -> Batch:
+    rng = np.random.default_rng(seed)
+    dofs = np.exp(rng.uniform(np.log(dof_lo), np.log(dof_hi), n_systems))
+    grids = np.maximum(4, np.round((dofs / 3.0) ** (1.0 / 3.0)).astype(int))
+    density = 2_000_000 / 64 ** 3
+    systems, seg = [], [0]
+    for k, g in enumerate(grids):
+        s = fem_system(int(g), int(round(density * g ** 3)), seed=seed * 1000 + k)
+        systems.append(s)
+        seg.append(seg[-1] + s.block_rows)
+    seg = np.array(seg, np.int32)
+    rows = np.concatenate([s.rows + seg[k] for k, s in enumerate(systems)]).astype(np.int32)
+    cols = np.concatenate([s.cols + seg[k] for k, s in enumerate(systems)]).astype(np.int32)
+    vals = np.concatenate([s.vals for s in systems])
+    rhs = np.concatenate([s.rhs for s in systems])
+    tets = np.concatenate([s.tets + seg[k] for k, s in enumerate(systems)]).astype(np.int32)
+    parts, off = [], 0
+    for s in systems:
+        parts.append(np.where(s.part_ids >= 0, s.part_ids + off, -1))
+        off += int(s.part_ids.max()) + 1
+    return Batch(systems, seg, rows, cols, vals, rhs, tets, np.concatenate(parts).astype(np.int32))
