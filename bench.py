@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--precond", choices=["mas", "diag"], default="mas")
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
+    ap.add_argument("--mas-levels", type=int, default=0,
+                    help="MAS levels applied (MASPreconditionerEngine::set_active_level_num); 0 = all, the default")
     ap.add_argument("--tol", type=float, default=1e-4)
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -130,15 +132,9 @@ def main():
     x = ctx.alloc(s.n, np.float64)
     precond = args.precond
     if precond == "mas":
-        try:
-            ls.add_preconditioner(abi.MI_PRECOND_MAS, 0, s.n, part_ids=s.part_ids, tets=s.tets,
-                                  mas_precision=prec)
-        except abi.MiError as e:
-            if e.code != abi.MI_ERR_UNSUPPORTED:
-                raise
-            log("MAS unavailable in this build, using 3x3 block Jacobi")
-            precond = "diag"
-    if precond == "diag":
+        ls.add_preconditioner(abi.MI_PRECOND_MAS, 0, s.n, part_ids=s.part_ids, tets=s.tets, mas_precision=prec,
+                              active_levels=args.mas_levels)
+    else:
         ls.add_preconditioner(abi.MI_PRECOND_DIAG3, 0, s.n)
 
     solve_kw = dict(criterion=abi.MI_CRIT_REL_RESIDUAL, tol=args.tol, check_interval=5,
@@ -215,6 +211,7 @@ def main():
                         f"assemble + {precond} precond + PCG to rel. residual {args.tol:g}",
             "block_rows": s.block_rows, "dofs": s.n, "triplets": s.n_triplets, "unique_upper_blocks": U,
             "preconditioner": precond, "precision": args.precision, "criterion": "rel_residual",
+            "mas_levels": args.mas_levels if precond == "mas" else None,
         },
         "pcg_iters": int(st["iters"]),
         "pcg_iters_exact": int(st["iters_exact"]),
