@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--precond", choices=["mas", "diag"], default="mas")
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
+    ap.add_argument("--mas-invert", choices=["sweep", "mfma"], default="sweep",
+                    help="cluster inverse kernel: the column sweep (default) or the FP64 MFMA block sweep")
     ap.add_argument("--mas-levels", type=int, default=0,
                     help="MAS levels applied (MASPreconditionerEngine::set_active_level_num); 0 = all, the default")
     ap.add_argument("--tol", type=float, default=1e-4)
@@ -133,7 +135,8 @@ def main():
     precond = args.precond
     if precond == "mas":
         ls.add_preconditioner(abi.MI_PRECOND_MAS, 0, s.n, part_ids=s.part_ids, tets=s.tets, mas_precision=prec,
-                              active_levels=args.mas_levels)
+                              active_levels=args.mas_levels,
+                              invert=abi.MI_MAS_INVERT_SWEEP if args.mas_invert == "sweep" else abi.MI_MAS_INVERT_MFMA)
     else:
         ls.add_preconditioner(abi.MI_PRECOND_DIAG3, 0, s.n)
 
@@ -212,6 +215,7 @@ def main():
             "block_rows": s.block_rows, "dofs": s.n, "triplets": s.n_triplets, "unique_upper_blocks": U,
             "preconditioner": precond, "precision": args.precision, "criterion": "rel_residual",
             "mas_levels": args.mas_levels if precond == "mas" else None,
+            "mas_invert": args.mas_invert if precond == "mas" else None,
         },
         "pcg_iters": int(st["iters"]),
         "pcg_iters_exact": int(st["iters_exact"]),

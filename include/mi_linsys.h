@@ -176,12 +176,23 @@ typedef struct mi_precond_desc
     const int32_t* tets_dev;
     const int32_t* tris_dev;
     const int32_t* edges_dev;
+    /* How the 48x48 cluster matrices are inverted (invert_cluster_matrices,
+     * mas_preconditioner_engine.cu:2567-2589): both are FP64 Gauss-Jordan
+     * without pivoting and differ only in rounding.  0 is the default. */
+    int32_t        mas_invert;    /* mi_mas_invert                                         */
 } mi_precond_desc;
 
 enum mi_mas_build
 {
     MI_MAS_BUILD_HOST   = 0,
     MI_MAS_BUILD_DEVICE = 1
+};
+
+enum mi_mas_invert
+{
+    MI_MAS_INVERT_SWEEP = 0, /* the reference's column sweep, one pivot at a time      */
+    MI_MAS_INVERT_MFMA  = 1  /* 16x16 pivot tiles: Cholesky form of the sweep, tile
+                                products on FP64 matrix cores                        */
 };
 
 /* GlobalLinearSystem's local-preconditioner registration + LocalPreconditioner::do_init.

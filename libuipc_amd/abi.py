@@ -29,6 +29,8 @@ MI_PRECOND_MAS = 3
 
 MI_MAS_BUILD_HOST = 0
 MI_MAS_BUILD_DEVICE = 1
+MI_MAS_INVERT_SWEEP = 0
+MI_MAS_INVERT_MFMA = 1
 
 MI_CRIT_RZ_RATIO = 0
 MI_CRIT_REL_RESIDUAL = 1
@@ -62,6 +64,7 @@ class PrecondDesc(C.Structure):
         ("tets_dev", C.c_void_p),
         ("tris_dev", C.c_void_p),
         ("edges_dev", C.c_void_p),
+        ("mas_invert", C.c_int32),
     ]
 
 

@@ -592,6 +592,17 @@ __global__ __launch_bounds__(256) void mas_push_kernel(int64_t        c_first,
     }
 }
 
+// upper block index -> (row node a, column node b), a <= b (inverse of sym_index)
+__device__ __forceinline__ void sym_rc(int blk, int& a, int& b)
+{
+    int r = 0;
+#pragma unroll
+    for(int t = 1; t < kBank; ++t)
+        r += blk >= kBank * t - t * (t - 1) / 2 ? 1 : 0;
+    a = r;
+    b = blk - (kBank * r - r * (r - 1) / 2) + r;
+}
+
 // one wave per cluster: symmetric Gauss-Jordan sweep in FP64 (lane = column;
 // the sweep of mas_preconditioner_engine.cu:1077-1175, in double).  The column
 // is kept rotated so that the current pivot row is always register 0: after
@@ -666,6 +677,207 @@ __global__ __launch_bounds__(64) void mas_invert_kernel(const double* __restrict
     }
 }
 
+// The same inverse by a block sweep on FP64 matrix cores: one wave per
+// cluster, the 48x48 matrix resident in LDS as 3x3 tiles of 16x16.  Sweeping
+// pivot tile k of the symmetric matrix (A_ik = A_ki^T throughout):
+//   A_kk = L L^T (Cholesky),  W_j = L^{-1} A_kj          (j != k)
+//   A_ij <- A_ij - W_i^T W_j,  A_kj <- L^{-T} W_j,  A_kk <- -L^{-T} L^{-1}
+// which is the sweep A_ij -= A_ik A_kk^{-1} A_kj, A_kj <- A_kk^{-1} A_kj,
+// A_kk <- -A_kk^{-1} with the Schur update in its Cholesky form (the explicit
+// 16x16 inverse in A_ik A_kk^{-1} A_kj loses digits on stiff clusters).
+// After the three pivots the matrix holds -A^{-1}.  The Cholesky factor and
+// the triangular solves for L^{-1} and W run in registers, one column per
+// lane; the seven 16x16x16 tile products per pivot are v_mfma_f64_16x16x4
+// chains of four.  No pivoting, as in the reference's sweep: the cluster
+// matrices are SPD and a zero diagonal (a padding slot) is set to 1.
+//
+// v_mfma_f64_16x16x4 lane layout (lane l, g = l / 16, c = l % 16):
+//   A operand A[c][g], B operand B[g][c], accumulator v holds D[4 v + g][c]
+// (checked on gfx950 with distinct, unsymmetric operands).
+typedef double mfma_f64x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kInvLd = 3 * kBank + 1;  // LDS row stride: 16 rows of one column fall in distinct banks
+
+// acc + sign * op(tile (ti, tk)) * tile (tk', tj): op = transpose when `ta`
+// (the A operand is then read from tile (tk, ti) of S)
+template <bool ta>
+__device__ __forceinline__ mfma_f64x4 tile_mma(const double* S, int ti, int tk, int tj, mfma_f64x4 acc, int lane,
+                                               double sign)
+{
+    const int g = lane >> 4, c = lane & 15;
+#pragma unroll
+    for(int s = 0; s < 4; ++s)
+    {
+        const double a = sign * (ta ? S[(16 * tk + 4 * s + g) * kInvLd + 16 * ti + c]
+                                    : S[(16 * ti + c) * kInvLd + 16 * tk + 4 * s + g]);
+        const double b = S[(16 * tk + 4 * s + g) * kInvLd + 16 * tj + c];
+        acc            = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    }
+    return acc;
+}
+
+__device__ __forceinline__ mfma_f64x4 tile_read(const double* S, int ti, int tj, int lane)
+{
+    const int  g = lane >> 4, c = lane & 15;
+    mfma_f64x4 v;
+#pragma unroll
+    for(int e = 0; e < 4; ++e)
+        v[e] = S[(16 * ti + 4 * e + g) * kInvLd + 16 * tj + c];
+    return v;
+}
+
+// tile (ti, tj) <- v, and tile (tj, ti) <- v^T when `both`
+__device__ __forceinline__ void tile_write(double* S, int ti, int tj, const mfma_f64x4& v, int lane, bool both)
+{
+    const int g = lane >> 4, c = lane & 15;
+#pragma unroll
+    for(int e = 0; e < 4; ++e)
+    {
+        S[(16 * ti + 4 * e + g) * kInvLd + 16 * tj + c] = v[e];
+        if(both)
+            S[(16 * tj + c) * kInvLd + 16 * ti + 4 * e + g] = v[e];
+    }
+}
+
+// v_readlane of a double from lane l (l the same on every lane)
+__device__ __forceinline__ double readlane_f64(double v, int l)
+{
+    const unsigned long long u  = (unsigned long long)__double_as_longlong(v);
+    const unsigned int       lo = (unsigned int)__builtin_amdgcn_readlane((int)(unsigned int)u, l);
+    const unsigned int       hi = (unsigned int)__builtin_amdgcn_readlane((int)(unsigned int)(u >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+template <typename TO>
+__global__ __launch_bounds__(64) void mas_invert_mfma_kernel(const double* __restrict__ hess, TO* inv, int64_t C)
+{
+    constexpr int D = 3 * kBank;
+    __shared__ double S[D * kInvLd];
+    const int         lane = threadIdx.x;
+    const int64_t     k    = blockIdx.x;
+    const double*     M    = hess + k * kCm;
+    // the upper blocks into both triangles (hess: 9 column-major scalars per
+    // block); a diagonal block's lower half comes from its upper half.  All
+    // of a lane's loads are issued before the first is used.
+    constexpr int kLoads = (kCm + kWave - 1) / kWave;
+    double        hv[kLoads];
+#pragma unroll
+    for(int t = 0; t < kLoads; ++t)
+        hv[t] = lane + kWave * t < kCm ? M[lane + kWave * t] : 0.0;
+#pragma unroll
+    for(int t = 0; t < kLoads; ++t)
+    {
+        const int e = lane + kWave * t;
+        if(e >= kCm)
+            break;
+        const int blk = e / 9, q = e % 9;
+        int       a, b;
+        sym_rc(blk, a, b);
+        const int r = 3 * a + q % 3, cc = 3 * b + q / 3;
+        if(r > cc)
+            continue;
+        double v = hv[t];
+        if(r == cc && v == 0.0)
+            v = 1.0;
+        S[r * kInvLd + cc] = v;
+        S[cc * kInvLd + r] = v;
+    }
+    __syncthreads();
+    const mfma_f64x4 zero = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 1
+    for(int kt = 0; kt < 3; ++kt)
+    {
+        double* T = S + 16 * kt * kInvLd + 16 * kt;  // pivot tile, row stride kInvLd
+        // Cholesky A_kk = L L^T in registers: lane j (mod 16) holds column j;
+        // column p of the trailing matrix reaches every lane by v_readlane
+        // (lane p, a compile-time index), so no step waits on LDS
+        const int j = lane & 15;
+        double    c[16];
+#pragma unroll
+        for(int i = 0; i < 16; ++i)
+            c[i] = T[i * kInvLd + j];
+        double ild[16];  // 1 / L_pp, the same on every lane
+#pragma unroll
+        for(int p = 0; p < 16; ++p)
+        {
+            const double il  = rsqrt(readlane_f64(c[p], p));
+            const double ljp = c[p] * il;  // L_jp (the trailing matrix is symmetric)
+            ild[p]           = il;
+#pragma unroll
+            for(int i = p + 1; i < 16; ++i)
+            {
+                const double lip = readlane_f64(c[i], p) * il;
+                if(j > p)
+                    c[i] = fma(-lip, ljp, c[i]);
+            }
+            if(j == p)
+            {
+#pragma unroll
+                for(int i = p; i < 16; ++i)
+                    c[i] *= il;
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        // forward solves L Y = B, one column per lane: lanes 0-15 B = I (Y =
+        // L^{-1}), lanes 16-47 B = a column of A_kj (Y = W_j); L_mq = c[m] of lane q
+        const int i0 = kt == 0 ? 1 : 0, i1 = kt == 2 ? 1 : 2;  // the two other tiles
+        const int tj = lane < 16 ? kt : (lane < 32 ? i0 : i1);
+        double    y[16];
+#pragma unroll
+        for(int m = 0; m < 16; ++m)
+            y[m] = lane < 16 ? (m == j ? 1.0 : 0.0) : S[(16 * kt + m) * kInvLd + 16 * tj + j];
+#pragma unroll
+        for(int q = 0; q < 16; ++q)
+        {
+            y[q] *= ild[q];
+#pragma unroll
+            for(int m = q + 1; m < 16; ++m)
+                y[m] = fma(-readlane_f64(c[m], q), y[q], y[m]);
+            __builtin_amdgcn_sched_barrier(0);  // one column of L in scalar registers at a time
+        }
+        __syncthreads();
+        if(lane < 48)
+        {
+#pragma unroll
+            for(int m = 0; m < 16; ++m)
+                S[(16 * kt + m) * kInvLd + 16 * tj + j] = y[m];
+        }
+        __syncthreads();
+        // the row of tile kt now holds L^{-1} | W_i0 | W_i1
+        const mfma_f64x4 u00 = tile_mma<true>(S, i0, kt, i0, tile_read(S, i0, i0, lane), lane, -1.0);
+        const mfma_f64x4 u01 = tile_mma<true>(S, i0, kt, i1, tile_read(S, i0, i1, lane), lane, -1.0);
+        const mfma_f64x4 u11 = tile_mma<true>(S, i1, kt, i1, tile_read(S, i1, i1, lane), lane, -1.0);
+        const mfma_f64x4 w0  = tile_mma<true>(S, kt, kt, i0, zero, lane, 1.0);
+        const mfma_f64x4 w1  = tile_mma<true>(S, kt, kt, i1, zero, lane, 1.0);
+        const mfma_f64x4 dk  = tile_mma<true>(S, kt, kt, kt, zero, lane, -1.0);
+        __syncthreads();
+        tile_write(S, i0, i0, u00, lane, false);
+        tile_write(S, i0, i1, u01, lane, true);
+        tile_write(S, i1, i1, u11, lane, false);
+        tile_write(S, kt, i0, w0, lane, true);
+        tile_write(S, kt, i1, w1, lane, true);
+        tile_write(S, kt, kt, dk, lane, false);
+        __syncthreads();
+    }
+    // -S, upper blocks out as 9 planes of 136 (inv_at); a diagonal block takes
+    // its upper triangle for both halves
+    TO* O = inv + k * kCm;
+    for(int e = lane; e < kCm; e += kWave)
+    {
+        const int q = e / kSym, blk = e % kSym;
+        int       a, b;
+        sym_rc(blk, a, b);
+        int r = 3 * a + q % 3, cc = 3 * b + q / 3;
+        if(r > cc)
+        {
+            const int t = r;
+            r           = cc;
+            cc          = t;
+        }
+        O[e] = (TO)(-S[r * kInvLd + cc]);
+    }
+}
+
 __device__ __forceinline__ void inv3(const double m[9], double o[9])
 {
     double c00 = m[4] * m[8] - m[7] * m[5];
@@ -712,16 +924,6 @@ __global__ void mas_unpart_kernel(const int32_t* row_ptr,
 }
 
 // ------------------------------------------------------------------ apply kernels
-// upper block index -> (row node a, column node b), a <= b (inverse of sym_index)
-__device__ __forceinline__ void sym_rc(int blk, int& a, int& b)
-{
-    int r = 0;
-#pragma unroll
-    for(int t = 1; t < kBank; ++t)
-        r += blk >= kBank * t - t * (t - 1) / 2 ? 1 : 0;
-    a = r;
-    b = blk - (kBank * r - r * (r - 1) / 2) + r;
-}
 
 // Z = C^-1 R of one cluster on one wave (schwarz_local_solve, 1603-1700).
 // Each lane owns the upper blocks lane, lane + 64, lane + 128; block (a, b)
@@ -1045,7 +1247,7 @@ class MasPreconditioner final : public LocalPreconditioner
 {
   public:
     Hierarchy h;
-    int       fp32 = 0, active_levels = 0;
+    int       fp32 = 0, active_levels = 0, invert = MI_MAS_INVERT_SWEEP;
     int64_t   C = 0, C0 = 0;  // clusters, fine clusters
     // device tables
     DBuf<int32_t> d_part_to_real, d_real_to_part, d_anc, d_anc_slot, d_parent, d_sib, d_unpart;
@@ -1074,6 +1276,9 @@ class MasPreconditioner final : public LocalPreconditioner
         dof_count     = d.dof_count;
         fp32          = d.mas_precision == MI_FP32;
         active_levels = d.mas_active_levels;
+        invert        = d.mas_invert;
+        MI_REQUIRE(invert == MI_MAS_INVERT_MFMA || invert == MI_MAS_INVERT_SWEEP, MI_ERR_ARG,
+                   "MAS: unknown inverse kernel");
         h             = on_device ? mas_build_hierarchy_device(d) : build_hierarchy(d);
         const int L   = h.L;
         C             = h.total / kBank;
@@ -1235,10 +1440,17 @@ class MasPreconditioner final : public LocalPreconditioner
             if(cnt > 0)
                 mas_push_kernel<<<ceil_div(cnt, 4), 256, 0, s>>>(first, cnt, d_parent.data(), d_hess.data());
         }
-        if(fp32)
-            mas_invert_kernel<float><<<(unsigned)C, 64, 0, s>>>(d_hess.data(), d_inv32.data(), C);
+        if(invert == MI_MAS_INVERT_SWEEP)
+        {
+            if(fp32)
+                mas_invert_kernel<float><<<(unsigned)C, 64, 0, s>>>(d_hess.data(), d_inv32.data(), C);
+            else
+                mas_invert_kernel<double><<<(unsigned)C, 64, 0, s>>>(d_hess.data(), d_inv64.data(), C);
+        }
+        else if(fp32)
+            mas_invert_mfma_kernel<float><<<(unsigned)C, 64, 0, s>>>(d_hess.data(), d_inv32.data(), C);
         else
-            mas_invert_kernel<double><<<(unsigned)C, 64, 0, s>>>(d_hess.data(), d_inv64.data(), C);
+            mas_invert_mfma_kernel<double><<<(unsigned)C, 64, 0, s>>>(d_hess.data(), d_inv64.data(), C);
         if(h.has_unpart)
             mas_unpart_kernel<<<grid_for(h.V), 256, 0, s>>>(A.row_ptr.data(), A.cols.data(), A.vals.data(), A.stride,
                                                             blk_off, d_real_to_part.data(), h.V, d_Dun.data());
@@ -1248,8 +1460,10 @@ class MasPreconditioner final : public LocalPreconditioner
     template <typename T>
     void apply_t(const double* r, double* z, const int32_t* conv, hipStream_t s, const T* inv, T* R, T* Z)
     {
-        const int L = h.L;
-        for(int l = 0; l < base_lvl; ++l)
+        const int L    = h.L;
+        const int Lact = (active_levels > 0 && active_levels <= L) ? active_levels : L;
+        // one active level is the 16-node block Jacobi: no coarse level is read
+        for(int l = 0; Lact > 1 && l < base_lvl; ++l)
         {
             const int32_t cnt = span.count[l];
             if(cnt == 0)
@@ -1261,10 +1475,9 @@ class MasPreconditioner final : public LocalPreconditioner
                 mas_restrict_level_kernel<T, false><<<ceil_div(cnt, 256), 256, 0, s>>>(
                     r, d_part_to_real.data(), R, d_parent.data(), d_sib.data(), span.begin[l], cnt, conv);
         }
-        if(C > c_first)
+        if(Lact > 1 && C > c_first)
             mas_apply_coarse_kernel<T><<<(unsigned)(C - c_first), 64, 0, s>>>(
                 inv, R, Z, c_first, c_desc, d_desc_ptr.data(), d_desc_idx.data(), d_desc_lpn.data(), conv);
-        const int     Lact   = (active_levels > 0 && active_levels <= L) ? active_levels : L;
         const int64_t blocks = C0 + ceil_div(n_unpart, kWave);
         if(blocks > 0)
             mas_apply_fine_kernel<T><<<(unsigned)blocks, 64, 0, s>>>(r, z, d_part_to_real.data(), inv, Z,

@@ -202,11 +202,12 @@ class LinearSystem:
     def add_preconditioner(self, kind: int, dof_offset: int, dof_count: int, *, part_ids=None, tets=None,
                            tris=None, edges=None, mas_precision: int = abi.MI_FP64, active_levels: int = 0,
                            build: int = abi.MI_MAS_BUILD_HOST, device_arrays: dict | None = None,
-                           reinit: int | None = None) -> int:
+                           reinit: int | None = None, invert: int = abi.MI_MAS_INVERT_SWEEP) -> int:
         """`device_arrays` (MI_MAS_BUILD_DEVICE only) maps "part_ids", "tets",
         "tris", "edges" to device pointers read instead of the host arrays;
         their counts still come from the host arrays' shapes.  `reinit` =
-        a handle rebuilds that preconditioner in place (mi_precond_reinit)."""
+        a handle rebuilds that preconditioner in place (mi_precond_reinit).
+        `invert` picks the cluster inverse kernel (mi_mas_invert)."""
         keep = []
 
         def arr(a, k):
@@ -223,7 +224,7 @@ class LinearSystem:
         dev = {k: (v.value if isinstance(v, C.c_void_p) else v) for k, v in (device_arrays or {}).items()}
         d = abi.PrecondDesc(kind, dof_offset, dof_count, dof_count // 3, pp, tp, nt, trp, ntr, ep, ne,
                             mas_precision, active_levels, build, dev.get("part_ids"), dev.get("tets"),
-                            dev.get("tris"), dev.get("edges"))
+                            dev.get("tris"), dev.get("edges"), invert)
         if reinit is not None:
             check(self.lib.mi_precond_reinit(self.h, reinit, C.byref(d)))
             return reinit

@@ -11,13 +11,13 @@ from tests.test_oracle_mas import system
 pytestmark = pytest.mark.gpu
 
 
-def gpu_mas(ctx, s, precision=abi.MI_FP64, active_levels=0):
+def gpu_mas(ctx, s, precision=abi.MI_FP64, active_levels=0, invert=abi.MI_MAS_INVERT_SWEEP):
     ls = LinearSystem(ctx, s.block_rows)
     rows, cols, vals = ctx.to_device(s.rows), ctx.to_device(s.cols), ctx.to_device(s.vals)
     ls.assemble(rows, cols, vals)
     ls._keep = (rows, cols, vals)
     h = ls.add_preconditioner(abi.MI_PRECOND_MAS, 0, s.n, part_ids=s.part_ids, tets=s.tets,
-                              mas_precision=precision, active_levels=active_levels)
+                              mas_precision=precision, active_levels=active_levels, invert=invert)
     ls.assemble_preconditioners()
     return ls, h
 
@@ -58,6 +58,49 @@ def test_mas_apply(ctx, precision, tol, partition):
     want = m.apply(rv)
     assert rel_err(z, want) < tol
     assert rv @ z > 0
+
+
+def dense_clusters(blocks):
+    """(C, 136, 9) upper blocks (column-major 3x3) -> (C, 48, 48) symmetric"""
+    C = len(blocks)
+    out = np.zeros((C, 48, 48))
+    k = 0
+    for a in range(16):
+        for b in range(a, 16):
+            B = blocks[:, k].reshape(C, 3, 3).transpose(0, 2, 1)
+            out[:, 3 * a:3 * a + 3, 3 * b:3 * b + 3] = B
+            out[:, 3 * b:3 * b + 3, 3 * a:3 * a + 3] = B.transpose(0, 2, 1)
+            k += 1
+    return out
+
+
+@pytest.mark.parametrize("precision,tol", [(abi.MI_FP64, 1e-10), (abi.MI_FP32, 1e-6)])
+@pytest.mark.parametrize("partition", ["geometric", "random"])
+def test_mas_inverse_kernels(ctx, partition, precision, tol):
+    """The MFMA block Gauss-Jordan and the column sweep give the same inverse to
+    rounding, H C^-1 = I on every cluster (a padding slot's diagonal counts as
+    1), and both reach about the same MAS-PCG iteration count."""
+    s, _ = system(12, 3000, 15, partition)
+    out = {}
+    for inv in (abi.MI_MAS_INVERT_MFMA, abi.MI_MAS_INVERT_SWEEP):
+        ls, h = gpu_mas(ctx, s, precision, invert=inv)
+        g = ls.mas_export(h)
+        x = ctx.alloc(s.n, np.float64)
+        st = ls.solve(ctx.to_device(s.rhs), x, criterion=abi.MI_CRIT_REL_RESIDUAL, tol=1e-4)
+        out[inv] = (g, st)
+    (gm, sm), (gs, ss) = out[abi.MI_MAS_INVERT_MFMA], out[abi.MI_MAS_INVERT_SWEEP]
+    # the cluster sums use FP64 atomics: equal to rounding between two builds
+    assert np.allclose(gm["hess"], gs["hess"], rtol=1e-12, atol=1e-13 * np.abs(gs["hess"]).max())
+    assert np.abs(gm["inv"] - gs["inv"]).max() <= max(1e-8, tol) * np.abs(gs["inv"]).max()
+    H, Ci = dense_clusters(gm["hess"]), dense_clusters(gm["inv"])
+    d = np.arange(48)
+    pad = H[:, d, d] == 0
+    H[:, d, d] = np.where(pad, 1.0, H[:, d, d])
+    E = np.einsum("kij,kjl->kil", H, Ci) - np.eye(48)
+    scale = np.abs(H).max(axis=(1, 2)) * np.abs(Ci).max(axis=(1, 2))
+    assert (np.abs(E).max(axis=(1, 2)) <= tol * scale).all()
+    print(f"\nMAS-PCG to 1e-4: {sm['iters_exact']} iterations (MFMA inverse), {ss['iters_exact']} (sweep)", flush=True)
+    assert abs(sm["iters_exact"] - ss["iters_exact"]) <= max(2, ss["iters_exact"] // 20)
 
 
 def test_mas_one_level_block_jacobi(ctx):
