@@ -951,11 +951,21 @@ void run_iteration(mi_sys* sys, double* x, bool bypass, const LocalPreconditione
     else
     {
         pcg_update_xr(sc, partA, gA, x, sys->p.data(), sys->r.data(), sys->Ap.data(), n, s);
+        gB = -1;
         if(bypass)
             pcg_copy_if_active(sc, sys->z.data(), sys->r.data(), n, s);
+        else if(sys->pcs.size() == 1 && sys->pcs[0]->dof_offset == 0 && sys->pcs[0]->dof_count == n)
+        {
+            // one preconditioner over the whole vector: its last kernel writes the r.z partials
+            pcg_prepare(sys->dot_scr);
+            gB = sys->pcs[0]->apply_dot(sys->r.data(), sys->z.data(), &sc->converged, partB, sys->dot_scr.grid, s);
+            if(gB < 0)
+                sys->pcs[0]->apply(sys->r.data(), sys->z.data(), &sc->converged, s);
+        }
         else
             sys->apply_preconditioner(sys->r.data(), sys->z.data(), &sc->converged, s);
-        gB = pcg_dot_partials(sys->r.data(), sys->z.data(), n, partB, &sc->converged, sys->dot_scr, s);
+        if(gB < 0)
+            gB = pcg_dot_partials(sys->r.data(), sys->z.data(), n, partB, &sc->converged, sys->dot_scr, s);
     }
     pcg_update_p(sc, partB, gB, sys->p.data(), sys->z.data(), n, p32, s);
 }

@@ -225,6 +225,10 @@ class LocalPreconditioner
     virtual void apply(const double* r, double* z, const int32_t* d_converged, hipStream_t s) = 0;
     // 3x3 diagonal-block preconditioners can be fused into the PCG update
     virtual const double* diag3_inverse() const { return nullptr; }
+    // a preconditioner covering the whole vector may also write the PCG's
+    // r.z and r.r partials (pairs, at most max_parts); returns their count,
+    // or -1 when it does not (the caller then runs the dot kernel)
+    virtual int apply_dot(const double*, double*, const int32_t*, double*, int, hipStream_t) { return -1; }
     // MAS only: host copies of the hierarchy and cluster matrices
     virtual bool mas_export(int32_t*, int32_t*, int32_t*, int32_t*, int32_t*, int32_t*, double*, double*)
     {
