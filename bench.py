@@ -43,8 +43,9 @@ def parse():
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--mas-invert", choices=["sweep", "mfma"], default="sweep",
                     help="cluster inverse kernel: the column sweep (default) or the FP64 MFMA block sweep")
-    ap.add_argument("--mas-levels", type=int, default=0,
-                    help="MAS levels applied (MASPreconditionerEngine::set_active_level_num); 0 = all, the default")
+    ap.add_argument("--mas-levels", type=lambda v: v if v == "auto" else int(v), default=0,
+                    help="MAS levels applied (MASPreconditionerEngine::set_active_level_num): 0 = all, the "
+                         "default; auto = the fastest level count by libuipc_amd.tune, picked before timing")
     ap.add_argument("--tol", type=float, default=1e-4)
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -135,7 +136,7 @@ def main():
     precond = args.precond
     if precond == "mas":
         ls.add_preconditioner(abi.MI_PRECOND_MAS, 0, s.n, part_ids=s.part_ids, tets=s.tets, mas_precision=prec,
-                              active_levels=args.mas_levels,
+                              active_levels=0 if args.mas_levels == "auto" else args.mas_levels,
                               invert=abi.MI_MAS_INVERT_SWEEP if args.mas_invert == "sweep" else abi.MI_MAS_INVERT_MFMA)
     else:
         ls.add_preconditioner(abi.MI_PRECOND_DIAG3, 0, s.n)
@@ -151,6 +152,13 @@ def main():
     for w in range(args.warmup):
         st = step()
         log(f"warmup {w}: iters={st['iters']} exact={st['iters_exact']} rel_res={st['rel_res']:.3e}")
+    tuned = None
+    if precond == "mas" and args.mas_levels == "auto":
+        # untimed, like the warm-up: one solve per level count on this system
+        from libuipc_amd.tune import tune_mas_levels
+
+        tuned = tune_mas_levels(ls, 0, b, x, **solve_kw)
+        log(f"MAS levels tuned: {tuned['best']} of {tuned['table']}")
 
     ev = EventTimer(stream)
     ctx.synchronize()
@@ -175,8 +183,11 @@ def main():
     ls.assemble_preconditioners()
     ms_precond = ev.stop()
     ev.start()
-    st = ls.solve(b, x, **solve_kw, true_residual=1)
+    ls.solve(b, x, **solve_kw)
     ms_solve = ev.stop()
+    # the true residual of the same solve, outside the timing (asking for it
+    # changes the solve's options, which re-captures its graph)
+    st = ls.solve(b, x, **solve_kw, true_residual=1)
     U = ls.nnz()
 
     # dominant kernel: the SpMV (+ fused x^T A x), timed with HIP events on its
@@ -214,7 +225,8 @@ def main():
                         f"assemble + {precond} precond + PCG to rel. residual {args.tol:g}",
             "block_rows": s.block_rows, "dofs": s.n, "triplets": s.n_triplets, "unique_upper_blocks": U,
             "preconditioner": precond, "precision": args.precision, "criterion": "rel_residual",
-            "mas_levels": args.mas_levels if precond == "mas" else None,
+            "mas_levels": (tuned["best"] if tuned else args.mas_levels) if precond == "mas" else None,
+            "mas_levels_tuned": {str(k): v for k, v in tuned["table"].items()} if tuned else None,
             "mas_invert": args.mas_invert if precond == "mas" else None,
         },
         "pcg_iters": int(st["iters"]),

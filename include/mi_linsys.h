@@ -208,6 +208,14 @@ int mi_precond_add(mi_sys* sys, const mi_precond_desc* desc, int* handle_out);
  * mi_precond_assemble before the next apply or solve. */
 int mi_precond_reinit(mi_sys* sys, int handle, const mi_precond_desc* desc);
 
+/* MASPreconditionerEngine::set_active_level_num
+ * (finite_element/mas_preconditioner_engine.h:104-110): the number of MAS
+ * levels apply uses, from the finest; 1 is the 16-vertex block Jacobi, 0 or
+ * more than the hierarchy has means every level.  Takes effect at the next
+ * apply or solve (a captured solve is re-captured).  MI_ERR_ARG for a
+ * handle that is not a MAS preconditioner. */
+int mi_precond_mas_set_active_levels(mi_sys* sys, int handle, int32_t levels);
+
 /* LocalPreconditioner::do_assemble for every registered preconditioner
  * (global_linear_system.cu:499-515).  Requires a prior mi_sys_assemble. */
 int mi_precond_assemble(mi_sys* sys, void* stream);

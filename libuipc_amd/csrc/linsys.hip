@@ -553,6 +553,17 @@ int mi_precond_reinit(mi_sys* sys, int handle, const mi_precond_desc* d)
     MI_API_END
 }
 
+int mi_precond_mas_set_active_levels(mi_sys* sys, int handle, int32_t levels)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(sys && handle >= 0 && handle < (int)sys->pcs.size(), MI_ERR_ARG,
+               "mi_precond_mas_set_active_levels: bad handle");
+    MI_REQUIRE(sys->pcs[handle]->set_active_levels(levels), MI_ERR_ARG,
+               "mi_precond_mas_set_active_levels: not a MAS preconditioner");
+    ++sys->pcs_gen;  // a captured solve launches the old level count's kernels
+    MI_API_END
+}
+
 int mi_precond_assemble(mi_sys* sys, void* stream)
 {
     MI_API_BEGIN

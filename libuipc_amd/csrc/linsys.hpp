@@ -229,6 +229,8 @@ class LocalPreconditioner
     // r.z and r.r partials (pairs, at most max_parts); returns their count,
     // or -1 when it does not (the caller then runs the dot kernel)
     virtual int apply_dot(const double*, double*, const int32_t*, double*, int, hipStream_t) { return -1; }
+    // MAS only: levels used by apply (MASPreconditionerEngine::set_active_level_num)
+    virtual bool set_active_levels(int) { return false; }
     // MAS only: host copies of the hierarchy and cluster matrices
     virtual bool mas_export(int32_t*, int32_t*, int32_t*, int32_t*, int32_t*, int32_t*, double*, double*)
     {

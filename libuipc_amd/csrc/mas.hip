@@ -1654,6 +1654,12 @@ class MasPreconditioner final : public LocalPreconditioner
         return true;
     }
 
+    bool set_active_levels(int levels) override
+    {
+        active_levels = levels;  // 0 or out of range: every level, as the reference clamps
+        return true;
+    }
+
     void apply(const double* r, double* z, const int32_t* conv, hipStream_t s) override
     {
         if(h.V == 0)

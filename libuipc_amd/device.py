@@ -232,6 +232,10 @@ class LinearSystem:
         check(self.lib.mi_precond_add(self.h, C.byref(d), C.byref(h)))
         return h.value
 
+    # MASPreconditionerEngine::set_active_level_num
+    def mas_set_active_levels(self, handle: int, levels: int) -> None:
+        check(self.lib.mi_precond_mas_set_active_levels(self.h, handle, levels))
+
     def mas_export(self, handle: int, matrices: bool = True) -> dict:
         info = np.zeros(6, np.int32)
         check(self.lib.mi_precond_mas_export(self.h, handle, info.ctypes.data_as(abi.c_i32p), None, None, None, None,

@@ -4,6 +4,7 @@ import numpy as np
 import pytest
 
 from libuipc_amd import abi, synth
+from libuipc_amd.abi import MiError
 from libuipc_amd.device import LinearSystem, rel_err
 from oracle import oracle as O
 from tests.test_oracle_mas import system
@@ -114,22 +115,77 @@ def test_mas_one_level_block_jacobi(ctx):
     assert rel_err(zd.numpy(), m.apply(rv, active_levels=1)) < 1e-9
 
 
+@pytest.mark.parametrize("levels", [0, 1])
 @pytest.mark.parametrize("precision", [abi.MI_FP64, abi.MI_FP32])
-def test_mas_pcg(ctx, precision):
+def test_mas_pcg(ctx, precision, levels):
+    """MAS-PCG against the oracle's, all levels and one level; FP32 runs the
+    fine apply fused with the r.z partials"""
     s, (r, c, v) = system(16, 20000, 14)
-    ls, h = gpu_mas(ctx, s, precision)
+    ls, h = gpu_mas(ctx, s, precision, active_levels=levels)
     x = ctx.alloc(s.n, np.float64)
     st = ls.solve(ctx.to_device(s.rhs), x, criterion=abi.MI_CRIT_REL_RESIDUAL, tol=1e-4, true_residual=1)
     m = O.Mas(s.block_rows, s.part_ids, s.tets)
     m.assemble(r, c, v)
     xo, ro = O.pcg(s.block_rows, r, c, v, s.rhs, [dict(kind=3, blk_cnt=s.block_rows, mas=m,
-                                                        mas_fp32=precision == abi.MI_FP32)], criterion=1, tol=1e-4)
+                                                        mas_fp32=precision == abi.MI_FP32, active_levels=levels)],
+                   criterion=1, tol=1e-4)
     assert st["converged"] == 1 and ro["converged"] == 1
     assert st["rel_res"] <= 1e-4 and st["rel_res_true"] < 2e-4
     assert abs(st["iters_exact"] - ro["iters_exact"]) <= max(2, ro["iters_exact"] // 10)
     # vs a tight FP64 solve
     xt, rt = O.pcg(s.block_rows, r, c, v, s.rhs, [dict(kind=3, blk_cnt=s.block_rows, mas=m)], criterion=1, tol=1e-11)
     assert rel_err(x.numpy(), xt) < 1e-2
+
+
+@pytest.mark.parametrize("partition", ["random", "unpart"])
+def test_mas_pcg_fp32_one_level(ctx, partition):
+    """FP32, one level (the fine kernel fused with the r.z partials), with
+    disconnected parts and with un-partitioned vertices (the 3x3 fallback):
+    the solve matches the oracle's and the returned x solves the system"""
+    s, (r, c, v) = system(12, 3000, 16, partition)
+    ls, h = gpu_mas(ctx, s, abi.MI_FP32, active_levels=1)
+    x = ctx.alloc(s.n, np.float64)
+    st = ls.solve(ctx.to_device(s.rhs), x, criterion=abi.MI_CRIT_REL_RESIDUAL, tol=1e-4, true_residual=1)
+    m = O.Mas(s.block_rows, s.part_ids, s.tets)
+    m.assemble(r, c, v)
+    xo, ro = O.pcg(s.block_rows, r, c, v, s.rhs, [dict(kind=3, blk_cnt=s.block_rows, mas=m, mas_fp32=True,
+                                                        active_levels=1)], criterion=1, tol=1e-4)
+    assert st["converged"] == 1 and st["rel_res_true"] < 2e-4
+    assert abs(st["iters_exact"] - ro["iters_exact"]) <= max(2, ro["iters_exact"] // 10)
+
+
+def test_mas_set_active_levels_and_tuner(ctx):
+    """mi_precond_mas_set_active_levels changes the apply to the oracle's with
+    that many levels (0 = all, as the reference clamps), also for a solve
+    captured before the change; the tuner times every level count and leaves
+    the fastest set"""
+    from libuipc_amd.tune import tune_mas_levels
+
+    s, (r, c, v) = system(10, 500, 17, "geometric")
+    ls, h = gpu_mas(ctx, s)
+    m = O.Mas(s.block_rows, s.part_ids, s.tets)
+    m.assemble(r, c, v)
+    L = ls.mas_export(h, matrices=False)["L"]
+    rv = np.random.default_rng(6).standard_normal(s.n)
+    zd = ctx.alloc(s.n, np.float64)
+    b, x = ctx.to_device(s.rhs), ctx.alloc(s.n, np.float64)
+    iters = {}
+    for lv in (1, 2, 0, L + 3):
+        ls.mas_set_active_levels(h, lv)
+        ls.apply_preconditioner(ctx.to_device(rv), zd)
+        want = m.apply(rv, active_levels=lv if 0 < lv <= L else 0)
+        assert rel_err(zd.numpy(), want) < 1e-9, lv
+        iters[lv] = ls.solve(b, x, criterion=abi.MI_CRIT_REL_RESIDUAL, tol=1e-4)["iters_exact"]
+    assert iters[0] == iters[L + 3]
+    with pytest.raises(MiError):
+        d = LinearSystem(ctx, s.block_rows)
+        hd = d.add_preconditioner(abi.MI_PRECOND_DIAG3, 0, s.n)
+        d.mas_set_active_levels(hd, 1)
+    t = tune_mas_levels(ls, h, b, x, criterion=abi.MI_CRIT_REL_RESIDUAL, tol=1e-4)
+    assert sorted(t["table"]) == list(range(1, L + 1))
+    assert t["best"] == min(t["table"], key=lambda k: t["table"][k]["ms"])
+    ls.apply_preconditioner(ctx.to_device(rv), zd)
+    assert rel_err(zd.numpy(), m.apply(rv, active_levels=t["best"])) < 1e-9
 
 
 def test_mas_full_size_apply(ctx):
