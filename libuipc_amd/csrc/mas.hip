@@ -687,7 +687,7 @@ __global__ __launch_bounds__(64) void mas_invert_kernel(const double* __restrict
 // 16x16 inverse in A_ik A_kk^{-1} A_kj loses digits on stiff clusters).
 // After the three pivots the matrix holds -A^{-1}.  The Cholesky factor and
 // the triangular solves for L^{-1} and W run in registers, one column per
-// lane; the seven 16x16x16 tile products per pivot are v_mfma_f64_16x16x4
+// lane; the six 16x16x16 tile products per pivot are v_mfma_f64_16x16x4
 // chains of four.  No pivoting, as in the reference's sweep: the cluster
 // matrices are SPD and a zero diagonal (a padding slot) is set to 1.
 //
