@@ -95,6 +95,8 @@ __global__ __launch_bounds__(kThreads) void cv_count_kernel(int64_t        T,
 // ---- 2. stable compaction of (key, source index).  The tile is staged in
 // LDS with coalesced loads; thread t then owns the contiguous items
 // [t * kItems, (t + 1) * kItems) so that the block prefix follows index order.
+// The kept entries are gathered back into LDS in that order and written out
+// by consecutive lanes (one lane per item: coalesced stores).
 template <int MODE>
 __global__ __launch_bounds__(kThreads) void cv_compact_kernel(int64_t         T,
                                                              const int32_t*  rows,
@@ -104,28 +106,42 @@ __global__ __launch_bounds__(kThreads) void cv_compact_kernel(int64_t         T,
                                                              uint64_t*       keys,
                                                              int32_t*        srcs)
 {
-    __shared__ int32_t  sr[kTile], sc[kTile];
+    __shared__ int32_t  sr[kTile], sc[kTile], si[kTile];
     __shared__ uint32_t lds[kWaves + 1];
     const int64_t       base = (int64_t)blockIdx.x * kTile;
     stage_rc(rows, cols, T, base, sr, sc);
     const int first = threadIdx.x * kItems;
     uint32_t  flags = 0, c = 0;
+    int32_t   rr[kItems], cc[kItems];
 #pragma unroll
     for(int k = 0; k < kItems; ++k)
-        if(base + first + k < T && keep_entry<MODE>(sr[first + k], sc[first + k], B))
+    {
+        rr[k] = sr[first + k];
+        cc[k] = sc[first + k];
+        if(base + first + k < T && keep_entry<MODE>(rr[k], cc[k], B))
         {
             flags |= 1u << k;
             ++c;
         }
-    uint32_t o = block_exclusive_scan<kThreads>(c, lds, (uint32_t*)nullptr) + tile_off[blockIdx.x];
+    }
+    uint32_t kept = 0;
+    uint32_t o    = block_exclusive_scan<kThreads>(c, lds, &kept);  // its barriers end every read of sr / sc
 #pragma unroll
     for(int k = 0; k < kItems; ++k)
         if(flags & (1u << k))
         {
-            keys[o] = entry_key<MODE>(sr[first + k], sc[first + k], B);
-            srcs[o] = (int32_t)(base + first + k);
+            sr[o] = rr[k];
+            sc[o] = cc[k];
+            si[o] = first + k;
             ++o;
         }
+    __syncthreads();
+    const uint32_t g = tile_off[blockIdx.x];
+    for(uint32_t i = threadIdx.x; i < kept; i += kThreads)
+    {
+        keys[g + i] = entry_key<MODE>(sr[i], sc[i], B);
+        srcs[g + i] = (int32_t)(base + si[i]);
+    }
 }
 
 // ---- single-workgroup exclusive scan (n up to a few 100k), total -> *total
@@ -381,34 +397,47 @@ __global__ __launch_bounds__(kThreads) void rle_write_kernel(const int64_t*  d_m
                                                             int32_t*        seg_start)
 {
     __shared__ uint64_t sk[kTile + 1];
+    __shared__ int32_t  hp[kTile];
     __shared__ uint32_t lds[kWaves + 1];
     const int64_t       m    = *d_m;
     const int64_t       base = (int64_t)blockIdx.x * kTile;
     stage_keys(keys, m, base, sk);
     const int first = threadIdx.x * kItems;
     uint32_t  flags = 0, c = 0;
+    uint64_t  kk[kItems];
 #pragma unroll
     for(int k = 0; k < kItems; ++k)
     {
         int64_t i = base + first + k;
-        if(i < m && (i == 0 || sk[first + k + 1] != sk[first + k]))
+        kk[k]     = sk[first + k + 1];
+        if(i < m && (i == 0 || kk[k] != sk[first + k]))
         {
             flags |= 1u << k;
             ++c;
         }
     }
-    uint32_t o = block_exclusive_scan<kThreads>(c, lds, (uint32_t*)nullptr) + tile_off[blockIdx.x];
+    uint32_t heads = 0;
+    uint32_t o     = block_exclusive_scan<kThreads>(c, lds, &heads);  // its barriers end every read of sk
+    // the heads' keys (in place, in order) and positions, then written out by
+    // consecutive lanes
 #pragma unroll
     for(int k = 0; k < kItems; ++k)
         if(flags & (1u << k))
         {
-            uint64_t key = sk[first + k + 1];
-            int32_t  r   = (int32_t)(key / (uint64_t)B);
-            out_rows[o]  = r;
-            out_cols[o]  = (int32_t)(key - (uint64_t)r * (uint64_t)B);
-            seg_start[o] = (int32_t)(base + first + k);
+            sk[o] = kk[k];
+            hp[o] = first + k;
             ++o;
         }
+    __syncthreads();
+    const uint32_t g = tile_off[blockIdx.x];
+    for(uint32_t i = threadIdx.x; i < heads; i += kThreads)
+    {
+        const uint64_t key = sk[i];
+        const int32_t  r   = (int32_t)(key / (uint64_t)B);
+        out_rows[g + i]    = r;
+        out_cols[g + i]    = (int32_t)(key - (uint64_t)r * (uint64_t)B);
+        seg_start[g + i]   = (int32_t)(base + hp[i]);
+    }
 }
 
 // ---- segmented reduce: one thread per unique block, duplicates summed in
@@ -419,8 +448,7 @@ __global__ __launch_bounds__(kThreads) void seg_reduce_kernel(int64_t        U,
                                                              const int32_t* seg_start,
                                                              const int32_t* srcs,
                                                              const double*  vals,
-                                                             double*        out,
-                                                             int64_t        stride)
+                                                             double*        out)
 {
     const int64_t m = *d_m;
     for(int64_t u = (int64_t)blockIdx.x * kThreads + threadIdx.x; u < U; u += (int64_t)gridDim.x * kThreads)
@@ -459,7 +487,7 @@ __global__ __launch_bounds__(kThreads) void seg_reduce_kernel(int64_t        U,
         }
 #pragma unroll
         for(int q = 0; q < 9; ++q)
-            out[q * stride + u] = acc[q];
+            out[upper_at(u, q)] = acc[q];
     }
 }
 
@@ -553,7 +581,6 @@ __global__ void full_lower_kernel(const int64_t*  d_n,
                                   const int32_t*  lower_ptr,
                                   const int32_t*  full_ptr,
                                   const double*   uvals,
-                                  int64_t         ustride,
                                   int32_t         B,
                                   int32_t*        fcols,
                                   VT*             fvals,
@@ -572,7 +599,7 @@ __global__ void full_lower_kernel(const int64_t*  d_n,
         for(int r = 0; r < 3; ++r)
 #pragma unroll
             for(int c = 0; c < 3; ++c)
-                fvals[(r + 3 * c) * fstride + d] = (VT)uvals[(c + 3 * r) * ustride + u];
+                fvals[(r + 3 * c) * fstride + d] = (VT)uvals[upper_at(u, c + 3 * r)];
     }
 }
 
@@ -584,7 +611,6 @@ __global__ void full_upper_kernel(int64_t        U,
                                   const int32_t* lower_ptr,
                                   const int32_t* full_ptr,
                                   const double*  uvals,
-                                  int64_t        ustride,
                                   int32_t*       fcols,
                                   VT*            fvals,
                                   int64_t        fstride)
@@ -596,7 +622,7 @@ __global__ void full_upper_kernel(int64_t        U,
         fcols[d]        = cols[u];
 #pragma unroll
         for(int q = 0; q < 9; ++q)
-            fvals[q * fstride + d] = (VT)uvals[q * ustride + u];
+            fvals[q * fstride + d] = (VT)uvals[upper_at(u, q)];
     }
 }
 
@@ -751,8 +777,7 @@ void MatrixConverter::convert_sym(int32_t        B,
     out.U           = U;
     out.rows.resize((size_t)(U > 0 ? U : 1), ratio);
     out.cols.resize((size_t)(U > 0 ? U : 1), ratio);
-    out.stride = (int64_t)out.rows.cap;
-    out.vals.resize((size_t)9 * out.stride);
+    out.vals.resize((size_t)9 * out.rows.cap);
     seg_start_.resize((size_t)(U > 0 ? U : 1), ratio);
     rle_write_kernel<<<ntiles, kThreads, 0, s>>>(
         d_m, keys_[cur].data(), tile_cnt_.data(), B, out.rows.data(), out.cols.data(), seg_start_.data());
@@ -760,7 +785,7 @@ void MatrixConverter::convert_sym(int32_t        B,
     // 7-8. segmented reduce over the sorted permutation (1652-1698)
     if(U > 0)
         seg_reduce_kernel<<<grid_of(U), kThreads, 0, s>>>(
-            U, d_m, seg_start_.data(), srcs_[cur].data(), vals, out.vals.data(), out.stride);
+            U, d_m, seg_start_.data(), srcs_[cur].data(), vals, out.vals.data());
     row_ptr_kernel<<<grid_of(U + 1), kThreads, 0, s>>>(nullptr, U, out.rows.data(), out.row_ptr.data(), B);
 
     // full BSR: transpose of the off-diagonal upper blocks + the upper blocks
@@ -789,11 +814,11 @@ void MatrixConverter::convert_sym(int32_t        B,
         out.fvals32.resize((size_t)9 * out.fstride);
         full_lower_kernel<float><<<grid_of(U), kThreads, 0, s>>>(d_off, keys_[tcur].data(), srcs_[tcur].data(),
                                                                  lower_ptr_.data(), out.full_ptr.data(),
-                                                                 out.vals.data(), out.stride, B, out.fcols.data(),
+                                                                 out.vals.data(), B, out.fcols.data(),
                                                                  out.fvals32.data(), out.fstride);
         full_upper_kernel<float><<<grid_of(U), kThreads, 0, s>>>(U, out.rows.data(), out.cols.data(),
                                                                  out.row_ptr.data(), lower_ptr_.data(),
-                                                                 out.full_ptr.data(), out.vals.data(), out.stride,
+                                                                 out.full_ptr.data(), out.vals.data(),
                                                                  out.fcols.data(), out.fvals32.data(), out.fstride);
         full_empty_kernel<float><<<grid_of(B), kThreads, 0, s>>>(lower_ptr_.data(), out.row_ptr.data(),
                                                                  out.full_ptr.data(), B, out.fcols.data(),
@@ -804,11 +829,11 @@ void MatrixConverter::convert_sym(int32_t        B,
         out.fvals.resize((size_t)9 * out.fstride);
         full_lower_kernel<double><<<grid_of(U), kThreads, 0, s>>>(d_off, keys_[tcur].data(), srcs_[tcur].data(),
                                                                   lower_ptr_.data(), out.full_ptr.data(),
-                                                                  out.vals.data(), out.stride, B, out.fcols.data(),
+                                                                  out.vals.data(), B, out.fcols.data(),
                                                                   out.fvals.data(), out.fstride);
         full_upper_kernel<double><<<grid_of(U), kThreads, 0, s>>>(U, out.rows.data(), out.cols.data(),
                                                                   out.row_ptr.data(), lower_ptr_.data(),
-                                                                  out.full_ptr.data(), out.vals.data(), out.stride,
+                                                                  out.full_ptr.data(), out.vals.data(),
                                                                   out.fcols.data(), out.fvals.data(), out.fstride);
         full_empty_kernel<double><<<grid_of(B), kThreads, 0, s>>>(lower_ptr_.data(), out.row_ptr.data(),
                                                                   out.full_ptr.data(), B, out.fcols.data(),

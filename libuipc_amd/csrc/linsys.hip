@@ -382,19 +382,6 @@ int mi_sys_nnz(mi_sys* sys, int64_t* nnz)
     MI_API_END
 }
 
-namespace
-{
-__global__ void soa_to_aos_kernel(const double* planes, int64_t stride, int64_t U, double* out)
-{
-    for(int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < 9 * U; t += (int64_t)gridDim.x * blockDim.x)
-    {
-        int64_t u = t / 9;
-        int     q = (int)(t % 9);
-        out[t]    = planes[q * stride + u];
-    }
-}
-}  // namespace
-
 int mi_sys_export_bcoo(mi_sys* sys, int32_t* rows, int32_t* cols, double* vals, int32_t* row_ptr, void* stream)
 {
     MI_API_BEGIN
@@ -408,9 +395,8 @@ int mi_sys_export_bcoo(mi_sys* sys, int32_t* rows, int32_t* cols, double* vals, 
         MI_HIP(hipMemcpyAsync(cols, A.cols.data(), sizeof(int32_t) * A.U, hipMemcpyDeviceToDevice, s));
     if(row_ptr)
         MI_HIP(hipMemcpyAsync(row_ptr, A.row_ptr.data(), sizeof(int32_t) * (A.B + 1), hipMemcpyDeviceToDevice, s));
-    if(vals && A.U)
-        soa_to_aos_kernel<<<std::max(1, std::min(ceil_div(9 * A.U, 256), device_cu_count() * 8)), 256, 0, s>>>(
-            A.vals.data(), A.stride, A.U, vals);
+    if(vals && A.U)  // the upper values are stored per block, as exported
+        MI_HIP(hipMemcpyAsync(vals, A.vals.data(), sizeof(double) * 9 * A.U, hipMemcpyDeviceToDevice, s));
     MI_HIP(hipGetLastError());
     MI_API_END
 }
@@ -429,14 +415,9 @@ int mi_sys_write_mtx(mi_sys* sys, const char* path, int one_based)
     std::vector<double>  vals(9 * A.U);
     if(A.U)
     {
-        DBuf<double> aos;
-        aos.resize(9 * A.U);
-        soa_to_aos_kernel<<<std::max(1, std::min(ceil_div(9 * A.U, 256), device_cu_count() * 8)), 256, 0, s>>>(
-            A.vals.data(), A.stride, A.U, aos.data());
-        MI_HIP(hipGetLastError());
         MI_HIP(hipMemcpyAsync(rows.data(), A.rows.data(), sizeof(int32_t) * A.U, hipMemcpyDeviceToHost, s));
         MI_HIP(hipMemcpyAsync(cols.data(), A.cols.data(), sizeof(int32_t) * A.U, hipMemcpyDeviceToHost, s));
-        MI_HIP(hipMemcpyAsync(vals.data(), aos.data(), sizeof(double) * 9 * A.U, hipMemcpyDeviceToHost, s));
+        MI_HIP(hipMemcpyAsync(vals.data(), A.vals.data(), sizeof(double) * 9 * A.U, hipMemcpyDeviceToHost, s));
         MI_HIP(hipStreamSynchronize(s));
     }
     FILE* fp = std::fopen(path, "w");

@@ -17,16 +17,23 @@ namespace mi
 //          preconditioner assemblies.
 //   full   both triangles as BSR (full_ptr, fcols, values), FP64 or FP32:
 //          the stream the SpMV reads.
-// HBM layout: indices int32; values as 9 planes (plane q holds entry q of
-// every block, column-major entry order), so that the lanes of a wave read
-// contiguous bytes per plane.
+// HBM layout: indices int32.  Full-BSR values as 9 planes (plane q holds
+// entry q of every block, column-major entry order), so that the lanes of a
+// wave read contiguous bytes per plane; upper values per block (upper_at).
+// entry q (column-major, r + 3 c) of upper block u: the upper BCOO keeps each
+// block's 9 scalars together, as the reference's bcoo_A does (its consumers
+// gather single blocks: the full-BSR transpose, the preconditioner assemblies)
+__host__ __device__ __forceinline__ int64_t upper_at(int64_t u, int q)
+{
+    return 9 * u + q;
+}
+
 struct BsrMatrix
 {
     int32_t       B      = 0;  // block rows
     int64_t       U      = 0;  // unique upper blocks (host copy, valid after assemble)
-    int64_t       stride = 0;  // upper plane stride (>= U)
     DBuf<int32_t> rows, cols, row_ptr;
-    DBuf<double>  vals;  // 9 * stride
+    DBuf<double>  vals;  // 9 per block, blocks consecutive (upper_at)
     // full BSR
     int64_t       fstride = 0;  // full plane stride (>= 2U - diagonal blocks)
     DBuf<int32_t> full_ptr, fcols;

@@ -423,7 +423,6 @@ __global__ __launch_bounds__(256) void mas_walk_kernel(int64_t        U,
                                                       const int32_t* __restrict__ rows,
                                                       const int32_t* __restrict__ cols,
                                                       const double* __restrict__ vals,
-                                                      int64_t        stride,
                                                       int32_t        blk_off,
                                                       int32_t        V,
                                                       const int32_t* __restrict__ real_to_part,
@@ -451,7 +450,7 @@ __global__ __launch_bounds__(256) void mas_walk_kernel(int64_t        U,
     {
 #pragma unroll
         for(int q = 0; q < 9; ++q)
-            H[q] = vals[q * stride + u];
+            H[q] = vals[upper_at(u, q)];
         if(sr / kBank == sc / kBank)
         {
             // level 0: the only source of this cluster entry (scatter k1, 530-682)
@@ -899,7 +898,6 @@ __device__ __forceinline__ void inv3(const double m[9], double o[9])
 __global__ void mas_unpart_kernel(const int32_t* row_ptr,
                                   const int32_t* cols,
                                   const double*  vals,
-                                  int64_t        stride,
                                   int32_t        blk_off,
                                   const int32_t* real_to_part,
                                   int32_t        V,
@@ -915,7 +913,7 @@ __global__ void mas_unpart_kernel(const int32_t* row_ptr,
         {
             double m[9];
             for(int q = 0; q < 9; ++q)
-                m[q] = vals[q * stride + u];
+                m[q] = vals[upper_at(u, q)];
             inv3(m, o);
         }
         for(int q = 0; q < 9; ++q)
@@ -1533,7 +1531,6 @@ class MasPreconditioner final : public LocalPreconditioner
                                                                A.rows.data(),
                                                                A.cols.data(),
                                                                A.vals.data(),
-                                                               A.stride,
                                                                blk_off,
                                                                h.V,
                                                                d_real_to_part.data(),
@@ -1558,7 +1555,7 @@ class MasPreconditioner final : public LocalPreconditioner
         else
             mas_invert_mfma_kernel<double><<<(unsigned)C, 64, 0, s>>>(d_hess.data(), d_inv64.data(), C);
         if(h.has_unpart)
-            mas_unpart_kernel<<<grid_for(h.V), 256, 0, s>>>(A.row_ptr.data(), A.cols.data(), A.vals.data(), A.stride,
+            mas_unpart_kernel<<<grid_for(h.V), 256, 0, s>>>(A.row_ptr.data(), A.cols.data(), A.vals.data(),
                                                             blk_off, d_real_to_part.data(), h.V, d_Dun.data());
         MI_HIP(hipGetLastError());
     }

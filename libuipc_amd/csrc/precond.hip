@@ -39,7 +39,6 @@ __device__ __forceinline__ void inv3(const double m[9], double o[9])
 __global__ void diag3_assemble_kernel(const int32_t* row_ptr,
                                       const int32_t* cols,
                                       const double*  vals,
-                                      int64_t        stride,
                                       int64_t        blk_off,
                                       int64_t        blk_cnt,
                                       double*        D)
@@ -54,7 +53,7 @@ __global__ void diag3_assemble_kernel(const int32_t* row_ptr,
             double m[9];
 #pragma unroll
             for(int q = 0; q < 9; ++q)
-                m[q] = vals[q * stride + u];
+                m[q] = vals[upper_at(u, q)];
             inv3(m, o);
         }
         else
@@ -89,7 +88,6 @@ __global__ void diag3_apply_kernel(const double* D, const double* r, double* z, 
 __global__ void diag12_assemble_kernel(const int32_t* row_ptr,
                                        const int32_t* cols,
                                        const double*  vals,
-                                       int64_t        stride,
                                        int64_t        blk_off,
                                        int64_t        bodies,
                                        double*        Hinv)
@@ -113,7 +111,7 @@ __global__ void diag12_assemble_kernel(const int32_t* row_ptr,
                 for(int a = 0; a < 3; ++a)
                     for(int b = 0; b < 3; ++b)
                     {
-                        double v             = vals[(a + 3 * b) * stride + u];
+                        double v             = vals[upper_at(u, a + 3 * b)];
                         A[3 * p + a][3 * q + b] = v;
                         A[3 * q + b][3 * p + a] = v;
                     }
@@ -189,7 +187,7 @@ class Diag3Preconditioner final : public LocalPreconditioner
         D.resize((size_t)std::max<int64_t>(1, 9 * cnt));
         if(cnt > 0)
             diag3_assemble_kernel<<<grid_for(cnt), kThreads, 0, s>>>(
-                A.row_ptr.data(), A.cols.data(), A.vals.data(), A.stride, off, cnt, D.data());
+                A.row_ptr.data(), A.cols.data(), A.vals.data(), off, cnt, D.data());
         MI_HIP(hipGetLastError());
     }
     void apply(const double* r, double* z, const int32_t* conv, hipStream_t s) override
@@ -212,7 +210,7 @@ class Diag12Preconditioner final : public LocalPreconditioner
         H.resize((size_t)std::max<int64_t>(1, 144 * bodies));
         if(bodies > 0)
             diag12_assemble_kernel<<<grid_for(bodies), 64, 0, s>>>(
-                A.row_ptr.data(), A.cols.data(), A.vals.data(), A.stride, off, bodies, H.data());
+                A.row_ptr.data(), A.cols.data(), A.vals.data(), off, bodies, H.data());
         MI_HIP(hipGetLastError());
     }
     void apply(const double* r, double* z, const int32_t* conv, hipStream_t s) override
