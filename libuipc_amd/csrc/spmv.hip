@@ -10,15 +10,18 @@
 // the full BSR (both triangles, BsrMatrix::full_*; every row holds at least
 // one block, an empty row gets a zero diagonal block) and the SpMV is a row
 // gather over it, deterministic and free of atomics:
-//   * work comes in chunks of whole rows (the rows whose first block lies in
-//     [c S, (c+1) S), kChunkStep S), read with consecutive lanes on
-//     consecutive blocks, so every value plane streams as 64 contiguous
-//     entries per wave instruction (16 lanes per row of ~40 blocks reached
-//     ~60% of that rate);
-//   * a workgroup stages a chunk's products w_k = A_k x_col(k) in LDS (one
-//     block per lane and round: load, gather, product) and sums each row from
-//     there, 16 lanes per row in a fixed order; 18 KB of LDS and at most 64
-//     VGPRs keep 8 waves per SIMD resident to hide the load latency;
+//   * a workgroup takes a contiguous range of chunks of whole rows (chunk c =
+//     the rows whose first block lies in [c S, (c+1) S), kChunkStep S) and
+//     streams the range's blocks in stages of 768 (three rounds of one block
+//     per lane, every stage full but the range's last), with consecutive
+//     lanes on consecutive blocks, so every value plane streams as 64
+//     contiguous entries per wave instruction (16 lanes per row of ~40 blocks
+//     reached ~60% of that rate; stages cut at chunk ends left a third of the
+//     rounds' lanes idle);
+//   * a stage's products w_k = A_k x_col(k) go to LDS and each row is summed
+//     from there, 16 lanes per row in a fixed order; a row running past the
+//     stage leaves its partial sum to the next stage; 18 KB of LDS and at
+//     most 64 VGPRs keep 8 waves per SIMD resident to hide the load latency;
 //   * FP64 planes gather x as doubles; FP32 planes gather the float4 copy of
 //     x (the PCG keeps one next to p) and form the products in double;
 //   * x^T A x = sum_i x_i . (A x)_i per workgroup into partials[blockIdx.x];
@@ -39,12 +42,11 @@ constexpr int kWaves    = kThreads / kWave;
 constexpr int kXcds     = 8;
 constexpr int kLanes    = 16;  // lanes per block row in the chunk kernel's row sums
 constexpr int kGroups   = kThreads / kLanes;
-// blocks a chunk may hold to be staged in LDS: the chunk step plus the longest
-// row that may run past it; 18 KB of LDS per workgroup, so that LDS does not
-// limit the occupancy below 8 waves per SIMD
-constexpr int kChunkCap = kChunkStep + kChunkStep / 2;
-constexpr int kPer      = kChunkCap / kThreads;
-static_assert(kChunkCap % kThreads == 0, "chunk capacity must be a multiple of the workgroup");
+// blocks staged in LDS at a time: three rounds of one block per lane; 18 KB
+// of LDS per workgroup, so that LDS does not limit the occupancy below 8
+// waves per SIMD
+constexpr int kPer   = 3;
+constexpr int kStage = kPer * kThreads;
 
 template <bool ACCUM>
 __device__ __forceinline__ void put_row(double* y, int64_t row, double a, double v0, double v1, double v2)
@@ -64,11 +66,6 @@ __device__ __forceinline__ void put_row(double* y, int64_t row, double a, double
     }
 }
 
-// FP64 planes: a workgroup takes whole chunks (rows whose first block lies in
-// [c S, (c+1) S)), loads the chunk's blocks with consecutive lanes on
-// consecutive blocks, stages w_k = A_k x_col(k) in LDS and sums each row from
-// LDS with 16 lanes in a fixed order.  A chunk longer than kChunkCap blocks
-// (a row of more than S blocks) is done row by row straight from memory.
 // x_j as three doubles: FP64 planes read x, FP32 planes the float4 copy
 template <typename VT>
 __device__ __forceinline__ void gather_x(const double* __restrict__ x,
@@ -123,8 +120,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) v
 {
     if(d_conv && *d_conv != 0)
         return;
-    __shared__ double w[3][kChunkCap];
-    __shared__ double lds[kThreads / kWave];
+    __shared__ double  w[3][kStage];
+    __shared__ double  lds[kThreads / kWave];
+    __shared__ double  carry[2][3];  // partial sums of the row running past a stage, by stage parity
+    __shared__ int32_t nxt[2][2];    // first row of the next stage, and whether it carries a partial
 
     const int G   = gridDim.x;
     const int b   = blockIdx.x;
@@ -137,19 +136,26 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) v
     const int g    = tid / kLanes;
     const int lane = tid % kLanes;
     double    dot  = 0.0;
-    for(int c = c0; c < c1; ++c)
+    if(c0 < c1)
     {
-        const int32_t row0 = chunk_row[c], row1 = chunk_row[c + 1];
-        const int32_t k0 = fptr[row0], k1 = fptr[row1];
-        if(k1 - k0 <= kChunkCap)
+        // the workgroup's rows [r_first, R1) hold the blocks [s0, K1); they are
+        // streamed in stages of kStage blocks, every stage full but the last
+        const int32_t R1      = chunk_row[c1];
+        const int32_t K1      = fptr[R1];
+        int32_t       r_first = chunk_row[c0];
+        int32_t       s0      = fptr[r_first];
+        bool          carried = false;
+        int           par     = 0;
+        for(; s0 < K1; s0 += kStage, par ^= 1)
         {
-            // products of the chunk's blocks, consecutive lanes on consecutive
-            // blocks, one block per lane and round (load, gather, product)
+            const int32_t s1 = s0 + kStage < K1 ? s0 + kStage : K1;
+            // products, consecutive lanes on consecutive blocks, one block per
+            // lane and round (load, gather, product)
 #pragma unroll
             for(int h = 0; h < kPer; ++h)
             {
-                const int32_t  k  = k0 + h * kThreads + tid;
-                const bool     ok = k < k1;
+                const int32_t  k  = s0 + h * kThreads + tid;
+                const bool     ok = k < s1;
                 const uint32_t j  = ok ? (uint32_t)stream_load<VT>(&fcols[k]) : 0u;
                 VT             A[9];
 #pragma unroll
@@ -166,11 +172,28 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) v
                 }
             }
             __syncthreads();
-            for(int32_t row = row0 + g; row < row1; row += kGroups)
+            // the rows meeting the stage, 16 lanes each: a row that ends here is
+            // finished (plus the partial carried into the stage, for its first
+            // row); a row that runs past the stage leaves its partial for the
+            // next one.  Every row holds at least one block, so exactly one row
+            // either crosses s1 or starts at it: its group names the next stage's
+            // first row.
+            for(int32_t row = r_first + g; row < R1; row += kGroups)
             {
-                const int32_t s = fptr[row] - k0, e = fptr[row + 1] - k0;
+                const int32_t rs = fptr[row];
+                if(rs >= s1)
+                {
+                    if(rs == s1 && lane == 0)
+                    {
+                        nxt[par][0] = row;
+                        nxt[par][1] = 0;
+                    }
+                    break;
+                }
+                const int32_t re = fptr[row + 1];
+                const int32_t lo = (rs > s0 ? rs : s0) - s0, hi = (re < s1 ? re : s1) - s0;
                 double        acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
-                for(int32_t i = s + lane; i < e; i += kLanes)
+                for(int32_t i = lo + lane; i < hi; i += kLanes)
                 {
                     acc0 += w[0][i];
                     acc1 += w[1][i];
@@ -185,46 +208,33 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) v
                 }
                 if(lane == 0)
                 {
-                    put_row<ACCUM>(y, row, a, acc0, acc1, acc2);
-                    dot += x[3 * (int64_t)row] * acc0 + x[3 * (int64_t)row + 1] * acc1
-                           + x[3 * (int64_t)row + 2] * acc2;
+                    if(row == r_first && carried)
+                    {
+                        acc0 = carry[par ^ 1][0] + acc0;
+                        acc1 = carry[par ^ 1][1] + acc1;
+                        acc2 = carry[par ^ 1][2] + acc2;
+                    }
+                    if(re <= s1)
+                    {
+                        put_row<ACCUM>(y, row, a, acc0, acc1, acc2);
+                        dot += x[3 * (int64_t)row] * acc0 + x[3 * (int64_t)row + 1] * acc1
+                               + x[3 * (int64_t)row + 2] * acc2;
+                    }
+                    else
+                    {
+                        carry[par][0] = acc0;
+                        carry[par][1] = acc1;
+                        carry[par][2] = acc2;
+                        nxt[par][0]   = row;
+                        nxt[par][1]   = 1;
+                    }
                 }
             }
-            __syncthreads();  // w is rewritten by the next chunk
-        }
-        else
-        {
-            // a chunk with a very long row: 16 lanes per row straight from memory
-            for(int32_t row = row0 + g; row < row1; row += kGroups)
+            __syncthreads();  // w is rewritten by the next stage
+            if(s1 < K1)
             {
-                const int32_t s = fptr[row], e = fptr[row + 1];
-                double        acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
-                for(int32_t k = s + lane; k < e; k += kLanes)
-                {
-                    const uint32_t j = (uint32_t)fcols[k];
-                    double         A[9];
-#pragma unroll
-                    for(int q = 0; q < 9; ++q)
-                        A[q] = (double)fvals[q * fstride + k];
-                    double x0, x1, x2;
-                    gather_x<VT>(x, x32, j, x0, x1, x2);
-                    acc0 += A[0] * x0 + A[3] * x1 + A[6] * x2;
-                    acc1 += A[1] * x0 + A[4] * x1 + A[7] * x2;
-                    acc2 += A[2] * x0 + A[5] * x1 + A[8] * x2;
-                }
-#pragma unroll
-                for(int o = kLanes / 2; o > 0; o >>= 1)
-                {
-                    acc0 += __shfl_xor(acc0, o, kWave);
-                    acc1 += __shfl_xor(acc1, o, kWave);
-                    acc2 += __shfl_xor(acc2, o, kWave);
-                }
-                if(lane == 0)
-                {
-                    put_row<ACCUM>(y, row, a, acc0, acc1, acc2);
-                    dot += x[3 * (int64_t)row] * acc0 + x[3 * (int64_t)row + 1] * acc1
-                           + x[3 * (int64_t)row + 2] * acc2;
-                }
+                r_first = nxt[par][0];
+                carried = nxt[par][1] != 0;
             }
         }
     }
