@@ -30,6 +30,7 @@
 //     the 8 XCDs) takes a contiguous range of chunks, so x is re-read from its
 //     own L2.
 #include <cstdlib>
+#include <type_traits>
 
 #include "linsys.hpp"
 
@@ -45,8 +46,14 @@ constexpr int kGroups   = kThreads / kLanes;
 // blocks staged in LDS at a time: three rounds of one block per lane; 18 KB
 // of LDS per workgroup, so that LDS does not limit the occupancy below 8
 // waves per SIMD
-constexpr int kPer   = 3;
-constexpr int kStage = kPer * kThreads;
+template <typename VT>
+struct Stage
+{
+    // FP32 planes stage float products: twice the blocks in the same 18 KB
+    static constexpr int kPer   = sizeof(VT) == 4 ? 6 : 3;
+    static constexpr int kBlocks = kPer * kThreads;
+    using W                      = std::conditional_t<sizeof(VT) == 4, float, double>;
+};
 
 template <bool ACCUM>
 __device__ __forceinline__ void put_row(double* y, int64_t row, double a, double v0, double v1, double v2)
@@ -120,7 +127,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) v
 {
     if(d_conv && *d_conv != 0)
         return;
-    __shared__ double  w[3][kStage];
+    using W                  = typename Stage<VT>::W;
+    constexpr int      kPer   = Stage<VT>::kPer;
+    constexpr int      kStage = Stage<VT>::kBlocks;
+    __shared__ W       w[3][kStage];
     __shared__ double  lds[kThreads / kWave];
     __shared__ double  carry[2][3];  // partial sums of the row running past a stage, by stage parity
     __shared__ int32_t nxt[2][2];    // first row of the next stage, and whether it carries a partial
@@ -166,9 +176,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) v
                 if(ok)
                 {
                     const int i = h * kThreads + tid;
-                    w[0][i]     = (double)A[0] * x0 + (double)A[3] * x1 + (double)A[6] * x2;
-                    w[1][i]     = (double)A[1] * x0 + (double)A[4] * x1 + (double)A[7] * x2;
-                    w[2][i]     = (double)A[2] * x0 + (double)A[5] * x1 + (double)A[8] * x2;
+                    w[0][i]     = (W)((double)A[0] * x0 + (double)A[3] * x1 + (double)A[6] * x2);
+                    w[1][i]     = (W)((double)A[1] * x0 + (double)A[4] * x1 + (double)A[7] * x2);
+                    w[2][i]     = (W)((double)A[2] * x0 + (double)A[5] * x1 + (double)A[8] * x2);
                 }
             }
             __syncthreads();
@@ -195,9 +205,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) v
                 double        acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
                 for(int32_t i = lo + lane; i < hi; i += kLanes)
                 {
-                    acc0 += w[0][i];
-                    acc1 += w[1][i];
-                    acc2 += w[2][i];
+                    acc0 += (double)w[0][i];
+                    acc1 += (double)w[1][i];
+                    acc2 += (double)w[2][i];
                 }
 #pragma unroll
                 for(int o = kLanes / 2; o > 0; o >>= 1)
