@@ -56,6 +56,12 @@ def parse():
                          "the solve step) instead of uploading the host generator's")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch the PCG iterations directly instead of through captured HIP graphs")
+    ap.add_argument("--workload", choices=["single", "batched"], default="single",
+                    help="batched: many independent FEM + contact systems of log-uniform size solved together "
+                         "(mi_pcg_solve_batched), the triplets written on the device by the Hessian producers")
+    ap.add_argument("--systems", type=int, default=256)
+    ap.add_argument("--dof-lo", type=float, default=3e3)
+    ap.add_argument("--dof-hi", type=float, default=6e5)
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="only run N device-to-device copies of this many MiB (the known-size kernel that "
                          "scales rocprofv3 FETCH_SIZE / WRITE_SIZE) and print the copy bandwidth")
@@ -110,9 +116,13 @@ def main():
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(json.dumps({"error": "this build runs on one GPU only: the per-Newton solve of one system is not "
-                                   "sharded across GPUs (see DESIGN.md for how it would shard)",
+                                   "sharded across GPUs",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+
+    if args.workload == "batched":
+        run_batched(args)
+        return
 
     from libuipc_amd import abi, synth
     from libuipc_amd.device import Context, LinearSystem
@@ -291,6 +301,118 @@ def run_producers(ctx, s, args, rows, cols, vals) -> dict:
         raise RuntimeError("device producers disagree with the host generator")
     out["triplets_per_s"] = round((B + 10 * nt + 10 * nc) / (sum(out.values()) * 1e-3), 1)
     return out
+
+
+def run_batched(args) -> None:
+    """Independent FEM + contact systems with DoF counts log-uniform in
+    [dof_lo, dof_hi] (grid N = round((DoF / 3)^(1/3)), contacts at the density
+    of 2M per 64^3 grid), all in one block-diagonal system: the device
+    producers write every system's triplets at its block-row offset, one
+    assembly, one MAS over the union (disconnected systems never share a
+    cluster), one batched PCG with per-system scalars.  A step is assembly +
+    MAS build + the batched solve, timed with HIP events."""
+    from libuipc_amd import abi, synth
+    from libuipc_amd.abi import check
+    from libuipc_amd.device import Context, LinearSystem
+    from libuipc_amd.hip_events import EventTimer
+
+    rng = np.random.default_rng(args.seed)
+    dofs = np.exp(rng.uniform(np.log(args.dof_lo), np.log(args.dof_hi), args.systems))
+    grids = np.maximum(4, np.round((dofs / 3.0) ** (1.0 / 3.0)).astype(int))
+    density = 2_000_000 / 64 ** 3
+    contacts = np.round(density * grids.astype(np.float64) ** 3).astype(np.int64)
+    ctx = Context(0)
+    st = ctx.stream()
+    # sizes first (cheap), so the triplet buffer is allocated once
+    t0 = time.perf_counter()
+    inputs, seg, toff = [], [0], [0]
+    for k, g in enumerate(grids):
+        p = synth.producer_inputs(int(g), int(contacts[k]), seed=args.seed * 1000 + k)
+        B = len(p.mass)
+        inputs.append(p)
+        seg.append(seg[-1] + B)
+        toff.append(toff[-1] + B + 10 * len(p.tets) + 10 * len(p.contact_kappa))
+        if (k + 1) % 32 == 0:
+            log(f"generated {k + 1}/{args.systems} systems' producer inputs ({time.perf_counter() - t0:.1f}s)")
+    seg, toff = np.array(seg, np.int64), np.array(toff, np.int64)
+    Btot, T = int(seg[-1]), int(toff[-1])
+    log(f"{args.systems} systems: {3 * Btot} DoFs, {Btot} block rows, {T} triplets "
+        f"({T * 80 / 1e9:.1f} GB of triplets on the device)")
+    rows, cols, vals = ctx.alloc(T, np.int32), ctx.alloc(T, np.int32), ctx.alloc((T, 9), np.float64)
+    r0, c0, v0 = rows.ptr.value, cols.ptr.value, vals.ptr.value
+    tets_all = np.empty((sum(len(p.tets) for p in inputs), 4), np.int32)
+    parts_all = np.empty(Btot, np.int32)
+    t_off, p_off = 0, 0
+    for k, p in enumerate(inputs):
+        vo, o = int(seg[k]), int(toff[k])
+        B, nt, nc = len(p.mass), len(p.tets), len(p.contact_kappa)
+        dm, dt_, dX, dx = (ctx.to_device(a) for a in (p.mass, p.tets, p.rest_positions, p.positions))
+        check(ctx.lib.mi_produce_kinetic(ctx.h, B, dm.ptr, vo, r0 + 4 * o, c0 + 4 * o, v0 + 72 * o, st))
+        o += B
+        check(ctx.lib.mi_produce_snh(ctx.h, nt, dt_.ptr, dX.ptr, dx.ptr, p.mu, p.lam, p.dt, vo, r0 + 4 * o,
+                                     c0 + 4 * o, v0 + 72 * o, st))
+        o += 10 * nt
+        if nc:
+            ci, cg, ck = (ctx.to_device(a) for a in (p.contact_idx, p.contact_g, p.contact_kappa))
+            check(ctx.lib.mi_produce_contact(ctx.h, nc, ci.ptr, cg.ptr, ck.ptr, p.dt, vo, r0 + 4 * o,
+                                             c0 + 4 * o, v0 + 72 * o, st))
+        ctx.synchronize()  # the per-system inputs are freed next
+        tets_all[t_off:t_off + nt] = p.tets + vo
+        t_off += nt
+        parts_all[vo:vo + B] = np.where(p.part_ids >= 0, p.part_ids + p_off, -1)
+        p_off += int(p.part_ids.max()) + 1
+        inputs[k] = None
+    log(f"device producers done ({time.perf_counter() - t0:.1f}s)")
+    b = ctx.to_device(np.random.default_rng(args.seed + 7).uniform(-1.0, 1.0, 3 * Btot))
+    x = ctx.alloc(3 * Btot, np.float64)
+    ls = LinearSystem(ctx, Btot)
+    ls.add_preconditioner(abi.MI_PRECOND_MAS, 0, 3 * Btot, part_ids=parts_all, tets=tets_all,
+                          build=abi.MI_MAS_BUILD_DEVICE)
+    log(f"MAS hierarchy over the union built ({time.perf_counter() - t0:.1f}s)")
+    seg32 = seg.astype(np.int32)
+    solve_kw = dict(criterion=abi.MI_CRIT_REL_RESIDUAL, tol=args.tol, check_interval=5)
+
+    def step():
+        ls.assemble(rows, cols, vals, st)
+        ls.assemble_preconditioners()
+        return ls.solve_batched(seg32, b, x, **solve_kw)
+
+    for w in range(args.warmup):
+        stats = step()
+        log(f"warmup {w}: iterations min/median/max "
+            f"{min(s['iters_exact'] for s in stats)}/{int(np.median([s['iters_exact'] for s in stats]))}/"
+            f"{max(s['iters_exact'] for s in stats)}")
+    ev = EventTimer(st)
+    ctx.synchronize()
+    times = []
+    for k in range(args.steps):
+        ev.start()
+        stats = step()
+        times.append(ev.stop())
+        log(f"step {k}: {times[-1]:.1f} ms")
+    ms = float(np.median(times))
+    its = np.array([s["iters_exact"] for s in stats])
+    conv = int(sum(s["converged"] for s in stats))
+    ev.start()
+    ls.assemble(rows, cols, vals, st)
+    ms_asm = ev.stop()
+    ev.start()
+    ls.assemble_preconditioners()
+    ms_pc = ev.stop()
+    print(json.dumps({
+        "metric": "ms per batched Newton solve to 1e-4 rel. residual per system; PCG iters",
+        "value": round(ms, 3), "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms, 3), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (seeded generator, device Hessian producers)",
+        "config": {"workload": f"{args.systems} independent FEM + contact systems, DoFs log-uniform in "
+                               f"[{args.dof_lo:g}, {args.dof_hi:g}], one batched solve",
+                   "systems": args.systems, "dofs": 3 * Btot, "block_rows": Btot, "triplets": T,
+                   "grid_min": int(grids.min()), "grid_max": int(grids.max()), "preconditioner": "mas (union)"},
+        "pcg_iters_exact": {"min": int(its.min()), "median": int(np.median(its)), "max": int(its.max())},
+        "converged": conv, "phase_ms": {"assemble": round(ms_asm, 3), "precond_assemble": round(ms_pc, 3),
+                                        "pcg": round(ms - ms_asm - ms_pc, 3)},
+        "ms_per_system_mean": round(ms / args.systems, 4),
+    }))
 
 
 def cpu_baseline(s, precond, args, gpu_iters):

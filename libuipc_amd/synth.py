@@ -109,6 +109,7 @@ class ProducerInputs:
     mu: float
     lam: float
     dt: float
+    part_ids: np.ndarray | None = None  # int32[B], the MAS partition of fem_system
 
 
 def producer_inputs(grid: int, n_contacts: int = 0, seed: int = 1, **kw) -> ProducerInputs:
@@ -125,8 +126,9 @@ def producer_inputs(grid: int, n_contacts: int = 0, seed: int = 1, **kw) -> Prod
     ci = np.empty((max(n_contacts, 1), 4), np.int32)
     cg = np.empty((max(n_contacts, 1), 12))
     ck = np.empty(max(n_contacts, 1))
-    if lib.mi_synth_fem_fill(C.byref(d), None, None, None, None, tets.ctypes.data_as(abi.c_i32p), None,
-                             x.ctypes.data_as(abi.c_f64p)) != 0:
+    parts = np.empty(B, np.int32)
+    if lib.mi_synth_fem_fill(C.byref(d), None, None, None, None, tets.ctypes.data_as(abi.c_i32p),
+                             parts.ctypes.data_as(abi.c_i32p), x.ctypes.data_as(abi.c_f64p)) != 0:
         raise RuntimeError(lib.mi_synth_last_error().decode())
     if lib.mi_synth_fem_aux(C.byref(d), X.ctypes.data_as(abi.c_f64p), m.ctypes.data_as(abi.c_f64p),
                             ci.ctypes.data_as(abi.c_i32p), cg.ctypes.data_as(abi.c_f64p),
@@ -135,7 +137,7 @@ def producer_inputs(grid: int, n_contacts: int = 0, seed: int = 1, **kw) -> Prod
     E, nu = d.youngs, d.poisson
     mu = E / (2.0 * (1.0 + nu))
     lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu))
-    return ProducerInputs(X, x, m, tets, ci[:n_contacts], cg[:n_contacts], ck[:n_contacts], mu, lam, d.dt)
+    return ProducerInputs(X, x, m, tets, ci[:n_contacts], cg[:n_contacts], ck[:n_contacts], mu, lam, d.dt, parts)
 
 
 @dataclass
