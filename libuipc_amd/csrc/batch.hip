@@ -196,6 +196,50 @@ __global__ __launch_bounds__(kThreads) void batch_update_p_kernel(const BatchTil
     }
 }
 
+// the system holding `row`: the last g with seg[g] <= row
+__device__ __forceinline__ int32_t system_of(const int32_t* seg, int32_t nseg, int32_t row)
+{
+    int32_t lo = 0, hi = nseg - 1;
+    while(lo < hi)
+    {
+        const int32_t mid = (lo + hi + 1) / 2;
+        if(seg[mid] <= row)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// a SpMV chunk is still needed while one of the systems it holds rows of runs
+__device__ __forceinline__ bool chunk_needed(const int32_t* chunk_row, int32_t c, const int32_t* seg, int32_t nseg,
+                                             const BatchScalars* sc)
+{
+    const int32_t r0 = chunk_row[c], r1 = chunk_row[c + 1];
+    if(r1 <= r0)
+        return false;
+    const int32_t g1 = system_of(seg, nseg, r1 - 1);
+    for(int32_t g = system_of(seg, nseg, r0); g <= g1; ++g)
+        if(!sc[g].stopped)
+            return true;
+    return false;
+}
+
+__global__ void chunk_flag_kernel(const int32_t* chunk_row, int32_t nchunks, const int32_t* seg, int32_t nseg,
+                                  const BatchScalars* sc, uint32_t* flags)
+{
+    for(int32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nchunks; c += gridDim.x * blockDim.x)
+        flags[c] = chunk_needed(chunk_row, c, seg, nseg, sc) ? 1u : 0u;
+}
+
+__global__ void chunk_list_kernel(const int32_t* chunk_row, int32_t nchunks, const int32_t* seg, int32_t nseg,
+                                  const BatchScalars* sc, const uint32_t* offs, int32_t* list)
+{
+    for(int32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nchunks; c += gridDim.x * blockDim.x)
+        if(chunk_needed(chunk_row, c, seg, nseg, sc))
+            list[offs[c]] = c;
+}
+
 int tile_grid(const BatchTiles& t)
 {
     return std::max(1, std::min(t.n, device_cu_count() * 8));
@@ -205,6 +249,18 @@ int seg_grid(int32_t nseg)
     return std::max(1, ceil_div((int64_t)nseg * kWave, kThreads));
 }
 }  // namespace
+
+void batch_active_chunks(const int32_t* chunk_row, int32_t nchunks, const int32_t* seg, int32_t nseg,
+                         const BatchScalars* sc, DBuf<uint32_t>& flags, DBuf<uint32_t>& scan_chunk, int32_t* list,
+                         int64_t* d_count, hipStream_t s)
+{
+    flags.resize((size_t)std::max(1, nchunks));
+    const int g = std::max(1, std::min(ceil_div((int64_t)nchunks, kThreads), device_cu_count() * 4));
+    chunk_flag_kernel<<<g, kThreads, 0, s>>>(chunk_row, nchunks, seg, nseg, sc, flags.data());
+    exclusive_scan_u32(flags.data(), nchunks, scan_chunk, d_count, s);
+    chunk_list_kernel<<<g, kThreads, 0, s>>>(chunk_row, nchunks, seg, nseg, sc, flags.data(), list);
+    MI_HIP(hipGetLastError());
+}
 
 void batch_tile_dot(const BatchTiles& t, const double* a, const double* b, double* ab, double* aa, hipStream_t s)
 {

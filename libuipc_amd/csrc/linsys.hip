@@ -77,14 +77,14 @@ struct mi_sys
         int64_t     n = -1, U = -1;
         uint64_t    pcs_gen = ~0ull;
         int         ci = 0, fused = -1, bypass = -1, use32 = -1;
-        const void* bufs[10] = {};
+        const void* bufs[12] = {};
         hipStream_t stream  = nullptr;
         bool        operator==(const GraphKey& o) const
         {
             if(x != o.x || b != o.b || n != o.n || U != o.U || pcs_gen != o.pcs_gen || ci != o.ci || fused != o.fused
                || bypass != o.bypass || use32 != o.use32 || stream != o.stream)
                 return false;
-            for(int k = 0; k < 10; ++k)
+            for(int k = 0; k < 12; ++k)
                 if(bufs[k] != o.bufs[k])
                     return false;
             return true;
@@ -105,6 +105,10 @@ struct mi_sys
         size_t               h_cap = 0;
         DBuf<double>         part_a, part_b;
         std::vector<double>  h_part;
+        // the SpMV chunks still holding rows of a running system
+        DBuf<int32_t>        seg_dev, active;
+        DBuf<uint32_t>       act_flags, act_scan;
+        DBuf<int64_t>        n_active;
         hipGraphExec_t       graph_exec = nullptr;
         hipGraph_t           graph      = nullptr;
         GraphKey             key;
@@ -1159,8 +1163,9 @@ void run_batch_iteration(mi_sys* sys, double* x, hipStream_t s)
     const int32_t nseg  = (int32_t)bt.seg_rows.size() - 1;
     const bool    use32 = sys->desc.value_precision == MI_FP32;
     float4*       p32   = use32 ? sys->p32.data() : nullptr;
+    // rows of stopped systems are skipped: their Ap is never read again
     spmv_full(sys->A, use32, sys->p.data(), p32, sys->Ap.data(), sys->spmv_scr.partials.data(), nullptr,
-              sys->spmv_scr, s);
+              sys->spmv_scr, s, bt.active.data(), bt.n_active.data());
     batch_tile_dot(bt.tiles, sys->p.data(), sys->Ap.data(), bt.part_a.data(), nullptr, s);
     batch_alpha_xr(bt.tiles, bt.sc.data(), nseg, bt.part_a.data(), x, sys->p.data(), sys->r.data(), sys->Ap.data(),
                    s);
@@ -1256,6 +1261,7 @@ extern "C" int mi_pcg_solve_batched(mi_sys*            sys,
             MI_HIP(hipMemcpyAsync(d.data(), h.data(), sizeof(h[0]) * h.size(), hipMemcpyHostToDevice, s));
         };
         up(bt.tile_lo, lo);
+        up(bt.seg_dev, bt.seg_rows);
         up(bt.tile_sys, tsys);
         up(bt.sys_tile, stile);
         up(bt.sys_lo, slo);
@@ -1333,7 +1339,29 @@ extern "C" int mi_pcg_solve_batched(mi_sys*            sys,
     key.bufs[7] = bt.sc.data();
     key.bufs[8] = bt.part_a.data();
     key.bufs[9] = sys->p32.data();
-    key.stream  = s;
+    // the SpMV's chunk list, rebuilt (outside the graph) whenever a system stops
+    bt.active.resize((size_t)std::max(1, sys->A.nchunks));
+    bt.n_active.resize(1);
+    int32_t n_stopped = -1;
+    auto    refresh   = [&](int32_t stopped_now)
+    {
+        if(stopped_now == n_stopped)
+            return;
+        batch_active_chunks(sys->A.chunk_row.data(), sys->A.nchunks, bt.seg_dev.data(), nseg, bt.sc.data(),
+                            bt.act_flags, bt.act_scan, bt.active.data(), bt.n_active.data(), s);
+        n_stopped = stopped_now;
+    };
+    auto count_stopped = [&]
+    {
+        int32_t c = 0;
+        for(int32_t g = 0; g < nseg; ++g)
+            c += bt.h_sc[g].stopped ? 1 : 0;
+        return c;
+    };
+    refresh(count_stopped());
+    key.bufs[10] = bt.active.data();
+    key.bufs[11] = bt.n_active.data();
+    key.stream   = s;
     int64_t done = 0;
     while(!all && done < total)
     {
@@ -1377,9 +1405,10 @@ extern "C" int mi_pcg_solve_batched(mi_sys*            sys,
         MI_HIP(hipGetLastError());
         done += blk;
         read_batch(sys, nseg, s);
-        all = true;
-        for(int32_t g = 0; g < nseg; ++g)
-            all = all && bt.h_sc[g].stopped;
+        const int32_t stopped_now = count_stopped();
+        all                       = stopped_now == nseg;
+        if(!all)
+            refresh(stopped_now);
     }
     read_batch(sys, nseg, s);
     for(int32_t g = 0; g < nseg; ++g)

@@ -114,10 +114,12 @@ struct SpmvScratch
 void spmv_prepare(SpmvScratch& scr, bool use32);
 // y = A x (overwritten); partials[g] = per-workgroup x^T A x; returns the grid g
 // x32: the FP32 SpMV's padded float4 copy of x (FP32 systems; packed into the
-// scratch when null)
+// scratch when null).  With `active`, only the rows of the listed chunks
+// (ascending, *d_nactive of them) are computed; the other rows of y keep
+// their values.
 int spmv_full(const BsrMatrix& A, bool use32, const double* x, const float4* x32, double* y, double* partials,
-              const int32_t* d_conv,
-              SpmvScratch& scr, hipStream_t s);
+              const int32_t* d_conv, SpmvScratch& scr, hipStream_t s, const int32_t* active = nullptr,
+              const int64_t* d_nactive = nullptr);
 // y += a A x; *d_dot = x^T A x when d_dot != null
 void spmv_accumulate_dot(const BsrMatrix& A, bool use32, double a, const double* x, double* y, double* d_dot,
                          SpmvScratch& scr, hipStream_t s);
@@ -291,6 +293,11 @@ struct BatchScalars
     int32_t criterion, converged, stopped, pad;
 };
 void batch_tile_dot(const BatchTiles& t, const double* a, const double* b, double* ab, double* aa, hipStream_t s);
+// the SpMV chunks that hold rows of a running system, ascending, into list;
+// *d_count = their number (seg: the systems' first block rows, device)
+void batch_active_chunks(const int32_t* chunk_row, int32_t nchunks, const int32_t* seg, int32_t nseg,
+                         const BatchScalars* sc, DBuf<uint32_t>& flags, DBuf<uint32_t>& scan_chunk, int32_t* list,
+                         int64_t* d_count, hipStream_t s);
 void batch_init(const BatchTiles& t, BatchScalars* sc, int32_t nseg, const double* rz, const double* rr,
                 int criterion, double tol, double max_iter_ratio, hipStream_t s);
 void batch_alpha_xr(const BatchTiles& t, BatchScalars* sc, int32_t nseg, const double* pap, double* x,

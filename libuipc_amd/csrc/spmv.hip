@@ -123,7 +123,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) v
                                                              double*       y,
                                                              double        a,
                                                              double*       partials,
-                                                             const int32_t* d_conv)
+                                                             const int32_t* d_conv,
+                                                             const int32_t* __restrict__ active,
+                                                             const int64_t* d_nactive)
 {
     if(d_conv && *d_conv != 0)
         return;
@@ -139,15 +141,28 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) v
     const int b   = blockIdx.x;
     const int per = G / kXcds;
     const int lb  = (per > 0 && b < per * kXcds) ? (b % kXcds) * per + b / kXcds : b;
-    const int c0  = (int)((int64_t)nchunks * lb / G);
-    const int c1  = (int)((int64_t)nchunks * (lb + 1) / G);
+    // the chunks to do: all of them, or the listed ones (ascending); a
+    // workgroup streams each run of consecutive chunks of its share
+    const int64_t nitems = active ? *d_nactive : (int64_t)nchunks;
+    const int     a0     = (int)(nitems * lb / G);
+    const int     a1     = (int)(nitems * (lb + 1) / G);
 
     const int tid  = threadIdx.x;
     const int g    = tid / kLanes;
     const int lane = tid % kLanes;
     double    dot  = 0.0;
-    if(c0 < c1)
+    for(int it = a0; it < a1;)
     {
+        int c0 = it, c1 = a1;
+        if(active)
+        {
+            c0 = active[it];
+            c1 = c0 + 1;
+            for(++it; it < a1 && active[it] == c1; ++it)
+                ++c1;
+        }
+        else
+            it = a1;
         // the workgroup's rows [r_first, R1) hold the blocks [s0, K1); they are
         // streamed in stages of kStage blocks, every stage full but the last
         const int32_t R1      = chunk_row[c1];
@@ -277,11 +292,12 @@ __global__ __launch_bounds__(kThreads) void sum_partials_kernel(const double* pa
 // launches the SpMV; returns how many partials it wrote from partials[0].
 template <typename VT, bool ACCUM>
 int launch_spmv(const SpmvScratch&, int g, hipStream_t s, const BsrMatrix& A, const VT* vals, const double* x,
-                const float4* x32, double* y, double a, double* partials, const int32_t* d_conv)
+                const float4* x32, double* y, double a, double* partials, const int32_t* d_conv,
+                const int32_t* active = nullptr, const int64_t* d_nactive = nullptr)
 {
     spmv_chunk_kernel<VT, ACCUM><<<g, kThreads, 0, s>>>(A.nchunks, A.chunk_row.data(), A.full_ptr.data(),
                                                         A.fcols.data(), vals, A.fstride, x, x32, y, a, partials,
-                                                        d_conv);
+                                                        d_conv, active, d_nactive);
     return g;
 }
 
@@ -335,7 +351,8 @@ const float4* x32_of(const BsrMatrix& A, const double* x, const float4* x32, Spm
 }
 
 int spmv_full(const BsrMatrix& A, bool use32, const double* x, const float4* x32, double* y, double* partials,
-              const int32_t* d_conv, SpmvScratch& scr, hipStream_t s)
+              const int32_t* d_conv, SpmvScratch& scr, hipStream_t s, const int32_t* active,
+              const int64_t* d_nactive)
 {
     spmv_prepare(scr, use32);
     if(A.B == 0)
@@ -346,9 +363,9 @@ int spmv_full(const BsrMatrix& A, bool use32, const double* x, const float4* x32
     MI_REQUIRE(partials >= scr.partials.data() && partials + g <= scr.partials.data() + scr.partials.size(),
                MI_ERR_STATE, "spmv_full: partials outside the scratch");
     const int np = use32 ? launch_spmv<float, false>(scr, g, s, A, A.fvals32.data(), x, x32_of(A, x, x32, scr, s), y,
-                                                     1.0, partials, d_conv)
+                                                     1.0, partials, d_conv, active, d_nactive)
                          : launch_spmv<double, false>(scr, g, s, A, A.fvals.data(), x, nullptr, y, 1.0, partials,
-                                                      d_conv);
+                                                      d_conv, active, d_nactive);
     MI_HIP(hipGetLastError());
     return np;
 }
