@@ -350,6 +350,10 @@ typedef struct mi_pcg_opts
                                convergence every iteration and idles once converged, so the
                                reported counts do not depend on it; larger values cut host
                                round trips and cost at most that many idle intervals.      */
+    int32_t x_init;         /* 1: x holds the initial guess, r0 = b - A x0 (a Newton step's
+                               previous solution, a learned guess); 0: x0 = 0, as
+                               LinearFusedPCG (linear_fused_pcg.cu:1811-1824).  The relative
+                               residual stays relative to ||b||; the rz criterion to r0.z0. */
 } mi_pcg_opts;
 
 typedef struct mi_pcg_stats

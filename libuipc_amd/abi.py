@@ -79,6 +79,7 @@ class PcgOpts(C.Structure):
         ("force_bypass", C.c_int32),
         ("true_residual", C.c_int32),
         ("graph_checks", C.c_int32),
+        ("x_init", C.c_int32),
     ]
 
 

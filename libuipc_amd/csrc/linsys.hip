@@ -918,6 +918,7 @@ int mi_pcg_default_opts(mi_pcg_opts* o)
     o->force_bypass   = 0;
     o->true_residual  = 0;
     o->graph_checks   = 4;
+    o->x_init         = 0;
     MI_API_END
 }
 }  // extern "C"
@@ -1011,15 +1012,33 @@ extern "C" int mi_pcg_solve(mi_sys* sys, const double* b, double* x, const mi_pc
     pcg_prepare(sys->dot_scr);
     PcgScalars& h = *sys->h_sc;
 
-    // r = b, x = 0, z = P r, p = z, rz0 = r.z (linear_fused_pcg.cu:1811-1824)
-    pcg_init_vectors(x, sys->r.data(), b, n, s);
+    // r = b, x = 0, z = P r, p = z, rz0 = r.z (linear_fused_pcg.cu:1811-1824);
+    // with a given x0, r = b - A x0
+    const bool use32_sys = sys->desc.value_precision == MI_FP32;
+    double     bb_given  = -1.0;
+    if(o.x_init)
+    {
+        copy_f64(sys->r.data(), b, n, s);
+        spmv_axpby(sys->A, use32_sys, -1.0, x, 1.0, sys->r.data(), sys->spmv_scr, s);
+        const double bn = norm_f64_host(b, n, s);
+        bb_given        = bn * bn;
+    }
+    else
+        pcg_init_vectors(x, sys->r.data(), b, n, s);
+    // the relative-residual reference stays ||b|| when x0 is given
+    auto init_scalars = [&](int g)
+    {
+        pcg_init_scalars(sys->d_sc, sys->dot_scr.partials.data(), g, o.criterion, o.tol, s);
+        if(bb_given >= 0.0)
+            pcg_set_bb(sys->d_sc, bb_given, o.tol, s);
+    };
     bool bypass = o.force_bypass != 0;
     if(bypass)
         copy_f64(sys->z.data(), sys->r.data(), n, s);
     else
         sys->apply_preconditioner(sys->r.data(), sys->z.data(), nullptr, s);
     int g0 = pcg_dot_partials(sys->r.data(), sys->z.data(), n, sys->dot_scr.partials.data(), nullptr, sys->dot_scr, s);
-    pcg_init_scalars(sys->d_sc, sys->dot_scr.partials.data(), g0, o.criterion, o.tol, s);
+    init_scalars(g0);
     read_scalars(sys, s);
 
     // PSD guard (linear_fused_pcg.cu:1843-1875): rz0 < 0, or non-finite z with a
@@ -1032,7 +1051,7 @@ extern "C" int mi_pcg_solve(mi_sys* sys, const double* b, double* x, const mi_pc
             copy_f64(sys->z.data(), sys->r.data(), n, s);
             g0 = pcg_dot_partials(sys->r.data(), sys->z.data(), n, sys->dot_scr.partials.data(), nullptr,
                                   sys->dot_scr, s);
-            pcg_init_scalars(sys->d_sc, sys->dot_scr.partials.data(), g0, o.criterion, o.tol, s);
+            init_scalars(g0);
             read_scalars(sys, s);
         }
     }

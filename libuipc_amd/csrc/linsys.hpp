@@ -312,6 +312,8 @@ void pcg_init_vectors(double* x, double* r, const double* b, int64_t n, hipStrea
 int pcg_dot_partials(const double* r, const double* z, int64_t n, double* part, const int32_t* conv,
                      SpmvScratch& scr, hipStream_t s);
 // PcgScalars from the initial partials (rz0 = r.z, bb = r.r with r = b)
+// b^T b of a solve started from a given x0 (the relative-residual reference)
+void pcg_set_bb(PcgScalars* sc, double bb, double tol, hipStream_t s);
 void pcg_init_scalars(PcgScalars* sc, const double* part, int g, int criterion, double tol, hipStream_t s);
 // x += alpha p, r -= alpha Ap with alpha = rz / sum(pAp partials)
 void pcg_update_xr(PcgScalars* sc, const double* pap_part, int gp, double* x, const double* p, double* r,

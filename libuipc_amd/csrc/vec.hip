@@ -469,6 +469,17 @@ void pcg_init_scalars(PcgScalars* sc, const double* part, int g, int criterion, 
     pcg_init_scalars_kernel<<<1, kThreads, 0, s>>>(sc, part, g, criterion, tol);
 }
 
+__global__ void pcg_set_bb_kernel(PcgScalars* sc, double bb, double tol)
+{
+    sc->bb     = bb;
+    sc->rr_tol = tol * tol * bb;
+}
+
+void pcg_set_bb(PcgScalars* sc, double bb, double tol, hipStream_t s)
+{
+    pcg_set_bb_kernel<<<1, 1, 0, s>>>(sc, bb, tol);
+}
+
 void pcg_update_xr(PcgScalars* sc, const double* pap_part, int gp, double* x, const double* p, double* r,
                    const double* Ap, int64_t n, hipStream_t s)
 {

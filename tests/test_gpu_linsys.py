@@ -274,6 +274,42 @@ def test_pcg_diag3(ctx, c1_contact, criterion, tol, use_graph):
     assert rel_err(x.numpy(), xo) < max(1e-6, 100 * tol)
 
 
+@pytest.mark.parametrize("precond", ["diag3", "mas"])
+def test_pcg_initial_guess(ctx, c1_contact, precond):
+    """x_init = 1: x0 = 0 given explicitly reproduces the default solve bit
+    for bit; x0 = the solution converges at once; x0 = the solution plus a
+    perturbation needs fewer iterations than x0 = 0 and still meets
+    ||b - A x|| <= tol ||b||"""
+    s = c1_contact
+    ls = assembled(ctx, s)
+    if precond == "diag3":
+        ls.add_preconditioner(abi.MI_PRECOND_DIAG3, 0, s.n)
+    else:
+        ls.add_preconditioner(abi.MI_PRECOND_MAS, 0, s.n, part_ids=s.part_ids, tets=s.tets)
+    ls.assemble_preconditioners()
+    b = ctx.to_device(s.rhs)
+    kw = dict(criterion=abi.MI_CRIT_REL_RESIDUAL, tol=1e-6, true_residual=1)
+    x = ctx.alloc(s.n, np.float64)
+    cold = ls.solve(b, x, **kw)
+    x_cold = x.numpy()
+    x.zero()
+    st = ls.solve(b, x, x_init=1, **kw)
+    assert st["iters_exact"] == cold["iters_exact"] and np.array_equal(x.numpy(), x_cold)
+    # the solution of a tighter solve as x0
+    xt = ctx.alloc(s.n, np.float64)
+    ls.solve(b, xt, criterion=abi.MI_CRIT_REL_RESIDUAL, tol=1e-10)
+    x_star = xt.numpy()
+    x.upload(x_star)
+    st = ls.solve(b, x, x_init=1, **kw)
+    assert st["converged"] == 1 and st["iters_exact"] <= 1 and st["rel_res_true"] <= 1e-6
+    rng = np.random.default_rng(9)
+    x.upload(x_star + 1e-3 * np.abs(x_star).max() * rng.standard_normal(s.n))
+    st = ls.solve(b, x, x_init=1, **kw)
+    assert st["converged"] == 1 and st["rel_res_true"] <= 2e-6
+    assert st["iters_exact"] < cold["iters_exact"]
+    assert rel_err(x.numpy(), x_star) < 1e-4
+
+
 def test_pcg_bypass_is_plain_cg(ctx, c1):
     ls = assembled(ctx, c1)
     ls.add_preconditioner(abi.MI_PRECOND_DIAG3, 0, c1.n)
