@@ -18,7 +18,7 @@ CSRC = os.path.join(PKG, "csrc")
 OBJ = os.path.join(PKG, "_build")
 ROOT = os.path.dirname(PKG)
 
-HIP_SOURCES = ["convert.hip", "spmv.hip", "vec.hip", "batch.hip", "precond.hip", "mas.hip", "mas_init.hip", "partition.hip", "subsystem.hip", "producers.hip", "producers_abd.hip", "producers_shell.hip", "linsys.hip"]
+HIP_SOURCES = ["convert.hip", "spmv.hip", "vec.hip", "batch.hip", "precond.hip", "mas.hip", "mas_init.hip", "partition.hip", "subsystem.hip", "producers.hip", "producers_abd.hip", "producers_shell.hip", "warmstart.hip", "linsys.hip"]
 HOST_SOURCES = ["mtx.cpp"]  # linked into libmi_linsys.so, no device code
 ARCH = "gfx950"
 

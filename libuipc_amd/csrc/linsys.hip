@@ -732,6 +732,63 @@ int mi_produce_bw_shell(mi_ctx*        ctx,
     MI_API_END
 }
 
+// ---- learned initial guess (include/mi_warmstart.h)
+int mi_ws_gemm(mi_ctx* ctx, int32_t M, int32_t N, int32_t K, const float* A, int64_t sa_m, int64_t sa_k,
+               const float* B, int64_t sb_k, int64_t sb_n, float* C, int64_t ldc, int32_t accumulate,
+               const float* bias, int32_t relu, void* stream)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(ctx && M >= 0 && N >= 0 && K >= 0 && (M == 0 || N == 0 || (A && B && C)), MI_ERR_ARG,
+               "mi_ws_gemm: bad argument");
+    ws_gemm(M, N, K, A, sa_m, sa_k, B, sb_k, sb_n, C, ldc, accumulate != 0, bias, relu != 0, pick(ctx, stream));
+    MI_API_END
+}
+
+int mi_ws_aggregate(mi_ctx* ctx, int32_t n, int32_t F, const int32_t* ptr, const int32_t* idx, const float* w,
+                    const float* H, float* Z, void* stream)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(ctx && n >= 0 && F > 0 && F <= 64, MI_ERR_ARG, "mi_ws_aggregate: 1 <= F <= 64");
+    MI_REQUIRE(n == 0 || (ptr && idx && w && H && Z), MI_ERR_ARG, "mi_ws_aggregate: null buffer");
+    ws_aggregate(n, F, ptr, idx, w, H, Z, pick(ctx, stream));
+    MI_API_END
+}
+
+int mi_ws_relu_backward(mi_ctx* ctx, int64_t n, const float* pre, float* d, void* stream)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(ctx && n >= 0 && (n == 0 || (pre && d)), MI_ERR_ARG, "mi_ws_relu_backward: bad argument");
+    ws_relu_backward(n, pre, d, pick(ctx, stream));
+    MI_API_END
+}
+
+int mi_ws_colsum(mi_ctx* ctx, int32_t n, int32_t F, const float* X, float* out, int32_t accumulate, void* stream)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(ctx && n >= 0 && F > 0 && X && out, MI_ERR_ARG, "mi_ws_colsum: bad argument");
+    ws_colsum(n, F, X, out, accumulate != 0, pick(ctx, stream));
+    MI_API_END
+}
+
+int mi_ws_adam(mi_ctx* ctx, int64_t n, float* w, const float* g, float* m, float* v, float lr, float beta1,
+               float beta2, float eps, int32_t step, void* stream)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(ctx && n >= 0 && step >= 1 && (n == 0 || (w && g && m && v)), MI_ERR_ARG, "mi_ws_adam: bad argument");
+    ws_adam(n, w, g, m, v, lr, beta1, beta2, eps, step, pick(ctx, stream));
+    MI_API_END
+}
+
+int mi_ws_block3(mi_ctx* ctx, int32_t n, const double* blocks, double scale, const float* v_f32, double* y_f64,
+                 const double* base, const double* v_f64, float* y_f32, void* stream)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(ctx && n >= 0 && blocks && ((v_f32 && y_f64 && !v_f64 && !y_f32) || (v_f64 && y_f32 && !v_f32 && !y_f64)),
+               MI_ERR_ARG, "mi_ws_block3: give (v_f32, y_f64) or (v_f64, y_f32)");
+    ws_block3(n, blocks, scale, v_f32, y_f64, base, v_f64, y_f32, pick(ctx, stream));
+    MI_API_END
+}
+
 int mi_produce_bending(mi_ctx*        ctx,
                        int64_t        n_hinges,
                        const int32_t* hinges,

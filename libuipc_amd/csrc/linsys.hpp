@@ -278,6 +278,19 @@ struct PcgScalars
 // Tiles of at most kBatchTile DoFs (whole block rows) that never cross a
 // system boundary; every per-system sum runs over its tiles in order.
 constexpr int64_t kBatchTile = 3 * 1365;
+// ----------------------------------------------------------------- learned initial guess
+void ws_gemm(int32_t M, int32_t N, int32_t K, const float* A, int64_t sa_m, int64_t sa_k, const float* B,
+             int64_t sb_k, int64_t sb_n, float* C, int64_t ldc, bool accumulate, const float* bias, bool relu,
+             hipStream_t s);
+void ws_aggregate(int32_t n, int32_t F, const int32_t* ptr, const int32_t* idx, const float* w, const float* H,
+                  float* Z, hipStream_t s);
+void ws_relu_backward(int64_t n, const float* pre, float* d, hipStream_t s);
+void ws_colsum(int32_t n, int32_t F, const float* X, float* out, bool accumulate, hipStream_t s);
+void ws_adam(int64_t n, float* w, const float* g, float* m, float* v, float lr, float beta1, float beta2, float eps,
+             int32_t step, hipStream_t s);
+void ws_block3(int32_t n, const double* blocks, double scale, const float* v_f32, double* y_f64, const double* base,
+               const double* v_f64, float* y_f32, hipStream_t s);
+
 struct BatchTiles
 {
     int32_t        n        = 0;        // tiles

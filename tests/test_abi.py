@@ -18,7 +18,7 @@ def declared(header):
     return sorted(set(re.findall(r"\b(mi_[a-z0-9_]+)\s*\(", txt)))
 
 
-@pytest.mark.parametrize("header", ["mi_linsys.h", "mi_producers.h"])
+@pytest.mark.parametrize("header", ["mi_linsys.h", "mi_producers.h", "mi_warmstart.h"])
 def test_linsys_exports_every_declared_symbol(header):
     lib = abi.linsys()
     names = declared(header)
