@@ -56,9 +56,16 @@ def parse():
                          "the solve step) instead of uploading the host generator's")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch the PCG iterations directly instead of through captured HIP graphs")
-    ap.add_argument("--workload", choices=["single", "batched"], default="single",
+    ap.add_argument("--workload", choices=["single", "batched", "learned"], default="single",
                     help="batched: many independent FEM + contact systems of log-uniform size solved together "
-                         "(mi_pcg_solve_batched), the triplets written on the device by the Hessian producers")
+                         "(mi_pcg_solve_batched), the triplets written on the device by the Hessian producers; "
+                         "learned: train the learned initial guess on 16^3 systems and compare MAS-PCG from it "
+                         "with MAS-PCG from x0 = 0 on held-out systems")
+    ap.add_argument("--train-systems", type=int, default=5000)
+    ap.add_argument("--test-systems", type=int, default=500)
+    ap.add_argument("--ws-loss", choices=["energy", "residual"], default="energy",
+                    help="training loss of the learned initial guess")
+    ap.add_argument("--ws-lr", type=float, default=1e-3)
     ap.add_argument("--systems", type=int, default=256)
     ap.add_argument("--dof-lo", type=float, default=3e3)
     ap.add_argument("--dof-hi", type=float, default=6e5)
@@ -122,6 +129,9 @@ def main():
 
     if args.workload == "batched":
         run_batched(args)
+        return
+    if args.workload == "learned":
+        run_learned(args)
         return
 
     from libuipc_amd import abi, synth
@@ -412,6 +422,111 @@ def run_batched(args) -> None:
         "converged": conv, "phase_ms": {"assemble": round(ms_asm, 3), "precond_assemble": round(ms_pc, 3),
                                         "pcg": round(ms - ms_asm - ms_pc, 3)},
         "ms_per_system_mean": round(ms / args.systems, 4),
+    }))
+
+
+def run_learned(args) -> None:
+    """16^3-vertex FEM systems (the 16^3 recipe with the jitter drawn from
+    [0.03, 0.08] cells and Young's modulus log-uniform in [5e4, 2e5], rhs
+    uniform in [-1, 1]), the triplets written by the device producers.  The
+    learned initial guess is trained one system per Adam step on
+    --train-systems seeds, then MAS-PCG to 1e-4 from x0 = 0, from the
+    block-Jacobi guess D^-1 b (the network with its output at zero) and from
+    the learned x0 (with its fallback) are compared on --test-systems held-out
+    seeds.  Each system gets one untimed solve first and the three timed
+    solves rotate their order; the learned solve's time includes the
+    network's inference and the residual check of its guess."""
+    from libuipc_amd import abi, synth
+    from libuipc_amd.abi import check
+    from libuipc_amd.device import Context, LinearSystem
+    from libuipc_amd.hip_events import EventTimer
+    from libuipc_amd.warmstart import WarmStartNet, build_graph
+
+    grid = 16
+    ctx = Context(0)
+    st = ctx.stream()
+    first = synth.producer_inputs(grid, 0, seed=args.seed)
+    B, nt = len(first.mass), len(first.tets)
+    T = B + 10 * nt
+    rows, cols, vals = ctx.alloc(T, np.int32), ctx.alloc(T, np.int32), ctx.alloc((T, 9), np.float64)
+    r0, c0, v0 = rows.ptr.value, cols.ptr.value, vals.ptr.value
+    dtets = ctx.to_device(first.tets)
+    ls = LinearSystem(ctx, B)
+    ls.add_preconditioner(abi.MI_PRECOND_MAS, 0, 3 * B, part_ids=first.part_ids, tets=first.tets)
+
+    def make(seed):
+        rng = np.random.default_rng(seed)
+        p = synth.producer_inputs(grid, 0, seed=seed, jitter=float(rng.uniform(0.03, 0.08)),
+                                  youngs=float(np.exp(rng.uniform(np.log(5e4), np.log(2e5)))))
+        dm, dX, dx = ctx.to_device(p.mass), ctx.to_device(p.rest_positions), ctx.to_device(p.positions)
+        check(ctx.lib.mi_produce_kinetic(ctx.h, B, dm.ptr, 0, r0, c0, v0, st))
+        check(ctx.lib.mi_produce_snh(ctx.h, nt, dtets.ptr, dX.ptr, dx.ptr, p.mu, p.lam, p.dt, 0, r0 + 4 * B,
+                                     c0 + 4 * B, v0 + 72 * B, st))
+        ls.assemble(rows, cols, vals, st)
+        return build_graph(ctx, ls, rng.uniform(-1.0, 1.0, 3 * B))
+
+    net = WarmStartNet(ctx, seed=args.seed)
+    t0 = time.perf_counter()
+    losses = []
+    for k in range(args.train_systems):
+        g = make(args.seed * 100_000 + k)
+        losses.append(net.loss_and_grad(ls, g, loss=args.ws_loss))
+        net.adam(lr=args.ws_lr)
+        if (k + 1) % 500 == 0:
+            log(f"trained on {k + 1} systems: mean {args.ws_loss} loss over the last 500: "
+                f"{np.mean(losses[-500:]):.4f} ({time.perf_counter() - t0:.1f}s)")
+    kw = dict(criterion=abi.MI_CRIT_REL_RESIDUAL, tol=args.tol, true_residual=1)
+    ev = EventTimer(st)
+    x = ctx.alloc(3 * B, np.float64)
+    arms = ("x0=0", "jacobi", "learned")
+    its = {a: [] for a in arms}
+    ms = {a: [] for a in arms}
+    used, res0, res0_j, ok = 0, [], [], 0
+
+    def run(arm, g):
+        ev.start()
+        if arm == "x0=0":
+            out = ls.solve(g.b, x, **kw)
+        elif arm == "jacobi":
+            ctx.lib.mi_memcpy_d2d(ctx.h, x.ptr, g.x_jacobi.ptr, 8 * 3 * B)
+            out = ls.solve(g.b, x, x_init=1, **kw)
+        else:
+            out = net.solve(ls, g, x, **kw)
+        ms[arm].append(ev.stop())
+        its[arm].append(out["iters_exact"])
+        return out
+
+    for k in range(args.test_systems):
+        g = make(args.seed * 100_000 + 50_000 + k)
+        ls.assemble_preconditioners()
+        ls.solve(g.b, x, **kw)
+        ctx.synchronize()
+        outs = {}
+        for j in range(3):
+            arm = arms[(k + j) % 3]
+            outs[arm] = run(arm, g)
+        w = outs["learned"]
+        used += w["warm_used"]
+        res0.append(w["warm_rel_res0"])
+        ok += int(all(o["converged"] == 1 and o["rel_res_true"] <= 2 * args.tol for o in outs.values()))
+    ctx.synchronize()
+    ev.start()
+    for _ in range(20):
+        net.forward(g)
+    infer = ev.stop() / 20
+    mean = lambda v: round(float(np.mean(v)), 4)  # noqa: E731
+    print(json.dumps({
+        "metric": "MAS-PCG iterations and ms to 1e-4 rel. residual from x0 = 0 and from the learned x0",
+        "value": mean(ms["learned"]), "unit": "ms", "n_gpus": 1, "steps": args.test_systems,
+        "warmup": 0, "ms_per_step": mean(ms["learned"]), "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64 solve, bf16 network", "data": "synthetic",
+        "config": {"workload": "learned initial guess, 16^3 FEM systems (jitter, Young's modulus perturbed)",
+                   "train_systems": args.train_systems, "test_systems": args.test_systems, "width": 64,
+                   "layers": 3, "loss": args.ws_loss, "lr": args.ws_lr},
+        "iters_mean": {a: mean(its[a]) for a in arms}, "ms_mean": {a: mean(ms[a]) for a in arms},
+        "inference_ms": round(infer, 4), "guess_used": used,
+        "guess_rel_res0_median": round(float(np.median(res0)), 4), "all_converged": ok,
+        "train_loss_first_last_500": [mean(losses[:500]), mean(losses[-500:])],
     }))
 
 

@@ -14,10 +14,12 @@ include/mi_warmstart.h) is
 
 with G the block graph weighted by |A_ij|_F / sqrt(|A_ii|_F |A_jj|_F), and D
 the 3x3 diagonal blocks: with Y = 0 the guess is the block-Jacobi one.  The
-training loss is the relative residual ||b - A x0||^2 / ||b||^2, its gradient
--2 A r / ||b||^2 through the library's SpMV, then the network's backward pass
-on the device; Adam on FP32 master weights.  At solve time the guess is used
-only when its residual is below ||b|| (else x0 = 0, the plain method)."""
+training loss is the relative residual ||b - A x0||^2 / ||b||^2 (gradient
+-2 A r / ||b||^2) or the energy (x0^T A x0 / 2 - b^T x0) / (b^T D^-1 b)
+(gradient -r / (b^T D^-1 b)), through the library's SpMV, then the network's
+backward pass on the device; Adam on FP32 master weights.  At solve time the
+guess is used only when its residual is below ||b|| (else x0 = 0, the plain
+method)."""
 from __future__ import annotations
 
 import ctypes as C
@@ -59,6 +61,7 @@ class Graph:
     b: DeviceArray  # float64[3n]
     s_b: float
     bb: float
+    bxj: float  # b . D^-1 b, the scale of the energy loss
 
 
 def build_graph(ctx: Context, ls: LinearSystem, rhs: np.ndarray) -> Graph:
@@ -100,7 +103,7 @@ def build_graph(ctx: Context, ls: LinearSystem, rhs: np.ndarray) -> Graph:
     dinv_cm = Dinv.transpose(0, 2, 1).reshape(n, 9)  # [r][c] -> column-major
     return Graph(n, ctx.to_device(ptr), ctx.to_device(dst.astype(np.int32)), ctx.to_device(ww),
                  ctx.to_device(np.ascontiguousarray(dinv_cm)), ctx.to_device(xj.reshape(-1)), ctx.to_device(X),
-                 ctx.to_device(rhs), s_b, float(rhs @ rhs))
+                 ctx.to_device(rhs), s_b, float(rhs @ rhs), float(b3.reshape(-1) @ xj.reshape(-1)))
 
 
 class WarmStartNet:
@@ -172,16 +175,29 @@ class WarmStartNet:
                                     None, self.st))
         return self.x0
 
-    # ---- loss ||b - A x0||^2 / ||b||^2 and the parameter gradient
-    def loss_and_grad(self, ls: LinearSystem, g: Graph) -> float:
+    # ---- the loss and the parameter gradient
+    def loss_and_grad(self, ls: LinearSystem, g: Graph, loss: str = "residual") -> float:
+        """loss "residual": ||b - A x0||^2 / ||b||^2, gradient -2 A r / ||b||^2
+        (two SpMVs); loss "energy": (x0^T A x0 / 2 - b^T x0) / (b^T D^-1 b),
+        which differs from ||x0 - A^-1 b||_A^2 / 2 (the error CG's iteration
+        count depends on) by a constant, gradient -r / (b^T D^-1 b) (one
+        SpMV)."""
         n, P = g.n, self._ptr
         x0 = self.forward(g)
         self.ctx.lib.mi_memcpy_d2d(self.ctx.h, self.r.ptr, g.b.ptr, 8 * 3 * n)
         ls.spmv(-1.0, x0, 1.0, self.r)
-        rr = self.ctx.dot(self.r, self.r)
-        ls.spmv(-2.0 / g.bb, self.r, 0.0, self.gx)
-        # dY = s_b D^-1 gx (D^-1 symmetric)
-        check(self.lib.mi_ws_block3(self.ctx.h, n, g.dinv.ptr, g.s_b, None, None, None, self.gx.ptr, self.dY.ptr,
+        # dY = s D^-1 gx (D^-1 symmetric), s = s_b times the gradient's scale
+        if loss == "residual":
+            value = self.ctx.dot(self.r, self.r) / g.bb
+            ls.spmv(-2.0 / g.bb, self.r, 0.0, self.gx)
+            gvec, s = self.gx, g.s_b
+        elif loss == "energy":
+            # x0^T A x0 = x0^T (b - r)  =>  E = -x0^T (b + r) / 2
+            value = -0.5 * (self.ctx.dot(x0, g.b) + self.ctx.dot(x0, self.r)) / g.bxj
+            gvec, s = self.r, -g.s_b / g.bxj
+        else:
+            raise ValueError(f"unknown loss {loss!r}")
+        check(self.lib.mi_ws_block3(self.ctx.h, n, g.dinv.ptr, s, None, None, None, gvec.ptr, self.dY.ptr,
                                     self.st))
         H3 = self.H[LAYERS]
         self._gemm(WIDTH, F_OUT, n, P(H3), 1, WIDTH, P(self.dY), F_OUT, 1, self.p("W_out", True), F_OUT)
@@ -202,7 +218,7 @@ class WarmStartNet:
         check(self.lib.mi_ws_relu_backward(self.ctx.h, n * WIDTH, self.H[0].ptr, self.dH.ptr, self.st))
         self._gemm(F_IN, WIDTH, n, P(g.X), 1, F_IN, P(self.dH), WIDTH, 1, self.p("W_in", True), WIDTH)
         check(self.lib.mi_ws_colsum(self.ctx.h, n, WIDTH, self.dH.ptr, self.p("b_in", True), 0, self.st))
-        return rr / g.bb
+        return value
 
     def adam(self, lr: float = 1e-3, beta1: float = 0.9, beta2: float = 0.999, eps: float = 1e-8) -> None:
         self.step += 1

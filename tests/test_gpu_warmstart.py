@@ -1,8 +1,9 @@
 """The learned initial guess (libuipc_amd/warmstart.py, include/mi_warmstart.h):
-the device forward pass and parameter gradient against a float64 numpy
-restatement of the same network with every matrix-product operand rounded to
-BF16 as on the device, training lowers the residual of the guess, and a solve
-from the guess meets the same tolerance as from x0 = 0."""
+the device forward pass and parameter gradient (residual and energy losses)
+against a float64 numpy restatement of the same network with every
+matrix-product operand rounded to BF16 as on the device, training lowers the
+residual of the guess, and a solve from the guess meets the same tolerance as
+from x0 = 0."""
 import numpy as np
 import pytest
 
@@ -35,7 +36,7 @@ def mm(a, b):
     return bf16(a) @ bf16(b)
 
 
-def np_forward_backward(P, X, ptr, idx, w, dinv, xj, s_b, A, b):
+def np_forward_backward(P, X, ptr, idx, w, dinv, xj, s_b, A, b, loss_kind):
     n = X.shape[0]
     G = np.zeros((n, n))
     for i in range(n):
@@ -51,8 +52,13 @@ def np_forward_backward(P, X, ptr, idx, w, dinv, xj, s_b, A, b):
     D = dinv.reshape(n, 3, 3).transpose(0, 2, 1)  # column-major -> [r][c]
     x0 = xj + s_b * np.einsum("nij,nj->ni", D, Y).reshape(-1)
     r = b - A @ x0
-    loss = r @ r / (b @ b)
-    gx = -2.0 * (A @ r) / (b @ b)
+    if loss_kind == "residual":
+        loss = r @ r / (b @ b)
+        gx = -2.0 * (A @ r) / (b @ b)
+    else:
+        bxj = b @ xj
+        loss = (0.5 * x0 @ (A @ x0) - b @ x0) / bxj
+        gx = -r / bxj
     dY = s_b * np.einsum("nji,nj->ni", D, gx.reshape(n, 3))
     dY = dY.astype(np.float32).astype(np.float64)
     g = {"W_out": mm(H[-1].T, dY), "b_out": dY.sum(0)}
@@ -83,7 +89,8 @@ def dense(s):
     return A
 
 
-def test_forward_and_gradient_match_numpy(ctx):
+@pytest.mark.parametrize("loss_kind", ["residual", "energy"])
+def test_forward_and_gradient_match_numpy(ctx, loss_kind):
     s, ls = system(ctx, 5, 40, 31)
     g = build_graph(ctx, ls, s.rhs)
     net = WarmStartNet(ctx, seed=3)
@@ -91,16 +98,16 @@ def test_forward_and_gradient_match_numpy(ctx):
     o, sh = LAYOUT["W_out"]
     p[o:o + int(np.prod(sh))] = 0.05 * np.random.default_rng(4).standard_normal(int(np.prod(sh)))
     net.load(p)
-    loss = net.loss_and_grad(ls, g)
+    loss = net.loss_and_grad(ls, g, loss=loss_kind)
     x0 = net.x0.numpy()
     grads = unpack(net.grads.numpy())
     X = g.X.numpy().astype(np.float64)
     xw, lw, gw = np_forward_backward(unpack(p), X, g.ptr.numpy(), g.idx.numpy(), g.w.numpy().astype(np.float64),
-                                     g.dinv.numpy(), g.x_jacobi.numpy(), g.s_b, dense(s), s.rhs)
+                                     g.dinv.numpy(), g.x_jacobi.numpy(), g.s_b, dense(s), s.rhs, loss_kind)
     # with the operands rounded to BF16 as the device rounds them, what is left
     # is FP32 accumulation against float64
     assert rel_err(x0 - g.x_jacobi.numpy(), xw - g.x_jacobi.numpy()) < 1e-3
-    assert abs(loss - lw) <= 1e-3 * lw
+    assert abs(loss - lw) <= 1e-3 * abs(lw)
     for k, want in gw.items():
         got = grads[k]
         assert np.abs(got - want).max() <= 1e-2 * np.abs(want).max() + 1e-12, k
