@@ -66,6 +66,8 @@ def parse():
     ap.add_argument("--ws-loss", choices=["energy", "residual"], default="energy",
                     help="training loss of the learned initial guess")
     ap.add_argument("--ws-lr", type=float, default=1e-3)
+    ap.add_argument("--ws-contacts", type=int, default=20000,
+                    help="contact stencils per system of the learned-guess workload (kappa_hi perturbed per system)")
     ap.add_argument("--systems", type=int, default=256)
     ap.add_argument("--dof-lo", type=float, default=3e3)
     ap.add_argument("--dof-hi", type=float, default=6e5)
@@ -426,9 +428,11 @@ def run_batched(args) -> None:
 
 
 def run_learned(args) -> None:
-    """16^3-vertex FEM systems (the 16^3 recipe with the jitter drawn from
-    [0.03, 0.08] cells and Young's modulus log-uniform in [5e4, 2e5], rhs
-    uniform in [-1, 1]), the triplets written by the device producers.  The
+    """16^3-vertex FEM systems plus --ws-contacts contact stencils (the jitter
+    drawn from [0.03, 0.08] cells, Young's modulus log-uniform in [5e4, 2e5],
+    the contact stiffness log-uniform in [1e4, kappa_hi] with kappa_hi
+    log-uniform in [1e6, 1e8], rhs uniform in [-1, 1]), the triplets written
+    by the device producers.  The
     learned initial guess is trained one system per Adam step on
     --train-systems seeds, then MAS-PCG to 1e-4 from x0 = 0, from the
     block-Jacobi guess D^-1 b (the network with its output at zero) and from
@@ -445,9 +449,10 @@ def run_learned(args) -> None:
     grid = 16
     ctx = Context(0)
     st = ctx.stream()
-    first = synth.producer_inputs(grid, 0, seed=args.seed)
+    nc = args.ws_contacts
+    first = synth.producer_inputs(grid, nc, seed=args.seed)
     B, nt = len(first.mass), len(first.tets)
-    T = B + 10 * nt
+    T = B + 10 * nt + 10 * nc
     rows, cols, vals = ctx.alloc(T, np.int32), ctx.alloc(T, np.int32), ctx.alloc((T, 9), np.float64)
     r0, c0, v0 = rows.ptr.value, cols.ptr.value, vals.ptr.value
     dtets = ctx.to_device(first.tets)
@@ -456,13 +461,20 @@ def run_learned(args) -> None:
 
     def make(seed):
         rng = np.random.default_rng(seed)
-        p = synth.producer_inputs(grid, 0, seed=seed, jitter=float(rng.uniform(0.03, 0.08)),
-                                  youngs=float(np.exp(rng.uniform(np.log(5e4), np.log(2e5)))))
+        p = synth.producer_inputs(grid, nc, seed=seed, jitter=float(rng.uniform(0.03, 0.08)),
+                                  youngs=float(np.exp(rng.uniform(np.log(5e4), np.log(2e5)))),
+                                  kappa_hi=float(10.0 ** rng.uniform(6.0, 8.0)))
         dm, dX, dx = ctx.to_device(p.mass), ctx.to_device(p.rest_positions), ctx.to_device(p.positions)
         check(ctx.lib.mi_produce_kinetic(ctx.h, B, dm.ptr, 0, r0, c0, v0, st))
         check(ctx.lib.mi_produce_snh(ctx.h, nt, dtets.ptr, dX.ptr, dx.ptr, p.mu, p.lam, p.dt, 0, r0 + 4 * B,
                                      c0 + 4 * B, v0 + 72 * B, st))
+        if nc:
+            o = B + 10 * nt
+            ci, cg, ck = (ctx.to_device(a) for a in (p.contact_idx, p.contact_g, p.contact_kappa))
+            check(ctx.lib.mi_produce_contact(ctx.h, nc, ci.ptr, cg.ptr, ck.ptr, p.dt, 0, r0 + 4 * o, c0 + 4 * o,
+                                             v0 + 72 * o, st))
         ls.assemble(rows, cols, vals, st)
+        ctx.synchronize()  # the per-system inputs are freed next
         return build_graph(ctx, ls, rng.uniform(-1.0, 1.0, 3 * B))
 
     net = WarmStartNet(ctx, seed=args.seed)
@@ -520,7 +532,8 @@ def run_learned(args) -> None:
         "value": mean(ms["learned"]), "unit": "ms", "n_gpus": 1, "steps": args.test_systems,
         "warmup": 0, "ms_per_step": mean(ms["learned"]), "higher_is_better": False,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64 solve, bf16 network", "data": "synthetic",
-        "config": {"workload": "learned initial guess, 16^3 FEM systems (jitter, Young's modulus perturbed)",
+        "config": {"workload": f"learned initial guess, 16^3 FEM systems + {nc} contacts "
+                               "(jitter, Young's modulus, contact stiffness perturbed)",
                    "train_systems": args.train_systems, "test_systems": args.test_systems, "width": 64,
                    "layers": 3, "loss": args.ws_loss, "lr": args.ws_lr},
         "iters_mean": {a: mean(its[a]) for a in arms}, "ms_mean": {a: mean(ms[a]) for a in arms},
