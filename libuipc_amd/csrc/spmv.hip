@@ -53,6 +53,10 @@ struct Stage
     static constexpr int kPer   = sizeof(VT) == 4 ? 6 : 3;
     static constexpr int kBlocks = kPer * kThreads;
     using W                      = std::conditional_t<sizeof(VT) == 4, float, double>;
+    // blocks per lane and round: FP32 planes are read as pairs of consecutive
+    // blocks (8-byte loads; the planes' stride is even and the arrays hold
+    // one block of padding past the last)
+    static constexpr int kVec = sizeof(VT) == 4 ? 2 : 1;
 };
 
 template <bool ACCUM>
@@ -131,6 +135,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) v
         return;
     using W                  = typename Stage<VT>::W;
     constexpr int      kPer   = Stage<VT>::kPer;
+    constexpr int      kVec   = Stage<VT>::kVec;
     constexpr int      kStage = Stage<VT>::kBlocks;
     __shared__ W       w[3][kStage];
     __shared__ double  lds[kThreads / kWave];
@@ -171,29 +176,72 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) v
         int32_t       s0      = fptr[r_first];
         bool          carried = false;
         int           par     = 0;
-        for(; s0 < K1; s0 += kStage, par ^= 1)
+        for(int32_t s1; s0 < K1; s0 = s1, par ^= 1)
         {
-            const int32_t s1 = s0 + kStage < K1 ? s0 + kStage : K1;
-            // products, consecutive lanes on consecutive blocks, one block per
-            // lane and round (load, gather, product)
+            // the stage's slots start at o0 (s0 rounded down to a multiple of
+            // kVec, so the first slot may precede s0); it covers [s0, s1)
+            const int32_t o0 = s0 & ~(kVec - 1);
+            s1               = o0 + kStage < K1 ? o0 + kStage : K1;
+            // products, consecutive lanes on consecutive blocks (kVec of them
+            // per lane and round: load, gather, product)
 #pragma unroll
-            for(int h = 0; h < kPer; ++h)
+            for(int h = 0; h < kPer / kVec; ++h)
             {
-                const int32_t  k  = s0 + h * kThreads + tid;
-                const bool     ok = k < s1;
-                const uint32_t j  = ok ? (uint32_t)stream_load<VT>(&fcols[k]) : 0u;
-                VT             A[9];
-#pragma unroll
-                for(int q = 0; q < 9; ++q)
-                    A[q] = ok ? stream_load<VT>(&fvals[q * fstride + k]) : VT(0);
-                double x0, x1, x2;
-                gather_x<VT>(x, x32, j, x0, x1, x2);
-                if(ok)
+                const int i = (h * kThreads + tid) * kVec;
+                if constexpr(kVec == 2)
                 {
-                    const int i = h * kThreads + tid;
-                    w[0][i]     = (W)((double)A[0] * x0 + (double)A[3] * x1 + (double)A[6] * x2);
-                    w[1][i]     = (W)((double)A[1] * x0 + (double)A[4] * x1 + (double)A[7] * x2);
-                    w[2][i]     = (W)((double)A[2] * x0 + (double)A[5] * x1 + (double)A[8] * x2);
+                    // k >= s0 - 1, so the second block is at or after s0
+                    const int32_t k   = o0 + i;
+                    const bool    ok0 = k >= s0 && k < s1, ok1 = k + 1 < s1;
+                    const bool    any = ok0 || ok1;
+                    int2          jj  = {0, 0};
+                    float2        A[9];
+                    if(any)
+                    {
+                        jj = *reinterpret_cast<const int2*>(&fcols[k]);
+#pragma unroll
+                        for(int q = 0; q < 9; ++q)
+                            A[q] = *reinterpret_cast<const float2*>(&fvals[q * fstride + k]);
+                    }
+                    else
+                    {
+#pragma unroll
+                        for(int q = 0; q < 9; ++q)
+                            A[q] = float2{0.f, 0.f};
+                    }
+                    double x0, x1, x2, z0, z1, z2;
+                    gather_x<VT>(x, x32, ok0 ? (uint32_t)jj.x : 0u, x0, x1, x2);
+                    gather_x<VT>(x, x32, ok1 ? (uint32_t)jj.y : 0u, z0, z1, z2);
+                    if(ok0)
+                    {
+                        w[0][i] = (W)((double)A[0].x * x0 + (double)A[3].x * x1 + (double)A[6].x * x2);
+                        w[1][i] = (W)((double)A[1].x * x0 + (double)A[4].x * x1 + (double)A[7].x * x2);
+                        w[2][i] = (W)((double)A[2].x * x0 + (double)A[5].x * x1 + (double)A[8].x * x2);
+                    }
+                    if(ok1)
+                    {
+                        w[0][i + 1] = (W)((double)A[0].y * z0 + (double)A[3].y * z1 + (double)A[6].y * z2);
+                        w[1][i + 1] = (W)((double)A[1].y * z0 + (double)A[4].y * z1 + (double)A[7].y * z2);
+                        w[2][i + 1] = (W)((double)A[2].y * z0 + (double)A[5].y * z1 + (double)A[8].y * z2);
+                    }
+                }
+                else
+                {
+                    const int32_t  k  = o0 + i;
+                    const bool     ok = k < s1;
+                    const uint32_t j  = ok ? (uint32_t)stream_load<VT>(&fcols[k]) : 0u;
+                    VT             A[9];
+#pragma unroll
+                    for(int q = 0; q < 9; ++q)
+                        A[q] = ok ? stream_load<VT>(&fvals[q * fstride + k]) : VT(0);
+                    double x0, x1, x2;
+                    gather_x<VT>(x, x32, j, x0, x1, x2);
+                    if(ok)
+                    {
+                        w[0][i] = (W)((double)A[0] * x0 + (double)A[3] * x1 + (double)A[6] * x2);
+                        w[1][i] = (W)((double)A[1] * x0 + (double)A[4] * x1 + (double)A[7] * x2);
+                        w[2][i] = (W)((double)A[2] * x0 + (double)A[5] * x1 + (double)A[8] * x2);
+                    }
                 }
             }
             __syncthreads();
@@ -216,7 +264,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) v
                     break;
                 }
                 const int32_t re = fptr[row + 1];
-                const int32_t lo = (rs > s0 ? rs : s0) - s0, hi = (re < s1 ? re : s1) - s0;
+                const int32_t lo = (rs > s0 ? rs : s0) - o0, hi = (re < s1 ? re : s1) - o0;
                 double        acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
                 for(int32_t i = lo + lane; i < hi; i += kLanes)
                 {
