@@ -240,6 +240,23 @@ __global__ void chunk_list_kernel(const int32_t* chunk_row, int32_t nchunks, con
             list[offs[c]] = c;
 }
 
+__global__ void cluster_flag_kernel(const int32_t* part_to_real, int64_t clusters, int bank, int32_t row_off,
+                                    const int32_t* seg, int32_t nseg, const BatchScalars* sc, int32_t* on)
+{
+    for(int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < clusters;
+        k += (int64_t)gridDim.x * blockDim.x)
+    {
+        int32_t f = 0;
+        for(int i = 0; i < bank && !f; ++i)
+        {
+            const int32_t v = part_to_real[k * bank + i];
+            if(v >= 0 && !sc[system_of(seg, nseg, row_off + v)].stopped)
+                f = 1;
+        }
+        on[k] = f;
+    }
+}
+
 int tile_grid(const BatchTiles& t)
 {
     return std::max(1, std::min(t.n, device_cu_count() * 8));
@@ -259,6 +276,17 @@ void batch_active_chunks(const int32_t* chunk_row, int32_t nchunks, const int32_
     chunk_flag_kernel<<<g, kThreads, 0, s>>>(chunk_row, nchunks, seg, nseg, sc, flags.data());
     exclusive_scan_u32(flags.data(), nchunks, scan_chunk, d_count, s);
     chunk_list_kernel<<<g, kThreads, 0, s>>>(chunk_row, nchunks, seg, nseg, sc, flags.data(), list);
+    MI_HIP(hipGetLastError());
+}
+
+void batch_cluster_flags(const int32_t* part_to_real, int64_t clusters, int bank, int32_t row_off, const int32_t* seg,
+                         int32_t nseg, const BatchScalars* sc, int32_t* on, hipStream_t s)
+{
+    if(clusters > 0)
+    {
+        const int g = (int)std::max<int64_t>(1, std::min<int64_t>(ceil_div(clusters, kThreads), device_cu_count() * 4));
+        cluster_flag_kernel<<<g, kThreads, 0, s>>>(part_to_real, clusters, bank, row_off, seg, nseg, sc, on);
+    }
     MI_HIP(hipGetLastError());
 }
 

@@ -1425,8 +1425,21 @@ extern "C" int mi_pcg_solve_batched(mi_sys*            sys,
             return;
         batch_active_chunks(sys->A.chunk_row.data(), sys->A.nchunks, bt.seg_dev.data(), nseg, bt.sc.data(),
                             bt.act_flags, bt.act_scan, bt.active.data(), bt.n_active.data(), s);
+        for(auto& pc : sys->pcs)
+            pc->skip_stopped(bt.seg_dev.data(), nseg, bt.sc.data(), s);
         n_stopped = stopped_now;
     };
+    // the preconditioners skip stopped systems only inside this solve
+    struct SkipOff
+    {
+        mi_sys*     sys;
+        hipStream_t s;
+        ~SkipOff()
+        {
+            for(auto& pc : sys->pcs)
+                pc->skip_stopped(nullptr, 0, nullptr, s);
+        }
+    } skip_off{sys, s};
     auto count_stopped = [&]
     {
         int32_t c = 0;

@@ -221,6 +221,7 @@ float  dot_f32_host(const float* x, const float* y, int64_t n, hipStream_t s);
 float  norm_f32_host(const float* x, int64_t n, hipStream_t s);
 
 // ----------------------------------------------------------------- preconditioners
+struct BatchScalars;
 // LocalPreconditioner (linear_system/local_preconditioner.h:9-49)
 class LocalPreconditioner
 {
@@ -240,6 +241,10 @@ class LocalPreconditioner
     virtual int apply_dot(const double*, double*, const int32_t*, double*, int, hipStream_t) { return -1; }
     // MAS only: levels used by apply (MASPreconditionerEngine::set_active_level_num)
     virtual bool set_active_levels(int) { return false; }
+    // batched solves: apply may skip the work of systems that have stopped
+    // (their r no longer changes, so neither does z); seg = the systems'
+    // first block rows (device); seg == nullptr switches the skipping off
+    virtual void skip_stopped(const int32_t*, int32_t, const BatchScalars*, hipStream_t) {}
     // MAS only: host copies of the hierarchy and cluster matrices
     virtual bool mas_export(int32_t*, int32_t*, int32_t*, int32_t*, int32_t*, int32_t*, double*, double*)
     {
@@ -306,6 +311,10 @@ struct BatchScalars
     int32_t criterion, converged, stopped, pad;
 };
 void batch_tile_dot(const BatchTiles& t, const double* a, const double* b, double* ab, double* aa, hipStream_t s);
+// on[k] = 1 while cluster k (slots [bank k, bank (k + 1)) of part_to_real,
+// local vertices, -1 = empty) holds a vertex of a running system
+void batch_cluster_flags(const int32_t* part_to_real, int64_t clusters, int bank, int32_t row_off, const int32_t* seg,
+                         int32_t nseg, const BatchScalars* sc, int32_t* on, hipStream_t s);
 // the SpMV chunks that hold rows of a running system, ascending, into list;
 // *d_count = their number (seg: the systems' first block rows, device)
 void batch_active_chunks(const int32_t* chunk_row, int32_t nchunks, const int32_t* seg, int32_t nseg,

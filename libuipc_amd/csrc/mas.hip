@@ -1167,9 +1167,13 @@ __global__ __launch_bounds__(64) void mas_apply_fine_kernel(const double* __rest
                                                            const int32_t* __restrict__ unpart,
                                                            int32_t        n_unpart,
                                                            const double* __restrict__ Dun,
-                                                           const int32_t* conv)
+                                                           const int32_t* conv,
+                                                           const int32_t* __restrict__ on)
 {
     if(conv && *conv != 0)
+        return;
+    // a cluster whose systems have all stopped keeps its z (batched solves)
+    if(on && (int64_t)blockIdx.x < C0 && on[blockIdx.x] == 0)
         return;
     const int lane = threadIdx.x;
     if((int64_t)blockIdx.x >= C0)
@@ -1366,6 +1370,8 @@ class MasPreconditioner final : public LocalPreconditioner
     DBuf<double>  d_R64, d_Z64;
     DBuf<float>   d_R32, d_Z32;
     std::vector<std::pair<int64_t, int64_t>> level_clusters;  // [first, count] per level
+    DBuf<int32_t>  d_fine_on;          // per fine cluster: 1 while it holds a running system
+    const int32_t* fine_on = nullptr;  // d_fine_on during a batched solve, else nullptr
 
     explicit MasPreconditioner(const mi_precond_desc& d)
     {
@@ -1596,7 +1602,8 @@ class MasPreconditioner final : public LocalPreconditioner
         if(blocks > 0)
             mas_apply_fine_kernel<T><<<(unsigned)blocks, 64, 0, s>>>(r, z, d_part_to_real.data(), inv, Z,
                                                                      d_anc_slot.data(), h.N, Lact, C0,
-                                                                     d_unpart.data(), n_unpart, d_Dun.data(), conv);
+                                                                     d_unpart.data(), n_unpart, d_Dun.data(), conv,
+                                                                     fine_on);
         return 0;
     }
 
@@ -1655,6 +1662,21 @@ class MasPreconditioner final : public LocalPreconditioner
     {
         active_levels = levels;  // 0 or out of range: every level, as the reference clamps
         return true;
+    }
+
+    // fine clusters of stopped systems are skipped; the restriction and the
+    // coarse levels (a few percent of the apply) still run over everything
+    void skip_stopped(const int32_t* seg, int32_t nseg, const BatchScalars* sc, hipStream_t s) override
+    {
+        if(!seg || C0 == 0)
+        {
+            fine_on = nullptr;
+            return;
+        }
+        d_fine_on.resize((size_t)C0);
+        batch_cluster_flags(d_part_to_real.data(), C0, kBank, (int32_t)(dof_offset / 3), seg, nseg, sc,
+                            d_fine_on.data(), s);
+        fine_on = d_fine_on.data();
     }
 
     void apply(const double* r, double* z, const int32_t* conv, hipStream_t s) override
