@@ -785,6 +785,7 @@ int mi_ws_block3(mi_ctx* ctx, int32_t n, const double* blocks, double scale, con
     MI_API_BEGIN
     MI_REQUIRE(ctx && n >= 0 && blocks && ((v_f32 && y_f64 && !v_f64 && !y_f32) || (v_f64 && y_f32 && !v_f32 && !y_f64)),
                MI_ERR_ARG, "mi_ws_block3: give (v_f32, y_f64) or (v_f64, y_f32)");
+    MI_REQUIRE(!base || v_f32, MI_ERR_ARG, "mi_ws_block3: base goes with (v_f32, y_f64)");
     ws_block3(n, blocks, scale, v_f32, y_f64, base, v_f64, y_f32, pick(ctx, stream));
     MI_API_END
 }
