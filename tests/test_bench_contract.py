@@ -1,6 +1,6 @@
 """bench.py's contract on the CPU: more than one GPU is refused with a JSON
-message and exit status 2 (before any device call), and the HBM traffic
-summary is read for the default workload only."""
+message and exit status 2 (before any device call), the HBM traffic summary
+is read for the default workload only, and every workload's options parse."""
 import json
 import os
 import subprocess
@@ -27,3 +27,11 @@ def test_traffic_summary_only_for_the_default_workload():
     assert t is not None and 4e8 < t < 2e9
     args.contacts = 1000
     assert bench.measured_traffic("spmv_chunk_kernel<double, true>", args) is None
+
+
+def test_every_workload_parses():
+    import bench
+
+    for w in ("single", "batched", "learned", "coupled"):
+        sys.argv = ["bench.py", "--workload", w]
+        assert bench.parse().workload == w
