@@ -71,6 +71,42 @@ def test_batch_mas_union(ctx):
     ls.close()
 
 
+@pytest.mark.parametrize("precision", [abi.MI_FP64, abi.MI_FP32], ids=["fp64", "fp32"])
+def test_batch_mas_skips_stopped_systems(ctx, precision):
+    """The SpMV chunks and the MAS fine clusters of stopped systems are
+    skipped.  Solving the same batch (same assembled MAS) with one system's
+    rhs zeroed stops that system at once, so its clusters are skipped for the
+    whole solve: every other system must reach the same solution (to
+    rounding: the SpMV's stage boundaries follow the active chunks), in the
+    same iterations, and the zeroed system must return x = 0."""
+    bt = synth.batch_systems(8, seed=5, dof_lo=3e3, dof_hi=3e4)
+    ls = LinearSystem(ctx, bt.block_rows, precision)
+    r, c, v = ctx.to_device(bt.rows), ctx.to_device(bt.cols), ctx.to_device(bt.vals)
+    ls.assemble(r, c, v)
+    ls.add_preconditioner(abi.MI_PRECOND_MAS, 0, 3 * bt.block_rows, part_ids=bt.part_ids, tets=bt.tets,
+                          build=abi.MI_MAS_BUILD_DEVICE, mas_precision=precision)
+    ls.assemble_preconditioners()
+    kw = dict(criterion=abi.MI_CRIT_REL_RESIDUAL, tol=1e-6, true_residual=1)
+    x1, x2 = ctx.alloc(3 * bt.block_rows, np.float64), ctx.alloc(3 * bt.block_rows, np.float64)
+    s1 = ls.solve_batched(bt.seg_rows, ctx.to_device(bt.rhs), x1, **kw)
+    j = 3
+    segj = slice(3 * bt.seg_rows[j], 3 * bt.seg_rows[j + 1])
+    rhs2 = bt.rhs.copy()
+    rhs2[segj] = 0.0
+    s2 = ls.solve_batched(bt.seg_rows, ctx.to_device(rhs2), x2, **kw)
+    a, b = x1.numpy(), x2.numpy()
+    tol = 1e-9 if precision == abi.MI_FP64 else 1e-4
+    for k in range(len(bt.systems)):
+        seg = slice(3 * bt.seg_rows[k], 3 * bt.seg_rows[k + 1])
+        if k == j:
+            assert s2[k]["converged"] == 1 and s2[k]["iters_exact"] == 0 and not b[seg].any()
+            continue
+        assert s1[k]["converged"] == 1 and s2[k]["converged"] == 1, (k, s1[k], s2[k])
+        assert abs(s1[k]["iters_exact"] - s2[k]["iters_exact"]) <= 1, (k, s1[k], s2[k])
+        assert rel_err(b[seg], a[seg]) < tol, k
+    ls.close()
+
+
 def test_batch_config4_recipe(ctx):
     """config 4's recipe at a tenth of its size range: 64 systems, DoF
     log-uniform in [3e3, 6e4] (the full range, up to 6e5 DoFs per system,
