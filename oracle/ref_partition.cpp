@@ -2,7 +2,7 @@
 // geometry::mesh_partition (src/geometry/mesh_partition.cpp:77-230) restated
 // around the reference's own METIS 5.2.1 port (src/geometry/metis/, compiled
 // in place by oracle/Makefile into oracle/_ref/libref_partition.so).  Tests
-// use it to judge the device partitioner's quality against METIS; it is never
+// use it to measure the device partitioner's quality against METIS; it is never
 // linked into the product.
 //
 // Same control flow as mesh_partition:

@@ -112,7 +112,7 @@ def test_convert_empty_and_reassemble(ctx, c1):
 
 
 def test_convert_full_size_fem_bitexact(ctx):
-    """64^3 FEM grid (the judged system's FEM part): U is the closed form and
+    """64^3 FEM grid (the FEM part of the 64^3 + 2M contact system): U is the closed form and
     the structure and values are bit-exact with the oracle"""
     s = synth.fem_system(64, 0, seed=1)
     ls = assembled(ctx, s)
