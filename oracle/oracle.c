@@ -13,8 +13,15 @@
  *   or_spmv_sym     vs a dense U + U^T - D expansion; symmetry x^T A y = y^T A x.
  *   or_dot/or_norm  the reference's KATs (apps/tests/backends/cuda/cuda_tool_linear_system.cu).
  *   or_diag3/12     closed form: (D^-1 r) re-multiplied by D gives r.
- *   or_mas_*        hierarchy invariants (one ancestor per level, bank-aligned
- *                   levels, level count), scatter vs dense R_L A R_L^T per level,
+ *   or_mas_*        hierarchy: every level's tables equal a brute force from the
+ *                   definition (in-bank connected components of the mesh edges
+ *                   mapped to the level below, numbered by lowest member), on
+ *                   geometric, random-disconnected and partly unpartitioned
+ *                   partitions up to the 6-level cap; that check fails on the
+ *                   OR_MUTANT_* builds below (planted mistakes in the closure,
+ *                   the neighbour compaction and the coarse mask, compiled only
+ *                   into test libraries).  Invariants (one ancestor per level,
+ *                   bank-aligned levels, level count), scatter vs dense R_L A R_L^T per level,
  *                   inverse C C^-1 = I, apply vs dense sum_L P_L^T C_L^-1 P_L r,
  *                   one-level apply = 16-node block Jacobi, 1-vertex clusters = 3x3 Jacobi.
  *   or_pcg          exact convergence in <= n steps on small SPD systems, the
@@ -444,6 +451,13 @@ static int cmp_u32(const void* a, const void* b)
  * 318-327), reading the neighbours' initial masks */
 static uint32_t bank_closure(const uint32_t cache[BANKSIZE], int lane, uint32_t start)
 {
+#ifdef OR_MUTANT_NO_CLOSURE
+    /* planted mistake, compiled only into a test's mutant library: direct
+     * neighbours only, no transitive closure */
+    (void)cache;
+    (void)lane;
+    return start;
+#endif
     uint32_t conn = start, visited = 1u << lane;
     for(;;)
     {
@@ -624,8 +638,10 @@ or_mas* or_mas_create(int32_t        V,
                     continue;
                 if(np / BANKSIZE == bank)
                     msk |= 1u << (np % BANKSIZE);
+#ifndef OR_MUTANT_DROP_NEIGHBOURS /* planted mistake (test mutant): coarse levels lose every edge */
                 else
                     nbr_list[st + nk++] = nb;
+#endif
             }
             nbr_num[idx]          = nk;
             m->fine_connect[idx]  = msk;
@@ -706,7 +722,11 @@ or_mas* or_mas_create(int32_t        V,
                         continue;
                     }
                     if(cidx / BANKSIZE == cc / BANKSIZE)
+#ifdef OR_MUTANT_COARSE_SELF /* planted mistake (test mutant): the coarse mask names the node itself */
+                        conn |= 1u << (cidx % BANKSIZE);
+#else
                         conn |= 1u << (cc % BANKSIZE);
+#endif
                     else
                         nbr_list[st + nk++] = nb;
                 }

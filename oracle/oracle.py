@@ -47,51 +47,64 @@ def lib() -> C.CDLL:
     if _lib is None:
         if not os.path.exists(SO):
             build()
-        L = C.CDLL(SO)
-        L.or_key_bits.restype = C.c_int
-        L.or_key_bits.argtypes = [C.c_uint64]
-        L.or_convert_sym.restype = C.c_int64
-        L.or_convert_sym.argtypes = [C.c_int32, C.c_int64, i32p, i32p, f64p, i32p, i32p, f64p, i32p]
-        L.or_spmv_sym.restype = C.c_double
-        L.or_spmv_sym.argtypes = [C.c_int32, C.c_int64, i32p, i32p, f64p, f64p, f64p]
-        L.or_dot.restype = C.c_double
-        L.or_dot.argtypes = [f64p, f64p, C.c_int64]
-        L.or_norm.restype = C.c_double
-        L.or_norm.argtypes = [f64p, C.c_int64]
-        L.or_diag3_assemble.restype = C.c_int
-        L.or_diag3_assemble.argtypes = [C.c_int64, i32p, i32p, f64p, C.c_int32, C.c_int32, f64p]
-        L.or_diag3_apply.restype = None
-        L.or_diag3_apply.argtypes = [C.c_int32, f64p, f64p, f64p]
-        L.or_diag12_assemble.restype = C.c_int
-        L.or_diag12_assemble.argtypes = [C.c_int64, i32p, i32p, f64p, C.c_int32, C.c_int32, f64p]
-        L.or_diag12_apply.restype = None
-        L.or_diag12_apply.argtypes = [C.c_int32, f64p, f64p, f64p]
-        L.or_mas_create.restype = C.c_void_p
-        L.or_mas_create.argtypes = [C.c_int32, i32p, i32p, C.c_int64, i32p, C.c_int64, i32p, C.c_int64]
-        L.or_mas_destroy.restype = None
-        L.or_mas_destroy.argtypes = [C.c_void_p]
-        L.or_mas_info.restype = None
-        L.or_mas_info.argtypes = [C.c_void_p, i32p]
-        L.or_mas_tables.restype = None
-        L.or_mas_tables.argtypes = [C.c_void_p, i32p, i32p, i32p, i32p, i32p, u32p, i32p]
-        L.or_mas_going_next_size.restype = C.c_int64
-        L.or_mas_going_next_size.argtypes = [C.c_void_p]
-        L.or_mas_assemble.restype = C.c_int
-        L.or_mas_assemble.argtypes = [C.c_void_p, C.c_int64, i32p, i32p, f64p, C.c_int32, C.c_int]
-        L.or_mas_num_clusters.restype = C.c_int64
-        L.or_mas_num_clusters.argtypes = [C.c_void_p]
-        L.or_mas_cluster_hessians.restype = None
-        L.or_mas_cluster_hessians.argtypes = [C.c_void_p, f64p]
-        L.or_mas_cluster_inverses.restype = None
-        L.or_mas_cluster_inverses.argtypes = [C.c_void_p, f64p]
-        L.or_mas_apply.restype = None
-        L.or_mas_apply.argtypes = [C.c_void_p, f64p, f64p, C.c_int, C.c_int]
-        L.or_pcg.restype = C.c_int
-        L.or_pcg.argtypes = [C.c_int32, C.c_int64, i32p, i32p, f64p, C.POINTER(OrPrecond), C.c_int32, f64p, f64p,
-                             C.c_int32, C.c_double, C.c_double, C.c_int32, C.c_int32, C.c_int32, C.c_int64,
-                             C.POINTER(OrPcgResult)]
-        _lib = L
+        _lib = _bind(C.CDLL(SO))
     return _lib
+
+
+def mutant(name: str) -> C.CDLL:
+    """A planted-mistake build of the oracle (-DOR_MUTANT_<name>, see
+    oracle.c), for tests that show a check detects the mistake."""
+    so = os.path.join(HERE, "_build", f"liboracle_mutant_{name}.so")
+    r = subprocess.run(["make", "-s", "-C", HERE, so], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"mutant build failed:\n{r.stdout}\n{r.stderr}")
+    return _bind(C.CDLL(so))
+
+
+def _bind(L: C.CDLL) -> C.CDLL:
+    L.or_key_bits.restype = C.c_int
+    L.or_key_bits.argtypes = [C.c_uint64]
+    L.or_convert_sym.restype = C.c_int64
+    L.or_convert_sym.argtypes = [C.c_int32, C.c_int64, i32p, i32p, f64p, i32p, i32p, f64p, i32p]
+    L.or_spmv_sym.restype = C.c_double
+    L.or_spmv_sym.argtypes = [C.c_int32, C.c_int64, i32p, i32p, f64p, f64p, f64p]
+    L.or_dot.restype = C.c_double
+    L.or_dot.argtypes = [f64p, f64p, C.c_int64]
+    L.or_norm.restype = C.c_double
+    L.or_norm.argtypes = [f64p, C.c_int64]
+    L.or_diag3_assemble.restype = C.c_int
+    L.or_diag3_assemble.argtypes = [C.c_int64, i32p, i32p, f64p, C.c_int32, C.c_int32, f64p]
+    L.or_diag3_apply.restype = None
+    L.or_diag3_apply.argtypes = [C.c_int32, f64p, f64p, f64p]
+    L.or_diag12_assemble.restype = C.c_int
+    L.or_diag12_assemble.argtypes = [C.c_int64, i32p, i32p, f64p, C.c_int32, C.c_int32, f64p]
+    L.or_diag12_apply.restype = None
+    L.or_diag12_apply.argtypes = [C.c_int32, f64p, f64p, f64p]
+    L.or_mas_create.restype = C.c_void_p
+    L.or_mas_create.argtypes = [C.c_int32, i32p, i32p, C.c_int64, i32p, C.c_int64, i32p, C.c_int64]
+    L.or_mas_destroy.restype = None
+    L.or_mas_destroy.argtypes = [C.c_void_p]
+    L.or_mas_info.restype = None
+    L.or_mas_info.argtypes = [C.c_void_p, i32p]
+    L.or_mas_tables.restype = None
+    L.or_mas_tables.argtypes = [C.c_void_p, i32p, i32p, i32p, i32p, i32p, u32p, i32p]
+    L.or_mas_going_next_size.restype = C.c_int64
+    L.or_mas_going_next_size.argtypes = [C.c_void_p]
+    L.or_mas_assemble.restype = C.c_int
+    L.or_mas_assemble.argtypes = [C.c_void_p, C.c_int64, i32p, i32p, f64p, C.c_int32, C.c_int]
+    L.or_mas_num_clusters.restype = C.c_int64
+    L.or_mas_num_clusters.argtypes = [C.c_void_p]
+    L.or_mas_cluster_hessians.restype = None
+    L.or_mas_cluster_hessians.argtypes = [C.c_void_p, f64p]
+    L.or_mas_cluster_inverses.restype = None
+    L.or_mas_cluster_inverses.argtypes = [C.c_void_p, f64p]
+    L.or_mas_apply.restype = None
+    L.or_mas_apply.argtypes = [C.c_void_p, f64p, f64p, C.c_int, C.c_int]
+    L.or_pcg.restype = C.c_int
+    L.or_pcg.argtypes = [C.c_int32, C.c_int64, i32p, i32p, f64p, C.POINTER(OrPrecond), C.c_int32, f64p, f64p,
+                         C.c_int32, C.c_double, C.c_double, C.c_int32, C.c_int32, C.c_int32, C.c_int64,
+                         C.POINTER(OrPcgResult)]
+    return L
 
 
 def _i(a):
@@ -174,8 +187,9 @@ def diag12_apply(H, r):
 class Mas:
     """MASPreconditionerEngine restated on the CPU."""
 
-    def __init__(self, vert_num, part_ids, tets=None, tris=None, edges=None):
+    def __init__(self, vert_num, part_ids, tets=None, tris=None, edges=None, *, library=None):
         self._keep = []
+        self._L = library if library is not None else lib()
 
         def arr(a, k):
             if a is None:
@@ -188,22 +202,22 @@ class Mas:
         tp, nt = arr(tets, 4)
         trp, ntr = arr(tris, 3)
         ep, ne = arr(edges, 2)
-        self.h = lib().or_mas_create(vert_num, pp, tp, nt, trp, ntr, ep, ne)
+        self.h = self._L.or_mas_create(vert_num, pp, tp, nt, trp, ntr, ep, ne)
         if not self.h:
             raise ValueError("or_mas_create failed (partition larger than 16?)")
         info = np.empty(5, np.int32)
-        lib().or_mas_info(self.h, info.ctypes.data_as(i32p))
+        self._L.or_mas_info(self.h, info.ctypes.data_as(i32p))
         self.V, self.N, self.L, self.total, self.level1_begin = (int(v) for v in info)
 
     def __del__(self):
         try:
             if self.h:
-                lib().or_mas_destroy(self.h)
+                self._L.or_mas_destroy(self.h)
         except Exception:
             pass
 
     def tables(self):
-        gsz = lib().or_mas_going_next_size(self.h)
+        gsz = self._L.or_mas_going_next_size(self.h)
         p2r = np.empty(self.N, np.int32)
         r2p = np.empty(self.V, np.int32)
         gn = np.empty(gsz, np.int32)
@@ -211,7 +225,7 @@ class Mas:
         ct = np.empty(self.V * 6, np.int32)
         fc = np.empty(self.V, np.uint32)
         po = np.empty(max(self.N // 16, 1), np.int32)
-        lib().or_mas_tables(self.h, p2r.ctypes.data_as(i32p), r2p.ctypes.data_as(i32p), gn.ctypes.data_as(i32p),
+        self._L.or_mas_tables(self.h, p2r.ctypes.data_as(i32p), r2p.ctypes.data_as(i32p), gn.ctypes.data_as(i32p),
                             ls.ctypes.data_as(i32p), ct.ctypes.data_as(i32p), fc.ctypes.data_as(u32p),
                             po.ctypes.data_as(i32p))
         return {"part_to_real": p2r, "real_to_part": r2p, "going_next": gn, "level_sizes": ls.reshape(-1, 2),
@@ -220,28 +234,28 @@ class Mas:
     def assemble(self, rows, cols, vals, blk_off=0, inv_fp32=False):
         rows, cols, vals = _i(rows), _i(cols), _d(vals).reshape(-1, 9)
         self._A = (rows, cols, vals)
-        lib().or_mas_assemble(self.h, rows.size, rows.ctypes.data_as(i32p), cols.ctypes.data_as(i32p),
+        self._L.or_mas_assemble(self.h, rows.size, rows.ctypes.data_as(i32p), cols.ctypes.data_as(i32p),
                               vals.ctypes.data_as(f64p), blk_off, 1 if inv_fp32 else 0)
 
     def num_clusters(self):
-        return lib().or_mas_num_clusters(self.h)
+        return self._L.or_mas_num_clusters(self.h)
 
     def hessians(self):
         C_ = self.num_clusters()
         out = np.empty((C_, 136, 9), np.float64)
-        lib().or_mas_cluster_hessians(self.h, out.ctypes.data_as(f64p))
+        self._L.or_mas_cluster_hessians(self.h, out.ctypes.data_as(f64p))
         return out
 
     def inverses(self):
         C_ = self.num_clusters()
         out = np.empty((C_, 136, 9), np.float64)
-        lib().or_mas_cluster_inverses(self.h, out.ctypes.data_as(f64p))
+        self._L.or_mas_cluster_inverses(self.h, out.ctypes.data_as(f64p))
         return out
 
     def apply(self, r, fp32=False, active_levels=0):
         r = _d(r)
         z = np.empty_like(r)
-        lib().or_mas_apply(self.h, r.ctypes.data_as(f64p), z.ctypes.data_as(f64p), 1 if fp32 else 0, active_levels)
+        self._L.or_mas_apply(self.h, r.ctypes.data_as(f64p), z.ctypes.data_as(f64p), 1 if fp32 else 0, active_levels)
         return z
 
 

@@ -127,6 +127,148 @@ def test_hierarchy_invariants(grid, partition):
         assert all(len(b) == 1 for b in kids.values())
 
 
+def _align16(n):
+    return -(-n // 16) * 16
+
+
+def mesh_edges(V, tets=None, tris=None, edges=None):
+    """undirected vertex pairs of the simplices (FEMMASPreconditioner::do_init's
+    adjacency, fem_mas_preconditioner.cu:163-203)"""
+    pairs = [np.zeros((0, 2), np.int64)]
+    for simp, k in ((tets, 4), (tris, 3), (edges, 2)):
+        if simp is None:
+            continue
+        S = np.asarray(simp, np.int64).reshape(-1, k)
+        for a in range(k):
+            for b in range(a + 1, k):
+                pairs.append(S[:, [a, b]])
+    E = np.concatenate(pairs)
+    E = E[(E[:, 0] != E[:, 1]) & (E.min(1) >= 0) & (E.max(1) < V)]
+    return np.unique(np.sort(E, axis=1), axis=0)
+
+
+def brute_force_hierarchy(V, part_ids, E):
+    """The MAS hierarchy from its definition, independently of or_mas_create
+    (mas_preconditioner_engine.cu:57-466, 1958-2078; fem_mas_preconditioner.cu:291-356):
+      level 0: partition p owns bank_align(|p|) slots from the running offset,
+               its vertices in ascending id order;
+      level l+1: the connected components, inside each 16-node bank of level l,
+               of the graph whose edges are the mesh edges mapped to level-l
+               nodes; numbered in order of each component's lowest member
+               (= by bank, then lowest lane); level 1 begins at
+               bank_align(max(bank_align(N), V)), level l+1 at
+               begin_l + bank_align(count_l);
+      going_next: a vertex's level-1 node at its vertex id, and every level-l
+               node's level-(l+1) node at the node's slot.
+    Returns (N, L, node_abs[L + 1][V] (-1 = none), level_sizes (L + 1, 2),
+    going_next)."""
+    part_ids = np.asarray(part_ids)
+    live = np.where(part_ids >= 0)[0]
+    psize = np.bincount(part_ids[live], minlength=int(part_ids.max()) + 1)
+    poff = np.concatenate([[0], np.cumsum(-(-psize // 16) * 16)])
+    N = int(poff[-1])
+    slot = np.full(V, -1, np.int64)
+    for p in range(psize.size):
+        verts = np.where(part_ids == p)[0]  # ascending
+        slot[verts] = poff[p] + np.arange(verts.size)
+    # compute_num_levels(max(N, V)), capped at 6 levels
+    L, sz = 1, _align16(max(N, V))
+    while sz > 16:
+        sz = _align16(sz // 16)
+        L += 1
+    L = min(L + 1, 6)
+    rel = [slot]  # node index of every vertex per level, relative to the level's begin
+    begin, count = [0], [N]
+    Ep = E[(slot[E[:, 0]] >= 0) & (slot[E[:, 1]] >= 0)]
+    for l in range(L):  # level l -> l + 1 (the last one only sizes the slot total)
+        a, b = rel[l][Ep[:, 0]], rel[l][Ep[:, 1]]
+        inb = (a // 16 == b // 16) & (a != b)
+        a, b = a[inb], b[inb]
+        nodes = np.unique(rel[l][live])
+        lab = np.arange(max(int(nodes.max()) + 1 if nodes.size else 0, 1), dtype=np.int64)
+        while True:  # min-label propagation: each node ends at its component's lowest member
+            m = np.minimum(lab[a], lab[b])
+            new = lab.copy()
+            np.minimum.at(new, a, m)
+            np.minimum.at(new, b, m)
+            if np.array_equal(new, lab):
+                break
+            lab = new
+        comps = np.unique(lab[nodes])
+        nxt = np.full(V, -1, np.int64)
+        nxt[live] = np.searchsorted(comps, lab[rel[l][live]])
+        rel.append(nxt)
+        count.append(comps.size)
+        begin.append(_align16(max(_align16(N), V)) if l == 0 else begin[l] + _align16(count[l]))
+    node_abs = [np.where(r >= 0, r + bg, -1) for r, bg in zip(rel, begin)]
+    ls = np.zeros((L + 1, 2), np.int64)
+    for l in range(1, L + 1):
+        ls[l] = (count[l], begin[l])
+    gsz = _align16(max(N, V)) * L
+    gn = np.full(gsz, -1, np.int64)
+    # a fine vertex's entry sits at its vertex id (build_level1 writes
+    # going_next[part_to_real[slot]]; level1_begin >= V keeps it apart from the
+    # coarse entries), a coarse node's at its absolute slot
+    gn[live] = node_abs[1][live]
+    for l in range(1, L):
+        gn[node_abs[l][live]] = node_abs[l + 1][live]
+    return N, L, node_abs, ls, gn
+
+
+def check_hierarchy(m, t, part_ids, E):
+    """asserts the tables of or_mas_create equal the brute force"""
+    N, L, node_abs, ls, gn = brute_force_hierarchy(m.V, part_ids, E)
+    assert (m.N, m.L) == (N, L)
+    assert np.array_equal(t["real_to_part"], node_abs[0])
+    assert np.array_equal(t["level_sizes"][1:], ls[1:]), (t["level_sizes"], ls)
+    for l in range(1, L):
+        assert np.array_equal(t["coarse_tables"][:, l - 1], node_abs[l]), f"level {l}"
+    assert np.array_equal(t["going_next"][:gn.size], gn)
+    assert m.total == ls[L, 1]
+
+
+def _hierarchy_case(grid, partition, seed, simplices):
+    s, _ = system(grid, 0, seed, partition)
+    kw = {"tets": s.tets}
+    if simplices == "faces+edges":
+        T = np.asarray(s.tets).reshape(-1, 4)
+        half = T.shape[0] // 2
+        kw = {"tets": T[:half], "tris": T[half:, :3], "edges": T[half:, [0, 3]]}
+    E = mesh_edges(s.block_rows, kw.get("tets"), kw.get("tris"), kw.get("edges"))
+    return s, kw, E
+
+
+@pytest.mark.parametrize("grid,partition,simplices", [
+    (6, "geometric", "tets"), (8, "unpart", "tets"), (16, "geometric", "tets"), (16, "random", "tets"),
+    (12, "unpart", "faces+edges"), (24, "random", "tets"), (24, "geometric", "faces+edges"),
+    (40, "random", "tets")])
+def test_hierarchy_brute_force(grid, partition, simplices):
+    """every level of or_mas_create against in-bank connected components of
+    the fine edges mapped to the level below"""
+    s, kw, E = _hierarchy_case(grid, partition, 11, simplices)
+    m = O.Mas(s.block_rows, s.part_ids, **kw)
+    t = m.tables()
+    check_hierarchy(m, t, s.part_ids, E)
+    if grid >= 24:
+        assert m.L >= 5
+    if (grid, partition) == (40, "random"):
+        assert m.L == 6  # the capped level count
+    if partition == "random":
+        # the random parts are disconnected: some partitions split at level 1
+        assert t["level_sizes"][1, 0] > s.part_ids.max() + 1
+
+
+@pytest.mark.parametrize("mistake", ["NO_CLOSURE", "DROP_NEIGHBOURS", "COARSE_SELF"])
+def test_hierarchy_brute_force_detects_planted_mistakes(mistake):
+    """the brute force fails on oracle builds with a planted mistake in the
+    bank closure, the neighbour compaction or the coarse connect mask"""
+    s, kw, E = _hierarchy_case(16, "random", 11, "tets")
+    m = O.Mas(s.block_rows, s.part_ids, library=O.mutant(mistake), **kw)
+    t = m.tables()
+    with pytest.raises(AssertionError):
+        check_hierarchy(m, t, s.part_ids, E)
+
+
 def test_geometric_partition_levels():
     """2x2x4 blocks of the grid are connected: one level-1 node per part"""
     s, _ = system(16, 0, 1)
