@@ -389,10 +389,15 @@ int mi_pcg_solve(mi_sys*            sys,
  * cross them).  Every system runs FusedPCG's semantics with its own alpha,
  * beta, convergence test and cap (max_iter_ratio times its own DoFs); all of
  * them advance in the same captured iterations (one graph replay serves the
- * batch), and a system that has stopped keeps its x.  The PSD fallback is not
- * run: a system whose rz0 is not positive stops at once, converged = 0.
- * stats_host: nseg entries.  Synchronises at each block of graph_checks
- * check intervals. */
+ * batch), and a system that has stopped keeps its x.  The PSD guard runs per
+ * system, as the reference's FusedPCG does once per system
+ * (linear_fused_pcg.cu:1843-1875): with psd_fallback, a system whose rz0 < 0,
+ * or whose z is non-finite while its r is finite, is solved as plain CG
+ * (stats.bypassed = 1) while the others keep the preconditioner; force_bypass
+ * bypasses every system.  A non-finite r (a NaN in b) fails the whole call
+ * with MI_ERR_NONFINITE.  max_iter_ratio caps each system at max_iter - 1
+ * iterations, as the single-system solve does.  stats_host: nseg entries.
+ * Synchronises at each block of graph_checks check intervals. */
 int mi_pcg_solve_batched(mi_sys*            sys,
                          int32_t            nseg,
                          const int32_t*     seg_rows_host,

@@ -61,7 +61,8 @@ __global__ __launch_bounds__(kThreads) void tile_dot_kernel(const BatchTiles til
 
 // rz0, bb and the tolerances of every system
 __global__ void batch_init_kernel(const BatchTiles tiles, BatchScalars* sc, int32_t nseg, const double* rz,
-                                  const double* rr, int criterion, double tol, double max_iter_ratio)
+                                  const double* rr, int criterion, double tol, double max_iter_ratio,
+                                  const int32_t* bypass)
 {
     const int32_t s = (int32_t)((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
     if(s >= nseg)
@@ -81,9 +82,32 @@ __global__ void batch_init_kernel(const BatchTiles tiles, BatchScalars* sc, int3
         q.conv_iter              = 0;
         const int64_t n_s        = tiles.sys_lo[s + 1] - tiles.sys_lo[s];
         q.max_iter               = (int64_t)(max_iter_ratio * (double)n_s);
-        // nothing to do, or a preconditioner that is not positive on r
+        q.bypass                 = bypass ? bypass[s] : 0;
+        // nothing to do (r0.z0 = 0), or no iteration allowed: FusedPCG runs at
+        // most max_iter - 1 iterations (linear_fused_pcg.cu:1994).  A
+        // non-finite r0.z0 stops the system too; the host then bypasses its
+        // preconditioner or reports the error.
         q.converged = (z0 == 0.0) ? 1 : 0;
-        q.stopped   = (z0 == 0.0 || !(z0 > 0.0) || !isfinite(z0) || q.max_iter < 1) ? 1 : 0;
+        q.stopped   = (z0 == 0.0 || !isfinite(z0) || q.max_iter < 2) ? 1 : 0;
+    }
+}
+
+// z = r on the tiles of the systems flagged in `flags` (the solve's start), or
+// else of the running systems whose scalars carry the bypass flag
+__global__ __launch_bounds__(kThreads) void batch_bypass_z_kernel(const BatchTiles tiles,
+                                                                 const BatchScalars* __restrict__ sc,
+                                                                 const int32_t* __restrict__ flags,
+                                                                 const double* __restrict__ r,
+                                                                 double* z)
+{
+    for(int32_t t = blockIdx.x; t < tiles.n; t += gridDim.x)
+    {
+        const int32_t g = tiles.sys[t];
+        if(flags ? !flags[g] : (!sc[g].bypass || sc[g].stopped))
+            continue;
+        const int64_t lo = tiles.lo[t], hi = tiles.lo[t + 1];
+        for(int64_t i = lo + threadIdx.x; i < hi; i += kThreads)
+            z[i] = r[i];
     }
 }
 
@@ -297,9 +321,17 @@ void batch_tile_dot(const BatchTiles& t, const double* a, const double* b, doubl
 }
 
 void batch_init(const BatchTiles& t, BatchScalars* sc, int32_t nseg, const double* rz, const double* rr,
-                int criterion, double tol, double max_iter_ratio, hipStream_t s)
+                int criterion, double tol, double max_iter_ratio, const int32_t* bypass, hipStream_t s)
 {
-    batch_init_kernel<<<seg_grid(nseg), kThreads, 0, s>>>(t, sc, nseg, rz, rr, criterion, tol, max_iter_ratio);
+    batch_init_kernel<<<seg_grid(nseg), kThreads, 0, s>>>(t, sc, nseg, rz, rr, criterion, tol, max_iter_ratio,
+                                                          bypass);
+    MI_HIP(hipGetLastError());
+}
+
+void batch_bypass_z(const BatchTiles& t, const BatchScalars* sc, const int32_t* flags, const double* r, double* z,
+                    hipStream_t s)
+{
+    batch_bypass_z_kernel<<<tile_grid(t), kThreads, 0, s>>>(t, sc, flags, r, z);
     MI_HIP(hipGetLastError());
 }
 

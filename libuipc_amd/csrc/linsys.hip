@@ -109,6 +109,8 @@ struct mi_sys
         DBuf<int32_t>        seg_dev, active;
         DBuf<uint32_t>       act_flags, act_scan;
         DBuf<int64_t>        n_active;
+        DBuf<int32_t>        bypass;      // per-system PSD-guard flags of the current solve
+        bool                 any_bypass = false;
         hipGraphExec_t       graph_exec = nullptr;
         hipGraph_t           graph      = nullptr;
         GraphKey             key;
@@ -1100,10 +1102,11 @@ extern "C" int mi_pcg_solve(mi_sys* sys, const double* b, double* x, const mi_pc
     read_scalars(sys, s);
 
     // PSD guard (linear_fused_pcg.cu:1843-1875): rz0 < 0, or non-finite z with a
-    // finite r, means the preconditioner is not usable -> plain CG for this solve
+    // finite r, means the preconditioner is not usable -> plain CG for this solve.
+    // h.rr is r0.r0 (h.bb holds ||b||^2 instead when x0 is given).
     if(!bypass && o.psd_fallback && (h.rz < 0.0 || !std::isfinite(h.rz)))
     {
-        if(h.rz < 0.0 || std::isfinite(h.bb))
+        if(h.rz < 0.0 || std::isfinite(h.rr))
         {
             bypass = true;
             copy_f64(sys->z.data(), sys->r.data(), n, s);
@@ -1116,7 +1119,8 @@ extern "C" int mi_pcg_solve(mi_sys* sys, const double* b, double* x, const mi_pc
     const double rz0 = h.rz, bb = h.bb;
     st->bypassed     = bypass ? 1 : 0;
     st->rz0          = rz0;
-    MI_REQUIRE(std::isfinite(rz0) && std::isfinite(bb), MI_ERR_NONFINITE, "mi_pcg_solve: non-finite r^T z at start");
+    MI_REQUIRE(std::isfinite(rz0) && std::isfinite(bb) && std::isfinite(h.rr), MI_ERR_NONFINITE,
+               "mi_pcg_solve: non-finite r^T z at start");
     copy_f64(sys->p.data(), sys->z.data(), n, s);
     if(sys->desc.value_precision == MI_FP32)
     {
@@ -1247,6 +1251,8 @@ void run_batch_iteration(mi_sys* sys, double* x, hipStream_t s)
     batch_alpha_xr(bt.tiles, bt.sc.data(), nseg, bt.part_a.data(), x, sys->p.data(), sys->r.data(), sys->Ap.data(),
                    s);
     sys->apply_preconditioner(sys->r.data(), sys->z.data(), nullptr, s);
+    if(bt.any_bypass)
+        batch_bypass_z(bt.tiles, bt.sc.data(), nullptr, sys->r.data(), sys->z.data(), s);
     batch_tile_dot(bt.tiles, sys->r.data(), sys->z.data(), bt.part_a.data(), bt.part_b.data(), s);
     batch_beta_p(bt.tiles, bt.sc.data(), nseg, bt.part_a.data(), bt.part_b.data(), sys->p.data(), sys->z.data(), p32,
                  s);
@@ -1377,11 +1383,38 @@ extern "C" int mi_pcg_solve_batched(mi_sys*            sys,
     sys->apply_preconditioner(sys->r.data(), sys->z.data(), nullptr, s);
     batch_tile_dot(bt.tiles, sys->r.data(), sys->z.data(), bt.part_a.data(), bt.part_b.data(), s);
     batch_init(bt.tiles, bt.sc.data(), nseg, bt.part_a.data(), bt.part_b.data(), o.criterion, o.tol, o.max_iter_ratio,
-               s);
+               nullptr, s);
+    read_batch(sys, nseg, s);
+    // PSD guard per system (linear_fused_pcg.cu:1843-1875, run by the reference
+    // once per system): rz0 < 0, or a non-finite z with a finite r, solves that
+    // system as plain CG; the other systems keep their preconditioner.  A
+    // non-finite r (a NaN in b) fails the call, as check_init_rz_nan_inf does.
+    std::vector<int32_t> byp((size_t)nseg, 0);
+    bt.any_bypass = false;
+    for(int32_t g = 0; g < nseg; ++g)
+    {
+        const double z0 = bt.h_sc[g].rz, r0 = bt.h_sc[g].rr;
+        MI_REQUIRE(std::isfinite(r0), MI_ERR_NONFINITE,
+                   "mi_pcg_solve_batched: non-finite r at start of system " + std::to_string(g));
+        if(o.force_bypass || (o.psd_fallback && (z0 < 0.0 || !std::isfinite(z0))))
+            byp[g] = 1;
+        MI_REQUIRE(byp[g] || std::isfinite(z0), MI_ERR_NONFINITE,
+                   "mi_pcg_solve_batched: non-finite r^T z at start of system " + std::to_string(g));
+        bt.any_bypass = bt.any_bypass || byp[g];
+    }
+    if(bt.any_bypass)
+    {
+        bt.bypass.resize((size_t)nseg);
+        MI_HIP(hipMemcpyAsync(bt.bypass.data(), byp.data(), sizeof(int32_t) * nseg, hipMemcpyHostToDevice, s));
+        batch_bypass_z(bt.tiles, bt.sc.data(), bt.bypass.data(), sys->r.data(), sys->z.data(), s);
+        batch_tile_dot(bt.tiles, sys->r.data(), sys->z.data(), bt.part_a.data(), bt.part_b.data(), s);
+        batch_init(bt.tiles, bt.sc.data(), nseg, bt.part_a.data(), bt.part_b.data(), o.criterion, o.tol,
+                   o.max_iter_ratio, bt.bypass.data(), s);
+        read_batch(sys, nseg, s);
+    }
     copy_f64(sys->p.data(), sys->z.data(), n, s);
     if(use32)
         pack_f32x4(sys->p.data(), sys->p32.data(), sys->desc.block_rows, s);
-    read_batch(sys, nseg, s);
     std::vector<double> rz0((size_t)nseg), bb((size_t)nseg);
     int64_t             longest = 0;
     bool                all     = true;
@@ -1406,6 +1439,7 @@ extern "C" int mi_pcg_solve_batched(mi_sys*            sys,
     key.pcs_gen = sys->pcs_gen;
     key.ci      = gb;
     key.use32   = sys->desc.value_precision;
+    key.bypass  = bt.any_bypass ? 1 : 0;
     key.bufs[0] = sys->r.data();
     key.bufs[1] = sys->z.data();
     key.bufs[2] = sys->p.data();
@@ -1509,6 +1543,7 @@ extern "C" int mi_pcg_solve_batched(mi_sys*            sys,
         st[g].iters_exact = c ? q.conv_iter : -1;
         st[g].converged   = c ? 1 : 0;
         st[g].rz0         = rz0[g];
+        st[g].bypassed    = byp[g];
         st[g].rz_final    = q.rz_new;
         st[g].rel_res     = bb[g] > 0 ? std::sqrt(q.rr / bb[g]) : 0.0;
     }

@@ -308,7 +308,8 @@ struct BatchScalars
 {
     double  rz, rz_pending, rz_new, pAp, alpha, beta, rr, bb, rz_tol, rr_tol;
     int64_t iter, conv_iter, max_iter;
-    int32_t criterion, converged, stopped, pad;
+    int32_t criterion, converged, stopped;
+    int32_t bypass;  // the PSD guard took this system's preconditioner out: z = r
 };
 void batch_tile_dot(const BatchTiles& t, const double* a, const double* b, double* ab, double* aa, hipStream_t s);
 // on[k] = 1 while cluster k (slots [bank k, bank (k + 1)) of part_to_real,
@@ -320,8 +321,13 @@ void batch_cluster_flags(const int32_t* part_to_real, int64_t clusters, int bank
 void batch_active_chunks(const int32_t* chunk_row, int32_t nchunks, const int32_t* seg, int32_t nseg,
                          const BatchScalars* sc, DBuf<uint32_t>& flags, DBuf<uint32_t>& scan_chunk, int32_t* list,
                          int64_t* d_count, hipStream_t s);
+// bypass: per-system flags (device, may be null = none)
 void batch_init(const BatchTiles& t, BatchScalars* sc, int32_t nseg, const double* rz, const double* rr,
-                int criterion, double tol, double max_iter_ratio, hipStream_t s);
+                int criterion, double tol, double max_iter_ratio, const int32_t* bypass, hipStream_t s);
+// z = r on the tiles of the systems flagged in `flags` (device, per system), or
+// with flags == null of the running systems whose preconditioner is bypassed
+void batch_bypass_z(const BatchTiles& t, const BatchScalars* sc, const int32_t* flags, const double* r, double* z,
+                    hipStream_t s);
 void batch_alpha_xr(const BatchTiles& t, BatchScalars* sc, int32_t nseg, const double* pap, double* x,
                     const double* p, double* r, const double* Ap, hipStream_t s);
 void batch_beta_p(const BatchTiles& t, BatchScalars* sc, int32_t nseg, const double* rz, const double* rr,

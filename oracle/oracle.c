@@ -255,8 +255,9 @@ static int inv3(const double m[9], double out[9])
     double c01 = m[7] * m[2] - m[1] * m[8];
     double c02 = m[1] * m[5] - m[4] * m[2];
     double det = m[0] * c00 + m[3] * c01 + m[6] * c02;
-    if(det == 0.0)
-        return -1;
+    /* a singular block gives non-finite entries, as the reference's analytic
+     * 3x3 inverse does (cuda_tool/eigen/analytic_inverse.h:76-92, no check on
+     * the determinant): the PCG's PSD guard then sees a non-finite z */
     double id = 1.0 / det;
     out[0]    = c00 * id;
     out[1]    = c01 * id;
@@ -267,7 +268,7 @@ static int inv3(const double m[9], double out[9])
     out[6]    = (m[3] * m[7] - m[6] * m[4]) * id;
     out[7]    = (m[6] * m[1] - m[0] * m[7]) * id;
     out[8]    = (m[0] * m[4] - m[3] * m[1]) * id;
-    return 0;
+    return det == 0.0 ? -1 : 0;
 }
 
 int or_diag3_assemble(int64_t        U,

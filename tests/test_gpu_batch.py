@@ -47,6 +47,52 @@ def test_batch_diag3_matches_single_solves(ctx):
     ls.close()
 
 
+def test_batch_psd_guard_per_system(ctx):
+    """one system of a batch has a singular diagonal block (z non-finite on a
+    finite r): the guard solves that system as plain CG, as FusedPCG run on it
+    alone would (linear_fused_pcg.cu:1843-1875); the other systems keep the
+    3x3 block Jacobi and their solutions do not change.  A max_iter of 1
+    stops every system before its first iteration, as the single solve does."""
+    from tests.test_gpu_linsys import PSD_CASES, with_decoupled_vertex
+
+    bt = synth.batch_systems(4, seed=5, dof_lo=3e3, dof_hi=1e4)
+    systems = list(bt.systems)
+    systems[2] = with_decoupled_vertex(systems[2], *PSD_CASES["singular"])
+    seg = np.concatenate([[0], np.cumsum([t.block_rows for t in systems])]).astype(np.int32)
+    rows = np.concatenate([t.rows + seg[k] for k, t in enumerate(systems)]).astype(np.int32)
+    cols = np.concatenate([t.cols + seg[k] for k, t in enumerate(systems)]).astype(np.int32)
+    vals = np.concatenate([np.asarray(t.vals).reshape(-1, 9) for t in systems])
+    rhs = np.concatenate([t.rhs for t in systems])
+    B = int(seg[-1])
+    ls = LinearSystem(ctx, B)
+    rd, cd, vd = ctx.to_device(rows), ctx.to_device(cols), ctx.to_device(vals)
+    ls.assemble(rd, cd, vd)
+    ls.add_preconditioner(abi.MI_PRECOND_DIAG3, 0, 3 * B)
+    ls.assemble_preconditioners()
+    x = ctx.alloc(3 * B, np.float64)
+    kw = dict(criterion=abi.MI_CRIT_REL_RESIDUAL, tol=1e-6)
+    sts = ls.solve_batched(seg, ctx.to_device(rhs), x, **kw)
+    xs = x.numpy()
+    for k, t in enumerate(systems):
+        r, c, v, _ = O.convert_sym(t.block_rows, t.rows, t.cols, t.vals)
+        D = O.diag3(r, c, v, 0, t.block_rows)
+        xo, ro = O.pcg(t.block_rows, r, c, v, t.rhs, [dict(kind=1, blk_cnt=t.block_rows, data=D)], **kw)
+        assert sts[k]["bypassed"] == ro["bypassed"] == (1 if k == 2 else 0), (k, sts[k], ro)
+        assert sts[k]["converged"] == 1 and abs(sts[k]["iters_exact"] - ro["iters_exact"]) <= 1, (k, sts[k], ro)
+        assert rel_err(xs[3 * seg[k]:3 * seg[k + 1]], xo) < 1e-6
+    # without the fallback the non-finite r0.z0 is an error
+    with pytest.raises(abi.MiError) as e:
+        ls.solve_batched(seg, ctx.to_device(rhs), x, psd_fallback=0, **kw)
+    assert e.value.code == abi.MI_ERR_NONFINITE
+    # max_iter = 1 for every system: no iteration runs, x stays 0
+    tiny = 1.5 / (3 * int(np.diff(seg).max()))
+    sts = ls.solve_batched(seg, ctx.to_device(rhs), x, max_iter_ratio=tiny, **kw)
+    for k, t in enumerate(systems):
+        assert sts[k]["converged"] == 0 and sts[k]["iters"] == int(tiny * float(3 * (seg[k + 1] - seg[k]))), (k, sts[k])
+    assert not np.any(x.numpy())
+    ls.close()
+
+
 def test_batch_mas_union(ctx):
     bt = synth.batch_systems(12, seed=3, dof_lo=3e3, dof_hi=6e4)
     ls = assembled(ctx, bt)

@@ -323,6 +323,102 @@ def test_pcg_bypass_is_plain_cg(ctx, c1):
     assert rel_err(x.numpy(), xo) < 1e-5
 
 
+class _Sys:
+    pass
+
+
+def with_decoupled_vertex(s, block, rhs3):
+    """s plus one vertex coupled to nothing: its diagonal block and rhs given"""
+    t = _Sys()
+    B = s.block_rows
+    t.block_rows, t.n = B + 1, 3 * (B + 1)
+    t.rows = np.concatenate([s.rows, [B]]).astype(np.int32)
+    t.cols = np.concatenate([s.cols, [B]]).astype(np.int32)
+    t.vals = np.concatenate([np.asarray(s.vals).reshape(-1, 9), np.asarray(block, np.float64).reshape(1, 9)])
+    t.rhs = np.concatenate([s.rhs, rhs3])
+    return t
+
+
+# the PSD guard's natural triggers (linear_fused_pcg.cu:1843-1875): a singular
+# diagonal block makes the 3x3 block Jacobi non-finite on a finite r (the
+# block's null direction has a zero rhs, so plain CG still converges); a
+# negative diagonal block with a large rhs makes r0.z0 < 0
+PSD_CASES = {
+    "singular": (np.diag([1.0, 1.0, 0.0]).reshape(-1, order="F"), [0.5, -0.25, 0.0]),
+    "negative": ((-np.eye(3)).reshape(-1), [30.0, 30.0, 30.0]),
+}
+
+
+@pytest.mark.parametrize("case", sorted(PSD_CASES))
+def test_pcg_psd_guard_natural(ctx, c1, case):
+    """the guard bypasses the preconditioner by itself: bypassed = 1 and the
+    same plain-CG iterations and solution as the oracle's FusedPCG"""
+    t = with_decoupled_vertex(c1, *PSD_CASES[case])
+    ls = assembled(ctx, t)
+    ls.add_preconditioner(abi.MI_PRECOND_DIAG3, 0, t.n)
+    ls.assemble_preconditioners()
+    x = ctx.alloc(t.n, np.float64)
+    st = ls.solve(ctx.to_device(t.rhs), x, criterion=abi.MI_CRIT_REL_RESIDUAL, tol=1e-6, true_residual=1)
+    xo, ro = oracle_pcg(t, abi.MI_CRIT_REL_RESIDUAL, 1e-6)
+    assert ro["bypassed"] == 1 and st["bypassed"] == 1
+    assert st["converged"] == 1 and ro["converged"] == 1
+    assert abs(st["iters_exact"] - ro["iters_exact"]) <= 1
+    assert st["rz0"] == pytest.approx(ro["rz0"], rel=1e-12)
+    assert rel_err(x.numpy(), xo) < 1e-6
+    # without the fallback a non-finite r0.z0 is an error (check_init_rz_nan_inf)
+    if case == "singular":
+        with pytest.raises(abi.MiError) as e:
+            ls.solve(ctx.to_device(t.rhs), x, psd_fallback=0)
+        assert e.value.code == abi.MI_ERR_NONFINITE
+    ls.close()
+
+
+def test_pcg_nonfinite_rhs_is_an_error(ctx, c1):
+    """a NaN in b makes r non-finite: no bypass, MI_ERR_NONFINITE (the
+    oracle's FusedPCG stops without iterating)"""
+    rhs = c1.rhs.copy()
+    rhs[5] = np.nan
+    ls = assembled(ctx, c1)
+    ls.add_preconditioner(abi.MI_PRECOND_DIAG3, 0, c1.n)
+    ls.assemble_preconditioners()
+    x = ctx.alloc(c1.n, np.float64)
+    with pytest.raises(abi.MiError) as e:
+        ls.solve(ctx.to_device(rhs), x)
+    assert e.value.code == abi.MI_ERR_NONFINITE
+    t = _Sys()
+    t.block_rows, t.rows, t.cols, t.vals, t.rhs = c1.block_rows, c1.rows, c1.cols, c1.vals, rhs
+    _, ro = oracle_pcg(t, abi.MI_CRIT_RZ_RATIO, 1e-3)
+    assert ro["bypassed"] == 0 and ro["converged"] == 0 and ro["iters"] == 0
+    # a NaN in a given x0 makes r0 = b - A x0 non-finite: the same error, not a bypass
+    x0 = np.zeros(c1.n)
+    x0[7] = np.nan
+    x.upload(x0)
+    with pytest.raises(abi.MiError) as e:
+        ls.solve(ctx.to_device(c1.rhs), x, x_init=1)
+    assert e.value.code == abi.MI_ERR_NONFINITE
+    ls.close()
+
+
+@pytest.mark.parametrize("use_graph", [0, 1])
+def test_pcg_iteration_cap(ctx, c1_contact, use_graph):
+    """a tiny max_iter_ratio: max_iter - 1 iterations, converged = 0 and
+    iters = max_iter (linear_fused_pcg.cu:1994), the oracle's iterate"""
+    s = c1_contact
+    ls = assembled(ctx, s)
+    ls.add_preconditioner(abi.MI_PRECOND_DIAG3, 0, s.n)
+    ls.assemble_preconditioners()
+    ratio = 12.5 / s.n  # max_iter = 12
+    x = ctx.alloc(s.n, np.float64)
+    st = ls.solve(ctx.to_device(s.rhs), x, criterion=abi.MI_CRIT_REL_RESIDUAL, tol=1e-12, max_iter_ratio=ratio,
+                  use_graph=use_graph)
+    xo, ro = oracle_pcg(s, abi.MI_CRIT_REL_RESIDUAL, 1e-12, max_iter_ratio=ratio)
+    assert ro["converged"] == 0 and ro["iters"] == 12
+    assert st["converged"] == 0 and st["iters"] == 12 and st["iters_exact"] == -1
+    assert st["rel_res"] == pytest.approx(ro["rel_res"], rel=1e-8)
+    assert rel_err(x.numpy(), xo) < 1e-10
+    ls.close()
+
+
 def test_pcg_fp32_matrix(ctx, c1_contact):
     """precision arm: FP32 matrix stream, FP64 vectors and dots"""
     s = c1_contact
