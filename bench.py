@@ -49,7 +49,6 @@ def parse():
     ap.add_argument("--tol", type=float, default=1e-4)
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-iters", type=int, default=30)
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--producers", choices=["host", "device"], default="host",
                     help="device: write the triplets with the device Hessian producers (timed apart, outside "
@@ -219,8 +218,9 @@ def main():
 
     # dominant kernel: the SpMV (+ fused x^T A x), timed with HIP events on its
     # stream.  mi_spmv_dot launches the accumulating SpMV kernel (y += A x over
-    # the full BSR, per-workgroup x^T A x partials; FP64 planes: the LDS chunk
-    # kernel with FP32 planes) and a one-workgroup partials sum.
+    # the window layout, per-workgroup x^T A x partials) and a one-workgroup
+    # partials sum.
+    lay = ls.spmv_layout()
     y = ctx.alloc(s.n, np.float64)
     d = ctx.alloc(1, np.float64)
     for _ in range(3):
@@ -232,7 +232,15 @@ def main():
     sval = 4 if prec == abi.MI_FP32 else 8
     spmv_bytes = U * (9 * sval + 4) + (s.block_rows + 1) * 4 + 2 * s.n * 8
     gbs = spmv_bytes / (ms_spmv * 1e-3) / 1e9
-    kname = "spmv_chunk_kernel<double, true>" if prec == abi.MI_FP64 else "spmv_chunk_kernel<float, true>"
+    vt = "double" if prec == abi.MI_FP64 else "float"
+    if lay["layout"] == "window":
+        kname = f"spmv_win_kernel<{vt}, true, {lay['threads']}, {lay['window_rows']}>"
+    else:
+        kname = f"spmv_chunk_kernel<{vt}, true>"
+    # bytes the layout streams per product: words + values of its blocks, the
+    # row pointers and window order, x and y
+    streamed = lay["blocks"] * (9 * sval + 4) + (s.block_rows + 1) * 4 + 2 * s.n * 8 + (
+        s.block_rows * 4 if lay["layout"] == "window" else 0)
 
     result = {
         "metric": "ms per Newton solve to 1e-4 rel. residual; PCG iters; SpMV HBM GB/s vs peak",
@@ -268,7 +276,8 @@ def main():
         "roofline": {"kernel": kname, "bound": "hbm", "achieved": round(gbs, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic(kname, args), "algorithmic_bytes": int(spmv_bytes),
-                     "avg_us": round(ms_spmv * 1e3, 2)},
+                     "avg_us": round(ms_spmv * 1e3, 2), "layout": lay["layout"], "window_rows": lay["window_rows"],
+                     "streamed_bytes": int(streamed), "streamed_GBps": round(streamed / (ms_spmv * 1e-3) / 1e9, 1)},
     }
 
     if not args.no_cpu_baseline:
@@ -643,13 +652,28 @@ def run_learned(args) -> None:
     }))
 
 
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(s, precond, args, gpu_iters):
     """The oracle (C restatement of the reference's solver logic) on this host's
-    cores: one full assembly + preconditioner build, then a bounded number of
-    PCG iterations; ms per solve extrapolated to the GPU run's iteration count."""
+    cores, the whole solve: assembly, preconditioner build, and PCG from x0 = 0
+    to the same tolerance (its own iteration count), nothing extrapolated.
+    Threads: this process's CPU share, at most 16 (the GPU box gives one GPU's
+    process 16 CPUs); the assembly and the MAS build are single-threaded
+    restatements, the PCG runs its SpMV, preconditioner and vector loops on all
+    of them."""
     from oracle import oracle as O
 
-    threads = min(os.cpu_count() or 1, 16)
+    threads = max(1, min(len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1),
+                         int(os.environ.get("OMP_NUM_THREADS", "16") or 16), 16))
     t = time.perf_counter()
     r, c, v, rp = O.convert_sym(s.block_rows, s.rows, s.cols, s.vals)
     t_asm = time.perf_counter() - t
@@ -661,16 +685,18 @@ def cpu_baseline(s, precond, args, gpu_iters):
     else:
         pc = [dict(kind=1, blk_off=0, blk_cnt=s.block_rows, data=O.diag3(r, c, v, 0, s.block_rows))]
     t_pc = time.perf_counter() - t
-    k = max(1, min(args.cpu_sample_iters, gpu_iters))
-    _, res = O.pcg(s.block_rows, r, c, v, s.rhs, pc, criterion=1, tol=args.tol, threads=threads,
-                   max_iter_cap=k)
-    it_done = res["iters_exact"] if res["converged"] else k
-    t_iter = res["seconds"] / max(1, it_done)
-    ms = 1e3 * (t_asm + t_pc + t_iter * gpu_iters)
+    t = time.perf_counter()
+    _, res = O.pcg(s.block_rows, r, c, v, s.rhs, pc, criterion=1, tol=args.tol, threads=threads)
+    t_pcg = time.perf_counter() - t
+    ms = 1e3 * (t_asm + t_pc + t_pcg)
     return {"value": round(ms, 1), "unit": "ms", "cores": threads, "kind": "port",
-            "sample": f"oracle convert_sym ({t_asm:.2f}s, 1 thread) + {precond} build ({t_pc:.2f}s, 1 thread) "
-                      f"+ {it_done} PCG iterations on {threads} threads ({t_iter * 1e3:.1f} ms/iter), "
-                      f"extrapolated to the GPU's {gpu_iters} iterations"}
+            "nproc": os.cpu_count(), "cpu_model": cpu_model(),
+            "iters": int(res["iters"]), "converged": int(res["converged"]),
+            "phase_s": {"assemble": round(t_asm, 3), "precond_build": round(t_pc, 3), "pcg": round(t_pcg, 3)},
+            "sample": f"the whole solve of the same system by the oracle: convert_sym ({t_asm:.2f}s, 1 thread) + "
+                      f"{precond} build ({t_pc:.2f}s, 1 thread) + PCG to rel. residual {args.tol:g} "
+                      f"({int(res['iters'])} iterations, {t_pcg:.2f}s on {threads} threads); GPU: {gpu_iters} "
+                      f"iterations"}
 
 
 if __name__ == "__main__":

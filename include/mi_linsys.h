@@ -111,6 +111,56 @@ int mi_sys_assemble(mi_sys*        sys,
                     const double*  vals,
                     void*          stream);
 
+/* ------------------------------------------------ library-owned triplets and gradient
+ * GlobalLinearSystem::Impl::_update_subsystem_extent (global_linear_system.cu:321-405).
+ * The subsystems report their extents in registration order.  Diagonal ones
+ * (DiagLinearSubsystem) own consecutive DoF ranges, in the order given; every
+ * subsystem owns a consecutive triplet range, in the order given (an
+ * off-diagonal one, OffDiagLinearSubsystem, holds its lr blocks, then its rl
+ * blocks).  The library keeps the triplet buffer and the gradient b; their
+ * capacity grows to reserve_ratio x the total when the total exceeds it, and
+ * the contents are discarded then (reserve_triplets_discard).  A new DoF total
+ * resizes the system (block_rows = DoFs / 3): assemble again, and re-create or
+ * re-init preconditioners whose ranges changed. */
+typedef struct mi_extent
+{
+    int32_t is_diag;        /* 1: diagonal subsystem, 0: off-diagonal coupling                   */
+    int64_t dof_count;      /* diagonal: scalar DoFs (a multiple of 3); off-diagonal: 0           */
+    int64_t block_count;    /* diagonal: triplets (DiagExtentInfo::m_block_count); off-diagonal:
+                               lr triplets (OffDiagExtentInfo::m_lr_block_count)                  */
+    int64_t rl_block_count; /* off-diagonal: rl triplets (m_rl_block_count); diagonal: 0         */
+    int32_t l, r;           /* off-diagonal: the diagonal subsystems it couples (their indices in
+                               this array); diagonal: ignored                                     */
+} mi_extent;
+
+typedef struct mi_extent_view /* per subsystem, written by mi_sys_update_extents */
+{
+    int64_t dof_offset;        /* diagonal: first scalar DoF (its gradient is b + dof_offset); else -1 */
+    int64_t triplet_offset;    /* first triplet of its range (lr part for an off-diagonal one)         */
+    int64_t rl_triplet_offset; /* off-diagonal: first rl triplet; diagonal: -1                         */
+    int32_t row_offset;        /* block offsets of the (lr) submatrix: a block (i, j) of it is written */
+    int32_t col_offset;        /* as row = row_offset + i, col = col_offset + j (rl: swapped offsets)  */
+} mi_extent_view;
+
+/* views[k] for ext[k]; *total_triplets (may be NULL) = the triplet total,
+ * *triplet_capacity (may be NULL) = the buffer's capacity after growth. */
+int mi_sys_update_extents(mi_sys*          sys,
+                          int32_t          n,
+                          const mi_extent* ext_host,
+                          mi_extent_view*  views_host,
+                          int64_t*         total_triplets,
+                          int64_t*         triplet_capacity);
+
+/* The start of GlobalLinearSystem::Impl::_assemble_linear_system
+ * (global_linear_system.cu:407-420): rows and cols of every slot = -1 (a slot
+ * no subsystem writes is dropped by the assembly's row >= 0 filter), b = 0;
+ * returns the device buffers (valid until the next mi_sys_update_extents), for
+ * the subsystems to write their slices.  Any output may be NULL. */
+int mi_sys_begin_assembly(mi_sys* sys, int32_t** rows, int32_t** cols, double** vals, double** b, void* stream);
+
+/* mi_sys_assemble of the library-owned triplet buffer (all of it). */
+int mi_sys_assemble_owned(mi_sys* sys, void* stream);
+
 /* Number of unique upper blocks of the last assembly.  Synchronises. */
 int mi_sys_nnz(mi_sys* sys, int64_t* nnz_host);
 
@@ -129,6 +179,13 @@ int mi_sys_export_bcoo(mi_sys*  sys,
  * overwritten).  As in the reference, y is accumulated into: pass a zeroed y
  * for y = A x (LinearFusedPCG zeroes Ap in its update kernel). */
 int mi_spmv_dot(mi_sys* sys, const double* x, double* y, double* d_dot, void* stream);
+
+/* Diagnostic, no counterpart in the reference: the layout the SpMV streams.
+ * layout: 1 = window layout (rows in the first MAS preconditioner's part
+ * order, windows of window_rows rows, a block inside a window stored once),
+ * 0 = full BSR (both triangles); blocks: the 3x3 blocks one product streams;
+ * threads: the SpMV kernel's workgroup size.  Any output may be NULL. */
+int mi_sys_spmv_layout(mi_sys* sys, int32_t* layout, int32_t* window_rows, int32_t* threads, int64_t* blocks);
 
 /* Spmv::rbk_sym_spmv (linear_system/spmv.cu:992-1021): y = a A x + b y. */
 int mi_spmv(mi_sys* sys, double a, const double* x, double b, double* y, void* stream);

@@ -44,6 +44,16 @@ class SysDesc(C.Structure):
     _fields_ = [("block_rows", C.c_int32), ("reserve_ratio", C.c_double), ("value_precision", C.c_int32)]
 
 
+class Extent(C.Structure):
+    _fields_ = [("is_diag", C.c_int32), ("dof_count", C.c_int64), ("block_count", C.c_int64),
+                ("rl_block_count", C.c_int64), ("l", C.c_int32), ("r", C.c_int32)]
+
+
+class ExtentView(C.Structure):
+    _fields_ = [("dof_offset", C.c_int64), ("triplet_offset", C.c_int64), ("rl_triplet_offset", C.c_int64),
+                ("row_offset", C.c_int32), ("col_offset", C.c_int32)]
+
+
 class PrecondDesc(C.Structure):
     _fields_ = [
         ("kind", C.c_int32),
@@ -144,6 +154,13 @@ LINSYS_PROTOS = {
     "mi_sys_nnz": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
     "mi_sys_export_bcoo": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "mi_spmv_dot": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "mi_sys_update_extents": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.POINTER(C.c_int64),
+                                        C.POINTER(C.c_int64)]),
+    "mi_sys_begin_assembly": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                        C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_void_p]),
+    "mi_sys_assemble_owned": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "mi_sys_spmv_layout": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                     C.POINTER(C.c_int64)]),
     "mi_spmv": (C.c_int, [C.c_void_p, C.c_double, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p]),
     "mi_precond_add": (C.c_int, [C.c_void_p, C.POINTER(PrecondDesc), C.POINTER(C.c_int)]),
     "mi_precond_reinit": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(PrecondDesc)]),

@@ -235,11 +235,14 @@ __device__ __forceinline__ int32_t system_of(const int32_t* seg, int32_t nseg, i
     return lo;
 }
 
-// a SpMV chunk is still needed while one of the systems it holds rows of runs
-__device__ __forceinline__ bool chunk_needed(const int32_t* chunk_row, int32_t c, const int32_t* seg, int32_t nseg,
-                                             const BatchScalars* sc)
+// a SpMV chunk is still needed while one of the systems whose rows lie in
+// its row range [lo[c], hi[c]) runs (the range may also hold rows of other
+// chunks: a chunk of stopped systems is then computed again, which leaves
+// its rows unchanged)
+__device__ __forceinline__ bool chunk_needed(const int32_t* lo, const int32_t* hi, int32_t c, const int32_t* seg,
+                                             int32_t nseg, const BatchScalars* sc)
 {
-    const int32_t r0 = chunk_row[c], r1 = chunk_row[c + 1];
+    const int32_t r0 = lo[c], r1 = hi[c];
     if(r1 <= r0)
         return false;
     const int32_t g1 = system_of(seg, nseg, r1 - 1);
@@ -249,18 +252,18 @@ __device__ __forceinline__ bool chunk_needed(const int32_t* chunk_row, int32_t c
     return false;
 }
 
-__global__ void chunk_flag_kernel(const int32_t* chunk_row, int32_t nchunks, const int32_t* seg, int32_t nseg,
-                                  const BatchScalars* sc, uint32_t* flags)
+__global__ void chunk_flag_kernel(const int32_t* lo, const int32_t* hi, int32_t nchunks, const int32_t* seg,
+                                  int32_t nseg, const BatchScalars* sc, uint32_t* flags)
 {
     for(int32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nchunks; c += gridDim.x * blockDim.x)
-        flags[c] = chunk_needed(chunk_row, c, seg, nseg, sc) ? 1u : 0u;
+        flags[c] = chunk_needed(lo, hi, c, seg, nseg, sc) ? 1u : 0u;
 }
 
-__global__ void chunk_list_kernel(const int32_t* chunk_row, int32_t nchunks, const int32_t* seg, int32_t nseg,
-                                  const BatchScalars* sc, const uint32_t* offs, int32_t* list)
+__global__ void chunk_list_kernel(const int32_t* lo, const int32_t* hi, int32_t nchunks, const int32_t* seg,
+                                  int32_t nseg, const BatchScalars* sc, const uint32_t* offs, int32_t* list)
 {
     for(int32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nchunks; c += gridDim.x * blockDim.x)
-        if(chunk_needed(chunk_row, c, seg, nseg, sc))
+        if(chunk_needed(lo, hi, c, seg, nseg, sc))
             list[offs[c]] = c;
 }
 
@@ -291,15 +294,15 @@ int seg_grid(int32_t nseg)
 }
 }  // namespace
 
-void batch_active_chunks(const int32_t* chunk_row, int32_t nchunks, const int32_t* seg, int32_t nseg,
+void batch_active_chunks(const int32_t* lo, const int32_t* hi, int32_t nchunks, const int32_t* seg, int32_t nseg,
                          const BatchScalars* sc, DBuf<uint32_t>& flags, DBuf<uint32_t>& scan_chunk, int32_t* list,
                          int64_t* d_count, hipStream_t s)
 {
     flags.resize((size_t)std::max(1, nchunks));
     const int g = std::max(1, std::min(ceil_div((int64_t)nchunks, kThreads), device_cu_count() * 4));
-    chunk_flag_kernel<<<g, kThreads, 0, s>>>(chunk_row, nchunks, seg, nseg, sc, flags.data());
+    chunk_flag_kernel<<<g, kThreads, 0, s>>>(lo, hi, nchunks, seg, nseg, sc, flags.data());
     exclusive_scan_u32(flags.data(), nchunks, scan_chunk, d_count, s);
-    chunk_list_kernel<<<g, kThreads, 0, s>>>(chunk_row, nchunks, seg, nseg, sc, flags.data(), list);
+    chunk_list_kernel<<<g, kThreads, 0, s>>>(lo, hi, nchunks, seg, nseg, sc, flags.data(), list);
     MI_HIP(hipGetLastError());
 }
 

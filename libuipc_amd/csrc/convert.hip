@@ -626,6 +626,123 @@ __global__ void full_upper_kernel(int64_t        U,
     }
 }
 
+// ---- window layout (BsrMatrix::w*): blocks per row position; a row holds its
+// upper blocks and the lower blocks whose other row is in another window
+__global__ void win_count_kernel(int32_t        B,
+                                 int            shift,
+                                 const int32_t* __restrict__ pos,
+                                 const int32_t* __restrict__ real,
+                                 const int32_t* __restrict__ row_ptr,
+                                 const int32_t* __restrict__ lower_ptr,
+                                 const uint64_t* __restrict__ lkeys,
+                                 uint32_t*      cnt)
+{
+    for(int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p <= B; p += (int64_t)gridDim.x * blockDim.x)
+    {
+        if(p == B)
+        {
+            cnt[p] = 0;
+            continue;
+        }
+        const int32_t v  = real[p];
+        const int32_t ch = (int32_t)p >> shift;
+        uint32_t      c  = (uint32_t)(row_ptr[v + 1] - row_ptr[v]);
+        for(int32_t k = lower_ptr[v]; k < lower_ptr[v + 1]; ++k)
+        {
+            const int32_t i = (int32_t)(lkeys[k] - (uint64_t)v * (uint64_t)B);  // the upper row
+            c += (pos[i] >> shift) != ch ? 1u : 0u;
+        }
+        cnt[p] = c;
+    }
+}
+
+__host__ __device__ __forceinline__ int32_t win_inner_word(int32_t li, int32_t lj)
+{
+    return (int32_t)(0x80000000u | ((uint32_t)li << 15) | (uint32_t)lj);
+}
+
+template <typename VT>
+__global__ void win_upper_kernel(int64_t        U,
+                                 int            shift,
+                                 const int32_t* __restrict__ rows,
+                                 const int32_t* __restrict__ cols,
+                                 const int32_t* __restrict__ row_ptr,
+                                 const int32_t* __restrict__ pos,
+                                 const int32_t* __restrict__ wptr,
+                                 const double* __restrict__ uvals,
+                                 int32_t*       words,
+                                 VT*            wvals,
+                                 int64_t        stride)
+{
+    const int32_t mask = (1 << shift) - 1;
+    for(int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < U; u += (int64_t)gridDim.x * blockDim.x)
+    {
+        const int32_t v = rows[u], j = cols[u];
+        const int32_t p = pos[v], pj = pos[j];
+        const int64_t d = (int64_t)wptr[p] + (u - row_ptr[v]);
+        words[d]        = (p >> shift) == (pj >> shift) ? win_inner_word(p & mask, pj & mask) : j;
+#pragma unroll
+        for(int q = 0; q < 9; ++q)
+            wvals[q * stride + d] = (VT)uvals[upper_at(u, q)];
+    }
+}
+
+template <typename VT>
+__global__ void win_lower_kernel(int32_t        B,
+                                 int            shift,
+                                 const int32_t* __restrict__ pos,
+                                 const int32_t* __restrict__ real,
+                                 const int32_t* __restrict__ row_ptr,
+                                 const int32_t* __restrict__ lower_ptr,
+                                 const uint64_t* __restrict__ lkeys,
+                                 const int32_t* __restrict__ lsrc,
+                                 const int32_t* __restrict__ wptr,
+                                 const double* __restrict__ uvals,
+                                 int32_t*       words,
+                                 VT*            wvals,
+                                 int64_t        stride)
+{
+    for(int32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < B; p += gridDim.x * blockDim.x)
+    {
+        const int32_t v  = real[p];
+        const int32_t ch = p >> shift;
+        int64_t       d  = (int64_t)wptr[p] + (row_ptr[v + 1] - row_ptr[v]);
+        for(int32_t k = lower_ptr[v]; k < lower_ptr[v + 1]; ++k)
+        {
+            const int32_t i = (int32_t)(lkeys[k] - (uint64_t)v * (uint64_t)B);
+            if((pos[i] >> shift) == ch)
+                continue;  // stored once, as the inner block of row i
+            const int64_t u = lsrc[k];
+            words[d]        = i;
+#pragma unroll
+            for(int r = 0; r < 3; ++r)
+#pragma unroll
+                for(int c = 0; c < 3; ++c)
+                    wvals[(r + 3 * c) * stride + d] = (VT)uvals[upper_at(u, c + 3 * r)];
+            ++d;
+        }
+    }
+}
+
+// lowest row and highest row + 1 of every window (batched solves skip the
+// windows that hold no row of a running system)
+__global__ void win_range_kernel(int32_t B, int shift, int32_t nchunks, const int32_t* __restrict__ real, int32_t* lo,
+                                 int32_t* hi)
+{
+    for(int32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nchunks; c += gridDim.x * blockDim.x)
+    {
+        const int32_t p0 = c << shift, p1 = (int32_t)std::min<int64_t>((int64_t)(c + 1) << shift, B);
+        int32_t       a = B, b = -1;
+        for(int32_t p = p0; p < p1; ++p)
+        {
+            a = std::min(a, real[p]);
+            b = std::max(b, real[p]);
+        }
+        lo[c] = a;
+        hi[c] = b + 1;
+    }
+}
+
 int key_bits_for(uint64_t max_key)
 {
     int bits = 0;
@@ -691,6 +808,71 @@ int MatrixConverter::compact_sort(int64_t        n,
                             hist_, chunk_, s);
 }
 
+void MatrixConverter::build_window(BsrMatrix& A, int shift, bool want32, double ratio, hipStream_t s)
+{
+    MI_REQUIRE(shift >= 4 && shift <= kWinShiftMax, MI_ERR_ARG, "window layout: window size out of range");
+    const int32_t B = A.B;
+    MI_REQUIRE(A.wpos.data() && A.wreal.data() && A.wpos.size() >= (size_t)B && A.wreal.size() >= (size_t)B,
+               MI_ERR_STATE, "window layout: no row order");
+    A.wshift   = shift;
+    A.wnchunks = (int32_t)(((int64_t)B + (1 << shift) - 1) >> shift);
+    A.wptr.resize((size_t)B + 1, ratio);
+    A.wchunk_lo.resize((size_t)std::max(1, A.wnchunks), ratio);
+    A.wchunk_hi.resize((size_t)std::max(1, A.wnchunks), ratio);
+    A.has_win = false;
+    if(B == 0)
+    {
+        A.wblocks = 0;
+        A.has_win = true;
+        return;
+    }
+    const int64_t U = A.U;
+    uint32_t*     cnt = reinterpret_cast<uint32_t*>(A.wptr.data());  // scanned in place
+    win_count_kernel<<<grid_of(B + 1), kThreads, 0, s>>>(B, shift, A.wpos.data(), A.wreal.data(),
+                                                         A.row_ptr.data(), lower_ptr_.data(),
+                                                         U > 0 ? keys_[tcur_].data() : nullptr, cnt);
+    scan_exclusive(cnt, (int64_t)B + 1, chunk_, counts_.data() + 4, s);
+    win_range_kernel<<<grid_of(A.wnchunks), kThreads, 0, s>>>(B, shift, A.wnchunks, A.wreal.data(),
+                                                              A.wchunk_lo.data(), A.wchunk_hi.data());
+    // the window layout holds at most the full BSR's 2U blocks; one block of
+    // padding past the last, and an even plane stride
+    const int64_t wmax = 2 * U;
+    A.wwords.resize((size_t)wmax + 2, ratio);
+    A.wstride = ((int64_t)A.wwords.cap + 1) & ~(int64_t)1;
+    if(U > 0)
+    {
+        if(want32)
+        {
+            A.wvals32.resize((size_t)9 * A.wstride);
+            win_upper_kernel<float><<<grid_of(U), kThreads, 0, s>>>(U, shift, A.rows.data(), A.cols.data(),
+                                                                    A.row_ptr.data(), A.wpos.data(), A.wptr.data(),
+                                                                    A.vals.data(), A.wwords.data(), A.wvals32.data(),
+                                                                    A.wstride);
+            win_lower_kernel<float><<<grid_of(B), kThreads, 0, s>>>(
+                B, shift, A.wpos.data(), A.wreal.data(), A.row_ptr.data(), lower_ptr_.data(), keys_[tcur_].data(),
+                srcs_[tcur_].data(), A.wptr.data(), A.vals.data(), A.wwords.data(), A.wvals32.data(), A.wstride);
+        }
+        else
+        {
+            A.wvals.resize((size_t)9 * A.wstride);
+            win_upper_kernel<double><<<grid_of(U), kThreads, 0, s>>>(U, shift, A.rows.data(), A.cols.data(),
+                                                                     A.row_ptr.data(), A.wpos.data(), A.wptr.data(),
+                                                                     A.vals.data(), A.wwords.data(), A.wvals.data(),
+                                                                     A.wstride);
+            win_lower_kernel<double><<<grid_of(B), kThreads, 0, s>>>(
+                B, shift, A.wpos.data(), A.wreal.data(), A.row_ptr.data(), lower_ptr_.data(), keys_[tcur_].data(),
+                srcs_[tcur_].data(), A.wptr.data(), A.vals.data(), A.wwords.data(), A.wvals.data(), A.wstride);
+        }
+    }
+    else if(want32)
+        A.wvals32.resize((size_t)9 * A.wstride);
+    else
+        A.wvals.resize((size_t)9 * A.wstride);
+    MI_HIP(hipGetLastError());
+    A.wblocks = -1;  // read back lazily (window_blocks)
+    A.has_win = true;
+}
+
 int radix_sort_pairs(uint64_t*       k0,
                      int32_t*        v0,
                      uint64_t*       k1,
@@ -739,6 +921,7 @@ void MatrixConverter::convert_sym(int32_t        B,
                                   const double*  vals,
                                   BsrMatrix&     out,
                                   bool           want32,
+                                  bool           build_full,
                                   double         ratio,
                                   hipStream_t    s)
 {
@@ -751,13 +934,16 @@ void MatrixConverter::convert_sym(int32_t        B,
     out.has32 = want32;
     if(!pinned_)
         MI_HIP(hipHostMalloc((void**)&pinned_, 4 * sizeof(int64_t)));
-    counts_.resize(4);
-    MI_HIP(hipMemsetAsync(counts_.data(), 0, 4 * sizeof(int64_t), s));
+    counts_.resize(5);
+    MI_HIP(hipMemsetAsync(counts_.data(), 0, 5 * sizeof(int64_t), s));
+    out.has_win = false;
     if(T == 0 || B == 0)
     {
         out.U = 0;
         MI_HIP(hipMemsetAsync(out.row_ptr.data(), 0, sizeof(int32_t) * ((size_t)B + 1), s));
         MI_HIP(hipMemsetAsync(out.full_ptr.data(), 0, sizeof(int32_t) * ((size_t)B + 1), s));
+        lower_ptr_.resize((size_t)B + 1, ratio);
+        MI_HIP(hipMemsetAsync(lower_ptr_.data(), 0, sizeof(int32_t) * ((size_t)B + 1), s));
         return;
     }
     int64_t* d_m   = counts_.data();      // kept triplets
@@ -790,10 +976,17 @@ void MatrixConverter::convert_sym(int32_t        B,
 
     // full BSR: transpose of the off-diagonal upper blocks + the upper blocks
     int tcur = compact_sort<1>(U, out.rows.data(), out.cols.data(), B, d_off, ratio, s);
+    tcur_    = tcur;
     lower_rows_.resize((size_t)(U > 0 ? U : 1), ratio);
     lower_ptr_.resize((size_t)B + 1, ratio);
     unpack_lower_kernel<<<grid_of(U), kThreads, 0, s>>>(d_off, keys_[tcur].data(), B, lower_rows_.data());
     row_ptr_kernel<<<grid_of(U + 1), kThreads, 0, s>>>(d_off, U, lower_rows_.data(), lower_ptr_.data(), B);
+    out.has_win = false;
+    if(!build_full)
+    {
+        MI_HIP(hipGetLastError());
+        return;
+    }
     {
         uint32_t* cnt = reinterpret_cast<uint32_t*>(out.full_ptr.data());  // scanned in place
         full_count_kernel<<<grid_of(B + 1), kThreads, 0, s>>>(lower_ptr_.data(), out.row_ptr.data(), cnt, B);

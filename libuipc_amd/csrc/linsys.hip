@@ -58,8 +58,14 @@ struct mi_sys
     SpmvScratch                                       spmv_scr, dot_scr;
     std::vector<std::unique_ptr<LocalPreconditioner>> pcs;
     uint64_t                                          pcs_gen       = 0;  // bumped when pcs change
+    uint64_t                                          order_gen     = 0;  // bumped when a preconditioner is added or rebuilt
     bool                                              assembled     = false;
     bool                                              pcs_assembled = false;
+
+    // library-owned triplets and gradient (mi_sys_update_extents)
+    DBuf<int32_t> own_rows, own_cols;
+    DBuf<double>  own_vals, own_b;
+    int64_t       own_T = -1;  // triplets of the last reported extents (-1: none reported)
 
     // PCG state (LinearFusedPCG's r, z, p, Ap and device scalars)
     DBuf<double> r, z, p, Ap, tmp;
@@ -77,14 +83,15 @@ struct mi_sys
         int64_t     n = -1, U = -1;
         uint64_t    pcs_gen = ~0ull;
         int         ci = 0, fused = -1, bypass = -1, use32 = -1;
-        const void* bufs[12] = {};
+        const void* bufs[16] = {};
+        int64_t     wstride  = -1;
         hipStream_t stream  = nullptr;
         bool        operator==(const GraphKey& o) const
         {
             if(x != o.x || b != o.b || n != o.n || U != o.U || pcs_gen != o.pcs_gen || ci != o.ci || fused != o.fused
-               || bypass != o.bypass || use32 != o.use32 || stream != o.stream)
+               || bypass != o.bypass || use32 != o.use32 || stream != o.stream || wstride != o.wstride)
                 return false;
-            for(int k = 0; k < 12; ++k)
+            for(int k = 0; k < 16; ++k)
                 if(bufs[k] != o.bufs[k])
                     return false;
             return true;
@@ -169,6 +176,53 @@ struct mi_sys
 static hipStream_t pick(mi_ctx* ctx, void* stream)
 {
     return stream ? (hipStream_t)stream : ctx->stream;
+}
+
+// the SpMV's window layout of the assembled matrix, for the current row order:
+// the identity, permuted inside every MAS preconditioner's range by its part order
+static void build_spmv_layout(mi_sys* sys, hipStream_t s)
+{
+    auto& A = sys->A;
+    if(!spmv_use_window())
+        return;
+    const int32_t B = A.B;
+    if(A.win_order != sys->order_gen || A.wpos.size() < (size_t)std::max(B, 1))
+    {
+        std::vector<int32_t> pos((size_t)std::max(B, 1)), real((size_t)std::max(B, 1));
+        for(int32_t v = 0; v < B; ++v)
+            pos[v] = v;
+        for(auto& pc : sys->pcs)
+            if(const std::vector<int32_t>* rk = pc->spmv_rank())
+            {
+                const int64_t off = pc->dof_offset / 3;
+                MI_REQUIRE(off + (int64_t)rk->size() <= B, MI_ERR_STATE, "spmv layout: preconditioner range");
+                for(size_t l = 0; l < rk->size(); ++l)
+                    pos[off + l] = (int32_t)(off + (*rk)[l]);
+            }
+        for(int32_t v = 0; v < B; ++v)
+            real[pos[v]] = v;
+        A.wpos.resize(pos.size());
+        A.wreal.resize(real.size());
+        MI_HIP(hipMemcpyAsync(A.wpos.data(), pos.data(), sizeof(int32_t) * pos.size(), hipMemcpyHostToDevice, s));
+        MI_HIP(hipMemcpyAsync(A.wreal.data(), real.data(), sizeof(int32_t) * real.size(), hipMemcpyHostToDevice, s));
+        MI_HIP(hipStreamSynchronize(s));  // the host vectors go out of scope
+        A.win_order = sys->order_gen;
+    }
+    sys->conv.build_window(A, spmv_window_shift(), sys->desc.value_precision == MI_FP32, sys->desc.reserve_ratio, s);
+}
+
+// the matrix buffers a captured PCG graph launches with
+template <typename K>
+static void key_matrix(K& key, const BsrMatrix& A)
+{
+    key.bufs[4]  = A.full_ptr.data();
+    key.bufs[5]  = A.fcols.data();
+    key.bufs[6]  = A.has32 ? (const void*)A.fvals32.data() : (const void*)A.fvals.data();
+    key.bufs[12] = A.has_win ? (const void*)A.wptr.data() : nullptr;
+    key.bufs[13] = A.has_win ? (const void*)A.wwords.data() : nullptr;
+    key.bufs[14] = A.has_win ? (A.wvals32.data() ? (const void*)A.wvals32.data() : (const void*)A.wvals.data()) : nullptr;
+    key.bufs[15] = A.has_win ? (const void*)A.wreal.data() : nullptr;
+    key.wstride  = A.has_win ? A.wstride : -1;
 }
 
 extern "C" {
@@ -372,11 +426,120 @@ int mi_sys_assemble(mi_sys* sys, int64_t T, const int32_t* rows, const int32_t* 
                           vals,
                           sys->A,
                           sys->desc.value_precision == MI_FP32,
+                          !spmv_use_window(),
                           sys->desc.reserve_ratio,
                           s);
+    build_spmv_layout(sys, s);
     sys->assembled     = true;
     sys->pcs_assembled = false;
     MI_API_END
+}
+
+int mi_sys_update_extents(mi_sys*          sys,
+                          int32_t          n,
+                          const mi_extent* ext,
+                          mi_extent_view*  view,
+                          int64_t*         total_triplets,
+                          int64_t*         triplet_capacity)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(sys && n >= 0 && (n == 0 || (ext && view)), MI_ERR_ARG, "mi_sys_update_extents: bad argument");
+    // DoF offsets: the diagonal subsystems in order (diag_dof_offsets_counts.scan)
+    std::vector<int64_t> dof_off((size_t)n, -1);
+    int64_t              dof = 0;
+    for(int32_t k = 0; k < n; ++k)
+        if(ext[k].is_diag)
+        {
+            MI_REQUIRE(ext[k].dof_count >= 0 && ext[k].dof_count % 3 == 0 && ext[k].rl_block_count == 0,
+                       MI_ERR_ARG, "mi_sys_update_extents: a diagonal subsystem needs 3k DoFs and no rl blocks");
+            dof_off[k] = dof;
+            dof += ext[k].dof_count;
+        }
+    // triplet offsets: every subsystem in order (subsystem_triplet_offsets_counts.scan)
+    int64_t T = 0;
+    for(int32_t k = 0; k < n; ++k)
+    {
+        const mi_extent& e = ext[k];
+        MI_REQUIRE(e.block_count >= 0 && e.rl_block_count >= 0, MI_ERR_ARG,
+                   "mi_sys_update_extents: negative triplet count");
+        mi_extent_view& v = view[k];
+        v.triplet_offset  = T;
+        if(e.is_diag)
+        {
+            v.dof_offset        = dof_off[k];
+            v.rl_triplet_offset = -1;
+            v.row_offset = v.col_offset = (int32_t)(dof_off[k] / 3);
+            T += e.block_count;
+        }
+        else
+        {
+            MI_REQUIRE(e.dof_count == 0 && e.l >= 0 && e.l < n && e.r >= 0 && e.r < n && ext[e.l].is_diag
+                           && ext[e.r].is_diag,
+                       MI_ERR_ARG, "mi_sys_update_extents: an off-diagonal subsystem couples two diagonal ones");
+            v.dof_offset        = -1;
+            v.rl_triplet_offset = T + e.block_count;
+            v.row_offset        = (int32_t)(dof_off[e.l] / 3);
+            v.col_offset        = (int32_t)(dof_off[e.r] / 3);
+            T += e.block_count + e.rl_block_count;
+        }
+    }
+    MI_REQUIRE(T < ((int64_t)1 << 31), MI_ERR_ARG, "mi_sys_update_extents: more than 2^31-1 triplets");
+    MI_REQUIRE(dof / 3 < ((int64_t)1 << 31), MI_ERR_ARG, "mi_sys_update_extents: too many DoFs");
+    const int32_t B = (int32_t)(dof / 3);
+    if(B != sys->desc.block_rows)
+    {
+        MI_HIP(hipStreamSynchronize(sys->ctx->stream));
+        sys->desc.block_rows = B;
+        sys->assembled       = false;
+        sys->pcs_assembled   = false;
+        ++sys->order_gen;
+    }
+    // capacity x reserve_ratio when exceeded, contents discarded
+    const double ratio = sys->desc.reserve_ratio;
+    const size_t nt    = (size_t)std::max<int64_t>(T, 1);
+    if(nt > sys->own_rows.cap)
+        MI_HIP(hipStreamSynchronize(sys->ctx->stream));  // no launch may still use the old buffers
+    sys->own_rows.resize(nt, ratio);
+    sys->own_cols.resize(nt, ratio);
+    sys->own_vals.resize(9 * nt, ratio);
+    sys->own_b.resize((size_t)std::max<int64_t>(dof, 1), ratio);
+    sys->own_T = T;
+    if(total_triplets)
+        *total_triplets = T;
+    if(triplet_capacity)
+        *triplet_capacity = (int64_t)sys->own_rows.cap;
+    MI_API_END
+}
+
+int mi_sys_begin_assembly(mi_sys* sys, int32_t** rows, int32_t** cols, double** vals, double** b, void* stream)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(sys, MI_ERR_ARG, "mi_sys_begin_assembly: null sys");
+    MI_REQUIRE(sys->own_T >= 0, MI_ERR_STATE, "mi_sys_begin_assembly: no extents reported");
+    hipStream_t s = pick(sys->ctx, stream);
+    fill_i32(sys->own_rows.data(), -1, sys->own_T, s);
+    fill_i32(sys->own_cols.data(), -1, sys->own_T, s);
+    fill_f64(sys->own_b.data(), 0.0, sys->n(), s);
+    MI_HIP(hipGetLastError());
+    if(rows)
+        *rows = sys->own_rows.data();
+    if(cols)
+        *cols = sys->own_cols.data();
+    if(vals)
+        *vals = sys->own_vals.data();
+    if(b)
+        *b = sys->own_b.data();
+    MI_API_END
+}
+
+int mi_sys_assemble_owned(mi_sys* sys, void* stream)
+{
+    if(!sys || sys->own_T < 0)
+    {
+        set_last_error(sys ? "mi_sys_assemble_owned: no extents reported" : "mi_sys_assemble_owned: null sys");
+        return sys ? MI_ERR_STATE : MI_ERR_ARG;
+    }
+    return mi_sys_assemble(sys, sys->own_T, sys->own_rows.data(), sys->own_cols.data(), sys->own_vals.data(), stream);
 }
 
 int mi_sys_nnz(mi_sys* sys, int64_t* nnz)
@@ -481,6 +644,32 @@ int mi_spmv_dot(mi_sys* sys, const double* x, double* y, double* d_dot, void* st
     MI_API_END
 }
 
+int mi_sys_spmv_layout(mi_sys* sys, int32_t* layout, int32_t* window_rows, int32_t* threads, int64_t* blocks)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(sys, MI_ERR_ARG, "mi_sys_spmv_layout: null sys");
+    MI_REQUIRE(sys->assembled, MI_ERR_STATE, "mi_sys_spmv_layout: not assembled");
+    const auto& A = sys->A;
+    if(layout)
+        *layout = A.has_win ? 1 : 0;
+    if(window_rows)
+        *window_rows = A.has_win ? (1 << A.wshift) : 0;
+    if(threads)
+        *threads = spmv_threads(A);
+    if(blocks)
+    {
+        int32_t n = 0;
+        const int32_t* ptr = A.has_win ? A.wptr.data() : A.full_ptr.data();
+        if(A.B > 0 && ptr)
+        {
+            MI_HIP(hipStreamSynchronize(sys->ctx->stream));
+            MI_HIP(hipMemcpy(&n, ptr + A.B, sizeof(int32_t), hipMemcpyDeviceToHost));
+        }
+        *blocks = n;
+    }
+    MI_API_END
+}
+
 int mi_spmv(mi_sys* sys, double a, const double* x, double b, double* y, void* stream)
 {
     MI_API_BEGIN
@@ -523,6 +712,7 @@ int mi_precond_add(mi_sys* sys, const mi_precond_desc* d, int* handle)
     sys->pcs.push_back(std::move(pc));
     sys->pcs_assembled = false;
     ++sys->pcs_gen;
+    ++sys->order_gen;
     if(handle)
         *handle = (int)sys->pcs.size() - 1;
     MI_API_END
@@ -537,6 +727,7 @@ int mi_precond_reinit(mi_sys* sys, int handle, const mi_precond_desc* d)
     sys->pcs[handle]   = std::move(pc);
     sys->pcs_assembled = false;
     ++sys->pcs_gen;
+    ++sys->order_gen;
     MI_API_END
 }
 
@@ -557,6 +748,9 @@ int mi_precond_assemble(mi_sys* sys, void* stream)
     MI_REQUIRE(sys, MI_ERR_ARG, "null sys");
     MI_REQUIRE(sys->assembled, MI_ERR_STATE, "mi_precond_assemble: assemble the system first");
     hipStream_t s = pick(sys->ctx, stream);
+    // a preconditioner added since the assembly may change the SpMV's row order
+    if(sys->A.has_win && sys->A.win_order != sys->order_gen)
+        build_spmv_layout(sys, s);
     for(auto& pc : sys->pcs)
         pc->assemble(sys->A, s);
     sys->pcs_assembled = true;
@@ -995,7 +1189,7 @@ void run_iteration(mi_sys* sys, double* x, bool bypass, const LocalPreconditione
     const bool    use32 = sys->desc.value_precision == MI_FP32;
     double*       partA = sys->spmv_scr.partials.data();
     double*       partB = sys->dot_scr.partials.data();
-    float4*       p32   = use32 ? sys->p32.data() : nullptr;
+    float4*       p32   = use32 && !sys->A.has_win ? sys->p32.data() : nullptr;
     const int     gA    = spmv_full(sys->A, use32, sys->p.data(), p32, sys->Ap.data(), partA, &sc->converged,
                                     sys->spmv_scr, s);
     int gB;
@@ -1122,7 +1316,7 @@ extern "C" int mi_pcg_solve(mi_sys* sys, const double* b, double* x, const mi_pc
     MI_REQUIRE(std::isfinite(rz0) && std::isfinite(bb) && std::isfinite(h.rr), MI_ERR_NONFINITE,
                "mi_pcg_solve: non-finite r^T z at start");
     copy_f64(sys->p.data(), sys->z.data(), n, s);
-    if(sys->desc.value_precision == MI_FP32)
+    if(sys->desc.value_precision == MI_FP32 && !sys->A.has_win)
     {
         sys->p32.resize((size_t)sys->desc.block_rows, ratio);
         pack_f32x4(sys->p.data(), sys->p32.data(), sys->desc.block_rows, s);
@@ -1160,9 +1354,7 @@ extern "C" int mi_pcg_solve(mi_sys* sys, const double* b, double* x, const mi_pc
     key.bufs[1] = sys->z.data();
     key.bufs[2] = sys->p.data();
     key.bufs[3] = sys->Ap.data();
-    key.bufs[4] = sys->A.full_ptr.data();
-    key.bufs[5] = sys->A.fcols.data();
-    key.bufs[6] = sys->A.has32 ? (const void*)sys->A.fvals32.data() : (const void*)sys->A.fvals.data();
+    key_matrix(key, sys->A);
     key.bufs[7] = sys->pcs.empty() ? nullptr : (const void*)sys->pcs[0].get();
     key.bufs[8] = fused ? (const void*)fused->diag3_inverse() : nullptr;
     key.bufs[9] = sys->p32.data();
@@ -1243,7 +1435,7 @@ void run_batch_iteration(mi_sys* sys, double* x, hipStream_t s)
     auto&         bt    = sys->batch;
     const int32_t nseg  = (int32_t)bt.seg_rows.size() - 1;
     const bool    use32 = sys->desc.value_precision == MI_FP32;
-    float4*       p32   = use32 ? sys->p32.data() : nullptr;
+    float4*       p32   = use32 && !sys->A.has_win ? sys->p32.data() : nullptr;
     // rows of stopped systems are skipped: their Ap is never read again
     spmv_full(sys->A, use32, sys->p.data(), p32, sys->Ap.data(), sys->spmv_scr.partials.data(), nullptr,
               sys->spmv_scr, s, bt.active.data(), bt.n_active.data());
@@ -1374,7 +1566,7 @@ extern "C" int mi_pcg_solve_batched(mi_sys*            sys,
     sys->p.resize((size_t)n, ratio);
     sys->Ap.resize((size_t)n, ratio);
     const bool use32 = sys->desc.value_precision == MI_FP32;
-    if(use32)
+    if(use32 && !sys->A.has_win)
         sys->p32.resize((size_t)sys->desc.block_rows, ratio);
     spmv_prepare(sys->spmv_scr, use32);
 
@@ -1413,7 +1605,7 @@ extern "C" int mi_pcg_solve_batched(mi_sys*            sys,
         read_batch(sys, nseg, s);
     }
     copy_f64(sys->p.data(), sys->z.data(), n, s);
-    if(use32)
+    if(use32 && !sys->A.has_win)
         pack_f32x4(sys->p.data(), sys->p32.data(), sys->desc.block_rows, s);
     std::vector<double> rz0((size_t)nseg), bb((size_t)nseg);
     int64_t             longest = 0;
@@ -1444,21 +1636,20 @@ extern "C" int mi_pcg_solve_batched(mi_sys*            sys,
     key.bufs[1] = sys->z.data();
     key.bufs[2] = sys->p.data();
     key.bufs[3] = sys->Ap.data();
-    key.bufs[4] = sys->A.full_ptr.data();
-    key.bufs[5] = sys->A.fcols.data();
-    key.bufs[6] = sys->A.has32 ? (const void*)sys->A.fvals32.data() : (const void*)sys->A.fvals.data();
+    key_matrix(key, sys->A);
     key.bufs[7] = bt.sc.data();
     key.bufs[8] = bt.part_a.data();
     key.bufs[9] = sys->p32.data();
     // the SpMV's chunk list, rebuilt (outside the graph) whenever a system stops
-    bt.active.resize((size_t)std::max(1, sys->A.nchunks));
+    bt.active.resize((size_t)std::max(1, spmv_items(sys->A)));
     bt.n_active.resize(1);
     int32_t n_stopped = -1;
     auto    refresh   = [&](int32_t stopped_now)
     {
         if(stopped_now == n_stopped)
             return;
-        batch_active_chunks(sys->A.chunk_row.data(), sys->A.nchunks, bt.seg_dev.data(), nseg, bt.sc.data(),
+        batch_active_chunks(spmv_item_lo(sys->A), spmv_item_hi(sys->A), spmv_items(sys->A), bt.seg_dev.data(), nseg,
+                            bt.sc.data(),
                             bt.act_flags, bt.act_scan, bt.active.data(), bt.n_active.data(), s);
         for(auto& pc : sys->pcs)
             pc->skip_stopped(bt.seg_dev.data(), nseg, bt.sc.data(), s);

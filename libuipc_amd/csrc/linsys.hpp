@@ -44,14 +44,40 @@ struct BsrMatrix
     // whose first full-BSR block lies in [c kChunkStep, (c + 1) kChunkStep)
     DBuf<int32_t> chunk_row;  // nchunks + 1
     int32_t       nchunks = 0;
+
+    // Window layout (the SpMV's stream).  The block rows are put in an order
+    // of locality (wpos / wreal; the MAS partition's part order when a MAS
+    // preconditioner covers them) and cut into windows of 2^wshift
+    // consecutive positions.  An upper block whose two rows fall in one
+    // window is stored once ("inner": word = bit 31 | li << 15 | lj, the two
+    // rows' offsets in the window) and the SpMV adds both A x_j to y_i and
+    // A^T x_i to y_j inside the window; a block whose rows fall in different
+    // windows is stored with each row ("outer": word = the other row, values
+    // transposed for the lower copy).  Rows are listed by position (wptr);
+    // a row holds its upper blocks in column order, then its outer lower
+    // blocks in row order.  Values as 9 planes of stride wstride.
+    DBuf<int32_t> wpos, wreal;  // B: position of every row, row at every position
+    DBuf<int32_t> wptr;         // B + 1, over positions
+    DBuf<int32_t> wwords;
+    DBuf<double>  wvals;    // 9 * wstride (FP64 systems)
+    DBuf<float>   wvals32;  // 9 * wstride (FP32 systems)
+    DBuf<int32_t> wchunk_lo, wchunk_hi;  // per window: lowest row and highest row + 1
+    int64_t       wstride   = 0;
+    int64_t       wblocks   = 0;  // host copy, valid after the build
+    int32_t       wshift    = 10;
+    int32_t       wnchunks  = 0;
+    bool          has_win   = false;
+    uint64_t      win_order = ~0ull;  // the row order the layout was built for
 };
 constexpr int kChunkStep = 512;  // nominal blocks per SpMV chunk
+constexpr int kWinShiftMax = 15;  // window offsets are 15-bit fields of the inner words
 
 // ----------------------------------------------------------------- converter
 // MatrixConverter<Float,3>::convert_sym (algorithm/details/matrix_converter.inl:1549-1699)
 class MatrixConverter
 {
   public:
+    // build_full: also the full BSR (the SpMV's other layout)
     void convert_sym(int32_t        B,
                      int64_t        T,
                      const int32_t* rows,
@@ -59,9 +85,13 @@ class MatrixConverter
                      const double*  vals,
                      BsrMatrix&     out,
                      bool           want32,
+                     bool           build_full,
                      double         reserve_ratio,
                      hipStream_t    s);
     ~MatrixConverter();
+    // the window layout of `A` (assembled by the last convert_sym) for the
+    // row order in A.wpos / A.wreal, windows of 2^shift rows
+    void build_window(BsrMatrix& A, int shift, bool want32, double reserve_ratio, hipStream_t s);
 
   private:
     template <int MODE>
@@ -78,7 +108,8 @@ class MatrixConverter
     DBuf<uint32_t> hist_, chunk_;
     DBuf<uint32_t> tile_cnt_;
     DBuf<int32_t>  seg_start_, lower_rows_, lower_ptr_;
-    DBuf<int64_t>  counts_;  // [0] kept triplets, [1] U, [2] off-diagonal upper blocks
+    DBuf<int64_t>  counts_;  // [0] kept triplets, [1] U, [2] off-diagonal upper blocks, [3] full, [4] window blocks
+    int            tcur_ = 0;  // buffer of the transposed (lower) sort
     int64_t*       pinned_ = nullptr;
 };
 
@@ -105,10 +136,30 @@ void exclusive_scan_u32(uint32_t* data, int64_t n, DBuf<uint32_t>& chunk, int64_
 struct SpmvScratch
 {
     DBuf<double> partials;
-    DBuf<float4> x32;  // FP32 systems: the packed copy of x when the caller has none
-    int          grid  = 0;
-    bool         use32 = false;
+    DBuf<float4> x32;  // FP32 full-BSR systems: the packed copy of x when the caller has none
+    int          grid     = 0;  // full-BSR kernel
+    int          grid_win = 0;  // window kernel
+    bool         use32    = false;
 };
+// the SpMV layout: the window layout unless MI_SPMV_LAYOUT=full selects the full BSR
+bool spmv_use_window();
+// log2 of the window rows the window kernel's launch shape takes
+int spmv_window_shift();
+// workgroup size of the SpMV kernel that streams A's layout
+int spmv_threads(const BsrMatrix& A);
+// the SpMV's work items (windows or chunks) and the row range [lo, hi) each touches
+inline int32_t spmv_items(const BsrMatrix& A)
+{
+    return A.has_win ? A.wnchunks : A.nchunks;
+}
+inline const int32_t* spmv_item_lo(const BsrMatrix& A)
+{
+    return A.has_win ? A.wchunk_lo.data() : A.chunk_row.data();
+}
+inline const int32_t* spmv_item_hi(const BsrMatrix& A)
+{
+    return A.has_win ? A.wchunk_hi.data() : A.chunk_row.data() + 1;
+}
 // allocates the SpMV's reduction scratch and fixes its launch shape for one
 // value precision (must run outside graph capture)
 void spmv_prepare(SpmvScratch& scr, bool use32);
@@ -129,6 +180,7 @@ void spmv_axpby(const BsrMatrix& A, bool use32, double a, const double* x, doubl
 
 // ----------------------------------------------------------------- vectors
 void   fill_f64(double* p, double v, int64_t n, hipStream_t s);
+void   fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t s);
 // FEM subsystem steps around the solve (subsystem.hip)
 void subsystem_fix_vertices(const int32_t* rows,
                             const int32_t* cols,
@@ -241,6 +293,10 @@ class LocalPreconditioner
     virtual int apply_dot(const double*, double*, const int32_t*, double*, int, hipStream_t) { return -1; }
     // MAS only: levels used by apply (MASPreconditionerEngine::set_active_level_num)
     virtual bool set_active_levels(int) { return false; }
+    // MAS only: the rank of every vertex of the range in the partition's part
+    // order (parts in id order, then the unpartitioned vertices): the order
+    // of locality the SpMV's window layout puts the rows in
+    virtual const std::vector<int32_t>* spmv_rank() const { return nullptr; }
     // batched solves: apply may skip the work of systems that have stopped
     // (their r no longer changes, so neither does z); seg = the systems'
     // first block rows (device); seg == nullptr switches the skipping off
@@ -317,8 +373,9 @@ void batch_tile_dot(const BatchTiles& t, const double* a, const double* b, doubl
 void batch_cluster_flags(const int32_t* part_to_real, int64_t clusters, int bank, int32_t row_off, const int32_t* seg,
                          int32_t nseg, const BatchScalars* sc, int32_t* on, hipStream_t s);
 // the SpMV chunks that hold rows of a running system, ascending, into list;
-// *d_count = their number (seg: the systems' first block rows, device)
-void batch_active_chunks(const int32_t* chunk_row, int32_t nchunks, const int32_t* seg, int32_t nseg,
+// *d_count = their number (seg: the systems' first block rows, device; chunk
+// c's rows lie in [lo[c], hi[c]))
+void batch_active_chunks(const int32_t* lo, const int32_t* hi, int32_t nchunks, const int32_t* seg, int32_t nseg,
                          const BatchScalars* sc, DBuf<uint32_t>& flags, DBuf<uint32_t>& scan_chunk, int32_t* list,
                          int64_t* d_count, hipStream_t s);
 // bypass: per-system flags (device, may be null = none)

@@ -1370,6 +1370,7 @@ class MasPreconditioner final : public LocalPreconditioner
     DBuf<double>  d_R64, d_Z64;
     DBuf<float>   d_R32, d_Z32;
     std::vector<std::pair<int64_t, int64_t>> level_clusters;  // [first, count] per level
+    std::vector<int32_t> rank_;        // part-order rank of every vertex
     DBuf<int32_t>  d_fine_on;          // per fine cluster: 1 while it holds a running system
     const int32_t* fine_on = nullptr;  // d_fine_on during a batched solve, else nullptr
 
@@ -1390,6 +1391,17 @@ class MasPreconditioner final : public LocalPreconditioner
         MI_REQUIRE(invert == MI_MAS_INVERT_MFMA || invert == MI_MAS_INVERT_SWEEP, MI_ERR_ARG,
                    "MAS: unknown inverse kernel");
         h             = on_device ? mas_build_hierarchy_device(d) : build_hierarchy(d);
+        // the part order of the vertices (the SpMV's window layout)
+        rank_.assign((size_t)h.V, -1);
+        {
+            int32_t next = 0;
+            for(int32_t s = 0; s < h.N; ++s)
+                if(h.part_to_real[s] >= 0)
+                    rank_[h.part_to_real[s]] = next++;
+            for(int32_t v = 0; v < h.V; ++v)
+                if(rank_[v] < 0)
+                    rank_[v] = next++;
+        }
         const int L   = h.L;
         C             = h.total / kBank;
         C0            = h.N / kBank;
@@ -1663,6 +1675,8 @@ class MasPreconditioner final : public LocalPreconditioner
         active_levels = levels;  // 0 or out of range: every level, as the reference clamps
         return true;
     }
+
+    const std::vector<int32_t>* spmv_rank() const override { return &rank_; }
 
     // fine clusters of stopped systems are skipped; the restriction and the
     // coarse levels (a few percent of the apply) still run over everything

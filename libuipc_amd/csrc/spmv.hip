@@ -319,6 +319,179 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) v
     }
 }
 
+// ---- the window-layout SpMV (BsrMatrix::w*): one window of KW consecutive
+// row positions per work item.  The window's x rows are staged in LDS and its
+// y rows accumulate in LDS:
+//   * products: consecutive lanes on consecutive blocks; an inner block adds
+//     A^T x_i into the LDS row j (LDS FP64 atomics) and stages A x_j; an outer
+//     block stages A x_j with x_j gathered from global memory;
+//   * row sums: the staged products of each row meeting the stage, 16 lanes
+//     per row in a fixed order, added to the row's LDS entry;
+//   * the window's rows are written once, through wreal, with x . (A x).
+template <typename VT>
+struct WinStage
+{
+    static constexpr int kPer = sizeof(VT) == 4 ? 6 : 3;  // blocks per lane and stage
+    using W                   = std::conditional_t<sizeof(VT) == 4, float, double>;
+};
+
+template <typename VT, bool ACCUM, int NT, int KW>
+__global__ __launch_bounds__(NT) void spmv_win_kernel(int32_t nchunks,
+                                                      int32_t B,
+                                                      const int32_t* __restrict__ wptr,
+                                                      const int32_t* __restrict__ words,
+                                                      const VT* __restrict__ vals,
+                                                      int64_t stride,
+                                                      const int32_t* __restrict__ real,
+                                                      const double* __restrict__ x,
+                                                      double*       y,
+                                                      double        a,
+                                                      double*       partials,
+                                                      const int32_t* d_conv,
+                                                      const int32_t* __restrict__ active,
+                                                      const int64_t* d_nactive)
+{
+    if(d_conv && *d_conv != 0)
+        return;
+    using W                   = typename WinStage<VT>::W;
+    constexpr int kPer        = WinStage<VT>::kPer;
+    constexpr int kStage      = kPer * NT;
+    constexpr int kGroupsW    = NT / kLanes;
+    __shared__ double  ys[3][KW];
+    __shared__ double  xs[3][KW];
+    __shared__ int32_t rp[KW + 1];
+    __shared__ W       w[3][kStage];
+    __shared__ double  red[NT / kWave];
+
+    const int G   = gridDim.x;
+    const int b   = blockIdx.x;
+    const int per = G / kXcds;
+    const int lb  = (per > 0 && b < per * kXcds) ? (b % kXcds) * per + b / kXcds : b;
+    const int64_t nitems = active ? *d_nactive : (int64_t)nchunks;
+    const int     a0     = (int)(nitems * lb / G);
+    const int     a1     = (int)(nitems * (lb + 1) / G);
+
+    const int tid  = threadIdx.x;
+    const int g    = tid / kLanes;
+    const int lane = tid % kLanes;
+    double    dot  = 0.0;
+    for(int it = a0; it < a1; ++it)
+    {
+        const int     c  = active ? active[it] : it;
+        const int32_t R0 = c * KW;
+        const int     nr = (int)std::min<int64_t>(KW, (int64_t)B - R0);
+        for(int l = tid; l < nr; l += NT)
+        {
+            const int64_t v = real[R0 + l];
+            xs[0][l]        = x[3 * v];
+            xs[1][l]        = x[3 * v + 1];
+            xs[2][l]        = x[3 * v + 2];
+            ys[0][l] = ys[1][l] = ys[2][l] = 0.0;
+            rp[l]                          = wptr[R0 + l];
+        }
+        if(tid == 0)
+            rp[nr] = wptr[R0 + nr];
+        __syncthreads();
+        const int32_t K0 = rp[0], K1 = rp[nr];
+        for(int32_t s0 = K0; s0 < K1; s0 += kStage)
+        {
+            const int32_t s1 = s0 + kStage < K1 ? s0 + kStage : K1;
+#pragma unroll
+            for(int h = 0; h < kPer; ++h)
+            {
+                const int     i = h * NT + tid;
+                const int32_t k = s0 + i;
+                if(k < s1)
+                {
+                    const int32_t wd = stream_load<VT>(&words[k]);
+                    VT            A[9];
+#pragma unroll
+                    for(int q = 0; q < 9; ++q)
+                        A[q] = stream_load<VT>(&vals[q * stride + k]);
+                    double x0, x1, x2;
+                    if(wd < 0)
+                    {
+                        const int li = (wd >> 15) & 0x7fff, lj = wd & 0x7fff;
+                        x0 = xs[0][lj];
+                        x1 = xs[1][lj];
+                        x2 = xs[2][lj];
+                        if(li != lj)
+                        {
+                            const double u0 = xs[0][li], u1 = xs[1][li], u2 = xs[2][li];
+                            atomicAdd(&ys[0][lj], (double)A[0] * u0 + (double)A[1] * u1 + (double)A[2] * u2);
+                            atomicAdd(&ys[1][lj], (double)A[3] * u0 + (double)A[4] * u1 + (double)A[5] * u2);
+                            atomicAdd(&ys[2][lj], (double)A[6] * u0 + (double)A[7] * u1 + (double)A[8] * u2);
+                        }
+                    }
+                    else
+                    {
+                        x0 = x[3 * (int64_t)wd];
+                        x1 = x[3 * (int64_t)wd + 1];
+                        x2 = x[3 * (int64_t)wd + 2];
+                    }
+                    w[0][i] = (W)((double)A[0] * x0 + (double)A[3] * x1 + (double)A[6] * x2);
+                    w[1][i] = (W)((double)A[1] * x0 + (double)A[4] * x1 + (double)A[7] * x2);
+                    w[2][i] = (W)((double)A[2] * x0 + (double)A[5] * x1 + (double)A[8] * x2);
+                }
+            }
+            __syncthreads();
+            // the rows meeting the stage: from the one holding block s0 (the
+            // last row starting at or before it)
+            int lo = 0, hi = nr - 1;
+            while(lo < hi)
+            {
+                const int mid = (lo + hi + 1) >> 1;
+                if(rp[mid] <= s0)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            for(int row = lo + g; row < nr; row += kGroupsW)
+            {
+                const int32_t rs = rp[row];
+                if(rs >= s1)
+                    break;
+                const int32_t re = rp[row + 1];
+                const int32_t b0 = (rs > s0 ? rs : s0) - s0, b1 = (re < s1 ? re : s1) - s0;
+                double        acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
+                for(int32_t i = b0 + lane; i < b1; i += kLanes)
+                {
+                    acc0 += (double)w[0][i];
+                    acc1 += (double)w[1][i];
+                    acc2 += (double)w[2][i];
+                }
+#pragma unroll
+                for(int o = kLanes / 2; o > 0; o >>= 1)
+                {
+                    acc0 += __shfl_xor(acc0, o, kWave);
+                    acc1 += __shfl_xor(acc1, o, kWave);
+                    acc2 += __shfl_xor(acc2, o, kWave);
+                }
+                if(lane == 0 && b1 > b0)
+                {
+                    ys[0][row] += acc0;
+                    ys[1][row] += acc1;
+                    ys[2][row] += acc2;
+                }
+            }
+            __syncthreads();  // w is rewritten by the next stage
+        }
+        for(int l = tid; l < nr; l += NT)
+        {
+            const double v0 = ys[0][l], v1 = ys[1][l], v2 = ys[2][l];
+            put_row<ACCUM>(y, real[R0 + l], a, v0, v1, v2);
+            dot += xs[0][l] * v0 + xs[1][l] * v1 + xs[2][l] * v2;
+        }
+        __syncthreads();  // the next window restages xs, ys, rp
+    }
+    if(partials)
+    {
+        double bs = block_sum<NT>(dot, red);
+        if(threadIdx.x == 0)
+            partials[blockIdx.x] = bs;
+    }
+}
+
 __global__ void scale_kernel(double* y, double b, int64_t n)
 {
     for(int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -337,26 +510,140 @@ __global__ __launch_bounds__(kThreads) void sum_partials_kernel(const double* pa
         *out = acc;
 }
 
-// launches the SpMV; returns how many partials it wrote from partials[0].
-template <typename VT, bool ACCUM>
-int launch_spmv(const SpmvScratch&, int g, hipStream_t s, const BsrMatrix& A, const VT* vals, const double* x,
-                const float4* x32, double* y, double a, double* partials, const int32_t* d_conv,
-                const int32_t* active = nullptr, const int64_t* d_nactive = nullptr)
+// launch shapes of the window kernel: threads per workgroup and log2 of the
+// window rows (the layout is built for one of them)
+struct WinShape
 {
+    int threads, shift;
+};
+constexpr WinShape kWinShapes[] = {{256, 10}, {256, 9}, {512, 10}, {512, 11}, {256, 11}, {256, 8}, {1024, 10}};
+constexpr int      kNumWinShapes = (int)(sizeof(kWinShapes) / sizeof(kWinShapes[0]));
+
+int win_shape_index()
+{
+    static int idx = -1;
+    if(idx < 0)
+    {
+        idx = 0;
+        if(const char* e = std::getenv("MI_SPMV_WIN"))
+            idx = std::max(0, std::min(kNumWinShapes - 1, std::atoi(e)));
+    }
+    return idx;
+}
+
+template <typename VT, bool ACCUM>
+const void* win_kernel(int idx)
+{
+    switch(idx)
+    {
+        case 0: return (const void*)spmv_win_kernel<VT, ACCUM, 256, 1024>;
+        case 1: return (const void*)spmv_win_kernel<VT, ACCUM, 256, 512>;
+        case 2: return (const void*)spmv_win_kernel<VT, ACCUM, 512, 1024>;
+        case 3: return (const void*)spmv_win_kernel<VT, ACCUM, 512, 2048>;
+        case 4: return (const void*)spmv_win_kernel<VT, ACCUM, 256, 2048>;
+        case 5: return (const void*)spmv_win_kernel<VT, ACCUM, 256, 256>;
+        default: return (const void*)spmv_win_kernel<VT, ACCUM, 1024, 1024>;
+    }
+}
+
+template <typename VT, bool ACCUM, int NT, int KW>
+void launch_win_t(int g, hipStream_t s, const BsrMatrix& A, const VT* vals, const double* x, double* y, double a,
+                  double* partials, const int32_t* d_conv, const int32_t* active, const int64_t* d_nactive)
+{
+    spmv_win_kernel<VT, ACCUM, NT, KW><<<g, NT, 0, s>>>(A.wnchunks, A.B, A.wptr.data(), A.wwords.data(), vals,
+                                                        A.wstride, A.wreal.data(), x, y, a, partials, d_conv, active,
+                                                        d_nactive);
+}
+
+// launches the SpMV on the layout the matrix holds (the window layout when
+// built, else the full BSR); returns how many partials it wrote from partials[0]
+template <typename VT, bool ACCUM>
+int launch_spmv(int g, hipStream_t s, const BsrMatrix& A, const VT* vals, const double* x, const float4* x32,
+                double* y, double a, double* partials, const int32_t* d_conv, const int32_t* active = nullptr,
+                const int64_t* d_nactive = nullptr)
+{
+    if(A.has_win)
+    {
+        const int idx = win_shape_index();
+        MI_REQUIRE(A.wshift == kWinShapes[idx].shift, MI_ERR_STATE, "spmv: window layout built for another shape");
+        switch(idx)
+        {
+            case 0: launch_win_t<VT, ACCUM, 256, 1024>(g, s, A, vals, x, y, a, partials, d_conv, active, d_nactive); break;
+            case 1: launch_win_t<VT, ACCUM, 256, 512>(g, s, A, vals, x, y, a, partials, d_conv, active, d_nactive); break;
+            case 2: launch_win_t<VT, ACCUM, 512, 1024>(g, s, A, vals, x, y, a, partials, d_conv, active, d_nactive); break;
+            case 3: launch_win_t<VT, ACCUM, 512, 2048>(g, s, A, vals, x, y, a, partials, d_conv, active, d_nactive); break;
+            case 4: launch_win_t<VT, ACCUM, 256, 2048>(g, s, A, vals, x, y, a, partials, d_conv, active, d_nactive); break;
+            case 5: launch_win_t<VT, ACCUM, 256, 256>(g, s, A, vals, x, y, a, partials, d_conv, active, d_nactive); break;
+            default: launch_win_t<VT, ACCUM, 1024, 1024>(g, s, A, vals, x, y, a, partials, d_conv, active, d_nactive); break;
+        }
+        return g;
+    }
     spmv_chunk_kernel<VT, ACCUM><<<g, kThreads, 0, s>>>(A.nchunks, A.chunk_row.data(), A.full_ptr.data(),
                                                         A.fcols.data(), vals, A.fstride, x, x32, y, a, partials,
                                                         d_conv, active, d_nactive);
     return g;
 }
 
+template <typename VT>
+const VT* values_of(const BsrMatrix& A);
+template <>
+const double* values_of<double>(const BsrMatrix& A)
+{
+    return A.has_win ? A.wvals.data() : A.fvals.data();
+}
+template <>
+const float* values_of<float>(const BsrMatrix& A)
+{
+    return A.has_win ? A.wvals32.data() : A.fvals32.data();
+}
+
 void check_matrix(const BsrMatrix& A, bool use32)
 {
+    if(A.has_win)
+    {
+        MI_REQUIRE(A.wptr.data() && A.wwords.data() && A.wreal.data(), MI_ERR_STATE, "spmv: matrix not assembled");
+        MI_REQUIRE(use32 ? A.wvals32.data() != nullptr : A.wvals.data() != nullptr, MI_ERR_STATE,
+                   "spmv: values of this precision were not assembled");
+        return;
+    }
     MI_REQUIRE(A.full_ptr.data() && A.fcols.data() && A.chunk_row.data() && A.nchunks > 0, MI_ERR_STATE,
                "spmv: matrix not assembled");
     MI_REQUIRE(use32 ? A.has32 : (A.fvals.data() != nullptr), MI_ERR_STATE,
                "spmv: values of this precision were not assembled");
 }
+
+// workgroups per CU that fit at once, for each layout's kernel
+int occupancy_per_cu(bool win, bool use32)
+{
+    const void* fn = win ? (use32 ? win_kernel<float, false>(win_shape_index()) : win_kernel<double, false>(win_shape_index()))
+                         : (use32 ? (const void*)spmv_chunk_kernel<float, false> : (const void*)spmv_chunk_kernel<double, false>);
+    const int   nt = win ? kWinShapes[win_shape_index()].threads : kThreads;
+    int         per_cu = 0;
+    MI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, nt, 0));
+    return std::max(1, per_cu);
+}
 }  // namespace
+
+int spmv_window_shift()
+{
+    return kWinShapes[win_shape_index()].shift;
+}
+
+int spmv_threads(const BsrMatrix& A)
+{
+    return A.has_win ? kWinShapes[win_shape_index()].threads : kThreads;
+}
+
+bool spmv_use_window()
+{
+    static int v = -1;
+    if(v < 0)
+    {
+        const char* e = std::getenv("MI_SPMV_LAYOUT");
+        v             = (e && std::string(e) == "full") ? 0 : 1;
+    }
+    return v == 1;
+}
 
 void spmv_prepare(SpmvScratch& scr, bool use32)
 {
@@ -364,26 +651,29 @@ void spmv_prepare(SpmvScratch& scr, bool use32)
     {
         scr.use32 = use32;
         // a persistent grid of as many workgroups per CU as fit at once (the
-        // kernel's occupancy); MI_SPMV_BLOCKS_PER_CU overrides for sweeps
+        // kernels' occupancy, the larger of the two layouts' grids);
+        // MI_SPMV_BLOCKS_PER_CU overrides for sweeps
         int per_cu = 0;
         if(const char* e = std::getenv("MI_SPMV_BLOCKS_PER_CU"))
             per_cu = std::max(1, std::min(32, std::atoi(e)));
-        if(per_cu == 0)
+        int gw = 0, gf = 0;
+        for(int win = 0; win < 2; ++win)
         {
-            const void* fn = use32 ? (const void*)spmv_chunk_kernel<float, false>
-                                   : (const void*)spmv_chunk_kernel<double, false>;
-            MI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kThreads, 0));
-            per_cu = std::max(1, per_cu);
+            int pc = per_cu > 0 ? per_cu : occupancy_per_cu(win == 1, use32);
+            int g  = device_cu_count() * pc;
+            g      = std::max(kXcds, (g / kXcds) * kXcds);
+            (win ? gw : gf) = g;
         }
-        int grid = device_cu_count() * per_cu;
-        grid     = std::max(kXcds, (grid / kXcds) * kXcds);
-        scr.partials.resize((size_t)2 * grid);
-        scr.grid = grid;
+        scr.grid_win = gw;
+        scr.grid     = gf;
+        scr.partials.resize((size_t)2 * std::max(gw, gf));
     }
 }
 
 int spmv_grid(const SpmvScratch& scr, const BsrMatrix& A)
 {
+    if(A.has_win)
+        return (int)std::max<int64_t>(1, std::min<int64_t>(scr.grid_win, A.wnchunks));
     return (int)std::max<int64_t>(1, std::min<int64_t>(scr.grid, A.nchunks));
 }
 
@@ -410,9 +700,10 @@ int spmv_full(const BsrMatrix& A, bool use32, const double* x, const float4* x32
     MI_REQUIRE(x && y && partials, MI_ERR_STATE, "spmv_full: null buffer");
     MI_REQUIRE(partials >= scr.partials.data() && partials + g <= scr.partials.data() + scr.partials.size(),
                MI_ERR_STATE, "spmv_full: partials outside the scratch");
-    const int np = use32 ? launch_spmv<float, false>(scr, g, s, A, A.fvals32.data(), x, x32_of(A, x, x32, scr, s), y,
-                                                     1.0, partials, d_conv, active, d_nactive)
-                         : launch_spmv<double, false>(scr, g, s, A, A.fvals.data(), x, nullptr, y, 1.0, partials,
+    const int np = use32 ? launch_spmv<float, false>(g, s, A, values_of<float>(A), x,
+                                                     A.has_win ? nullptr : x32_of(A, x, x32, scr, s), y, 1.0,
+                                                     partials, d_conv, active, d_nactive)
+                         : launch_spmv<double, false>(g, s, A, values_of<double>(A), x, nullptr, y, 1.0, partials,
                                                       d_conv, active, d_nactive);
     MI_HIP(hipGetLastError());
     return np;
@@ -431,9 +722,10 @@ void spmv_accumulate_dot(const BsrMatrix& A, bool use32, double a, const double*
     check_matrix(A, use32);
     const int g = spmv_grid(scr, A);
     double*   p = d_dot ? scr.partials.data() : nullptr;
-    const int np = use32 ? launch_spmv<float, true>(scr, g, s, A, A.fvals32.data(), x, x32_of(A, x, nullptr, scr, s),
-                                                    y, a, p, nullptr)
-                         : launch_spmv<double, true>(scr, g, s, A, A.fvals.data(), x, nullptr, y, a, p, nullptr);
+    const int np = use32 ? launch_spmv<float, true>(g, s, A, values_of<float>(A), x,
+                                                    A.has_win ? nullptr : x32_of(A, x, nullptr, scr, s), y, a, p,
+                                                    nullptr)
+                         : launch_spmv<double, true>(g, s, A, values_of<double>(A), x, nullptr, y, a, p, nullptr);
     if(d_dot)
         sum_partials_kernel<<<1, kThreads, 0, s>>>(scr.partials.data(), np, d_dot);
     MI_HIP(hipGetLastError());
@@ -448,4 +740,5 @@ void spmv_axpby(const BsrMatrix& A, bool use32, double a, const double* x, doubl
     if(a != 0.0)
         spmv_accumulate_dot(A, use32, a, x, y, nullptr, scr, s);
 }
+
 }  // namespace mi

@@ -31,6 +31,11 @@ __global__ void fill_kernel(double* p, double v, int64_t n)
     for(int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         p[i] = v;
 }
+__global__ void fill_i32_kernel(int32_t* p, int32_t v, int64_t n)
+{
+    for(int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
 __global__ void copy_kernel(double* dst, const double* src, int64_t n)
 {
     for(int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -408,6 +413,11 @@ void fill_f64(double* p, double v, int64_t n, hipStream_t s)
 {
     if(n > 0)
         fill_kernel<<<vec_grid(n), kThreads, 0, s>>>(p, v, n);
+}
+void fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t s)
+{
+    if(n > 0)
+        fill_i32_kernel<<<vec_grid(n), kThreads, 0, s>>>(p, v, n);
 }
 void copy_f64(double* dst, const double* src, int64_t n, hipStream_t s)
 {

@@ -104,12 +104,18 @@ class Context:
 
 
 class DeviceArray:
-    def __init__(self, ctx: Context, shape, dtype):
+    def __init__(self, ctx: Context, shape, dtype, borrow: int | None = None):
+        """a device buffer of this shape; `borrow` = the address of memory the
+        library owns (not freed here) instead of a new allocation"""
         self.ctx = ctx
         self.shape = tuple(shape) if isinstance(shape, (tuple, list)) else (int(shape),)
         self.dtype = np.dtype(dtype)
         self.size = int(np.prod(self.shape)) if self.shape else 1
         self.nbytes = self.size * self.dtype.itemsize
+        self.owned = borrow is None
+        if borrow is not None:
+            self.ptr = C.c_void_p(borrow)
+            return
         p = C.c_void_p()
         check(ctx.lib.mi_malloc(ctx.h, C.byref(p), max(1, self.nbytes)))
         self.ptr = p
@@ -128,13 +134,13 @@ class DeviceArray:
         check(self.ctx.lib.mi_memset(self.ctx.h, self.ptr, 0, self.nbytes))
 
     def free(self) -> None:
-        if self.ptr:
+        if self.ptr and self.owned:
             check(self.ctx.lib.mi_free(self.ctx.h, self.ptr))
-            self.ptr = None
+        self.ptr = None
 
     def __del__(self):
         try:
-            if self.ptr and self.ctx.h:
+            if self.ptr and self.owned and self.ctx.h:
                 self.ctx.lib.mi_free(self.ctx.h, self.ptr)
         except Exception:
             pass
@@ -195,6 +201,41 @@ class LinearSystem:
         d = self.ctx.alloc(1, np.float64)
         check(self.lib.mi_spmv_dot(self.h, x.ptr, y.ptr, d.ptr, None))
         return float(d.numpy()[0])
+
+    # GlobalLinearSystem::Impl::_update_subsystem_extent / _assemble_linear_system
+    def update_extents(self, extents: list[dict]) -> tuple[list[dict], int, int]:
+        """extents: dicts with is_diag, dof_count, block_count, rl_block_count,
+        l, r; returns (views, total triplets, triplet capacity)"""
+        n = len(extents)
+        ext = (abi.Extent * max(1, n))()
+        for k, e in enumerate(extents):
+            ext[k] = abi.Extent(int(e.get("is_diag", 1)), int(e.get("dof_count", 0)), int(e.get("block_count", 0)),
+                                int(e.get("rl_block_count", 0)), int(e.get("l", -1)), int(e.get("r", -1)))
+        views = (abi.ExtentView * max(1, n))()
+        tot, cap = C.c_int64(), C.c_int64()
+        check(self.lib.mi_sys_update_extents(self.h, n, ext, views, C.byref(tot), C.byref(cap)))
+        out = [{f: getattr(views[k], f) for f, _ in abi.ExtentView._fields_} for k in range(n)]
+        self.B = sum(int(e.get("dof_count", 0)) for e in extents if e.get("is_diag", 1)) // 3
+        self.n = 3 * self.B
+        return out, tot.value, cap.value
+
+    def begin_assembly(self, stream=None) -> dict:
+        """fills the owned triplet rows/cols with -1 and b with 0; returns the
+        device addresses of rows, cols, vals and b"""
+        ptrs = [C.c_void_p() for _ in range(4)]
+        check(self.lib.mi_sys_begin_assembly(self.h, *(C.byref(p) for p in ptrs), stream))
+        return dict(zip(("rows", "cols", "vals", "b"), (p.value for p in ptrs)))
+
+    def assemble_owned(self, stream=None) -> None:
+        check(self.lib.mi_sys_assemble_owned(self.h, stream))
+
+    def spmv_layout(self) -> dict:
+        """the layout the SpMV streams: window (1) or full BSR (0), window rows,
+        workgroup size and blocks per product"""
+        lay, wr, nt, nb = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int64()
+        check(self.lib.mi_sys_spmv_layout(self.h, C.byref(lay), C.byref(wr), C.byref(nt), C.byref(nb)))
+        return {"layout": "window" if lay.value == 1 else "full", "window_rows": wr.value, "threads": nt.value,
+                "blocks": nb.value}
 
     def spmv(self, a: float, x: DeviceArray, b: float, y: DeviceArray) -> None:
         check(self.lib.mi_spmv(self.h, a, x.ptr, b, y.ptr, None))

@@ -144,6 +144,38 @@ def test_spmv_dot(ctx, c1_contact, precision):
 
 
 @pytest.mark.parametrize("precision", [abi.MI_FP64, abi.MI_FP32])
+@pytest.mark.parametrize("unpart", [False, True])
+def test_spmv_in_part_order(ctx, c1_contact, precision, unpart):
+    """with a MAS preconditioner attached, the SpMV's windows follow its part
+    order (rows permuted; a block inside one window stored once and applied
+    both ways there, a block across windows stored with both rows): the same
+    product as the oracle's, when the MAS is added after the assembly (the
+    layout is rebuilt by the preconditioner assembly) and on a reassembly;
+    also with every seventh vertex unpartitioned (ranked after the parts)"""
+    s = c1_contact
+    parts = s.part_ids.copy()
+    if unpart:
+        parts[::7] = -1
+    ls = assembled(ctx, s, precision)
+    ls.add_preconditioner(abi.MI_PRECOND_MAS, 0, s.n, part_ids=parts, tets=s.tets)
+    ls.assemble_preconditioners()
+    orr, oc, ov, _ = O.convert_sym(s.block_rows, s.rows, s.cols, s.vals)
+    rng = np.random.default_rng(21)
+    tol = 1e-13 if precision == abi.MI_FP64 else 1e-6
+    for reassemble in (False, True):
+        if reassemble:
+            ls.assemble(*ls._trip)
+            ls.assemble_preconditioners()
+        x = rng.standard_normal(s.n)
+        yref, dref = O.spmv_sym(s.block_rows, orr, oc, ov, x)
+        yd = ctx.alloc(s.n, np.float64)
+        d = ls.spmv_dot(ctx.to_device(x), yd)
+        assert np.abs(yd.numpy() - yref).max() <= tol * np.abs(yref).max(), reassemble
+        assert d == pytest.approx(dref, rel=tol * 10)
+    ls.close()
+
+
+@pytest.mark.parametrize("precision", [abi.MI_FP64, abi.MI_FP32])
 def test_spmv_irregular_rows(ctx, precision):
     """Rows the SpMV's 64-block windows must get right: runs of one-block rows
     (64 row starts in one window), empty rows (a zero block is stored for
