@@ -49,6 +49,8 @@ def parse():
     ap.add_argument("--tol", type=float, default=1e-4)
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp32-arm", action="store_true",
+                    help="skip the FP32 arm that the default FP64 MAS run also times")
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--producers", choices=["host", "device"], default="host",
                     help="device: write the triplets with the device Hessian producers (timed apart, outside "
@@ -280,11 +282,65 @@ def main():
                      "streamed_bytes": int(streamed), "streamed_GBps": round(streamed / (ms_spmv * 1e-3) / 1e9, 1)},
     }
 
+    if prec == abi.MI_FP64 and precond == "mas" and not args.no_fp32_arm and args.mas_levels != "auto":
+        result["fp32_arm"] = fp32_arm(ctx, s, args, rows, cols, vals, b, x.numpy(), ms_per_step, solve_kw)
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(s, precond, args, int(st["iters"]))
     print(json.dumps(result), flush=True)
     ls.close()
     ctx.close()
+
+
+def fp32_arm(ctx, s, args, rows, cols, vals, b, x64, ms64, solve_kw) -> dict:
+    """The precision arm on the same system and triplets: FP32 matrix stream
+    (window layout values) and FP32 MAS inverses / R / Z, FP64 vectors, dots
+    and scalars; timed like `value` (median of the same number of steps) at
+    the same MAS level count, then at the level count the tuner picks.
+    Reports ms, iterations, the true residual and ||x32 - x64|| / ||x64||."""
+    from libuipc_amd import abi
+    from libuipc_amd.device import LinearSystem
+    from libuipc_amd.hip_events import EventTimer
+    from libuipc_amd.tune import tune_mas_levels
+
+    stream = ctx.stream()
+    ls = LinearSystem(ctx, s.block_rows, abi.MI_FP32)
+    h = ls.add_preconditioner(abi.MI_PRECOND_MAS, 0, s.n, part_ids=s.part_ids, tets=s.tets, mas_precision=abi.MI_FP32,
+                              active_levels=args.mas_levels)
+    x = ctx.alloc(s.n, np.float64)
+
+    def step():
+        ls.assemble(rows, cols, vals, stream)
+        ls.assemble_preconditioners()
+        return ls.solve(b, x, **solve_kw)
+
+    def timed():
+        for _ in range(args.warmup):
+            step()
+        ev = EventTimer(stream)
+        ctx.synchronize()
+        times = []
+        for _ in range(args.steps):
+            ev.start()
+            step()
+            times.append(ev.stop())
+        st = ls.solve(b, x, **solve_kw, true_residual=1)
+        x32 = x.numpy()
+        ms = float(np.median(times))
+        return {"ms": round(ms, 3), "ratio_to_fp64": round(ms / ms64, 4), "pcg_iters": int(st["iters"]),
+                "rel_res_true": st["rel_res_true"],
+                "x_rel_diff_to_fp64": float(np.linalg.norm(x32 - x64) / max(np.linalg.norm(x64), 1e-300))}
+
+    same = timed()
+    same["mas_levels"] = args.mas_levels
+    log(f"fp32 arm, same levels: {same}")
+    tuned = tune_mas_levels(ls, h, b, x, **solve_kw)
+    best = timed()
+    best["mas_levels"] = tuned["best"]
+    best["tuner_table"] = {str(k): v for k, v in tuned["table"].items()}
+    log(f"fp32 arm, tuned levels: {best['ms']} ms at {tuned['best']} levels")
+    ls.close()
+    return {"same_levels": same, "tuned_levels": best,
+            "note": "FP32 matrix stream and MAS storage, FP64 vectors, dots and scalars; median of the same steps"}
 
 
 def run_producers(ctx, s, args, rows, cols, vals) -> dict:

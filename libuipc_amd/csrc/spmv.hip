@@ -328,14 +328,16 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) v
 //   * row sums: the staged products of each row meeting the stage, 16 lanes
 //     per row in a fixed order, added to the row's LDS entry;
 //   * the window's rows are written once, through wreal, with x . (A x).
-template <typename VT>
+template <typename VT, int PER>
 struct WinStage
 {
-    static constexpr int kPer = sizeof(VT) == 4 ? 6 : 3;  // blocks per lane and stage
+    // blocks per lane and stage (FP32 products are staged as floats: twice
+    // the blocks in the same LDS)
+    static constexpr int kPer = sizeof(VT) == 4 ? 2 * PER : PER;
     using W                   = std::conditional_t<sizeof(VT) == 4, float, double>;
 };
 
-template <typename VT, bool ACCUM, int NT, int KW>
+template <typename VT, bool ACCUM, int NT, int KW, int PER>
 __global__ __launch_bounds__(NT) void spmv_win_kernel(int32_t nchunks,
                                                       int32_t B,
                                                       const int32_t* __restrict__ wptr,
@@ -353,8 +355,8 @@ __global__ __launch_bounds__(NT) void spmv_win_kernel(int32_t nchunks,
 {
     if(d_conv && *d_conv != 0)
         return;
-    using W                   = typename WinStage<VT>::W;
-    constexpr int kPer        = WinStage<VT>::kPer;
+    using W                   = typename WinStage<VT, PER>::W;
+    constexpr int kPer        = WinStage<VT, PER>::kPer;
     constexpr int kStage      = kPer * NT;
     constexpr int kGroupsW    = NT / kLanes;
     __shared__ double  ys[3][KW];
@@ -510,14 +512,26 @@ __global__ __launch_bounds__(kThreads) void sum_partials_kernel(const double* pa
         *out = acc;
 }
 
-// launch shapes of the window kernel: threads per workgroup and log2 of the
-// window rows (the layout is built for one of them)
+// launch shapes of the window kernel: threads per workgroup, log2 of the
+// window rows (the layout is built for one of them) and FP64 blocks per lane
+// and stage; MI_SPMV_WIN picks one by index (the first is the default).
+// Measured on the 64^3 + 2M contact system (FP64 / FP32 planes, us per
+// accumulating product): 512 threads, 512 rows, 2 blocks 116.3 / 80.4;
+// 256, 256, 3: 119.7 / 83.9; 256, 256, 1: 128.3 / 93.2; 128, 256, 2: 162 /
+// 122; 512, 512, 1: 127 / 88; 1024, 1024, 1: 147 / 94; 256, 128, 2: 132 / 87;
+// 512, 1024, 1: 185 / 125; 256, 1024, 3: 198 / 165 (2 workgroups per CU).
+// Small windows at high occupancy beat large windows with fewer stored blocks.
 struct WinShape
 {
-    int threads, shift;
+    int threads, shift, per;
 };
-constexpr WinShape kWinShapes[] = {{256, 10}, {256, 9}, {512, 10}, {512, 11}, {256, 11}, {256, 8}, {1024, 10}};
-constexpr int      kNumWinShapes = (int)(sizeof(kWinShapes) / sizeof(kWinShapes[0]));
+#define MI_WIN_SHAPES(X)                                                                                              \
+    X(0, 512, 9, 2) X(1, 256, 8, 3) X(2, 256, 8, 1) X(3, 128, 8, 2) X(4, 256, 8, 2) X(5, 512, 9, 1) X(6, 1024, 10, 1) \
+    X(7, 256, 7, 2) X(8, 512, 10, 1)
+#define MI_WIN_ENTRY(i, nt, sh, per) {nt, sh, per},
+constexpr WinShape kWinShapes[] = {MI_WIN_SHAPES(MI_WIN_ENTRY)};
+#undef MI_WIN_ENTRY
+constexpr int kNumWinShapes = (int)(sizeof(kWinShapes) / sizeof(kWinShapes[0]));
 
 int win_shape_index()
 {
@@ -536,21 +550,19 @@ const void* win_kernel(int idx)
 {
     switch(idx)
     {
-        case 0: return (const void*)spmv_win_kernel<VT, ACCUM, 256, 1024>;
-        case 1: return (const void*)spmv_win_kernel<VT, ACCUM, 256, 512>;
-        case 2: return (const void*)spmv_win_kernel<VT, ACCUM, 512, 1024>;
-        case 3: return (const void*)spmv_win_kernel<VT, ACCUM, 512, 2048>;
-        case 4: return (const void*)spmv_win_kernel<VT, ACCUM, 256, 2048>;
-        case 5: return (const void*)spmv_win_kernel<VT, ACCUM, 256, 256>;
-        default: return (const void*)spmv_win_kernel<VT, ACCUM, 1024, 1024>;
+#define MI_WIN_FN(i, nt, sh, per) \
+    case i: return (const void*)spmv_win_kernel<VT, ACCUM, nt, (1 << sh), per>;
+        MI_WIN_SHAPES(MI_WIN_FN)
+#undef MI_WIN_FN
     }
+    return nullptr;
 }
 
-template <typename VT, bool ACCUM, int NT, int KW>
+template <typename VT, bool ACCUM, int NT, int KW, int PER>
 void launch_win_t(int g, hipStream_t s, const BsrMatrix& A, const VT* vals, const double* x, double* y, double a,
                   double* partials, const int32_t* d_conv, const int32_t* active, const int64_t* d_nactive)
 {
-    spmv_win_kernel<VT, ACCUM, NT, KW><<<g, NT, 0, s>>>(A.wnchunks, A.B, A.wptr.data(), A.wwords.data(), vals,
+    spmv_win_kernel<VT, ACCUM, NT, KW, PER><<<g, NT, 0, s>>>(A.wnchunks, A.B, A.wptr.data(), A.wwords.data(), vals,
                                                         A.wstride, A.wreal.data(), x, y, a, partials, d_conv, active,
                                                         d_nactive);
 }
@@ -568,13 +580,12 @@ int launch_spmv(int g, hipStream_t s, const BsrMatrix& A, const VT* vals, const 
         MI_REQUIRE(A.wshift == kWinShapes[idx].shift, MI_ERR_STATE, "spmv: window layout built for another shape");
         switch(idx)
         {
-            case 0: launch_win_t<VT, ACCUM, 256, 1024>(g, s, A, vals, x, y, a, partials, d_conv, active, d_nactive); break;
-            case 1: launch_win_t<VT, ACCUM, 256, 512>(g, s, A, vals, x, y, a, partials, d_conv, active, d_nactive); break;
-            case 2: launch_win_t<VT, ACCUM, 512, 1024>(g, s, A, vals, x, y, a, partials, d_conv, active, d_nactive); break;
-            case 3: launch_win_t<VT, ACCUM, 512, 2048>(g, s, A, vals, x, y, a, partials, d_conv, active, d_nactive); break;
-            case 4: launch_win_t<VT, ACCUM, 256, 2048>(g, s, A, vals, x, y, a, partials, d_conv, active, d_nactive); break;
-            case 5: launch_win_t<VT, ACCUM, 256, 256>(g, s, A, vals, x, y, a, partials, d_conv, active, d_nactive); break;
-            default: launch_win_t<VT, ACCUM, 1024, 1024>(g, s, A, vals, x, y, a, partials, d_conv, active, d_nactive); break;
+#define MI_WIN_LAUNCH(i, nt, sh, per)                                                                          \
+    case i:                                                                                                   \
+        launch_win_t<VT, ACCUM, nt, (1 << sh), per>(g, s, A, vals, x, y, a, partials, d_conv, active, d_nactive); \
+        break;
+            MI_WIN_SHAPES(MI_WIN_LAUNCH)
+#undef MI_WIN_LAUNCH
         }
         return g;
     }

@@ -125,9 +125,11 @@ def test_batch_mas_skips_stopped_systems(ctx, precision):
     """The SpMV chunks and the MAS fine clusters of stopped systems are
     skipped.  Solving the same batch (same assembled MAS) with one system's
     rhs zeroed stops that system at once, so its clusters are skipped for the
-    whole solve: every other system must reach the same solution (to
-    rounding: the SpMV's stage boundaries follow the active chunks), in the
-    same iterations, and the zeroed system must return x = 0."""
+    whole solve: every other system must reach the same solution (to the
+    solve's tolerance: the SpMV adds its in-window transposed products with
+    LDS atomics, so the rounding differs from run to run, as with the
+    reference's FP64 atomics), in the same iterations (within 2%), and the
+    zeroed system must return x = 0."""
     bt = synth.batch_systems(8, seed=5, dof_lo=3e3, dof_hi=3e4)
     ls = LinearSystem(ctx, bt.block_rows, precision)
     r, c, v = ctx.to_device(bt.rows), ctx.to_device(bt.cols), ctx.to_device(bt.vals)
@@ -144,14 +146,14 @@ def test_batch_mas_skips_stopped_systems(ctx, precision):
     rhs2[segj] = 0.0
     s2 = ls.solve_batched(bt.seg_rows, ctx.to_device(rhs2), x2, **kw)
     a, b = x1.numpy(), x2.numpy()
-    tol = 1e-9 if precision == abi.MI_FP64 else 1e-4
+    tol = 1e-5 if precision == abi.MI_FP64 else 1e-4
     for k in range(len(bt.systems)):
         seg = slice(3 * bt.seg_rows[k], 3 * bt.seg_rows[k + 1])
         if k == j:
             assert s2[k]["converged"] == 1 and s2[k]["iters_exact"] == 0 and not b[seg].any()
             continue
         assert s1[k]["converged"] == 1 and s2[k]["converged"] == 1, (k, s1[k], s2[k])
-        assert abs(s1[k]["iters_exact"] - s2[k]["iters_exact"]) <= 1, (k, s1[k], s2[k])
+        assert abs(s1[k]["iters_exact"] - s2[k]["iters_exact"]) <= max(2, s1[k]["iters_exact"] // 50), (k, s1[k], s2[k])
         assert rel_err(b[seg], a[seg]) < tol, k
     ls.close()
 

@@ -77,15 +77,19 @@ def test_owned_assembly_and_solve_match_the_oracle(ctx):
     cl = rng.integers(0, a.block_rows, n_cpl).astype(np.int32)
     cr = rng.integers(0, c.block_rows, n_cpl).astype(np.int32)
     g = rng.standard_normal((n_cpl, 3))
-    cv = 1e2 * np.einsum("ki,kj->kij", g, g).reshape(n_cpl, 9)   # lr blocks; rl = transposes (dropped: lower)
+    # a PSD coupling kappa [g; -g][g; -g]^T per pair: its diagonal blocks are
+    # written by the two diagonal subsystems, its off-diagonal ones by the
+    # coupling (lr; the rl transposes are lower blocks, dropped)
+    gg = 1e2 * np.einsum("ki,kj->kij", g, g).reshape(n_cpl, 9)
+    cv = -gg
     spare = 77
-    ext = [dict(is_diag=1, dof_count=a.n, block_count=a.n_triplets),
-           dict(is_diag=1, dof_count=c.n, block_count=c.n_triplets + spare),
+    ext = [dict(is_diag=1, dof_count=a.n, block_count=a.n_triplets + n_cpl),
+           dict(is_diag=1, dof_count=c.n, block_count=c.n_triplets + n_cpl + spare),
            dict(is_diag=0, block_count=n_cpl, rl_block_count=n_cpl, l=0, r=1)]
     ls = LinearSystem(ctx, 0)
     views, T, _ = ls.update_extents(ext)
     B = a.block_rows + c.block_rows
-    assert ls.B == B and T == a.n_triplets + c.n_triplets + spare + 2 * n_cpl
+    assert ls.B == B and T == a.n_triplets + c.n_triplets + spare + 4 * n_cpl
     bufs = ls.begin_assembly()
     # what each subsystem writes into its slice: indices + its block offsets
     host_rows = np.full(T, -1, np.int32)
@@ -99,7 +103,9 @@ def test_owned_assembly_and_solve_match_the_oracle(ctx):
         _put(ctx, bufs["vals"], 9 * off, np.asarray(v, np.float64).reshape(-1))
     va, vc, vx = views
     write(va["triplet_offset"], a.rows + va["row_offset"], a.cols + va["col_offset"], a.vals)
+    write(va["triplet_offset"] + a.n_triplets, cl + va["row_offset"], cl + va["col_offset"], gg)
     write(vc["triplet_offset"], c.rows + vc["row_offset"], c.cols + vc["col_offset"], c.vals)
+    write(vc["triplet_offset"] + c.n_triplets, cr + vc["row_offset"], cr + vc["col_offset"], gg)
     write(vx["triplet_offset"], cl + vx["row_offset"], cr + vx["col_offset"], cv)
     cvt = cv.reshape(-1, 3, 3).transpose(0, 2, 1).reshape(-1, 9)
     write(vx["rl_triplet_offset"], cr + vx["col_offset"], cl + vx["row_offset"], cvt)
@@ -122,7 +128,7 @@ def test_owned_assembly_and_solve_match_the_oracle(ctx):
     assert abs(st["iters_exact"] - ro["iters_exact"]) <= max(1, ro["iters_exact"] // 50)
     assert rel_err(x.numpy(), xo) < 1e-6
     # a DoF change resizes the system: drop the coupling and the second mesh
-    views, T, _ = ls.update_extents(ext[:1])
+    views, T, _ = ls.update_extents([dict(is_diag=1, dof_count=a.n, block_count=a.n_triplets)])
     assert ls.B == a.block_rows and T == a.n_triplets
     bufs = ls.begin_assembly()
     _put(ctx, bufs["rows"], 0, a.rows)

@@ -308,10 +308,11 @@ def test_pcg_diag3(ctx, c1_contact, criterion, tol, use_graph):
 
 @pytest.mark.parametrize("precond", ["diag3", "mas"])
 def test_pcg_initial_guess(ctx, c1_contact, precond):
-    """x_init = 1: x0 = 0 given explicitly reproduces the default solve bit
-    for bit; x0 = the solution converges at once; x0 = the solution plus a
-    perturbation needs fewer iterations than x0 = 0 and still meets
-    ||b - A x|| <= tol ||b||"""
+    """x_init = 1: x0 = 0 given explicitly reproduces the default solve (to
+    the rounding of the SpMV's LDS atomics: iterations within 2%, the same
+    solution to the tolerance); x0 = the solution converges at once; x0 = the
+    solution plus a perturbation needs fewer iterations than x0 = 0 and still
+    meets ||b - A x|| <= tol ||b||"""
     s = c1_contact
     ls = assembled(ctx, s)
     if precond == "diag3":
@@ -326,7 +327,8 @@ def test_pcg_initial_guess(ctx, c1_contact, precond):
     x_cold = x.numpy()
     x.zero()
     st = ls.solve(b, x, x_init=1, **kw)
-    assert st["iters_exact"] == cold["iters_exact"] and np.array_equal(x.numpy(), x_cold)
+    assert abs(st["iters_exact"] - cold["iters_exact"]) <= max(2, cold["iters_exact"] // 50)
+    assert st["rel_res_true"] <= 2e-6 and rel_err(x.numpy(), x_cold) < 1e-4
     # the solution of a tighter solve as x0
     xt = ctx.alloc(s.n, np.float64)
     ls.solve(b, xt, criterion=abi.MI_CRIT_REL_RESIDUAL, tol=1e-10)

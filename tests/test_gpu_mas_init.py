@@ -10,7 +10,7 @@ import pytest
 
 from libuipc_amd import abi, synth
 from libuipc_amd.abi import MiError
-from libuipc_amd.device import LinearSystem
+from libuipc_amd.device import LinearSystem, rel_err
 from oracle import oracle as O
 from tests.test_oracle_mas import random_partition, system
 
@@ -140,9 +140,10 @@ def test_reinit_after_a_topology_change(ctx):
     hf = fresh.add_preconditioner(abi.MI_PRECOND_MAS, 0, s.n, part_ids=s2parts, tets=s.tets)
     xf, stf = solved(fresh)
     assert_same(ls.mas_export(h, matrices=False), fresh.mas_export(hf, matrices=False))
-    # the coarse cluster sums use FP64 atomics, so equal to rounding only
-    assert abs(st2["iters_exact"] - stf["iters_exact"]) <= 1
-    assert np.linalg.norm(x2 - xf) <= 1e-9 * np.linalg.norm(xf)
+    # the coarse cluster sums and the SpMV's in-window transposed products use
+    # atomics, so equal to rounding only
+    assert abs(st2["iters_exact"] - stf["iters_exact"]) <= max(2, stf["iters_exact"] // 50)
+    assert np.linalg.norm(x2 - xf) <= 1e-6 * np.linalg.norm(xf)
     assert not np.array_equal(x1, x2)
     ls.close()
     fresh.close()
@@ -159,17 +160,22 @@ def test_adding_a_preconditioner_after_a_solve(ctx):
         ls.assemble(rows, cols, vals)
         ls.assemble_preconditioners()
         x = ctx.alloc(s.n, np.float64)
-        ls.solve(b, x, criterion=abi.MI_CRIT_REL_RESIDUAL, tol=1e-8)
-        return x.numpy()
+        st = ls.solve(b, x, criterion=abi.MI_CRIT_REL_RESIDUAL, tol=1e-8)
+        return x.numpy(), st["iters_exact"]
 
     ls = LinearSystem(ctx, s.block_rows)
     ls.add_preconditioner(abi.MI_PRECOND_DIAG3, 0, half)
-    run(ls)
+    _, it1 = run(ls)
     ls.add_preconditioner(abi.MI_PRECOND_DIAG3, half, s.n - half)
-    x = run(ls)
+    x, it2 = run(ls)
     fresh = LinearSystem(ctx, s.block_rows)
     fresh.add_preconditioner(abi.MI_PRECOND_DIAG3, 0, half)
     fresh.add_preconditioner(abi.MI_PRECOND_DIAG3, half, s.n - half)
-    assert np.array_equal(x, run(fresh))
+    xf, itf = run(fresh)
+    # the same solve as a fresh system with both (to the rounding of the
+    # SpMV's LDS atomics), not a replay of the one-preconditioner solve
+    tol_it = max(2, itf // 50)
+    assert abs(it2 - itf) <= tol_it and abs(it1 - itf) > tol_it, (it1, it2, itf)
+    assert rel_err(x, xf) < 1e-6
     ls.close()
     fresh.close()
