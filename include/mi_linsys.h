@@ -180,6 +180,18 @@ int mi_sys_export_bcoo(mi_sys*  sys,
  * for y = A x (LinearFusedPCG zeroes Ap in its update kernel). */
 int mi_spmv_dot(mi_sys* sys, const double* x, double* y, double* d_dot, void* stream);
 
+/* No counterpart in the reference: mi_spmv_dot for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same operation and results as
+ * mi_spmv_dot, but its workgroup partials (and, for FP32 value planes on the
+ * full-BSR layout, MI_SPMV_LAYOUT=full, the float4 copy of x) live in a fixed
+ * device array of 4 MiB + 64 KiB per GPU, set up by mi_ctx_create; it calls
+ * no allocator.  On the full-BSR layout FP32 systems are limited to 262144
+ * block rows (MI_ERR_ARG beyond); the default window layout has no limit.
+ * All calls on one GPU share that array: calls that overlap on one GPU
+ * (other streams, other threads) race. */
+int mi_spmv_dot_noalloc(mi_sys* sys, const double* x, double* y, double* d_dot, void* stream);
+
 /* Diagnostic, no counterpart in the reference: the layout the SpMV streams.
  * layout: 1 = window layout (rows in the first MAS preconditioner's part
  * order, windows of window_rows rows, a block inside a window stored once),

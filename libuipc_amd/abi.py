@@ -154,6 +154,7 @@ LINSYS_PROTOS = {
     "mi_sys_nnz": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
     "mi_sys_export_bcoo": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "mi_spmv_dot": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "mi_spmv_dot_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "mi_sys_update_extents": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.POINTER(C.c_int64),
                                         C.POINTER(C.c_int64)]),
     "mi_sys_begin_assembly": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),

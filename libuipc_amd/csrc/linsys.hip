@@ -245,6 +245,7 @@ int mi_ctx_create(int device, mi_ctx** out)
     MI_HIP(hipGetDeviceCount(&count));
     MI_REQUIRE(device >= 0 && device < count, MI_ERR_ARG, "mi_ctx_create: no such device");
     MI_HIP(hipSetDevice(device));
+    spmv_noalloc_init(device);
     auto* c   = new mi_ctx;
     c->device = device;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
@@ -667,6 +668,16 @@ int mi_sys_spmv_layout(mi_sys* sys, int32_t* layout, int32_t* window_rows, int32
         }
         *blocks = n;
     }
+    MI_API_END
+}
+
+int mi_spmv_dot_noalloc(mi_sys* sys, const double* x, double* y, double* d_dot, void* stream)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(sys && x && y, MI_ERR_ARG, "mi_spmv_dot_noalloc: null argument");
+    MI_REQUIRE(sys->assembled, MI_ERR_STATE, "mi_spmv_dot_noalloc: not assembled");
+    spmv_accumulate_dot_noalloc(sys->A, sys->desc.value_precision == MI_FP32, x, y, d_dot, sys->ctx->device,
+                                pick(sys->ctx, stream));
     MI_API_END
 }
 

@@ -177,6 +177,13 @@ void spmv_accumulate_dot(const BsrMatrix& A, bool use32, double a, const double*
 // y = a A x + b y (Spmv::rbk_sym_spmv)
 void spmv_axpby(const BsrMatrix& A, bool use32, double a, const double* x, double b, double* y, SpmvScratch& scr,
                 hipStream_t s);
+// the fixed scratch of spmv_accumulate_dot_noalloc: its device address on
+// `device` (which must be current), looked up once
+void spmv_noalloc_init(int device);
+// y += A x, *d_dot = x^T A x (when d_dot != null) with the partials in the
+// fixed scratch of `device`: no allocation; FP32 systems up to 2^18 block rows
+void spmv_accumulate_dot_noalloc(const BsrMatrix& A, bool use32, const double* x, double* y, double* d_dot, int device,
+                                 hipStream_t s);
 
 // ----------------------------------------------------------------- vectors
 void   fill_f64(double* p, double v, int64_t n, hipStream_t s);

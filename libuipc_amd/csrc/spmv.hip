@@ -30,6 +30,7 @@
 //     the 8 XCDs) takes a contiguous range of chunks, so x is re-read from its
 //     own L2.
 #include <cstdlib>
+#include <mutex>
 #include <type_traits>
 
 #include "linsys.hpp"
@@ -752,4 +753,83 @@ void spmv_axpby(const BsrMatrix& A, bool use32, double a, const double* x, doubl
         spmv_accumulate_dot(A, use32, a, x, y, nullptr, scr, s);
 }
 
+// ---- y += A x and x^T A x without allocating: the workgroup partials (and,
+// for FP32 value planes, the float4 copy of x) live in a fixed device array.
+// Every call on one GPU shares that GPU's copy, so calls that overlap on one
+// GPU race; nothing in it has to start at zero (each launch writes every
+// partial it later sums, and the whole copy of x).
+namespace
+{
+constexpr size_t kNoAllocParts = 8192;         // workgroup partials (doubles): 64 KiB
+constexpr size_t kNoAllocX32   = size_t(1) << 18;  // float4 rows of x for FP32 planes: 4 MiB
+constexpr size_t kNoAllocBytes = kNoAllocParts * sizeof(double) + kNoAllocX32 * sizeof(float4);
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kNoAllocDevices = 64;
+std::mutex    g_noalloc_mu;
+void*         g_noalloc_ptr[kNoAllocDevices]     = {};
+int           g_noalloc_grid[kNoAllocDevices][2][2] = {};  // [device][window layout][FP32]
+}  // namespace
+
+void spmv_noalloc_init(int device)
+{
+    MI_REQUIRE(device >= 0 && device < kNoAllocDevices, MI_ERR_ARG, "spmv_noalloc_init: device id out of range");
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if(g_noalloc_ptr[device])
+        return;
+    int cur = -1;
+    MI_HIP(hipGetDevice(&cur));
+    MI_REQUIRE(cur == device, MI_ERR_STATE, "spmv_noalloc_init: the device is not current");
+    // the array's name is a host-side stand-in: its device address comes from
+    // the runtime, per device
+    void* p = nullptr;
+    MI_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    MI_REQUIRE(p != nullptr, MI_ERR_HIP, "spmv_noalloc_init: no device address for the scratch array");
+    // the launch shapes of spmv_prepare (per layout and value precision),
+    // without its buffer
+    int cu = 0;
+    MI_HIP(hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, device));
+    for(int win = 0; win < 2; ++win)
+        for(int k = 0; k < 2; ++k)
+        {
+            int g = std::max(1, cu) * occupancy_per_cu(win == 1, k == 1);
+            g     = std::max(kXcds, (g / kXcds) * kXcds);
+            g_noalloc_grid[device][win][k] = (int)std::min<size_t>((size_t)g, kNoAllocParts);
+        }
+    g_noalloc_ptr[device] = p;
+}
+
+void spmv_accumulate_dot_noalloc(const BsrMatrix& A, bool use32, const double* x, double* y, double* d_dot, int device,
+                                 hipStream_t s)
+{
+    MI_REQUIRE(device >= 0 && device < kNoAllocDevices && g_noalloc_ptr[device], MI_ERR_STATE,
+               "spmv no-alloc: scratch not set up on this device");
+    MI_REQUIRE(!use32 || A.has_win || (size_t)A.B <= kNoAllocX32, MI_ERR_ARG,
+               "spmv no-alloc: an FP32 system of more than 262144 block rows exceeds the fixed scratch");
+    if(A.B == 0)
+    {
+        if(d_dot)
+            fill_f64(d_dot, 0.0, 1, s);
+        return;
+    }
+    check_matrix(A, use32);
+    auto*     base     = static_cast<unsigned char*>(g_noalloc_ptr[device]);
+    double*   partials = reinterpret_cast<double*>(base);
+    float4*   x32      = reinterpret_cast<float4*>(base + kNoAllocParts * sizeof(double));
+    const int g = (int)std::max<int64_t>(
+        1, std::min<int64_t>(g_noalloc_grid[device][A.has_win ? 1 : 0][use32 ? 1 : 0], A.has_win ? A.wnchunks : A.nchunks));
+    double* p = d_dot ? partials : nullptr;
+    int     np;
+    if(use32)
+    {
+        if(!A.has_win)
+            pack_f32x4(x, x32, A.B, s);
+        np = launch_spmv<float, true>(g, s, A, values_of<float>(A), x, A.has_win ? nullptr : x32, y, 1.0, p, nullptr);
+    }
+    else
+        np = launch_spmv<double, true>(g, s, A, values_of<double>(A), x, nullptr, y, 1.0, p, nullptr);
+    if(d_dot)
+        sum_partials_kernel<<<1, kThreads, 0, s>>>(partials, np, d_dot);
+    MI_HIP(hipGetLastError());
+}
 }  // namespace mi
