@@ -46,6 +46,9 @@ def parse():
     ap.add_argument("--mas-levels", type=lambda v: v if v == "auto" else int(v), default=0,
                     help="MAS levels applied (MASPreconditionerEngine::set_active_level_num): 0 = all, the "
                          "default; auto = the fastest level count by libuipc_amd.tune, picked before timing")
+    ap.add_argument("--spmv-shape", type=lambda v: v if v == "auto" else int(v), default="auto",
+                    help="SpMV launch shape: an index of mi_spmv_shapes, or auto = the fastest on this matrix by "
+                         "libuipc_amd.tune.tune_spmv_shape, picked after the warm-up, before timing")
     ap.add_argument("--tol", type=float, default=1e-4)
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -104,9 +107,9 @@ def measured_traffic(kernel: str, args) -> float | None:
     """HBM bytes per launch of `kernel` from the committed rocprofv3 summary
     (FETCH_SIZE / WRITE_SIZE runs scaled by a copy kernel of known size), when
     it was measured on this very workload; None otherwise."""
-    if (args.grid, args.contacts, args.seed, args.precision) != (64, 2_000_000, 1, "fp64"):
+    if (args.grid, args.contacts, args.seed) != (64, 2_000_000, 1):
         return None
-    path = os.path.join(ROOT, "profiles", "c2_fp64_traffic.md")
+    path = os.path.join(ROOT, "profiles", f"c2_{args.precision}_traffic.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -177,9 +180,18 @@ def main():
         ls.assemble_preconditioners()
         return ls.solve(b, x, **solve_kw)
 
+    if args.spmv_shape != "auto":
+        ls.set_spmv_shape(args.spmv_shape)
     for w in range(args.warmup):
         st = step()
         log(f"warmup {w}: iters={st['iters']} exact={st['iters_exact']} rel_res={st['rel_res']:.3e}")
+    shape_tuned = None
+    if args.spmv_shape == "auto":
+        from libuipc_amd.tune import tune_spmv_shape
+
+        step()
+        shape_tuned = tune_spmv_shape(ls, x)
+        log(f"SpMV shape tuned: {shape_tuned['best']} of {shape_tuned['table']}")
     tuned = None
     if precond == "mas" and args.mas_levels == "auto":
         # untimed, like the warm-up: one solve per level count on this system
@@ -222,23 +234,34 @@ def main():
     # stream.  mi_spmv_dot launches the accumulating SpMV kernel (y += A x over
     # the window layout, per-workgroup x^T A x partials) and a one-workgroup
     # partials sum.
+    # The PCG's product y = A x (mi_spmv with a = 1, b = 0 launches exactly
+    # the PCG's kernel, once) is the roofline's kernel, timed as the PCG runs
+    # it: spmv_reps launches captured in one HIP graph, replayed between HIP
+    # events on the library's stream.  The accumulating mi_spmv_dot (+ its
+    # one-workgroup partials sum, launched eagerly) rides along.
+    from libuipc_amd.hip_events import graph_ms_per_call
+
     lay = ls.spmv_layout()
     y = ctx.alloc(s.n, np.float64)
     d = ctx.alloc(1, np.float64)
     for _ in range(3):
-        ctx.lib.mi_spmv_dot(ls.h, x.ptr, y.ptr, d.ptr, stream)
+        ctx.lib.mi_spmv(ls.h, 1.0, x.ptr, 0.0, y.ptr, stream)
+    ctx.synchronize()
+    ms_spmv = graph_ms_per_call(stream, lambda: ctx.lib.mi_spmv(ls.h, 1.0, x.ptr, 0.0, y.ptr, stream),
+                                args.spmv_reps)
+    ctx.lib.mi_spmv_dot(ls.h, x.ptr, y.ptr, d.ptr, stream)
     ev.start()
     for _ in range(args.spmv_reps):
         ctx.lib.mi_spmv_dot(ls.h, x.ptr, y.ptr, d.ptr, stream)
-    ms_spmv = ev.stop() / args.spmv_reps
+    ms_spmv_dot = ev.stop() / args.spmv_reps
     sval = 4 if prec == abi.MI_FP32 else 8
     spmv_bytes = U * (9 * sval + 4) + (s.block_rows + 1) * 4 + 2 * s.n * 8
     gbs = spmv_bytes / (ms_spmv * 1e-3) / 1e9
     vt = "double" if prec == abi.MI_FP64 else "float"
     if lay["layout"] == "window":
-        kname = f"spmv_win_kernel<{vt}, true, {lay['threads']}, {lay['window_rows']}>"
+        kname = f"spmv_win_kernel<{vt}, false, {lay['threads']}, {lay['window_rows']}, {lay['per']}, {lay['vec']}>"
     else:
-        kname = f"spmv_chunk_kernel<{vt}, true>"
+        kname = f"spmv_chunk_kernel<{vt}, false>"
     # bytes the layout streams per product: words + values of its blocks, the
     # row pointers and window order, x and y
     streamed = lay["blocks"] * (9 * sval + 4) + (s.block_rows + 1) * 4 + 2 * s.n * 8 + (
@@ -265,6 +288,7 @@ def main():
             "mas_levels": (tuned["best"] if tuned else args.mas_levels) if precond == "mas" else None,
             "mas_levels_tuned": {str(k): v for k, v in tuned["table"].items()} if tuned else None,
             "mas_invert": args.mas_invert if precond == "mas" else None,
+            "spmv_shape": {**lay, "tuned": shape_tuned} if shape_tuned else lay,
         },
         "pcg_iters": int(st["iters"]),
         "pcg_iters_exact": int(st["iters_exact"]),
@@ -279,7 +303,8 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic(kname, args), "algorithmic_bytes": int(spmv_bytes),
                      "avg_us": round(ms_spmv * 1e3, 2), "layout": lay["layout"], "window_rows": lay["window_rows"],
-                     "streamed_bytes": int(streamed), "streamed_GBps": round(streamed / (ms_spmv * 1e-3) / 1e9, 1)},
+                     "streamed_bytes": int(streamed), "streamed_GBps": round(streamed / (ms_spmv * 1e-3) / 1e9, 1),
+                     "spmv_dot_avg_us": round(ms_spmv_dot * 1e3, 2)},
     }
 
     if prec == abi.MI_FP64 and precond == "mas" and not args.no_fp32_arm and args.mas_levels != "auto":
@@ -316,6 +341,12 @@ def fp32_arm(ctx, s, args, rows, cols, vals, b, x64, ms64, solve_kw) -> dict:
     def timed():
         for _ in range(args.warmup):
             step()
+        if args.spmv_shape == "auto":
+            from libuipc_amd.tune import tune_spmv_shape
+
+            tune_spmv_shape(ls, x)
+        else:
+            ls.set_spmv_shape(args.spmv_shape)
         ev = EventTimer(stream)
         ctx.synchronize()
         times = []

@@ -192,12 +192,29 @@ int mi_spmv_dot(mi_sys* sys, const double* x, double* y, double* d_dot, void* st
  * (other streams, other threads) race. */
 int mi_spmv_dot_noalloc(mi_sys* sys, const double* x, double* y, double* d_dot, void* stream);
 
+/* The SpMV's launch shapes, for a tuner (the reference fits its SpMV grid to
+ * each matrix, Spmv::fit_block_count, linear_system/spmv.cu:909-933).  count:
+ * how many; dflt: the one new systems use; per shape (arrays of count
+ * entries, any may be NULL): threads per workgroup, window rows, FP64 blocks
+ * per lane and stage, blocks per lane and load (2: paired 16-byte FP64 /
+ * 8-byte FP32 loads). */
+int mi_spmv_shapes(int32_t* count, int32_t* dflt, int32_t* threads, int32_t* rows, int32_t* per, int32_t* vec);
+/* The shape this system's SpMV uses; an assembled window layout is rebuilt
+ * for it on `stream` (its window size may differ). */
+int mi_sys_set_spmv_shape(mi_sys* sys, int32_t shape, void* stream);
+
 /* Diagnostic, no counterpart in the reference: the layout the SpMV streams.
  * layout: 1 = window layout (rows in the first MAS preconditioner's part
  * order, windows of window_rows rows, a block inside a window stored once),
  * 0 = full BSR (both triangles); blocks: the 3x3 blocks one product streams;
- * threads: the SpMV kernel's workgroup size.  Any output may be NULL. */
-int mi_sys_spmv_layout(mi_sys* sys, int32_t* layout, int32_t* window_rows, int32_t* threads, int64_t* blocks);
+ * threads: the SpMV kernel's workgroup size; shape: its launch shape (an
+ * index of mi_spmv_shapes, -1 on the full BSR).  Any output may be NULL. */
+int mi_sys_spmv_layout(mi_sys*  sys,
+                       int32_t* layout,
+                       int32_t* window_rows,
+                       int32_t* threads,
+                       int64_t* blocks,
+                       int32_t* shape);
 
 /* Spmv::rbk_sym_spmv (linear_system/spmv.cu:992-1021): y = a A x + b y. */
 int mi_spmv(mi_sys* sys, double a, const double* x, double b, double* y, void* stream);

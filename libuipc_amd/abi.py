@@ -160,8 +160,11 @@ LINSYS_PROTOS = {
     "mi_sys_begin_assembly": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                         C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_void_p]),
     "mi_sys_assemble_owned": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "mi_spmv_shapes": (C.c_int, [C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_void_p, C.c_void_p, C.c_void_p,
+                                 C.c_void_p]),
+    "mi_sys_set_spmv_shape": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
     "mi_sys_spmv_layout": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
-                                     C.POINTER(C.c_int64)]),
+                                     C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
     "mi_spmv": (C.c_int, [C.c_void_p, C.c_double, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p]),
     "mi_precond_add": (C.c_int, [C.c_void_p, C.POINTER(PrecondDesc), C.POINTER(C.c_int)]),
     "mi_precond_reinit": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(PrecondDesc)]),

@@ -808,9 +808,12 @@ int MatrixConverter::compact_sort(int64_t        n,
                             hist_, chunk_, s);
 }
 
-void MatrixConverter::build_window(BsrMatrix& A, int shift, bool want32, double ratio, hipStream_t s)
+void MatrixConverter::build_window(BsrMatrix& A, int shape, bool want32, double ratio, hipStream_t s)
 {
+    MI_REQUIRE(shape >= 0 && shape < spmv_shape_count(), MI_ERR_ARG, "window layout: unknown kernel shape");
+    const int shift = spmv_shape_shift(shape);
     MI_REQUIRE(shift >= 4 && shift <= kWinShiftMax, MI_ERR_ARG, "window layout: window size out of range");
+    A.wshape = shape;
     const int32_t B = A.B;
     MI_REQUIRE(A.wpos.data() && A.wreal.data() && A.wpos.size() >= (size_t)B && A.wreal.size() >= (size_t)B,
                MI_ERR_STATE, "window layout: no row order");

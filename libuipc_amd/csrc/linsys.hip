@@ -59,6 +59,7 @@ struct mi_sys
     std::vector<std::unique_ptr<LocalPreconditioner>> pcs;
     uint64_t                                          pcs_gen       = 0;  // bumped when pcs change
     uint64_t                                          order_gen     = 0;  // bumped when a preconditioner is added or rebuilt
+    int32_t                                           spmv_shape    = spmv_default_shape();
     bool                                              assembled     = false;
     bool                                              pcs_assembled = false;
 
@@ -85,11 +86,14 @@ struct mi_sys
         int         ci = 0, fused = -1, bypass = -1, use32 = -1;
         const void* bufs[16] = {};
         int64_t     wstride  = -1;
+        int32_t     wshape   = -2;  // the SpMV kernel's launch shape (-1: full BSR)
+        int32_t     items    = -1;  // the SpMV's work items (windows or chunks)
         hipStream_t stream  = nullptr;
         bool        operator==(const GraphKey& o) const
         {
             if(x != o.x || b != o.b || n != o.n || U != o.U || pcs_gen != o.pcs_gen || ci != o.ci || fused != o.fused
-               || bypass != o.bypass || use32 != o.use32 || stream != o.stream || wstride != o.wstride)
+               || bypass != o.bypass || use32 != o.use32 || stream != o.stream || wstride != o.wstride
+               || wshape != o.wshape || items != o.items)
                 return false;
             for(int k = 0; k < 16; ++k)
                 if(bufs[k] != o.bufs[k])
@@ -208,7 +212,7 @@ static void build_spmv_layout(mi_sys* sys, hipStream_t s)
         MI_HIP(hipStreamSynchronize(s));  // the host vectors go out of scope
         A.win_order = sys->order_gen;
     }
-    sys->conv.build_window(A, spmv_window_shift(), sys->desc.value_precision == MI_FP32, sys->desc.reserve_ratio, s);
+    sys->conv.build_window(A, sys->spmv_shape, sys->desc.value_precision == MI_FP32, sys->desc.reserve_ratio, s);
 }
 
 // the matrix buffers a captured PCG graph launches with
@@ -223,6 +227,9 @@ static void key_matrix(K& key, const BsrMatrix& A)
     key.bufs[14] = A.has_win ? (A.wvals32.data() ? (const void*)A.wvals32.data() : (const void*)A.wvals.data()) : nullptr;
     key.bufs[15] = A.has_win ? (const void*)A.wreal.data() : nullptr;
     key.wstride  = A.has_win ? A.wstride : -1;
+    // the captured launch is one kernel instantiation with its window size
+    key.wshape   = A.has_win ? A.wshape : -1;
+    key.items    = spmv_items(A);
 }
 
 extern "C" {
@@ -645,7 +652,34 @@ int mi_spmv_dot(mi_sys* sys, const double* x, double* y, double* d_dot, void* st
     MI_API_END
 }
 
-int mi_sys_spmv_layout(mi_sys* sys, int32_t* layout, int32_t* window_rows, int32_t* threads, int64_t* blocks)
+int mi_spmv_shapes(int32_t* count, int32_t* dflt, int32_t* threads, int32_t* rows, int32_t* per, int32_t* vec)
+{
+    MI_API_BEGIN
+    const int n = spmv_shape_count();
+    if(count)
+        *count = n;
+    if(dflt)
+        *dflt = spmv_default_shape();
+    for(int k = 0; k < n; ++k)
+        spmv_shape_info(k, threads ? threads + k : nullptr, rows ? rows + k : nullptr, per ? per + k : nullptr,
+                        vec ? vec + k : nullptr);
+    MI_API_END
+}
+
+int mi_sys_set_spmv_shape(mi_sys* sys, int32_t shape, void* stream)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(sys, MI_ERR_ARG, "mi_sys_set_spmv_shape: null sys");
+    MI_REQUIRE(shape >= 0 && shape < spmv_shape_count(), MI_ERR_ARG, "mi_sys_set_spmv_shape: unknown shape");
+    sys->spmv_shape = shape;
+    // an assembled window layout is rebuilt for the new shape at once
+    if(sys->assembled && sys->A.has_win && sys->A.wshape != shape)
+        build_spmv_layout(sys, pick(sys->ctx, stream));
+    MI_API_END
+}
+
+int mi_sys_spmv_layout(mi_sys* sys, int32_t* layout, int32_t* window_rows, int32_t* threads, int64_t* blocks,
+                       int32_t* shape)
 {
     MI_API_BEGIN
     MI_REQUIRE(sys, MI_ERR_ARG, "mi_sys_spmv_layout: null sys");
@@ -657,6 +691,8 @@ int mi_sys_spmv_layout(mi_sys* sys, int32_t* layout, int32_t* window_rows, int32
         *window_rows = A.has_win ? (1 << A.wshift) : 0;
     if(threads)
         *threads = spmv_threads(A);
+    if(shape)
+        *shape = A.has_win ? A.wshape : -1;
     if(blocks)
     {
         int32_t n = 0;
@@ -1212,11 +1248,15 @@ void run_iteration(mi_sys* sys, double* x, bool bypass, const LocalPreconditione
     }
     else
     {
-        pcg_update_xr(sc, partA, gA, x, sys->p.data(), sys->r.data(), sys->Ap.data(), n, s);
+        const bool whole = sys->pcs.size() == 1 && sys->pcs[0]->dof_offset == 0 && sys->pcs[0]->dof_count == n;
+        // one preconditioner over the whole vector may fold its first pass into the x, r update
+        if(bypass || !whole
+           || !sys->pcs[0]->update_xr(sc, partA, gA, x, sys->p.data(), sys->r.data(), sys->Ap.data(), s))
+            pcg_update_xr(sc, partA, gA, x, sys->p.data(), sys->r.data(), sys->Ap.data(), n, s);
         gB = -1;
         if(bypass)
             pcg_copy_if_active(sc, sys->z.data(), sys->r.data(), n, s);
-        else if(sys->pcs.size() == 1 && sys->pcs[0]->dof_offset == 0 && sys->pcs[0]->dof_count == n)
+        else if(whole)
         {
             // one preconditioner over the whole vector: its last kernel writes the r.z partials
             pcg_prepare(sys->dot_scr);

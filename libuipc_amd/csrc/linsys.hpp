@@ -65,6 +65,7 @@ struct BsrMatrix
     int64_t       wstride   = 0;
     int64_t       wblocks   = 0;  // host copy, valid after the build
     int32_t       wshift    = 10;
+    int32_t       wshape    = 0;  // the window kernel's launch shape the layout was built for
     int32_t       wnchunks  = 0;
     bool          has_win   = false;
     uint64_t      win_order = ~0ull;  // the row order the layout was built for
@@ -90,8 +91,8 @@ class MatrixConverter
                      hipStream_t    s);
     ~MatrixConverter();
     // the window layout of `A` (assembled by the last convert_sym) for the
-    // row order in A.wpos / A.wreal, windows of 2^shift rows
-    void build_window(BsrMatrix& A, int shift, bool want32, double reserve_ratio, hipStream_t s);
+    // row order in A.wpos / A.wreal and a window kernel shape
+    void build_window(BsrMatrix& A, int shape, bool want32, double reserve_ratio, hipStream_t s);
 
   private:
     template <int MODE>
@@ -143,8 +144,13 @@ struct SpmvScratch
 };
 // the SpMV layout: the window layout unless MI_SPMV_LAYOUT=full selects the full BSR
 bool spmv_use_window();
-// log2 of the window rows the window kernel's launch shape takes
-int spmv_window_shift();
+// the window kernel's launch shapes: how many, the default (MI_SPMV_WIN
+// overrides it), each one's threads, window rows and blocks per lane and
+// stage, and log2 of its window rows (what the layout is built for)
+int  spmv_shape_count();
+int  spmv_default_shape();
+bool spmv_shape_info(int shape, int* threads, int* rows, int* per, int* vec);
+int  spmv_shape_shift(int shape);
 // workgroup size of the SpMV kernel that streams A's layout
 int spmv_threads(const BsrMatrix& A);
 // the SpMV's work items (windows or chunks) and the row range [lo, hi) each touches
@@ -281,6 +287,7 @@ float  norm_f32_host(const float* x, int64_t n, hipStream_t s);
 
 // ----------------------------------------------------------------- preconditioners
 struct BatchScalars;
+struct PcgScalars;
 // LocalPreconditioner (linear_system/local_preconditioner.h:9-49)
 class LocalPreconditioner
 {
@@ -298,6 +305,15 @@ class LocalPreconditioner
     // r.z and r.r partials (pairs, at most max_parts); returns their count,
     // or -1 when it does not (the caller then runs the dot kernel)
     virtual int apply_dot(const double*, double*, const int32_t*, double*, int, hipStream_t) { return -1; }
+    // a preconditioner covering the whole vector may take over the PCG's x, r
+    // update (x += alpha p, r -= alpha Ap, alpha from the pAp partials) and
+    // fold its own first pass over the new r into it; returns false when it
+    // does not (the caller then runs pcg_update_xr)
+    virtual bool update_xr(PcgScalars*, const double*, int, double*, const double*, double*, const double*,
+                           hipStream_t)
+    {
+        return false;
+    }
     // MAS only: levels used by apply (MASPreconditionerEngine::set_active_level_num)
     virtual bool set_active_levels(int) { return false; }
     // MAS only: the rank of every vertex of the range in the partition's part
@@ -341,6 +357,34 @@ struct PcgScalars
     int32_t iter;
     int32_t conv_iter;
 };
+
+// the fixed-order sum of a reduction's partials (stride apart), redone by every
+// workgroup of NT threads of the consuming kernel (no fence, no atomics,
+// deterministic); `lds` holds NT / 64 values
+template <int NT>
+__device__ __forceinline__ double pcg_sum_partials(const double* part, int g, int stride, double* lds)
+{
+    double acc = 0.0;
+    for(int k = threadIdx.x; k < g; k += NT)
+        acc += part[stride * k];
+    return block_sum<NT>(acc, lds);
+}
+
+// alpha = rz / pAp from the SpMV's pAp partials; workgroup 0 records it
+template <int NT>
+__device__ __forceinline__ double pcg_alpha_of(PcgScalars* sc, const double* pap_part, int gp, double* lds)
+{
+    const double pAp   = pcg_sum_partials<NT>(pap_part, gp, 1, lds);
+    const double rz    = sc->rz_pending;
+    const double alpha = rz / pAp;
+    if(blockIdx.x == 0 && threadIdx.x == 0)
+    {
+        sc->pAp   = pAp;
+        sc->alpha = alpha;
+        sc->rz    = rz;
+    }
+    return alpha;
+}
 
 // ---- batched PCG: independent systems on consecutive block-row ranges
 // Tiles of at most kBatchTile DoFs (whole block rows) that never cross a

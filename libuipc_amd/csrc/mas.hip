@@ -46,6 +46,8 @@
 #include <numeric>
 #include <type_traits>
 
+#include <cstdlib>
+
 #include "mas.hpp"
 
 namespace mi
@@ -1005,6 +1007,37 @@ struct LevelSpan
 // atomics and the sum is deterministic.  At level 0 the values are r gathered
 // through part_to_real.  Level starts are bank-aligned, so a bank never
 // straddles two waves.
+template <typename T>
+__device__ __forceinline__ void restrict_sum_store(T              v0,
+                                                   T              v1,
+                                                   T              v2,
+                                                   int32_t        m,
+                                                   T*             Rb,
+                                                   const int32_t* __restrict__ parent,
+                                                   int32_t        slot)
+{
+    const int b0 = (int)(threadIdx.x % kWave) & ~(kBank - 1);
+    T         s0 = 0, s1 = 0, s2 = 0;
+#pragma unroll
+    for(int q = 0; q < kBank; ++q)
+    {
+        const T w0 = __shfl(v0, b0 + q, kWave), w1 = __shfl(v1, b0 + q, kWave), w2 = __shfl(v2, b0 + q, kWave);
+        if(m & (1 << q))
+        {
+            s0 += w0;
+            s1 += w1;
+            s2 += w2;
+        }
+    }
+    if(m & 0x10000)
+    {
+        const int64_t pp = 3 * (int64_t)parent[slot];
+        Rb[pp]           = s0;
+        Rb[pp + 1]       = s1;
+        Rb[pp + 2]       = s2;
+    }
+}
+
 template <typename T, bool kLevel0>
 __device__ __forceinline__ void restrict_bank(const double* __restrict__ r,
                                               const int32_t* __restrict__ part_to_real,
@@ -1036,26 +1069,7 @@ __device__ __forceinline__ void restrict_bank(const double* __restrict__ r,
             v2 = Rb[3 * (int64_t)slot + 2];
         }
     }
-    const int b0 = (int)(threadIdx.x % kWave) & ~(kBank - 1);
-    T         s0 = 0, s1 = 0, s2 = 0;
-#pragma unroll
-    for(int q = 0; q < kBank; ++q)
-    {
-        const T w0 = __shfl(v0, b0 + q, kWave), w1 = __shfl(v1, b0 + q, kWave), w2 = __shfl(v2, b0 + q, kWave);
-        if(m & (1 << q))
-        {
-            s0 += w0;
-            s1 += w1;
-            s2 += w2;
-        }
-    }
-    if(m & 0x10000)
-    {
-        const int64_t pp = 3 * (int64_t)parent[slot];
-        Rb[pp]           = s0;
-        Rb[pp + 1]       = s1;
-        Rb[pp + 2]       = s2;
-    }
+    restrict_sum_store<T>(v0, v1, v2, m, Rb, parent, slot);
 }
 
 template <typename T, bool kLevel0>
@@ -1072,6 +1086,124 @@ __global__ __launch_bounds__(256) void mas_restrict_level_kernel(const double* _
         return;
     const int32_t x = blockIdx.x * blockDim.x + threadIdx.x;
     restrict_bank<T, kLevel0>(r, part_to_real, Rb, parent, sib, begin + x, x < cnt);
+}
+
+// fused_update_xr (linear_fused_pcg.cu:504-549) with the level-0 restriction
+// folded in: one lane per partition slot (a whole number of workgroups),
+// then one lane per unpartitioned vertex.  A slot's lane updates x and r of
+// its vertex and hands the new r to the sibling sums of restrict_bank, so r
+// is not read back by a separate restriction kernel.
+//
+// With kLevel1 (every level-1 bank's parts lie in one 256-slot range, checked
+// at the build) the workgroup also holds the whole level-1 bank its slots
+// restrict to, and the bank's sibling sums give level 2's R as well.
+template <typename T, bool kLevel1>
+__global__ __launch_bounds__(256) void mas_update_xr_restrict_kernel(PcgScalars*   sc,
+                                                                    const double* pap_part,
+                                                                    int           gp,
+                                                                    double*       x,
+                                                                    const double* __restrict__ p,
+                                                                    double*       r,
+                                                                    const double* __restrict__ Ap,
+                                                                    const int32_t* __restrict__ part_to_real,
+                                                                    T*            Rb,
+                                                                    const int32_t* __restrict__ parent,
+                                                                    const int32_t* __restrict__ sib,
+                                                                    int32_t       nslots,
+                                                                    int32_t       slot_blocks,
+                                                                    const int32_t* __restrict__ unpart,
+                                                                    int32_t       n_unpart,
+                                                                    int32_t       begin1,
+                                                                    int32_t       count1)
+{
+    if(sc->converged)
+        return;
+    __shared__ double lds[256 / kWave];
+    __shared__ T      r1s[kBank][3];
+    __shared__ int32_t bank1;
+    const double      alpha = pcg_alpha_of<256>(sc, pap_part, gp, lds);
+    if((int32_t)blockIdx.x < slot_blocks)
+    {
+        const int32_t slot = blockIdx.x * 256 + threadIdx.x;
+        const bool    live = slot < nslots;
+        const int32_t v    = live ? part_to_real[slot] : -1;
+        T             v0 = 0, v1 = 0, v2 = 0;
+        if constexpr(kLevel1)
+        {
+            if(threadIdx.x < kBank)
+                r1s[threadIdx.x][0] = r1s[threadIdx.x][1] = r1s[threadIdx.x][2] = T(0);
+            if(threadIdx.x == 0)
+                bank1 = -1;
+            __syncthreads();
+        }
+        if(v >= 0)
+        {
+            const int64_t i  = 3 * (int64_t)v;
+            const double  r0 = r[i] - alpha * Ap[i], r1 = r[i + 1] - alpha * Ap[i + 1],
+                         r2 = r[i + 2] - alpha * Ap[i + 2];
+            x[i] += alpha * p[i];
+            x[i + 1] += alpha * p[i + 1];
+            x[i + 2] += alpha * p[i + 2];
+            r[i]     = r0;
+            r[i + 1] = r1;
+            r[i + 2] = r2;
+            v0 = (T)r0;
+            v1 = (T)r1;
+            v2 = (T)r2;
+        }
+        const int32_t m = live ? sib[slot] : 0;
+        restrict_sum_store<T>(v0, v1, v2, m, Rb, parent, slot);
+        if constexpr(kLevel1)
+        {
+            // the level-1 sums again, into LDS by the node's place in its bank
+            const int b0 = (int)(threadIdx.x % kWave) & ~(kBank - 1);
+            T         s0 = 0, s1 = 0, s2 = 0;
+#pragma unroll
+            for(int q = 0; q < kBank; ++q)
+            {
+                const T w0 = __shfl(v0, b0 + q, kWave), w1 = __shfl(v1, b0 + q, kWave),
+                        w2 = __shfl(v2, b0 + q, kWave);
+                if(m & (1 << q))
+                {
+                    s0 += w0;
+                    s1 += w1;
+                    s2 += w2;
+                }
+            }
+            if(m & 0x10000)
+            {
+                const int32_t node = parent[slot] - begin1;
+                r1s[node % kBank][0] = s0;
+                r1s[node % kBank][1] = s1;
+                r1s[node % kBank][2] = s2;
+                bank1                = node / kBank;  // the same bank for every node of the workgroup
+            }
+            __syncthreads();
+            if(threadIdx.x < kWave && bank1 >= 0)
+            {
+                const int     q     = threadIdx.x;
+                const int32_t node  = bank1 * kBank + q;
+                const bool    in    = q < kBank && node < count1;
+                const int32_t slot1 = begin1 + node;
+                restrict_sum_store<T>(in ? r1s[q][0] : T(0), in ? r1s[q][1] : T(0), in ? r1s[q][2] : T(0),
+                                      in ? sib[slot1] : 0, Rb, parent, slot1);
+            }
+        }
+    }
+    else
+    {
+        const int32_t u = (blockIdx.x - slot_blocks) * 256 + threadIdx.x;
+        if(u < n_unpart)
+        {
+            const int64_t i = 3 * (int64_t)unpart[u];
+#pragma unroll
+            for(int c = 0; c < 3; ++c)
+            {
+                x[i + c] += alpha * p[i + c];
+                r[i + c] -= alpha * Ap[i + c];
+            }
+        }
+    }
 }
 
 // one wave per coarse cluster: Z = C^-1 R on the restricted residual.  The
@@ -1371,6 +1503,9 @@ class MasPreconditioner final : public LocalPreconditioner
     DBuf<float>   d_R32, d_Z32;
     std::vector<std::pair<int64_t, int64_t>> level_clusters;  // [first, count] per level
     std::vector<int32_t> rank_;        // part-order rank of every vertex
+    bool           r0_fused_ = false;  // the PCG's x, r update wrote level 1's R (update_xr)
+    bool           r1_fused_ = false;  // ... and level 2's
+    bool           l1_local_ = false;  // every level-1 bank's parts lie in one 256-slot range
     DBuf<int32_t>  d_fine_on;          // per fine cluster: 1 while it holds a running system
     const int32_t* fine_on = nullptr;  // d_fine_on during a batched solve, else nullptr
 
@@ -1518,6 +1653,27 @@ class MasPreconditioner final : public LocalPreconditioner
         upload(d_real_to_part, h.real_to_part);
         upload(d_anc, anc);
         upload(d_parent, parent);
+        // can one workgroup of 256 partition slots restrict level 1 too? Every
+        // level-1 bank's nodes must come from slots of one 256-slot range, and
+        // a range must feed at most one bank
+        l1_local_ = L > 2;
+        if(l1_local_)
+        {
+            const int32_t ranges = (int32_t)((h.N + 255) / 256);
+            std::vector<int32_t> bank_range((size_t)std::max(1, (h.level_count[1] + kBank - 1) / kBank), -1),
+                range_bank((size_t)std::max(1, ranges), -1);
+            for(int32_t slot = 0; slot < h.N && l1_local_; ++slot)
+            {
+                if(h.part_to_real[slot] < 0)
+                    continue;
+                const int32_t node = parent[slot] - h.level_begin[1], bank = node / kBank, range = slot / 256;
+                if(bank_range[bank] < 0)
+                    bank_range[bank] = range;
+                if(range_bank[range] < 0)
+                    range_bank[range] = bank;
+                l1_local_ = bank_range[bank] == range && range_bank[range] == bank;
+            }
+        }
         d_hess.resize((size_t)std::max<int64_t>(1, C * kCm));
         if(fp32)
         {
@@ -1587,10 +1743,12 @@ class MasPreconditioner final : public LocalPreconditioner
         const int L    = h.L;
         const int Lact = (active_levels > 0 && active_levels <= L) ? active_levels : L;
         // one active level is the 16-node block Jacobi: no coarse level is read
+        const bool r0_done = r0_fused_, r1_done = r1_fused_;
+        r0_fused_ = r1_fused_ = false;
         for(int l = 0; Lact > 1 && l < base_lvl; ++l)
         {
             const int32_t cnt = span.count[l];
-            if(cnt == 0)
+            if(cnt == 0 || (l == 0 && r0_done) || (l == 1 && r1_done))
                 continue;
             if(l == 0)
                 mas_restrict_level_kernel<T, true><<<ceil_div(cnt, 256), 256, 0, s>>>(
@@ -1678,6 +1836,50 @@ class MasPreconditioner final : public LocalPreconditioner
 
     const std::vector<int32_t>* spmv_rank() const override { return &rank_; }
 
+    // the PCG's x, r update with the level-0 restriction folded in, when the
+    // apply restricts at all (more than one active level) and this MAS covers
+    // the whole vector (the caller checks the range); MI_MAS_FUSE_XR=0 keeps
+    // the separate kernels
+    bool update_xr(PcgScalars* sc, const double* pap_part, int gp, double* x, const double* p, double* r,
+                   const double* Ap, hipStream_t s) override
+    {
+        static const int on = [] {
+            const char* e = std::getenv("MI_MAS_FUSE_XR");
+            return e ? std::atoi(e) : 1;
+        }();
+        const int Lact = (active_levels > 0 && active_levels <= h.L) ? active_levels : h.L;
+        if(!on || h.V == 0 || Lact <= 1 || base_lvl < 1 || span.count[0] == 0)
+            return false;
+        const int32_t slot_blocks = ceil_div(span.count[0], 256);
+        const int32_t blocks      = slot_blocks + ceil_div(n_unpart, 256);
+        // level 2's R too, when level 1 is restricted by a kernel of the apply
+        // (base level above 1) and its banks lie in whole workgroups
+        static const int l1 = [] {
+            const char* e = std::getenv("MI_MAS_FUSE_R1");
+            return e ? std::atoi(e) : 1;
+        }();
+        const bool lvl1 = l1 && l1_local_ && base_lvl >= 2 && Lact > 2;
+        auto       go   = [&](auto* R, auto tag) {
+            using T = decltype(tag);
+            if(lvl1)
+                mas_update_xr_restrict_kernel<T, true><<<blocks, 256, 0, s>>>(
+                    sc, pap_part, gp, x, p, r, Ap, d_part_to_real.data(), R, d_parent.data(), d_sib.data(),
+                    span.count[0], slot_blocks, d_unpart.data(), n_unpart, span.begin[1], span.count[1]);
+            else
+                mas_update_xr_restrict_kernel<T, false><<<blocks, 256, 0, s>>>(
+                    sc, pap_part, gp, x, p, r, Ap, d_part_to_real.data(), R, d_parent.data(), d_sib.data(),
+                    span.count[0], slot_blocks, d_unpart.data(), n_unpart, span.begin[1], span.count[1]);
+        };
+        if(fp32)
+            go(d_R32.data(), float{});
+        else
+            go(d_R64.data(), double{});
+        MI_HIP(hipGetLastError());
+        r0_fused_ = true;
+        r1_fused_ = lvl1;
+        return true;
+    }
+
     // fine clusters of stopped systems are skipped; the restriction and the
     // coarse levels (a few percent of the apply) still run over everything
     void skip_stopped(const int32_t* seg, int32_t nseg, const BatchScalars* sc, hipStream_t s) override
@@ -1709,8 +1911,14 @@ class MasPreconditioner final : public LocalPreconditioner
         // FP32 inverses only: with FP64 cluster blocks the fused kernel holds
         // 132 VGPRs (3 waves per SIMD against 5 for the plain fine kernel)
         // and the solve measured slower than with the separate dot kernel
-        if(h.V == 0 || max_parts < 1 || !fp32)
+        static const int fused64 = [] {
+            const char* e = std::getenv("MI_MAS_FINE_DOT64");
+            return e ? std::atoi(e) : 0;
+        }();
+        if(h.V == 0 || max_parts < 1 || (!fp32 && !fused64))
             return -1;
+        if(!fp32)
+            return apply_t<double>(r, z, conv, s, d_inv64.data(), d_R64.data(), d_Z64.data(), part, max_parts);
         return apply_t<float>(r, z, conv, s, d_inv32.data(), d_R32.data(), d_Z32.data(), part, max_parts);
     }
 };

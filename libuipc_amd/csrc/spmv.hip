@@ -338,7 +338,65 @@ struct WinStage
     using W                   = std::conditional_t<sizeof(VT) == 4, float, double>;
 };
 
-template <typename VT, bool ACCUM, int NT, int KW, int PER>
+// one block of a window stage: stage A x_j in slot i of w; an inner block
+// (both rows in the window) also adds A^T x_i into the LDS row j
+template <typename VT, typename W, int KW, int STAGE>
+__device__ __forceinline__ void win_block(int32_t wd,
+                                          const VT (&A)[9],
+                                          const double* __restrict__ x,
+                                          double (&xs)[3][KW],
+                                          double (&ys)[3][KW],
+                                          W (&w)[3][STAGE],
+                                          int i)
+{
+    double x0, x1, x2;
+    if(wd < 0)
+    {
+        const int li = (wd >> 15) & 0x7fff, lj = wd & 0x7fff;
+        x0 = xs[0][lj];
+        x1 = xs[1][lj];
+        x2 = xs[2][lj];
+        if(li != lj)
+        {
+            const double u0 = xs[0][li], u1 = xs[1][li], u2 = xs[2][li];
+            atomicAdd(&ys[0][lj], (double)A[0] * u0 + (double)A[1] * u1 + (double)A[2] * u2);
+            atomicAdd(&ys[1][lj], (double)A[3] * u0 + (double)A[4] * u1 + (double)A[5] * u2);
+            atomicAdd(&ys[2][lj], (double)A[6] * u0 + (double)A[7] * u1 + (double)A[8] * u2);
+        }
+    }
+    else
+    {
+        x0 = x[3 * (int64_t)wd];
+        x1 = x[3 * (int64_t)wd + 1];
+        x2 = x[3 * (int64_t)wd + 2];
+    }
+    w[0][i] = (W)((double)A[0] * x0 + (double)A[3] * x1 + (double)A[6] * x2);
+    w[1][i] = (W)((double)A[1] * x0 + (double)A[4] * x1 + (double)A[7] * x2);
+    w[2][i] = (W)((double)A[2] * x0 + (double)A[5] * x1 + (double)A[8] * x2);
+}
+
+typedef int    int2v __attribute__((ext_vector_type(2)));
+typedef float  float2v __attribute__((ext_vector_type(2)));
+typedef double double2v __attribute__((ext_vector_type(2)));
+template <typename VT>
+struct Vec2;
+template <>
+struct Vec2<double>
+{
+    using T = double2v;
+};
+template <>
+struct Vec2<float>
+{
+    using T = float2v;
+};
+
+// VEC blocks per lane and load: with VEC = 2 a lane reads two consecutive
+// blocks with 8-byte (FP32) or 16-byte (FP64) loads; a stage's slots then
+// start at an even block (the first slot may precede the stage, the last
+// may pass the window: both are masked; the plane stride is even and the
+// arrays hold two blocks of padding past the last)
+template <typename VT, bool ACCUM, int NT, int KW, int PER, int VEC>
 __global__ __launch_bounds__(NT) void spmv_win_kernel(int32_t nchunks,
                                                       int32_t B,
                                                       const int32_t* __restrict__ wptr,
@@ -354,12 +412,14 @@ __global__ __launch_bounds__(NT) void spmv_win_kernel(int32_t nchunks,
                                                       const int32_t* __restrict__ active,
                                                       const int64_t* d_nactive)
 {
+    static_assert(VEC == 1 || VEC == 2, "one or two blocks per load");
     if(d_conv && *d_conv != 0)
         return;
     using W                   = typename WinStage<VT, PER>::W;
     constexpr int kPer        = WinStage<VT, PER>::kPer;
     constexpr int kStage      = kPer * NT;
     constexpr int kGroupsW    = NT / kLanes;
+    static_assert(kPer % VEC == 0, "whole pairs per lane");
     __shared__ double  ys[3][KW];
     __shared__ double  xs[3][KW];
     __shared__ int32_t rp[KW + 1];
@@ -396,45 +456,61 @@ __global__ __launch_bounds__(NT) void spmv_win_kernel(int32_t nchunks,
             rp[nr] = wptr[R0 + nr];
         __syncthreads();
         const int32_t K0 = rp[0], K1 = rp[nr];
-        for(int32_t s0 = K0; s0 < K1; s0 += kStage)
+        for(int32_t s0 = K0, s1; s0 < K1; s0 = s1)
         {
-            const int32_t s1 = s0 + kStage < K1 ? s0 + kStage : K1;
-#pragma unroll
-            for(int h = 0; h < kPer; ++h)
+            // slots [o0, o0 + kStage) cover the stage's blocks [s0, s1)
+            const int32_t o0 = s0 & ~(VEC - 1);
+            s1               = o0 + kStage < K1 ? o0 + kStage : K1;
+            if constexpr(VEC == 1)
             {
-                const int     i = h * NT + tid;
-                const int32_t k = s0 + i;
-                if(k < s1)
-                {
-                    const int32_t wd = stream_load<VT>(&words[k]);
-                    VT            A[9];
 #pragma unroll
-                    for(int q = 0; q < 9; ++q)
-                        A[q] = stream_load<VT>(&vals[q * stride + k]);
-                    double x0, x1, x2;
-                    if(wd < 0)
+                for(int h = 0; h < kPer; ++h)
+                {
+                    const int     i = h * NT + tid;
+                    const int32_t k = o0 + i;
+                    if(k < s1)
                     {
-                        const int li = (wd >> 15) & 0x7fff, lj = wd & 0x7fff;
-                        x0 = xs[0][lj];
-                        x1 = xs[1][lj];
-                        x2 = xs[2][lj];
-                        if(li != lj)
+                        const int32_t wd = stream_load<VT>(&words[k]);
+                        VT            A[9];
+#pragma unroll
+                        for(int q = 0; q < 9; ++q)
+                            A[q] = stream_load<VT>(&vals[q * stride + k]);
+                        win_block<VT, W, KW, kStage>(wd, A, x, xs, ys, w, i);
+                    }
+                }
+            }
+            else
+            {
+                using V2 = typename Vec2<VT>::T;
+#pragma unroll
+                for(int h = 0; h < kPer / 2; ++h)
+                {
+                    const int     i   = 2 * (h * NT + tid);
+                    const int32_t k   = o0 + i;
+                    const bool    ok0 = k >= s0 && k < s1, ok1 = k + 1 < s1;
+                    if(ok0 || ok1)
+                    {
+                        const int2v wd = stream_load<VT>(reinterpret_cast<const int2v*>(&words[k]));
+                        V2         A2[9];
+#pragma unroll
+                        for(int q = 0; q < 9; ++q)
+                            A2[q] = stream_load<VT>(reinterpret_cast<const V2*>(&vals[q * stride + k]));
+                        VT A[9];
+                        if(ok0)
                         {
-                            const double u0 = xs[0][li], u1 = xs[1][li], u2 = xs[2][li];
-                            atomicAdd(&ys[0][lj], (double)A[0] * u0 + (double)A[1] * u1 + (double)A[2] * u2);
-                            atomicAdd(&ys[1][lj], (double)A[3] * u0 + (double)A[4] * u1 + (double)A[5] * u2);
-                            atomicAdd(&ys[2][lj], (double)A[6] * u0 + (double)A[7] * u1 + (double)A[8] * u2);
+#pragma unroll
+                            for(int q = 0; q < 9; ++q)
+                                A[q] = A2[q].x;
+                            win_block<VT, W, KW, kStage>(wd.x, A, x, xs, ys, w, i);
+                        }
+                        if(ok1)
+                        {
+#pragma unroll
+                            for(int q = 0; q < 9; ++q)
+                                A[q] = A2[q].y;
+                            win_block<VT, W, KW, kStage>(wd.y, A, x, xs, ys, w, i + 1);
                         }
                     }
-                    else
-                    {
-                        x0 = x[3 * (int64_t)wd];
-                        x1 = x[3 * (int64_t)wd + 1];
-                        x2 = x[3 * (int64_t)wd + 2];
-                    }
-                    w[0][i] = (W)((double)A[0] * x0 + (double)A[3] * x1 + (double)A[6] * x2);
-                    w[1][i] = (W)((double)A[1] * x0 + (double)A[4] * x1 + (double)A[7] * x2);
-                    w[2][i] = (W)((double)A[2] * x0 + (double)A[5] * x1 + (double)A[8] * x2);
                 }
             }
             __syncthreads();
@@ -455,7 +531,7 @@ __global__ __launch_bounds__(NT) void spmv_win_kernel(int32_t nchunks,
                 if(rs >= s1)
                     break;
                 const int32_t re = rp[row + 1];
-                const int32_t b0 = (rs > s0 ? rs : s0) - s0, b1 = (re < s1 ? re : s1) - s0;
+                const int32_t b0 = (rs > s0 ? rs : s0) - o0, b1 = (re < s1 ? re : s1) - o0;
                 double        acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
                 for(int32_t i = b0 + lane; i < b1; i += kLanes)
                 {
@@ -514,27 +590,31 @@ __global__ __launch_bounds__(kThreads) void sum_partials_kernel(const double* pa
 }
 
 // launch shapes of the window kernel: threads per workgroup, log2 of the
-// window rows (the layout is built for one of them) and FP64 blocks per lane
-// and stage; MI_SPMV_WIN picks one by index (the first is the default).
-// Measured on the 64^3 + 2M contact system (FP64 / FP32 planes, us per
-// accumulating product): 512 threads, 512 rows, 2 blocks 116.3 / 80.4;
-// 256, 256, 3: 119.7 / 83.9; 256, 256, 1: 128.3 / 93.2; 128, 256, 2: 162 /
-// 122; 512, 512, 1: 127 / 88; 1024, 1024, 1: 147 / 94; 256, 128, 2: 132 / 87;
-// 512, 1024, 1: 185 / 125; 256, 1024, 3: 198 / 165 (2 workgroups per CU).
-// Small windows at high occupancy beat large windows with fewer stored blocks.
+// window rows (the layout is built for one of them), FP64 blocks per lane and
+// stage, and blocks per lane and load; MI_SPMV_WIN picks one by index (the
+// first is the default).  PCG products on the 64^3 + 2M contact system, us
+// FP64 / FP32 planes (threads, rows, blocks per stage, per load):
+// 512, 512, 2, 2: 104.3 / 76.0;  512, 512, 2, 1: 109.8 / 75.3;
+// 256, 256, 4, 2: 108.5 / 76.5;  256, 256, 2, 2: 110.7 / 79.2;
+// 256, 256, 3, 1: 111.9 / 77.7;  256, 256, 2, 1: 112.9 / 79.1;
+// 512, 512, 1, 1: 114.9 / 82.4;  256, 128, 2, 1: 124.0 / 79.5;
+// 1024, 1024, 1, 1: 141.3 / 88.2.  Small windows at high occupancy beat large
+// windows with fewer stored blocks (the LDS window and stage buffer bound the
+// workgroups per CU; 512 threads over 1024 rows: 185 us).
 struct WinShape
 {
-    int threads, shift, per;
+    int threads, shift, per, vec;
 };
-#define MI_WIN_SHAPES(X)                                                                                              \
-    X(0, 512, 9, 2) X(1, 256, 8, 3) X(2, 256, 8, 1) X(3, 128, 8, 2) X(4, 256, 8, 2) X(5, 512, 9, 1) X(6, 1024, 10, 1) \
-    X(7, 256, 7, 2) X(8, 512, 10, 1)
-#define MI_WIN_ENTRY(i, nt, sh, per) {nt, sh, per},
+#define MI_WIN_SHAPES(X)                                                                              \
+    X(0, 512, 9, 2, 2) X(1, 256, 8, 3, 1) X(2, 512, 9, 2, 1) X(3, 256, 8, 2, 2) X(4, 256, 8, 2, 1) \
+    X(5, 512, 9, 1, 1) X(6, 1024, 10, 1, 1) X(7, 256, 7, 2, 1) X(8, 256, 8, 4, 2)
+#define MI_WIN_ENTRY(i, nt, sh, per, vec) {nt, sh, per, vec},
 constexpr WinShape kWinShapes[] = {MI_WIN_SHAPES(MI_WIN_ENTRY)};
 #undef MI_WIN_ENTRY
 constexpr int kNumWinShapes = (int)(sizeof(kWinShapes) / sizeof(kWinShapes[0]));
 
-int win_shape_index()
+// MI_SPMV_WIN overrides the default shape (for sweeps)
+int default_shape_index()
 {
     static int idx = -1;
     if(idx < 0)
@@ -551,19 +631,19 @@ const void* win_kernel(int idx)
 {
     switch(idx)
     {
-#define MI_WIN_FN(i, nt, sh, per) \
-    case i: return (const void*)spmv_win_kernel<VT, ACCUM, nt, (1 << sh), per>;
+#define MI_WIN_FN(i, nt, sh, per, vec) \
+    case i: return (const void*)spmv_win_kernel<VT, ACCUM, nt, (1 << sh), per, vec>;
         MI_WIN_SHAPES(MI_WIN_FN)
 #undef MI_WIN_FN
     }
     return nullptr;
 }
 
-template <typename VT, bool ACCUM, int NT, int KW, int PER>
+template <typename VT, bool ACCUM, int NT, int KW, int PER, int VEC>
 void launch_win_t(int g, hipStream_t s, const BsrMatrix& A, const VT* vals, const double* x, double* y, double a,
                   double* partials, const int32_t* d_conv, const int32_t* active, const int64_t* d_nactive)
 {
-    spmv_win_kernel<VT, ACCUM, NT, KW, PER><<<g, NT, 0, s>>>(A.wnchunks, A.B, A.wptr.data(), A.wwords.data(), vals,
+    spmv_win_kernel<VT, ACCUM, NT, KW, PER, VEC><<<g, NT, 0, s>>>(A.wnchunks, A.B, A.wptr.data(), A.wwords.data(), vals,
                                                         A.wstride, A.wreal.data(), x, y, a, partials, d_conv, active,
                                                         d_nactive);
 }
@@ -577,13 +657,15 @@ int launch_spmv(int g, hipStream_t s, const BsrMatrix& A, const VT* vals, const 
 {
     if(A.has_win)
     {
-        const int idx = win_shape_index();
-        MI_REQUIRE(A.wshift == kWinShapes[idx].shift, MI_ERR_STATE, "spmv: window layout built for another shape");
+        const int idx = A.wshape;
+        MI_REQUIRE(idx >= 0 && idx < kNumWinShapes && A.wshift == kWinShapes[idx].shift, MI_ERR_STATE,
+                   "spmv: window layout built for another shape");
         switch(idx)
         {
-#define MI_WIN_LAUNCH(i, nt, sh, per)                                                                          \
-    case i:                                                                                                   \
-        launch_win_t<VT, ACCUM, nt, (1 << sh), per>(g, s, A, vals, x, y, a, partials, d_conv, active, d_nactive); \
+#define MI_WIN_LAUNCH(i, nt, sh, per, vec)                                                                    \
+    case i:                                                                                                      \
+        launch_win_t<VT, ACCUM, nt, (1 << sh), per, vec>(g, s, A, vals, x, y, a, partials, d_conv, active,      \
+                                                         d_nactive);                                             \
         break;
             MI_WIN_SHAPES(MI_WIN_LAUNCH)
 #undef MI_WIN_LAUNCH
@@ -624,26 +706,75 @@ void check_matrix(const BsrMatrix& A, bool use32)
                "spmv: values of this precision were not assembled");
 }
 
-// workgroups per CU that fit at once, for each layout's kernel
-int occupancy_per_cu(bool win, bool use32)
+// workgroups per CU that fit at once: the full-BSR kernel (shape -1) or a
+// window shape, per value precision; looked up once each
+int occupancy_per_cu(int shape, bool use32)
 {
-    const void* fn = win ? (use32 ? win_kernel<float, false>(win_shape_index()) : win_kernel<double, false>(win_shape_index()))
-                         : (use32 ? (const void*)spmv_chunk_kernel<float, false> : (const void*)spmv_chunk_kernel<double, false>);
-    const int   nt = win ? kWinShapes[win_shape_index()].threads : kThreads;
-    int         per_cu = 0;
-    MI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, nt, 0));
-    return std::max(1, per_cu);
+    static int cache[kNumWinShapes + 1][2] = {};
+    int&       c = cache[shape + 1][use32 ? 1 : 0];
+    if(c == 0)
+    {
+        const void* fn = shape >= 0 ? (use32 ? win_kernel<float, false>(shape) : win_kernel<double, false>(shape))
+                                    : (use32 ? (const void*)spmv_chunk_kernel<float, false>
+                                             : (const void*)spmv_chunk_kernel<double, false>);
+        const int   nt     = shape >= 0 ? kWinShapes[shape].threads : kThreads;
+        int         per_cu = 0;
+        MI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, nt, 0));
+        c = std::max(1, per_cu);
+    }
+    return c;
 }
+
+// the persistent grid of a layout: as many workgroups per CU as fit at once
+// (MI_SPMV_BLOCKS_PER_CU overrides for sweeps), a multiple of the XCD count
+int grid_of_shape(int shape, bool use32, int cu)
+{
+    static int over = -1;
+    if(over < 0)
+    {
+        over = 0;
+        if(const char* e = std::getenv("MI_SPMV_BLOCKS_PER_CU"))
+            over = std::max(1, std::min(32, std::atoi(e)));
+    }
+    const int g = cu * (over > 0 ? over : occupancy_per_cu(shape, use32));
+    return std::max(kXcds, (g / kXcds) * kXcds);
+}
+constexpr int kMaxGridPerCu = 32;  // workgroups per CU of any shape (the partials bound)
 }  // namespace
 
-int spmv_window_shift()
+int spmv_shape_count()
 {
-    return kWinShapes[win_shape_index()].shift;
+    return kNumWinShapes;
+}
+
+int spmv_default_shape()
+{
+    return default_shape_index();
+}
+
+bool spmv_shape_info(int shape, int* threads, int* rows, int* per, int* vec)
+{
+    if(shape < 0 || shape >= kNumWinShapes)
+        return false;
+    if(vec)
+        *vec = kWinShapes[shape].vec;
+    if(threads)
+        *threads = kWinShapes[shape].threads;
+    if(rows)
+        *rows = 1 << kWinShapes[shape].shift;
+    if(per)
+        *per = kWinShapes[shape].per;
+    return true;
+}
+
+int spmv_shape_shift(int shape)
+{
+    return kWinShapes[shape].shift;
 }
 
 int spmv_threads(const BsrMatrix& A)
 {
-    return A.has_win ? kWinShapes[win_shape_index()].threads : kThreads;
+    return A.has_win ? kWinShapes[A.wshape].threads : kThreads;
 }
 
 bool spmv_use_window()
@@ -662,31 +793,17 @@ void spmv_prepare(SpmvScratch& scr, bool use32)
     if(scr.grid == 0 || scr.use32 != use32)
     {
         scr.use32 = use32;
-        // a persistent grid of as many workgroups per CU as fit at once (the
-        // kernels' occupancy, the larger of the two layouts' grids);
-        // MI_SPMV_BLOCKS_PER_CU overrides for sweeps
-        int per_cu = 0;
-        if(const char* e = std::getenv("MI_SPMV_BLOCKS_PER_CU"))
-            per_cu = std::max(1, std::min(32, std::atoi(e)));
-        int gw = 0, gf = 0;
-        for(int win = 0; win < 2; ++win)
-        {
-            int pc = per_cu > 0 ? per_cu : occupancy_per_cu(win == 1, use32);
-            int g  = device_cu_count() * pc;
-            g      = std::max(kXcds, (g / kXcds) * kXcds);
-            (win ? gw : gf) = g;
-        }
-        scr.grid_win = gw;
-        scr.grid     = gf;
-        scr.partials.resize((size_t)2 * std::max(gw, gf));
+        scr.grid  = device_cu_count();  // marks the scratch prepared; the grids depend on the layout
+        scr.partials.resize((size_t)2 * device_cu_count() * kMaxGridPerCu);
     }
 }
 
 int spmv_grid(const SpmvScratch& scr, const BsrMatrix& A)
 {
+    const int cu = device_cu_count();
     if(A.has_win)
-        return (int)std::max<int64_t>(1, std::min<int64_t>(scr.grid_win, A.wnchunks));
-    return (int)std::max<int64_t>(1, std::min<int64_t>(scr.grid, A.nchunks));
+        return (int)std::max<int64_t>(1, std::min<int64_t>(grid_of_shape(A.wshape, scr.use32, cu), A.wnchunks));
+    return (int)std::max<int64_t>(1, std::min<int64_t>(grid_of_shape(-1, scr.use32, cu), A.nchunks));
 }
 
 // the FP32 kernel gathers x from a float4 copy: the caller's (the PCG keeps
@@ -747,6 +864,20 @@ void spmv_axpby(const BsrMatrix& A, bool use32, double a, const double* x, doubl
                 hipStream_t s)
 {
     int64_t n = 3 * (int64_t)A.B;
+    if(a == 1.0 && b == 0.0 && A.B > 0)
+    {
+        // y = A x: the PCG's product (one launch, y overwritten)
+        spmv_prepare(scr, use32);
+        check_matrix(A, use32);
+        const int g = spmv_grid(scr, A);
+        if(use32)
+            launch_spmv<float, false>(g, s, A, values_of<float>(A), x,
+                                      A.has_win ? nullptr : x32_of(A, x, nullptr, scr, s), y, 1.0, nullptr, nullptr);
+        else
+            launch_spmv<double, false>(g, s, A, values_of<double>(A), x, nullptr, y, 1.0, nullptr, nullptr);
+        MI_HIP(hipGetLastError());
+        return;
+    }
     if(n > 0 && b != 1.0)
         scale_kernel<<<std::max(1, std::min(ceil_div(n, 256), device_cu_count() * 8)), 256, 0, s>>>(y, b, n);
     if(a != 0.0)
@@ -768,7 +899,7 @@ static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
 constexpr int kNoAllocDevices = 64;
 std::mutex    g_noalloc_mu;
 void*         g_noalloc_ptr[kNoAllocDevices]     = {};
-int           g_noalloc_grid[kNoAllocDevices][2][2] = {};  // [device][window layout][FP32]
+int           g_noalloc_grid[kNoAllocDevices][kNumWinShapes + 1][2] = {};  // [device][1 + shape (0: full BSR)][FP32]
 }  // namespace
 
 void spmv_noalloc_init(int device)
@@ -789,13 +920,10 @@ void spmv_noalloc_init(int device)
     // without its buffer
     int cu = 0;
     MI_HIP(hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, device));
-    for(int win = 0; win < 2; ++win)
+    for(int shape = -1; shape < kNumWinShapes; ++shape)
         for(int k = 0; k < 2; ++k)
-        {
-            int g = std::max(1, cu) * occupancy_per_cu(win == 1, k == 1);
-            g     = std::max(kXcds, (g / kXcds) * kXcds);
-            g_noalloc_grid[device][win][k] = (int)std::min<size_t>((size_t)g, kNoAllocParts);
-        }
+            g_noalloc_grid[device][shape + 1][k] =
+                (int)std::min<size_t>((size_t)grid_of_shape(shape, k == 1, std::max(1, cu)), kNoAllocParts);
     g_noalloc_ptr[device] = p;
 }
 
@@ -817,7 +945,8 @@ void spmv_accumulate_dot_noalloc(const BsrMatrix& A, bool use32, const double* x
     double*   partials = reinterpret_cast<double*>(base);
     float4*   x32      = reinterpret_cast<float4*>(base + kNoAllocParts * sizeof(double));
     const int g = (int)std::max<int64_t>(
-        1, std::min<int64_t>(g_noalloc_grid[device][A.has_win ? 1 : 0][use32 ? 1 : 0], A.has_win ? A.wnchunks : A.nchunks));
+        1, std::min<int64_t>(g_noalloc_grid[device][A.has_win ? A.wshape + 1 : 0][use32 ? 1 : 0],
+                             A.has_win ? A.wnchunks : A.nchunks));
     double* p = d_dot ? partials : nullptr;
     int     np;
     if(use32)

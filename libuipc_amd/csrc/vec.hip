@@ -13,6 +13,8 @@
 // an L2 write-back on each of the 8 XCDs.
 #include <cmath>
 
+#include <cstdlib>
+
 #include "linsys.hpp"
 
 namespace mi
@@ -24,6 +26,21 @@ constexpr int kThreads = 256;
 int vec_grid(int64_t n)
 {
     return std::max(1, std::min(ceil_div(n, kThreads), device_cu_count() * 8));
+}
+
+int env_per_cu(const char* name, int dflt)
+{
+    if(const char* e = std::getenv(name))
+        return std::max(1, std::min(32, std::atoi(e)));
+    return dflt;
+}
+
+// the PCG's vector updates: every workgroup first re-sums the previous
+// kernel's partials, so fewer, fatter workgroups (MI_PCG_WG_PER_CU per CU)
+int pcg_grid(int64_t n)
+{
+    static const int per = env_per_cu("MI_PCG_WG_PER_CU", 8);
+    return std::max(1, std::min(ceil_div(n, kThreads), device_cu_count() * per));
 }
 
 __global__ void fill_kernel(double* p, double v, int64_t n)
@@ -123,10 +140,7 @@ __global__ void pcg_init_kernel(double* x, double* r, const double* b, int64_t n
 // the consuming kernel (no fence, no atomics, deterministic)
 __device__ __forceinline__ double sum_partials(const double* part, int g, int stride, double* lds)
 {
-    double acc = 0.0;
-    for(int k = threadIdx.x; k < g; k += kThreads)
-        acc += part[stride * k];
-    return block_sum<kThreads>(acc, lds);
+    return pcg_sum_partials<kThreads>(part, g, stride, lds);
 }
 
 __global__ __launch_bounds__(kThreads) void pcg_dot_partials_kernel(const double* r,
@@ -182,16 +196,7 @@ __global__ __launch_bounds__(kThreads) void pcg_init_scalars_kernel(PcgScalars* 
 // (nothing else reads rz in this kernel).
 __device__ __forceinline__ double pcg_alpha(PcgScalars* sc, const double* pap_part, int gp, double* lds)
 {
-    const double pAp   = sum_partials(pap_part, gp, 1, lds);
-    const double rz    = sc->rz_pending;
-    const double alpha = rz / pAp;
-    if(blockIdx.x == 0 && threadIdx.x == 0)
-    {
-        sc->pAp   = pAp;
-        sc->alpha = alpha;
-        sc->rz    = rz;
-    }
-    return alpha;
+    return pcg_alpha_of<kThreads>(sc, pap_part, gp, lds);
 }
 
 __global__ __launch_bounds__(kThreads) void pcg_update_xr_kernel(PcgScalars*   sc,
@@ -446,7 +451,9 @@ void pcg_prepare(SpmvScratch& scr)
 {
     if(scr.grid == 0)
     {
-        scr.grid = device_cu_count() * 4;
+        // the r.z / r.r partials: MI_DOT_WG_PER_CU workgroups per CU
+        static const int per = env_per_cu("MI_DOT_WG_PER_CU", 4);
+        scr.grid = device_cu_count() * per;
         scr.partials.resize((size_t)2 * scr.grid);
     }
 }
@@ -494,7 +501,7 @@ void pcg_update_xr(PcgScalars* sc, const double* pap_part, int gp, double* x, co
                    const double* Ap, int64_t n, hipStream_t s)
 {
     MI_REQUIRE(sc && pap_part && x && p && r && Ap, MI_ERR_STATE, "pcg_update_xr: null buffer");
-    pcg_update_xr_kernel<<<vec_grid(n), kThreads, 0, s>>>(sc, pap_part, gp, x, p, r, Ap, n);
+    pcg_update_xr_kernel<<<pcg_grid(n), kThreads, 0, s>>>(sc, pap_part, gp, x, p, r, Ap, n);
 }
 
 int pcg_update_xr_diag3(PcgScalars* sc, const double* pap_part, int gp, double* x, const double* p, double* r,
@@ -514,7 +521,7 @@ void pcg_update_p(PcgScalars* sc, const double* part, int gd, double* p, const d
                   hipStream_t s)
 {
     MI_REQUIRE(sc && part && p && z, MI_ERR_STATE, "pcg_update_p: null buffer");
-    pcg_update_p_kernel<<<vec_grid(n), kThreads, 0, s>>>(sc, part, gd, p, z, n, p32);
+    pcg_update_p_kernel<<<pcg_grid(n), kThreads, 0, s>>>(sc, part, gd, p, z, n, p32);
 }
 
 void pack_f32x4(const double* x, float4* x32, int64_t nb, hipStream_t s)
@@ -525,6 +532,6 @@ void pack_f32x4(const double* x, float4* x32, int64_t nb, hipStream_t s)
 
 void pcg_copy_if_active(PcgScalars* sc, double* dst, const double* src, int64_t n, hipStream_t s)
 {
-    pcg_copy_if_active_kernel<<<vec_grid(n), kThreads, 0, s>>>(sc, dst, src, n);
+    pcg_copy_if_active_kernel<<<pcg_grid(n), kThreads, 0, s>>>(sc, dst, src, n);
 }
 }  // namespace mi

@@ -150,6 +150,17 @@ def _i32p(a):
     return None if a is None else np.ascontiguousarray(a, dtype=np.int32).ctypes.data_as(abi.c_i32p)
 
 
+def spmv_shapes(lib=None) -> dict:
+    """the SpMV's launch shapes: {"default": k, "shapes": [{threads, rows, per}, ...]}"""
+    lib = lib or abi.linsys()
+    n, d = C.c_int32(), C.c_int32()
+    check(lib.mi_spmv_shapes(C.byref(n), C.byref(d), None, None, None, None))
+    t, r, p, v = ((C.c_int32 * n.value)() for _ in range(4))
+    check(lib.mi_spmv_shapes(C.byref(n), C.byref(d), t, r, p, v))
+    return {"default": d.value,
+            "shapes": [{"threads": t[k], "rows": r[k], "per": p[k], "vec": v[k]} for k in range(n.value)]}
+
+
 class LinearSystem:
     """GlobalLinearSystem-shaped handle over one mi_sys."""
 
@@ -229,13 +240,21 @@ class LinearSystem:
     def assemble_owned(self, stream=None) -> None:
         check(self.lib.mi_sys_assemble_owned(self.h, stream))
 
+    def set_spmv_shape(self, shape: int, stream=None) -> None:
+        """the window kernel's launch shape (an assembled layout is rebuilt)"""
+        check(self.lib.mi_sys_set_spmv_shape(self.h, int(shape), stream))
+
     def spmv_layout(self) -> dict:
         """the layout the SpMV streams: window (1) or full BSR (0), window rows,
         workgroup size and blocks per product"""
-        lay, wr, nt, nb = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int64()
-        check(self.lib.mi_sys_spmv_layout(self.h, C.byref(lay), C.byref(wr), C.byref(nt), C.byref(nb)))
-        return {"layout": "window" if lay.value == 1 else "full", "window_rows": wr.value, "threads": nt.value,
-                "blocks": nb.value}
+        lay, wr, nt, nb, sh = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int64(), C.c_int32()
+        check(self.lib.mi_sys_spmv_layout(self.h, C.byref(lay), C.byref(wr), C.byref(nt), C.byref(nb), C.byref(sh)))
+        out = {"layout": "window" if lay.value == 1 else "full", "window_rows": wr.value, "threads": nt.value,
+               "blocks": nb.value, "shape": sh.value}
+        if sh.value >= 0:
+            info = spmv_shapes(self.lib)["shapes"][sh.value]
+            out["per"], out["vec"] = info["per"], info["vec"]
+        return out
 
     def spmv(self, a: float, x: DeviceArray, b: float, y: DeviceArray) -> None:
         check(self.lib.mi_spmv(self.h, a, x.ptr, b, y.ptr, None))

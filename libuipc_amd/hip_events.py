@@ -24,6 +24,12 @@ def hip() -> C.CDLL:
         _hip.hipEventSynchronize.argtypes = [C.c_void_p]
         _hip.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
         _hip.hipEventDestroy.argtypes = [C.c_void_p]
+        _hip.hipStreamBeginCapture.argtypes = [C.c_void_p, C.c_int]
+        _hip.hipStreamEndCapture.argtypes = [C.c_void_p, C.POINTER(C.c_void_p)]
+        _hip.hipGraphInstantiate.argtypes = [C.POINTER(C.c_void_p), C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]
+        _hip.hipGraphLaunch.argtypes = [C.c_void_p, C.c_void_p]
+        _hip.hipGraphExecDestroy.argtypes = [C.c_void_p]
+        _hip.hipGraphDestroy.argtypes = [C.c_void_p]
     return _hip
 
 
@@ -57,3 +63,33 @@ def available() -> bool:
         return os.path.exists("/dev/kfd")
     except Exception:
         return False
+
+
+def graph_ms_per_call(stream, fn, reps: int, launches: int = 3) -> float:
+    """fn() launches work on `stream`; it is captured `reps` times into one HIP
+    graph (relaxed capture mode), which is replayed once untimed and then
+    `launches` times between events: the average ms per fn() as the GPU runs
+    it back to back, without the host's per-launch gaps."""
+    h = hip()
+    assert h.hipStreamBeginCapture(stream, 2) == 0
+    try:
+        for _ in range(reps):
+            fn()
+    finally:
+        graph = C.c_void_p()
+        rc = h.hipStreamEndCapture(stream, C.byref(graph))
+    assert rc == 0, "stream capture failed"
+    exe = C.c_void_p()
+    assert h.hipGraphInstantiate(C.byref(exe), graph, None, None, 0) == 0
+    try:
+        assert h.hipGraphLaunch(exe, stream) == 0
+        ev = EventTimer(stream)
+        ev.start()
+        for _ in range(launches):
+            assert h.hipGraphLaunch(exe, stream) == 0
+        ms = ev.stop()
+        ev.close()
+    finally:
+        h.hipGraphExecDestroy(exe)
+        h.hipGraphDestroy(graph)
+    return ms / (launches * reps)

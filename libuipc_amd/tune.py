@@ -1,4 +1,4 @@
-"""Level-count tuner for the MAS preconditioner.
+"""Tuners: the MAS level count and the SpMV launch shape.
 
 The number of active MAS levels (MASPreconditionerEngine::set_active_level_num,
 finite_element/mas_preconditioner_engine.h:104-110) moves the iteration count
@@ -6,7 +6,12 @@ and the cost per iteration in opposite directions, and which wins depends on
 the system: only whole solves to the tolerance compare them.  The tuner times
 one solve per candidate on the caller's system (meant for the first Newton
 iteration after a topology change; the choice then holds until the next one),
-keeps the fastest, and leaves it set on the preconditioner."""
+keeps the fastest, and leaves it set on the preconditioner.
+
+The SpMV launch shape (threads per workgroup, rows per window, blocks per lane
+and stage) changes the window layout and the kernel's occupancy but not the
+product; the shape tuner times products on the caller's matrix and keeps the
+fastest."""
 from __future__ import annotations
 
 import time
@@ -38,4 +43,36 @@ def tune_mas_levels(ls: LinearSystem, handle: int, b, x, levels=None, repeats: i
     ok = [lv for lv in cand if table[lv]["converged"]] or cand
     best = min(ok, key=lambda lv: table[lv]["ms"])
     ls.mas_set_active_levels(handle, best)
+    return {"best": best, "table": table}
+
+
+def tune_spmv_shape(ls: LinearSystem, x, reps: int = 20, shapes=None) -> dict:
+    """Times `reps` products mi_spmv_dot(x) per SpMV launch shape (each one's
+    window layout rebuilt for it, one untimed product first) and keeps the
+    fastest, as the reference fits its SpMV grid to each matrix
+    (Spmv::fit_block_count, linear_system/spmv.cu:909-933).  Returns
+    {"best": k, "table": {k: {"us", "threads", "rows", "per"}}}."""
+    import numpy as np
+
+    from .device import spmv_shapes
+    from .hip_events import EventTimer
+
+    info = spmv_shapes(ls.lib)["shapes"]
+    cand = list(range(len(info))) if shapes is None else [int(k) for k in shapes]
+    ctx = ls.ctx
+    stream = ctx.stream()
+    y = ctx.alloc(ls.n, np.float64)
+    d = ctx.alloc(1, np.float64)
+    ev = EventTimer(stream)
+    table = {}
+    for k in cand:
+        ls.set_spmv_shape(k, stream)
+        ctx.lib.mi_spmv_dot(ls.h, x.ptr, y.ptr, d.ptr, stream)
+        ev.start()
+        for _ in range(max(1, reps)):
+            ctx.lib.mi_spmv_dot(ls.h, x.ptr, y.ptr, d.ptr, stream)
+        table[k] = {"us": round(1e3 * ev.stop() / max(1, reps), 2), **info[k]}
+    best = min(table, key=lambda k: table[k]["us"])
+    ls.set_spmv_shape(best, stream)
+    ctx.synchronize()
     return {"best": best, "table": table}

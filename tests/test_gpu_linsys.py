@@ -176,6 +176,70 @@ def test_spmv_in_part_order(ctx, c1_contact, precision, unpart):
 
 
 @pytest.mark.parametrize("precision", [abi.MI_FP64, abi.MI_FP32])
+def test_spmv_every_launch_shape(ctx, c1_contact, precision):
+    """every SpMV launch shape (its window size, workgroup and stage) gives
+    the oracle's product, with the windows in a MAS part order; the shape
+    tuner times them all and leaves the fastest set"""
+    from libuipc_amd.device import spmv_shapes
+    from libuipc_amd.tune import tune_spmv_shape
+
+    s = c1_contact
+    ls = assembled(ctx, s, precision)
+    ls.add_preconditioner(abi.MI_PRECOND_MAS, 0, s.n, part_ids=s.part_ids, tets=s.tets)
+    ls.assemble_preconditioners()
+    orr, oc, ov, _ = O.convert_sym(s.block_rows, s.rows, s.cols, s.vals)
+    x = np.random.default_rng(8).standard_normal(s.n)
+    yref, dref = O.spmv_sym(s.block_rows, orr, oc, ov, x)
+    xd = ctx.to_device(x)
+    tol = 1e-13 if precision == abi.MI_FP64 else 1e-6
+    info = spmv_shapes()
+    assert 0 <= info["default"] < len(info["shapes"])
+    for k, sh in enumerate(info["shapes"]):
+        ls.set_spmv_shape(k)
+        lay = ls.spmv_layout()
+        assert lay["layout"] == "window" and lay["window_rows"] == sh["rows"] and lay["threads"] == sh["threads"]
+        yd = ctx.alloc(s.n, np.float64)
+        d = ls.spmv_dot(xd, yd)
+        assert np.abs(yd.numpy() - yref).max() <= tol * np.abs(yref).max(), k
+        assert d == pytest.approx(dref, rel=tol * 10), k
+    t = tune_spmv_shape(ls, xd, reps=3)
+    assert sorted(t["table"]) == list(range(len(info["shapes"])))
+    assert ls.spmv_layout()["window_rows"] == t["table"][t["best"]]["rows"]
+    with pytest.raises(abi.MiError):
+        ls.set_spmv_shape(len(info["shapes"]))
+    ls.close()
+
+
+def test_pcg_after_a_shape_change(ctx, c1_contact):
+    """a solve captured in a HIP graph with one SpMV launch shape is not
+    replayed after the shape (and with it the window size) changes: the next
+    solve re-captures and gives the same result"""
+    from libuipc_amd.device import spmv_shapes
+
+    s = c1_contact
+    ls = assembled(ctx, s)
+    ls.add_preconditioner(abi.MI_PRECOND_MAS, 0, s.n, part_ids=s.part_ids, tets=s.tets)
+    ls.assemble_preconditioners()
+    shapes = spmv_shapes()["shapes"]
+    k0 = ls.spmv_layout()["shape"]
+    k1 = next(k for k, sh in enumerate(shapes) if sh["rows"] != shapes[k0]["rows"])
+    b = ctx.to_device(s.rhs)
+    kw = dict(criterion=abi.MI_CRIT_REL_RESIDUAL, tol=1e-6, true_residual=1, use_graph=1)
+    x0 = ctx.alloc(s.n, np.float64)
+    st0 = ls.solve(b, x0, **kw)
+    for k in (k1, k0, k1):
+        ls.set_spmv_shape(k)
+        ls.assemble(*ls._trip)
+        ls.assemble_preconditioners()
+        x = ctx.alloc(s.n, np.float64)
+        st = ls.solve(b, x, **kw)
+        assert st["converged"] == 1 and st["rel_res_true"] <= 2e-6, (k, st)
+        assert abs(st["iters_exact"] - st0["iters_exact"]) <= max(2, st0["iters_exact"] // 50), (k, st, st0)
+        assert rel_err(x.numpy(), x0.numpy()) < 1e-4
+    ls.close()
+
+
+@pytest.mark.parametrize("precision", [abi.MI_FP64, abi.MI_FP32])
 def test_spmv_irregular_rows(ctx, precision):
     """Rows the SpMV's 64-block windows must get right: runs of one-block rows
     (64 row starts in one window), empty rows (a zero block is stored for

@@ -137,6 +137,30 @@ def test_mas_pcg(ctx, precision, levels):
     assert rel_err(x.numpy(), xt) < 1e-2
 
 
+@pytest.mark.parametrize("grid,partition", [(48, "geometric"), (48, "random"), (12, "unpart")])
+def test_mas_pcg_iterate_after_a_few_iterations(ctx, grid, partition):
+    """MAS-PCG stopped after 12 iterations gives the oracle's iterate to FP64
+    rounding: every step of the iteration -- the x, r update with the
+    level-0 (and, on the 48^3 grid's geometric parts, level-1) restriction
+    folded in, the coarse and fine solves, the scalars -- is the oracle's.
+    48^3 has a base level above 1 (6,912 level-1 nodes), so the level-1
+    restriction runs; random parts break the alignment it needs and fall back
+    to the separate kernel; un-partitioned vertices take the 3x3 fallback."""
+    s, (r, c, v) = system(grid, 20 * grid ** 2, 31, partition)
+    ls, h = gpu_mas(ctx, s)
+    ratio = 12.5 / s.n  # max_iter = 12
+    x = ctx.alloc(s.n, np.float64)
+    st = ls.solve(ctx.to_device(s.rhs), x, criterion=abi.MI_CRIT_REL_RESIDUAL, tol=1e-14, max_iter_ratio=ratio)
+    m = O.Mas(s.block_rows, s.part_ids, s.tets)
+    m.assemble(r, c, v)
+    xo, ro = O.pcg(s.block_rows, r, c, v, s.rhs, [dict(kind=3, blk_cnt=s.block_rows, mas=m)], criterion=1, tol=1e-14,
+                   max_iter_ratio=ratio, threads=8)
+    assert ro["converged"] == 0 and ro["iters"] == 12 and st["converged"] == 0 and st["iters"] == 12
+    assert rel_err(x.numpy(), xo) < 1e-9
+    assert st["rel_res"] == pytest.approx(ro["rel_res"], rel=1e-8)
+    ls.close()
+
+
 @pytest.mark.parametrize("partition", ["random", "unpart"])
 def test_mas_pcg_fp32_one_level(ctx, partition):
     """FP32, one level (the fine kernel fused with the r.z partials), with
