@@ -31,8 +31,8 @@ constexpr int kWaves   = kThreads / kWave;
 
 // kept entries and their sort keys.  Mode 0: the triplet filter of
 // matrix_converter.inl:142 (plus a bound on col, which the reference leaves
-// to its producers), key row * B + col.  Mode 1: the off-diagonal blocks of
-// the upper BCOO, key col * B + row (the transpose).
+// to its producers), key (row, col).  Mode 1: the off-diagonal blocks of the
+// upper BCOO, key (col, row) (the transpose).
 template <int MODE>
 __device__ __forceinline__ bool keep_entry(int32_t r, int32_t c, int32_t B)
 {
@@ -42,12 +42,12 @@ __device__ __forceinline__ bool keep_entry(int32_t r, int32_t c, int32_t B)
         return r != c;
 }
 template <int MODE>
-__device__ __forceinline__ uint64_t entry_key(int32_t r, int32_t c, int32_t B)
+__device__ __forceinline__ uint64_t entry_key(int32_t r, int32_t c, const KeyCodec& kc)
 {
     if constexpr(MODE == 0)
-        return (uint64_t)r * (uint64_t)B + (uint64_t)c;
+        return kc.up(r, c);
     else
-        return (uint64_t)c * (uint64_t)B + (uint64_t)r;
+        return kc.lo(r, c);
 }
 
 // stage one tile of (row, col) into LDS with coalesced loads
@@ -70,26 +70,53 @@ __device__ __forceinline__ void stage_rc(const int32_t* rows,
 }
 
 // ---- 1. count kept entries per tile
+// (and, with d_band, the largest col - row of the kept entries)
 template <int MODE>
 __global__ __launch_bounds__(kThreads) void cv_count_kernel(int64_t        T,
                                                            const int32_t* rows,
                                                            const int32_t* cols,
                                                            int32_t        B,
-                                                           uint32_t*      tile_cnt)
+                                                           uint32_t*      tile_cnt,
+                                                           int32_t*       d_band)
 {
     __shared__ uint32_t lds[kWaves];
+    __shared__ int32_t  bmax[kWaves];
     const int64_t       base = (int64_t)blockIdx.x * kTile;
     uint32_t            c    = 0;
+    int32_t             band = 0;
 #pragma unroll
     for(int k = 0; k < kItems; ++k)
     {
         int64_t i = base + (int64_t)k * kThreads + threadIdx.x;
-        if(i < T && keep_entry<MODE>(rows[i], cols[i], B))
-            ++c;
+        if(i < T)
+        {
+            const int32_t r = rows[i], cc = cols[i];
+            if(keep_entry<MODE>(r, cc, B))
+            {
+                ++c;
+                band = max(band, cc - r);
+            }
+        }
     }
     uint32_t s = block_sum<kThreads>(c, lds);
     if(threadIdx.x == 0)
         tile_cnt[blockIdx.x] = s;
+    if(d_band)
+    {
+#pragma unroll
+        for(int o = kWave / 2; o > 0; o >>= 1)
+            band = max(band, __shfl_xor(band, o, kWave));
+        if(threadIdx.x % kWave == 0)
+            bmax[threadIdx.x / kWave] = band;
+        __syncthreads();
+        if(threadIdx.x == 0)
+        {
+            int32_t m = bmax[0];
+            for(int w = 1; w < kWaves; ++w)
+                m = max(m, bmax[w]);
+            atomicMax(d_band, m);
+        }
+    }
 }
 
 // ---- 2. stable compaction of (key, source index).  The tile is staged in
@@ -97,13 +124,14 @@ __global__ __launch_bounds__(kThreads) void cv_count_kernel(int64_t        T,
 // [t * kItems, (t + 1) * kItems) so that the block prefix follows index order.
 // The kept entries are gathered back into LDS in that order and written out
 // by consecutive lanes (one lane per item: coalesced stores).
-template <int MODE>
+template <int MODE, typename KeyT>
 __global__ __launch_bounds__(kThreads) void cv_compact_kernel(int64_t         T,
                                                              const int32_t*  rows,
                                                              const int32_t*  cols,
                                                              int32_t         B,
+                                                             KeyCodec        kc,
                                                              const uint32_t* tile_off,
-                                                             uint64_t*       keys,
+                                                             KeyT*           keys,
                                                              int32_t*        srcs)
 {
     __shared__ int32_t  sr[kTile], sc[kTile], si[kTile];
@@ -139,7 +167,7 @@ __global__ __launch_bounds__(kThreads) void cv_compact_kernel(int64_t         T,
     const uint32_t g = tile_off[blockIdx.x];
     for(uint32_t i = threadIdx.x; i < kept; i += kThreads)
     {
-        keys[g + i] = entry_key<MODE>(sr[i], sc[i], B);
+        keys[g + i] = (KeyT)entry_key<MODE>(sr[i], sc[i], kc);
         srcs[g + i] = (int32_t)(base + si[i]);
     }
 }
@@ -219,8 +247,9 @@ __global__ __launch_bounds__(kThreads) void scan_down_kernel(uint32_t* data, int
 }
 
 // ---- radix sort: tile histogram (digit-major layout hist[d * ntiles + tile])
+template <typename KeyT>
 __global__ __launch_bounds__(kThreads) void rs_hist_kernel(const int64_t*  d_m,
-                                                          const uint64_t* keys,
+                                                          const KeyT*     keys,
                                                           int             shift,
                                                           uint32_t        dmask,
                                                           uint32_t*       hist,
@@ -251,17 +280,18 @@ __device__ __forceinline__ uint64_t lanemask_lt64()
 // of its contiguous quarter of the tile in order (ballot-matched peers +
 // per-wave digit counters), the tile is reordered by digit in LDS, then each
 // digit run is written out contiguously.
+template <typename KeyT>
 __global__ __launch_bounds__(kThreads) void rs_scatter_kernel(const int64_t*  d_m,
-                                                             const uint64_t* kin,
+                                                             const KeyT*     kin,
                                                              const int32_t*  vin,
-                                                             uint64_t*       kout,
+                                                             KeyT*           kout,
                                                              int32_t*        vout,
                                                              int             shift,
                                                              uint32_t        dmask,
                                                              const uint32_t* hist_scan,
                                                              int             ntiles)
 {
-    __shared__ uint64_t sk[kTile];
+    __shared__ KeyT     sk[kTile];
     __shared__ int32_t  sv[kTile];
     __shared__ uint32_t wcnt[kWaves][256];
     __shared__ uint32_t dstart[256];
@@ -279,7 +309,7 @@ __global__ __launch_bounds__(kThreads) void rs_scatter_kernel(const int64_t*  d_
         wcnt[q][threadIdx.x] = 0;
     __syncthreads();
 
-    uint64_t       key[kItems];
+    KeyT           key[kItems];
     int32_t        val[kItems];
     uint32_t       rank[kItems];
     uint32_t       dig[kItems];
@@ -289,7 +319,7 @@ __global__ __launch_bounds__(kThreads) void rs_scatter_kernel(const int64_t*  d_
     {
         int64_t  i     = base + (int64_t)w * (kWave * kItems) + (int64_t)k * kWave + lane;
         bool     valid = i < m;
-        key[k]         = valid ? kin[i] : 0ull;
+        key[k]         = valid ? kin[i] : KeyT(0);
         val[k]         = valid ? vin[i] : 0;
         uint32_t d     = valid ? ((uint32_t)(key[k] >> shift) & dmask) : 0u;
         dig[k]         = valid ? d : 0xFFFFFFFFu;
@@ -345,7 +375,7 @@ __global__ __launch_bounds__(kThreads) void rs_scatter_kernel(const int64_t*  d_
         int i = k * kThreads + threadIdx.x;
         if(i < cnt)
         {
-            uint64_t kk  = sk[i];
+            KeyT     kk  = sk[i];
             uint32_t d   = (uint32_t)(kk >> shift) & dmask;
             uint32_t out = gbase[d] + ((uint32_t)i - dstart[d]);
             kout[out]    = kk;
@@ -355,22 +385,24 @@ __global__ __launch_bounds__(kThreads) void rs_scatter_kernel(const int64_t*  d_
 }
 
 // stage one tile of sorted keys into LDS (plus the key before the tile)
-__device__ __forceinline__ void stage_keys(const uint64_t* keys, int64_t m, int64_t base, uint64_t* sk)
+template <typename KeyT>
+__device__ __forceinline__ void stage_keys(const KeyT* keys, int64_t m, int64_t base, KeyT* sk)
 {
 #pragma unroll
     for(int k = 0; k < kItems; ++k)
     {
         int     t = k * kThreads + threadIdx.x;
         int64_t i = base + t;
-        sk[t + 1] = i < m ? keys[i] : ~0ull;
+        sk[t + 1] = i < m ? keys[i] : ~KeyT(0);
     }
     if(threadIdx.x == 0)
-        sk[0] = base > 0 ? keys[base - 1] : ~0ull;
+        sk[0] = base > 0 ? keys[base - 1] : ~KeyT(0);
     __syncthreads();
 }
 
 // ---- run-length encode: heads per tile
-__global__ __launch_bounds__(kThreads) void rle_count_kernel(const int64_t* d_m, const uint64_t* keys, uint32_t* tile_cnt)
+template <typename KeyT>
+__global__ __launch_bounds__(kThreads) void rle_count_kernel(const int64_t* d_m, const KeyT* keys, uint32_t* tile_cnt)
 {
     __shared__ uint32_t lds[kWaves];
     const int64_t       m    = *d_m;
@@ -388,15 +420,16 @@ __global__ __launch_bounds__(kThreads) void rle_count_kernel(const int64_t* d_m,
         tile_cnt[blockIdx.x] = s;
 }
 
+template <typename KeyT>
 __global__ __launch_bounds__(kThreads) void rle_write_kernel(const int64_t*  d_m,
-                                                            const uint64_t* keys,
+                                                            const KeyT*     keys,
                                                             const uint32_t* tile_off,
-                                                            int32_t         B,
+                                                            KeyCodec        kc,
                                                             int32_t*        out_rows,
                                                             int32_t*        out_cols,
                                                             int32_t*        seg_start)
 {
-    __shared__ uint64_t sk[kTile + 1];
+    __shared__ KeyT     sk[kTile + 1];
     __shared__ int32_t  hp[kTile];
     __shared__ uint32_t lds[kWaves + 1];
     const int64_t       m    = *d_m;
@@ -404,7 +437,7 @@ __global__ __launch_bounds__(kThreads) void rle_write_kernel(const int64_t*  d_m
     stage_keys(keys, m, base, sk);
     const int first = threadIdx.x * kItems;
     uint32_t  flags = 0, c = 0;
-    uint64_t  kk[kItems];
+    KeyT      kk[kItems];
 #pragma unroll
     for(int k = 0; k < kItems; ++k)
     {
@@ -432,10 +465,10 @@ __global__ __launch_bounds__(kThreads) void rle_write_kernel(const int64_t*  d_m
     const uint32_t g = tile_off[blockIdx.x];
     for(uint32_t i = threadIdx.x; i < heads; i += kThreads)
     {
-        const uint64_t key = sk[i];
-        const int32_t  r   = (int32_t)(key / (uint64_t)B);
+        int32_t r, c;
+        kc.up_rc((uint64_t)sk[i], r, c);
         out_rows[g + i]    = r;
-        out_cols[g + i]    = (int32_t)(key - (uint64_t)r * (uint64_t)B);
+        out_cols[g + i]    = c;
         seg_start[g + i]   = (int32_t)(base + hp[i]);
     }
 }
@@ -506,11 +539,16 @@ __global__ __launch_bounds__(kThreads) void row_ptr_kernel(const int64_t* d_n, i
 
 // ---- full BSR: lower part (transposed upper blocks, sorted by (col, row)) then
 // upper part of each row
-__global__ void unpack_lower_kernel(const int64_t* d_n, const uint64_t* keys, int32_t B, int32_t* lrow)
+template <typename KeyT>
+__global__ void unpack_lower_kernel(const int64_t* d_n, const KeyT* keys, KeyCodec kc, int32_t* lrow)
 {
     const int64_t n = *d_n;
     for(int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
-        lrow[k] = (int32_t)(keys[k] / (uint64_t)B);
+    {
+        int32_t lr, lc;
+        kc.lo_rc((uint64_t)keys[k], lr, lc);
+        lrow[k] = lr;
+    }
 }
 
 // chunk_row[c] = the first row whose first full-BSR block is at or after
@@ -574,9 +612,10 @@ __global__ void full_empty_kernel(const int32_t* lower_ptr,
         }
 }
 
-template <typename VT>
+template <typename VT, typename KeyT>
 __global__ void full_lower_kernel(const int64_t*  d_n,
-                                  const uint64_t* keys,
+                                  const KeyT*     keys,
+                                  KeyCodec        kc,
                                   const int32_t*  src,
                                   const int32_t*  lower_ptr,
                                   const int32_t*  full_ptr,
@@ -589,9 +628,8 @@ __global__ void full_lower_kernel(const int64_t*  d_n,
     const int64_t n = *d_n;
     for(int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
     {
-        const uint64_t key = keys[k];
-        const int32_t  i   = (int32_t)(key / (uint64_t)B);        // lower row = upper col
-        const int32_t  j   = (int32_t)(key - (uint64_t)i * B);    // lower col = upper row
+        int32_t i, j;  // lower row = upper col, lower col = upper row
+        kc.lo_rc((uint64_t)keys[k], i, j);
         const int64_t  u   = src[k];
         const int64_t  d   = (int64_t)full_ptr[i] + (k - lower_ptr[i]);
         fcols[d]           = j;
@@ -627,32 +665,46 @@ __global__ void full_upper_kernel(int64_t        U,
 }
 
 // ---- window layout (BsrMatrix::w*): blocks per row position; a row holds its
-// upper blocks and the lower blocks whose other row is in another window
-__global__ void win_count_kernel(int32_t        B,
-                                 int            shift,
-                                 const int32_t* __restrict__ pos,
-                                 const int32_t* __restrict__ real,
-                                 const int32_t* __restrict__ row_ptr,
-                                 const int32_t* __restrict__ lower_ptr,
-                                 const uint64_t* __restrict__ lkeys,
-                                 uint32_t*      cnt)
+// upper blocks and the lower blocks whose other row is in another window.
+// 16 lanes per row walk the row's lower entries together (coalesced reads of
+// the transposed keys; one lane per row read them 16 times over).
+constexpr int kRowLanes = 16;
+
+template <typename KeyT>
+__global__ __launch_bounds__(kThreads) void win_count_kernel(int32_t        B,
+                                                            int            shift,
+                                                            const int32_t* __restrict__ pos,
+                                                            const int32_t* __restrict__ real,
+                                                            const int32_t* __restrict__ row_ptr,
+                                                            const int32_t* __restrict__ lower_ptr,
+                                                            const KeyT* __restrict__ lkeys,
+                                                            KeyCodec       kc,
+                                                            uint32_t*      cnt)
 {
-    for(int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p <= B; p += (int64_t)gridDim.x * blockDim.x)
+    const int64_t groups = (int64_t)gridDim.x * (kThreads / kRowLanes);
+    const int     lane   = threadIdx.x % kRowLanes;
+    for(int64_t p = ((int64_t)blockIdx.x * kThreads + threadIdx.x) / kRowLanes; p <= B; p += groups)
     {
         if(p == B)
         {
-            cnt[p] = 0;
+            if(lane == 0)
+                cnt[p] = 0;
             continue;
         }
         const int32_t v  = real[p];
         const int32_t ch = (int32_t)p >> shift;
-        uint32_t      c  = (uint32_t)(row_ptr[v + 1] - row_ptr[v]);
-        for(int32_t k = lower_ptr[v]; k < lower_ptr[v + 1]; ++k)
+        uint32_t      c  = 0;
+        for(int32_t k = lower_ptr[v] + lane; k < lower_ptr[v + 1]; k += kRowLanes)
         {
-            const int32_t i = (int32_t)(lkeys[k] - (uint64_t)v * (uint64_t)B);  // the upper row
+            int32_t lr, i;  // i: the upper row
+            kc.lo_rc((uint64_t)lkeys[k], lr, i);
             c += (pos[i] >> shift) != ch ? 1u : 0u;
         }
-        cnt[p] = c;
+#pragma unroll
+        for(int o = kRowLanes / 2; o > 0; o >>= 1)
+            c += __shfl_xor(c, o, kWave);
+        if(lane == 0)
+            cnt[p] = c + (uint32_t)(row_ptr[v + 1] - row_ptr[v]);
     }
 }
 
@@ -687,39 +739,67 @@ __global__ void win_upper_kernel(int64_t        U,
     }
 }
 
-template <typename VT>
-__global__ void win_lower_kernel(int32_t        B,
-                                 int            shift,
-                                 const int32_t* __restrict__ pos,
-                                 const int32_t* __restrict__ real,
-                                 const int32_t* __restrict__ row_ptr,
-                                 const int32_t* __restrict__ lower_ptr,
-                                 const uint64_t* __restrict__ lkeys,
-                                 const int32_t* __restrict__ lsrc,
-                                 const int32_t* __restrict__ wptr,
-                                 const double* __restrict__ uvals,
-                                 int32_t*       words,
-                                 VT*            wvals,
-                                 int64_t        stride)
+template <typename VT, typename KeyT>
+__global__ __launch_bounds__(kThreads) void win_lower_kernel(int32_t        B,
+                                                            int            shift,
+                                                            const int32_t* __restrict__ pos,
+                                                            const int32_t* __restrict__ real,
+                                                            const int32_t* __restrict__ row_ptr,
+                                                            const int32_t* __restrict__ lower_ptr,
+                                                            const KeyT* __restrict__ lkeys,
+                                                            KeyCodec       kc,
+                                                            const int32_t* __restrict__ lsrc,
+                                                            const int32_t* __restrict__ wptr,
+                                                            const double* __restrict__ uvals,
+                                                            int32_t*       words,
+                                                            VT*            wvals,
+                                                            int64_t        stride)
 {
-    for(int32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < B; p += gridDim.x * blockDim.x)
+    const int64_t groups = (int64_t)gridDim.x * (kThreads / kRowLanes);
+    const int     lane   = threadIdx.x % kRowLanes;
+    const int     gsh    = (int)(threadIdx.x % kWave) & ~(kRowLanes - 1);  // the group's first lane in the wave
+    // the group's rows run in lockstep: every group of a wave takes the same
+    // number of rounds (the wave-wide ballots below need all 64 lanes)
+    const int64_t first  = ((int64_t)blockIdx.x * kThreads + threadIdx.x) / kRowLanes;
+    const int64_t rounds = (B + groups - 1) / groups;
+    for(int64_t it = 0; it < rounds; ++it)
     {
-        const int32_t v  = real[p];
-        const int32_t ch = p >> shift;
-        int64_t       d  = (int64_t)wptr[p] + (row_ptr[v + 1] - row_ptr[v]);
-        for(int32_t k = lower_ptr[v]; k < lower_ptr[v + 1]; ++k)
+        const int64_t p    = first + it * groups;
+        const bool    live = p < B;
+        const int32_t v    = live ? real[p] : 0;
+        const int32_t ch   = (int32_t)p >> shift;
+        const int32_t lo = live ? lower_ptr[v] : 0, hi = live ? lower_ptr[v + 1] : 0;
+        int64_t       d  = live ? (int64_t)wptr[p] + (row_ptr[v + 1] - row_ptr[v]) : 0;
+        // the wave's longest lower list bounds its rounds
+        int32_t len = hi - lo;
+#pragma unroll
+        for(int o = kWave / 2; o > 0; o >>= 1)
+            len = max(len, __shfl_xor(len, o, kWave));
+        for(int32_t b = 0; b < len; b += kRowLanes)
         {
-            const int32_t i = (int32_t)(lkeys[k] - (uint64_t)v * (uint64_t)B);
-            if((pos[i] >> shift) == ch)
-                continue;  // stored once, as the inner block of row i
-            const int64_t u = lsrc[k];
-            words[d]        = i;
+            const int32_t k   = lo + b + lane;
+            int32_t       i   = 0;
+            bool          out = false;
+            if(k < hi)
+            {
+                int32_t lr;
+                kc.lo_rc((uint64_t)lkeys[k], lr, i);
+                out = (pos[i] >> shift) != ch;  // an inner block is stored once, with row i
+            }
+            const uint64_t  bal  = __ballot(out);
+            const uint32_t  mine = (uint32_t)(bal >> gsh) & ((1u << kRowLanes) - 1u);
+            if(out)
+            {
+                const int64_t dk = d + __popc(mine & ((1u << lane) - 1u));
+                const int64_t u  = lsrc[k];
+                words[dk]        = i;
 #pragma unroll
-            for(int r = 0; r < 3; ++r)
+                for(int r = 0; r < 3; ++r)
 #pragma unroll
-                for(int c = 0; c < 3; ++c)
-                    wvals[(r + 3 * c) * stride + d] = (VT)uvals[upper_at(u, c + 3 * r)];
-            ++d;
+                    for(int c = 0; c < 3; ++c)
+                        wvals[(r + 3 * c) * stride + dk] = (VT)uvals[upper_at(u, c + 3 * r)];
+            }
+            d += __popc(mine);
         }
     }
 }
@@ -782,7 +862,7 @@ MatrixConverter::~MatrixConverter()
 
 // compaction + stable radix sort of (key, source) pairs; returns the buffer
 // index holding the result, *d_count = number of kept entries (device)
-template <int MODE>
+template <int MODE, typename KeyT>
 int MatrixConverter::compact_sort(int64_t        n,
                                   const int32_t* rows,
                                   const int32_t* cols,
@@ -798,14 +878,16 @@ int MatrixConverter::compact_sort(int64_t        n,
     srcs_[0].resize((size_t)n, ratio);
     srcs_[1].resize((size_t)n, ratio);
     hist_.resize((size_t)256 * ntiles, ratio);
-    cv_count_kernel<MODE><<<ntiles, kThreads, 0, s>>>(n, rows, cols, B, tile_cnt_.data());
+    if(!(MODE == 0 && counted_))
+        cv_count_kernel<MODE><<<ntiles, kThreads, 0, s>>>(n, rows, cols, B, tile_cnt_.data(), nullptr);
+    counted_ = false;
     scan_exclusive(tile_cnt_.data(), ntiles, chunk_, d_count, s);
-    cv_compact_kernel<MODE><<<ntiles, kThreads, 0, s>>>(n, rows, cols, B, tile_cnt_.data(), keys_[0].data(),
-                                                        srcs_[0].data());
-    // stable LSD radix sort (matrix_converter.inl:1610-1616)
-    const int bits = key_bits_for((uint64_t)B * (uint64_t)B - 1ull);
-    return radix_sort_pairs(keys_[0].data(), srcs_[0].data(), keys_[1].data(), srcs_[1].data(), d_count, n, bits,
-                            hist_, chunk_, s);
+    KeyT* k0 = reinterpret_cast<KeyT*>(keys_[0].data());
+    KeyT* k1 = reinterpret_cast<KeyT*>(keys_[1].data());
+    cv_compact_kernel<MODE, KeyT><<<ntiles, kThreads, 0, s>>>(n, rows, cols, B, codec_, tile_cnt_.data(), k0,
+                                                              srcs_[0].data());
+    // stable LSD radix sort (matrix_converter.inl:1610-1616) on the key's bits
+    return radix_sort_pairs_t<KeyT>(k0, srcs_[0].data(), k1, srcs_[1].data(), d_count, n, key_bits_, hist_, chunk_, s);
 }
 
 void MatrixConverter::build_window(BsrMatrix& A, int shape, bool want32, double ratio, hipStream_t s)
@@ -831,9 +913,15 @@ void MatrixConverter::build_window(BsrMatrix& A, int shape, bool want32, double 
     }
     const int64_t U = A.U;
     uint32_t*     cnt = reinterpret_cast<uint32_t*>(A.wptr.data());  // scanned in place
-    win_count_kernel<<<grid_of(B + 1), kThreads, 0, s>>>(B, shift, A.wpos.data(), A.wreal.data(),
-                                                         A.row_ptr.data(), lower_ptr_.data(),
-                                                         U > 0 ? keys_[tcur_].data() : nullptr, cnt);
+    const bool narrow = codec_.bb > 0;
+    if(narrow)
+        win_count_kernel<uint32_t><<<grid_of(((int64_t)B + 1) * kRowLanes), kThreads, 0, s>>>(
+            B, shift, A.wpos.data(), A.wreal.data(), A.row_ptr.data(), lower_ptr_.data(),
+            U > 0 ? key_ptr<uint32_t>(tcur_) : nullptr, codec(), cnt);
+    else
+        win_count_kernel<uint64_t><<<grid_of(((int64_t)B + 1) * kRowLanes), kThreads, 0, s>>>(
+            B, shift, A.wpos.data(), A.wreal.data(), A.row_ptr.data(), lower_ptr_.data(),
+            U > 0 ? key_ptr<uint64_t>(tcur_) : nullptr, codec(), cnt);
     scan_exclusive(cnt, (int64_t)B + 1, chunk_, counts_.data() + 4, s);
     win_range_kernel<<<grid_of(A.wnchunks), kThreads, 0, s>>>(B, shift, A.wnchunks, A.wreal.data(),
                                                               A.wchunk_lo.data(), A.wchunk_hi.data());
@@ -842,6 +930,19 @@ void MatrixConverter::build_window(BsrMatrix& A, int shape, bool want32, double 
     const int64_t wmax = 2 * U;
     A.wwords.resize((size_t)wmax + 2, ratio);
     A.wstride = ((int64_t)A.wwords.cap + 1) & ~(int64_t)1;
+    auto win_lower = [&](auto* wv) {
+        using VT = std::remove_pointer_t<decltype(wv)>;
+        if(narrow)
+            win_lower_kernel<VT, uint32_t><<<grid_of((int64_t)B * kRowLanes), kThreads, 0, s>>>(
+                B, shift, A.wpos.data(), A.wreal.data(), A.row_ptr.data(), lower_ptr_.data(),
+                key_ptr<uint32_t>(tcur_), codec(), srcs_[tcur_].data(), A.wptr.data(), A.vals.data(),
+                A.wwords.data(), wv, A.wstride);
+        else
+            win_lower_kernel<VT, uint64_t><<<grid_of((int64_t)B * kRowLanes), kThreads, 0, s>>>(
+                B, shift, A.wpos.data(), A.wreal.data(), A.row_ptr.data(), lower_ptr_.data(),
+                key_ptr<uint64_t>(tcur_), codec(), srcs_[tcur_].data(), A.wptr.data(), A.vals.data(),
+                A.wwords.data(), wv, A.wstride);
+    };
     if(U > 0)
     {
         if(want32)
@@ -851,9 +952,7 @@ void MatrixConverter::build_window(BsrMatrix& A, int shape, bool want32, double 
                                                                     A.row_ptr.data(), A.wpos.data(), A.wptr.data(),
                                                                     A.vals.data(), A.wwords.data(), A.wvals32.data(),
                                                                     A.wstride);
-            win_lower_kernel<float><<<grid_of(B), kThreads, 0, s>>>(
-                B, shift, A.wpos.data(), A.wreal.data(), A.row_ptr.data(), lower_ptr_.data(), keys_[tcur_].data(),
-                srcs_[tcur_].data(), A.wptr.data(), A.vals.data(), A.wwords.data(), A.wvals32.data(), A.wstride);
+            win_lower(A.wvals32.data());
         }
         else
         {
@@ -862,9 +961,7 @@ void MatrixConverter::build_window(BsrMatrix& A, int shape, bool want32, double 
                                                                      A.row_ptr.data(), A.wpos.data(), A.wptr.data(),
                                                                      A.vals.data(), A.wwords.data(), A.wvals.data(),
                                                                      A.wstride);
-            win_lower_kernel<double><<<grid_of(B), kThreads, 0, s>>>(
-                B, shift, A.wpos.data(), A.wreal.data(), A.row_ptr.data(), lower_ptr_.data(), keys_[tcur_].data(),
-                srcs_[tcur_].data(), A.wptr.data(), A.vals.data(), A.wwords.data(), A.wvals.data(), A.wstride);
+            win_lower(A.wvals.data());
         }
     }
     else if(want32)
@@ -874,6 +971,38 @@ void MatrixConverter::build_window(BsrMatrix& A, int shape, bool want32, double 
     MI_HIP(hipGetLastError());
     A.wblocks = -1;  // read back lazily (window_blocks)
     A.has_win = true;
+}
+
+template <typename KeyT>
+int radix_sort_pairs_t(KeyT*           k0,
+                       int32_t*        v0,
+                       KeyT*           k1,
+                       int32_t*        v1,
+                       const int64_t*  d_n,
+                       int64_t         nmax,
+                       int             bits,
+                       DBuf<uint32_t>& hist,
+                       DBuf<uint32_t>& chunk,
+                       hipStream_t     s)
+{
+    const int ntiles = ceil_div(nmax, kTile);
+    if(ntiles == 0)
+        return 0;
+    hist.resize((size_t)256 * ntiles);
+    KeyT*    kb[2] = {k0, k1};
+    int32_t* vb[2] = {v0, v1};
+    int      cur   = 0;
+    for(int shift = 0; shift < bits; shift += 8)
+    {
+        int      w     = (bits - shift) < 8 ? (bits - shift) : 8;
+        uint32_t dmask = (1u << w) - 1u;
+        rs_hist_kernel<KeyT><<<ntiles, kThreads, 0, s>>>(d_n, kb[cur], shift, dmask, hist.data(), ntiles);
+        scan_exclusive(hist.data(), (int64_t)256 * ntiles, chunk, nullptr, s);
+        rs_scatter_kernel<KeyT><<<ntiles, kThreads, 0, s>>>(d_n, kb[cur], vb[cur], kb[cur ^ 1], vb[cur ^ 1], shift,
+                                                            dmask, hist.data(), ntiles);
+        cur ^= 1;
+    }
+    return cur;
 }
 
 int radix_sort_pairs(uint64_t*       k0,
@@ -887,24 +1016,7 @@ int radix_sort_pairs(uint64_t*       k0,
                      DBuf<uint32_t>& chunk,
                      hipStream_t     s)
 {
-    const int ntiles = ceil_div(nmax, kTile);
-    if(ntiles == 0)
-        return 0;
-    hist.resize((size_t)256 * ntiles);
-    uint64_t* kb[2] = {k0, k1};
-    int32_t*  vb[2] = {v0, v1};
-    int       cur   = 0;
-    for(int shift = 0; shift < bits; shift += 8)
-    {
-        int      w     = (bits - shift) < 8 ? (bits - shift) : 8;
-        uint32_t dmask = (1u << w) - 1u;
-        rs_hist_kernel<<<ntiles, kThreads, 0, s>>>(d_n, kb[cur], shift, dmask, hist.data(), ntiles);
-        scan_exclusive(hist.data(), (int64_t)256 * ntiles, chunk, nullptr, s);
-        rs_scatter_kernel<<<ntiles, kThreads, 0, s>>>(d_n, kb[cur], vb[cur], kb[cur ^ 1], vb[cur ^ 1], shift, dmask,
-                                                      hist.data(), ntiles);
-        cur ^= 1;
-    }
-    return cur;
+    return radix_sort_pairs_t<uint64_t>(k0, v0, k1, v1, d_n, nmax, bits, hist, chunk, s);
 }
 
 void exclusive_scan_u32(uint32_t* data, int64_t n, DBuf<uint32_t>& chunk, int64_t* d_total, hipStream_t s)
@@ -936,9 +1048,10 @@ void MatrixConverter::convert_sym(int32_t        B,
     out.full_ptr.resize((size_t)B + 1, ratio);
     out.has32 = want32;
     if(!pinned_)
-        MI_HIP(hipHostMalloc((void**)&pinned_, 4 * sizeof(int64_t)));
-    counts_.resize(5);
-    MI_HIP(hipMemsetAsync(counts_.data(), 0, 5 * sizeof(int64_t), s));
+        MI_HIP(hipHostMalloc((void**)&pinned_, 8 * sizeof(int64_t)));
+    counts_.resize(6);
+    MI_HIP(hipMemsetAsync(counts_.data(), 0, 6 * sizeof(int64_t), s));
+    codec_ = KeyCodec{(uint64_t)std::max(B, 1), 0, 0};
     out.has_win = false;
     if(T == 0 || B == 0)
     {
@@ -953,12 +1066,44 @@ void MatrixConverter::convert_sym(int32_t        B,
     int64_t* d_u   = counts_.data() + 1;  // unique upper blocks
     int64_t* d_off = counts_.data() + 2;  // off-diagonal upper blocks
 
-    // 1-4. flag, scan, compact, stable radix sort (matrix_converter.inl:1579-1616)
-    int cur          = compact_sort<0>(T, rows, cols, B, d_m, ratio, s);
+    // 1. count the kept triplets and their largest col - row; one host read
+    // of the band chooses the key (narrow 32-bit keys when they fit)
     const int ntiles = ceil_div(T, kTile);
+    {
+        tile_cnt_.resize((size_t)ntiles + 1, ratio);
+        int32_t* d_band = reinterpret_cast<int32_t*>(counts_.data() + 5);
+        cv_count_kernel<0><<<ntiles, kThreads, 0, s>>>(T, rows, cols, B, tile_cnt_.data(), d_band);
+        MI_HIP(hipMemcpyAsync(pinned_ + 5, counts_.data() + 5, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        MI_HIP(hipStreamSynchronize(s));
+        const int32_t band  = (int32_t)(pinned_[5] & 0x7fffffff);
+        const int     bb    = key_bits_for((uint64_t)std::max(band, 1));
+        const int     nbits = key_bits_for(((uint64_t)(B - 1) << bb) | (uint64_t)band);
+        static const bool wide_only = [] {
+            const char* e = std::getenv("MI_SORT_KEY");
+            return e && std::string(e) == "wide";
+        }();
+        if(!wide_only && nbits <= 32)
+        {
+            codec_    = KeyCodec{(uint64_t)B, bb, band};
+            key_bits_ = nbits;
+        }
+        else
+        {
+            codec_    = KeyCodec{(uint64_t)B, 0, 0};
+            key_bits_ = key_bits_for((uint64_t)B * (uint64_t)B - 1ull);
+        }
+        counted_ = true;
+    }
+    const bool narrow = codec_.bb > 0;
+    // 2-4. scan, compact, stable radix sort (matrix_converter.inl:1579-1616)
+    const int cur = narrow ? compact_sort<0, uint32_t>(T, rows, cols, B, d_m, ratio, s)
+                           : compact_sort<0, uint64_t>(T, rows, cols, B, d_m, ratio, s);
 
     // 5-6. run-length encode (1626-1650); the one host read-back sizes the outputs
-    rle_count_kernel<<<ntiles, kThreads, 0, s>>>(d_m, keys_[cur].data(), tile_cnt_.data());
+    if(narrow)
+        rle_count_kernel<uint32_t><<<ntiles, kThreads, 0, s>>>(d_m, key_ptr<uint32_t>(cur), tile_cnt_.data());
+    else
+        rle_count_kernel<uint64_t><<<ntiles, kThreads, 0, s>>>(d_m, key_ptr<uint64_t>(cur), tile_cnt_.data());
     scan_exclusive(tile_cnt_.data(), ntiles, chunk_, d_u, s);
     MI_HIP(hipMemcpyAsync(pinned_, counts_.data(), 2 * sizeof(int64_t), hipMemcpyDeviceToHost, s));
     MI_HIP(hipStreamSynchronize(s));
@@ -968,8 +1113,14 @@ void MatrixConverter::convert_sym(int32_t        B,
     out.cols.resize((size_t)(U > 0 ? U : 1), ratio);
     out.vals.resize((size_t)9 * out.rows.cap);
     seg_start_.resize((size_t)(U > 0 ? U : 1), ratio);
-    rle_write_kernel<<<ntiles, kThreads, 0, s>>>(
-        d_m, keys_[cur].data(), tile_cnt_.data(), B, out.rows.data(), out.cols.data(), seg_start_.data());
+    if(narrow)
+        rle_write_kernel<uint32_t><<<ntiles, kThreads, 0, s>>>(d_m, key_ptr<uint32_t>(cur), tile_cnt_.data(),
+                                                               codec(), out.rows.data(), out.cols.data(),
+                                                               seg_start_.data());
+    else
+        rle_write_kernel<uint64_t><<<ntiles, kThreads, 0, s>>>(d_m, key_ptr<uint64_t>(cur), tile_cnt_.data(),
+                                                               codec(), out.rows.data(), out.cols.data(),
+                                                               seg_start_.data());
 
     // 7-8. segmented reduce over the sorted permutation (1652-1698)
     if(U > 0)
@@ -977,12 +1128,18 @@ void MatrixConverter::convert_sym(int32_t        B,
             U, d_m, seg_start_.data(), srcs_[cur].data(), vals, out.vals.data());
     row_ptr_kernel<<<grid_of(U + 1), kThreads, 0, s>>>(nullptr, U, out.rows.data(), out.row_ptr.data(), B);
 
-    // full BSR: transpose of the off-diagonal upper blocks + the upper blocks
-    int tcur = compact_sort<1>(U, out.rows.data(), out.cols.data(), B, d_off, ratio, s);
-    tcur_    = tcur;
+    // the transpose of the off-diagonal upper blocks (the lower triangle)
+    const int tcur = narrow ? compact_sort<1, uint32_t>(U, out.rows.data(), out.cols.data(), B, d_off, ratio, s)
+                            : compact_sort<1, uint64_t>(U, out.rows.data(), out.cols.data(), B, d_off, ratio, s);
+    tcur_          = tcur;
     lower_rows_.resize((size_t)(U > 0 ? U : 1), ratio);
     lower_ptr_.resize((size_t)B + 1, ratio);
-    unpack_lower_kernel<<<grid_of(U), kThreads, 0, s>>>(d_off, keys_[tcur].data(), B, lower_rows_.data());
+    if(narrow)
+        unpack_lower_kernel<uint32_t><<<grid_of(U), kThreads, 0, s>>>(d_off, key_ptr<uint32_t>(tcur), codec(),
+                                                                      lower_rows_.data());
+    else
+        unpack_lower_kernel<uint64_t><<<grid_of(U), kThreads, 0, s>>>(d_off, key_ptr<uint64_t>(tcur), codec(),
+                                                                      lower_rows_.data());
     row_ptr_kernel<<<grid_of(U + 1), kThreads, 0, s>>>(d_off, U, lower_rows_.data(), lower_ptr_.data(), B);
     out.has_win = false;
     if(!build_full)
@@ -1005,13 +1162,21 @@ void MatrixConverter::convert_sym(int32_t        B,
     // an even plane stride keeps every pair 8-byte aligned
     out.fcols.resize((size_t)fmax + 2, ratio);
     out.fstride = ((int64_t)out.fcols.cap + 1) & ~(int64_t)1;
+    auto full_lower = [&](auto* fv) {
+        using VT = std::remove_pointer_t<decltype(fv)>;
+        if(narrow)
+            full_lower_kernel<VT, uint32_t><<<grid_of(U), kThreads, 0, s>>>(
+                d_off, key_ptr<uint32_t>(tcur), codec(), srcs_[tcur].data(), lower_ptr_.data(), out.full_ptr.data(),
+                out.vals.data(), B, out.fcols.data(), fv, out.fstride);
+        else
+            full_lower_kernel<VT, uint64_t><<<grid_of(U), kThreads, 0, s>>>(
+                d_off, key_ptr<uint64_t>(tcur), codec(), srcs_[tcur].data(), lower_ptr_.data(), out.full_ptr.data(),
+                out.vals.data(), B, out.fcols.data(), fv, out.fstride);
+    };
     if(want32)
     {
         out.fvals32.resize((size_t)9 * out.fstride);
-        full_lower_kernel<float><<<grid_of(U), kThreads, 0, s>>>(d_off, keys_[tcur].data(), srcs_[tcur].data(),
-                                                                 lower_ptr_.data(), out.full_ptr.data(),
-                                                                 out.vals.data(), B, out.fcols.data(),
-                                                                 out.fvals32.data(), out.fstride);
+        full_lower(out.fvals32.data());
         full_upper_kernel<float><<<grid_of(U), kThreads, 0, s>>>(U, out.rows.data(), out.cols.data(),
                                                                  out.row_ptr.data(), lower_ptr_.data(),
                                                                  out.full_ptr.data(), out.vals.data(),
@@ -1023,10 +1188,7 @@ void MatrixConverter::convert_sym(int32_t        B,
     else
     {
         out.fvals.resize((size_t)9 * out.fstride);
-        full_lower_kernel<double><<<grid_of(U), kThreads, 0, s>>>(d_off, keys_[tcur].data(), srcs_[tcur].data(),
-                                                                  lower_ptr_.data(), out.full_ptr.data(),
-                                                                  out.vals.data(), B, out.fcols.data(),
-                                                                  out.fvals.data(), out.fstride);
+        full_lower(out.fvals.data());
         full_upper_kernel<double><<<grid_of(U), kThreads, 0, s>>>(U, out.rows.data(), out.cols.data(),
                                                                   out.row_ptr.data(), lower_ptr_.data(),
                                                                   out.full_ptr.data(), out.vals.data(),

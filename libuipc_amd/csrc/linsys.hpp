@@ -74,6 +74,55 @@ constexpr int kChunkStep = 512;  // nominal blocks per SpMV chunk
 constexpr int kWinShiftMax = 15;  // window offsets are 15-bit fields of the inner words
 
 // ----------------------------------------------------------------- converter
+// Sort keys of (row, col) pairs.  Wide: row * B + col on 64 bits, the
+// reference's key.  Narrow: when every kept block lies within `band` of the
+// diagonal and the rows and offsets fit in 32 bits, (row << bb) | (col - row)
+// on 32 bits -- the same order, sorted in fewer passes of two thirds the
+// bytes.  The transposed (lower) entries of the off-diagonal upper blocks sort
+// by (col, row): col * B + row, or (col << bb) | (row - col + band).
+struct KeyCodec
+{
+    uint64_t B    = 0;
+    int      bb   = 0;  // 0: wide keys
+    int32_t  band = 0;
+    __host__ __device__ __forceinline__ uint64_t up(int32_t r, int32_t c) const
+    {
+        return bb ? ((uint64_t)r << bb) | (uint64_t)(c - r) : (uint64_t)r * B + (uint64_t)c;
+    }
+    __host__ __device__ __forceinline__ uint64_t lo(int32_t r, int32_t c) const
+    {
+        return bb ? ((uint64_t)c << bb) | (uint64_t)(r - c + band) : (uint64_t)c * B + (uint64_t)r;
+    }
+    __host__ __device__ __forceinline__ void up_rc(uint64_t k, int32_t& r, int32_t& c) const
+    {
+        if(bb)
+        {
+            r = (int32_t)(k >> bb);
+            c = r + (int32_t)(k & ((1ull << bb) - 1ull));
+        }
+        else
+        {
+            r = (int32_t)(k / B);
+            c = (int32_t)(k - (uint64_t)r * B);
+        }
+    }
+    // a lower entry's row (the upper block's column) and column (its row)
+    __host__ __device__ __forceinline__ void lo_rc(uint64_t k, int32_t& lr, int32_t& lc) const
+    {
+        if(bb)
+        {
+            lr = (int32_t)(k >> bb);
+            lc = lr - band + (int32_t)(k & ((1ull << bb) - 1ull));
+        }
+        else
+        {
+            lr = (int32_t)(k / B);
+            lc = (int32_t)(k - (uint64_t)lr * B);
+        }
+    }
+};
+
+
 // MatrixConverter<Float,3>::convert_sym (algorithm/details/matrix_converter.inl:1549-1699)
 class MatrixConverter
 {
@@ -95,7 +144,7 @@ class MatrixConverter
     void build_window(BsrMatrix& A, int shape, bool want32, double reserve_ratio, hipStream_t s);
 
   private:
-    template <int MODE>
+    template <int MODE, typename KeyT>
     int compact_sort(int64_t        n,
                      const int32_t* rows,
                      const int32_t* cols,
@@ -111,6 +160,15 @@ class MatrixConverter
     DBuf<int32_t>  seg_start_, lower_rows_, lower_ptr_;
     DBuf<int64_t>  counts_;  // [0] kept triplets, [1] U, [2] off-diagonal upper blocks, [3] full, [4] window blocks
     int            tcur_ = 0;  // buffer of the transposed (lower) sort
+    KeyCodec       codec_{};
+    int            key_bits_ = 0;
+    bool           counted_  = false;  // the triplet count of this conversion is done
+    template <typename KeyT>
+    KeyT* key_ptr(int k)
+    {
+        return reinterpret_cast<KeyT*>(keys_[k].data());
+    }
+    KeyCodec codec() const { return codec_; }
     int64_t*       pinned_ = nullptr;
 };
 

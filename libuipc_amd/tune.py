@@ -46,18 +46,21 @@ def tune_mas_levels(ls: LinearSystem, handle: int, b, x, levels=None, repeats: i
     return {"best": best, "table": table}
 
 
-def tune_spmv_shape(ls: LinearSystem, x, reps: int = 20, shapes=None) -> dict:
+def tune_spmv_shape(ls: LinearSystem, x, reps: int = 20, shapes=None, margin: float = 0.02) -> dict:
     """Times `reps` products mi_spmv_dot(x) per SpMV launch shape (each one's
     window layout rebuilt for it, one untimed product first) and keeps the
     fastest, as the reference fits its SpMV grid to each matrix
-    (Spmv::fit_block_count, linear_system/spmv.cu:909-933).  Returns
-    {"best": k, "table": {k: {"us", "threads", "rows", "per"}}}."""
+    (Spmv::fit_block_count, linear_system/spmv.cu:909-933).  The library's
+    default shape stays unless another is faster by more than `margin`
+    (timing noise would otherwise flip between near-equal shapes).  Returns
+    {"best": k, "table": {k: {"us", "threads", "rows", "per", "vec"}}}."""
     import numpy as np
 
     from .device import spmv_shapes
     from .hip_events import EventTimer
 
-    info = spmv_shapes(ls.lib)["shapes"]
+    allinfo = spmv_shapes(ls.lib)
+    info, dflt = allinfo["shapes"], allinfo["default"]
     cand = list(range(len(info))) if shapes is None else [int(k) for k in shapes]
     ctx = ls.ctx
     stream = ctx.stream()
@@ -73,6 +76,8 @@ def tune_spmv_shape(ls: LinearSystem, x, reps: int = 20, shapes=None) -> dict:
             ctx.lib.mi_spmv_dot(ls.h, x.ptr, y.ptr, d.ptr, stream)
         table[k] = {"us": round(1e3 * ev.stop() / max(1, reps), 2), **info[k]}
     best = min(table, key=lambda k: table[k]["us"])
+    if dflt in table and table[best]["us"] > (1.0 - margin) * table[dflt]["us"]:
+        best = dflt
     ls.set_spmv_shape(best, stream)
     ctx.synchronize()
     return {"best": best, "table": table}
