@@ -158,7 +158,11 @@ def main():
     stream = ctx.stream()
     prec = abi.MI_FP32 if args.precision == "fp32" else abi.MI_FP64
     ls = LinearSystem(ctx, s.block_rows, prec)
-    rows, cols, vals = ctx.to_device(s.rows), ctx.to_device(s.cols), ctx.to_device(s.vals)
+    # an FP32 system takes FP32 triplet values (mi_sys_assemble_f32), except
+    # from the device producers, which write doubles
+    fp32_trip = prec == abi.MI_FP32 and args.producers != "device"
+    rows, cols = ctx.to_device(s.rows), ctx.to_device(s.cols)
+    vals = ctx.to_device(s.vals.astype(np.float32) if fp32_trip else s.vals)
     b = ctx.to_device(s.rhs)
     producers_ms = None
     if args.producers == "device":
@@ -231,31 +235,37 @@ def main():
     U = ls.nnz()
 
     # dominant kernel: the SpMV (+ fused x^T A x), timed with HIP events on its
-    # stream.  mi_spmv_dot launches the accumulating SpMV kernel (y += A x over
-    # the window layout, per-workgroup x^T A x partials) and a one-workgroup
-    # partials sum.
-    # The PCG's product y = A x (mi_spmv with a = 1, b = 0 launches exactly
-    # the PCG's kernel, once) is the roofline's kernel, timed as the PCG runs
-    # it: spmv_reps launches captured in one HIP graph, replayed between HIP
-    # events on the library's stream.  The accumulating mi_spmv_dot (+ its
-    # one-workgroup partials sum, launched eagerly) rides along.
+    # stream.  The PCG's product is the roofline's kernel, launched as the PCG
+    # launches it (on the window layout: y = A x and the per-workgroup x^T A x
+    # partials on vectors in the SpMV's row order, mi_spmv_ordered without the
+    # partials' sum; on the full BSR: mi_spmv with a = 1, b = 0), timed as the
+    # PCG runs it: spmv_reps launches captured in one HIP graph, replayed
+    # between HIP events on the library's stream.  The ABI's accumulating
+    # mi_spmv_dot on the caller's vertex order (+ its one-workgroup partials
+    # sum, launched eagerly) rides along.
     from libuipc_amd.hip_events import graph_ms_per_call
 
     lay = ls.spmv_layout()
     y = ctx.alloc(s.n, np.float64)
     d = ctx.alloc(1, np.float64)
+    ordered = lay["layout"] == "window"
+    if ordered:
+        def pcg_product():
+            ctx.lib.mi_spmv_ordered(ls.h, x.ptr, y.ptr, None, stream)
+    else:
+        def pcg_product():
+            ctx.lib.mi_spmv(ls.h, 1.0, x.ptr, 0.0, y.ptr, stream)
     for _ in range(3):
-        ctx.lib.mi_spmv(ls.h, 1.0, x.ptr, 0.0, y.ptr, stream)
+        pcg_product()
     ctx.synchronize()
     try:
-        ms_spmv = graph_ms_per_call(stream, lambda: ctx.lib.mi_spmv(ls.h, 1.0, x.ptr, 0.0, y.ptr, stream),
-                                    args.spmv_reps)
+        ms_spmv = graph_ms_per_call(stream, pcg_product, args.spmv_reps)
         spmv_timing = "captured graph"
     except AssertionError:  # no stream capture (e.g. under a counter-collecting profiler): eager launches
         ctx.synchronize()
         ev.start()
         for _ in range(args.spmv_reps):
-            ctx.lib.mi_spmv(ls.h, 1.0, x.ptr, 0.0, y.ptr, stream)
+            pcg_product()
         ms_spmv = ev.stop() / args.spmv_reps
         spmv_timing = "eager launches"
     ctx.lib.mi_spmv_dot(ls.h, x.ptr, y.ptr, d.ptr, stream)
@@ -267,14 +277,14 @@ def main():
     spmv_bytes = U * (9 * sval + 4) + (s.block_rows + 1) * 4 + 2 * s.n * 8
     gbs = spmv_bytes / (ms_spmv * 1e-3) / 1e9
     vt = "double" if prec == abi.MI_FP64 else "float"
-    if lay["layout"] == "window":
-        kname = f"spmv_win_kernel<{vt}, false, {lay['threads']}, {lay['window_rows']}, {lay['per']}, {lay['vec']}>"
+    if ordered:
+        kname = (f"spmv_win_kernel<{vt}, false, true, {lay['threads']}, {lay['window_rows']}, {lay['per']}, "
+                 f"{lay['vec']}>")
     else:
         kname = f"spmv_chunk_kernel<{vt}, false>"
     # bytes the layout streams per product: words + values of its blocks, the
-    # row pointers and window order, x and y
-    streamed = lay["blocks"] * (9 * sval + 4) + (s.block_rows + 1) * 4 + 2 * s.n * 8 + (
-        s.block_rows * 4 if lay["layout"] == "window" else 0)
+    # row pointers, x and y
+    streamed = lay["blocks"] * (9 * sval + 4) + (s.block_rows + 1) * 4 + 2 * s.n * 8
 
     result = {
         "metric": "ms per Newton solve to 1e-4 rel. residual; PCG iters; SpMV HBM GB/s vs peak",
@@ -326,24 +336,28 @@ def main():
 
 
 def fp32_arm(ctx, s, args, rows, cols, vals, b, x64, ms64, solve_kw) -> dict:
-    """The precision arm on the same system and triplets: FP32 matrix stream
-    (window layout values) and FP32 MAS inverses / R / Z, FP64 vectors, dots
-    and scalars; timed like `value` (median of the same number of steps) at
-    the same MAS level count, then at the level count the tuner picks.
-    Reports ms, iterations, the true residual and ||x32 - x64|| / ||x64||."""
+    """The precision arm on the same system: FP32 triplet values (the
+    producers' blocks rounded to float, in HBM before the timing like the
+    FP64 ones), FP32 matrix stream (window layout values) and FP32 MAS
+    inverses / R / Z, FP64 sums of duplicates, vectors, dots and scalars;
+    timed like `value` (median of the same number of steps) at the same MAS
+    level count, then at the level count the tuner picks.  Reports ms, the
+    assembly's share, iterations, the true residual (against the FP32-rounded
+    triplets' matrix) and ||x32 - x64|| / ||x64||."""
     from libuipc_amd import abi
     from libuipc_amd.device import LinearSystem
     from libuipc_amd.hip_events import EventTimer
     from libuipc_amd.tune import tune_mas_levels
 
     stream = ctx.stream()
+    vals32 = ctx.to_device(s.vals.astype(np.float32))
     ls = LinearSystem(ctx, s.block_rows, abi.MI_FP32)
     h = ls.add_preconditioner(abi.MI_PRECOND_MAS, 0, s.n, part_ids=s.part_ids, tets=s.tets, mas_precision=abi.MI_FP32,
                               active_levels=args.mas_levels)
     x = ctx.alloc(s.n, np.float64)
 
     def step():
-        ls.assemble(rows, cols, vals, stream)
+        ls.assemble(rows, cols, vals32, stream)
         ls.assemble_preconditioners()
         return ls.solve(b, x, **solve_kw)
 
@@ -363,11 +377,15 @@ def fp32_arm(ctx, s, args, rows, cols, vals, b, x64, ms64, solve_kw) -> dict:
             ev.start()
             step()
             times.append(ev.stop())
+        ev.start()
+        ls.assemble(rows, cols, vals32, stream)
+        ms_asm = ev.stop()
+        ls.assemble_preconditioners()
         st = ls.solve(b, x, **solve_kw, true_residual=1)
         x32 = x.numpy()
         ms = float(np.median(times))
-        return {"ms": round(ms, 3), "ratio_to_fp64": round(ms / ms64, 4), "pcg_iters": int(st["iters"]),
-                "rel_res_true": st["rel_res_true"],
+        return {"ms": round(ms, 3), "ratio_to_fp64": round(ms / ms64, 4), "assemble_ms": round(ms_asm, 3),
+                "pcg_iters": int(st["iters"]), "rel_res_true": st["rel_res_true"],
                 "x_rel_diff_to_fp64": float(np.linalg.norm(x32 - x64) / max(np.linalg.norm(x64), 1e-300))}
 
     same = timed()
@@ -380,7 +398,8 @@ def fp32_arm(ctx, s, args, rows, cols, vals, b, x64, ms64, solve_kw) -> dict:
     log(f"fp32 arm, tuned levels: {best['ms']} ms at {tuned['best']} levels")
     ls.close()
     return {"same_levels": same, "tuned_levels": best,
-            "note": "FP32 matrix stream and MAS storage, FP64 vectors, dots and scalars; median of the same steps"}
+            "note": "FP32 triplet values, matrix stream and MAS storage; FP64 sums of duplicates, vectors, dots "
+                    "and scalars; median of the same steps"}
 
 
 def run_producers(ctx, s, args, rows, cols, vals) -> dict:

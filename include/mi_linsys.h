@@ -110,6 +110,18 @@ int mi_sys_assemble(mi_sys*        sys,
                     const int32_t* cols,
                     const double*  vals,
                     void*          stream);
+/* The same conversion from FP32 triplet values (the precision arm: producers
+ * that write their Hessian blocks in float).  Duplicates are summed in FP64 in
+ * ascending source order, so the blocks equal mi_sys_assemble's on the same
+ * values widened to double.  FP32 systems only (value_precision = MI_FP32,
+ * MI_ERR_ARG otherwise).  No counterpart in the reference, whose triplets
+ * are Float = double. */
+int mi_sys_assemble_f32(mi_sys*        sys,
+                        int64_t        n_triplets,
+                        const int32_t* rows,
+                        const int32_t* cols,
+                        const float*   vals,
+                        void*          stream);
 
 /* ------------------------------------------------ library-owned triplets and gradient
  * GlobalLinearSystem::Impl::_update_subsystem_extent (global_linear_system.cu:321-405).
@@ -218,6 +230,17 @@ int mi_sys_spmv_layout(mi_sys*  sys,
 
 /* Spmv::rbk_sym_spmv (linear_system/spmv.cu:992-1021): y = a A x + b y. */
 int mi_spmv(mi_sys* sys, double a, const double* x, double b, double* y, void* stream);
+
+/* The SpMV's row order (no counterpart in the reference; for callers that
+ * keep vectors in it, and for timing the PCG's own product).  pos (host,
+ * block_rows entries): the row position of every block row; the identity
+ * unless a window layout puts the rows in a MAS part order. */
+int mi_sys_spmv_order(mi_sys* sys, int32_t* pos);
+/* y = A x, *d_dot = x^T A x (when d_dot != NULL) with x and y in the SpMV's
+ * row order (row position k holds block row pos^-1(k)): the one kernel the
+ * PCG launches per iteration.  No counterpart in the reference.  Requires
+ * the window layout (MI_ERR_STATE on the full BSR). */
+int mi_spmv_ordered(mi_sys* sys, const double* x, double* y, double* d_dot, void* stream);
 
 /* ------------------------------------------------------------ preconditioners */
 enum mi_precond_kind
@@ -440,6 +463,11 @@ typedef struct mi_pcg_opts
                                previous solution, a learned guess); 0: x0 = 0, as
                                LinearFusedPCG (linear_fused_pcg.cu:1811-1824).  The relative
                                residual stays relative to ||b||; the rz criterion to r0.z0. */
+    int32_t keep_order;     /* 0 (default): when the SpMV's window layout puts the rows in a
+                               MAS part order, the iteration runs on vectors in that order (b
+                               gathered in, x scattered out; same iterates up to rounding);
+                               1: on the caller's vertex order throughout, as the reference.
+                               No counterpart in the reference.                             */
 } mi_pcg_opts;
 
 typedef struct mi_pcg_stats

@@ -90,6 +90,7 @@ class PcgOpts(C.Structure):
         ("true_residual", C.c_int32),
         ("graph_checks", C.c_int32),
         ("x_init", C.c_int32),
+        ("keep_order", C.c_int32),
     ]
 
 
@@ -151,6 +152,7 @@ LINSYS_PROTOS = {
     "mi_sys_create": (C.c_int, [C.c_void_p, C.POINTER(SysDesc), C.POINTER(C.c_void_p)]),
     "mi_sys_destroy": (C.c_int, [C.c_void_p]),
     "mi_sys_assemble": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "mi_sys_assemble_f32": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "mi_sys_nnz": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
     "mi_sys_export_bcoo": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "mi_spmv_dot": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
@@ -166,6 +168,8 @@ LINSYS_PROTOS = {
     "mi_sys_spmv_layout": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                      C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
     "mi_spmv": (C.c_int, [C.c_void_p, C.c_double, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p]),
+    "mi_sys_spmv_order": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "mi_spmv_ordered": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "mi_precond_add": (C.c_int, [C.c_void_p, C.POINTER(PrecondDesc), C.POINTER(C.c_int)]),
     "mi_precond_reinit": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(PrecondDesc)]),
     "mi_precond_mas_set_active_levels": (C.c_int, [C.c_void_p, C.c_int, C.c_int32]),

@@ -474,13 +474,15 @@ __global__ __launch_bounds__(kThreads) void rle_write_kernel(const int64_t*  d_m
 }
 
 // ---- segmented reduce: one thread per unique block, duplicates summed in
-// ascending source order (bit-identical to a sequential left fold).  The
-// gathers of a group of 4 are issued before their (ordered) additions.
+// FP64 in ascending source order (bit-identical to a sequential left fold of
+// the values as doubles).  The gathers of a group of 4 are issued before
+// their (ordered) additions.  TV: the triplet values' type.
+template <typename TV>
 __global__ __launch_bounds__(kThreads) void seg_reduce_kernel(int64_t        U,
                                                              const int64_t* d_m,
                                                              const int32_t* seg_start,
                                                              const int32_t* srcs,
-                                                             const double*  vals,
+                                                             const TV*      vals,
                                                              double*        out)
 {
     const int64_t m = *d_m;
@@ -489,10 +491,10 @@ __global__ __launch_bounds__(kThreads) void seg_reduce_kernel(int64_t        U,
         const int64_t b = seg_start[u];
         const int64_t e = (u + 1 < U) ? (int64_t)seg_start[u + 1] : m;
         double        acc[9];
-        const double* v0 = vals + 9 * (int64_t)srcs[b];
+        const TV*     v0 = vals + 9 * (int64_t)srcs[b];
 #pragma unroll
         for(int q = 0; q < 9; ++q)
-            acc[q] = v0[q];
+            acc[q] = (double)v0[q];
         int64_t k = b + 1;
         for(; k + 4 <= e; k += 4)
         {
@@ -500,10 +502,10 @@ __global__ __launch_bounds__(kThreads) void seg_reduce_kernel(int64_t        U,
 #pragma unroll
             for(int g = 0; g < 4; ++g)
             {
-                const double* v = vals + 9 * (int64_t)srcs[k + g];
+                const TV* v = vals + 9 * (int64_t)srcs[k + g];
 #pragma unroll
                 for(int q = 0; q < 9; ++q)
-                    t[g][q] = v[q];
+                    t[g][q] = (double)v[q];
             }
 #pragma unroll
             for(int g = 0; g < 4; ++g)
@@ -513,10 +515,10 @@ __global__ __launch_bounds__(kThreads) void seg_reduce_kernel(int64_t        U,
         }
         for(; k < e; ++k)
         {
-            const double* v = vals + 9 * (int64_t)srcs[k];
+            const TV* v = vals + 9 * (int64_t)srcs[k];
 #pragma unroll
             for(int q = 0; q < 9; ++q)
-                acc[q] = acc[q] + v[q];
+                acc[q] = acc[q] + (double)v[q];
         }
 #pragma unroll
         for(int q = 0; q < 9; ++q)
@@ -732,7 +734,7 @@ __global__ void win_upper_kernel(int64_t        U,
         const int32_t v = rows[u], j = cols[u];
         const int32_t p = pos[v], pj = pos[j];
         const int64_t d = (int64_t)wptr[p] + (u - row_ptr[v]);
-        words[d]        = (p >> shift) == (pj >> shift) ? win_inner_word(p & mask, pj & mask) : j;
+        words[d]        = (p >> shift) == (pj >> shift) ? win_inner_word(p & mask, pj & mask) : pj;
 #pragma unroll
         for(int q = 0; q < 9; ++q)
             wvals[q * stride + d] = (VT)uvals[upper_at(u, q)];
@@ -778,13 +780,14 @@ __global__ __launch_bounds__(kThreads) void win_lower_kernel(int32_t        B,
         for(int32_t b = 0; b < len; b += kRowLanes)
         {
             const int32_t k   = lo + b + lane;
-            int32_t       i   = 0;
+            int32_t       i   = 0, pi = 0;
             bool          out = false;
             if(k < hi)
             {
                 int32_t lr;
                 kc.lo_rc((uint64_t)lkeys[k], lr, i);
-                out = (pos[i] >> shift) != ch;  // an inner block is stored once, with row i
+                pi  = pos[i];
+                out = (pi >> shift) != ch;  // an inner block is stored once, with row i
             }
             const uint64_t  bal  = __ballot(out);
             const uint32_t  mine = (uint32_t)(bal >> gsh) & ((1u << kRowLanes) - 1u);
@@ -792,7 +795,7 @@ __global__ __launch_bounds__(kThreads) void win_lower_kernel(int32_t        B,
             {
                 const int64_t dk = d + __popc(mine & ((1u << lane) - 1u));
                 const int64_t u  = lsrc[k];
-                words[dk]        = i;
+                words[dk]        = pi;
 #pragma unroll
                 for(int r = 0; r < 3; ++r)
 #pragma unroll
@@ -1029,16 +1032,17 @@ int radix_key_bits(uint64_t max_key)
     return key_bits_for(max_key);
 }
 
-void MatrixConverter::convert_sym(int32_t        B,
-                                  int64_t        T,
-                                  const int32_t* rows,
-                                  const int32_t* cols,
-                                  const double*  vals,
-                                  BsrMatrix&     out,
-                                  bool           want32,
-                                  bool           build_full,
-                                  double         ratio,
-                                  hipStream_t    s)
+void MatrixConverter::convert(int32_t        B,
+                              int64_t        T,
+                              const int32_t* rows,
+                              const int32_t* cols,
+                              const void*    vals,
+                              bool           vals32,
+                              BsrMatrix&     out,
+                              bool           want32,
+                              bool           build_full,
+                              double         ratio,
+                              hipStream_t    s)
 {
     MI_REQUIRE(B >= 0 && T >= 0, MI_ERR_ARG, "convert_sym: negative size");
     MI_REQUIRE(T < ((int64_t)1 << 31), MI_ERR_ARG, "convert_sym: more than 2^31-1 triplets");
@@ -1123,9 +1127,12 @@ void MatrixConverter::convert_sym(int32_t        B,
                                                                seg_start_.data());
 
     // 7-8. segmented reduce over the sorted permutation (1652-1698)
-    if(U > 0)
-        seg_reduce_kernel<<<grid_of(U), kThreads, 0, s>>>(
-            U, d_m, seg_start_.data(), srcs_[cur].data(), vals, out.vals.data());
+    if(U > 0 && vals32)
+        seg_reduce_kernel<float><<<grid_of(U), kThreads, 0, s>>>(
+            U, d_m, seg_start_.data(), srcs_[cur].data(), static_cast<const float*>(vals), out.vals.data());
+    else if(U > 0)
+        seg_reduce_kernel<double><<<grid_of(U), kThreads, 0, s>>>(
+            U, d_m, seg_start_.data(), srcs_[cur].data(), static_cast<const double*>(vals), out.vals.data());
     row_ptr_kernel<<<grid_of(U + 1), kThreads, 0, s>>>(nullptr, U, out.rows.data(), out.row_ptr.data(), B);
 
     // the transpose of the off-diagonal upper blocks (the lower triangle)

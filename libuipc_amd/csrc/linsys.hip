@@ -70,6 +70,7 @@ struct mi_sys
 
     // PCG state (LinearFusedPCG's r, z, p, Ap and device scalars)
     DBuf<double> r, z, p, Ap, tmp;
+    DBuf<double> xo;   // x in the SpMV's row order (solves that iterate in it)
     DBuf<float4> p32;  // FP32 systems: the SpMV's packed copy of p
     PcgScalars*  d_sc = nullptr;
     PcgScalars*  h_sc = nullptr;  // pinned
@@ -83,7 +84,7 @@ struct mi_sys
         const void* b = nullptr;
         int64_t     n = -1, U = -1;
         uint64_t    pcs_gen = ~0ull;
-        int         ci = 0, fused = -1, bypass = -1, use32 = -1;
+        int         ci = 0, fused = -1, bypass = -1, use32 = -1, ordered = -1;
         const void* bufs[16] = {};
         int64_t     wstride  = -1;
         int32_t     wshape   = -2;  // the SpMV kernel's launch shape (-1: full BSR)
@@ -92,8 +93,8 @@ struct mi_sys
         bool        operator==(const GraphKey& o) const
         {
             if(x != o.x || b != o.b || n != o.n || U != o.U || pcs_gen != o.pcs_gen || ci != o.ci || fused != o.fused
-               || bypass != o.bypass || use32 != o.use32 || stream != o.stream || wstride != o.wstride
-               || wshape != o.wshape || items != o.items)
+               || bypass != o.bypass || use32 != o.use32 || ordered != o.ordered || stream != o.stream
+               || wstride != o.wstride || wshape != o.wshape || items != o.items)
                 return false;
             for(int k = 0; k < 16; ++k)
                 if(bufs[k] != o.bufs[k])
@@ -421,9 +422,13 @@ int mi_sys_destroy(mi_sys* sys)
     MI_API_END
 }
 
-int mi_sys_assemble(mi_sys* sys, int64_t T, const int32_t* rows, const int32_t* cols, const double* vals, void* stream)
+}  // extern "C"
+
+namespace
 {
-    MI_API_BEGIN
+template <typename TV>
+void assemble_t(mi_sys* sys, int64_t T, const int32_t* rows, const int32_t* cols, const TV* vals, void* stream)
+{
     MI_REQUIRE(sys, MI_ERR_ARG, "mi_sys_assemble: null sys");
     MI_REQUIRE(T >= 0, MI_ERR_ARG, "mi_sys_assemble: negative triplet count");
     hipStream_t s = pick(sys->ctx, stream);
@@ -440,6 +445,26 @@ int mi_sys_assemble(mi_sys* sys, int64_t T, const int32_t* rows, const int32_t* 
     build_spmv_layout(sys, s);
     sys->assembled     = true;
     sys->pcs_assembled = false;
+}
+}  // namespace
+
+extern "C" {
+
+int mi_sys_assemble(mi_sys* sys, int64_t T, const int32_t* rows, const int32_t* cols, const double* vals, void* stream)
+{
+    MI_API_BEGIN
+    assemble_t(sys, T, rows, cols, vals, stream);
+    MI_API_END
+}
+
+int mi_sys_assemble_f32(mi_sys* sys, int64_t T, const int32_t* rows, const int32_t* cols, const float* vals,
+                        void* stream)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(sys, MI_ERR_ARG, "mi_sys_assemble_f32: null sys");
+    MI_REQUIRE(sys->desc.value_precision == MI_FP32, MI_ERR_ARG,
+               "mi_sys_assemble_f32: FP32 triplets need an FP32 system (value_precision = MI_FP32)");
+    assemble_t(sys, T, rows, cols, vals, stream);
     MI_API_END
 }
 
@@ -723,6 +748,39 @@ int mi_spmv(mi_sys* sys, double a, const double* x, double b, double* y, void* s
     MI_REQUIRE(sys && x && y, MI_ERR_ARG, "mi_spmv: null argument");
     MI_REQUIRE(sys->assembled, MI_ERR_STATE, "mi_spmv: not assembled");
     spmv_axpby(sys->A, sys->desc.value_precision == MI_FP32, a, x, b, y, sys->spmv_scr, pick(sys->ctx, stream));
+    MI_API_END
+}
+
+int mi_sys_spmv_order(mi_sys* sys, int32_t* pos)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(sys && pos, MI_ERR_ARG, "mi_sys_spmv_order: null argument");
+    MI_REQUIRE(sys->assembled, MI_ERR_STATE, "mi_sys_spmv_order: not assembled");
+    const auto& A = sys->A;
+    if(A.has_win && A.B > 0)
+    {
+        MI_HIP(hipStreamSynchronize(sys->ctx->stream));
+        MI_HIP(hipMemcpy(pos, A.wpos.data(), sizeof(int32_t) * A.B, hipMemcpyDeviceToHost));
+    }
+    else
+        for(int32_t v = 0; v < A.B; ++v)
+            pos[v] = v;
+    MI_API_END
+}
+
+int mi_spmv_ordered(mi_sys* sys, const double* x, double* y, double* d_dot, void* stream)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(sys && x && y, MI_ERR_ARG, "mi_spmv_ordered: null argument");
+    MI_REQUIRE(sys->assembled, MI_ERR_STATE, "mi_spmv_ordered: not assembled");
+    MI_REQUIRE(sys->A.has_win, MI_ERR_STATE, "mi_spmv_ordered: the matrix has no window layout");
+    hipStream_t s     = pick(sys->ctx, stream);
+    const bool  use32 = sys->desc.value_precision == MI_FP32;
+    spmv_prepare(sys->spmv_scr, use32);
+    const int np = spmv_full(sys->A, use32, x, nullptr, y, sys->spmv_scr.partials.data(), nullptr, sys->spmv_scr, s,
+                             nullptr, nullptr, true);
+    if(d_dot)
+        sum_partials(sys->spmv_scr.partials.data(), np, d_dot, s);
     MI_API_END
 }
 
@@ -1220,6 +1278,7 @@ int mi_pcg_default_opts(mi_pcg_opts* o)
     o->true_residual  = 0;
     o->graph_checks   = 4;
     o->x_init         = 0;
+    o->keep_order     = 0;
     MI_API_END
 }
 }  // extern "C"
@@ -1229,7 +1288,8 @@ namespace
 // one PCG iteration (LinearFusedPCG::run_iteration, linear_fused_pcg.cu:1374-1594):
 // SpMV + pAp partials -> x, r update (alpha) [+ block Jacobi + r.z partials]
 // -> preconditioner -> r.z partials -> convergence, beta, p update
-void run_iteration(mi_sys* sys, double* x, bool bypass, const LocalPreconditioner* fused_diag, hipStream_t s)
+void run_iteration(mi_sys* sys, double* x, bool bypass, const LocalPreconditioner* fused_diag, bool ordered,
+                   hipStream_t s)
 {
     const int64_t n     = sys->n();
     PcgScalars*   sc    = sys->d_sc;
@@ -1238,7 +1298,7 @@ void run_iteration(mi_sys* sys, double* x, bool bypass, const LocalPreconditione
     double*       partB = sys->dot_scr.partials.data();
     float4*       p32   = use32 && !sys->A.has_win ? sys->p32.data() : nullptr;
     const int     gA    = spmv_full(sys->A, use32, sys->p.data(), p32, sys->Ap.data(), partA, &sc->converged,
-                                    sys->spmv_scr, s);
+                                    sys->spmv_scr, s, nullptr, nullptr, ordered);
     int gB;
     if(fused_diag)
     {
@@ -1277,6 +1337,34 @@ void read_scalars(mi_sys* sys, hipStream_t s)
     MI_HIP(hipMemcpyAsync(sys->h_sc, sys->d_sc, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
     MI_HIP(hipStreamSynchronize(s));
 }
+
+// is the SpMV's row order another than the caller's? (a window layout built
+// for the current MAS part orders)
+bool spmv_reorders(const mi_sys* sys)
+{
+    if(!sys->A.has_win || sys->A.win_order != sys->order_gen)
+        return false;
+    for(auto& pc : sys->pcs)
+        if(pc->spmv_rank())
+            return true;
+    return false;
+}
+
+// the preconditioners see the solve's vector order while it runs
+struct RowOrderScope
+{
+    mi_sys* sys;
+    RowOrderScope(mi_sys* s_, bool on) : sys(s_)
+    {
+        for(auto& pc : sys->pcs)
+            pc->set_row_order(on);
+    }
+    ~RowOrderScope()
+    {
+        for(auto& pc : sys->pcs)
+            pc->set_row_order(false);
+    }
+};
 }  // namespace
 
 extern "C" int mi_pcg_solve(mi_sys* sys, const double* b, double* x, const mi_pcg_opts* opts, mi_pcg_stats* st, void* stream)
@@ -1317,16 +1405,50 @@ extern "C" int mi_pcg_solve(mi_sys* sys, const double* b, double* x, const mi_pc
     pcg_prepare(sys->dot_scr);
     PcgScalars& h = *sys->h_sc;
 
+    // The iteration runs on vectors in the SpMV's row order when that is a
+    // MAS part order: the window layout's rows and the MAS slots are then
+    // read contiguously.  b is gathered into it, x scattered back at the end.
+    const bool    ordered = !o.keep_order && spmv_reorders(sys);
+    RowOrderScope order_scope(sys, ordered);
+    const int64_t B       = sys->A.B;
+    double*       xs      = x;  // the iterate, in the order the iteration runs on
+    if(ordered)
+    {
+        sys->xo.resize((size_t)n, ratio);
+        xs = sys->xo.data();
+    }
+    auto x_out = [&]
+    {
+        if(ordered)
+            gather_rows3(x, xs, sys->A.wpos.data(), B, s);
+    };
+
     // r = b, x = 0, z = P r, p = z, rz0 = r.z (linear_fused_pcg.cu:1811-1824);
     // with a given x0, r = b - A x0
     const bool use32_sys = sys->desc.value_precision == MI_FP32;
     double     bb_given  = -1.0;
     if(o.x_init)
     {
-        copy_f64(sys->r.data(), b, n, s);
-        spmv_axpby(sys->A, use32_sys, -1.0, x, 1.0, sys->r.data(), sys->spmv_scr, s);
+        if(ordered)
+        {
+            gather_rows3(xs, x, sys->A.wreal.data(), B, s);
+            gather_rows3(sys->r.data(), b, sys->A.wreal.data(), B, s);
+            spmv_full(sys->A, use32_sys, xs, nullptr, sys->Ap.data(), sys->spmv_scr.partials.data(), nullptr,
+                      sys->spmv_scr, s, nullptr, nullptr, true);
+            axpy_f64(-1.0, sys->Ap.data(), sys->r.data(), n, s);
+        }
+        else
+        {
+            copy_f64(sys->r.data(), b, n, s);
+            spmv_axpby(sys->A, use32_sys, -1.0, x, 1.0, sys->r.data(), sys->spmv_scr, s);
+        }
         const double bn = norm_f64_host(b, n, s);
         bb_given        = bn * bn;
+    }
+    else if(ordered)
+    {
+        gather_rows3(sys->r.data(), b, sys->A.wreal.data(), B, s);
+        fill_f64(xs, 0.0, n, s);
     }
     else
         pcg_init_vectors(x, sys->r.data(), b, n, s);
@@ -1376,6 +1498,7 @@ extern "C" int mi_pcg_solve(mi_sys* sys, const double* b, double* x, const mi_pc
     {
         st->converged = 1;
         st->rel_res   = 0.0;
+        x_out();
         MI_HIP(hipStreamSynchronize(s));
         return MI_OK;
     }
@@ -1392,7 +1515,7 @@ extern "C" int mi_pcg_solve(mi_sys* sys, const double* b, double* x, const mi_pc
     bool          converged = false;
 
     mi_sys::GraphKey key;
-    key.x       = x;
+    key.x       = xs;
     key.b       = b;
     key.n       = n;
     key.U       = sys->A.U;
@@ -1401,6 +1524,7 @@ extern "C" int mi_pcg_solve(mi_sys* sys, const double* b, double* x, const mi_pc
     key.fused   = fused ? 1 : 0;
     key.bypass  = bypass ? 1 : 0;
     key.use32   = sys->desc.value_precision;
+    key.ordered = ordered ? 1 : 0;
     key.bufs[0] = sys->r.data();
     key.bufs[1] = sys->z.data();
     key.bufs[2] = sys->p.data();
@@ -1427,7 +1551,7 @@ extern "C" int mi_pcg_solve(mi_sys* sys, const double* b, double* x, const mi_pc
             if(e == hipSuccess)
             {
                 for(int k = 0; k < gb; ++k)
-                    run_iteration(sys, x, bypass, fused, s);
+                    run_iteration(sys, xs, bypass, fused, ordered, s);
                 e = hipStreamEndCapture(s, &sys->graph);
                 if(e == hipSuccess)
                     e = hipGraphInstantiate(&sys->graph_exec, sys->graph, nullptr, nullptr, 0);
@@ -1446,7 +1570,7 @@ extern "C" int mi_pcg_solve(mi_sys* sys, const double* b, double* x, const mi_pc
             MI_HIP(hipGraphLaunch(sys->graph_exec, s));
         else
             for(int64_t k = 0; k < blk; ++k)
-                run_iteration(sys, x, bypass, fused, s);
+                run_iteration(sys, xs, bypass, fused, ordered, s);
         MI_HIP(hipGetLastError());
         done += blk;
         read_scalars(sys, s);
@@ -1464,6 +1588,7 @@ extern "C" int mi_pcg_solve(mi_sys* sys, const double* b, double* x, const mi_pc
     st->converged   = converged ? 1 : 0;
     st->rz_final    = h.rz_new;
     st->rel_res     = bb > 0 ? std::sqrt(h.rr / bb) : 0.0;
+    x_out();
 
     if(o.true_residual)
     {

@@ -52,8 +52,8 @@ struct BsrMatrix
     // window is stored once ("inner": word = bit 31 | li << 15 | lj, the two
     // rows' offsets in the window) and the SpMV adds both A x_j to y_i and
     // A^T x_i to y_j inside the window; a block whose rows fall in different
-    // windows is stored with each row ("outer": word = the other row, values
-    // transposed for the lower copy).  Rows are listed by position (wptr);
+    // windows is stored with each row ("outer": word = the other row's
+    // position, values transposed for the lower copy).  Rows are listed by position (wptr);
     // a row holds its upper blocks in column order, then its outer lower
     // blocks in row order.  Values as 9 planes of stride wstride.
     DBuf<int32_t> wpos, wreal;  // B: position of every row, row at every position
@@ -127,7 +127,8 @@ struct KeyCodec
 class MatrixConverter
 {
   public:
-    // build_full: also the full BSR (the SpMV's other layout)
+    // build_full: also the full BSR (the SpMV's other layout).  FP32 triplet
+    // values (the precision arm's producers) are summed in FP64 like FP64 ones
     void convert_sym(int32_t        B,
                      int64_t        T,
                      const int32_t* rows,
@@ -137,13 +138,40 @@ class MatrixConverter
                      bool           want32,
                      bool           build_full,
                      double         reserve_ratio,
-                     hipStream_t    s);
+                     hipStream_t    s)
+    {
+        convert(B, T, rows, cols, vals, false, out, want32, build_full, reserve_ratio, s);
+    }
+    void convert_sym(int32_t        B,
+                     int64_t        T,
+                     const int32_t* rows,
+                     const int32_t* cols,
+                     const float*   vals,
+                     BsrMatrix&     out,
+                     bool           want32,
+                     bool           build_full,
+                     double         reserve_ratio,
+                     hipStream_t    s)
+    {
+        convert(B, T, rows, cols, vals, true, out, want32, build_full, reserve_ratio, s);
+    }
     ~MatrixConverter();
     // the window layout of `A` (assembled by the last convert_sym) for the
     // row order in A.wpos / A.wreal and a window kernel shape
     void build_window(BsrMatrix& A, int shape, bool want32, double reserve_ratio, hipStream_t s);
 
   private:
+    void convert(int32_t        B,
+                 int64_t        T,
+                 const int32_t* rows,
+                 const int32_t* cols,
+                 const void*    vals,
+                 bool           vals32,
+                 BsrMatrix&     out,
+                 bool           want32,
+                 bool           build_full,
+                 double         reserve_ratio,
+                 hipStream_t    s);
     template <int MODE, typename KeyT>
     int compact_sort(int64_t        n,
                      const int32_t* rows,
@@ -231,10 +259,13 @@ void spmv_prepare(SpmvScratch& scr, bool use32);
 // x32: the FP32 SpMV's padded float4 copy of x (FP32 systems; packed into the
 // scratch when null).  With `active`, only the rows of the listed chunks
 // (ascending, *d_nactive of them) are computed; the other rows of y keep
-// their values.
+// their values.  `ordered`: x and y are in the window layout's row order
+// (A.has_win; no `active` list)
 int spmv_full(const BsrMatrix& A, bool use32, const double* x, const float4* x32, double* y, double* partials,
               const int32_t* d_conv, SpmvScratch& scr, hipStream_t s, const int32_t* active = nullptr,
-              const int64_t* d_nactive = nullptr);
+              const int64_t* d_nactive = nullptr, bool ordered = false);
+// *out = the sum of partials[0 .. n) (one workgroup, a fixed order)
+void sum_partials(const double* partials, int n, double* out, hipStream_t s);
 // y += a A x; *d_dot = x^T A x when d_dot != null
 void spmv_accumulate_dot(const BsrMatrix& A, bool use32, double a, const double* x, double* y, double* d_dot,
                          SpmvScratch& scr, hipStream_t s);
@@ -251,6 +282,11 @@ void spmv_accumulate_dot_noalloc(const BsrMatrix& A, bool use32, const double* x
 
 // ----------------------------------------------------------------- vectors
 void   fill_f64(double* p, double v, int64_t n, hipStream_t s);
+// dst[3 k + c] = src[3 idx[k] + c] for k < rows: vectors between the caller's
+// vertex order and the SpMV's row order (idx = wreal in, wpos out)
+void   gather_rows3(double* dst, const double* src, const int32_t* idx, int64_t rows, hipStream_t s);
+// y += a x
+void   axpy_f64(double a, const double* x, double* y, int64_t n, hipStream_t s);
 void   fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t s);
 // FEM subsystem steps around the solve (subsystem.hip)
 void subsystem_fix_vertices(const int32_t* rows,
@@ -378,6 +414,11 @@ class LocalPreconditioner
     // order (parts in id order, then the unpartitioned vertices): the order
     // of locality the SpMV's window layout puts the rows in
     virtual const std::vector<int32_t>* spmv_rank() const { return nullptr; }
+    // the vectors apply, apply_dot and update_xr see: the caller's vertex
+    // order (false) or the SpMV's row order (true, the PCG's own vectors).
+    // Only a preconditioner with an spmv_rank reorders its range; the others
+    // see the same order either way
+    virtual void set_row_order(bool) {}
     // batched solves: apply may skip the work of systems that have stopped
     // (their r no longer changes, so neither does z); seg = the systems'
     // first block rows (device); seg == nullptr switches the skipping off

@@ -1005,8 +1005,9 @@ struct LevelSpan
 // lanes of its bank by shuffles in ascending slot order, and stores the
 // parent's sum: a parent's children all live in one bank, so there are no
 // atomics and the sum is deterministic.  At level 0 the values are r gathered
-// through part_to_real.  Level starts are bank-aligned, so a bank never
-// straddles two waves.
+// through slot_vec (the vector index of a slot's vertex, in the order the
+// vectors are in).  Level starts are bank-aligned, so a bank never straddles
+// two waves.
 template <typename T>
 __device__ __forceinline__ void restrict_sum_store(T              v0,
                                                    T              v1,
@@ -1040,7 +1041,7 @@ __device__ __forceinline__ void restrict_sum_store(T              v0,
 
 template <typename T, bool kLevel0>
 __device__ __forceinline__ void restrict_bank(const double* __restrict__ r,
-                                              const int32_t* __restrict__ part_to_real,
+                                              const int32_t* __restrict__ slot_vec,
                                               T*             Rb,
                                               const int32_t* __restrict__ parent,
                                               const int32_t* __restrict__ sib,
@@ -1054,7 +1055,7 @@ __device__ __forceinline__ void restrict_bank(const double* __restrict__ r,
         m = sib[slot];
         if constexpr(kLevel0)
         {
-            const int32_t v = part_to_real[slot];
+            const int32_t v = slot_vec[slot];
             if(v >= 0)
             {
                 v0 = (T)r[3 * (int64_t)v];
@@ -1074,7 +1075,7 @@ __device__ __forceinline__ void restrict_bank(const double* __restrict__ r,
 
 template <typename T, bool kLevel0>
 __global__ __launch_bounds__(256) void mas_restrict_level_kernel(const double* __restrict__ r,
-                                                                const int32_t* __restrict__ part_to_real,
+                                                                const int32_t* __restrict__ slot_vec,
                                                                 T*             Rb,
                                                                 const int32_t* __restrict__ parent,
                                                                 const int32_t* __restrict__ sib,
@@ -1085,7 +1086,7 @@ __global__ __launch_bounds__(256) void mas_restrict_level_kernel(const double* _
     if(conv && *conv != 0)
         return;
     const int32_t x = blockIdx.x * blockDim.x + threadIdx.x;
-    restrict_bank<T, kLevel0>(r, part_to_real, Rb, parent, sib, begin + x, x < cnt);
+    restrict_bank<T, kLevel0>(r, slot_vec, Rb, parent, sib, begin + x, x < cnt);
 }
 
 // fused_update_xr (linear_fused_pcg.cu:504-549) with the level-0 restriction
@@ -1105,13 +1106,13 @@ __global__ __launch_bounds__(256) void mas_update_xr_restrict_kernel(PcgScalars*
                                                                     const double* __restrict__ p,
                                                                     double*       r,
                                                                     const double* __restrict__ Ap,
-                                                                    const int32_t* __restrict__ part_to_real,
+                                                                    const int32_t* __restrict__ slot_vec,
                                                                     T*            Rb,
                                                                     const int32_t* __restrict__ parent,
                                                                     const int32_t* __restrict__ sib,
                                                                     int32_t       nslots,
                                                                     int32_t       slot_blocks,
-                                                                    const int32_t* __restrict__ unpart,
+                                                                    const int32_t* __restrict__ unpart_vec,
                                                                     int32_t       n_unpart,
                                                                     int32_t       begin1,
                                                                     int32_t       count1)
@@ -1126,7 +1127,7 @@ __global__ __launch_bounds__(256) void mas_update_xr_restrict_kernel(PcgScalars*
     {
         const int32_t slot = blockIdx.x * 256 + threadIdx.x;
         const bool    live = slot < nslots;
-        const int32_t v    = live ? part_to_real[slot] : -1;
+        const int32_t v    = live ? slot_vec[slot] : -1;
         T             v0 = 0, v1 = 0, v2 = 0;
         if constexpr(kLevel1)
         {
@@ -1195,7 +1196,7 @@ __global__ __launch_bounds__(256) void mas_update_xr_restrict_kernel(PcgScalars*
         const int32_t u = (blockIdx.x - slot_blocks) * 256 + threadIdx.x;
         if(u < n_unpart)
         {
-            const int64_t i = 3 * (int64_t)unpart[u];
+            const int64_t i = 3 * (int64_t)unpart_vec[u];
 #pragma unroll
             for(int c = 0; c < 3; ++c)
             {
@@ -1289,7 +1290,7 @@ __global__ __launch_bounds__(64) void mas_apply_coarse_kernel(const T* __restric
 template <typename T>
 __global__ __launch_bounds__(64) void mas_apply_fine_kernel(const double* __restrict__ r,
                                                            double*        z,
-                                                           const int32_t* __restrict__ part_to_real,
+                                                           const int32_t* __restrict__ slot_vec,
                                                            const T* __restrict__ inv,
                                                            const T* __restrict__ Zb,
                                                            const int32_t* __restrict__ anc_slot,
@@ -1297,6 +1298,7 @@ __global__ __launch_bounds__(64) void mas_apply_fine_kernel(const double* __rest
                                                            int            Lact,
                                                            int64_t        C0,
                                                            const int32_t* __restrict__ unpart,
+                                                           const int32_t* __restrict__ unpart_vec,
                                                            int32_t        n_unpart,
                                                            const double* __restrict__ Dun,
                                                            const int32_t* conv,
@@ -1313,11 +1315,11 @@ __global__ __launch_bounds__(64) void mas_apply_fine_kernel(const double* __rest
         const int64_t u = ((int64_t)blockIdx.x - C0) * kWave + lane;
         if(u < n_unpart)
         {
-            const int32_t v  = unpart[u];
-            const double* D  = Dun + 9 * (int64_t)v;
-            const double  r0 = r[3 * (int64_t)v], r1 = r[3 * (int64_t)v + 1], r2 = r[3 * (int64_t)v + 2];
+            const double* D  = Dun + 9 * (int64_t)unpart[u];
+            const int64_t q  = 3 * (int64_t)unpart_vec[u];
+            const double  r0 = r[q], r1 = r[q + 1], r2 = r[q + 2];
             for(int a = 0; a < 3; ++a)
-                z[3 * (int64_t)v + a] = D[a] * r0 + D[a + 3] * r1 + D[a + 6] * r2;
+                z[q + a] = D[a] * r0 + D[a + 3] * r1 + D[a + 6] * r2;
         }
         return;
     }
@@ -1334,7 +1336,7 @@ __global__ __launch_bounds__(64) void mas_apply_fine_kernel(const double* __rest
     int32_t       anc[kMaxLvl - 1];
     if(own)
     {
-        v = part_to_real[slot];
+        v = slot_vec[slot];
 #pragma unroll
         for(int l = 1; l < kMaxLvl; ++l)
             anc[l - 1] = l < Lact ? anc_slot[(int64_t)(l - 1) * N + slot] : 0;
@@ -1371,7 +1373,7 @@ __global__ __launch_bounds__(64) void mas_apply_fine_kernel(const double* __rest
 template <typename T>
 __global__ __launch_bounds__(256) void mas_apply_fine_dot_kernel(const double* __restrict__ r,
                                                                 double*        z,
-                                                                const int32_t* __restrict__ part_to_real,
+                                                                const int32_t* __restrict__ slot_vec,
                                                                 const T* __restrict__ inv,
                                                                 const T* __restrict__ Zb,
                                                                 const int32_t* __restrict__ anc_slot,
@@ -1379,6 +1381,7 @@ __global__ __launch_bounds__(256) void mas_apply_fine_dot_kernel(const double* _
                                                                 int            Lact,
                                                                 int64_t        C0,
                                                                 const int32_t* __restrict__ unpart,
+                                                                const int32_t* __restrict__ unpart_vec,
                                                                 int32_t        n_unpart,
                                                                 const double* __restrict__ Dun,
                                                                 const int32_t* conv,
@@ -1404,13 +1407,13 @@ __global__ __launch_bounds__(256) void mas_apply_fine_dot_kernel(const double* _
             const int64_t u = (k - C0) * kWave + lane;
             if(u < n_unpart)
             {
-                const int32_t v  = unpart[u];
-                const double* D  = Dun + 9 * (int64_t)v;
-                const double  r0 = r[3 * (int64_t)v], r1 = r[3 * (int64_t)v + 1], r2 = r[3 * (int64_t)v + 2];
+                const double* D  = Dun + 9 * (int64_t)unpart[u];
+                const int64_t q  = 3 * (int64_t)unpart_vec[u];
+                const double  r0 = r[q], r1 = r[q + 1], r2 = r[q + 2];
                 for(int a = 0; a < 3; ++a)
                 {
-                    const double za        = D[a] * r0 + D[a + 3] * r1 + D[a + 6] * r2;
-                    z[3 * (int64_t)v + a] = za;
+                    const double za = D[a] * r0 + D[a + 3] * r1 + D[a + 6] * r2;
+                    z[q + a]        = za;
                     s0 += (a == 0 ? r0 : a == 1 ? r1 : r2) * za;
                 }
                 s1 += r0 * r0 + r1 * r1 + r2 * r2;
@@ -1425,7 +1428,7 @@ __global__ __launch_bounds__(256) void mas_apply_fine_dot_kernel(const double* _
             int32_t       anc[kMaxLvl - 1];
             if(own)
             {
-                v = part_to_real[slot];
+                v = slot_vec[slot];
 #pragma unroll
                 for(int l = 1; l < kMaxLvl; ++l)
                     anc[l - 1] = l < Lact ? anc_slot[(int64_t)(l - 1) * N + slot] : 0;
@@ -1491,6 +1494,10 @@ class MasPreconditioner final : public LocalPreconditioner
     int64_t   C = 0, C0 = 0;  // clusters, fine clusters
     // device tables
     DBuf<int32_t> d_part_to_real, d_real_to_part, d_anc, d_anc_slot, d_parent, d_sib, d_unpart;
+    // the vector index of every slot and unpartitioned vertex when the
+    // vectors are in the SpMV's row order (the part order: rank_)
+    DBuf<int32_t> d_part_to_pos, d_unpart_pos;
+    bool          row_order_ = false;
     DBuf<int32_t> d_desc_ptr, d_desc_idx, d_desc_lpn;
     int32_t       n_unpart  = 0;
     int           base_lvl  = 1;  // restriction kernels produce R up to this level
@@ -1601,6 +1608,16 @@ class MasPreconditioner final : public LocalPreconditioner
         n_unpart = (int32_t)unpart.size();
         upload(d_anc_slot, anc_slot);
         upload(d_unpart, unpart);
+        {
+            std::vector<int32_t> part_pos((size_t)h.N, -1), unpart_pos(unpart.size());
+            for(int32_t slot = 0; slot < h.N; ++slot)
+                if(h.part_to_real[slot] >= 0)
+                    part_pos[slot] = rank_[h.part_to_real[slot]];
+            for(size_t u = 0; u < unpart.size(); ++u)
+                unpart_pos[u] = rank_[unpart[u]];
+            upload(d_part_to_pos, part_pos);
+            upload(d_unpart_pos, unpart_pos);
+        }
         // base level: the first level at most kTailLevel nodes (restriction
         // kernels stop there); level-b descendants of every node above it
         base_lvl = L - 1;
@@ -1742,6 +1759,8 @@ class MasPreconditioner final : public LocalPreconditioner
     {
         const int L    = h.L;
         const int Lact = (active_levels > 0 && active_levels <= L) ? active_levels : L;
+        const int32_t* pv = slot_vec();
+        const int32_t* uv = unpart_vec();
         // one active level is the 16-node block Jacobi: no coarse level is read
         const bool r0_done = r0_fused_, r1_done = r1_fused_;
         r0_fused_ = r1_fused_ = false;
@@ -1752,10 +1771,10 @@ class MasPreconditioner final : public LocalPreconditioner
                 continue;
             if(l == 0)
                 mas_restrict_level_kernel<T, true><<<ceil_div(cnt, 256), 256, 0, s>>>(
-                    r, d_part_to_real.data(), R, d_parent.data(), d_sib.data(), span.begin[l], cnt, conv);
+                    r, pv, R, d_parent.data(), d_sib.data(), span.begin[l], cnt, conv);
             else
                 mas_restrict_level_kernel<T, false><<<ceil_div(cnt, 256), 256, 0, s>>>(
-                    r, d_part_to_real.data(), R, d_parent.data(), d_sib.data(), span.begin[l], cnt, conv);
+                    r, pv, R, d_parent.data(), d_sib.data(), span.begin[l], cnt, conv);
         }
         if(Lact > 1 && C > c_first)
             mas_apply_coarse_kernel<T><<<(unsigned)(C - c_first), 64, 0, s>>>(
@@ -1764,16 +1783,14 @@ class MasPreconditioner final : public LocalPreconditioner
         if(part)
         {
             const int g = (int)std::max<int64_t>(1, std::min<int64_t>(max_parts, ceil_div(blocks, (int64_t)4)));
-            mas_apply_fine_dot_kernel<T><<<g, 256, 0, s>>>(r, z, d_part_to_real.data(), inv, Z, d_anc_slot.data(),
-                                                           h.N, Lact, C0, d_unpart.data(), n_unpart, d_Dun.data(),
-                                                           conv, part);
+            mas_apply_fine_dot_kernel<T><<<g, 256, 0, s>>>(r, z, pv, inv, Z, d_anc_slot.data(), h.N, Lact, C0,
+                                                           d_unpart.data(), uv, n_unpart, d_Dun.data(), conv, part);
             return g;
         }
         if(blocks > 0)
-            mas_apply_fine_kernel<T><<<(unsigned)blocks, 64, 0, s>>>(r, z, d_part_to_real.data(), inv, Z,
-                                                                     d_anc_slot.data(), h.N, Lact, C0,
-                                                                     d_unpart.data(), n_unpart, d_Dun.data(), conv,
-                                                                     fine_on);
+            mas_apply_fine_kernel<T><<<(unsigned)blocks, 64, 0, s>>>(r, z, pv, inv, Z, d_anc_slot.data(), h.N,
+                                                                     Lact, C0, d_unpart.data(), uv, n_unpart,
+                                                                     d_Dun.data(), conv, fine_on);
         return 0;
     }
 
@@ -1836,6 +1853,12 @@ class MasPreconditioner final : public LocalPreconditioner
 
     const std::vector<int32_t>* spmv_rank() const override { return &rank_; }
 
+    void set_row_order(bool on) override { row_order_ = on; }
+    // the vector index of every partition slot (-1: empty) and of every
+    // unpartitioned vertex, in the order the vectors are in
+    const int32_t* slot_vec() const { return row_order_ ? d_part_to_pos.data() : d_part_to_real.data(); }
+    const int32_t* unpart_vec() const { return row_order_ ? d_unpart_pos.data() : d_unpart.data(); }
+
     // the PCG's x, r update with the level-0 restriction folded in, when the
     // apply restricts at all (more than one active level) and this MAS covers
     // the whole vector (the caller checks the range); MI_MAS_FUSE_XR=0 keeps
@@ -1863,12 +1886,12 @@ class MasPreconditioner final : public LocalPreconditioner
             using T = decltype(tag);
             if(lvl1)
                 mas_update_xr_restrict_kernel<T, true><<<blocks, 256, 0, s>>>(
-                    sc, pap_part, gp, x, p, r, Ap, d_part_to_real.data(), R, d_parent.data(), d_sib.data(),
-                    span.count[0], slot_blocks, d_unpart.data(), n_unpart, span.begin[1], span.count[1]);
+                    sc, pap_part, gp, x, p, r, Ap, slot_vec(), R, d_parent.data(), d_sib.data(), span.count[0],
+                    slot_blocks, unpart_vec(), n_unpart, span.begin[1], span.count[1]);
             else
                 mas_update_xr_restrict_kernel<T, false><<<blocks, 256, 0, s>>>(
-                    sc, pap_part, gp, x, p, r, Ap, d_part_to_real.data(), R, d_parent.data(), d_sib.data(),
-                    span.count[0], slot_blocks, d_unpart.data(), n_unpart, span.begin[1], span.count[1]);
+                    sc, pap_part, gp, x, p, r, Ap, slot_vec(), R, d_parent.data(), d_sib.data(), span.count[0],
+                    slot_blocks, unpart_vec(), n_unpart, span.begin[1], span.count[1]);
         };
         if(fp32)
             go(d_R32.data(), float{});

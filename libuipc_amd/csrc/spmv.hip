@@ -325,10 +325,14 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) v
 // y rows accumulate in LDS:
 //   * products: consecutive lanes on consecutive blocks; an inner block adds
 //     A^T x_i into the LDS row j (LDS FP64 atomics) and stages A x_j; an outer
-//     block stages A x_j with x_j gathered from global memory;
+//     block stages A x_j with x_j gathered from global memory (its word is
+//     the row position of j);
 //   * row sums: the staged products of each row meeting the stage, 16 lanes
 //     per row in a fixed order, added to the row's LDS entry;
-//   * the window's rows are written once, through wreal, with x . (A x).
+//   * the window's rows are written once, with x . (A x).
+// With ORD the vectors are in the layout's row order (x, y indexed by row
+// position: the window's rows are contiguous); else in the caller's vertex
+// order, reached through real (position -> vertex).
 template <typename VT, int PER>
 struct WinStage
 {
@@ -340,10 +344,11 @@ struct WinStage
 
 // one block of a window stage: stage A x_j in slot i of w; an inner block
 // (both rows in the window) also adds A^T x_i into the LDS row j
-template <typename VT, typename W, int KW, int STAGE>
+template <typename VT, typename W, bool ORD, int KW, int STAGE>
 __device__ __forceinline__ void win_block(int32_t wd,
                                           const VT (&A)[9],
                                           const double* __restrict__ x,
+                                          const int32_t* __restrict__ real,
                                           double (&xs)[3][KW],
                                           double (&ys)[3][KW],
                                           W (&w)[3][STAGE],
@@ -366,9 +371,10 @@ __device__ __forceinline__ void win_block(int32_t wd,
     }
     else
     {
-        x0 = x[3 * (int64_t)wd];
-        x1 = x[3 * (int64_t)wd + 1];
-        x2 = x[3 * (int64_t)wd + 2];
+        const int64_t j = ORD ? wd : real[wd];
+        x0              = x[3 * j];
+        x1              = x[3 * j + 1];
+        x2              = x[3 * j + 2];
     }
     w[0][i] = (W)((double)A[0] * x0 + (double)A[3] * x1 + (double)A[6] * x2);
     w[1][i] = (W)((double)A[1] * x0 + (double)A[4] * x1 + (double)A[7] * x2);
@@ -396,7 +402,7 @@ struct Vec2<float>
 // start at an even block (the first slot may precede the stage, the last
 // may pass the window: both are masked; the plane stride is even and the
 // arrays hold two blocks of padding past the last)
-template <typename VT, bool ACCUM, int NT, int KW, int PER, int VEC>
+template <typename VT, bool ACCUM, bool ORD, int NT, int KW, int PER, int VEC>
 __global__ __launch_bounds__(NT) void spmv_win_kernel(int32_t nchunks,
                                                       int32_t B,
                                                       const int32_t* __restrict__ wptr,
@@ -413,6 +419,7 @@ __global__ __launch_bounds__(NT) void spmv_win_kernel(int32_t nchunks,
                                                       const int64_t* d_nactive)
 {
     static_assert(VEC == 1 || VEC == 2, "one or two blocks per load");
+    static_assert(!(ORD && ACCUM), "row-order products overwrite y");
     if(d_conv && *d_conv != 0)
         return;
     using W                   = typename WinStage<VT, PER>::W;
@@ -445,7 +452,7 @@ __global__ __launch_bounds__(NT) void spmv_win_kernel(int32_t nchunks,
         const int     nr = (int)std::min<int64_t>(KW, (int64_t)B - R0);
         for(int l = tid; l < nr; l += NT)
         {
-            const int64_t v = real[R0 + l];
+            const int64_t v = ORD ? R0 + l : real[R0 + l];
             xs[0][l]        = x[3 * v];
             xs[1][l]        = x[3 * v + 1];
             xs[2][l]        = x[3 * v + 2];
@@ -475,7 +482,7 @@ __global__ __launch_bounds__(NT) void spmv_win_kernel(int32_t nchunks,
 #pragma unroll
                         for(int q = 0; q < 9; ++q)
                             A[q] = stream_load<VT>(&vals[q * stride + k]);
-                        win_block<VT, W, KW, kStage>(wd, A, x, xs, ys, w, i);
+                        win_block<VT, W, ORD, KW, kStage>(wd, A, x, real, xs, ys, w, i);
                     }
                 }
             }
@@ -501,14 +508,14 @@ __global__ __launch_bounds__(NT) void spmv_win_kernel(int32_t nchunks,
 #pragma unroll
                             for(int q = 0; q < 9; ++q)
                                 A[q] = A2[q].x;
-                            win_block<VT, W, KW, kStage>(wd.x, A, x, xs, ys, w, i);
+                            win_block<VT, W, ORD, KW, kStage>(wd.x, A, x, real, xs, ys, w, i);
                         }
                         if(ok1)
                         {
 #pragma unroll
                             for(int q = 0; q < 9; ++q)
                                 A[q] = A2[q].y;
-                            win_block<VT, W, KW, kStage>(wd.y, A, x, xs, ys, w, i + 1);
+                            win_block<VT, W, ORD, KW, kStage>(wd.y, A, x, real, xs, ys, w, i + 1);
                         }
                     }
                 }
@@ -558,7 +565,7 @@ __global__ __launch_bounds__(NT) void spmv_win_kernel(int32_t nchunks,
         for(int l = tid; l < nr; l += NT)
         {
             const double v0 = ys[0][l], v1 = ys[1][l], v2 = ys[2][l];
-            put_row<ACCUM>(y, real[R0 + l], a, v0, v1, v2);
+            put_row<ACCUM>(y, ORD ? R0 + l : real[R0 + l], a, v0, v1, v2);
             dot += xs[0][l] * v0 + xs[1][l] * v1 + xs[2][l] * v2;
         }
         __syncthreads();  // the next window restages xs, ys, rp
@@ -632,29 +639,32 @@ const void* win_kernel(int idx)
     switch(idx)
     {
 #define MI_WIN_FN(i, nt, sh, per, vec) \
-    case i: return (const void*)spmv_win_kernel<VT, ACCUM, nt, (1 << sh), per, vec>;
+    case i: return (const void*)spmv_win_kernel<VT, ACCUM, false, nt, (1 << sh), per, vec>;
         MI_WIN_SHAPES(MI_WIN_FN)
 #undef MI_WIN_FN
     }
     return nullptr;
 }
 
-template <typename VT, bool ACCUM, int NT, int KW, int PER, int VEC>
+template <typename VT, bool ACCUM, bool ORD, int NT, int KW, int PER, int VEC>
 void launch_win_t(int g, hipStream_t s, const BsrMatrix& A, const VT* vals, const double* x, double* y, double a,
                   double* partials, const int32_t* d_conv, const int32_t* active, const int64_t* d_nactive)
 {
-    spmv_win_kernel<VT, ACCUM, NT, KW, PER, VEC><<<g, NT, 0, s>>>(A.wnchunks, A.B, A.wptr.data(), A.wwords.data(), vals,
-                                                        A.wstride, A.wreal.data(), x, y, a, partials, d_conv, active,
-                                                        d_nactive);
+    spmv_win_kernel<VT, ACCUM, ORD, NT, KW, PER, VEC><<<g, NT, 0, s>>>(A.wnchunks, A.B, A.wptr.data(),
+                                                                      A.wwords.data(), vals, A.wstride,
+                                                                      A.wreal.data(), x, y, a, partials, d_conv,
+                                                                      active, d_nactive);
 }
 
 // launches the SpMV on the layout the matrix holds (the window layout when
-// built, else the full BSR); returns how many partials it wrote from partials[0]
-template <typename VT, bool ACCUM>
+// built, else the full BSR); returns how many partials it wrote from partials[0].
+// ORD: x and y in the window layout's row order (window layout only)
+template <typename VT, bool ACCUM, bool ORD = false>
 int launch_spmv(int g, hipStream_t s, const BsrMatrix& A, const VT* vals, const double* x, const float4* x32,
                 double* y, double a, double* partials, const int32_t* d_conv, const int32_t* active = nullptr,
                 const int64_t* d_nactive = nullptr)
 {
+    MI_REQUIRE(!ORD || (A.has_win && !active), MI_ERR_STATE, "spmv: row-order product without a window layout");
     if(A.has_win)
     {
         const int idx = A.wshape;
@@ -664,8 +674,8 @@ int launch_spmv(int g, hipStream_t s, const BsrMatrix& A, const VT* vals, const 
         {
 #define MI_WIN_LAUNCH(i, nt, sh, per, vec)                                                                    \
     case i:                                                                                                      \
-        launch_win_t<VT, ACCUM, nt, (1 << sh), per, vec>(g, s, A, vals, x, y, a, partials, d_conv, active,      \
-                                                         d_nactive);                                             \
+        launch_win_t<VT, ACCUM, ORD, nt, (1 << sh), per, vec>(g, s, A, vals, x, y, a, partials, d_conv, active, \
+                                                              d_nactive);                                        \
         break;
             MI_WIN_SHAPES(MI_WIN_LAUNCH)
 #undef MI_WIN_LAUNCH
@@ -819,7 +829,7 @@ const float4* x32_of(const BsrMatrix& A, const double* x, const float4* x32, Spm
 
 int spmv_full(const BsrMatrix& A, bool use32, const double* x, const float4* x32, double* y, double* partials,
               const int32_t* d_conv, SpmvScratch& scr, hipStream_t s, const int32_t* active,
-              const int64_t* d_nactive)
+              const int64_t* d_nactive, bool ordered)
 {
     spmv_prepare(scr, use32);
     if(A.B == 0)
@@ -829,13 +839,26 @@ int spmv_full(const BsrMatrix& A, bool use32, const double* x, const float4* x32
     MI_REQUIRE(x && y && partials, MI_ERR_STATE, "spmv_full: null buffer");
     MI_REQUIRE(partials >= scr.partials.data() && partials + g <= scr.partials.data() + scr.partials.size(),
                MI_ERR_STATE, "spmv_full: partials outside the scratch");
-    const int np = use32 ? launch_spmv<float, false>(g, s, A, values_of<float>(A), x,
-                                                     A.has_win ? nullptr : x32_of(A, x, x32, scr, s), y, 1.0,
-                                                     partials, d_conv, active, d_nactive)
-                         : launch_spmv<double, false>(g, s, A, values_of<double>(A), x, nullptr, y, 1.0, partials,
+    int np;
+    if(ordered)
+        np = use32 ? launch_spmv<float, false, true>(g, s, A, values_of<float>(A), x, nullptr, y, 1.0, partials,
+                                                     d_conv, active, d_nactive)
+                   : launch_spmv<double, false, true>(g, s, A, values_of<double>(A), x, nullptr, y, 1.0, partials,
                                                       d_conv, active, d_nactive);
+    else
+        np = use32 ? launch_spmv<float, false>(g, s, A, values_of<float>(A), x,
+                                               A.has_win ? nullptr : x32_of(A, x, x32, scr, s), y, 1.0, partials,
+                                               d_conv, active, d_nactive)
+                   : launch_spmv<double, false>(g, s, A, values_of<double>(A), x, nullptr, y, 1.0, partials, d_conv,
+                                                active, d_nactive);
     MI_HIP(hipGetLastError());
     return np;
+}
+
+void sum_partials(const double* partials, int n, double* out, hipStream_t s)
+{
+    sum_partials_kernel<<<1, kThreads, 0, s>>>(partials, n, out);
+    MI_HIP(hipGetLastError());
 }
 
 void spmv_accumulate_dot(const BsrMatrix& A, bool use32, double a, const double* x, double* y, double* d_dot,

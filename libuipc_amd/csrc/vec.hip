@@ -58,6 +58,22 @@ __global__ void copy_kernel(double* dst, const double* src, int64_t n)
     for(int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         dst[i] = src[i];
 }
+__global__ void axpy_kernel(double a, const double* __restrict__ x, double* y, int64_t n)
+{
+    for(int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        y[i] += a * x[i];
+}
+// one thread per scalar: consecutive threads write consecutive entries
+__global__ void gather_rows3_kernel(double* dst, const double* __restrict__ src, const int32_t* __restrict__ idx,
+                                    int64_t rows)
+{
+    const int64_t n = 3 * rows;
+    for(int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    {
+        const int64_t k = i / 3;
+        dst[i]          = src[3 * (int64_t)idx[k] + (i - 3 * k)];
+    }
+}
 
 // ---------------------------------------------------------------- KAT dot / norm
 template <typename T>
@@ -423,6 +439,16 @@ void fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t s)
 {
     if(n > 0)
         fill_i32_kernel<<<vec_grid(n), kThreads, 0, s>>>(p, v, n);
+}
+void gather_rows3(double* dst, const double* src, const int32_t* idx, int64_t rows, hipStream_t s)
+{
+    if(rows > 0)
+        gather_rows3_kernel<<<vec_grid(3 * rows), kThreads, 0, s>>>(dst, src, idx, rows);
+}
+void axpy_f64(double a, const double* x, double* y, int64_t n, hipStream_t s)
+{
+    if(n > 0)
+        axpy_kernel<<<vec_grid(n), kThreads, 0, s>>>(a, x, y, n);
 }
 void copy_f64(double* dst, const double* src, int64_t n, hipStream_t s)
 {

@@ -182,8 +182,10 @@ class LinearSystem:
 
     # MatrixConverter::convert_sym
     def assemble(self, rows: DeviceArray, cols: DeviceArray, vals: DeviceArray, stream=None) -> None:
+        """mi_sys_assemble, or mi_sys_assemble_f32 for float32 values"""
         T = rows.size
-        check(self.lib.mi_sys_assemble(self.h, T, rows.ptr, cols.ptr, vals.ptr, stream))
+        fn = self.lib.mi_sys_assemble_f32 if vals.dtype == np.float32 else self.lib.mi_sys_assemble
+        check(fn(self.h, T, rows.ptr, cols.ptr, vals.ptr, stream))
 
     def nnz(self) -> int:
         v = C.c_int64()
@@ -258,6 +260,16 @@ class LinearSystem:
 
     def spmv(self, a: float, x: DeviceArray, b: float, y: DeviceArray) -> None:
         check(self.lib.mi_spmv(self.h, a, x.ptr, b, y.ptr, None))
+
+    def spmv_order(self) -> np.ndarray:
+        """mi_sys_spmv_order: the row position of every block row"""
+        pos = np.empty(max(self.B, 1), np.int32)
+        check(self.lib.mi_sys_spmv_order(self.h, pos.ctypes.data_as(C.c_void_p)))
+        return pos[: self.B]
+
+    def spmv_ordered(self, x: DeviceArray, y: DeviceArray, d_dot: DeviceArray | None = None, stream=None) -> None:
+        """mi_spmv_ordered: y = A x (and x^T A x) on vectors in the SpMV's row order"""
+        check(self.lib.mi_spmv_ordered(self.h, x.ptr, y.ptr, d_dot.ptr if d_dot is not None else None, stream))
 
     def add_preconditioner(self, kind: int, dof_offset: int, dof_count: int, *, part_ids=None, tets=None,
                            tris=None, edges=None, mas_precision: int = abi.MI_FP64, active_levels: int = 0,

@@ -47,8 +47,9 @@ def tune_mas_levels(ls: LinearSystem, handle: int, b, x, levels=None, repeats: i
 
 
 def tune_spmv_shape(ls: LinearSystem, x, reps: int = 20, shapes=None, margin: float = 0.02) -> dict:
-    """Times `reps` products mi_spmv_dot(x) per SpMV launch shape (each one's
-    window layout rebuilt for it, one untimed product first) and keeps the
+    """Times `reps` products per SpMV launch shape (each one's window layout
+    rebuilt for it, one untimed product first; the PCG's own product,
+    mi_spmv_ordered, on a window layout, else mi_spmv_dot) and keeps the
     fastest, as the reference fits its SpMV grid to each matrix
     (Spmv::fit_block_count, linear_system/spmv.cu:909-933).  The library's
     default shape stays unless another is faster by more than `margin`
@@ -70,10 +71,16 @@ def tune_spmv_shape(ls: LinearSystem, x, reps: int = 20, shapes=None, margin: fl
     table = {}
     for k in cand:
         ls.set_spmv_shape(k, stream)
-        ctx.lib.mi_spmv_dot(ls.h, x.ptr, y.ptr, d.ptr, stream)
+        if ls.spmv_layout()["layout"] == "window":
+            def product():
+                ctx.lib.mi_spmv_ordered(ls.h, x.ptr, y.ptr, None, stream)
+        else:
+            def product():
+                ctx.lib.mi_spmv_dot(ls.h, x.ptr, y.ptr, d.ptr, stream)
+        product()
         ev.start()
         for _ in range(max(1, reps)):
-            ctx.lib.mi_spmv_dot(ls.h, x.ptr, y.ptr, d.ptr, stream)
+            product()
         table[k] = {"us": round(1e3 * ev.stop() / max(1, reps), 2), **info[k]}
     best = min(table, key=lambda k: table[k]["us"])
     if dflt in table and table[best]["us"] > (1.0 - margin) * table[dflt]["us"]:

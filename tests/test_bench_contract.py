@@ -21,12 +21,18 @@ def test_more_than_one_gpu_is_refused():
 def test_traffic_summary_only_for_the_default_workload():
     import bench
 
+    import re
+
     sys.argv = ["bench.py"]
     args = bench.parse()
-    t = bench.measured_traffic("spmv_chunk_kernel<double, true>", args)
+    here = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    with open(os.path.join(here, "profiles", "c2_fp64_traffic.md")) as f:
+        names = re.findall(r"`(spmv_\w+<[^`]*>)`", f.read())
+    assert names
+    t = bench.measured_traffic(names[0], args)
     assert t is not None and 4e8 < t < 2e9
     args.contacts = 1000
-    assert bench.measured_traffic("spmv_chunk_kernel<double, true>", args) is None
+    assert bench.measured_traffic(names[0], args) is None
 
 
 def test_every_workload_parses():

@@ -77,6 +77,37 @@ def test_convert_contacts_bitexact(ctx, c1_contact):
     check_bcoo_bitexact(assembled(ctx, c1_contact), c1_contact)
 
 
+def test_convert_f32_triplets(ctx, c1_contact):
+    """mi_sys_assemble_f32: FP32 triplet values (the precision arm's
+    producers) give, bit for bit, the blocks of the oracle's conversion of the
+    same values widened to double (sums in FP64, ascending source order), and
+    the FP32 system's product is the oracle's on those blocks; the same
+    triplets as doubles give the same system; an FP64 system refuses them"""
+    s = c1_contact
+    v32 = s.vals.astype(np.float32)
+    r, c = ctx.to_device(s.rows), ctx.to_device(s.cols)
+    ls = LinearSystem(ctx, s.block_rows, abi.MI_FP32)
+    ls.assemble(r, c, ctx.to_device(v32))
+    gr, gc, gv, gp = ls.export_bcoo()
+    orr, oc, ov, orp = O.convert_sym(s.block_rows, s.rows, s.cols, v32.astype(np.float64))
+    assert np.array_equal(gr, orr) and np.array_equal(gc, oc) and np.array_equal(gp, orp)
+    assert np.array_equal(gv.view(np.uint64), ov.view(np.uint64))
+    x = np.random.default_rng(12).standard_normal(s.n)
+    yref, dref = O.spmv_sym(s.block_rows, orr, oc, ov, x)
+    yd = ctx.alloc(s.n, np.float64)
+    d = ls.spmv_dot(ctx.to_device(x), yd)
+    assert np.abs(yd.numpy() - yref).max() <= 1e-6 * np.abs(yref).max()
+    assert d == pytest.approx(dref, rel=1e-5)
+    ls2 = LinearSystem(ctx, s.block_rows, abi.MI_FP32)
+    ls2.assemble(r, c, ctx.to_device(v32.astype(np.float64)))
+    assert np.array_equal(ls2.export_bcoo()[2].view(np.uint64), gv.view(np.uint64))
+    ls64 = LinearSystem(ctx, s.block_rows, abi.MI_FP64)
+    with pytest.raises(abi.MiError):
+        ls64.assemble(r, c, ctx.to_device(v32))
+    for h in (ls, ls2, ls64):
+        h.close()
+
+
 @pytest.mark.parametrize("B,T,seed", [(1, 5, 0), (7, 300, 1), (300, 20000, 2), (5000, 70000, 3)])
 def test_convert_random_bitexact(ctx, B, T, seed):
     """lower-triangle and invalid (-1) entries are dropped; heavy duplication"""
@@ -236,6 +267,79 @@ def test_pcg_after_a_shape_change(ctx, c1_contact):
         assert st["converged"] == 1 and st["rel_res_true"] <= 2e-6, (k, st)
         assert abs(st["iters_exact"] - st0["iters_exact"]) <= max(2, st0["iters_exact"] // 50), (k, st, st0)
         assert rel_err(x.numpy(), x0.numpy()) < 1e-4
+    ls.close()
+
+
+@pytest.mark.parametrize("precision", [abi.MI_FP64, abi.MI_FP32])
+@pytest.mark.parametrize("unpart", [False, True])
+def test_spmv_ordered(ctx, c1_contact, precision, unpart):
+    """mi_spmv_ordered, the PCG's own product on vectors in the SpMV's row
+    order, is the oracle's product permuted by mi_sys_spmv_order: the order
+    is a permutation, other than the identity with a MAS attached, that puts
+    the unpartitioned vertices after the partitioned ones; x^T A x is the
+    oracle's"""
+    s = c1_contact
+    parts = s.part_ids.copy()
+    if unpart:
+        parts[::7] = -1
+    ls = assembled(ctx, s, precision)
+    assert np.array_equal(ls.spmv_order(), np.arange(s.block_rows))  # no MAS: the identity
+    ls.add_preconditioner(abi.MI_PRECOND_MAS, 0, s.n, part_ids=parts, tets=s.tets)
+    ls.assemble_preconditioners()
+    pos = ls.spmv_order()
+    assert np.array_equal(np.sort(pos), np.arange(s.block_rows))
+    assert not np.array_equal(pos, np.arange(s.block_rows))
+    if unpart:
+        assert pos[parts < 0].min() > pos[parts >= 0].max()
+    orr, oc, ov, _ = O.convert_sym(s.block_rows, s.rows, s.cols, s.vals)
+    x = np.random.default_rng(31).standard_normal(s.n)
+    yref, dref = O.spmv_sym(s.block_rows, orr, oc, ov, x)
+    xo = np.empty((s.block_rows, 3))
+    xo[pos] = x.reshape(-1, 3)
+    yd = ctx.alloc(s.n, np.float64)
+    dd = ctx.alloc(1, np.float64)
+    ls.spmv_ordered(ctx.to_device(xo.ravel()), yd, dd)
+    y = yd.numpy().reshape(-1, 3)[pos].ravel()
+    tol = 1e-13 if precision == abi.MI_FP64 else 1e-6
+    assert np.abs(y - yref).max() <= tol * np.abs(yref).max()
+    assert dd.numpy()[0] == pytest.approx(dref, rel=tol * 10)
+    ls.close()
+
+
+@pytest.mark.parametrize("precision", [abi.MI_FP64, abi.MI_FP32])
+def test_pcg_row_order_and_caller_order(ctx, c1_contact, precision):
+    """the PCG on vectors in the SpMV's row order (the default with a MAS
+    attached) and on the caller's vertex order (keep_order = 1) run the same
+    iteration up to rounding: iterations within 2%, the same solution to the
+    tolerance, in both directions of a switch between captured graphs, and
+    from a given x0"""
+    s = c1_contact
+    ls = assembled(ctx, s, precision)
+    ls.add_preconditioner(abi.MI_PRECOND_MAS, 0, s.n, part_ids=s.part_ids, tets=s.tets)
+    ls.assemble_preconditioners()
+    b = ctx.to_device(s.rhs)
+    # FP32 value planes: the true residual levels off near 1e-5 (the float
+    # products), so the FP32 solve stops there
+    tol, true_tol, x_tol = (1e-8, 2e-8, 1e-6) if precision == abi.MI_FP64 else (1e-5, 1e-4, 1e-3)
+    kw = dict(criterion=abi.MI_CRIT_REL_RESIDUAL, tol=tol, true_residual=1, use_graph=1)
+    res = []
+    for keep in (0, 1, 0, 1):
+        x = ctx.alloc(s.n, np.float64)
+        st = ls.solve(b, x, keep_order=keep, **kw)
+        assert st["converged"] == 1 and st["rel_res_true"] <= true_tol, (keep, st)
+        res.append((st, x.numpy()))
+    it0, x0 = res[0][0]["iters_exact"], res[0][1]
+    for st, x in res[1:]:
+        assert abs(st["iters_exact"] - it0) <= max(2, it0 // 50)
+        assert rel_err(x, x0) < x_tol
+    # from x0 = a perturbed solution, in either order
+    rng = np.random.default_rng(4)
+    xg = x0 + 1e-3 * np.abs(x0).max() * rng.standard_normal(s.n)
+    for keep in (0, 1):
+        x = ctx.to_device(xg)
+        st = ls.solve(b, x, x_init=1, keep_order=keep, **kw)
+        assert st["converged"] == 1 and st["rel_res_true"] <= true_tol, (keep, st)
+        assert st["iters_exact"] < it0 and rel_err(x.numpy(), x0) < x_tol
     ls.close()
 
 
