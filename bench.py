@@ -364,10 +364,12 @@ def fp32_arm(ctx, s, args, rows, cols, vals, b, x64, ms64, solve_kw) -> dict:
     def timed():
         for _ in range(args.warmup):
             step()
+        shapes = None
         if args.spmv_shape == "auto":
             from libuipc_amd.tune import tune_spmv_shape
 
-            tune_spmv_shape(ls, x)
+            t = tune_spmv_shape(ls, x)
+            shapes = {"best": t["best"], "us": {str(k): v["us"] for k, v in t["table"].items()}}
         else:
             ls.set_spmv_shape(args.spmv_shape)
         ev = EventTimer(stream)
@@ -385,7 +387,7 @@ def fp32_arm(ctx, s, args, rows, cols, vals, b, x64, ms64, solve_kw) -> dict:
         x32 = x.numpy()
         ms = float(np.median(times))
         return {"ms": round(ms, 3), "ratio_to_fp64": round(ms / ms64, 4), "assemble_ms": round(ms_asm, 3),
-                "pcg_iters": int(st["iters"]), "rel_res_true": st["rel_res_true"],
+                "spmv_shape": shapes, "pcg_iters": int(st["iters"]), "rel_res_true": st["rel_res_true"],
                 "x_rel_diff_to_fp64": float(np.linalg.norm(x32 - x64) / max(np.linalg.norm(x64), 1e-300))}
 
     same = timed()
@@ -523,6 +525,16 @@ def run_batched(args) -> None:
         log(f"warmup {w}: iterations min/median/max "
             f"{min(s['iters_exact'] for s in stats)}/{int(np.median([s['iters_exact'] for s in stats]))}/"
             f"{max(s['iters_exact'] for s in stats)}")
+    lv_tuned = None
+    if args.mas_levels == "auto":
+        # untimed, like the warm-up: a MAS level count per system
+        from libuipc_amd.tune import tune_mas_levels_batched
+
+        lv_tuned = tune_mas_levels_batched(ls, 0, seg32, b, x, **solve_kw)
+        log(f"per-system MAS levels: {np.bincount(lv_tuned['levels']).tolist()} systems at 0.. levels; "
+            f"one count for all: {lv_tuned['uniform_best']}")
+    elif args.mas_levels:
+        ls.mas_set_active_levels(0, args.mas_levels)
     ev = EventTimer(st)
     ctx.synchronize()
     times = []
@@ -548,7 +560,9 @@ def run_batched(args) -> None:
         "config": {"workload": f"{args.systems} independent FEM + contact systems, DoFs log-uniform in "
                                f"[{args.dof_lo:g}, {args.dof_hi:g}], one batched solve",
                    "systems": args.systems, "dofs": 3 * Btot, "block_rows": Btot, "triplets": T,
-                   "grid_min": int(grids.min()), "grid_max": int(grids.max()), "preconditioner": "mas (union)"},
+                   "grid_min": int(grids.min()), "grid_max": int(grids.max()), "preconditioner": "mas (union)",
+                   "mas_levels": ({"per_system_counts": np.bincount(lv_tuned["levels"]).tolist(),
+                                   "uniform_best": lv_tuned["uniform_best"]} if lv_tuned else args.mas_levels)},
         "pcg_iters_exact": {"min": int(its.min()), "median": int(np.median(its)), "max": int(its.max())},
         "converged": conv, "phase_ms": {"assemble": round(ms_asm, 3), "precond_assemble": round(ms_pc, 3),
                                         "pcg": round(ms - ms_asm - ms_pc, 3)},
@@ -657,7 +671,8 @@ def run_learned(args) -> None:
     the contact stiffness log-uniform in [1e4, kappa_hi] with kappa_hi
     log-uniform in [1e6, 1e8], rhs uniform in [-1, 1]), the triplets written
     by the device producers.  The
-    learned initial guess is trained one system per Adam step on
+    learned initial guess (with its coarse-level correction on the MAS's
+    first coarse level) is trained one system per Adam step on
     --train-systems seeds, then MAS-PCG to 1e-4 from x0 = 0, from the
     block-Jacobi guess D^-1 b (the network with its output at zero) and from
     the learned x0 (with its fallback) are compared on --test-systems held-out
@@ -668,7 +683,7 @@ def run_learned(args) -> None:
     from libuipc_amd.abi import check
     from libuipc_amd.device import Context, LinearSystem
     from libuipc_amd.hip_events import EventTimer
-    from libuipc_amd.warmstart import WarmStartNet, build_graph
+    from libuipc_amd.warmstart import WarmStartNet, build_graph, coarse_of_mas
 
     grid = 16
     ctx = Context(0)
@@ -682,6 +697,7 @@ def run_learned(args) -> None:
     dtets = ctx.to_device(first.tets)
     ls = LinearSystem(ctx, B)
     ls.add_preconditioner(abi.MI_PRECOND_MAS, 0, 3 * B, part_ids=first.part_ids, tets=first.tets)
+    cmap = coarse_of_mas(ls, 0)  # the coarse-level correction's nodes: the MAS level 1
 
     def make(seed):
         rng = np.random.default_rng(seed)
@@ -699,7 +715,7 @@ def run_learned(args) -> None:
                                              v0 + 72 * o, st))
         ls.assemble(rows, cols, vals, st)
         ctx.synchronize()  # the per-system inputs are freed next
-        return build_graph(ctx, ls, rng.uniform(-1.0, 1.0, 3 * B))
+        return build_graph(ctx, ls, rng.uniform(-1.0, 1.0, 3 * B), coarse=cmap)
 
     net = WarmStartNet(ctx, seed=args.seed)
     t0 = time.perf_counter()

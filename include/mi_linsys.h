@@ -324,6 +324,19 @@ int mi_precond_reinit(mi_sys* sys, int handle, const mi_precond_desc* desc);
  * apply or solve (a captured solve is re-captured).  MI_ERR_ARG for a
  * handle that is not a MAS preconditioner. */
 int mi_precond_mas_set_active_levels(mi_sys* sys, int handle, int32_t levels);
+/* A level count per system for batched solves (no counterpart in the
+ * reference, which holds one count per engine): system g, block rows
+ * [seg_rows[g], seg_rows[g + 1]) (host arrays, nseg + 1 and nseg entries),
+ * uses levels[g] levels (0 or out of range: every level).  A fine cluster
+ * takes its system's count; the restriction and coarse solves run to the
+ * largest.  nseg = 0, or mi_precond_mas_set_active_levels, returns to one
+ * count.  MI_ERR_ARG for a handle that is not a MAS preconditioner or a
+ * vertex outside the systems' rows. */
+int mi_precond_mas_set_system_levels(mi_sys*        sys,
+                                     int            handle,
+                                     int32_t        nseg,
+                                     const int32_t* seg_rows,
+                                     const int32_t* levels);
 
 /* LocalPreconditioner::do_assemble for every registered preconditioner
  * (global_linear_system.cu:499-515).  Requires a prior mi_sys_assemble. */

@@ -173,6 +173,7 @@ LINSYS_PROTOS = {
     "mi_precond_add": (C.c_int, [C.c_void_p, C.POINTER(PrecondDesc), C.POINTER(C.c_int)]),
     "mi_precond_reinit": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(PrecondDesc)]),
     "mi_precond_mas_set_active_levels": (C.c_int, [C.c_void_p, C.c_int, C.c_int32]),
+    "mi_precond_mas_set_system_levels": (C.c_int, [C.c_void_p, C.c_int, C.c_int32, C.c_void_p, C.c_void_p]),
     "mi_precond_assemble": (C.c_int, [C.c_void_p, C.c_void_p]),
     "mi_precond_apply": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "mi_precond_mas_export": (C.c_int, [C.c_void_p, C.c_int, c_i32p, c_i32p, c_i32p, c_i32p, c_i32p, c_i32p, c_f64p,

@@ -847,6 +847,23 @@ int mi_precond_mas_set_active_levels(mi_sys* sys, int handle, int32_t levels)
     MI_API_END
 }
 
+int mi_precond_mas_set_system_levels(mi_sys* sys, int handle, int32_t nseg, const int32_t* seg_rows,
+                                     const int32_t* levels)
+{
+    MI_API_BEGIN
+    MI_REQUIRE(sys && handle >= 0 && handle < (int)sys->pcs.size(), MI_ERR_ARG,
+               "mi_precond_mas_set_system_levels: bad handle");
+    MI_REQUIRE(nseg >= 0 && (nseg == 0 || (seg_rows && levels)), MI_ERR_ARG,
+               "mi_precond_mas_set_system_levels: null arrays");
+    for(int32_t g = 0; g < nseg; ++g)
+        MI_REQUIRE(seg_rows[g + 1] > seg_rows[g], MI_ERR_ARG,
+                   "mi_precond_mas_set_system_levels: empty or decreasing segment");
+    MI_REQUIRE(sys->pcs[handle]->set_system_levels(nseg, seg_rows, levels), MI_ERR_ARG,
+               "mi_precond_mas_set_system_levels: not a MAS preconditioner");
+    ++sys->pcs_gen;  // a captured solve launches the old counts' kernels
+    MI_API_END
+}
+
 int mi_precond_assemble(mi_sys* sys, void* stream)
 {
     MI_API_BEGIN

@@ -410,6 +410,8 @@ class LocalPreconditioner
     }
     // MAS only: levels used by apply (MASPreconditionerEngine::set_active_level_num)
     virtual bool set_active_levels(int) { return false; }
+    // MAS only, batched solves: a level count per system (block-row ranges)
+    virtual bool set_system_levels(int32_t, const int32_t*, const int32_t*) { return false; }
     // MAS only: the rank of every vertex of the range in the partition's part
     // order (parts in id order, then the unpartitioned vertices): the order
     // of locality the SpMV's window layout puts the rows in

@@ -1098,7 +1098,12 @@ __global__ __launch_bounds__(256) void mas_restrict_level_kernel(const double* _
 // With kLevel1 (every level-1 bank's parts lie in one 256-slot range, checked
 // at the build) the workgroup also holds the whole level-1 bank its slots
 // restrict to, and the bank's sibling sums give level 2's R as well.
-template <typename T, bool kLevel1>
+//
+// With kOrd the vectors are in the part order: the live slots of workgroup b
+// hold the consecutive vector rows [blk_first[b], blk_first[b + 1]), so the
+// x, r update runs one lane per scalar over that range (coalesced) and hands
+// the new r to the slot lanes through LDS.
+template <typename T, bool kLevel1, bool kOrd>
 __global__ __launch_bounds__(256) void mas_update_xr_restrict_kernel(PcgScalars*   sc,
                                                                     const double* pap_part,
                                                                     int           gp,
@@ -1115,13 +1120,15 @@ __global__ __launch_bounds__(256) void mas_update_xr_restrict_kernel(PcgScalars*
                                                                     const int32_t* __restrict__ unpart_vec,
                                                                     int32_t       n_unpart,
                                                                     int32_t       begin1,
-                                                                    int32_t       count1)
+                                                                    int32_t       count1,
+                                                                    const int32_t* __restrict__ blk_first)
 {
     if(sc->converged)
         return;
     __shared__ double lds[256 / kWave];
     __shared__ T      r1s[kBank][3];
     __shared__ int32_t bank1;
+    __shared__ double rs[kOrd ? 3 * 256 : 1];  // the workgroup's new r (kOrd)
     const double      alpha = pcg_alpha_of<256>(sc, pap_part, gp, lds);
     if((int32_t)blockIdx.x < slot_blocks)
     {
@@ -1137,7 +1144,28 @@ __global__ __launch_bounds__(256) void mas_update_xr_restrict_kernel(PcgScalars*
                 bank1 = -1;
             __syncthreads();
         }
-        if(v >= 0)
+        if constexpr(kOrd)
+        {
+            const int64_t f0 = blk_first[blockIdx.x];
+            const int64_t ne = 3 * ((int64_t)blk_first[blockIdx.x + 1] - f0);
+            for(int64_t e = threadIdx.x; e < ne; e += 256)
+            {
+                const int64_t i  = 3 * f0 + e;
+                const double  rn = r[i] - alpha * Ap[i];
+                x[i] += alpha * p[i];
+                r[i]  = rn;
+                rs[e] = rn;
+            }
+            __syncthreads();
+            if(v >= 0)
+            {
+                const int64_t q = 3 * ((int64_t)v - f0);
+                v0              = (T)rs[q];
+                v1              = (T)rs[q + 1];
+                v2              = (T)rs[q + 2];
+            }
+        }
+        else if(v >= 0)
         {
             const int64_t i  = 3 * (int64_t)v;
             const double  r0 = r[i] - alpha * Ap[i], r1 = r[i + 1] - alpha * Ap[i + 1],
@@ -1302,7 +1330,8 @@ __global__ __launch_bounds__(64) void mas_apply_fine_kernel(const double* __rest
                                                            int32_t        n_unpart,
                                                            const double* __restrict__ Dun,
                                                            const int32_t* conv,
-                                                           const int32_t* __restrict__ on)
+                                                           const int32_t* __restrict__ on,
+                                                           const int32_t* __restrict__ lvl_of)
 {
     if(conv && *conv != 0)
         return;
@@ -1326,6 +1355,8 @@ __global__ __launch_bounds__(64) void mas_apply_fine_kernel(const double* __rest
     __shared__ T  sR[kBank][4];
     __shared__ T  sC[kBank][3 * kBank];
     const int64_t k = blockIdx.x;
+    // the cluster's own level count (batched systems with their own counts)
+    const int La = lvl_of ? lvl_of[k] : Lact;
     // lane 3 i + c (< 48) owns component c of node i: its vertex and its
     // ancestors are loaded first, then the inverse stream is issued, then the
     // gathers of r and of the coarse Z, so the gathers overlap the stream
@@ -1339,7 +1370,7 @@ __global__ __launch_bounds__(64) void mas_apply_fine_kernel(const double* __rest
         v = slot_vec[slot];
 #pragma unroll
         for(int l = 1; l < kMaxLvl; ++l)
-            anc[l - 1] = l < Lact ? anc_slot[(int64_t)(l - 1) * N + slot] : 0;
+            anc[l - 1] = l < La ? anc_slot[(int64_t)(l - 1) * N + slot] : 0;
     }
     ClusterBlocks<T> cb;
     cluster_load<T>(inv + k * kCm, lane, cb);
@@ -1349,7 +1380,7 @@ __global__ __launch_bounds__(64) void mas_apply_fine_kernel(const double* __rest
         sR[i][c] = v >= 0 ? (T)r[3 * (int64_t)v + c] : T(0);
 #pragma unroll
         for(int l = 1; l < kMaxLvl; ++l)
-            zl[l - 1] = (v >= 0 && l < Lact) ? Zb[3 * (int64_t)anc[l - 1] + c] : T(0);
+            zl[l - 1] = (v >= 0 && l < La) ? Zb[3 * (int64_t)anc[l - 1] + c] : T(0);
     }
     __syncthreads();
     cluster_contrib<T>(cb, lane, sR, sC);
@@ -1359,7 +1390,7 @@ __global__ __launch_bounds__(64) void mas_apply_fine_kernel(const double* __rest
         T acc = cluster_rowsum<T>(sC, lane);
 #pragma unroll
         for(int l = 1; l < kMaxLvl; ++l)
-            if(l < Lact)
+            if(l < La)
                 acc += zl[l - 1];
         z[3 * (int64_t)v + c] = (double)acc;
     }
@@ -1385,7 +1416,8 @@ __global__ __launch_bounds__(256) void mas_apply_fine_dot_kernel(const double* _
                                                                 int32_t        n_unpart,
                                                                 const double* __restrict__ Dun,
                                                                 const int32_t* conv,
-                                                                double*        part)
+                                                                double*        part,
+                                                                const int32_t* __restrict__ lvl_of)
 {
     if(conv && *conv != 0)
         return;
@@ -1422,6 +1454,7 @@ __global__ __launch_bounds__(256) void mas_apply_fine_dot_kernel(const double* _
         int32_t v  = -1;
         double  rv = 0.0;
         T       zl[kMaxLvl - 1];
+        const int La = (clus && lvl_of) ? lvl_of[k] : Lact;  // the cluster's own level count
         if(clus)
         {
             const int64_t slot = k * kBank + i;
@@ -1431,7 +1464,7 @@ __global__ __launch_bounds__(256) void mas_apply_fine_dot_kernel(const double* _
                 v = slot_vec[slot];
 #pragma unroll
                 for(int l = 1; l < kMaxLvl; ++l)
-                    anc[l - 1] = l < Lact ? anc_slot[(int64_t)(l - 1) * N + slot] : 0;
+                    anc[l - 1] = l < La ? anc_slot[(int64_t)(l - 1) * N + slot] : 0;
             }
             ClusterBlocks<T> cb;
             cluster_load<T>(inv + k * kCm, lane, cb);
@@ -1441,7 +1474,7 @@ __global__ __launch_bounds__(256) void mas_apply_fine_dot_kernel(const double* _
                 sR[w][i][c] = (T)rv;
 #pragma unroll
                 for(int l = 1; l < kMaxLvl; ++l)
-                    zl[l - 1] = (v >= 0 && l < Lact) ? Zb[3 * (int64_t)anc[l - 1] + c] : T(0);
+                    zl[l - 1] = (v >= 0 && l < La) ? Zb[3 * (int64_t)anc[l - 1] + c] : T(0);
             }
             __syncthreads();
             cluster_contrib<T>(cb, lane, sR[w], sC[w]);
@@ -1454,7 +1487,7 @@ __global__ __launch_bounds__(256) void mas_apply_fine_dot_kernel(const double* _
             T acc = cluster_rowsum<T>(sC[w], lane);
 #pragma unroll
             for(int l = 1; l < kMaxLvl; ++l)
-                if(l < Lact)
+                if(l < La)
                     acc += zl[l - 1];
             const double zv        = (double)acc;
             z[3 * (int64_t)v + c] = zv;
@@ -1497,6 +1530,7 @@ class MasPreconditioner final : public LocalPreconditioner
     // the vector index of every slot and unpartitioned vertex when the
     // vectors are in the SpMV's row order (the part order: rank_)
     DBuf<int32_t> d_part_to_pos, d_unpart_pos;
+    DBuf<int32_t> d_blk_first;  // per 256-slot block: the part-order rank of its first live slot (+ the total)
     bool          row_order_ = false;
     DBuf<int32_t> d_desc_ptr, d_desc_idx, d_desc_lpn;
     int32_t       n_unpart  = 0;
@@ -1515,6 +1549,9 @@ class MasPreconditioner final : public LocalPreconditioner
     bool           l1_local_ = false;  // every level-1 bank's parts lie in one 256-slot range
     DBuf<int32_t>  d_fine_on;          // per fine cluster: 1 while it holds a running system
     const int32_t* fine_on = nullptr;  // d_fine_on during a batched solve, else nullptr
+    DBuf<int32_t>  d_fine_lvl;         // per fine cluster: its system's level count (set_system_levels)
+    const int32_t* fine_lvl = nullptr;  // d_fine_lvl while per-system counts are set, else nullptr
+    int            lvl_max  = 0;        // the largest per-system count (the levels restricted and solved)
 
     explicit MasPreconditioner(const mi_precond_desc& d)
     {
@@ -1617,6 +1654,16 @@ class MasPreconditioner final : public LocalPreconditioner
                 unpart_pos[u] = rank_[unpart[u]];
             upload(d_part_to_pos, part_pos);
             upload(d_unpart_pos, unpart_pos);
+            std::vector<int32_t> blk_first((size_t)(h.N + 255) / 256 + 1, 0);
+            int32_t              live = 0;
+            for(int32_t slot = 0; slot < h.N; ++slot)
+            {
+                if(slot % 256 == 0)
+                    blk_first[slot / 256] = live;
+                live += h.part_to_real[slot] >= 0 ? 1 : 0;
+            }
+            blk_first.back() = live;
+            upload(d_blk_first, blk_first);
         }
         // base level: the first level at most kTailLevel nodes (restriction
         // kernels stop there); level-b descendants of every node above it
@@ -1758,7 +1805,7 @@ class MasPreconditioner final : public LocalPreconditioner
                 double* part = nullptr, int max_parts = 0)
     {
         const int L    = h.L;
-        const int Lact = (active_levels > 0 && active_levels <= L) ? active_levels : L;
+        const int Lact = levels_run();
         const int32_t* pv = slot_vec();
         const int32_t* uv = unpart_vec();
         // one active level is the 16-node block Jacobi: no coarse level is read
@@ -1784,13 +1831,14 @@ class MasPreconditioner final : public LocalPreconditioner
         {
             const int g = (int)std::max<int64_t>(1, std::min<int64_t>(max_parts, ceil_div(blocks, (int64_t)4)));
             mas_apply_fine_dot_kernel<T><<<g, 256, 0, s>>>(r, z, pv, inv, Z, d_anc_slot.data(), h.N, Lact, C0,
-                                                           d_unpart.data(), uv, n_unpart, d_Dun.data(), conv, part);
+                                                           d_unpart.data(), uv, n_unpart, d_Dun.data(), conv, part,
+                                                           fine_lvl);
             return g;
         }
         if(blocks > 0)
             mas_apply_fine_kernel<T><<<(unsigned)blocks, 64, 0, s>>>(r, z, pv, inv, Z, d_anc_slot.data(), h.N,
                                                                      Lact, C0, d_unpart.data(), uv, n_unpart,
-                                                                     d_Dun.data(), conv, fine_on);
+                                                                     d_Dun.data(), conv, fine_on, fine_lvl);
         return 0;
     }
 
@@ -1848,6 +1896,51 @@ class MasPreconditioner final : public LocalPreconditioner
     bool set_active_levels(int levels) override
     {
         active_levels = levels;  // 0 or out of range: every level, as the reference clamps
+        fine_lvl      = nullptr;  // one count for every system again
+        return true;
+    }
+
+    // the level count the restriction and the coarse solves run to
+    int levels_run() const
+    {
+        if(fine_lvl)
+            return lvl_max;
+        return (active_levels > 0 && active_levels <= h.L) ? active_levels : h.L;
+    }
+
+    // batched solves: system g (block rows [seg[g], seg[g+1]), global) uses
+    // levels[g] levels (0 or out of range: every level); a fine cluster takes
+    // its system's count (the largest of its systems' should a cluster hold
+    // vertices of several); nseg = 0 returns to the one count
+    bool set_system_levels(int32_t nseg, const int32_t* seg, const int32_t* levels) override
+    {
+        if(nseg <= 0)
+        {
+            fine_lvl = nullptr;
+            return true;
+        }
+        const int32_t        off = (int32_t)(dof_offset / 3);
+        std::vector<int32_t> lv((size_t)std::max<int64_t>(C0, 1), 1);
+        int                  mx = 1;
+        for(int64_t k = 0; k < C0; ++k)
+            for(int i = 0; i < kBank; ++i)
+            {
+                const int32_t v = h.part_to_real[k * kBank + i];
+                if(v < 0)
+                    continue;
+                const int32_t  row = off + v;
+                const int32_t* it  = std::upper_bound(seg, seg + nseg + 1, row);
+                MI_REQUIRE(it != seg && it != seg + nseg + 1, MI_ERR_ARG,
+                           "mi_precond_mas_set_system_levels: a vertex outside the systems' rows");
+                const int32_t g  = (int32_t)(it - seg) - 1;
+                const int     lg = (levels[g] > 0 && levels[g] <= h.L) ? levels[g] : h.L;
+                lv[k]            = std::max<int32_t>(lv[k], lg);
+                mx               = std::max(mx, lg);
+            }
+        upload(d_fine_lvl, lv);
+        MI_HIP(hipDeviceSynchronize());  // the host vector goes out of scope
+        fine_lvl = d_fine_lvl.data();
+        lvl_max  = mx;
         return true;
     }
 
@@ -1870,7 +1963,7 @@ class MasPreconditioner final : public LocalPreconditioner
             const char* e = std::getenv("MI_MAS_FUSE_XR");
             return e ? std::atoi(e) : 1;
         }();
-        const int Lact = (active_levels > 0 && active_levels <= h.L) ? active_levels : h.L;
+        const int Lact = levels_run();
         if(!on || h.V == 0 || Lact <= 1 || base_lvl < 1 || span.count[0] == 0)
             return false;
         const int32_t slot_blocks = ceil_div(span.count[0], 256);
@@ -1882,16 +1975,27 @@ class MasPreconditioner final : public LocalPreconditioner
             return e ? std::atoi(e) : 1;
         }();
         const bool lvl1 = l1 && l1_local_ && base_lvl >= 2 && Lact > 2;
-        auto       go   = [&](auto* R, auto tag) {
+        // MI_MAS_XR_ORD=0 keeps the per-slot update with vectors in the part order
+        static const int ord_env = [] {
+            const char* e = std::getenv("MI_MAS_XR_ORD");
+            return e ? std::atoi(e) : 1;
+        }();
+        const bool ord = row_order_ && ord_env;
+        auto       go  = [&](auto* R, auto tag) {
             using T = decltype(tag);
-            if(lvl1)
-                mas_update_xr_restrict_kernel<T, true><<<blocks, 256, 0, s>>>(
-                    sc, pap_part, gp, x, p, r, Ap, slot_vec(), R, d_parent.data(), d_sib.data(), span.count[0],
-                    slot_blocks, unpart_vec(), n_unpart, span.begin[1], span.count[1]);
+            auto launch = [&](auto kernel) {
+                kernel<<<blocks, 256, 0, s>>>(sc, pap_part, gp, x, p, r, Ap, slot_vec(), R, d_parent.data(),
+                                              d_sib.data(), span.count[0], slot_blocks, unpart_vec(), n_unpart,
+                                              span.begin[1], span.count[1], d_blk_first.data());
+            };
+            if(lvl1 && ord)
+                launch(mas_update_xr_restrict_kernel<T, true, true>);
+            else if(lvl1)
+                launch(mas_update_xr_restrict_kernel<T, true, false>);
+            else if(ord)
+                launch(mas_update_xr_restrict_kernel<T, false, true>);
             else
-                mas_update_xr_restrict_kernel<T, false><<<blocks, 256, 0, s>>>(
-                    sc, pap_part, gp, x, p, r, Ap, slot_vec(), R, d_parent.data(), d_sib.data(), span.count[0],
-                    slot_blocks, unpart_vec(), n_unpart, span.begin[1], span.count[1]);
+                launch(mas_update_xr_restrict_kernel<T, false, false>);
         };
         if(fp32)
             go(d_R32.data(), float{});

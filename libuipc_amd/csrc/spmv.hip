@@ -463,63 +463,71 @@ __global__ __launch_bounds__(NT) void spmv_win_kernel(int32_t nchunks,
             rp[nr] = wptr[R0 + nr];
         __syncthreads();
         const int32_t K0 = rp[0], K1 = rp[nr];
-        for(int32_t s0 = K0, s1; s0 < K1; s0 = s1)
+        // the stage's blocks in registers: lane slot j = VEC * h + e holds
+        // block o0 + VEC * (h * NT + tid) + e.  The next stage's loads are
+        // issued as soon as this stage's products are out, so they are in
+        // flight through the barriers and the row sums.
+        int32_t wd[kPer];
+        VT      A[kPer][9];
+        auto    load_stage = [&](int32_t o0, int32_t s0, int32_t s1)
         {
-            // slots [o0, o0 + kStage) cover the stage's blocks [s0, s1)
-            const int32_t o0 = s0 & ~(VEC - 1);
-            s1               = o0 + kStage < K1 ? o0 + kStage : K1;
-            if constexpr(VEC == 1)
-            {
 #pragma unroll
-                for(int h = 0; h < kPer; ++h)
+            for(int h = 0; h < kPer / VEC; ++h)
+            {
+                const int32_t k = o0 + VEC * (h * NT + tid);
+                if constexpr(VEC == 1)
                 {
-                    const int     i = h * NT + tid;
-                    const int32_t k = o0 + i;
                     if(k < s1)
                     {
-                        const int32_t wd = stream_load<VT>(&words[k]);
-                        VT            A[9];
+                        wd[h] = stream_load<VT>(&words[k]);
 #pragma unroll
                         for(int q = 0; q < 9; ++q)
-                            A[q] = stream_load<VT>(&vals[q * stride + k]);
-                        win_block<VT, W, ORD, KW, kStage>(wd, A, x, real, xs, ys, w, i);
+                            A[h][q] = stream_load<VT>(&vals[q * stride + k]);
                     }
                 }
-            }
-            else
-            {
-                using V2 = typename Vec2<VT>::T;
-#pragma unroll
-                for(int h = 0; h < kPer / 2; ++h)
+                else
                 {
-                    const int     i   = 2 * (h * NT + tid);
-                    const int32_t k   = o0 + i;
-                    const bool    ok0 = k >= s0 && k < s1, ok1 = k + 1 < s1;
-                    if(ok0 || ok1)
+                    using V2 = typename Vec2<VT>::T;
+                    if((k >= s0 && k < s1) || k + 1 < s1)
                     {
-                        const int2v wd = stream_load<VT>(reinterpret_cast<const int2v*>(&words[k]));
-                        V2         A2[9];
+                        const int2v w2 = stream_load<VT>(reinterpret_cast<const int2v*>(&words[k]));
+                        wd[2 * h]      = w2.x;
+                        wd[2 * h + 1]  = w2.y;
 #pragma unroll
                         for(int q = 0; q < 9; ++q)
-                            A2[q] = stream_load<VT>(reinterpret_cast<const V2*>(&vals[q * stride + k]));
-                        VT A[9];
-                        if(ok0)
                         {
-#pragma unroll
-                            for(int q = 0; q < 9; ++q)
-                                A[q] = A2[q].x;
-                            win_block<VT, W, ORD, KW, kStage>(wd.x, A, x, real, xs, ys, w, i);
-                        }
-                        if(ok1)
-                        {
-#pragma unroll
-                            for(int q = 0; q < 9; ++q)
-                                A[q] = A2[q].y;
-                            win_block<VT, W, ORD, KW, kStage>(wd.y, A, x, real, xs, ys, w, i + 1);
+                            const V2 a2      = stream_load<VT>(reinterpret_cast<const V2*>(&vals[q * stride + k]));
+                            A[2 * h][q]      = a2.x;
+                            A[2 * h + 1][q]  = a2.y;
                         }
                     }
                 }
             }
+        };
+        auto compute_stage = [&](int32_t o0, int32_t s0, int32_t s1)
+        {
+#pragma unroll
+            for(int j = 0; j < kPer; ++j)
+            {
+                const int     i = VEC * ((j / VEC) * NT + tid) + j % VEC;
+                const int32_t k = o0 + i;
+                if(k >= s0 && k < s1)
+                    win_block<VT, W, ORD, KW, kStage>(wd[j], A[j], x, real, xs, ys, w, i);
+            }
+        };
+        int32_t s0 = K0;
+        int32_t o0 = s0 & ~(VEC - 1);  // slots [o0, o0 + kStage) cover the stage's blocks [s0, s1)
+        int32_t s1 = o0 + kStage < K1 ? o0 + kStage : K1;
+        if(s0 < K1)
+            load_stage(o0, s0, s1);
+        while(s0 < K1)
+        {
+            compute_stage(o0, s0, s1);
+            const int32_t n0  = s1;
+            const int32_t no0 = n0 & ~(VEC - 1);
+            const int32_t n1  = no0 + kStage < K1 ? no0 + kStage : K1;
+            if(n0 < K1)
+                load_stage(no0, n0, n1);
             __syncthreads();
             // the rows meeting the stage: from the one holding block s0 (the
             // last row starting at or before it)
@@ -561,6 +569,9 @@ __global__ __launch_bounds__(NT) void spmv_win_kernel(int32_t nchunks,
                 }
             }
             __syncthreads();  // w is rewritten by the next stage
+            s0 = n0;
+            o0 = no0;
+            s1 = n1;
         }
         for(int l = tid; l < nr; l += NT)
         {
@@ -614,7 +625,8 @@ struct WinShape
 };
 #define MI_WIN_SHAPES(X)                                                                              \
     X(0, 512, 9, 2, 2) X(1, 256, 8, 3, 1) X(2, 512, 9, 2, 1) X(3, 256, 8, 2, 2) X(4, 256, 8, 2, 1) \
-    X(5, 512, 9, 1, 1) X(6, 1024, 10, 1, 1) X(7, 256, 7, 2, 1) X(8, 256, 8, 4, 2)
+    X(5, 512, 9, 1, 1) X(6, 1024, 10, 1, 1) X(7, 256, 7, 2, 1) X(8, 256, 8, 4, 2) X(9, 512, 9, 4, 2) \
+    X(10, 1024, 9, 2, 2)
 #define MI_WIN_ENTRY(i, nt, sh, per, vec) {nt, sh, per, vec},
 constexpr WinShape kWinShapes[] = {MI_WIN_SHAPES(MI_WIN_ENTRY)};
 #undef MI_WIN_ENTRY

@@ -308,6 +308,17 @@ class LinearSystem:
     def mas_set_active_levels(self, handle: int, levels: int) -> None:
         check(self.lib.mi_precond_mas_set_active_levels(self.h, handle, levels))
 
+    def mas_set_system_levels(self, handle: int, seg_rows, levels) -> None:
+        """mi_precond_mas_set_system_levels: levels[g] for the batched system
+        on block rows seg_rows[g] .. seg_rows[g + 1]; empty lists clear"""
+        seg = np.ascontiguousarray(seg_rows, dtype=np.int32)
+        lv = np.ascontiguousarray(levels, dtype=np.int32)
+        nseg = max(0, seg.size - 1) if lv.size else 0
+        if nseg and lv.size != nseg:
+            raise ValueError("one level count per system")
+        check(self.lib.mi_precond_mas_set_system_levels(self.h, handle, nseg, seg.ctypes.data_as(C.c_void_p),
+                                                        lv.ctypes.data_as(C.c_void_p)))
+
     def mas_export(self, handle: int, matrices: bool = True) -> dict:
         info = np.zeros(6, np.int32)
         check(self.lib.mi_precond_mas_export(self.h, handle, info.ctypes.data_as(abi.c_i32p), None, None, None, None,

@@ -46,6 +46,30 @@ def tune_mas_levels(ls: LinearSystem, handle: int, b, x, levels=None, repeats: i
     return {"best": best, "table": table}
 
 
+def tune_mas_levels_batched(ls: LinearSystem, handle: int, seg_rows, b, x, levels=None, **solve_kw) -> dict:
+    """Per-system MAS level counts for a batched solve: the batch is solved
+    once per candidate count (every system at that count) and each system
+    keeps the count that took it the fewest iterations (ties: fewer levels,
+    the cheaper iteration); the per-system counts are left set
+    (mi_precond_mas_set_system_levels).  Returns {"levels": [per system],
+    "table": {L: [iterations per system]}, "uniform_best": the single count
+    with the fewest iterations of the slowest system}."""
+    n_levels = ls.mas_export(handle, matrices=False)["L"]
+    cand = sorted({int(v) for v in (levels or range(1, n_levels + 1)) if 1 <= int(v) <= n_levels})
+    if not cand:
+        raise ValueError("tune_mas_levels_batched: no candidate level count in 1..%d" % n_levels)
+    table = {}
+    for lv in cand:
+        ls.mas_set_active_levels(handle, lv)
+        sts = ls.solve_batched(seg_rows, b, x, **solve_kw)
+        table[lv] = [int(st["iters_exact"]) if st["converged"] else 1 << 30 for st in sts]
+    nsys = len(table[cand[0]])
+    per = [min(cand, key=lambda lv: (table[lv][k], lv)) for k in range(nsys)]
+    uniform = min(cand, key=lambda lv: (max(table[lv]), lv))
+    ls.mas_set_system_levels(handle, seg_rows, per)
+    return {"levels": per, "table": table, "uniform_best": uniform}
+
+
 def tune_spmv_shape(ls: LinearSystem, x, reps: int = 20, shapes=None, margin: float = 0.02) -> dict:
     """Times `reps` products per SpMV launch shape (each one's window layout
     rebuilt for it, one untimed product first; the PCG's own product,

@@ -120,6 +120,55 @@ def test_batch_mas_union(ctx):
     ls.close()
 
 
+def test_batch_mas_per_system_levels(ctx):
+    """A MAS level count per system (mi_precond_mas_set_system_levels): every
+    system of the batch takes the iterations and the solution it takes when
+    the whole batch runs at its count (the union's clusters and coarse levels
+    never mix systems).  The batched tuner keeps, per system, the count with
+    the fewest iterations and leaves it set; mi_precond_mas_set_active_levels
+    returns to one count."""
+    from libuipc_amd.tune import tune_mas_levels_batched
+
+    bt = synth.batch_systems(6, seed=7, dof_lo=3e3, dof_hi=3e4)
+    ls = assembled(ctx, bt)
+    h = ls.add_preconditioner(abi.MI_PRECOND_MAS, 0, 3 * bt.block_rows, part_ids=bt.part_ids, tets=bt.tets,
+                              build=abi.MI_MAS_BUILD_DEVICE)
+    ls.assemble_preconditioners()
+    L = ls.mas_export(h, matrices=False)["L"]
+    cand = sorted({1, 2, L} & set(range(1, L + 1)))
+    assert len(cand) >= 2
+    b = ctx.to_device(bt.rhs)
+    kw = dict(criterion=abi.MI_CRIT_REL_RESIDUAL, tol=1e-6)
+    ref = {}
+    for lv in cand:
+        ls.mas_set_active_levels(h, lv)
+        x = ctx.alloc(3 * bt.block_rows, np.float64)
+        ref[lv] = (ls.solve_batched(bt.seg_rows, b, x, **kw), x.numpy())
+    levels = [cand[k % len(cand)] for k in range(len(bt.systems))]
+    ls.mas_set_system_levels(h, bt.seg_rows, levels)
+    x = ctx.alloc(3 * bt.block_rows, np.float64)
+    sts = ls.solve_batched(bt.seg_rows, b, x, **kw)
+    xs = x.numpy()
+    for k, lv in enumerate(levels):
+        seg = slice(3 * bt.seg_rows[k], 3 * bt.seg_rows[k + 1])
+        it = ref[lv][0][k]["iters_exact"]
+        assert sts[k]["converged"] == 1 and abs(sts[k]["iters_exact"] - it) <= max(2, it // 50), (k, lv, sts[k], it)
+        assert rel_err(xs[seg], ref[lv][1][seg]) < 1e-5, (k, lv)
+    t = tune_mas_levels_batched(ls, h, bt.seg_rows, b, x, levels=cand, **kw)
+    for k, lv in enumerate(t["levels"]):
+        assert t["table"][lv][k] == min(t["table"][c][k] for c in cand), (k, t)
+    sts = ls.solve_batched(bt.seg_rows, b, x, **kw)
+    assert all(st["converged"] == 1 for st in sts)
+    ls.mas_set_active_levels(h, L)
+    sts = ls.solve_batched(bt.seg_rows, b, x, **kw)
+    for k, st in enumerate(sts):
+        it = ref[L][0][k]["iters_exact"]
+        assert abs(st["iters_exact"] - it) <= max(2, it // 50), (k, st, it)
+    with pytest.raises(abi.MiError):
+        ls.mas_set_system_levels(h, [0, bt.block_rows // 2], [1])  # rows outside the systems
+    ls.close()
+
+
 @pytest.mark.parametrize("precision", [abi.MI_FP64, abi.MI_FP32], ids=["fp64", "fp32"])
 def test_batch_mas_skips_stopped_systems(ctx, precision):
     """The SpMV chunks and the MAS fine clusters of stopped systems are
