@@ -41,8 +41,8 @@ def parse():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--precond", choices=["mas", "diag"], default="mas")
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
-    ap.add_argument("--mas-invert", choices=["sweep", "mfma"], default="sweep",
-                    help="cluster inverse kernel: the column sweep (default) or the FP64 MFMA block sweep")
+    ap.add_argument("--mas-invert", choices=["sweep", "mfma"], default="mfma",
+                    help="cluster inverse kernel: the FP64 MFMA panel sweep (default) or the column sweep")
     ap.add_argument("--mas-levels", type=lambda v: v if v == "auto" else int(v), default=0,
                     help="MAS levels applied (MASPreconditionerEngine::set_active_level_num): 0 = all, the "
                          "default; auto = the fastest level count by libuipc_amd.tune, picked before timing")

@@ -206,13 +206,20 @@ int mi_spmv_dot_noalloc(mi_sys* sys, const double* x, double* y, double* d_dot, 
 
 /* The SpMV's launch shapes, for a tuner (the reference fits its SpMV grid to
  * each matrix, Spmv::fit_block_count, linear_system/spmv.cu:909-933).  count:
- * how many; dflt: the one new systems use; per shape (arrays of count
- * entries, any may be NULL): threads per workgroup, window rows, FP64 blocks
- * per lane and stage, blocks per lane and load (2: paired 16-byte FP64 /
- * 8-byte FP32 loads). */
+ * how many; dflt: the one systems with enough rows get; per shape (arrays of
+ * count entries, any may be NULL): threads per workgroup, window rows, FP64
+ * blocks per lane and stage, blocks per lane and load (2: paired 16-byte
+ * FP64 / 8-byte FP32 loads). */
 int mi_spmv_shapes(int32_t* count, int32_t* dflt, int32_t* threads, int32_t* rows, int32_t* per, int32_t* vec);
-/* The shape this system's SpMV uses; an assembled window layout is rebuilt
- * for it on `stream` (its window size may differ). */
+enum
+{
+    MI_SPMV_SHAPE_AUTO = -1, /* by the system's size (the default): the largest window whose count
+                                fills the GPU (1.5 windows per CU), else the full BSR             */
+    MI_SPMV_SHAPE_FULL = -2  /* the full BSR (both triangles, a row gather)                       */
+};
+/* The layout this system's SpMV uses: a window shape (an index of
+ * mi_spmv_shapes), MI_SPMV_SHAPE_AUTO or MI_SPMV_SHAPE_FULL.  An assembled
+ * system gets the new layout at once, on `stream`. */
 int mi_sys_set_spmv_shape(mi_sys* sys, int32_t shape, void* stream);
 
 /* Diagnostic, no counterpart in the reference: the layout the SpMV streams.
@@ -299,9 +306,9 @@ enum mi_mas_build
 
 enum mi_mas_invert
 {
-    MI_MAS_INVERT_SWEEP = 0, /* the reference's column sweep, one pivot at a time      */
-    MI_MAS_INVERT_MFMA  = 1  /* 16x16 pivot tiles: Cholesky form of the sweep, tile
-                                products on FP64 matrix cores                        */
+    MI_MAS_INVERT_MFMA  = 0, /* the default: 4-row pivot panels, the sweep in its Cholesky
+                                form, the Schur updates on FP64 matrix cores            */
+    MI_MAS_INVERT_SWEEP = 1  /* the reference's column sweep, one pivot at a time       */
 };
 
 /* GlobalLinearSystem's local-preconditioner registration + LocalPreconditioner::do_init.

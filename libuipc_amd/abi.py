@@ -29,8 +29,8 @@ MI_PRECOND_MAS = 3
 
 MI_MAS_BUILD_HOST = 0
 MI_MAS_BUILD_DEVICE = 1
-MI_MAS_INVERT_SWEEP = 0
-MI_MAS_INVERT_MFMA = 1
+MI_MAS_INVERT_MFMA = 0
+MI_MAS_INVERT_SWEEP = 1
 
 MI_CRIT_RZ_RATIO = 0
 MI_CRIT_REL_RESIDUAL = 1
@@ -76,6 +76,9 @@ class PrecondDesc(C.Structure):
         ("edges_dev", C.c_void_p),
         ("mas_invert", C.c_int32),
     ]
+
+
+MI_SPMV_SHAPE_AUTO, MI_SPMV_SHAPE_FULL = -1, -2
 
 
 class PcgOpts(C.Structure):

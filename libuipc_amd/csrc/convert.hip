@@ -1056,7 +1056,8 @@ void MatrixConverter::convert(int32_t        B,
     counts_.resize(6);
     MI_HIP(hipMemsetAsync(counts_.data(), 0, 6 * sizeof(int64_t), s));
     codec_ = KeyCodec{(uint64_t)std::max(B, 1), 0, 0};
-    out.has_win = false;
+    out.has_win  = false;
+    out.has_full = false;
     if(T == 0 || B == 0)
     {
         out.U = 0;
@@ -1148,12 +1149,24 @@ void MatrixConverter::convert(int32_t        B,
         unpack_lower_kernel<uint64_t><<<grid_of(U), kThreads, 0, s>>>(d_off, key_ptr<uint64_t>(tcur), codec(),
                                                                       lower_rows_.data());
     row_ptr_kernel<<<grid_of(U + 1), kThreads, 0, s>>>(d_off, U, lower_rows_.data(), lower_ptr_.data(), B);
-    out.has_win = false;
-    if(!build_full)
-    {
-        MI_HIP(hipGetLastError());
+    out.has_win  = false;
+    out.has_full = false;
+    if(build_full)
+        this->build_full(out, want32, ratio, s);
+    MI_HIP(hipGetLastError());
+}
+
+// the full BSR (both triangles) of the last conversion, from its upper BCOO
+// and its sorted lower keys (both kept until the next conversion)
+void MatrixConverter::build_full(BsrMatrix& out, bool want32, double ratio, hipStream_t s)
+{
+    const int32_t B      = out.B;
+    const int64_t U      = out.U;
+    const bool    narrow = codec_.bb > 0;
+    const int     tcur   = tcur_;
+    int64_t*      d_off  = counts_.data() + 2;
+    if(B == 0)
         return;
-    }
     {
         uint32_t* cnt = reinterpret_cast<uint32_t*>(out.full_ptr.data());  // scanned in place
         full_count_kernel<<<grid_of(B + 1), kThreads, 0, s>>>(lower_ptr_.data(), out.row_ptr.data(), cnt, B);
@@ -1204,6 +1217,7 @@ void MatrixConverter::convert(int32_t        B,
                                                                   out.full_ptr.data(), B, out.fcols.data(),
                                                                   out.fvals.data(), out.fstride);
     }
+    out.has_full = true;
     MI_HIP(hipGetLastError());
 }
 }  // namespace mi

@@ -59,7 +59,7 @@ struct mi_sys
     std::vector<std::unique_ptr<LocalPreconditioner>> pcs;
     uint64_t                                          pcs_gen       = 0;  // bumped when pcs change
     uint64_t                                          order_gen     = 0;  // bumped when a preconditioner is added or rebuilt
-    int32_t                                           spmv_shape    = spmv_default_shape();
+    int32_t                                           spmv_shape    = spmv_shape_auto;  // or full, or a window shape
     bool                                              assembled     = false;
     bool                                              pcs_assembled = false;
 
@@ -183,13 +183,31 @@ static hipStream_t pick(mi_ctx* ctx, void* stream)
     return stream ? (hipStream_t)stream : ctx->stream;
 }
 
-// the SpMV's window layout of the assembled matrix, for the current row order:
-// the identity, permuted inside every MAS preconditioner's range by its part order
+// the layout the system's SpMV streams: a window shape index, or the full BSR
+static int layout_shape(const mi_sys* sys)
+{
+    if(!spmv_use_window())
+        return spmv_shape_full;
+    if(sys->spmv_shape == spmv_shape_auto)
+        return spmv_auto_shape(sys->desc.block_rows);
+    return sys->spmv_shape;
+}
+
+// the SpMV's layout of the assembled matrix: the full BSR (built from the
+// last conversion if it was not), or the window layout for the current row
+// order: the identity, permuted inside every MAS preconditioner's range by its
+// part order
 static void build_spmv_layout(mi_sys* sys, hipStream_t s)
 {
-    auto& A = sys->A;
-    if(!spmv_use_window())
+    auto&     A     = sys->A;
+    const int shape = layout_shape(sys);
+    if(shape < 0)
+    {
+        if(!A.has_full)
+            sys->conv.build_full(A, sys->desc.value_precision == MI_FP32, sys->desc.reserve_ratio, s);
+        A.has_win = false;
         return;
+    }
     const int32_t B = A.B;
     if(A.win_order != sys->order_gen || A.wpos.size() < (size_t)std::max(B, 1))
     {
@@ -213,7 +231,7 @@ static void build_spmv_layout(mi_sys* sys, hipStream_t s)
         MI_HIP(hipStreamSynchronize(s));  // the host vectors go out of scope
         A.win_order = sys->order_gen;
     }
-    sys->conv.build_window(A, sys->spmv_shape, sys->desc.value_precision == MI_FP32, sys->desc.reserve_ratio, s);
+    sys->conv.build_window(A, shape, sys->desc.value_precision == MI_FP32, sys->desc.reserve_ratio, s);
 }
 
 // the matrix buffers a captured PCG graph launches with
@@ -439,7 +457,7 @@ void assemble_t(mi_sys* sys, int64_t T, const int32_t* rows, const int32_t* cols
                           vals,
                           sys->A,
                           sys->desc.value_precision == MI_FP32,
-                          !spmv_use_window(),
+                          layout_shape(sys) < 0,
                           sys->desc.reserve_ratio,
                           s);
     build_spmv_layout(sys, s);
@@ -695,10 +713,12 @@ int mi_sys_set_spmv_shape(mi_sys* sys, int32_t shape, void* stream)
 {
     MI_API_BEGIN
     MI_REQUIRE(sys, MI_ERR_ARG, "mi_sys_set_spmv_shape: null sys");
-    MI_REQUIRE(shape >= 0 && shape < spmv_shape_count(), MI_ERR_ARG, "mi_sys_set_spmv_shape: unknown shape");
+    MI_REQUIRE(shape == spmv_shape_auto || shape == spmv_shape_full || (shape >= 0 && shape < spmv_shape_count()),
+               MI_ERR_ARG, "mi_sys_set_spmv_shape: unknown shape");
     sys->spmv_shape = shape;
-    // an assembled window layout is rebuilt for the new shape at once
-    if(sys->assembled && sys->A.has_win && sys->A.wshape != shape)
+    // an assembled system gets the new layout at once
+    const int want = layout_shape(sys);
+    if(sys->assembled && (want < 0 ? sys->A.has_win : (!sys->A.has_win || sys->A.wshape != want)))
         build_spmv_layout(sys, pick(sys->ctx, stream));
     MI_API_END
 }

@@ -68,6 +68,7 @@ struct BsrMatrix
     int32_t       wshape    = 0;  // the window kernel's launch shape the layout was built for
     int32_t       wnchunks  = 0;
     bool          has_win   = false;
+    bool          has_full  = false;  // the full BSR of the current assembly is built
     uint64_t      win_order = ~0ull;  // the row order the layout was built for
 };
 constexpr int kChunkStep = 512;  // nominal blocks per SpMV chunk
@@ -156,6 +157,8 @@ class MatrixConverter
         convert(B, T, rows, cols, vals, true, out, want32, build_full, reserve_ratio, s);
     }
     ~MatrixConverter();
+    // the full BSR of `A` (assembled by the last convert_sym)
+    void build_full(BsrMatrix& A, bool want32, double reserve_ratio, hipStream_t s);
     // the window layout of `A` (assembled by the last convert_sym) for the
     // row order in A.wpos / A.wreal and a window kernel shape
     void build_window(BsrMatrix& A, int shape, bool want32, double reserve_ratio, hipStream_t s);
@@ -237,6 +240,12 @@ int  spmv_shape_count();
 int  spmv_default_shape();
 bool spmv_shape_info(int shape, int* threads, int* rows, int* per, int* vec);
 int  spmv_shape_shift(int shape);
+// the layout codes beside the window shapes (mi_sys_set_spmv_shape)
+constexpr int spmv_shape_auto = -1;  // chosen by the system's size
+constexpr int spmv_shape_full = -2;  // the full BSR
+// the layout a system of B block rows gets when none is set: a window shape
+// index, or spmv_shape_full
+int spmv_auto_shape(int64_t B);
 // workgroup size of the SpMV kernel that streams A's layout
 int spmv_threads(const BsrMatrix& A);
 // the SpMV's work items (windows or chunks) and the row range [lo, hi) each touches

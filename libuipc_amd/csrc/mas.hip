@@ -678,205 +678,206 @@ __global__ __launch_bounds__(64) void mas_invert_kernel(const double* __restrict
     }
 }
 
-// The same inverse by a block sweep on FP64 matrix cores: one wave per
-// cluster, the 48x48 matrix resident in LDS as 3x3 tiles of 16x16.  Sweeping
-// pivot tile k of the symmetric matrix (A_ik = A_ki^T throughout):
-//   A_kk = L L^T (Cholesky),  W_j = L^{-1} A_kj          (j != k)
-//   A_ij <- A_ij - W_i^T W_j,  A_kj <- L^{-T} W_j,  A_kk <- -L^{-T} L^{-1}
-// which is the sweep A_ij -= A_ik A_kk^{-1} A_kj, A_kj <- A_kk^{-1} A_kj,
-// A_kk <- -A_kk^{-1} with the Schur update in its Cholesky form (the explicit
-// 16x16 inverse in A_ik A_kk^{-1} A_kj loses digits on stiff clusters).
-// After the three pivots the matrix holds -A^{-1}.  The Cholesky factor and
-// the triangular solves for L^{-1} and W run in registers, one column per
-// lane; the six 16x16x16 tile products per pivot are v_mfma_f64_16x16x4
-// chains of four.  No pivoting, as in the reference's sweep: the cluster
-// matrices are SPD and a zero diagonal (a padding slot) is set to 1.
+// The same inverse on FP64 matrix cores: one wave per cluster, the 48x48
+// matrix resident in MFMA accumulators (3x3 tiles of 16x16; tile (I, J)
+// entry v of lane 16g + c holds S(16I + 4v + g, 16J + c)), swept in twelve
+// pivot panels of 4 rows.  For panel K, with P = S_KK = L L^T (Cholesky):
+//   W   = L^-1 S_K,:  (its K columns zero)
+//   S  -= W^T W                    nine v_mfma_f64_16x16x4, one per tile
+//   S_K,: = L^-T W,  S_:,K = its transpose,  S_KK = -L^-T L^-1
+// which is the symmetric sweep S_ij -= S_iK P^-1 S_Kj, S_Kj <- P^-1 S_Kj,
+// S_KK <- -P^-1 with the Schur update in its Cholesky form: the explicit
+// inverse of a pivot block (S_iK P^-1 S_Kj) loses up to all digits on
+// clusters with stiff contact couplings (nearly rank-one pivot blocks),
+// the Cholesky form keeps the column sweep's accuracy.  After the twelve
+// panels S = -A^-1.  The cluster's blocks come in and its inverse goes out
+// through a 9.8 KB LDS copy (coalesced global accesses; scattered ones held
+// 36 addresses per lane).  Both MFMA operands of tile (I, J) are lane values of W
+// (A = W_I^T: lane 16g + c holds W(g, 16I + c); B = W_J: W(g, 16J + c)), so
+// a lane computes three values of W per panel; the panel's rows, W and
+// L^-T W pass between lanes through 4.5 KB of LDS.  The 4x4 Cholesky
+// factor and its inverse run redundantly on every lane.  No pivoting, as in
+// the reference's sweep: the cluster matrices are SPD and a zero diagonal (a
+// padding slot) is set to 1.
 //
 // v_mfma_f64_16x16x4 lane layout (lane l, g = l / 16, c = l % 16):
 //   A operand A[c][g], B operand B[g][c], accumulator v holds D[4 v + g][c]
 // (checked on gfx950 with distinct, unsymmetric operands).
 typedef double mfma_f64x4 __attribute__((ext_vector_type(4)));
 
-constexpr int kInvLd = 3 * kBank + 1;  // LDS row stride: 16 rows of one column fall in distinct banks
-
-// acc + sign * op(tile (ti, tk)) * tile (tk', tj): op = transpose when `ta`
-// (the A operand is then read from tile (tk, ti) of S)
-template <bool ta>
-__device__ __forceinline__ mfma_f64x4 tile_mma(const double* S, int ti, int tk, int tj, mfma_f64x4 acc, int lane,
-                                               double sign)
-{
-    const int g = lane >> 4, c = lane & 15;
-#pragma unroll
-    for(int s = 0; s < 4; ++s)
-    {
-        const double a = sign * (ta ? S[(16 * tk + 4 * s + g) * kInvLd + 16 * ti + c]
-                                    : S[(16 * ti + c) * kInvLd + 16 * tk + 4 * s + g]);
-        const double b = S[(16 * tk + 4 * s + g) * kInvLd + 16 * tj + c];
-        acc            = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
-    }
-    return acc;
-}
-
-__device__ __forceinline__ mfma_f64x4 tile_read(const double* S, int ti, int tj, int lane)
-{
-    const int  g = lane >> 4, c = lane & 15;
-    mfma_f64x4 v;
-#pragma unroll
-    for(int e = 0; e < 4; ++e)
-        v[e] = S[(16 * ti + 4 * e + g) * kInvLd + 16 * tj + c];
-    return v;
-}
-
-// tile (ti, tj) <- v, and tile (tj, ti) <- v^T when `both`
-__device__ __forceinline__ void tile_write(double* S, int ti, int tj, const mfma_f64x4& v, int lane, bool both)
-{
-    const int g = lane >> 4, c = lane & 15;
-#pragma unroll
-    for(int e = 0; e < 4; ++e)
-    {
-        S[(16 * ti + 4 * e + g) * kInvLd + 16 * tj + c] = v[e];
-        if(both)
-            S[(16 * tj + c) * kInvLd + 16 * ti + 4 * e + g] = v[e];
-    }
-}
-
-// v_readlane of a double from lane l (l the same on every lane)
-__device__ __forceinline__ double readlane_f64(double v, int l)
-{
-    const unsigned long long u  = (unsigned long long)__double_as_longlong(v);
-    const unsigned int       lo = (unsigned int)__builtin_amdgcn_readlane((int)(unsigned int)u, l);
-    const unsigned int       hi = (unsigned int)__builtin_amdgcn_readlane((int)(unsigned int)(u >> 32), l);
-    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
-
 template <typename TO>
-__global__ __launch_bounds__(64) void mas_invert_mfma_kernel(const double* __restrict__ hess, TO* inv, int64_t C)
+__global__ __launch_bounds__(64, 2) void mas_invert_mfma_kernel(const double* __restrict__ hess, TO* inv, int64_t C)
 {
     constexpr int D = 3 * kBank;
-    __shared__ double S[D * kInvLd];
+    __shared__ double Ms[kCm];     // the cluster's upper blocks in, its inverse's planes out
+    __shared__ double rowb[4][D];  // the panel's rows of S
+    __shared__ double wb[4][D];    // W
+    __shared__ double xb[4][D];    // L^-T W
     const int         lane = threadIdx.x;
-    const int64_t     k    = blockIdx.x;
-    const double*     M    = hess + k * kCm;
-    // the upper blocks into both triangles (hess: 9 column-major scalars per
-    // block); a diagonal block's lower half comes from its upper half.  All
-    // of a lane's loads are issued before the first is used.
-    constexpr int kLoads = (kCm + kWave - 1) / kWave;
-    double        hv[kLoads];
-#pragma unroll
-    for(int t = 0; t < kLoads; ++t)
-        hv[t] = lane + kWave * t < kCm ? M[lane + kWave * t] : 0.0;
-#pragma unroll
-    for(int t = 0; t < kLoads; ++t)
-    {
-        const int e = lane + kWave * t;
-        if(e >= kCm)
-            break;
-        const int blk = e / 9, q = e % 9;
-        int       a, b;
-        sym_rc(blk, a, b);
-        const int r = 3 * a + q % 3, cc = 3 * b + q / 3;
-        if(r > cc)
-            continue;
-        double v = hv[t];
-        if(r == cc && v == 0.0)
-            v = 1.0;
-        S[r * kInvLd + cc] = v;
-        S[cc * kInvLd + r] = v;
-    }
+    const int         g = lane >> 4, c = lane & 15;
+    const int64_t     k = blockIdx.x;
+    const double*     M = hess + k * kCm;
+    // the upper blocks (9 column-major scalars each), staged by coalesced
+    // loads, into both triangles
+    for(int e = lane; e < kCm; e += kWave)
+        Ms[e] = M[e];
     __syncthreads();
-    const mfma_f64x4 zero = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll 1
-    for(int kt = 0; kt < 3; ++kt)
+    mfma_f64x4 S[3][3];
+#pragma unroll
+    for(int I = 0; I < 3; ++I)
+#pragma unroll
+        for(int J = 0; J < 3; ++J)
+#pragma unroll
+            for(int v = 0; v < 4; ++v)
+            {
+                const int row = 16 * I + 4 * v + g, col = 16 * J + c;
+                const int a = row / 3, ra = row % 3, b = col / 3, cb = col % 3;
+                double val = a <= b ? Ms[9 * sym_index(a, b) + ra + 3 * cb] : Ms[9 * sym_index(b, a) + cb + 3 * ra];
+                if(row == col && val == 0.0)
+                    val = 1.0;
+                S[I][J][v] = val;
+            }
+#pragma unroll
+    for(int J = 0; J < 3; ++J)
+        rowb[g][16 * J + c] = S[0][J][0];
+    __syncthreads();
+#pragma unroll
+    for(int kp = 0; kp < D / 4; ++kp)
     {
-        double* T = S + 16 * kt * kInvLd + 16 * kt;  // pivot tile, row stride kInvLd
-        // Cholesky A_kk = L L^T in registers: lane j (mod 16) holds column j;
-        // column p of the trailing matrix reaches every lane by v_readlane
-        // (lane p, a compile-time index), so no step waits on LDS
-        const int j = lane & 15;
-        double    c[16];
+        const int k0 = 4 * kp, Kt = k0 / 16, ko = k0 % 16, vK = ko / 4;
+        // P = L L^T and L^-1 (lower triangular), on every lane: one rsqrt per
+        // pivot (rl = 1 / L_mm), no divisions
+        double L[4][4] = {}, Li[4][4] = {}, rl[4];
 #pragma unroll
-        for(int i = 0; i < 16; ++i)
-            c[i] = T[i * kInvLd + j];
-        double ild[16];  // 1 / L_pp, the same on every lane
-#pragma unroll
-        for(int p = 0; p < 16; ++p)
+        for(int m = 0; m < 4; ++m)
         {
-            const double il  = rsqrt(readlane_f64(c[p], p));
-            const double ljp = c[p] * il;  // L_jp (the trailing matrix is symmetric)
-            ild[p]           = il;
 #pragma unroll
-            for(int i = p + 1; i < 16; ++i)
+            for(int n = 0; n < m; ++n)
             {
-                const double lip = readlane_f64(c[i], p) * il;
-                if(j > p)
-                    c[i] = fma(-lip, ljp, c[i]);
+                double t = rowb[m][k0 + n];
+#pragma unroll
+                for(int q = 0; q < n; ++q)
+                    t -= L[m][q] * L[n][q];
+                L[m][n] = t * rl[n];
             }
-            if(j == p)
+            double t = rowb[m][k0 + m];
+#pragma unroll
+            for(int q = 0; q < m; ++q)
+                t -= L[m][q] * L[m][q];
+            rl[m]   = rsqrt(t);
+            L[m][m] = t * rl[m];
+        }
+#pragma unroll
+        for(int m = 0; m < 4; ++m)
+        {
+            Li[m][m] = rl[m];
+#pragma unroll
+            for(int n = 0; n < m; ++n)
             {
+                double t = 0.0;
 #pragma unroll
-                for(int i = p; i < 16; ++i)
-                    c[i] *= il;
+                for(int q = n; q < m; ++q)
+                    t += L[m][q] * Li[q][n];
+                Li[m][n] = -t * rl[m];
             }
-            __builtin_amdgcn_sched_barrier(0);
         }
-        // forward solves L Y = B, one column per lane: lanes 0-15 B = I (Y =
-        // L^{-1}), lanes 16-47 B = a column of A_kj (Y = W_j); L_mq = c[m] of lane q
-        const int i0 = kt == 0 ? 1 : 0, i1 = kt == 2 ? 1 : 2;  // the two other tiles
-        const int tj = lane < 16 ? kt : (lane < 32 ? i0 : i1);
-        double    y[16];
+        // the lane's row and column of L^-1 (g is a lane value: selects, not
+        // an indexed array, keep Li in registers)
+        double lrow[4], lcol[4];
 #pragma unroll
-        for(int m = 0; m < 16; ++m)
-            y[m] = lane < 16 ? (m == j ? 1.0 : 0.0) : S[(16 * kt + m) * kInvLd + 16 * tj + j];
-#pragma unroll
-        for(int q = 0; q < 16; ++q)
+        for(int m = 0; m < 4; ++m)
         {
-            y[q] *= ild[q];
-#pragma unroll
-            for(int m = q + 1; m < 16; ++m)
-                y[m] = fma(-readlane_f64(c[m], q), y[q], y[m]);
-            __builtin_amdgcn_sched_barrier(0);  // one column of L in scalar registers at a time
+            lrow[m] = g == 0 ? Li[0][m] : g == 1 ? Li[1][m] : g == 2 ? Li[2][m] : Li[3][m];
+            lcol[m] = g == 0 ? Li[m][0] : g == 1 ? Li[m][1] : g == 2 ? Li[m][2] : Li[m][3];
         }
-        __syncthreads();
-        if(lane < 48)
+        // W(g, 16T + c) = sum_m Li[g][m] S(k0 + m, 16T + c), zero in the K columns
+        double w[3];
+#pragma unroll
+        for(int T = 0; T < 3; ++T)
         {
+            double t = 0.0;
 #pragma unroll
-            for(int m = 0; m < 16; ++m)
-                S[(16 * kt + m) * kInvLd + 16 * tj + j] = y[m];
+            for(int m = 0; m < 4; ++m)
+                t += lrow[m] * rowb[m][16 * T + c];
+            w[T]             = (T == Kt && c >= ko && c < ko + 4) ? 0.0 : t;
+            wb[g][16 * T + c] = w[T];
         }
         __syncthreads();
-        // the row of tile kt now holds L^{-1} | W_i0 | W_i1
-        const mfma_f64x4 u00 = tile_mma<true>(S, i0, kt, i0, tile_read(S, i0, i0, lane), lane, -1.0);
-        const mfma_f64x4 u01 = tile_mma<true>(S, i0, kt, i1, tile_read(S, i0, i1, lane), lane, -1.0);
-        const mfma_f64x4 u11 = tile_mma<true>(S, i1, kt, i1, tile_read(S, i1, i1, lane), lane, -1.0);
-        const mfma_f64x4 w0  = tile_mma<true>(S, kt, kt, i0, zero, lane, 1.0);
-        const mfma_f64x4 w1  = tile_mma<true>(S, kt, kt, i1, zero, lane, 1.0);
-        const mfma_f64x4 dk  = tile_mma<true>(S, kt, kt, kt, zero, lane, -1.0);
+        // X(g, 16T + c) = sum_{q >= g} Li[q][g] W(q, 16T + c)
+        double x[3];
+#pragma unroll
+        for(int T = 0; T < 3; ++T)
+        {
+            double t = 0.0;
+#pragma unroll
+            for(int q = 0; q < 4; ++q)
+                t += lcol[q] * wb[q][16 * T + c];
+            x[T]             = t;
+            xb[g][16 * T + c] = t;
+        }
+        // S -= W^T W
+#pragma unroll
+        for(int I = 0; I < 3; ++I)
+#pragma unroll
+            for(int J = 0; J < 3; ++J)
+                S[I][J] = __builtin_amdgcn_mfma_f64_16x16x4f64(-w[I], w[J], S[I][J], 0, 0, 0);
         __syncthreads();
-        tile_write(S, i0, i0, u00, lane, false);
-        tile_write(S, i0, i1, u01, lane, true);
-        tile_write(S, i1, i1, u11, lane, false);
-        tile_write(S, kt, i0, w0, lane, true);
-        tile_write(S, kt, i1, w1, lane, true);
-        tile_write(S, kt, kt, dk, lane, false);
+        // the pivot rows (lane row k0 + g), then the pivot columns (lanes of
+        // columns k0 .. k0 + 3), then the pivot block
+#pragma unroll
+        for(int J = 0; J < 3; ++J)
+            S[Kt][J][vK] = x[J];
+        if(c >= ko && c < ko + 4)
+        {
+            const int m = c - ko;
+#pragma unroll
+            for(int I = 0; I < 3; ++I)
+#pragma unroll
+                for(int v = 0; v < 4; ++v)
+                    S[I][Kt][v] = xb[m][16 * I + 4 * v + g];
+            double t = 0.0;
+#pragma unroll
+            for(int q = 0; q < 4; ++q)
+                t += lcol[q] * (m == 0 ? Li[q][0] : m == 1 ? Li[q][1] : m == 2 ? Li[q][2] : Li[q][3]);
+            S[Kt][Kt][vK] = -t;
+        }
+        if(kp + 1 < D / 4)
+        {
+            const int k1 = k0 + 4, Kt1 = k1 / 16, vK1 = (k1 % 16) / 4;
+#pragma unroll
+            for(int J = 0; J < 3; ++J)
+                rowb[g][16 * J + c] = S[Kt1][J][vK1];
+        }
         __syncthreads();
+        __builtin_amdgcn_sched_barrier(0);  // no work moves between panels (register pressure)
     }
-    // -S, upper blocks out as 9 planes of 136 (inv_at); a diagonal block takes
-    // its upper triangle for both halves
+    // A^-1 = -S, upper blocks as 9 planes of 136 (inv_at) into LDS, then
+    // coalesced stores; a diagonal block takes its upper triangle for both
+    // halves
+#pragma unroll
+    for(int I = 0; I < 3; ++I)
+#pragma unroll
+        for(int J = 0; J < 3; ++J)
+#pragma unroll
+            for(int v = 0; v < 4; ++v)
+            {
+                const int    row = 16 * I + 4 * v + g, col = 16 * J + c;
+                const int    a = row / 3, ra = row % 3, b = col / 3, cb = col % 3;
+                const double val = -S[I][J][v];
+                if(a < b)
+                    Ms[inv_at(sym_index(a, b), ra + 3 * cb)] = val;
+                else if(a == b && ra <= cb)
+                {
+                    const int si = sym_index(a, a);
+                    Ms[inv_at(si, ra + 3 * cb)] = val;
+                    if(ra < cb)
+                        Ms[inv_at(si, cb + 3 * ra)] = val;
+                }
+            }
+    __syncthreads();
     TO* O = inv + k * kCm;
     for(int e = lane; e < kCm; e += kWave)
-    {
-        const int q = e / kSym, blk = e % kSym;
-        int       a, b;
-        sym_rc(blk, a, b);
-        int r = 3 * a + q % 3, cc = 3 * b + q / 3;
-        if(r > cc)
-        {
-            const int t = r;
-            r           = cc;
-            cc          = t;
-        }
-        O[e] = (TO)(-S[r * kInvLd + cc]);
-    }
+        O[e] = (TO)Ms[e];
 }
 
 __device__ __forceinline__ void inv3(const double m[9], double o[9])
@@ -1098,12 +1099,7 @@ __global__ __launch_bounds__(256) void mas_restrict_level_kernel(const double* _
 // With kLevel1 (every level-1 bank's parts lie in one 256-slot range, checked
 // at the build) the workgroup also holds the whole level-1 bank its slots
 // restrict to, and the bank's sibling sums give level 2's R as well.
-//
-// With kOrd the vectors are in the part order: the live slots of workgroup b
-// hold the consecutive vector rows [blk_first[b], blk_first[b + 1]), so the
-// x, r update runs one lane per scalar over that range (coalesced) and hands
-// the new r to the slot lanes through LDS.
-template <typename T, bool kLevel1, bool kOrd>
+template <typename T, bool kLevel1>
 __global__ __launch_bounds__(256) void mas_update_xr_restrict_kernel(PcgScalars*   sc,
                                                                     const double* pap_part,
                                                                     int           gp,
@@ -1120,15 +1116,13 @@ __global__ __launch_bounds__(256) void mas_update_xr_restrict_kernel(PcgScalars*
                                                                     const int32_t* __restrict__ unpart_vec,
                                                                     int32_t       n_unpart,
                                                                     int32_t       begin1,
-                                                                    int32_t       count1,
-                                                                    const int32_t* __restrict__ blk_first)
+                                                                    int32_t       count1)
 {
     if(sc->converged)
         return;
     __shared__ double lds[256 / kWave];
     __shared__ T      r1s[kBank][3];
     __shared__ int32_t bank1;
-    __shared__ double rs[kOrd ? 3 * 256 : 1];  // the workgroup's new r (kOrd)
     const double      alpha = pcg_alpha_of<256>(sc, pap_part, gp, lds);
     if((int32_t)blockIdx.x < slot_blocks)
     {
@@ -1144,28 +1138,7 @@ __global__ __launch_bounds__(256) void mas_update_xr_restrict_kernel(PcgScalars*
                 bank1 = -1;
             __syncthreads();
         }
-        if constexpr(kOrd)
-        {
-            const int64_t f0 = blk_first[blockIdx.x];
-            const int64_t ne = 3 * ((int64_t)blk_first[blockIdx.x + 1] - f0);
-            for(int64_t e = threadIdx.x; e < ne; e += 256)
-            {
-                const int64_t i  = 3 * f0 + e;
-                const double  rn = r[i] - alpha * Ap[i];
-                x[i] += alpha * p[i];
-                r[i]  = rn;
-                rs[e] = rn;
-            }
-            __syncthreads();
-            if(v >= 0)
-            {
-                const int64_t q = 3 * ((int64_t)v - f0);
-                v0              = (T)rs[q];
-                v1              = (T)rs[q + 1];
-                v2              = (T)rs[q + 2];
-            }
-        }
-        else if(v >= 0)
+        if(v >= 0)
         {
             const int64_t i  = 3 * (int64_t)v;
             const double  r0 = r[i] - alpha * Ap[i], r1 = r[i + 1] - alpha * Ap[i + 1],
@@ -1523,14 +1496,13 @@ class MasPreconditioner final : public LocalPreconditioner
 {
   public:
     Hierarchy h;
-    int       fp32 = 0, active_levels = 0, invert = MI_MAS_INVERT_SWEEP;
+    int       fp32 = 0, active_levels = 0, invert = MI_MAS_INVERT_MFMA;
     int64_t   C = 0, C0 = 0;  // clusters, fine clusters
     // device tables
     DBuf<int32_t> d_part_to_real, d_real_to_part, d_anc, d_anc_slot, d_parent, d_sib, d_unpart;
     // the vector index of every slot and unpartitioned vertex when the
     // vectors are in the SpMV's row order (the part order: rank_)
     DBuf<int32_t> d_part_to_pos, d_unpart_pos;
-    DBuf<int32_t> d_blk_first;  // per 256-slot block: the part-order rank of its first live slot (+ the total)
     bool          row_order_ = false;
     DBuf<int32_t> d_desc_ptr, d_desc_idx, d_desc_lpn;
     int32_t       n_unpart  = 0;
@@ -1654,16 +1626,6 @@ class MasPreconditioner final : public LocalPreconditioner
                 unpart_pos[u] = rank_[unpart[u]];
             upload(d_part_to_pos, part_pos);
             upload(d_unpart_pos, unpart_pos);
-            std::vector<int32_t> blk_first((size_t)(h.N + 255) / 256 + 1, 0);
-            int32_t              live = 0;
-            for(int32_t slot = 0; slot < h.N; ++slot)
-            {
-                if(slot % 256 == 0)
-                    blk_first[slot / 256] = live;
-                live += h.part_to_real[slot] >= 0 ? 1 : 0;
-            }
-            blk_first.back() = live;
-            upload(d_blk_first, blk_first);
         }
         // base level: the first level at most kTailLevel nodes (restriction
         // kernels stop there); level-b descendants of every node above it
@@ -1975,27 +1937,17 @@ class MasPreconditioner final : public LocalPreconditioner
             return e ? std::atoi(e) : 1;
         }();
         const bool lvl1 = l1 && l1_local_ && base_lvl >= 2 && Lact > 2;
-        // MI_MAS_XR_ORD=0 keeps the per-slot update with vectors in the part order
-        static const int ord_env = [] {
-            const char* e = std::getenv("MI_MAS_XR_ORD");
-            return e ? std::atoi(e) : 1;
-        }();
-        const bool ord = row_order_ && ord_env;
-        auto       go  = [&](auto* R, auto tag) {
+        auto go = [&](auto* R, auto tag) {
             using T = decltype(tag);
             auto launch = [&](auto kernel) {
                 kernel<<<blocks, 256, 0, s>>>(sc, pap_part, gp, x, p, r, Ap, slot_vec(), R, d_parent.data(),
                                               d_sib.data(), span.count[0], slot_blocks, unpart_vec(), n_unpart,
-                                              span.begin[1], span.count[1], d_blk_first.data());
+                                              span.begin[1], span.count[1]);
             };
-            if(lvl1 && ord)
-                launch(mas_update_xr_restrict_kernel<T, true, true>);
-            else if(lvl1)
-                launch(mas_update_xr_restrict_kernel<T, true, false>);
-            else if(ord)
-                launch(mas_update_xr_restrict_kernel<T, false, true>);
+            if(lvl1)
+                launch(mas_update_xr_restrict_kernel<T, true>);
             else
-                launch(mas_update_xr_restrict_kernel<T, false, false>);
+                launch(mas_update_xr_restrict_kernel<T, false>);
         };
         if(fp32)
             go(d_R32.data(), float{});

@@ -632,17 +632,21 @@ constexpr WinShape kWinShapes[] = {MI_WIN_SHAPES(MI_WIN_ENTRY)};
 #undef MI_WIN_ENTRY
 constexpr int kNumWinShapes = (int)(sizeof(kWinShapes) / sizeof(kWinShapes[0]));
 
-// MI_SPMV_WIN overrides the default shape (for sweeps)
-int default_shape_index()
+// MI_SPMV_WIN overrides the default shape (for sweeps); -1 when unset
+int env_shape_index()
 {
-    static int idx = -1;
-    if(idx < 0)
+    static int idx = -2;
+    if(idx == -2)
     {
-        idx = 0;
+        idx = -1;
         if(const char* e = std::getenv("MI_SPMV_WIN"))
             idx = std::max(0, std::min(kNumWinShapes - 1, std::atoi(e)));
     }
     return idx;
+}
+int default_shape_index()
+{
+    return env_shape_index() >= 0 ? env_shape_index() : 0;
 }
 
 template <typename VT, bool ACCUM>
@@ -772,6 +776,23 @@ int spmv_shape_count()
 int spmv_default_shape()
 {
     return default_shape_index();
+}
+
+// the layout of a system of B block rows when none is set: the largest
+// window (512, 256 or 128 rows) whose count fills the chip, at least 1.5
+// windows per CU (one workgroup streams a window, so fewer windows leave
+// CUs idle: 8 windows of 512 rows ran a 4,096-row system on 8 CUs), else the
+// full BSR (spmv_shape_full)
+int spmv_auto_shape(int64_t B)
+{
+    if(env_shape_index() >= 0)
+        return env_shape_index();
+    const int64_t need   = (3 * (int64_t)device_cu_count() + 1) / 2;
+    const int     cand[] = {default_shape_index(), 3, 7};
+    for(int k : cand)
+        if((B + (int64_t(1) << kWinShapes[k].shift) - 1) >> kWinShapes[k].shift >= need)
+            return k;
+    return spmv_shape_full;
 }
 
 bool spmv_shape_info(int shape, int* threads, int* rows, int* per, int* vec)

@@ -274,7 +274,7 @@ class LinearSystem:
     def add_preconditioner(self, kind: int, dof_offset: int, dof_count: int, *, part_ids=None, tets=None,
                            tris=None, edges=None, mas_precision: int = abi.MI_FP64, active_levels: int = 0,
                            build: int = abi.MI_MAS_BUILD_HOST, device_arrays: dict | None = None,
-                           reinit: int | None = None, invert: int = abi.MI_MAS_INVERT_SWEEP) -> int:
+                           reinit: int | None = None, invert: int = abi.MI_MAS_INVERT_MFMA) -> int:
         """`device_arrays` (MI_MAS_BUILD_DEVICE only) maps "part_ids", "tets",
         "tris", "edges" to device pointers read instead of the host arrays;
         their counts still come from the host arrays' shapes.  `reinit` =

@@ -71,41 +71,46 @@ def tune_mas_levels_batched(ls: LinearSystem, handle: int, seg_rows, b, x, level
 
 
 def tune_spmv_shape(ls: LinearSystem, x, reps: int = 20, shapes=None, margin: float = 0.02) -> dict:
-    """Times `reps` products per SpMV launch shape (each one's window layout
-    rebuilt for it, one untimed product first; the PCG's own product,
-    mi_spmv_ordered, on a window layout, else mi_spmv_dot) and keeps the
-    fastest, as the reference fits its SpMV grid to each matrix
-    (Spmv::fit_block_count, linear_system/spmv.cu:909-933).  The library's
-    default shape stays unless another is faster by more than `margin`
-    (timing noise would otherwise flip between near-equal shapes).  Returns
-    {"best": k, "table": {k: {"us", "threads", "rows", "per", "vec"}}}."""
+    """Times `reps` products per SpMV layout -- the full BSR
+    (MI_SPMV_SHAPE_FULL) and every window launch shape, each layout built for
+    it, one untimed product first; the PCG's own product (mi_spmv_ordered on
+    a window layout, y = A x on the full BSR) -- and keeps the fastest, as
+    the reference fits its SpMV grid to each matrix (Spmv::fit_block_count,
+    linear_system/spmv.cu:909-933).  The system's layout stays unless another
+    is faster by more than `margin` (timing noise would otherwise flip
+    between near-equal layouts).  Returns {"best": k, "table": {k: {"us",
+    "threads", "rows", "per", "vec"}}}."""
     import numpy as np
 
     from .device import spmv_shapes
     from .hip_events import EventTimer
 
-    allinfo = spmv_shapes(ls.lib)
-    info, dflt = allinfo["shapes"], allinfo["default"]
-    cand = list(range(len(info))) if shapes is None else [int(k) for k in shapes]
+    from .abi import MI_SPMV_SHAPE_FULL
+
+    info = spmv_shapes(ls.lib)["shapes"]
+    lay0 = ls.spmv_layout()
+    dflt = lay0["shape"] if lay0["layout"] == "window" else MI_SPMV_SHAPE_FULL  # the system's own layout
+    cand = [MI_SPMV_SHAPE_FULL] + list(range(len(info))) if shapes is None else [int(k) for k in shapes]
     ctx = ls.ctx
     stream = ctx.stream()
     y = ctx.alloc(ls.n, np.float64)
-    d = ctx.alloc(1, np.float64)
     ev = EventTimer(stream)
     table = {}
     for k in cand:
         ls.set_spmv_shape(k, stream)
-        if ls.spmv_layout()["layout"] == "window":
+        if k >= 0:
             def product():
                 ctx.lib.mi_spmv_ordered(ls.h, x.ptr, y.ptr, None, stream)
+            desc = info[k]
         else:
             def product():
-                ctx.lib.mi_spmv_dot(ls.h, x.ptr, y.ptr, d.ptr, stream)
+                ctx.lib.mi_spmv(ls.h, 1.0, x.ptr, 0.0, y.ptr, stream)
+            desc = {"threads": 256, "rows": 0, "per": 0, "vec": 0, "layout": "full"}
         product()
         ev.start()
         for _ in range(max(1, reps)):
             product()
-        table[k] = {"us": round(1e3 * ev.stop() / max(1, reps), 2), **info[k]}
+        table[k] = {"us": round(1e3 * ev.stop() / max(1, reps), 2), **desc}
     best = min(table, key=lambda k: table[k]["us"])
     if dflt in table and table[best]["us"] > (1.0 - margin) * table[dflt]["us"]:
         best = dflt

@@ -44,7 +44,10 @@ def c2_gpu(ctx, c2):
     ls = LinearSystem(ctx, s.block_rows)
     rows, cols, vals = ctx.to_device(s.rows), ctx.to_device(s.cols), ctx.to_device(s.vals)
     ls.assemble(rows, cols, vals)
-    h = ls.add_preconditioner(abi.MI_PRECOND_MAS, 0, s.n, part_ids=s.part_ids, tets=s.tets)
+    # the reference's column sweep for the cluster inverses, as the oracle
+    # (the default FP64-MFMA panel sweep: test_c2_mas_apply_mfma)
+    h = ls.add_preconditioner(abi.MI_PRECOND_MAS, 0, s.n, part_ids=s.part_ids, tets=s.tets,
+                              invert=abi.MI_MAS_INVERT_SWEEP)
     ls.assemble_preconditioners()
     ls._keep = (rows, cols, vals)
     yield ls, h
@@ -70,6 +73,9 @@ def test_c2_spmv(ctx, c2, c2_gpu):
     y = yd.numpy()
     assert np.abs(y - yref).max() <= 1e-13 * np.abs(yref).max()
     assert d == pytest.approx(dref, rel=1e-12)
+    # 262,144 rows: 512-row windows by size (512 windows, two per CU)
+    lay = ls.spmv_layout()
+    assert lay["layout"] == "window" and lay["window_rows"] == 512
 
 
 def test_c2_mas_tables_and_apply(ctx, c2, c2_mas, c2_gpu):
@@ -90,6 +96,28 @@ def test_c2_mas_tables_and_apply(ctx, c2, c2_mas, c2_gpu):
     print(f"\n[c2] MAS apply FP64 |z - z_oracle| / |z| = {err:.2e}", flush=True)
     assert err < 1e-9
     assert rv @ z > 0
+
+
+def test_c2_mas_apply_mfma(ctx, c2, c2_mas):
+    """the default cluster inverse (4-row pivot panels on FP64 matrix cores,
+    the sweep in its Cholesky form) on the 64^3 + 2M-contact system: the MAS
+    apply against the oracle's (whose inverse is the column sweep), to the
+    rounding the two inverses differ by"""
+    s, _ = c2
+    ls = LinearSystem(ctx, s.block_rows)
+    rows, cols, vals = ctx.to_device(s.rows), ctx.to_device(s.cols), ctx.to_device(s.vals)
+    ls.assemble(rows, cols, vals)
+    del rows, cols, vals
+    ls.add_preconditioner(abi.MI_PRECOND_MAS, 0, s.n, part_ids=s.part_ids, tets=s.tets)
+    ls.assemble_preconditioners()
+    rv = np.random.default_rng(24).standard_normal(s.n)
+    zd = ctx.alloc(s.n, np.float64)
+    ls.apply_preconditioner(ctx.to_device(rv), zd)
+    z = zd.numpy()
+    err = rel_err(z, c2_mas.apply(rv))
+    print(f"\n[c2] MAS apply, MFMA inverse: |z - z_oracle| / |z| = {err:.2e}", flush=True)
+    assert err < 1e-7 and rv @ z > 0
+    ls.close()
 
 
 def test_c2_mas_apply_fp32(ctx, c2, c2_mas):

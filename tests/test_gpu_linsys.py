@@ -188,6 +188,7 @@ def test_spmv_in_part_order(ctx, c1_contact, precision, unpart):
     if unpart:
         parts[::7] = -1
     ls = assembled(ctx, s, precision)
+    ls.set_spmv_shape(0)  # 4,096 rows take the full BSR by size
     ls.add_preconditioner(abi.MI_PRECOND_MAS, 0, s.n, part_ids=parts, tets=s.tets)
     ls.assemble_preconditioners()
     orr, oc, ov, _ = O.convert_sym(s.block_rows, s.rows, s.cols, s.vals)
@@ -218,6 +219,7 @@ def test_spmv_every_launch_shape(ctx, c1_contact, precision):
     ls = assembled(ctx, s, precision)
     ls.add_preconditioner(abi.MI_PRECOND_MAS, 0, s.n, part_ids=s.part_ids, tets=s.tets)
     ls.assemble_preconditioners()
+    assert ls.spmv_layout()["layout"] == "full"  # 4,096 rows: the full BSR by size
     orr, oc, ov, _ = O.convert_sym(s.block_rows, s.rows, s.cols, s.vals)
     x = np.random.default_rng(8).standard_normal(s.n)
     yref, dref = O.spmv_sym(s.block_rows, orr, oc, ov, x)
@@ -233,8 +235,15 @@ def test_spmv_every_launch_shape(ctx, c1_contact, precision):
         d = ls.spmv_dot(xd, yd)
         assert np.abs(yd.numpy() - yref).max() <= tol * np.abs(yref).max(), k
         assert d == pytest.approx(dref, rel=tol * 10), k
+        # the PCG's product, on vectors in the row order
+        pos = ls.spmv_order()
+        xo = np.empty((s.block_rows, 3))
+        xo[pos] = x.reshape(-1, 3)
+        ls.spmv_ordered(ctx.to_device(xo.ravel()), yd)
+        yo = yd.numpy().reshape(-1, 3)[pos].ravel()
+        assert np.abs(yo - yref).max() <= tol * np.abs(yref).max(), k
     t = tune_spmv_shape(ls, xd, reps=3)
-    assert sorted(t["table"]) == list(range(len(info["shapes"])))
+    assert sorted(t["table"]) == [abi.MI_SPMV_SHAPE_FULL] + list(range(len(info["shapes"])))
     assert ls.spmv_layout()["window_rows"] == t["table"][t["best"]]["rows"]
     with pytest.raises(abi.MiError):
         ls.set_spmv_shape(len(info["shapes"]))
@@ -242,26 +251,35 @@ def test_spmv_every_launch_shape(ctx, c1_contact, precision):
 
 
 def test_pcg_after_a_shape_change(ctx, c1_contact):
-    """a solve captured in a HIP graph with one SpMV launch shape is not
-    replayed after the shape (and with it the window size) changes: the next
-    solve re-captures and gives the same result"""
+    """a solve captured in a HIP graph with one SpMV layout is not replayed
+    after the layout changes (another window size, the full BSR, the size's
+    default): the next solve re-captures and gives the same result, after a
+    reassembly and without one"""
     from libuipc_amd.device import spmv_shapes
 
     s = c1_contact
     ls = assembled(ctx, s)
+    assert ls.spmv_layout()["layout"] == "full"  # 4,096 rows: the full BSR by size
+    ls.set_spmv_shape(0)
     ls.add_preconditioner(abi.MI_PRECOND_MAS, 0, s.n, part_ids=s.part_ids, tets=s.tets)
     ls.assemble_preconditioners()
     shapes = spmv_shapes()["shapes"]
     k0 = ls.spmv_layout()["shape"]
+    assert k0 == 0
     k1 = next(k for k, sh in enumerate(shapes) if sh["rows"] != shapes[k0]["rows"])
     b = ctx.to_device(s.rhs)
     kw = dict(criterion=abi.MI_CRIT_REL_RESIDUAL, tol=1e-6, true_residual=1, use_graph=1)
     x0 = ctx.alloc(s.n, np.float64)
     st0 = ls.solve(b, x0, **kw)
-    for k in (k1, k0, k1):
+    for k, reassemble in ((k1, True), (abi.MI_SPMV_SHAPE_FULL, False), (k0, True), (abi.MI_SPMV_SHAPE_AUTO, False),
+                          (k1, False)):
         ls.set_spmv_shape(k)
-        ls.assemble(*ls._trip)
-        ls.assemble_preconditioners()
+        want = "window" if k >= 0 else "full"
+        assert ls.spmv_layout()["layout"] == want, k
+        if reassemble:
+            ls.assemble(*ls._trip)
+            ls.assemble_preconditioners()
+        assert ls.spmv_layout()["layout"] == want, k
         x = ctx.alloc(s.n, np.float64)
         st = ls.solve(b, x, **kw)
         assert st["converged"] == 1 and st["rel_res_true"] <= 2e-6, (k, st)
@@ -283,6 +301,7 @@ def test_spmv_ordered(ctx, c1_contact, precision, unpart):
     if unpart:
         parts[::7] = -1
     ls = assembled(ctx, s, precision)
+    ls.set_spmv_shape(0)  # 4,096 rows take the full BSR by size
     assert np.array_equal(ls.spmv_order(), np.arange(s.block_rows))  # no MAS: the identity
     ls.add_preconditioner(abi.MI_PRECOND_MAS, 0, s.n, part_ids=parts, tets=s.tets)
     ls.assemble_preconditioners()
@@ -315,6 +334,7 @@ def test_pcg_row_order_and_caller_order(ctx, c1_contact, precision):
     from a given x0"""
     s = c1_contact
     ls = assembled(ctx, s, precision)
+    ls.set_spmv_shape(0)  # 4,096 rows take the full BSR by size
     ls.add_preconditioner(abi.MI_PRECOND_MAS, 0, s.n, part_ids=s.part_ids, tets=s.tets)
     ls.assemble_preconditioners()
     b = ctx.to_device(s.rhs)

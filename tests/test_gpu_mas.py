@@ -12,8 +12,12 @@ from tests.test_oracle_mas import system
 pytestmark = pytest.mark.gpu
 
 
-def gpu_mas(ctx, s, precision=abi.MI_FP64, active_levels=0, invert=abi.MI_MAS_INVERT_SWEEP):
+def gpu_mas(ctx, s, precision=abi.MI_FP64, active_levels=0, invert=abi.MI_MAS_INVERT_SWEEP, shape=None):
+    """shape: the SpMV layout (None: by size, the full BSR below ~200k rows;
+    0: 512-row windows in the MAS part order, the PCG then iterating in it)"""
     ls = LinearSystem(ctx, s.block_rows)
+    if shape is not None:
+        ls.set_spmv_shape(shape)
     rows, cols, vals = ctx.to_device(s.rows), ctx.to_device(s.cols), ctx.to_device(s.vals)
     ls.assemble(rows, cols, vals)
     ls._keep = (rows, cols, vals)
@@ -117,11 +121,13 @@ def test_mas_one_level_block_jacobi(ctx):
 
 @pytest.mark.parametrize("levels", [0, 1])
 @pytest.mark.parametrize("precision", [abi.MI_FP64, abi.MI_FP32])
-def test_mas_pcg(ctx, precision, levels):
+@pytest.mark.parametrize("shape", [None, 0], ids=["auto", "window"])
+def test_mas_pcg(ctx, precision, levels, shape):
     """MAS-PCG against the oracle's, all levels and one level; FP32 runs the
-    fine apply fused with the r.z partials"""
+    fine apply fused with the r.z partials; on the size's layout (the full
+    BSR here) and on the window layout, whose PCG iterates in the part order"""
     s, (r, c, v) = system(16, 20000, 14)
-    ls, h = gpu_mas(ctx, s, precision, active_levels=levels)
+    ls, h = gpu_mas(ctx, s, precision, active_levels=levels, shape=shape)
     x = ctx.alloc(s.n, np.float64)
     st = ls.solve(ctx.to_device(s.rhs), x, criterion=abi.MI_CRIT_REL_RESIDUAL, tol=1e-4, true_residual=1)
     m = O.Mas(s.block_rows, s.part_ids, s.tets)
@@ -147,7 +153,7 @@ def test_mas_pcg_iterate_after_a_few_iterations(ctx, grid, partition):
     restriction runs; random parts break the alignment it needs and fall back
     to the separate kernel; un-partitioned vertices take the 3x3 fallback."""
     s, (r, c, v) = system(grid, 20 * grid ** 2, 31, partition)
-    ls, h = gpu_mas(ctx, s)
+    ls, h = gpu_mas(ctx, s, shape=0)
     ratio = 12.5 / s.n  # max_iter = 12
     x = ctx.alloc(s.n, np.float64)
     st = ls.solve(ctx.to_device(s.rhs), x, criterion=abi.MI_CRIT_REL_RESIDUAL, tol=1e-14, max_iter_ratio=ratio)
@@ -167,7 +173,7 @@ def test_mas_pcg_fp32_one_level(ctx, partition):
     disconnected parts and with un-partitioned vertices (the 3x3 fallback):
     the solve matches the oracle's and the returned x solves the system"""
     s, (r, c, v) = system(12, 3000, 16, partition)
-    ls, h = gpu_mas(ctx, s, abi.MI_FP32, active_levels=1)
+    ls, h = gpu_mas(ctx, s, abi.MI_FP32, active_levels=1, shape=0)
     x = ctx.alloc(s.n, np.float64)
     st = ls.solve(ctx.to_device(s.rhs), x, criterion=abi.MI_CRIT_REL_RESIDUAL, tol=1e-4, true_residual=1)
     m = O.Mas(s.block_rows, s.part_ids, s.tets)
