@@ -1346,10 +1346,15 @@ void run_iteration(mi_sys* sys, double* x, bool bypass, const LocalPreconditione
     else
     {
         const bool whole = sys->pcs.size() == 1 && sys->pcs[0]->dof_offset == 0 && sys->pcs[0]->dof_count == n;
+        // a preconditioner whose apply adds the r.z partials into dot slots
+        // gets them cleared by the x, r update (its apply_dot refuses a
+        // buffer too small for them)
+        const int nz   = (whole && !bypass) ? sys->pcs[0]->dot_slots() : 0;
+        double*   zero = (nz > 0 && nz <= 2 * sys->dot_scr.grid) ? partB : nullptr;
         // one preconditioner over the whole vector may fold its first pass into the x, r update
         if(bypass || !whole
-           || !sys->pcs[0]->update_xr(sc, partA, gA, x, sys->p.data(), sys->r.data(), sys->Ap.data(), s))
-            pcg_update_xr(sc, partA, gA, x, sys->p.data(), sys->r.data(), sys->Ap.data(), n, s);
+           || !sys->pcs[0]->update_xr(sc, partA, gA, x, sys->p.data(), sys->r.data(), sys->Ap.data(), s, zero, nz))
+            pcg_update_xr(sc, partA, gA, x, sys->p.data(), sys->r.data(), sys->Ap.data(), n, s, zero, nz);
         gB = -1;
         if(bypass)
             pcg_copy_if_active(sc, sys->z.data(), sys->r.data(), n, s);

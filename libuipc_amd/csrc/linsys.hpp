@@ -413,10 +413,15 @@ class LocalPreconditioner
     // fold its own first pass over the new r into it; returns false when it
     // does not (the caller then runs pcg_update_xr)
     virtual bool update_xr(PcgScalars*, const double*, int, double*, const double*, double*, const double*,
-                           hipStream_t)
+                           hipStream_t, double* /*zero*/ = nullptr, int /*nzero*/ = 0)
     {
         return false;
     }
+    // a preconditioner covering the whole vector whose apply_dot adds its
+    // r.z and r.r into dot slots with atomics: how many doubles (pairs x 2)
+    // must be zero when it starts (the caller clears them in the x, r
+    // update); 0 when apply_dot writes its partials without atomics
+    virtual int dot_slots() const { return 0; }
     // MAS only: levels used by apply (MASPreconditionerEngine::set_active_level_num)
     virtual bool set_active_levels(int) { return false; }
     // MAS only, batched solves: a level count per system (block-row ranges)
@@ -561,9 +566,11 @@ int pcg_dot_partials(const double* r, const double* z, int64_t n, double* part, 
 // b^T b of a solve started from a given x0 (the relative-residual reference)
 void pcg_set_bb(PcgScalars* sc, double bb, double tol, hipStream_t s);
 void pcg_init_scalars(PcgScalars* sc, const double* part, int g, int criterion, double tol, hipStream_t s);
-// x += alpha p, r -= alpha Ap with alpha = rz / sum(pAp partials)
+// x += alpha p, r -= alpha Ap with alpha = rz / sum(pAp partials); zero[0,
+// nzero) is cleared on the way (the preconditioner's dot slots, see
+// LocalPreconditioner::dot_slots)
 void pcg_update_xr(PcgScalars* sc, const double* pap_part, int gp, double* x, const double* p, double* r,
-                   const double* Ap, int64_t n, hipStream_t s);
+                   const double* Ap, int64_t n, hipStream_t s, double* zero = nullptr, int nzero = 0);
 // the same, fused with z = D^-1 r on [blk_off, blk_off + blk_cnt) (z = r elsewhere)
 // and the (r.z, r.r) partials -> part; returns the grid
 int pcg_update_xr_diag3(PcgScalars* sc, const double* pap_part, int gp, double* x, const double* p, double* r,

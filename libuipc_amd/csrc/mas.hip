@@ -1116,10 +1116,15 @@ __global__ __launch_bounds__(256) void mas_update_xr_restrict_kernel(PcgScalars*
                                                                     const int32_t* __restrict__ unpart_vec,
                                                                     int32_t       n_unpart,
                                                                     int32_t       begin1,
-                                                                    int32_t       count1)
+                                                                    int32_t       count1,
+                                                                    double*       zero,
+                                                                    int           nzero)
 {
     if(sc->converged)
         return;
+    if(blockIdx.x == 0)
+        for(int k = threadIdx.x; k < nzero; k += 256)
+            zero[k] = 0.0;
     __shared__ double lds[256 / kWave];
     __shared__ T      r1s[kBank][3];
     __shared__ int32_t bank1;
@@ -1284,10 +1289,31 @@ __global__ __launch_bounds__(64) void mas_apply_coarse_kernel(const T* __restric
         Zb[3 * k * kBank + lane] = cluster_rowsum<T>(sC, lane);
 }
 
+// the PCG's r.z and r.r of one wave (each summed over the wave in a fixed
+// order) added into dot slot blk % kDotSlots (pairs) with two atomics: the
+// slots' sums are in arrival order, as the SpMV's in-window sums are
+constexpr int kDotSlots = 256;
+__device__ __forceinline__ void wave_dot_to_slots(double s0, double s1, double* slots, int64_t blk)
+{
+#pragma unroll
+    for(int o = kWave / 2; o > 0; o >>= 1)
+    {
+        s0 += __shfl_xor(s0, o, kWave);
+        s1 += __shfl_xor(s1, o, kWave);
+    }
+    if((threadIdx.x % kWave) == 0)
+    {
+        double* d = slots + 2 * (blk % kDotSlots);
+        atomicAdd(d, s0);
+        atomicAdd(d + 1, s1);
+    }
+}
+
 // one wave per fine cluster: gather r, local solve, and the prolongation of
 // every level (injection): z_i = Z0_i + sum_{l >= 1} Z_l[ancestor_l(i)].
 // Blocks past the fine clusters take the un-partitioned vertices (3x3 inverse
-// diagonal block, fem_mas_preconditioner.cu:24-45).
+// diagonal block, fem_mas_preconditioner.cu:24-45).  With dot_slots the
+// wave also adds its r.z and r.r into them (wave_dot_to_slots).
 template <typename T>
 __global__ __launch_bounds__(64) void mas_apply_fine_kernel(const double* __restrict__ r,
                                                            double*        z,
@@ -1304,7 +1330,8 @@ __global__ __launch_bounds__(64) void mas_apply_fine_kernel(const double* __rest
                                                            const double* __restrict__ Dun,
                                                            const int32_t* conv,
                                                            const int32_t* __restrict__ on,
-                                                           const int32_t* __restrict__ lvl_of)
+                                                           const int32_t* __restrict__ lvl_of,
+                                                           double*        dot_slots)
 {
     if(conv && *conv != 0)
         return;
@@ -1315,14 +1342,22 @@ __global__ __launch_bounds__(64) void mas_apply_fine_kernel(const double* __rest
     if((int64_t)blockIdx.x >= C0)
     {
         const int64_t u = ((int64_t)blockIdx.x - C0) * kWave + lane;
+        double s0 = 0.0, s1 = 0.0;
         if(u < n_unpart)
         {
             const double* D  = Dun + 9 * (int64_t)unpart[u];
             const int64_t q  = 3 * (int64_t)unpart_vec[u];
             const double  r0 = r[q], r1 = r[q + 1], r2 = r[q + 2];
             for(int a = 0; a < 3; ++a)
-                z[q + a] = D[a] * r0 + D[a + 3] * r1 + D[a + 6] * r2;
+            {
+                const double za = D[a] * r0 + D[a + 3] * r1 + D[a + 6] * r2;
+                z[q + a]        = za;
+                s0 += (a == 0 ? r0 : a == 1 ? r1 : r2) * za;
+            }
+            s1 = r0 * r0 + r1 * r1 + r2 * r2;
         }
+        if(dot_slots)
+            wave_dot_to_slots(s0, s1, dot_slots, blockIdx.x);
         return;
     }
     __shared__ T  sR[kBank][4];
@@ -1347,10 +1382,12 @@ __global__ __launch_bounds__(64) void mas_apply_fine_kernel(const double* __rest
     }
     ClusterBlocks<T> cb;
     cluster_load<T>(inv + k * kCm, lane, cb);
-    T zl[kMaxLvl - 1];
+    T      zl[kMaxLvl - 1];
+    double rv = 0.0;
     if(own)
     {
-        sR[i][c] = v >= 0 ? (T)r[3 * (int64_t)v + c] : T(0);
+        rv       = v >= 0 ? r[3 * (int64_t)v + c] : 0.0;
+        sR[i][c] = (T)rv;
 #pragma unroll
         for(int l = 1; l < kMaxLvl; ++l)
             zl[l - 1] = (v >= 0 && l < La) ? Zb[3 * (int64_t)anc[l - 1] + c] : T(0);
@@ -1358,6 +1395,7 @@ __global__ __launch_bounds__(64) void mas_apply_fine_kernel(const double* __rest
     __syncthreads();
     cluster_contrib<T>(cb, lane, sR, sC);
     __syncthreads();
+    double s0 = 0.0;
     if(own && v >= 0)
     {
         T acc = cluster_rowsum<T>(sC, lane);
@@ -1365,8 +1403,12 @@ __global__ __launch_bounds__(64) void mas_apply_fine_kernel(const double* __rest
         for(int l = 1; l < kMaxLvl; ++l)
             if(l < La)
                 acc += zl[l - 1];
-        z[3 * (int64_t)v + c] = (double)acc;
+        const double zv       = (double)acc;
+        z[3 * (int64_t)v + c] = zv;
+        s0                    = rv * zv;
     }
+    if(dot_slots)
+        wave_dot_to_slots(s0, rv * rv, dot_slots, blockIdx.x);
 }
 
 // The fine apply of a MAS preconditioner that covers the whole vector, fused
@@ -1762,9 +1804,12 @@ class MasPreconditioner final : public LocalPreconditioner
 
     // the fine kernel writes the PCG's dot partials when `part` is given and
     // returns their count (at most max_parts)
+    // part: the PCG's r.z and r.r partials, written by the persistent fused
+    // kernel (at most max_parts pairs, their count returned), or, with
+    // slots, added into kDotSlots zeroed pairs with atomics by the plain one
     template <typename T>
     int apply_t(const double* r, double* z, const int32_t* conv, hipStream_t s, const T* inv, T* R, T* Z,
-                double* part = nullptr, int max_parts = 0)
+                double* part = nullptr, int max_parts = 0, bool slots = false)
     {
         const int L    = h.L;
         const int Lact = levels_run();
@@ -1789,7 +1834,7 @@ class MasPreconditioner final : public LocalPreconditioner
             mas_apply_coarse_kernel<T><<<(unsigned)(C - c_first), 64, 0, s>>>(
                 inv, R, Z, c_first, c_desc, d_desc_ptr.data(), d_desc_idx.data(), d_desc_lpn.data(), conv);
         const int64_t blocks = C0 + ceil_div(n_unpart, kWave);
-        if(part)
+        if(part && !slots)
         {
             const int g = (int)std::max<int64_t>(1, std::min<int64_t>(max_parts, ceil_div(blocks, (int64_t)4)));
             mas_apply_fine_dot_kernel<T><<<g, 256, 0, s>>>(r, z, pv, inv, Z, d_anc_slot.data(), h.N, Lact, C0,
@@ -1800,8 +1845,9 @@ class MasPreconditioner final : public LocalPreconditioner
         if(blocks > 0)
             mas_apply_fine_kernel<T><<<(unsigned)blocks, 64, 0, s>>>(r, z, pv, inv, Z, d_anc_slot.data(), h.N,
                                                                      Lact, C0, d_unpart.data(), uv, n_unpart,
-                                                                     d_Dun.data(), conv, fine_on, fine_lvl);
-        return 0;
+                                                                     d_Dun.data(), conv, fine_on, fine_lvl,
+                                                                     slots ? part : nullptr);
+        return slots ? kDotSlots : 0;
     }
 
     bool mas_export(int32_t* info,
@@ -1919,7 +1965,7 @@ class MasPreconditioner final : public LocalPreconditioner
     // the whole vector (the caller checks the range); MI_MAS_FUSE_XR=0 keeps
     // the separate kernels
     bool update_xr(PcgScalars* sc, const double* pap_part, int gp, double* x, const double* p, double* r,
-                   const double* Ap, hipStream_t s) override
+                   const double* Ap, hipStream_t s, double* zero, int nzero) override
     {
         static const int on = [] {
             const char* e = std::getenv("MI_MAS_FUSE_XR");
@@ -1942,7 +1988,7 @@ class MasPreconditioner final : public LocalPreconditioner
             auto launch = [&](auto kernel) {
                 kernel<<<blocks, 256, 0, s>>>(sc, pap_part, gp, x, p, r, Ap, slot_vec(), R, d_parent.data(),
                                               d_sib.data(), span.count[0], slot_blocks, unpart_vec(), n_unpart,
-                                              span.begin[1], span.count[1]);
+                                              span.begin[1], span.count[1], zero, zero ? nzero : 0);
             };
             if(lvl1)
                 launch(mas_update_xr_restrict_kernel<T, true>);
@@ -1984,21 +2030,31 @@ class MasPreconditioner final : public LocalPreconditioner
             apply_t<double>(r, z, conv, s, d_inv64.data(), d_R64.data(), d_Z64.data());
     }
 
+    // The r.z and r.r partials from the fine apply: by default each wave of
+    // the plain fine kernel (one cluster) adds them into 256 dot slots with
+    // FP64 atomics, the slots cleared by the x, r update before it (FP64
+    // solve 176 -> 167 us per iteration, FP32 119 -> 117 us, against the
+    // separate dot kernel and the persistent fused kernel respectively).
+    // MI_MAS_DOT_SLOTS=0 restores the per-block partials.
+    bool use_slots() const
+    {
+        static const int env = [] {
+            const char* e = std::getenv("MI_MAS_DOT_SLOTS");
+            return e ? std::atoi(e) : 1;
+        }();
+        return env != 0;
+    }
+    int dot_slots() const override { return h.V > 0 && use_slots() ? 2 * kDotSlots : 0; }
+
     int apply_dot(const double* r, double* z, const int32_t* conv, double* part, int max_parts,
                   hipStream_t s) override
     {
-        // FP32 inverses only: with FP64 cluster blocks the fused kernel holds
-        // 132 VGPRs (3 waves per SIMD against 5 for the plain fine kernel)
-        // and the solve measured slower than with the separate dot kernel
-        static const int fused64 = [] {
-            const char* e = std::getenv("MI_MAS_FINE_DOT64");
-            return e ? std::atoi(e) : 0;
-        }();
-        if(h.V == 0 || max_parts < 1 || (!fp32 && !fused64))
+        const bool slots = use_slots();
+        if(h.V == 0 || max_parts < (slots ? kDotSlots : 1))
             return -1;
         if(!fp32)
-            return apply_t<double>(r, z, conv, s, d_inv64.data(), d_R64.data(), d_Z64.data(), part, max_parts);
-        return apply_t<float>(r, z, conv, s, d_inv32.data(), d_R32.data(), d_Z32.data(), part, max_parts);
+            return apply_t<double>(r, z, conv, s, d_inv64.data(), d_R64.data(), d_Z64.data(), part, max_parts, slots);
+        return apply_t<float>(r, z, conv, s, d_inv32.data(), d_R32.data(), d_Z32.data(), part, max_parts, slots);
     }
 };
 }  // namespace

@@ -222,10 +222,15 @@ __global__ __launch_bounds__(kThreads) void pcg_update_xr_kernel(PcgScalars*   s
                                                                 const double* p,
                                                                 double*       r,
                                                                 const double* Ap,
-                                                                int64_t       n)
+                                                                int64_t       n,
+                                                                double*       zero,
+                                                                int           nzero)
 {
     if(sc->converged)
         return;
+    if(blockIdx.x == 0)
+        for(int k = threadIdx.x; k < nzero; k += kThreads)
+            zero[k] = 0.0;
     __shared__ double lds[kThreads / kWave];
     const double      alpha = pcg_alpha(sc, pap_part, gp, lds);
     for(int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
@@ -524,10 +529,11 @@ void pcg_set_bb(PcgScalars* sc, double bb, double tol, hipStream_t s)
 }
 
 void pcg_update_xr(PcgScalars* sc, const double* pap_part, int gp, double* x, const double* p, double* r,
-                   const double* Ap, int64_t n, hipStream_t s)
+                   const double* Ap, int64_t n, hipStream_t s, double* zero, int nzero)
 {
     MI_REQUIRE(sc && pap_part && x && p && r && Ap, MI_ERR_STATE, "pcg_update_xr: null buffer");
-    pcg_update_xr_kernel<<<pcg_grid(n), kThreads, 0, s>>>(sc, pap_part, gp, x, p, r, Ap, n);
+    pcg_update_xr_kernel<<<pcg_grid(n), kThreads, 0, s>>>(sc, pap_part, gp, x, p, r, Ap, n, zero,
+                                                          zero ? nzero : 0);
 }
 
 int pcg_update_xr_diag3(PcgScalars* sc, const double* pap_part, int gp, double* x, const double* p, double* r,
