@@ -130,6 +130,32 @@ __device__ __forceinline__ T block_sum(T v, T* lds)
     return acc;
 }
 
+// Two sums over a workgroup in one pass (one pair of barriers); each is added
+// in the same order as block_sum's.  `lds` must hold 2 * NT / 64 values.
+template <int NT, typename T>
+__device__ __forceinline__ void block_sum2(T& a, T& b, T* lds)
+{
+    a           = wave_sum(a);
+    b           = wave_sum(b);
+    const int w = threadIdx.x / kWave;
+    if(threadIdx.x % kWave == 0)
+    {
+        lds[2 * w]     = a;
+        lds[2 * w + 1] = b;
+    }
+    __syncthreads();
+    T s0 = T(0), s1 = T(0);
+#pragma unroll
+    for(int k = 0; k < NT / kWave; ++k)
+    {
+        s0 += lds[2 * k];
+        s1 += lds[2 * k + 1];
+    }
+    __syncthreads();
+    a = s0;
+    b = s1;
+}
+
 // Exclusive prefix within a workgroup of NT threads (values >= 0).
 // Returns the exclusive prefix of this thread; *total = workgroup total.
 // `lds` must hold NT / 64 + 1 values.

@@ -485,12 +485,91 @@ __device__ __forceinline__ double pcg_sum_partials(const double* part, int g, in
     return block_sum<NT>(acc, lds);
 }
 
-// alpha = rz / pAp from the SpMV's pAp partials; workgroup 0 records it
-template <int NT>
-__device__ __forceinline__ double pcg_alpha_of(PcgScalars* sc, const double* pap_part, int gp, double* lds)
+// A thread's first K partials, loaded ahead of their re-sum: a kernel issues
+// its own operand loads between load() and sum(), so that the loads' latency
+// overlaps the sum's chain of shuffles and barriers.  The sum adds in
+// pcg_sum_partials' order (entries past g hold 0.0); partials beyond K * NT
+// are loaded by sum() itself.  The loads go through a buffer resource of g
+// entries, whose range check returns 0.0 past the end: no branch, so the
+// compiler does not wait for them before the caller's own loads.
+constexpr uint32_t kBufferRsrcWord3 = 0x00020000;  // raw (untyped) buffer, gfx9 family
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t partials_rsrc(const double* part, int g, int per)
 {
-    const double pAp   = pcg_sum_partials<NT>(pap_part, gp, 1, lds);
-    const double rz    = sc->rz_pending;
+    return __builtin_amdgcn_make_buffer_rsrc((void*)part, 0, g * per * (int)sizeof(double), kBufferRsrcWord3);
+}
+template <int NT, int K = 4>
+struct PartialsAhead
+{
+    double q[K];
+    __device__ __forceinline__ void load(const double* part, int g)
+    {
+        const __amdgpu_buffer_rsrc_t rs = partials_rsrc(part, g, 1);
+#pragma unroll
+        for(int j = 0; j < K; ++j)
+            q[j] = __builtin_bit_cast(
+                double, __builtin_amdgcn_raw_buffer_load_b64(rs, ((int)threadIdx.x + j * NT) * 8, 0, 0));
+    }
+    __device__ __forceinline__ double sum(const double* part, int g, double* lds) const
+    {
+        double acc = 0.0;
+#pragma unroll
+        for(int j = 0; j < K; ++j)
+            acc += q[j];
+        for(int k = (int)threadIdx.x + K * NT; k < g; k += NT)
+            acc += part[k];
+        return block_sum<NT>(acc, lds);
+    }
+};
+
+// the same for (r.z, r.r) partial pairs: both sums in one pass, each in
+// pcg_sum_partials' order; `lds` holds 2 * NT / 64 values
+template <int NT, int K = 2>
+struct PairsAhead
+{
+    double2 q[K];
+    __device__ __forceinline__ void load(const double* part, int g)
+    {
+        const __amdgpu_buffer_rsrc_t rs = partials_rsrc(part, g, 2);
+#pragma unroll
+        for(int j = 0; j < K; ++j)
+            q[j] = __builtin_bit_cast(
+                double2, __builtin_amdgcn_raw_buffer_load_b128(rs, ((int)threadIdx.x + j * NT) * 16, 0, 0));
+    }
+    __device__ __forceinline__ void sum(const double* part, int g, double* lds, double& s0, double& s1) const
+    {
+        double a = 0.0, b = 0.0;
+#pragma unroll
+        for(int j = 0; j < K; ++j)
+        {
+            a += q[j].x;
+            b += q[j].y;
+        }
+        for(int k = (int)threadIdx.x + K * NT; k < g; k += NT)
+        {
+            const double2 w = reinterpret_cast<const double2*>(part)[k];
+            a += w.x;
+            b += w.y;
+        }
+        block_sum2<NT>(a, b, lds);
+        s0 = a;
+        s1 = b;
+    }
+};
+
+template <int NT>
+__device__ __forceinline__ void pcg_sum_partials2(const double* part, int g, double* lds, double& s0, double& s1)
+{
+    PairsAhead<NT> ahead;
+    ahead.load(part, g);
+    ahead.sum(part, g, lds, s0, s1);
+}
+
+// alpha = rz / pAp from the SpMV's pAp partials; workgroup 0 records it.
+// rz is sc->rz_pending, which a kernel may read at its start (one scalar load
+// fewer on the chain after the partials' re-sum)
+template <int NT>
+__device__ __forceinline__ double pcg_alpha_of(PcgScalars* sc, double pAp, double rz)
+{
     const double alpha = rz / pAp;
     if(blockIdx.x == 0 && threadIdx.x == 0)
     {
@@ -499,6 +578,11 @@ __device__ __forceinline__ double pcg_alpha_of(PcgScalars* sc, const double* pap
         sc->rz    = rz;
     }
     return alpha;
+}
+template <int NT>
+__device__ __forceinline__ double pcg_alpha_of(PcgScalars* sc, const double* pap_part, int gp, double* lds)
+{
+    return pcg_alpha_of<NT>(sc, pcg_sum_partials<NT>(pap_part, gp, 1, lds), sc->rz_pending);
 }
 
 // ---- batched PCG: independent systems on consecutive block-row ranges

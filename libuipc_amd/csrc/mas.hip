@@ -1003,12 +1003,46 @@ struct LevelSpan
 // Restriction R_{l+1} = sum of children.  Every lane loads its own node's
 // value (coalesced); the lowest node of each in-bank sibling group (sib = bank
 // mask | 0x10000 on the lowest) adds the group's values, taken from the 16
-// lanes of its bank by shuffles in ascending slot order, and stores the
+// lanes of its bank (a DPP row) by row broadcasts in ascending slot order, and stores the
 // parent's sum: a parent's children all live in one bank, so there are no
 // atomics and the sum is deterministic.  At level 0 the values are r gathered
 // through slot_vec (the vector index of a slot's vertex, in the order the
 // vectors are in).  Level starts are bank-aligned, so a bank never straddles
 // two waves.
+// lane Q of the lane's 16-lane bank (a DPP row): row_newbcast, a VALU move
+// (no LDS round trip as with a shuffle).  Every lane of the row must be active.
+template <int Q>
+__device__ __forceinline__ float bank_lane(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x150 + Q, 0xf, 0xf,
+                                                                 false));
+}
+template <int Q>
+__device__ __forceinline__ double bank_lane(double v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x150 + Q, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x150 + Q, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+// s = sum of the bank's lanes q with bit q of m set, in ascending q
+template <typename T, int Q = 0>
+__device__ __forceinline__ void bank_sums(T v0, T v1, T v2, int32_t m, T& s0, T& s1, T& s2)
+{
+    if constexpr(Q == 0)
+        s0 = s1 = s2 = T(0);
+    if constexpr(Q < kBank)
+    {
+        const T w0 = bank_lane<Q>(v0), w1 = bank_lane<Q>(v1), w2 = bank_lane<Q>(v2);
+        if(m & (1 << Q))
+        {
+            s0 += w0;
+            s1 += w1;
+            s2 += w2;
+        }
+        bank_sums<T, Q + 1>(v0, v1, v2, m, s0, s1, s2);
+    }
+}
+
 template <typename T>
 __device__ __forceinline__ void restrict_sum_store(T              v0,
                                                    T              v1,
@@ -1018,22 +1052,27 @@ __device__ __forceinline__ void restrict_sum_store(T              v0,
                                                    const int32_t* __restrict__ parent,
                                                    int32_t        slot)
 {
-    const int b0 = (int)(threadIdx.x % kWave) & ~(kBank - 1);
-    T         s0 = 0, s1 = 0, s2 = 0;
-#pragma unroll
-    for(int q = 0; q < kBank; ++q)
-    {
-        const T w0 = __shfl(v0, b0 + q, kWave), w1 = __shfl(v1, b0 + q, kWave), w2 = __shfl(v2, b0 + q, kWave);
-        if(m & (1 << q))
-        {
-            s0 += w0;
-            s1 += w1;
-            s2 += w2;
-        }
-    }
+    T s0, s1, s2;
+    bank_sums<T>(v0, v1, v2, m, s0, s1, s2);
     if(m & 0x10000)
     {
         const int64_t pp = 3 * (int64_t)parent[slot];
+        Rb[pp]           = s0;
+        Rb[pp + 1]       = s1;
+        Rb[pp + 2]       = s2;
+    }
+}
+
+// the same sums, with the slot's parent already loaded (par); the lane's sums
+// are returned in s0, s1, s2 (zero unless m holds siblings)
+template <typename T>
+__device__ __forceinline__ void restrict_sum_store_to(T v0, T v1, T v2, int32_t m, T* Rb, int32_t par, T& s0, T& s1,
+                                                      T& s2)
+{
+    bank_sums<T>(v0, v1, v2, m, s0, s1, s2);
+    if(m & 0x10000)
+    {
+        const int64_t pp = 3 * (int64_t)par;
         Rb[pp]           = s0;
         Rb[pp + 1]       = s1;
         Rb[pp + 2]       = s2;
@@ -1117,98 +1156,116 @@ __global__ __launch_bounds__(256) void mas_update_xr_restrict_kernel(PcgScalars*
                                                                     int32_t       n_unpart,
                                                                     int32_t       begin1,
                                                                     int32_t       count1,
+                                                                    const int32_t* __restrict__ range_bank,
                                                                     double*       zero,
                                                                     int           nzero)
 {
-    if(sc->converged)
+    __shared__ double lds[256 / kWave];
+    __shared__ T      r1s[kBank][3];
+    // the lane's share of the SpMV's pAp partials, its indices, then its
+    // vertex's x, p, r and Ap are loaded before alpha's re-sum, so that the
+    // operands' latency overlaps the sum; the converged flag is tested once
+    // the index loads are out.  (Loads are unconditional, from clamped
+    // in-bounds indices, so that no branch makes the compiler wait for them.)
+    const int32_t      conv = sc->converged;
+    const double       rz   = sc->rz_pending;
+    PartialsAhead<256> ahead;
+    ahead.load(pap_part, gp);
+    const bool    slot_blk = (int32_t)blockIdx.x < slot_blocks;
+    const int32_t slot     = blockIdx.x * 256 + threadIdx.x;
+    const bool    live     = slot_blk && slot < nslots;
+    int32_t       v = -1, m = 0, par = 0;
+    if(slot_blk)
+    {
+        const int32_t sl = live ? slot : nslots - 1;
+        const int32_t vv = slot_vec[sl], mm = sib[sl], pp = parent[sl];
+        v                = live ? vv : -1;
+        m                = live ? mm : 0;
+        par              = live ? pp : 0;
+    }
+    else
+    {
+        const int32_t u  = (blockIdx.x - slot_blocks) * 256 + threadIdx.x;
+        const int32_t vv = unpart_vec[u < n_unpart ? u : n_unpart - 1];
+        v                = u < n_unpart ? vv : -1;
+    }
+    // kLevel1: the workgroup's level-1 bank (from the build) and, for lanes
+    // below kBank, its node's sibling mask and parent
+    int32_t bank1 = -1, m1 = 0, par1 = 0;
+    if constexpr(kLevel1)
+    {
+        if(slot_blk)
+        {
+            bank1              = range_bank[blockIdx.x];
+            const int32_t node = min(max(bank1, 0) * kBank + (int)(threadIdx.x % kBank), count1 - 1);
+            m1                 = sib[begin1 + node];
+            par1               = parent[begin1 + node];
+        }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    if(conv)
         return;
     if(blockIdx.x == 0)
         for(int k = threadIdx.x; k < nzero; k += 256)
             zero[k] = 0.0;
-    __shared__ double lds[256 / kWave];
-    __shared__ T      r1s[kBank][3];
-    __shared__ int32_t bank1;
-    const double      alpha = pcg_alpha_of<256>(sc, pap_part, gp, lds);
-    if((int32_t)blockIdx.x < slot_blocks)
-    {
-        const int32_t slot = blockIdx.x * 256 + threadIdx.x;
-        const bool    live = slot < nslots;
-        const int32_t v    = live ? slot_vec[slot] : -1;
-        T             v0 = 0, v1 = 0, v2 = 0;
-        if constexpr(kLevel1)
-        {
-            if(threadIdx.x < kBank)
-                r1s[threadIdx.x][0] = r1s[threadIdx.x][1] = r1s[threadIdx.x][2] = T(0);
-            if(threadIdx.x == 0)
-                bank1 = -1;
-            __syncthreads();
-        }
-        if(v >= 0)
-        {
-            const int64_t i  = 3 * (int64_t)v;
-            const double  r0 = r[i] - alpha * Ap[i], r1 = r[i + 1] - alpha * Ap[i + 1],
-                         r2 = r[i + 2] - alpha * Ap[i + 2];
-            x[i] += alpha * p[i];
-            x[i + 1] += alpha * p[i + 1];
-            x[i + 2] += alpha * p[i + 2];
-            r[i]     = r0;
-            r[i + 1] = r1;
-            r[i + 2] = r2;
-            v0 = (T)r0;
-            v1 = (T)r1;
-            v2 = (T)r2;
-        }
-        const int32_t m = live ? sib[slot] : 0;
-        restrict_sum_store<T>(v0, v1, v2, m, Rb, parent, slot);
-        if constexpr(kLevel1)
-        {
-            // the level-1 sums again, into LDS by the node's place in its bank
-            const int b0 = (int)(threadIdx.x % kWave) & ~(kBank - 1);
-            T         s0 = 0, s1 = 0, s2 = 0;
+    const int64_t i = 3 * (int64_t)(v < 0 ? 0 : v);
+    double        xv[3], pv[3], rv[3], av[3];
 #pragma unroll
-            for(int q = 0; q < kBank; ++q)
-            {
-                const T w0 = __shfl(v0, b0 + q, kWave), w1 = __shfl(v1, b0 + q, kWave),
-                        w2 = __shfl(v2, b0 + q, kWave);
-                if(m & (1 << q))
-                {
-                    s0 += w0;
-                    s1 += w1;
-                    s2 += w2;
-                }
-            }
-            if(m & 0x10000)
-            {
-                const int32_t node = parent[slot] - begin1;
-                r1s[node % kBank][0] = s0;
-                r1s[node % kBank][1] = s1;
-                r1s[node % kBank][2] = s2;
-                bank1                = node / kBank;  // the same bank for every node of the workgroup
-            }
-            __syncthreads();
-            if(threadIdx.x < kWave && bank1 >= 0)
-            {
-                const int     q     = threadIdx.x;
-                const int32_t node  = bank1 * kBank + q;
-                const bool    in    = q < kBank && node < count1;
-                const int32_t slot1 = begin1 + node;
-                restrict_sum_store<T>(in ? r1s[q][0] : T(0), in ? r1s[q][1] : T(0), in ? r1s[q][2] : T(0),
-                                      in ? sib[slot1] : 0, Rb, parent, slot1);
-            }
+    for(int c = 0; c < 3; ++c)
+    {
+        rv[c] = r[i + c];
+        av[c] = Ap[i + c];
+        pv[c] = p[i + c];
+        xv[c] = x[i + c];
+    }
+    if(v < 0)
+    {
+#pragma unroll
+        for(int c = 0; c < 3; ++c)
+            rv[c] = 0.0;  // a dead lane adds nothing to the sibling sums
+    }
+    if constexpr(kLevel1)
+    {
+        // ordered before the writes below by the barriers of alpha's sum
+        if(slot_blk && threadIdx.x < kBank)
+            r1s[threadIdx.x][0] = r1s[threadIdx.x][1] = r1s[threadIdx.x][2] = T(0);
+    }
+    const double alpha = pcg_alpha_of<256>(sc, ahead.sum(pap_part, gp, lds), rz);
+    if(v >= 0)
+    {
+#pragma unroll
+        for(int c = 0; c < 3; ++c)
+        {
+            rv[c]    = rv[c] - alpha * av[c];
+            x[i + c] = xv[c] + alpha * pv[c];
+            r[i + c] = rv[c];
         }
     }
-    else
+    if(!slot_blk)
+        return;
+    const T v0 = (T)rv[0], v1 = (T)rv[1], v2 = (T)rv[2];
+    T       s0, s1, s2;
+    restrict_sum_store_to<T>(v0, v1, v2, m, Rb, par, s0, s1, s2);
+    if constexpr(kLevel1)
     {
-        const int32_t u = (blockIdx.x - slot_blocks) * 256 + threadIdx.x;
-        if(u < n_unpart)
+        // the level-1 sums, also into LDS by the node's place in its bank
+        if(m & 0x10000)
         {
-            const int64_t i = 3 * (int64_t)unpart_vec[u];
-#pragma unroll
-            for(int c = 0; c < 3; ++c)
-            {
-                x[i + c] += alpha * p[i + c];
-                r[i + c] -= alpha * Ap[i + c];
-            }
+            // every node of the workgroup lies in bank1 (checked at the build)
+            const int32_t node   = par - begin1;
+            r1s[node % kBank][0] = s0;
+            r1s[node % kBank][1] = s1;
+            r1s[node % kBank][2] = s2;
+        }
+        __syncthreads();
+        if(threadIdx.x < kWave && bank1 >= 0)
+        {
+            const int     q    = threadIdx.x;
+            const int32_t node = bank1 * kBank + q;
+            const bool    in   = q < kBank && node < count1;
+            T t0, t1, t2;
+            restrict_sum_store_to<T>(in ? r1s[q][0] : T(0), in ? r1s[q][1] : T(0), in ? r1s[q][2] : T(0),
+                                     in ? m1 : 0, Rb, par1, t0, t1, t2);
         }
     }
 }
@@ -1542,6 +1599,7 @@ class MasPreconditioner final : public LocalPreconditioner
     int64_t   C = 0, C0 = 0;  // clusters, fine clusters
     // device tables
     DBuf<int32_t> d_part_to_real, d_real_to_part, d_anc, d_anc_slot, d_parent, d_sib, d_unpart;
+    DBuf<int32_t> d_range_bank;  // l1_local_: the level-1 bank of every 256-slot range (-1: none)
     // the vector index of every slot and unpartitioned vertex when the
     // vectors are in the SpMV's row order (the part order: rank_)
     DBuf<int32_t> d_part_to_pos, d_unpart_pos;
@@ -1741,6 +1799,8 @@ class MasPreconditioner final : public LocalPreconditioner
                     range_bank[range] = bank;
                 l1_local_ = bank_range[bank] == range && range_bank[range] == bank;
             }
+            if(l1_local_)
+                upload(d_range_bank, range_bank);
         }
         d_hess.resize((size_t)std::max<int64_t>(1, C * kCm));
         if(fp32)
@@ -1988,7 +2048,8 @@ class MasPreconditioner final : public LocalPreconditioner
             auto launch = [&](auto kernel) {
                 kernel<<<blocks, 256, 0, s>>>(sc, pap_part, gp, x, p, r, Ap, slot_vec(), R, d_parent.data(),
                                               d_sib.data(), span.count[0], slot_blocks, unpart_vec(), n_unpart,
-                                              span.begin[1], span.count[1], zero, zero ? nzero : 0);
+                                              span.begin[1], span.count[1], lvl1 ? d_range_bank.data() : nullptr,
+                                              zero, zero ? nzero : 0);
             };
             if(lvl1)
                 launch(mas_update_xr_restrict_kernel<T, true>);

@@ -152,13 +152,6 @@ __global__ void pcg_init_kernel(double* x, double* r, const double* b, int64_t n
     }
 }
 
-// the fixed-order sum of a reduction's partials, redone by every workgroup of
-// the consuming kernel (no fence, no atomics, deterministic)
-__device__ __forceinline__ double sum_partials(const double* part, int g, int stride, double* lds)
-{
-    return pcg_sum_partials<kThreads>(part, g, stride, lds);
-}
-
 __global__ __launch_bounds__(kThreads) void pcg_dot_partials_kernel(const double* r,
                                                                    const double* z,
                                                                    int64_t       n,
@@ -187,9 +180,9 @@ __global__ __launch_bounds__(kThreads) void pcg_dot_partials_kernel(const double
 __global__ __launch_bounds__(kThreads) void pcg_init_scalars_kernel(PcgScalars* sc, const double* part, int g,
                                                                    int criterion, double tol)
 {
-    __shared__ double lds[kThreads / kWave];
-    double            rz = sum_partials(part, g, 2, lds);
-    double            bb = sum_partials(part + 1, g, 2, lds);
+    __shared__ double lds[2 * kThreads / kWave];
+    double            rz, bb;
+    pcg_sum_partials2<kThreads>(part, g, lds, rz, bb);
     if(threadIdx.x == 0)
     {
         sc->rz         = rz;
@@ -228,12 +221,26 @@ __global__ __launch_bounds__(kThreads) void pcg_update_xr_kernel(PcgScalars*   s
 {
     if(sc->converged)
         return;
+    const double rz = sc->rz_pending;
     if(blockIdx.x == 0)
         for(int k = threadIdx.x; k < nzero; k += kThreads)
             zero[k] = 0.0;
     __shared__ double lds[kThreads / kWave];
-    const double      alpha = pcg_alpha(sc, pap_part, gp, lds);
-    for(int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
+    // the partials, then the first entry's operands are loaded before
+    // alpha's re-sum (their latency overlaps it)
+    PartialsAhead<kThreads> ahead;
+    ahead.load(pap_part, gp);
+    // (unconditional loads from a clamped index: no branch, no early wait)
+    const int64_t i0 = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    const int64_t ic = i0 < n ? i0 : n - 1;
+    const double  xa = x[ic], pa = p[ic], ra = r[ic], aa = Ap[ic];
+    const double alpha = pcg_alpha_of<kThreads>(sc, ahead.sum(pap_part, gp, lds), rz);
+    if(i0 < n)
+    {
+        x[i0] = xa + alpha * pa;
+        r[i0] = ra - alpha * aa;
+    }
+    for(int64_t i = i0 + (int64_t)gridDim.x * kThreads; i < n; i += (int64_t)gridDim.x * kThreads)
     {
         x[i] += alpha * p[i];
         r[i] -= alpha * Ap[i];
@@ -308,24 +315,32 @@ __global__ __launch_bounds__(kThreads) void pcg_update_xr_diag3_kernel(PcgScalar
 // fused_update_p_beta (593-611): converged = |rz_new| <= rz_tol (or
 // ||r||^2 <= rr_tol), beta = rz_new / rz, p = z + beta p unless converged.
 // Every workgroup derives the same values from the partials; workgroup 0
-// records them.
-__global__ __launch_bounds__(kThreads) void pcg_update_p_kernel(PcgScalars*   sc,
-                                                               const double* part,
-                                                               int           gd,
-                                                               double*       p,
-                                                               const double* z,
-                                                               int64_t       n,
-                                                               float4*       p32)
+// records them.  Returns whether the solve has converged.
+// The scalars it reads are loaded at the kernel's start (PScalarsIn), off the
+// chain that follows the partials' re-sum.
+struct PScalarsIn
 {
-    if(sc->converged)
-        return;
-    __shared__ double lds[kThreads / kWave];
-    const double      rz_new = sum_partials(part, gd, 2, lds);
-    const double      rr     = sum_partials(part + 1, gd, 2, lds);
-    bool conv = sc->criterion == 0 ? (fabs(rz_new) <= sc->rz_tol) : (rr <= sc->rr_tol);
+    double rz, rz_tol, rr_tol;
+    int    criterion;
+    __device__ __forceinline__ explicit PScalarsIn(const PcgScalars* sc)
+        : rz(sc->rz), rz_tol(sc->rz_tol), rr_tol(sc->rr_tol), criterion(sc->criterion)
+    {
+    }
+};
+__device__ __forceinline__ bool pcg_p_scalars(PcgScalars*                 sc,
+                                              const PScalarsIn&           in,
+                                              const PairsAhead<kThreads>& ahead,
+                                              const double*               part,
+                                              int                         gd,
+                                              double*                     lds,
+                                              double&                     beta)
+{
+    double rz_new, rr;
+    ahead.sum(part, gd, lds, rz_new, rr);
+    bool conv = in.criterion == 0 ? (fabs(rz_new) <= in.rz_tol) : (rr <= in.rr_tol);
     if(!isfinite(rz_new) || !isfinite(rr))
         conv = false;
-    const double beta = rz_new / sc->rz;
+    beta = rz_new / in.rz;
     if(blockIdx.x == 0 && threadIdx.x == 0)
     {
         const int it   = sc->iter + 1;
@@ -340,15 +355,55 @@ __global__ __launch_bounds__(kThreads) void pcg_update_p_kernel(PcgScalars*   sc
             sc->conv_iter = it;
         }
     }
-    if(conv)
+    return conv;
+}
+
+// A thread's partials, then its first operands are loaded before the
+// partials' re-sum, so that their latency overlaps the sum's chain of
+// shuffles and barriers.
+__global__ __launch_bounds__(kThreads) void pcg_update_p_kernel(PcgScalars*   sc,
+                                                               const double* part,
+                                                               int           gd,
+                                                               double*       p,
+                                                               const double* z,
+                                                               int64_t       n,
+                                                               float4*       p32)
+{
+    if(sc->converged)
         return;
+    const PScalarsIn  in(sc);
+    __shared__ double lds[2 * kThreads / kWave];
+    const int64_t     i0     = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    const int64_t     stride = (int64_t)gridDim.x * kThreads;
+    double            beta;
+    PairsAhead<kThreads> ahead;
+    ahead.load(part, gd);
     if(p32)
     {
         // the FP32 SpMV's copy of p: one padded float4 per block row
-        for(int64_t k = (int64_t)blockIdx.x * kThreads + threadIdx.x; 3 * k < n; k += (int64_t)gridDim.x * kThreads)
+        // (unconditional loads from a clamped row: no branch, no early wait)
+        const int64_t kc = 3 * i0 < n ? i0 : n / 3 - 1;
+        double        zq[3], pq[3];
+#pragma unroll
+        for(int c = 0; c < 3; ++c)
         {
-            const double q0 = z[3 * k] + beta * p[3 * k], q1 = z[3 * k + 1] + beta * p[3 * k + 1],
-                         q2 = z[3 * k + 2] + beta * p[3 * k + 2];
+            zq[c] = z[3 * kc + c];
+            pq[c] = p[3 * kc + c];
+        }
+        if(pcg_p_scalars(sc, in, ahead, part, gd, lds, beta))
+            return;
+        for(int64_t k = i0; 3 * k < n; k += stride)
+        {
+            if(k != i0)
+            {
+#pragma unroll
+                for(int c = 0; c < 3; ++c)
+                {
+                    zq[c] = z[3 * k + c];
+                    pq[c] = p[3 * k + c];
+                }
+            }
+            const double q0 = zq[0] + beta * pq[0], q1 = zq[1] + beta * pq[1], q2 = zq[2] + beta * pq[2];
             p[3 * k]     = q0;
             p[3 * k + 1] = q1;
             p[3 * k + 2] = q2;
@@ -356,7 +411,16 @@ __global__ __launch_bounds__(kThreads) void pcg_update_p_kernel(PcgScalars*   sc
         }
         return;
     }
-    for(int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
+    const int64_t i1 = i0 + stride;
+    const int64_t c0 = i0 < n ? i0 : n - 1, c1 = i1 < n ? i1 : n - 1;
+    const double  za = z[c0], pa = p[c0], zb = z[c1], pb = p[c1];
+    if(pcg_p_scalars(sc, in, ahead, part, gd, lds, beta))
+        return;
+    if(i0 < n)
+        p[i0] = za + beta * pa;
+    if(i1 < n)
+        p[i1] = zb + beta * pb;
+    for(int64_t i = i1 + stride; i < n; i += stride)
         p[i] = z[i] + beta * p[i];
 }
 
