@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 #include <stdexcept>
 #include <string>
 
@@ -111,12 +112,55 @@ __device__ __forceinline__ T wave_sum(T v)
     return v;
 }
 
+// ---- DPP cross-lane moves (VALU; a shuffle is an LDS round trip).  A DPP
+// row is 16 lanes.  Every lane of the rows read must be active.
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ double dpp_f64(double v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWS, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWS, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+constexpr int kDppQuadXor1   = 0xB1;   // quad_perm [1,0,3,2]
+constexpr int kDppQuadXor2   = 0x4E;   // quad_perm [2,3,0,1]
+constexpr int kDppHalfMirror = 0x141;  // row_half_mirror: lane i of 8 <- 7 - i
+constexpr int kDppMirror     = 0x140;  // row_mirror: lane i of 16 <- 15 - i
+constexpr int kDppBcast15    = 0x142;  // row_bcast:15: lane 15 of a row into the next row
+constexpr int kDppBcast31    = 0x143;  // row_bcast:31: lane 31 into rows 2 and 3
+
+// Sum over the lane's 16-lane row, the same bits in every lane of the row
+// (each step adds two equal-ordered halves, and a + b == b + a exactly):
+// ((x0 + x1) + (x2 + x3)) + ... quads, then halves, then the row.
+__device__ __forceinline__ double row16_sum(double v)
+{
+    v += dpp_f64<kDppQuadXor1>(v);
+    v += dpp_f64<kDppQuadXor2>(v);
+    v += dpp_f64<kDppHalfMirror>(v);
+    v += dpp_f64<kDppMirror>(v);
+    return v;
+}
+
+// Sum over the wave, (R3 + R2) + (R1 + R0) of its row sums, uniform (read
+// from lane 63).  All 64 lanes must be active.
+__device__ __forceinline__ double wave_total(double v)
+{
+    v = row16_sum(v);
+    v += dpp_f64<kDppBcast15, 0xa>(v);  // rows 1, 3: R1 + R0, R3 + R2
+    v += dpp_f64<kDppBcast31, 0xc>(v);  // rows 2, 3: + (R1 + R0)
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane(__double2loint(v), 63);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane(__double2hiint(v), 63);
+    return __hiloint2double((int)hi, (int)lo);
+}
+
 // Sum over a workgroup of `NT` threads; every thread gets the result.
-// `lds` must hold NT / 64 values.
+// `lds` must hold NT / 64 values.  Called by every thread of the workgroup.
 template <int NT, typename T>
 __device__ __forceinline__ T block_sum(T v, T* lds)
 {
-    v             = wave_sum(v);
+    if constexpr(std::is_same<T, double>::value)
+        v = wave_total(v);
+    else
+        v = wave_sum(v);
     const int w   = threadIdx.x / kWave;
     const int l   = threadIdx.x % kWave;
     if(l == 0)
@@ -135,8 +179,8 @@ __device__ __forceinline__ T block_sum(T v, T* lds)
 template <int NT, typename T>
 __device__ __forceinline__ void block_sum2(T& a, T& b, T* lds)
 {
-    a           = wave_sum(a);
-    b           = wave_sum(b);
+    a           = wave_total(a);
+    b           = wave_total(b);
     const int w = threadIdx.x / kWave;
     if(threadIdx.x % kWave == 0)
     {

@@ -1352,12 +1352,8 @@ __global__ __launch_bounds__(64) void mas_apply_coarse_kernel(const T* __restric
 constexpr int kDotSlots = 256;
 __device__ __forceinline__ void wave_dot_to_slots(double s0, double s1, double* slots, int64_t blk)
 {
-#pragma unroll
-    for(int o = kWave / 2; o > 0; o >>= 1)
-    {
-        s0 += __shfl_xor(s0, o, kWave);
-        s1 += __shfl_xor(s1, o, kWave);
-    }
+    s0 = wave_total(s0);
+    s1 = wave_total(s1);
     if((threadIdx.x % kWave) == 0)
     {
         double* d = slots + 2 * (blk % kDotSlots);

@@ -273,13 +273,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) v
                     acc1 += (double)w[1][i];
                     acc2 += (double)w[2][i];
                 }
-#pragma unroll
-                for(int o = kLanes / 2; o > 0; o >>= 1)
-                {
-                    acc0 += __shfl_xor(acc0, o, kWave);
-                    acc1 += __shfl_xor(acc1, o, kWave);
-                    acc2 += __shfl_xor(acc2, o, kWave);
-                }
+                // the row's 16 lanes (a DPP row) in a fixed order
+                acc0 = row16_sum(acc0);
+                acc1 = row16_sum(acc1);
+                acc2 = row16_sum(acc2);
                 if(lane == 0)
                 {
                     if(row == r_first && carried)
@@ -554,13 +551,10 @@ __global__ __launch_bounds__(NT) void spmv_win_kernel(int32_t nchunks,
                     acc1 += (double)w[1][i];
                     acc2 += (double)w[2][i];
                 }
-#pragma unroll
-                for(int o = kLanes / 2; o > 0; o >>= 1)
-                {
-                    acc0 += __shfl_xor(acc0, o, kWave);
-                    acc1 += __shfl_xor(acc1, o, kWave);
-                    acc2 += __shfl_xor(acc2, o, kWave);
-                }
+                // the row's 16 lanes (a DPP row) in a fixed order
+                acc0 = row16_sum(acc0);
+                acc1 = row16_sum(acc1);
+                acc2 = row16_sum(acc2);
                 if(lane == 0 && b1 > b0)
                 {
                     ys[0][row] += acc0;
