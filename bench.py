@@ -54,6 +54,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32-arm", action="store_true",
                     help="skip the FP32 arm that the default FP64 MAS run also times")
+    ap.add_argument("--no-ref-arm", action="store_true",
+                    help="skip the reference-precision solve (FP64 system, float MAS storage) it also times")
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--producers", choices=["host", "device"], default="host",
                     help="device: write the triplets with the device Hessian producers (timed apart, outside "
@@ -328,11 +330,66 @@ def main():
 
     if prec == abi.MI_FP64 and precond == "mas" and not args.no_fp32_arm and args.mas_levels != "auto":
         result["fp32_arm"] = fp32_arm(ctx, s, args, rows, cols, vals, b, x.numpy(), ms_per_step, solve_kw)
+    if prec == abi.MI_FP64 and precond == "mas" and not args.no_ref_arm and args.mas_levels != "auto":
+        result["ref_precision_arm"] = ref_precision_arm(ctx, s, args, rows, cols, vals, b, x.numpy(), ms_per_step,
+                                                        solve_kw)
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(s, precond, args, int(st["iters"]))
     print(json.dumps(result), flush=True)
     ls.close()
     ctx.close()
+
+
+def ref_precision_arm(ctx, s, args, rows, cols, vals, b, x64, ms64, solve_kw) -> dict:
+    """The same solve at the reference's own precision: FP64 triplets, matrix
+    stream, vectors and dots, and the MAS preconditioner's inverses and
+    multilevel R / Z stored in float with the factorisation in FP64
+    (mas_preconditioner_engine.h:17-22, 241, 254: "matches GIPC").  `value`
+    keeps the MAS in FP64 throughout (the plain method); this line says what
+    the reference's storage types cost or save.  Same MAS level count and
+    SpMV layout choice as `value`, median of the same number of steps."""
+    from libuipc_amd import abi
+    from libuipc_amd.device import LinearSystem
+    from libuipc_amd.hip_events import EventTimer
+
+    stream = ctx.stream()
+    ls = LinearSystem(ctx, s.block_rows, abi.MI_FP64)
+    ls.add_preconditioner(abi.MI_PRECOND_MAS, 0, s.n, part_ids=s.part_ids, tets=s.tets, mas_precision=abi.MI_FP32,
+                          active_levels=args.mas_levels)
+    x = ctx.alloc(s.n, np.float64)
+
+    def step():
+        ls.assemble(rows, cols, vals, stream)
+        ls.assemble_preconditioners()
+        return ls.solve(b, x, **solve_kw)
+
+    for _ in range(args.warmup):
+        step()
+    if args.spmv_shape == "auto":
+        from libuipc_amd.tune import tune_spmv_shape
+
+        tune_spmv_shape(ls, x)
+    else:
+        ls.set_spmv_shape(args.spmv_shape)
+    ev = EventTimer(stream)
+    ctx.synchronize()
+    times = []
+    for _ in range(args.steps):
+        ev.start()
+        step()
+        times.append(ev.stop())
+    st = ls.solve(b, x, **solve_kw, true_residual=1)
+    xr = x.numpy()
+    ms = float(np.median(times))
+    ls.close()
+    out = {"ms": round(ms, 3), "ratio_to_fp64": round(ms / ms64, 4), "pcg_iters": int(st["iters"]),
+           "rel_res_true": st["rel_res_true"],
+           "x_rel_diff_to_fp64": float(np.linalg.norm(xr - x64) / max(np.linalg.norm(x64), 1e-300)),
+           "mas_levels": args.mas_levels,
+           "note": "FP64 system, vectors and dots; MAS inverses and R / Z stored in float (the reference's MAS "
+                   "storage types), factorisation in FP64"}
+    log(f"reference-precision arm: {out}")
+    return out
 
 
 def fp32_arm(ctx, s, args, rows, cols, vals, b, x64, ms64, solve_kw) -> dict:

@@ -121,6 +121,21 @@ __device__ __forceinline__ double dpp_f64(double v)
     const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWS, 0xf, false);
     return __hiloint2double(hi, lo);
 }
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ float dpp_f32(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, ROWS, 0xf, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_t(double v)
+{
+    return dpp_f64<CTRL>(v);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_t(float v)
+{
+    return dpp_f32<CTRL>(v);
+}
 constexpr int kDppQuadXor1   = 0xB1;   // quad_perm [1,0,3,2]
 constexpr int kDppQuadXor2   = 0x4E;   // quad_perm [2,3,0,1]
 constexpr int kDppHalfMirror = 0x141;  // row_half_mirror: lane i of 8 <- 7 - i
@@ -137,6 +152,25 @@ __device__ __forceinline__ double row16_sum(double v)
     v += dpp_f64<kDppQuadXor2>(v);
     v += dpp_f64<kDppHalfMirror>(v);
     v += dpp_f64<kDppMirror>(v);
+    return v;
+}
+
+// Sum over aligned groups of g lanes (g = 4, 8, ..., 64, uniform over the
+// wave), every lane of a group getting it: the same bits as the ascending xor
+// butterfly (after each step a half-group holds one value, so the mirror's
+// partner lane and the xor partner carry the same bits); strides of 16 and up
+// are shuffles.
+template <typename T>
+__device__ __forceinline__ T group_sum(T v, int g)
+{
+    v += dpp_t<kDppQuadXor1>(v);
+    v += dpp_t<kDppQuadXor2>(v);
+    if(g >= 8)
+        v += dpp_t<kDppHalfMirror>(v);
+    if(g >= 16)
+        v += dpp_t<kDppMirror>(v);
+    for(int o = 16; o < g; o <<= 1)
+        v += __shfl_xor(v, o, kWave);
     return v;
 }
 

@@ -407,7 +407,8 @@ __device__ __forceinline__ void agg_atomic_add(double* hess, int64_t dst, const 
 #pragma unroll
         for(int q = 0; q < 9; ++q)
         {
-            double s = wave_sum(mine ? v[q] : 0.0);
+            // every lane is active here (the loop's exits are wave-uniform)
+            const double s = wave_total(mine ? v[q] : 0.0);
             if(__lane_id() == leader)
                 atomic_add_f64(hess + ld + q, s);
         }
@@ -1319,12 +1320,10 @@ __global__ __launch_bounds__(64) void mas_apply_coarse_kernel(const T* __restric
                 s2 += Rb[y + 2];
             }
         }
-        for(int o = 1; o < lpn; o <<= 1)
-        {
-            s0 += __shfl_xor(s0, o, kWave);
-            s1 += __shfl_xor(s1, o, kWave);
-            s2 += __shfl_xor(s2, o, kWave);
-        }
+        // the node's lpn lanes (an aligned group; lpn >= 4, uniform)
+        s0 = group_sum(s0, lpn);
+        s1 = group_sum(s1, lpn);
+        s2 = group_sum(s2, lpn);
         if(lane < kBank)
         {
             sR[lane][0] = 0;
