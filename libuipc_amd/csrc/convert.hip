@@ -50,25 +50,6 @@ __device__ __forceinline__ uint64_t entry_key(int32_t r, int32_t c, const KeyCod
         return kc.lo(r, c);
 }
 
-// stage one tile of (row, col) into LDS with coalesced loads
-__device__ __forceinline__ void stage_rc(const int32_t* rows,
-                                         const int32_t* cols,
-                                         int64_t        n,
-                                         int64_t        base,
-                                         int32_t*       sr,
-                                         int32_t*       sc)
-{
-#pragma unroll
-    for(int k = 0; k < kItems; ++k)
-    {
-        int     t = k * kThreads + threadIdx.x;
-        int64_t i = base + t;
-        sr[t]     = i < n ? rows[i] : -1;
-        sc[t]     = i < n ? cols[i] : -2;
-    }
-    __syncthreads();
-}
-
 // ---- 1. count kept entries per tile
 // (and, with d_band, the largest col - row of the kept entries)
 template <int MODE>
@@ -116,59 +97,6 @@ __global__ __launch_bounds__(kThreads) void cv_count_kernel(int64_t        T,
                 m = max(m, bmax[w]);
             atomicMax(d_band, m);
         }
-    }
-}
-
-// ---- 2. stable compaction of (key, source index).  The tile is staged in
-// LDS with coalesced loads; thread t then owns the contiguous items
-// [t * kItems, (t + 1) * kItems) so that the block prefix follows index order.
-// The kept entries are gathered back into LDS in that order and written out
-// by consecutive lanes (one lane per item: coalesced stores).
-template <int MODE, typename KeyT>
-__global__ __launch_bounds__(kThreads) void cv_compact_kernel(int64_t         T,
-                                                             const int32_t*  rows,
-                                                             const int32_t*  cols,
-                                                             int32_t         B,
-                                                             KeyCodec        kc,
-                                                             const uint32_t* tile_off,
-                                                             KeyT*           keys,
-                                                             int32_t*        srcs)
-{
-    __shared__ int32_t  sr[kTile], sc[kTile], si[kTile];
-    __shared__ uint32_t lds[kWaves + 1];
-    const int64_t       base = (int64_t)blockIdx.x * kTile;
-    stage_rc(rows, cols, T, base, sr, sc);
-    const int first = threadIdx.x * kItems;
-    uint32_t  flags = 0, c = 0;
-    int32_t   rr[kItems], cc[kItems];
-#pragma unroll
-    for(int k = 0; k < kItems; ++k)
-    {
-        rr[k] = sr[first + k];
-        cc[k] = sc[first + k];
-        if(base + first + k < T && keep_entry<MODE>(rr[k], cc[k], B))
-        {
-            flags |= 1u << k;
-            ++c;
-        }
-    }
-    uint32_t kept = 0;
-    uint32_t o    = block_exclusive_scan<kThreads>(c, lds, &kept);  // its barriers end every read of sr / sc
-#pragma unroll
-    for(int k = 0; k < kItems; ++k)
-        if(flags & (1u << k))
-        {
-            sr[o] = rr[k];
-            sc[o] = cc[k];
-            si[o] = first + k;
-            ++o;
-        }
-    __syncthreads();
-    const uint32_t g = tile_off[blockIdx.x];
-    for(uint32_t i = threadIdx.x; i < kept; i += kThreads)
-    {
-        keys[g + i] = (KeyT)entry_key<MODE>(sr[i], sc[i], kc);
-        srcs[g + i] = (int32_t)(base + si[i]);
     }
 }
 
@@ -246,26 +174,81 @@ __global__ __launch_bounds__(kThreads) void scan_down_kernel(uint32_t* data, int
     }
 }
 
-// ---- radix sort: tile histogram (digit-major layout hist[d * ntiles + tile])
+// ---- radix sort inputs.  PairSource: the (key, source) pairs of the last
+// pass, m of them.  TripletSource: the first pass straight from the
+// triplets' (row, col) -- entries that keep_entry drops are skipped and the
+// key is built on the fly, so the compaction writes no pairs of its own (a
+// stable pass over the kept entries in index order, as compacting first
+// would give).
 template <typename KeyT>
-__global__ __launch_bounds__(kThreads) void rs_hist_kernel(const int64_t*  d_m,
-                                                          const KeyT*     keys,
-                                                          int             shift,
-                                                          uint32_t        dmask,
-                                                          uint32_t*       hist,
-                                                          int             ntiles)
+struct PairSource
+{
+    const int64_t* d_m;
+    const KeyT*    keys;
+    const int32_t* srcs;
+    __device__ int64_t count() const { return *d_m; }
+    __device__ bool    key_at(int64_t i, int64_t m, KeyT& k) const
+    {
+        if(i >= m)
+            return false;
+        k = keys[i];
+        return true;
+    }
+    __device__ bool pair_at(int64_t i, int64_t m, KeyT& k, int32_t& v) const
+    {
+        if(i >= m)
+            return false;
+        k = keys[i];
+        v = srcs[i];
+        return true;
+    }
+};
+template <int MODE, typename KeyT>
+struct TripletSource
+{
+    const int32_t* rows;
+    const int32_t* cols;
+    int64_t        n;
+    int32_t        B;
+    KeyCodec       kc;
+    __device__ int64_t count() const { return n; }
+    __device__ bool    key_at(int64_t i, int64_t m, KeyT& k) const
+    {
+        if(i >= m)
+            return false;
+        const int32_t r = rows[i], c = cols[i];
+        if(!keep_entry<MODE>(r, c, B))
+            return false;
+        k = (KeyT)entry_key<MODE>(r, c, kc);
+        return true;
+    }
+    __device__ bool pair_at(int64_t i, int64_t m, KeyT& k, int32_t& v) const
+    {
+        v = (int32_t)i;
+        return key_at(i, m, k);
+    }
+};
+
+// ---- radix sort: tile histogram (digit-major layout hist[d * ntiles + tile])
+template <typename KeyT, typename Src>
+__global__ __launch_bounds__(kThreads) void rs_hist_kernel(Src           src,
+                                                          int           shift,
+                                                          uint32_t      dmask,
+                                                          uint32_t*     hist,
+                                                          int           ntiles)
 {
     __shared__ uint32_t h[256];
     h[threadIdx.x] = 0;
     __syncthreads();
-    const int64_t m    = *d_m;
+    const int64_t m    = src.count();
     const int64_t base = (int64_t)blockIdx.x * kTile;
 #pragma unroll
     for(int k = 0; k < kItems; ++k)
     {
-        int64_t i = base + (int64_t)k * kThreads + threadIdx.x;
-        if(i < m)
-            atomicAdd(&h[(uint32_t)(keys[i] >> shift) & dmask], 1u);
+        const int64_t i = base + (int64_t)k * kThreads + threadIdx.x;
+        KeyT          key;
+        if(src.key_at(i, m, key))
+            atomicAdd(&h[(uint32_t)(key >> shift) & dmask], 1u);
     }
     __syncthreads();
     hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
@@ -280,10 +263,8 @@ __device__ __forceinline__ uint64_t lanemask_lt64()
 // of its contiguous quarter of the tile in order (ballot-matched peers +
 // per-wave digit counters), the tile is reordered by digit in LDS, then each
 // digit run is written out contiguously.
-template <typename KeyT>
-__global__ __launch_bounds__(kThreads) void rs_scatter_kernel(const int64_t*  d_m,
-                                                             const KeyT*     kin,
-                                                             const int32_t*  vin,
+template <typename KeyT, typename Src>
+__global__ __launch_bounds__(kThreads) void rs_scatter_kernel(Src             src,
                                                              KeyT*           kout,
                                                              int32_t*        vout,
                                                              int             shift,
@@ -298,7 +279,7 @@ __global__ __launch_bounds__(kThreads) void rs_scatter_kernel(const int64_t*  d_
     __shared__ uint32_t gbase[256];
     __shared__ uint32_t lds[kWaves + 1];
 
-    const int64_t m    = *d_m;
+    const int64_t m    = src.count();
     const int64_t base = (int64_t)blockIdx.x * kTile;
     if(base >= m)
         return;  // uniform per block
@@ -318,9 +299,9 @@ __global__ __launch_bounds__(kThreads) void rs_scatter_kernel(const int64_t*  d_
     for(int k = 0; k < kItems; ++k)
     {
         int64_t  i     = base + (int64_t)w * (kWave * kItems) + (int64_t)k * kWave + lane;
-        bool     valid = i < m;
-        key[k]         = valid ? kin[i] : KeyT(0);
-        val[k]         = valid ? vin[i] : 0;
+        key[k]         = KeyT(0);
+        val[k]         = 0;
+        const bool valid = src.pair_at(i, m, key[k], val[k]);
         uint32_t d     = valid ? ((uint32_t)(key[k] >> shift) & dmask) : 0u;
         dig[k]         = valid ? d : 0xFFFFFFFFu;
         uint64_t peers = __ballot(valid);
@@ -344,6 +325,7 @@ __global__ __launch_bounds__(kThreads) void rs_scatter_kernel(const int64_t*  d_
     }
     __syncthreads();
     // per digit: wave prefixes (in place) and the tile total
+    uint32_t tile_total;  // the tile's kept entries
     {
         const int d   = threadIdx.x;
         uint32_t  acc = 0;
@@ -354,7 +336,7 @@ __global__ __launch_bounds__(kThreads) void rs_scatter_kernel(const int64_t*  d_
             wcnt[q][d] = acc;
             acc += c;
         }
-        uint32_t e = block_exclusive_scan<kThreads>(acc, lds, (uint32_t*)nullptr);
+        uint32_t e = block_exclusive_scan<kThreads>(acc, lds, &tile_total);
         dstart[d]  = e;
         gbase[d]   = hist_scan[(int64_t)d * ntiles + blockIdx.x];
     }
@@ -368,7 +350,7 @@ __global__ __launch_bounds__(kThreads) void rs_scatter_kernel(const int64_t*  d_
             sv[pos]      = val[k];
         }
     __syncthreads();
-    const int64_t cnt = (m - base) < kTile ? (m - base) : kTile;
+    const int64_t cnt = tile_total;
 #pragma unroll
     for(int k = 0; k < kItems; ++k)
     {
@@ -808,21 +790,44 @@ __global__ __launch_bounds__(kThreads) void win_lower_kernel(int32_t        B,
 }
 
 // lowest row and highest row + 1 of every window (batched solves skip the
-// windows that hold no row of a running system)
-__global__ void win_range_kernel(int32_t B, int shift, int32_t nchunks, const int32_t* __restrict__ real, int32_t* lo,
-                                 int32_t* hi)
+// windows that hold no row of a running system): one workgroup per window
+__global__ __launch_bounds__(kThreads) void win_range_kernel(int32_t B, int shift, int32_t nchunks,
+                                                            const int32_t* __restrict__ real, int32_t* lo,
+                                                            int32_t* hi)
 {
-    for(int32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nchunks; c += gridDim.x * blockDim.x)
+    __shared__ int32_t smin[kThreads / kWave], smax[kThreads / kWave];
+    for(int32_t c = blockIdx.x; c < nchunks; c += gridDim.x)
     {
         const int32_t p0 = c << shift, p1 = (int32_t)std::min<int64_t>((int64_t)(c + 1) << shift, B);
         int32_t       a = B, b = -1;
-        for(int32_t p = p0; p < p1; ++p)
+        for(int32_t p = p0 + (int32_t)threadIdx.x; p < p1; p += kThreads)
         {
             a = std::min(a, real[p]);
             b = std::max(b, real[p]);
         }
-        lo[c] = a;
-        hi[c] = b + 1;
+#pragma unroll
+        for(int o = kWave / 2; o > 0; o >>= 1)
+        {
+            a = std::min(a, __shfl_xor(a, o, kWave));
+            b = std::max(b, __shfl_xor(b, o, kWave));
+        }
+        if(threadIdx.x % kWave == 0)
+        {
+            smin[threadIdx.x / kWave] = a;
+            smax[threadIdx.x / kWave] = b;
+        }
+        __syncthreads();
+        if(threadIdx.x == 0)
+        {
+            for(int w = 1; w < kThreads / kWave; ++w)
+            {
+                a = std::min(a, smin[w]);
+                b = std::max(b, smax[w]);
+            }
+            lo[c] = a;
+            hi[c] = b + 1;
+        }
+        __syncthreads();
     }
 }
 
@@ -887,10 +892,12 @@ int MatrixConverter::compact_sort(int64_t        n,
     scan_exclusive(tile_cnt_.data(), ntiles, chunk_, d_count, s);
     KeyT* k0 = reinterpret_cast<KeyT*>(keys_[0].data());
     KeyT* k1 = reinterpret_cast<KeyT*>(keys_[1].data());
-    cv_compact_kernel<MODE, KeyT><<<ntiles, kThreads, 0, s>>>(n, rows, cols, B, codec_, tile_cnt_.data(), k0,
-                                                              srcs_[0].data());
-    // stable LSD radix sort (matrix_converter.inl:1610-1616) on the key's bits
-    return radix_sort_pairs_t<KeyT>(k0, srcs_[0].data(), k1, srcs_[1].data(), d_count, n, key_bits_, hist_, chunk_, s);
+    // compaction (the kept count, *d_count, from the tile counts) and a stable
+    // LSD radix sort (matrix_converter.inl:1579-1616) on the key's bits, whose
+    // first pass reads the triplets' (row, col) and skips the dropped entries
+    TripletSource<MODE, KeyT> first{rows, cols, n, B, codec_};
+    return radix_sort_from<KeyT>(first, k0, srcs_[0].data(), k1, srcs_[1].data(), d_count, n, key_bits_, hist_,
+                                 chunk_, s);
 }
 
 void MatrixConverter::build_window(BsrMatrix& A, int shape, bool want32, double ratio, hipStream_t s)
@@ -926,8 +933,8 @@ void MatrixConverter::build_window(BsrMatrix& A, int shape, bool want32, double 
             B, shift, A.wpos.data(), A.wreal.data(), A.row_ptr.data(), lower_ptr_.data(),
             U > 0 ? key_ptr<uint64_t>(tcur_) : nullptr, codec(), cnt);
     scan_exclusive(cnt, (int64_t)B + 1, chunk_, counts_.data() + 4, s);
-    win_range_kernel<<<grid_of(A.wnchunks), kThreads, 0, s>>>(B, shift, A.wnchunks, A.wreal.data(),
-                                                              A.wchunk_lo.data(), A.wchunk_hi.data());
+    win_range_kernel<<<std::max(1, std::min(A.wnchunks, device_cu_count() * 8)), kThreads, 0, s>>>(
+        B, shift, A.wnchunks, A.wreal.data(), A.wchunk_lo.data(), A.wchunk_hi.data());
     // the window layout holds at most the full BSR's 2U blocks; one block of
     // padding past the last, and an even plane stride
     const int64_t wmax = 2 * U;
@@ -976,17 +983,20 @@ void MatrixConverter::build_window(BsrMatrix& A, int shape, bool want32, double 
     A.has_win = true;
 }
 
-template <typename KeyT>
-int radix_sort_pairs_t(KeyT*           k0,
-                       int32_t*        v0,
-                       KeyT*           k1,
-                       int32_t*        v1,
-                       const int64_t*  d_n,
-                       int64_t         nmax,
-                       int             bits,
-                       DBuf<uint32_t>& hist,
-                       DBuf<uint32_t>& chunk,
-                       hipStream_t     s)
+// stable LSD passes of 8 bits; the first reads `first` (pairs or triplets),
+// the rest ping-pong between the buffers; returns the buffer holding the result
+template <typename KeyT, typename First>
+int radix_sort_from(First           first,
+                    KeyT*           k0,
+                    int32_t*        v0,
+                    KeyT*           k1,
+                    int32_t*        v1,
+                    const int64_t*  d_n,
+                    int64_t         nmax,
+                    int             bits,
+                    DBuf<uint32_t>& hist,
+                    DBuf<uint32_t>& chunk,
+                    hipStream_t     s)
 {
     const int ntiles = ceil_div(nmax, kTile);
     if(ntiles == 0)
@@ -999,13 +1009,35 @@ int radix_sort_pairs_t(KeyT*           k0,
     {
         int      w     = (bits - shift) < 8 ? (bits - shift) : 8;
         uint32_t dmask = (1u << w) - 1u;
-        rs_hist_kernel<KeyT><<<ntiles, kThreads, 0, s>>>(d_n, kb[cur], shift, dmask, hist.data(), ntiles);
-        scan_exclusive(hist.data(), (int64_t)256 * ntiles, chunk, nullptr, s);
-        rs_scatter_kernel<KeyT><<<ntiles, kThreads, 0, s>>>(d_n, kb[cur], vb[cur], kb[cur ^ 1], vb[cur ^ 1], shift,
-                                                            dmask, hist.data(), ntiles);
+        auto     pass  = [&](auto src) {
+            using Src = decltype(src);
+            rs_hist_kernel<KeyT, Src><<<ntiles, kThreads, 0, s>>>(src, shift, dmask, hist.data(), ntiles);
+            scan_exclusive(hist.data(), (int64_t)256 * ntiles, chunk, nullptr, s);
+            rs_scatter_kernel<KeyT, Src><<<ntiles, kThreads, 0, s>>>(src, kb[cur ^ 1], vb[cur ^ 1], shift, dmask,
+                                                                     hist.data(), ntiles);
+        };
+        if(shift == 0)
+            pass(first);
+        else
+            pass(PairSource<KeyT>{d_n, kb[cur], vb[cur]});
         cur ^= 1;
     }
     return cur;
+}
+
+template <typename KeyT>
+int radix_sort_pairs_t(KeyT*           k0,
+                       int32_t*        v0,
+                       KeyT*           k1,
+                       int32_t*        v1,
+                       const int64_t*  d_n,
+                       int64_t         nmax,
+                       int             bits,
+                       DBuf<uint32_t>& hist,
+                       DBuf<uint32_t>& chunk,
+                       hipStream_t     s)
+{
+    return radix_sort_from<KeyT>(PairSource<KeyT>{d_n, k0, v0}, k0, v0, k1, v1, d_n, nmax, bits, hist, chunk, s);
 }
 
 int radix_sort_pairs(uint64_t*       k0,
