@@ -209,7 +209,8 @@ int mi_spmv_dot_noalloc(mi_sys* sys, const double* x, double* y, double* d_dot, 
  * how many; dflt: the one systems with enough rows get; per shape (arrays of
  * count entries, any may be NULL): threads per workgroup, window rows, FP64
  * blocks per lane and stage, blocks per lane and load (2: paired 16-byte
- * FP64 / 8-byte FP32 loads). */
+ * FP64 / 8-byte FP32 loads; 4: 16-byte loads of four FP32 blocks, which an
+ * FP64 system runs as 2). */
 int mi_spmv_shapes(int32_t* count, int32_t* dflt, int32_t* threads, int32_t* rows, int32_t* per, int32_t* vec);
 enum
 {

@@ -935,11 +935,12 @@ void MatrixConverter::build_window(BsrMatrix& A, int shape, bool want32, double 
     scan_exclusive(cnt, (int64_t)B + 1, chunk_, counts_.data() + 4, s);
     win_range_kernel<<<std::max(1, std::min(A.wnchunks, device_cu_count() * 8)), kThreads, 0, s>>>(
         B, shift, A.wnchunks, A.wreal.data(), A.wchunk_lo.data(), A.wchunk_hi.data());
-    // the window layout holds at most the full BSR's 2U blocks; one block of
-    // padding past the last, and an even plane stride
+    // the window layout holds at most the full BSR's 2U blocks; four blocks of
+    // padding past the last, and a plane stride that is a multiple of 4 (the
+    // SpMV reads up to four consecutive blocks per load)
     const int64_t wmax = 2 * U;
-    A.wwords.resize((size_t)wmax + 2, ratio);
-    A.wstride = ((int64_t)A.wwords.cap + 1) & ~(int64_t)1;
+    A.wwords.resize((size_t)wmax + 4, ratio);
+    A.wstride = ((int64_t)A.wwords.cap + 3) & ~(int64_t)3;
     auto win_lower = [&](auto* wv) {
         using VT = std::remove_pointer_t<decltype(wv)>;
         if(narrow)

@@ -189,7 +189,7 @@ static int layout_shape(const mi_sys* sys)
     if(!spmv_use_window())
         return spmv_shape_full;
     if(sys->spmv_shape == spmv_shape_auto)
-        return spmv_auto_shape(sys->desc.block_rows);
+        return spmv_auto_shape(sys->desc.block_rows, sys->desc.value_precision == MI_FP32);
     return sys->spmv_shape;
 }
 
@@ -803,6 +803,7 @@ int mi_spmv_ordered(mi_sys* sys, const double* x, double* y, double* d_dot, void
         sum_partials(sys->spmv_scr.partials.data(), np, d_dot, s);
     MI_API_END
 }
+}  // extern "C"
 
 namespace
 {
@@ -830,6 +831,7 @@ std::unique_ptr<LocalPreconditioner> build_precond(mi_sys* sys, const mi_precond
 }
 }  // namespace
 
+extern "C" {
 int mi_precond_add(mi_sys* sys, const mi_precond_desc* d, int* handle)
 {
     MI_API_BEGIN
@@ -1214,6 +1216,7 @@ int mi_produce_contact(mi_ctx*        ctx,
     produce_contact(n_stencils, stencils, g, kappa, dt, vert_offset, rows, cols, vals, pick(ctx, stream));
     MI_API_END
 }
+}  // extern "C"
 
 namespace
 {
@@ -1225,6 +1228,8 @@ void write_int_array(FILE* fp, const char* key, const int32_t* a, int64_t n, boo
     std::fprintf(fp, "]%s\n", last ? "" : ",");
 }
 }  // namespace
+
+extern "C" {
 
 // The files of MASPreconditionerEngine::dump_cluster_matrices_debug
 // (finite_element/mas_preconditioner_engine.cu:3154-3301):

@@ -243,9 +243,9 @@ int  spmv_shape_shift(int shape);
 // the layout codes beside the window shapes (mi_sys_set_spmv_shape)
 constexpr int spmv_shape_auto = -1;  // chosen by the system's size
 constexpr int spmv_shape_full = -2;  // the full BSR
-// the layout a system of B block rows gets when none is set: a window shape
-// index, or spmv_shape_full
-int spmv_auto_shape(int64_t B);
+// the layout a system of B block rows gets when none is set (use32: FP32
+// value planes): a window shape index, or spmv_shape_full
+int spmv_auto_shape(int64_t B, bool use32);
 // workgroup size of the SpMV kernel that streams A's layout
 int spmv_threads(const BsrMatrix& A);
 // the SpMV's work items (windows or chunks) and the row range [lo, hi) each touches

@@ -379,7 +379,9 @@ __device__ __forceinline__ void win_block(int32_t wd,
 }
 
 typedef int    int2v __attribute__((ext_vector_type(2)));
+typedef int    int4v __attribute__((ext_vector_type(4)));
 typedef float  float2v __attribute__((ext_vector_type(2)));
+typedef float  float4v __attribute__((ext_vector_type(4)));
 typedef double double2v __attribute__((ext_vector_type(2)));
 template <typename VT>
 struct Vec2;
@@ -395,10 +397,12 @@ struct Vec2<float>
 };
 
 // VEC blocks per lane and load: with VEC = 2 a lane reads two consecutive
-// blocks with 8-byte (FP32) or 16-byte (FP64) loads; a stage's slots then
-// start at an even block (the first slot may precede the stage, the last
-// may pass the window: both are masked; the plane stride is even and the
-// arrays hold two blocks of padding past the last)
+// blocks with 8-byte (FP32) or 16-byte (FP64) loads, with VEC = 4 four FP32
+// blocks with 16-byte loads (where the stage holds a multiple of four blocks
+// per lane, else two); a stage's slots then start at a multiple of VEC (the
+// first slot may precede the stage, the last may pass the window: both are
+// masked; the plane stride is a multiple of 4 and the arrays hold four
+// blocks of padding past the last)
 template <typename VT, bool ACCUM, bool ORD, int NT, int KW, int PER, int VEC>
 __global__ __launch_bounds__(NT) void spmv_win_kernel(int32_t nchunks,
                                                       int32_t B,
@@ -415,7 +419,7 @@ __global__ __launch_bounds__(NT) void spmv_win_kernel(int32_t nchunks,
                                                       const int32_t* __restrict__ active,
                                                       const int64_t* d_nactive)
 {
-    static_assert(VEC == 1 || VEC == 2, "one or two blocks per load");
+    static_assert(VEC == 1 || VEC == 2 || VEC == 4, "one, two or four blocks per load");
     static_assert(!(ORD && ACCUM), "row-order products overwrite y");
     if(d_conv && *d_conv != 0)
         return;
@@ -423,7 +427,8 @@ __global__ __launch_bounds__(NT) void spmv_win_kernel(int32_t nchunks,
     constexpr int kPer        = WinStage<VT, PER>::kPer;
     constexpr int kStage      = kPer * NT;
     constexpr int kGroupsW    = NT / kLanes;
-    static_assert(kPer % VEC == 0, "whole pairs per lane");
+    constexpr int kVec        = (VEC == 4 && (kPer % 4 != 0 || sizeof(VT) != 4)) ? 2 : VEC;
+    static_assert(kPer % kVec == 0, "whole load groups per lane");
     __shared__ double  ys[3][KW];
     __shared__ double  xs[3][KW];
     __shared__ int32_t rp[KW + 1];
@@ -460,8 +465,8 @@ __global__ __launch_bounds__(NT) void spmv_win_kernel(int32_t nchunks,
             rp[nr] = wptr[R0 + nr];
         __syncthreads();
         const int32_t K0 = rp[0], K1 = rp[nr];
-        // the stage's blocks in registers: lane slot j = VEC * h + e holds
-        // block o0 + VEC * (h * NT + tid) + e.  The next stage's loads are
+        // the stage's blocks in registers: lane slot j = kVec * h + e holds
+        // block o0 + kVec * (h * NT + tid) + e.  The next stage's loads are
         // issued as soon as this stage's products are out, so they are in
         // flight through the barriers and the row sums.
         int32_t wd[kPer];
@@ -469,10 +474,10 @@ __global__ __launch_bounds__(NT) void spmv_win_kernel(int32_t nchunks,
         auto    load_stage = [&](int32_t o0, int32_t s0, int32_t s1)
         {
 #pragma unroll
-            for(int h = 0; h < kPer / VEC; ++h)
+            for(int h = 0; h < kPer / kVec; ++h)
             {
-                const int32_t k = o0 + VEC * (h * NT + tid);
-                if constexpr(VEC == 1)
+                const int32_t k = o0 + kVec * (h * NT + tid);
+                if constexpr(kVec == 1)
                 {
                     if(k < s1)
                     {
@@ -482,7 +487,7 @@ __global__ __launch_bounds__(NT) void spmv_win_kernel(int32_t nchunks,
                             A[h][q] = stream_load<VT>(&vals[q * stride + k]);
                     }
                 }
-                else
+                else if constexpr(kVec == 2)
                 {
                     using V2 = typename Vec2<VT>::T;
                     if((k >= s0 && k < s1) || k + 1 < s1)
@@ -499,6 +504,27 @@ __global__ __launch_bounds__(NT) void spmv_win_kernel(int32_t nchunks,
                         }
                     }
                 }
+                else
+                {
+                    // k >= o0 >= s0 - 3: one of the four blocks is in the stage when k < s1
+                    if(k < s1)
+                    {
+                        const int4v w4 = stream_load<VT>(reinterpret_cast<const int4v*>(&words[k]));
+                        wd[4 * h]      = w4.x;
+                        wd[4 * h + 1]  = w4.y;
+                        wd[4 * h + 2]  = w4.z;
+                        wd[4 * h + 3]  = w4.w;
+#pragma unroll
+                        for(int q = 0; q < 9; ++q)
+                        {
+                            const float4v a4 = stream_load<VT>(reinterpret_cast<const float4v*>(&vals[q * stride + k]));
+                            A[4 * h][q]      = a4.x;
+                            A[4 * h + 1][q]  = a4.y;
+                            A[4 * h + 2][q]  = a4.z;
+                            A[4 * h + 3][q]  = a4.w;
+                        }
+                    }
+                }
             }
         };
         auto compute_stage = [&](int32_t o0, int32_t s0, int32_t s1)
@@ -506,14 +532,14 @@ __global__ __launch_bounds__(NT) void spmv_win_kernel(int32_t nchunks,
 #pragma unroll
             for(int j = 0; j < kPer; ++j)
             {
-                const int     i = VEC * ((j / VEC) * NT + tid) + j % VEC;
+                const int     i = kVec * ((j / kVec) * NT + tid) + j % kVec;
                 const int32_t k = o0 + i;
                 if(k >= s0 && k < s1)
                     win_block<VT, W, ORD, KW, kStage>(wd[j], A[j], x, real, xs, ys, w, i);
             }
         };
         int32_t s0 = K0;
-        int32_t o0 = s0 & ~(VEC - 1);  // slots [o0, o0 + kStage) cover the stage's blocks [s0, s1)
+        int32_t o0 = s0 & ~(kVec - 1);  // slots [o0, o0 + kStage) cover the stage's blocks [s0, s1)
         int32_t s1 = o0 + kStage < K1 ? o0 + kStage : K1;
         if(s0 < K1)
             load_stage(o0, s0, s1);
@@ -521,7 +547,7 @@ __global__ __launch_bounds__(NT) void spmv_win_kernel(int32_t nchunks,
         {
             compute_stage(o0, s0, s1);
             const int32_t n0  = s1;
-            const int32_t no0 = n0 & ~(VEC - 1);
+            const int32_t no0 = n0 & ~(kVec - 1);
             const int32_t n1  = no0 + kStage < K1 ? no0 + kStage : K1;
             if(n0 < K1)
                 load_stage(no0, n0, n1);
@@ -620,7 +646,7 @@ struct WinShape
 #define MI_WIN_SHAPES(X)                                                                              \
     X(0, 512, 9, 2, 2) X(1, 256, 8, 3, 1) X(2, 512, 9, 2, 1) X(3, 256, 8, 2, 2) X(4, 256, 8, 2, 1) \
     X(5, 512, 9, 1, 1) X(6, 1024, 10, 1, 1) X(7, 256, 7, 2, 1) X(8, 256, 8, 4, 2) X(9, 512, 9, 4, 2) \
-    X(10, 1024, 9, 2, 2)
+    X(10, 1024, 9, 2, 2) X(11, 512, 9, 2, 4)  /* 11: FP32 systems' 512-row default */
 #define MI_WIN_ENTRY(i, nt, sh, per, vec) {nt, sh, per, vec},
 constexpr WinShape kWinShapes[] = {MI_WIN_SHAPES(MI_WIN_ENTRY)};
 #undef MI_WIN_ENTRY
@@ -777,12 +803,13 @@ int spmv_default_shape()
 // windows per CU (one workgroup streams a window, so fewer windows leave
 // CUs idle: 8 windows of 512 rows ran a 4,096-row system on 8 CUs), else the
 // full BSR (spmv_shape_full)
-int spmv_auto_shape(int64_t B)
+int spmv_auto_shape(int64_t B, bool use32)
 {
     if(env_shape_index() >= 0)
         return env_shape_index();
     const int64_t need   = (3 * (int64_t)device_cu_count() + 1) / 2;
-    const int     cand[] = {default_shape_index(), 3, 7};
+    // FP32 value planes: the 512-row shape with four blocks per 16-byte load
+    const int     cand[] = {use32 ? 11 : default_shape_index(), 3, 7};
     for(int k : cand)
         if((B + (int64_t(1) << kWinShapes[k].shift) - 1) >> kWinShapes[k].shift >= need)
             return k;

@@ -169,6 +169,7 @@ class LinearSystem:
         self.lib = ctx.lib
         self.B = int(block_rows)
         self.n = 3 * self.B
+        self.precision = int(precision)
         d = abi.SysDesc(self.B, reserve_ratio, precision)
         h = C.c_void_p()
         check(self.lib.mi_sys_create(ctx.h, C.byref(d), C.byref(h)))

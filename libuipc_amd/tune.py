@@ -90,7 +90,15 @@ def tune_spmv_shape(ls: LinearSystem, x, reps: int = 20, shapes=None, margin: fl
     info = spmv_shapes(ls.lib)["shapes"]
     lay0 = ls.spmv_layout()
     dflt = lay0["shape"] if lay0["layout"] == "window" else MI_SPMV_SHAPE_FULL  # the system's own layout
-    cand = [MI_SPMV_SHAPE_FULL] + list(range(len(info))) if shapes is None else [int(k) for k in shapes]
+    from .abi import MI_FP64
+
+    if shapes is None:
+        # an FP64 system runs a four-blocks-per-load shape as its two-block
+        # twin (16-byte loads hold two FP64 values): not a layout of its own
+        cand = [MI_SPMV_SHAPE_FULL] + [k for k in range(len(info))
+                                       if not (ls.precision == MI_FP64 and info[k]["vec"] == 4)]
+    else:
+        cand = [int(k) for k in shapes]
     ctx = ls.ctx
     stream = ctx.stream()
     y = ctx.alloc(ls.n, np.float64)

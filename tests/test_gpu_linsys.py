@@ -243,7 +243,9 @@ def test_spmv_every_launch_shape(ctx, c1_contact, precision):
         yo = yd.numpy().reshape(-1, 3)[pos].ravel()
         assert np.abs(yo - yref).max() <= tol * np.abs(yref).max(), k
     t = tune_spmv_shape(ls, xd, reps=3)
-    assert sorted(t["table"]) == [abi.MI_SPMV_SHAPE_FULL] + list(range(len(info["shapes"])))
+    # an FP64 system runs a four-block-load shape as its two-block twin: not timed
+    want = [k for k, sh in enumerate(info["shapes"]) if not (precision == abi.MI_FP64 and sh["vec"] == 4)]
+    assert sorted(t["table"]) == [abi.MI_SPMV_SHAPE_FULL] + want
     assert ls.spmv_layout()["window_rows"] == t["table"][t["best"]]["rows"]
     with pytest.raises(abi.MiError):
         ls.set_spmv_shape(len(info["shapes"]))
