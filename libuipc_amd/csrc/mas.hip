@@ -421,34 +421,28 @@ __device__ __forceinline__ void agg_atomic_add(double* hess, int64_t dst, const 
             atomic_add_f64(hess + dst + q, v[q]);
 }
 
-// one thread per upper block (see the file comment)
-__global__ __launch_bounds__(256) void mas_walk_kernel(int64_t        U,
-                                                      const int32_t* __restrict__ rows,
-                                                      const int32_t* __restrict__ cols,
-                                                      const double* __restrict__ vals,
-                                                      int32_t        blk_off,
-                                                      int32_t        V,
-                                                      const int32_t* __restrict__ real_to_part,
-                                                      const int32_t* __restrict__ anc,  // (L-1) x V
-                                                      int            L,
-                                                      double*        hess)
+// One upper block (r, c) of the MAS's range (local rows, sr / sc their
+// partition slots; live = a block to walk, every lane of the wave calls this):
+// level 0 by a plain store (each cluster entry has this one source, scatter
+// k1, mas_preconditioner_engine.cu:530-682), the coarse levels' off-diagonal
+// in-bank contributions until the ancestors merge by wave-aggregated atomics
+// -- level 1 into the caller's LDS copy of its level-1 cluster instead when
+// l1 is given (lds_base: that cluster's first entry in hess).
+__device__ __forceinline__ void walk_block(bool           live,
+                                           int64_t        u,
+                                           int32_t        r,
+                                           int32_t        c,
+                                           int32_t        sr,
+                                           int32_t        sc,
+                                           const double* __restrict__ vals,
+                                           int32_t        V,
+                                           const int32_t* __restrict__ anc,
+                                           int            L,
+                                           double*        hess,
+                                           double*        l1,
+                                           int64_t        lds_base)
 {
-    const int64_t u    = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    bool          live = u < U;
-    int32_t       r = 0, c = 0, sr = -1, sc = -1;
-    double        H[9];
-    if(live)
-    {
-        r = rows[u] - blk_off;
-        c = cols[u] - blk_off;
-        live = r >= 0 && r < V && c >= 0 && c < V;
-    }
-    if(live)
-    {
-        sr   = real_to_part[r];
-        sc   = real_to_part[c];
-        live = sr >= 0 && sc >= 0;
-    }
+    double H[9];
     if(live)
     {
 #pragma unroll
@@ -523,8 +517,133 @@ __global__ __launch_bounds__(256) void mas_walk_kernel(int64_t        U,
         }
         if(__ballot(live) == 0)
             break;
-        agg_atomic_add(hess, dst, v);
+        if(l == 1 && l1)
+        {
+            // the workgroup's own level-1 cluster: LDS atomics (an entry
+            // outside it, which the build's check rules out, goes to memory)
+            const int64_t e = dst - lds_base;
+            if(dst >= 0 && e >= 0 && e < kCm)
+#pragma unroll
+                for(int q = 0; q < 9; ++q)
+                    atomicAdd(&l1[e + q], v[q]);
+            else if(dst >= 0)
+#pragma unroll
+                for(int q = 0; q < 9; ++q)
+                    atomic_add_f64(hess + dst + q, v[q]);
+        }
+        else
+            agg_atomic_add(hess, dst, v);
     }
+}
+
+// one thread per upper block (see the file comment)
+__global__ __launch_bounds__(256) void mas_walk_kernel(int64_t        U,
+                                                      const int32_t* __restrict__ rows,
+                                                      const int32_t* __restrict__ cols,
+                                                      const double* __restrict__ vals,
+                                                      int32_t        blk_off,
+                                                      int32_t        V,
+                                                      const int32_t* __restrict__ real_to_part,
+                                                      const int32_t* __restrict__ anc,  // (L-1) x V
+                                                      int            L,
+                                                      double*        hess)
+{
+    const int64_t u    = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool          live = u < U;
+    int32_t       r = 0, c = 0, sr = -1, sc = -1;
+    if(live)
+    {
+        r = rows[u] - blk_off;
+        c = cols[u] - blk_off;
+        live = r >= 0 && r < V && c >= 0 && c < V;
+    }
+    if(live)
+    {
+        sr   = real_to_part[r];
+        sc   = real_to_part[c];
+        live = sr >= 0 && sc >= 0;
+    }
+    walk_block(live, u, r, c, sr, sc, vals, V, anc, L, hess, nullptr, 0);
+}
+
+// The same walk in the partition order, one workgroup per 256-slot range
+// whose parts make up one level-1 bank (the build checks this): every
+// contribution to that level-1 cluster comes from the range's own rows, so
+// the workgroup adds them in LDS and stores the cluster once, where the
+// block-order walk spent most of its time in global atomics at level 1.
+// Blocks are taken row by row (the upper blocks of each slot's vertex), each
+// exactly once, as in the block-order walk.
+__global__ __launch_bounds__(256) void mas_walk_range_kernel(const int32_t* __restrict__ row_ptr,  // upper BCOO, global rows
+                                                            const int32_t* __restrict__ cols,
+                                                            const double* __restrict__ vals,
+                                                            int32_t        blk_off,
+                                                            int32_t        V,
+                                                            const int32_t* __restrict__ part_to_real,
+                                                            const int32_t* __restrict__ real_to_part,
+                                                            const int32_t* __restrict__ anc,
+                                                            int            L,
+                                                            const int32_t* __restrict__ range_bank,
+                                                            int64_t        c1_first,
+                                                            int32_t        nslots,
+                                                            double*        hess)
+{
+    __shared__ double   l1[kCm];
+    __shared__ uint32_t off[256 + 1];
+    __shared__ uint32_t lds[256 / kWave + 1];
+    const int     t     = threadIdx.x;
+    const int32_t bank  = range_bank[blockIdx.x];
+    const int64_t base1 = (c1_first + (bank < 0 ? 0 : bank)) * kCm;
+    for(int k = t; k < kCm; k += 256)
+        l1[k] = 0.0;
+    // the range's rows and their upper-block counts
+    const int32_t slot = blockIdx.x * 256 + t;
+    const int32_t r    = slot < nslots ? part_to_real[slot] : -1;
+    const int32_t b0   = r >= 0 ? row_ptr[blk_off + r] : 0;
+    const uint32_t cnt = r >= 0 ? (uint32_t)(row_ptr[blk_off + r + 1] - b0) : 0u;
+    uint32_t      total;
+    const uint32_t o = block_exclusive_scan<256>(cnt, lds, &total);  // its barriers also order the l1 zeros
+    off[t]           = o;
+    if(t == 0)
+        off[256] = total;
+    __shared__ int32_t rs[256], bs[256];
+    rs[t] = r;
+    bs[t] = b0;
+    __syncthreads();
+    for(uint32_t i0 = 0; i0 < total; i0 += 256)
+    {
+        const uint32_t i    = i0 + t;
+        bool           live = i < total;
+        int32_t        rr = 0, c = 0, sr = -1, sc = -1;
+        int64_t        u = 0;
+        if(live)
+        {
+            // the slot holding flattened block i: the last offset <= i
+            int lo = 0, hi = 255;
+            while(lo < hi)
+            {
+                const int mid = (lo + hi + 1) >> 1;
+                if(off[mid] <= i)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            rr = rs[lo];
+            u  = (int64_t)bs[lo] + (i - off[lo]);
+            c  = cols[u] - blk_off;
+            live = c >= 0 && c < V;
+        }
+        if(live)
+        {
+            sr   = real_to_part[rr];
+            sc   = real_to_part[c];
+            live = sr >= 0 && sc >= 0;
+        }
+        walk_block(live, u, rr, c, sr, sc, vals, V, anc, L, hess, l1, base1);
+    }
+    __syncthreads();
+    if(bank >= 0)
+        for(int k = t; k < kCm; k += 256)
+            hess[base1 + k] = l1[k];
 }
 
 // push(level): for every level-`level` cluster (one wave each), the full sum of
@@ -1824,16 +1943,30 @@ class MasPreconditioner final : public LocalPreconditioner
             return;
         fill_f64_kernel<<<grid_for(C * kCm), 256, 0, s>>>(d_hess.data(), C * kCm);
         if(A.U > 0)
-            mas_walk_kernel<<<ceil_div(A.U, 256), 256, 0, s>>>(A.U,
-                                                               A.rows.data(),
-                                                               A.cols.data(),
-                                                               A.vals.data(),
-                                                               blk_off,
-                                                               h.V,
-                                                               d_real_to_part.data(),
-                                                               d_anc.data(),
-                                                               h.L,
-                                                               d_hess.data());
+        {
+            // in the partition order when every level-1 bank is one 256-slot
+            // range (its cluster then summed in LDS); MI_MAS_WALK=blocks keeps
+            // the block-order walk
+            static const bool by_blocks = [] {
+                const char* e = std::getenv("MI_MAS_WALK");
+                return e && std::string(e) == "blocks";
+            }();
+            if(l1_local_ && !by_blocks)
+                mas_walk_range_kernel<<<ceil_div(h.N, 256), 256, 0, s>>>(
+                    A.row_ptr.data(), A.cols.data(), A.vals.data(), blk_off, h.V, d_part_to_real.data(),
+                    d_real_to_part.data(), d_anc.data(), h.L, d_range_bank.data(), c_first, h.N, d_hess.data());
+            else
+                mas_walk_kernel<<<ceil_div(A.U, 256), 256, 0, s>>>(A.U,
+                                                                   A.rows.data(),
+                                                                   A.cols.data(),
+                                                                   A.vals.data(),
+                                                                   blk_off,
+                                                                   h.V,
+                                                                   d_real_to_part.data(),
+                                                                   d_anc.data(),
+                                                                   h.L,
+                                                                   d_hess.data());
+        }
         for(int l = 0; l + 1 < h.L; ++l)
         {
             auto [first, cnt] = level_clusters[l];
