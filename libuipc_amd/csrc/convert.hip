@@ -455,40 +455,72 @@ __global__ __launch_bounds__(kThreads) void rle_write_kernel(const int64_t*  d_m
     }
 }
 
+// one 72-byte FP64 triplet record (index src) by five 16-byte buffer loads
+// from the 16-byte boundary at or below it (odd records start 8 bytes past
+// one); the buffer's range check returns zeros past the array's end
+__device__ __forceinline__ void load_record64(__amdgpu_buffer_rsrc_t rs, int32_t src, double (&v)[9])
+{
+    typedef double d2v __attribute__((ext_vector_type(2)));
+    const uint32_t off = 72u * (uint32_t)src;
+    const uint32_t b   = off & ~15u;
+    const d2v      t0  = __builtin_bit_cast(d2v, __builtin_amdgcn_raw_buffer_load_b128(rs, b, 0, 0));
+    const d2v      t1  = __builtin_bit_cast(d2v, __builtin_amdgcn_raw_buffer_load_b128(rs, b + 16u, 0, 0));
+    const d2v      t2  = __builtin_bit_cast(d2v, __builtin_amdgcn_raw_buffer_load_b128(rs, b + 32u, 0, 0));
+    const d2v      t3  = __builtin_bit_cast(d2v, __builtin_amdgcn_raw_buffer_load_b128(rs, b + 48u, 0, 0));
+    const d2v      t4  = __builtin_bit_cast(d2v, __builtin_amdgcn_raw_buffer_load_b128(rs, b + 64u, 0, 0));
+    const bool     odd = (off & 8u) != 0;
+    v[0]               = odd ? t0.y : t0.x;
+    v[1]               = odd ? t1.x : t0.y;
+    v[2]               = odd ? t1.y : t1.x;
+    v[3]               = odd ? t2.x : t1.y;
+    v[4]               = odd ? t2.y : t2.x;
+    v[5]               = odd ? t3.x : t2.y;
+    v[6]               = odd ? t3.y : t3.x;
+    v[7]               = odd ? t4.x : t3.y;
+    v[8]               = odd ? t4.y : t4.x;
+}
+
 // ---- segmented reduce: one thread per unique block, duplicates summed in
 // FP64 in ascending source order (bit-identical to a sequential left fold of
 // the values as doubles).  The gathers of a group of 4 are issued before
-// their (ordered) additions.  TV: the triplet values' type.
-template <typename TV>
+// their (ordered) additions.  TV: the triplet values' type;
+// kWide: FP64 records by 16-byte loads (T records, 72 T < 2^32).
+template <typename TV, bool kWide = false>
 __global__ __launch_bounds__(kThreads) void seg_reduce_kernel(int64_t        U,
                                                              const int64_t* d_m,
                                                              const int32_t* seg_start,
                                                              const int32_t* srcs,
                                                              const TV*      vals,
-                                                             double*        out)
+                                                             double*        out,
+                                                             int64_t        T = 0)
 {
     const int64_t m = *d_m;
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc((void*)vals, 0, kWide ? (int)(uint32_t)(72 * T) : 0, 0x00020000);
+    auto load = [&](int32_t src, double (&v)[9]) {
+        if constexpr(kWide)
+            load_record64(rs, src, v);
+        else
+        {
+            const TV* p = vals + 9 * (int64_t)src;
+#pragma unroll
+            for(int q = 0; q < 9; ++q)
+                v[q] = (double)p[q];
+        }
+    };
     for(int64_t u = (int64_t)blockIdx.x * kThreads + threadIdx.x; u < U; u += (int64_t)gridDim.x * kThreads)
     {
         const int64_t b = seg_start[u];
         const int64_t e = (u + 1 < U) ? (int64_t)seg_start[u + 1] : m;
         double        acc[9];
-        const TV*     v0 = vals + 9 * (int64_t)srcs[b];
-#pragma unroll
-        for(int q = 0; q < 9; ++q)
-            acc[q] = (double)v0[q];
+        load(srcs[b], acc);
         int64_t k = b + 1;
         for(; k + 4 <= e; k += 4)
         {
             double t[4][9];
 #pragma unroll
             for(int g = 0; g < 4; ++g)
-            {
-                const TV* v = vals + 9 * (int64_t)srcs[k + g];
-#pragma unroll
-                for(int q = 0; q < 9; ++q)
-                    t[g][q] = (double)v[q];
-            }
+                load(srcs[k + g], t[g]);
 #pragma unroll
             for(int g = 0; g < 4; ++g)
 #pragma unroll
@@ -497,10 +529,11 @@ __global__ __launch_bounds__(kThreads) void seg_reduce_kernel(int64_t        U,
         }
         for(; k < e; ++k)
         {
-            const TV* v = vals + 9 * (int64_t)srcs[k];
+            double t[9];
+            load(srcs[k], t);
 #pragma unroll
             for(int q = 0; q < 9; ++q)
-                acc[q] = acc[q] + (double)v[q];
+                acc[q] = acc[q] + t[q];
         }
 #pragma unroll
         for(int q = 0; q < 9; ++q)
@@ -868,6 +901,19 @@ MatrixConverter::~MatrixConverter()
         (void)hipHostFree(pinned_);
 }
 
+template <typename KeyT, typename First>
+int radix_sort_from(First           first,
+                    KeyT*           k0,
+                    int32_t*        v0,
+                    KeyT*           k1,
+                    int32_t*        v1,
+                    const int64_t*  d_n,
+                    int64_t         nmax,
+                    int             bits,
+                    DBuf<uint32_t>& hist,
+                    DBuf<uint32_t>& chunk,
+                    hipStream_t     s);
+
 // compaction + stable radix sort of (key, source) pairs; returns the buffer
 // index holding the result, *d_count = number of kept entries (device)
 template <int MODE, typename KeyT>
@@ -1164,6 +1210,9 @@ void MatrixConverter::convert(int32_t        B,
     if(U > 0 && vals32)
         seg_reduce_kernel<float><<<grid_of(U), kThreads, 0, s>>>(
             U, d_m, seg_start_.data(), srcs_[cur].data(), static_cast<const float*>(vals), out.vals.data());
+    else if(U > 0 && 72 * T + 16 < ((int64_t)1 << 32))
+        seg_reduce_kernel<double, true><<<grid_of(U), kThreads, 0, s>>>(
+            U, d_m, seg_start_.data(), srcs_[cur].data(), static_cast<const double*>(vals), out.vals.data(), T);
     else if(U > 0)
         seg_reduce_kernel<double><<<grid_of(U), kThreads, 0, s>>>(
             U, d_m, seg_start_.data(), srcs_[cur].data(), static_cast<const double*>(vals), out.vals.data());
