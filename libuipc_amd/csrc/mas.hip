@@ -566,13 +566,103 @@ __global__ __launch_bounds__(256) void mas_walk_kernel(int64_t        U,
     walk_block(live, u, r, c, sr, sc, vals, V, anc, L, hess, nullptr, 0);
 }
 
+
+// One round of the push for four clusters, one per wave (k; live: a
+// cluster to do): the full sum of each group of its nodes that share a
+// parent -> the parent's diagonal block.  A parent in the cluster whose
+// entries start at hess[l1_base] is written to l1 (LDS) when l1 is given.
+// Every thread of the workgroup calls this (it holds barriers).
+__device__ __forceinline__ void push_round(int64_t        k,
+                                           bool           live,
+                                           const int32_t* __restrict__ parent,
+                                           double*        hess,
+                                           double (*S)[kBank][9],
+                                           int32_t (*P)[kBank],
+                                           double*        l1,
+                                           int64_t        l1_base)
+{
+    const int w    = threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    if(lane < kBank)
+    {
+        P[w][lane] = live ? parent[k * kBank + lane] : -1;
+#pragma unroll
+        for(int q = 0; q < 9; ++q)
+            S[w][lane][q] = 0.0;
+    }
+    __syncthreads();
+    if(live)
+    {
+        for(int blk = lane; blk < kSym; blk += kWave)
+        {
+            // blk -> (a, b), a <= b
+            int a = 0, rem = blk;
+            while(rem >= kBank - a)
+            {
+                rem -= kBank - a;
+                ++a;
+            }
+            int     b  = a + rem;
+            int32_t pa = P[w][a];
+            if(pa < 0 || pa != P[w][b])
+                continue;
+            int rep = 0;
+            while(P[w][rep] != pa)
+                ++rep;
+            const double* M = hess + k * kCm + 9 * blk;
+            for(int i = 0; i < 3; ++i)
+                for(int j = 0; j < 3; ++j)
+                {
+                    double val = M[i + 3 * j];
+                    if(a != b)
+                        val += M[j + 3 * i];
+                    atomicAdd(&S[w][rep][i + 3 * j], val);
+                }
+        }
+    }
+    __syncthreads();
+    if(live && lane < kBank)
+    {
+        int32_t pa = P[w][lane];
+        bool    first = pa >= 0;
+        for(int q = 0; q < lane && first; ++q)
+            first = P[w][q] != pa;
+        if(first)
+        {
+            const int64_t d = (int64_t)(pa / kBank) * kCm + 9 * sym_index(pa % kBank, pa % kBank);
+            double*       D = (l1 && d >= l1_base && d < l1_base + kCm) ? l1 + (d - l1_base) : hess + d;
+#pragma unroll
+            for(int q = 0; q < 9; ++q)
+                D[q] = S[w][lane][q];
+        }
+    }
+    __syncthreads();  // S and P are reused by the next round
+}
+
+// push(level): for every level-`level` cluster (one wave each), the full sum of
+// each group of its nodes that share a parent -> the parent's diagonal block
+__global__ __launch_bounds__(256) void mas_push_kernel(int64_t        c_first,
+                                                      int64_t        c_count,
+                                                      const int32_t* __restrict__ parent,
+                                                      double*        hess)
+{
+    __shared__ double  S[4][kBank][9];
+    __shared__ int32_t P[4][kBank];
+    const int     w    = threadIdx.x / kWave;
+    const int64_t k    = c_first + (int64_t)blockIdx.x * 4 + w;
+    const bool    live = (int64_t)blockIdx.x * 4 + w < c_count;
+    push_round(k, live, parent, hess, S, P, nullptr, 0);
+}
+
 // The same walk in the partition order, one workgroup per 256-slot range
 // whose parts make up one level-1 bank (the build checks this): every
 // contribution to that level-1 cluster comes from the range's own rows, so
 // the workgroup adds them in LDS and stores the cluster once, where the
 // block-order walk spent most of its time in global atomics at level 1.
 // Blocks are taken row by row (the upper blocks of each slot's vertex), each
-// exactly once, as in the block-order walk.
+// exactly once, as in the block-order walk; the level-0 push of the range's
+// parts follows, into the same LDS cluster (the push kernel then starts at
+// level 1).
 __global__ __launch_bounds__(256) void mas_walk_range_kernel(const int32_t* __restrict__ row_ptr,  // upper BCOO, global rows
                                                             const int32_t* __restrict__ cols,
                                                             const double* __restrict__ vals,
@@ -585,6 +675,7 @@ __global__ __launch_bounds__(256) void mas_walk_range_kernel(const int32_t* __re
                                                             const int32_t* __restrict__ range_bank,
                                                             int64_t        c1_first,
                                                             int32_t        nslots,
+                                                            const int32_t* __restrict__ parent,
                                                             double*        hess)
 {
     __shared__ double   l1[kCm];
@@ -640,77 +731,21 @@ __global__ __launch_bounds__(256) void mas_walk_range_kernel(const int32_t* __re
         }
         walk_block(live, u, rr, c, sr, sc, vals, V, anc, L, hess, l1, base1);
     }
+    // the level-0 push of the range's 16 parts (whose clusters this workgroup
+    // has just stored; the barriers order those stores before the reads):
+    // each level-1 node's diagonal block, into the LDS cluster
+    __shared__ double  S[4][kBank][9];
+    __shared__ int32_t P[4][kBank];
     __syncthreads();
+    const int64_t c0 = nslots / kBank;  // level-0 clusters
+    for(int round = 0; round < 4; ++round)
+    {
+        const int64_t k = (int64_t)blockIdx.x * kBank + round * 4 + t / kWave;
+        push_round(k, k < c0, parent, hess, S, P, l1, base1);
+    }
     if(bank >= 0)
         for(int k = t; k < kCm; k += 256)
             hess[base1 + k] = l1[k];
-}
-
-// push(level): for every level-`level` cluster (one wave each), the full sum of
-// each group of its nodes that share a parent -> the parent's diagonal block
-__global__ __launch_bounds__(256) void mas_push_kernel(int64_t        c_first,
-                                                      int64_t        c_count,
-                                                      const int32_t* __restrict__ parent,
-                                                      double*        hess)
-{
-    __shared__ double  S[4][kBank][9];
-    __shared__ int32_t P[4][kBank];
-    const int     w    = threadIdx.x / kWave;
-    const int     lane = threadIdx.x % kWave;
-    const int64_t k    = c_first + (int64_t)blockIdx.x * 4 + w;
-    const bool    live = (int64_t)blockIdx.x * 4 + w < c_count;
-    if(lane < kBank)
-    {
-        P[w][lane] = live ? parent[k * kBank + lane] : -1;
-#pragma unroll
-        for(int q = 0; q < 9; ++q)
-            S[w][lane][q] = 0.0;
-    }
-    __syncthreads();
-    if(live)
-    {
-        for(int blk = lane; blk < kSym; blk += kWave)
-        {
-            // blk -> (a, b), a <= b
-            int a = 0, rem = blk;
-            while(rem >= kBank - a)
-            {
-                rem -= kBank - a;
-                ++a;
-            }
-            int     b  = a + rem;
-            int32_t pa = P[w][a];
-            if(pa < 0 || pa != P[w][b])
-                continue;
-            int rep = 0;
-            while(P[w][rep] != pa)
-                ++rep;
-            const double* M = hess + k * kCm + 9 * blk;
-            for(int i = 0; i < 3; ++i)
-                for(int j = 0; j < 3; ++j)
-                {
-                    double val = M[i + 3 * j];
-                    if(a != b)
-                        val += M[j + 3 * i];
-                    atomicAdd(&S[w][rep][i + 3 * j], val);
-                }
-        }
-    }
-    __syncthreads();
-    if(live && lane < kBank)
-    {
-        int32_t pa = P[w][lane];
-        bool    first = pa >= 0;
-        for(int q = 0; q < lane && first; ++q)
-            first = P[w][q] != pa;
-        if(first)
-        {
-            double* D = hess + (int64_t)(pa / kBank) * kCm + 9 * sym_index(pa % kBank, pa % kBank);
-#pragma unroll
-            for(int q = 0; q < 9; ++q)
-                D[q] = S[w][lane][q];
-        }
-    }
 }
 
 // upper block index -> (row node a, column node b), a <= b (inverse of sym_index)
@@ -1942,6 +1977,7 @@ class MasPreconditioner final : public LocalPreconditioner
         if(C == 0)
             return;
         fill_f64_kernel<<<grid_for(C * kCm), 256, 0, s>>>(d_hess.data(), C * kCm);
+        bool range_walk = false;
         if(A.U > 0)
         {
             // in the partition order when every level-1 bank is one 256-slot
@@ -1951,10 +1987,12 @@ class MasPreconditioner final : public LocalPreconditioner
                 const char* e = std::getenv("MI_MAS_WALK");
                 return e && std::string(e) == "blocks";
             }();
-            if(l1_local_ && !by_blocks)
+            range_walk = l1_local_ && !by_blocks;
+            if(range_walk)
                 mas_walk_range_kernel<<<ceil_div(h.N, 256), 256, 0, s>>>(
                     A.row_ptr.data(), A.cols.data(), A.vals.data(), blk_off, h.V, d_part_to_real.data(),
-                    d_real_to_part.data(), d_anc.data(), h.L, d_range_bank.data(), c_first, h.N, d_hess.data());
+                    d_real_to_part.data(), d_anc.data(), h.L, d_range_bank.data(), c_first, h.N, d_parent.data(),
+                    d_hess.data());
             else
                 mas_walk_kernel<<<ceil_div(A.U, 256), 256, 0, s>>>(A.U,
                                                                    A.rows.data(),
@@ -1967,7 +2005,8 @@ class MasPreconditioner final : public LocalPreconditioner
                                                                    h.L,
                                                                    d_hess.data());
         }
-        for(int l = 0; l + 1 < h.L; ++l)
+        // (the partition-order walk has done level 0's push)
+        for(int l = range_walk ? 1 : 0; l + 1 < h.L; ++l)
         {
             auto [first, cnt] = level_clusters[l];
             if(cnt > 0)
