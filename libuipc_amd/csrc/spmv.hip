@@ -452,19 +452,6 @@ __global__ __launch_bounds__(NT) void spmv_win_kernel(int32_t nchunks,
         const int     c  = active ? active[it] : it;
         const int32_t R0 = c * KW;
         const int     nr = (int)std::min<int64_t>(KW, (int64_t)B - R0);
-        for(int l = tid; l < nr; l += NT)
-        {
-            const int64_t v = ORD ? R0 + l : real[R0 + l];
-            xs[0][l]        = x[3 * v];
-            xs[1][l]        = x[3 * v + 1];
-            xs[2][l]        = x[3 * v + 2];
-            ys[0][l] = ys[1][l] = ys[2][l] = 0.0;
-            rp[l]                          = wptr[R0 + l];
-        }
-        if(tid == 0)
-            rp[nr] = wptr[R0 + nr];
-        __syncthreads();
-        const int32_t K0 = rp[0], K1 = rp[nr];
         // the stage's blocks in registers: lane slot j = kVec * h + e holds
         // block o0 + kVec * (h * NT + tid) + e.  The next stage's loads are
         // issued as soon as this stage's products are out, so they are in
@@ -538,11 +525,26 @@ __global__ __launch_bounds__(NT) void spmv_win_kernel(int32_t nchunks,
                     win_block<VT, W, ORD, KW, kStage>(wd[j], A[j], x, real, xs, ys, w, i);
             }
         };
-        int32_t s0 = K0;
-        int32_t o0 = s0 & ~(kVec - 1);  // slots [o0, o0 + kStage) cover the stage's blocks [s0, s1)
-        int32_t s1 = o0 + kStage < K1 ? o0 + kStage : K1;
+        // the window's blocks [K0, K1), read from wptr: the first stage's
+        // loads go out before x is staged, so that their latency overlaps it
+        const int32_t K0 = wptr[R0], K1 = wptr[R0 + nr];
+        int32_t       s0 = K0;
+        int32_t       o0 = s0 & ~(kVec - 1);  // slots [o0, o0 + kStage) cover the stage's blocks [s0, s1)
+        int32_t       s1 = o0 + kStage < K1 ? o0 + kStage : K1;
         if(s0 < K1)
             load_stage(o0, s0, s1);
+        for(int l = tid; l < nr; l += NT)
+        {
+            const int64_t v = ORD ? R0 + l : real[R0 + l];
+            xs[0][l]        = x[3 * v];
+            xs[1][l]        = x[3 * v + 1];
+            xs[2][l]        = x[3 * v + 2];
+            ys[0][l] = ys[1][l] = ys[2][l] = 0.0;
+            rp[l]                          = wptr[R0 + l];
+        }
+        if(tid == 0)
+            rp[nr] = wptr[R0 + nr];
+        __syncthreads();
         while(s0 < K1)
         {
             compute_stage(o0, s0, s1);
