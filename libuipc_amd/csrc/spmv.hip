@@ -545,6 +545,7 @@ __global__ __launch_bounds__(NT) void spmv_win_kernel(int32_t nchunks,
         if(tid == 0)
             rp[nr] = wptr[R0 + nr];
         __syncthreads();
+        int row0 = 0;  // the row holding the stage's first block
         while(s0 < K1)
         {
             compute_stage(o0, s0, s1);
@@ -555,16 +556,29 @@ __global__ __launch_bounds__(NT) void spmv_win_kernel(int32_t nchunks,
                 load_stage(no0, n0, n1);
             __syncthreads();
             // the rows meeting the stage: from the one holding block s0 (the
-            // last row starting at or before it)
-            int lo = 0, hi = nr - 1;
-            while(lo < hi)
+            // last row starting at or before it).  It is at or after the last
+            // stage's (row0): the wave tests the next 64 row starts at once
+            // (rp is non-decreasing, so the hits are a prefix), else a binary
+            // search past them
+            int lo;
             {
-                const int mid = (lo + hi + 1) >> 1;
-                if(rp[mid] <= s0)
-                    lo = mid;
-                else
-                    hi = mid - 1;
+                const int      cand = row0 + (int)(tid % kWave);
+                const uint64_t hits = __ballot(cand < nr && rp[cand < nr ? cand : nr] <= s0);
+                lo                  = row0 + __popcll(hits) - 1;  // bit 0 holds: row0 starts at or before s0
+                if(hits == ~0ull && row0 + kWave < nr)
+                {
+                    int hi = nr - 1;
+                    while(lo < hi)
+                    {
+                        const int mid = (lo + hi + 1) >> 1;
+                        if(rp[mid] <= s0)
+                            lo = mid;
+                        else
+                            hi = mid - 1;
+                    }
+                }
             }
+            row0 = lo;
             for(int row = lo + g; row < nr; row += kGroupsW)
             {
                 const int32_t rs = rp[row];
