@@ -102,15 +102,17 @@ __device__ __forceinline__ void gather_x(const double* __restrict__ x,
     }
 }
 
-// The matrix stream of the FP64 SpMV is read once per product and is larger
-// than the caches: non-temporal loads keep it from evicting the vectors (x,
-// and the PCG's r, z, p, Ap between kernels), 277 -> 260 us per MAS-PCG
-// iteration on the 64^3 + 2M contact system.  The FP32 stream measured no
-// better with the hint, so it keeps the default policy.
-template <typename VT, typename E>
+// The matrix stream of the SpMV is read once per product and is larger than
+// the caches: non-temporal loads keep it from evicting the vectors (x, and the
+// PCG's r, z, p, Ap between kernels).  On the 64^3 + 2M contact system: full
+// BSR, FP64 277 -> 260 us per MAS-PCG iteration (its FP32 stream measured no
+// better, and keeps the default policy); window layout, FP32 planes 117.5 ->
+// 108.9 us per iteration, while FP64 planes loaded with the default policy
+// by a third to all of the windows took 151.6 -> 160-181 us.
+template <typename VT, typename E, bool NTH = sizeof(VT) == 8>
 __device__ __forceinline__ E stream_load(const E* p)
 {
-    if constexpr(sizeof(VT) == 8)
+    if constexpr(NTH)
         return __builtin_nontemporal_load(p);
     else
         return *p;
@@ -468,10 +470,10 @@ __global__ __launch_bounds__(NT) void spmv_win_kernel(int32_t nchunks,
                 {
                     if(k < s1)
                     {
-                        wd[h] = stream_load<VT>(&words[k]);
+                        wd[h] = stream_load<VT, int32_t, true>(&words[k]);
 #pragma unroll
                         for(int q = 0; q < 9; ++q)
-                            A[h][q] = stream_load<VT>(&vals[q * stride + k]);
+                            A[h][q] = stream_load<VT, VT, true>(&vals[q * stride + k]);
                     }
                 }
                 else if constexpr(kVec == 2)
@@ -479,13 +481,13 @@ __global__ __launch_bounds__(NT) void spmv_win_kernel(int32_t nchunks,
                     using V2 = typename Vec2<VT>::T;
                     if((k >= s0 && k < s1) || k + 1 < s1)
                     {
-                        const int2v w2 = stream_load<VT>(reinterpret_cast<const int2v*>(&words[k]));
+                        const int2v w2 = stream_load<VT, int2v, true>(reinterpret_cast<const int2v*>(&words[k]));
                         wd[2 * h]      = w2.x;
                         wd[2 * h + 1]  = w2.y;
 #pragma unroll
                         for(int q = 0; q < 9; ++q)
                         {
-                            const V2 a2      = stream_load<VT>(reinterpret_cast<const V2*>(&vals[q * stride + k]));
+                            const V2 a2      = stream_load<VT, V2, true>(reinterpret_cast<const V2*>(&vals[q * stride + k]));
                             A[2 * h][q]      = a2.x;
                             A[2 * h + 1][q]  = a2.y;
                         }
@@ -496,7 +498,7 @@ __global__ __launch_bounds__(NT) void spmv_win_kernel(int32_t nchunks,
                     // k >= o0 >= s0 - 3: one of the four blocks is in the stage when k < s1
                     if(k < s1)
                     {
-                        const int4v w4 = stream_load<VT>(reinterpret_cast<const int4v*>(&words[k]));
+                        const int4v w4 = stream_load<VT, int4v, true>(reinterpret_cast<const int4v*>(&words[k]));
                         wd[4 * h]      = w4.x;
                         wd[4 * h + 1]  = w4.y;
                         wd[4 * h + 2]  = w4.z;
@@ -504,7 +506,7 @@ __global__ __launch_bounds__(NT) void spmv_win_kernel(int32_t nchunks,
 #pragma unroll
                         for(int q = 0; q < 9; ++q)
                         {
-                            const float4v a4 = stream_load<VT>(reinterpret_cast<const float4v*>(&vals[q * stride + k]));
+                            const float4v a4 = stream_load<VT, float4v, true>(reinterpret_cast<const float4v*>(&vals[q * stride + k]));
                             A[4 * h][q]      = a4.x;
                             A[4 * h + 1][q]  = a4.y;
                             A[4 * h + 2][q]  = a4.z;
